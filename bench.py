@@ -1,0 +1,144 @@
+"""Benchmark of the Schur-complement hot path on one MI355X.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+
+Prints one JSON line.  Inputs are generated on the host, copied to HBM
+before the timed region, and every timed step is one full pass of the
+workload's hot path on device-resident data.  --gpus above 1 is refused:
+this build targets one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "sdpx.rs_amd")]
+
+import numpy as np  # noqa: E402
+
+METRIC = json.load(open(os.path.join(ROOT, "BASELINE.json")))["metric"]
+INT8_PEAK_OPS = 5.0e15      # MI355X dense INT8 MFMA
+FP64_PEAK = 78.6e12         # MI355X dense FP64 MFMA
+HBM_PEAK = 8.0e12           # bytes/s
+
+
+def to_device(rec):
+    import torch
+
+    t = torch.from_numpy(rec.view(np.uint8).reshape(-1).copy())
+    return t.to("cuda")
+
+
+def rns_gemm_workload(args):
+    """Exact 256-bit GEMM 4096 x 4096 x 4096, fixed-point entries uniform in
+    [-1, 1) with 256 fraction bits (the RNS micro-workload)."""
+    import torch
+
+    import sdpx_gpu
+    from sdpx_gpu import mp
+
+    nl, m, n, k = 4, 4096, 4096, 4096
+    rng = np.random.default_rng(42)
+    a = mp.random_fixed(m * k, nl, 256, rng)
+    b = mp.random_fixed(k * n, nl, 256, rng)
+    da, db = to_device(a), to_device(b)
+    dout = torch.empty(m * n * mp.dtype(nl).itemsize, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    L = sdpx_gpu.lib()
+    sdpx_gpu.check(L.sdpx_gpu_init(0))
+
+    def step():
+        code = sdpx_gpu.rns_gemm_dev("N", "N", m, n, k, da.data_ptr(), m, db.data_ptr(), k, nl,
+                                     False, dout.data_ptr(), m * n)
+        assert code == sdpx_gpu.OK
+
+    for _ in range(args.warmup):
+        step()
+    # per-stage device times on the library's stream (HIP events)
+    sdpx_gpu.set_profiling(True)
+    step()
+    stages = sdpx_gpu.last_stage_times()
+    sdpx_gpu.set_profiling(False)
+    moduli, planes, bits = sdpx_gpu.last_plan()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    int8_ops = 2.0 * m * n * k * planes_ops(moduli, planes)
+    product_s = stages[2] / 1e3
+    achieved = int8_ops / product_s
+    result = {
+        "value": dt, "unit": "s/GEMM", "ms_per_step": dt * 1e3, "dtype": "int8->int32 (u64 limbs)",
+        "config": {"workload": "rns_gemm 4096x4096x4096 MP256 fixed-point (exact, bit-exact RNS)",
+                   "m": m, "n": n, "k": k, "precision_bits": 256, "moduli": moduli,
+                   "int8_planes": planes, "log2_modulus": round(bits, 1),
+                   "stage_ms": {"scan": stages[0], "encode": stages[1], "product": stages[2],
+                                "crt": stages[3]},
+                   "mp_macs_per_s": m * n * k / dt},
+        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": INT8_PEAK_OPS / 1e12,
+                     "unit": "TOP/s (INT8)", "frac": achieved / INT8_PEAK_OPS, "traffic": None,
+                     "kernel": "product_kernel (all moduli)", "int8_ops_per_step": int8_ops},
+    }
+    if args.cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline_gemm(a, b, m, n, k, nl)
+    return result
+
+
+def planes_ops(moduli, planes):
+    """INT8 GEMMs per exact product: one per single-digit modulus, four per
+    two-digit modulus (planes = single + 2 * double)."""
+    double = planes - moduli
+    single = moduli - double
+    return single + 4 * double
+
+
+def cpu_baseline_gemm(a, b, m, n, k, nl):
+    """The C oracle (exact dots, OpenMP over all host cores) on a bounded
+    sample of outputs, scaled to the full product."""
+    sys.path[:0] = [os.path.join(ROOT, "tests"), os.path.join(ROOT, "oracle")]
+    from helpers import oracle_entries
+
+    rng = np.random.default_rng(0)
+    cores = os.cpu_count() or 1
+    count = 4096
+    rows = rng.integers(0, m, count)
+    cols = rng.integers(0, n, count)
+    t0 = time.perf_counter()
+    oracle_entries("N", "N", k, a, m, b, k, rows, cols, threads=cores)
+    dt = time.perf_counter() - t0
+    return {"value": dt * (m * n) / count, "unit": "s/GEMM", "cores": cores, "kind": "port",
+            "sample": f"{count} random outputs (exact {k}-term dots) of the same product, "
+                      f"{dt:.2f} s, scaled by {m * n}/{count}"}
+
+
+WORKLOADS = {"rns_gemm": rns_gemm_workload}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="rns_gemm", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+    r = WORKLOADS[args.workload](args)
+    line = {"metric": METRIC, "value": r["value"], "unit": r["unit"], "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["ms_per_step"],
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+            "dtype": r["dtype"], "data": "synthetic", "config": r["config"],
+            "roofline": r["roofline"]}
+    if "cpu_baseline" in r:
+        line["cpu_baseline"] = r["cpu_baseline"]
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

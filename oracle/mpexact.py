@@ -1,0 +1,308 @@
+"""Exact big-integer oracle for the MpFloat<N> exact-product contract.
+
+TEST INFRASTRUCTURE ONLY.  Nothing in the product path imports this module;
+it is used by ``tests/``, by ``tests/golden/make_golden.py`` (which writes the
+committed fixtures) and by ``__graft_entry__.smoke()`` as the checker.
+
+Number format (crates/arithmetic/src/lib.rs:333-340, dyadic.rs:1-14): a value
+is ``limbs[N]`` (u64, least significant first), ``kind`` (MPFR kind code:
+0 NaN, 1 Inf, 2 zero, 3 regular; negative kind = negative sign) and an i32
+``exponent``; a regular value is ``M * 2**(exponent - 64N)`` with
+``2**(64N-1) <= M < 2**(64N)``.  Zero is all-zero limbs, kind 2, exponent 0
+(lib.rs:366-373).
+
+Rounding contract (crates/arithmetic/src/exactdot.rs:1-13,
+crates/solver/src/algebra/dense/blas/rns_blas.rs:1-21): every product entry
+is the exact dot product rounded once, to nearest with ties to even, at the
+destination precision.  ``from_scaled_integer`` follows exactdot.rs:1316-1391.
+
+Everything here is plain Python integers, independent of the C restatement
+(oracle/oracle.c) and of the GPU library, so the three can check each other.
+"""
+from fractions import Fraction
+
+NAN_KIND, INF_KIND, ZERO_KIND, REGULAR_KIND = 0, 1, 2, 3
+# MPFR's default exponent range (lib.rs:337-339): +-(2**30 - 1).
+EMAX = (1 << 30) - 1
+EMIN = 1 - (1 << 30)
+# Largest admitted per-operand exponent spread (rns_blas.rs:40).
+MAX_SPREAD = 4096
+
+
+class Mp:
+    """One MpFloat<N> value as (limbs tuple, kind, exponent)."""
+
+    __slots__ = ("n", "limbs", "kind", "exponent")
+
+    def __init__(self, n, limbs, kind, exponent):
+        self.n = n
+        self.limbs = tuple(int(x) for x in limbs)
+        self.kind = int(kind)
+        self.exponent = int(exponent)
+
+    # ---- views -------------------------------------------------------
+    @property
+    def p(self):
+        return 64 * self.n
+
+    def mantissa(self):
+        m = 0
+        for i, l in enumerate(self.limbs):
+            m |= l << (64 * i)
+        return m
+
+    def is_zero(self):
+        return abs(self.kind) == ZERO_KIND
+
+    def is_regular(self):
+        return abs(self.kind) == REGULAR_KIND
+
+    def is_finite(self):
+        return abs(self.kind) in (ZERO_KIND, REGULAR_KIND)
+
+    def negative(self):
+        return self.kind < 0
+
+    def scaled(self):
+        """(signed integer S, scale s) with value = S * 2**s; zero -> (0, 0)."""
+        if not self.is_regular():
+            if self.is_zero():
+                return 0, 0
+            raise ValueError("non-finite value has no scaled form")
+        m = self.mantissa()
+        return (-m if self.negative() else m), self.exponent - self.p
+
+    def to_fraction(self):
+        s, e = self.scaled()
+        return Fraction(s) * (Fraction(2) ** e)
+
+    def key(self):
+        """Bitwise identity with all zeros identified (the reference compares
+        ``x == y || both zero``, rns_blas_tests.rs:101-105)."""
+        if self.is_zero():
+            return ("zero",)
+        return (self.limbs, self.kind, self.exponent)
+
+    def __repr__(self):
+        return f"Mp(n={self.n}, kind={self.kind}, e={self.exponent}, M={self.mantissa():#x})"
+
+
+def zero(n):
+    return Mp(n, [0] * n, ZERO_KIND, 0)
+
+
+def from_scaled_integer(negative, mag, scale, n):
+    """Round ``(-1)**negative * mag * 2**scale`` once, nearest-even, to 64n
+    bits (exactdot.rs:1316-1391).  ``mag`` is a non-negative Python int."""
+    if mag < 0:
+        raise ValueError("mag must be non-negative")
+    if mag == 0:
+        return zero(n)
+    p = 64 * n
+    bits = mag.bit_length()
+    shift = bits - p
+    if shift > 0:
+        m = mag >> shift
+        rbit = (mag >> (shift - 1)) & 1
+        sticky = (mag & ((1 << (shift - 1)) - 1)) != 0
+        if rbit and (sticky or (m & 1)):
+            m += 1
+            if m >> p:
+                m = 1 << (p - 1)
+                shift += 1
+    else:
+        m = mag << (-shift)
+    exponent = scale + shift + p
+    if exponent > EMAX:
+        return Mp(n, [0] * n, -INF_KIND if negative else INF_KIND, 0)
+    if exponent < EMIN:
+        # MPFR's RNDN underflow: below half the smallest positive -> 0.
+        return zero(n)
+    limbs = [(m >> (64 * i)) & ((1 << 64) - 1) for i in range(n)]
+    return Mp(n, limbs, -REGULAR_KIND if negative else REGULAR_KIND, exponent)
+
+
+def round_fraction(x, n):
+    """Correctly rounded (RNE) 64n-bit value of an exact rational."""
+    x = Fraction(x)
+    if x == 0:
+        return zero(n)
+    neg = x < 0
+    x = abs(x)
+    p = 64 * n
+    # find e with 2**(e-1) <= x < 2**e
+    e = x.numerator.bit_length() - x.denominator.bit_length()
+    while Fraction(2) ** e <= x:
+        e += 1
+    while Fraction(2) ** (e - 1) > x:
+        e -= 1
+    scaled = x * (Fraction(2) ** (p - e))  # in [2**(p-1), 2**p)
+    q, r = divmod(scaled.numerator, scaled.denominator)
+    twice = 2 * r
+    if twice > scaled.denominator or (twice == scaled.denominator and (q & 1)):
+        q += 1
+    if q >> p:
+        q >>= 1
+        e += 1
+    return from_scaled_integer(neg, q, e - p, n)
+
+
+def from_int(v, n):
+    return round_fraction(Fraction(v), n)
+
+
+def exact_dot(pairs, n):
+    """Exact sum of products rounded once.  ``pairs`` yields (Mp, Mp)."""
+    terms = []
+    for a, b in pairs:
+        if a.is_zero() or b.is_zero():
+            continue
+        sa, ea = a.scaled()
+        sb, eb = b.scaled()
+        terms.append((sa * sb, ea + eb))
+    if not terms:
+        return zero(n)
+    lo = min(e for _, e in terms)
+    total = sum(s << (e - lo) for s, e in terms)
+    return from_scaled_integer(total < 0, abs(total), lo, n)
+
+
+def _exponent_range(vals):
+    """(min, max) of ``exponent - P`` over non-zero entries; None if any
+    entry is non-finite (rns_blas.rs:620-632)."""
+    lo, hi = None, None
+    for v in vals:
+        if not v.is_finite():
+            return "nonfinite"
+        if v.is_zero():
+            continue
+        x = v.exponent - v.p
+        lo = x if lo is None else min(lo, x)
+        hi = x if hi is None else max(hi, x)
+    return lo, hi
+
+
+def gemm_exact(ta, tb, m, n, k, a, lda, b, ldb, upper_only=False, packed=None):
+    """The exact product contract of rns_blas.rs:2134-2417.
+
+    ``a`` and ``b`` are flat column-major lists of Mp.  Returns None on a
+    decline (non-finite operand entry, or an operand exponent spread above
+    MAX_SPREAD), else the output list: full m*n column-major (ldc = m), or
+    the column-packed upper triangle when ``packed`` (square, upper_only).
+    Upper-only, unpacked outputs return the full list with the strictly lower
+    entries left as None (unwritten)."""
+    ta = ta.upper() if isinstance(ta, str) else chr(ta).upper()
+    tb = tb.upper() if isinstance(tb, str) else chr(tb).upper()
+    nl = (a[0].n if a else b[0].n) if (a or b) else 1
+    ar, ac = (m, k) if ta == "N" else (k, m)
+    br, bc = (k, n) if tb == "N" else (n, k)
+    stored_a = [a[i + j * lda] for j in range(ac) for i in range(ar)] if ar and ac else []
+    stored_b = [b[i + j * ldb] for j in range(bc) for i in range(br)] if br and bc else []
+    ra, rb = _exponent_range(stored_a), _exponent_range(stored_b)
+    if ra == "nonfinite" or rb == "nonfinite":
+        return None
+    for r in (ra, rb):
+        if r[0] is not None and r[1] - r[0] > MAX_SPREAD:
+            return None
+    if packed is None:
+        packed = upper_only and m == n
+
+    def at(i, p):
+        return a[i + p * lda] if ta == "N" else a[p + i * lda]
+
+    def bt(p, j):
+        return b[p + j * ldb] if tb == "N" else b[j + p * ldb]
+
+    if packed:
+        out = []
+        for j in range(n):
+            for i in range(j + 1):
+                out.append(exact_dot(((at(i, p), bt(p, j)) for p in range(k)), nl))
+        return out
+    out = [None] * (m * n)
+    for j in range(n):
+        for i in range(m):
+            if upper_only and i > j:
+                continue
+            out[i + j * m] = exact_dot(((at(i, p), bt(p, j)) for p in range(k)), nl)
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Scalar correctly rounded operations (used by the LDL^T restatement).
+# ---------------------------------------------------------------------------
+
+def mul(x, y):
+    if x.is_zero() or y.is_zero():
+        return zero(x.n)
+    sx, ex = x.scaled()
+    sy, ey = y.scaled()
+    s = sx * sy
+    return from_scaled_integer(s < 0, abs(s), ex + ey, x.n)
+
+
+def add(x, y):
+    if x.is_zero():
+        return y
+    if y.is_zero():
+        return x
+    sx, ex = x.scaled()
+    sy, ey = y.scaled()
+    lo = min(ex, ey)
+    s = (sx << (ex - lo)) + (sy << (ey - lo))
+    return from_scaled_integer(s < 0, abs(s), lo, x.n)
+
+
+def neg(x):
+    if x.is_zero():
+        return x
+    return Mp(x.n, x.limbs, -x.kind, x.exponent)
+
+
+def sub(x, y):
+    return add(x, neg(y))
+
+
+def div(x, y):
+    return round_fraction(x.to_fraction() / y.to_fraction(), x.n)
+
+
+# ---------------------------------------------------------------------------
+# Test-data generator of the reference's residue-BLAS tests
+# (crates/solver/src/algebra/dense/blas/rns_blas_tests.rs:4-28).
+# ---------------------------------------------------------------------------
+
+class Lcg:
+    MUL = 6364136223846793005
+    INC = 1442695040888963407
+    MASK = (1 << 64) - 1
+
+    def __init__(self, seed):
+        self.state = seed & self.MASK
+
+    def next(self):
+        """Binary64 value in [0, 1), exactly as the reference computes it."""
+        self.state = (self.state * self.MUL + self.INC) & self.MASK
+        return float(self.state >> 11) / float(1 << 53)
+
+
+def lcg_random(rng, length, spread, n):
+    """rns_blas_tests.rs:16-28: 5% zeros, else (2u-1)/(v+0.5) rounded at 64n
+    bits, scaled by 2**shift with shift uniform in [-spread, spread]."""
+    out = []
+    for _ in range(length):
+        # Python floats are IEEE binary64, so every comparison, sum and
+        # product below rounds exactly as the Rust f64 code does.
+        if rng.next() < 0.05:
+            out.append(zero(n))
+            continue
+        # from_f64 of a binary64 value is exact at >= 64 bits.
+        num = round_fraction(Fraction(rng.next() * 2.0 - 1.0), n)
+        den = round_fraction(Fraction(rng.next() + 0.5), n)
+        shift = int(rng.next() * float(2 * spread + 1)) - spread
+        q = div(num, den)
+        if q.is_zero():
+            out.append(q)
+            continue
+        out.append(Mp(n, q.limbs, q.kind, q.exponent + shift))
+    return out

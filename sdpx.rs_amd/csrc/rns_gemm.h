@@ -1,0 +1,37 @@
+// Exact RNS product on device-resident values (see rns_gemm.hip).
+#pragma once
+#include <cstdint>
+
+namespace sdpx {
+
+struct RnsGemmArgs {
+    int limbs;          // N: 64-bit limbs per mantissa
+    char ta, tb;        // 'N' or 'T'
+    int64_t m, n, k;
+    const void *a;      // device values, column-major, leading dimension lda
+    int64_t lda;
+    const void *b;
+    int64_t ldb;
+    bool upper;         // only i <= j
+    uint64_t out_len;   // packed upper when upper, m == n and out_len == n(n+1)/2
+    void *out;          // device values, ldc = m unless packed
+};
+
+struct RnsStats {
+    int moduli = 0, planes = 0;
+    double bits = 0;
+    int64_t width_a = 0, width_b = 0;
+    // stage times (ms) when profiling is on: scan, encode, product, crt
+    double stage_ms[4] = {0, 0, 0, 0};
+    int64_t product_launches = 0;
+};
+
+// Record HIP events around the pipeline stages of later calls.
+void rns_set_profiling(bool on);
+
+// Runs on the library stream and returns after the results are complete.
+// Returns an SDPX_GPU_* code; throws sdpx::Error on device failures.
+int rns_gemm_device(const RnsGemmArgs &args);
+const RnsStats &rns_last_stats();
+
+}  // namespace sdpx

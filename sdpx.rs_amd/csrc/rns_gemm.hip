@@ -1,0 +1,800 @@
+// Exact residue-number-system (RNS) products of multiple-precision matrices
+// on gfx950 INT8 matrix cores.
+//
+// Contract (crates/solver/src/algebra/dense/blas/rns_blas.rs:1-21,
+// 2134-2417): out = op(A) op(B) with every entry the exact dot product
+// rounded once, nearest-even, at 64N bits.  The result does not depend on
+// moduli, windows or tiling, so this pipeline is free to choose its own:
+//
+//   1. scan    per row of op(A) / column of op(B): the lowest set bit
+//              position (exponent - 64N + trailing zeros), the exponent range
+//              and non-finite flags.  Each row/column gets its own binary
+//              window, so a' = A_ik 2^-lo_i is an integer of at most
+//              w_a = max_i (emax_i - lo_i) bits.
+//   2. encode  a' -> symmetric residues mod q_l, written as signed-byte
+//              digit planes [plane][row][k] (k contiguous, zero padded).
+//   3. product per modulus and 128x128 output tile, INT8 MFMA
+//              (v_mfma_i32_32x32x32_i8) with exact INT32 accumulation,
+//              then x_l = (S mod q_l) (M/q_l)^-1 mod q_l as u16.
+//   4. crt     S = sum_l x_l (M/q_l) - round(sum_l x_l/q_l) M in 24-bit
+//              limbs (exact FP64 arithmetic), then one nearest-even rounding
+//              to 64N bits at scale 2^(lo_i + lo_j).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+
+#include "common.h"
+#include "rns_gemm.h"
+#include "rns_plan.h"
+
+namespace sdpx {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+constexpr int TILE = 128;        // output tile edge of the product kernel
+constexpr int KSTEP = 64;        // inner-dimension bytes per LDS stage
+constexpr int LDS_ROW = 80;      // padded LDS row (bytes): conflict-free b128 reads
+constexpr int KCHUNK = 32768;    // inner elements per exact INT32 accumulation
+constexpr int XPAD = 130;        // padded LDS column of the transposed x tile
+
+// ---------------------------------------------------------------------------
+// 1. scan
+// ---------------------------------------------------------------------------
+struct ScanAcc {
+    int lo, emin, emax, flags;  // flags: 1 non-finite, 2 has a non-zero
+};
+
+__device__ __forceinline__ void scan_value(const uint32_t *rec, int n, ScanAcc &s) {
+    int kind = (int)rec[2 * n];
+    int ak = kind < 0 ? -kind : kind;
+    if (ak == KIND_ZERO) return;
+    if (ak != KIND_REGULAR) {
+        s.flags |= 1;
+        return;
+    }
+    int e = (int)rec[2 * n + 1];
+    int tz = 0;
+    for (int w = 0; w < 2 * n; w++) {
+        uint32_t x = rec[w];
+        if (x) {
+            tz += __builtin_ctz(x);
+            break;
+        }
+        tz += 32;
+    }
+    s.lo = min(s.lo, e - 64 * n + tz);
+    s.emin = min(s.emin, e);
+    s.emax = max(s.emax, e);
+    s.flags |= 2;
+}
+
+__device__ __forceinline__ void scan_merge(ScanAcc &a, const ScanAcc &b) {
+    a.lo = min(a.lo, b.lo);
+    a.emin = min(a.emin, b.emin);
+    a.emax = max(a.emax, b.emax);
+    a.flags |= b.flags;
+}
+
+// Vectors contiguous in memory (sv == 1): 64 vectors x 4 position lanes.
+__global__ __launch_bounds__(256) void scan_vec_fast(const uint32_t *__restrict__ data, int n,
+                                                     int64_t V, int64_t K, int64_t sv, int64_t sp,
+                                                     int4 *__restrict__ out) {
+    __shared__ ScanAcc red[4][64];
+    int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    int64_t v = (int64_t)blockIdx.x * 64 + tx;
+    ScanAcc s{INT_MAX, INT_MAX, INT_MIN, 0};
+    size_t W = 2 * (size_t)n + 2;
+    if (v < V)
+        for (int64_t p = ty; p < K; p += 4) scan_value(data + (v * sv + p * sp) * W, n, s);
+    red[ty][tx] = s;
+    __syncthreads();
+    if (ty == 0 && v < V) {
+        for (int t = 1; t < 4; t++) scan_merge(s, red[t][tx]);
+        out[v] = make_int4(s.lo, s.emin, s.emax, s.flags);
+    }
+}
+
+// Positions contiguous in memory (sp == 1): one wave per vector.
+__global__ __launch_bounds__(256) void scan_pos_fast(const uint32_t *__restrict__ data, int n,
+                                                     int64_t V, int64_t K, int64_t sv, int64_t sp,
+                                                     int4 *__restrict__ out) {
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int64_t v = (int64_t)blockIdx.x * 4 + w;
+    if (v >= V) return;
+    ScanAcc s{INT_MAX, INT_MAX, INT_MIN, 0};
+    size_t W = 2 * (size_t)n + 2;
+    for (int64_t p = lane; p < K; p += 64) scan_value(data + (v * sv + p * sp) * W, n, s);
+    for (int off = 32; off > 0; off >>= 1) {
+        ScanAcc o{__shfl_xor(s.lo, off), __shfl_xor(s.emin, off), __shfl_xor(s.emax, off),
+                  __shfl_xor(s.flags, off)};
+        scan_merge(s, o);
+    }
+    if (lane == 0) out[v] = make_int4(s.lo, s.emin, s.emax, s.flags);
+}
+
+// ---------------------------------------------------------------------------
+// 2. encode
+// ---------------------------------------------------------------------------
+// Bits [b, b + 24) of the mantissa (2n words, LSB first); outside reads 0.
+__device__ __forceinline__ uint32_t bits24(const uint32_t *m, int words, int b) {
+    if (b <= -24 || b >= 32 * words) return 0;
+    if (b < 0) return (m[0] << (-b)) & 0xFFFFFFu;
+    int w = b >> 5, off = b & 31;
+    uint64_t x = m[w];
+    if (w + 1 < words) x |= (uint64_t)m[w + 1] << 32;
+    return (uint32_t)(x >> off) & 0xFFFFFFu;
+}
+
+// Records of a 32 x 32 (vector x position) tile are staged in LDS with
+// coalesced runs along whichever index is contiguous in memory; each thread
+// then encodes entries of one position column so the digit-plane writes are
+// contiguous along k.
+template <int TJMAX>
+__global__ __launch_bounds__(256) void encode_kernel(
+    const uint32_t *__restrict__ data, int n, int64_t V, int64_t K, int64_t sv, int64_t sp,
+    const int4 *__restrict__ scan, int TJ, const double *__restrict__ pow2,
+    const uint32_t *__restrict__ mq, const double *__restrict__ mqinv, int L, int single,
+    int8_t *__restrict__ planes, int64_t plane_stride, int64_t k_pad) {
+    extern __shared__ uint32_t lds[];
+    const int W = 2 * n + 2;
+    const int tid = threadIdx.x;
+    const int64_t v0 = (int64_t)blockIdx.y * 32, p0 = (int64_t)blockIdx.x * 32;
+    const bool runs_along_p = (sp == 1);
+    for (int r = 0; r < 32; r++) {
+        int64_t run_len;
+        const uint32_t *src;
+        if (runs_along_p) {
+            int64_t v = v0 + r;
+            run_len = v < V ? min((int64_t)32, max((int64_t)0, K - p0)) : 0;
+            src = data + (size_t)(v * sv + p0) * W;
+        } else {
+            int64_t p = p0 + r;
+            run_len = p < K ? min((int64_t)32, max((int64_t)0, V - v0)) : 0;
+            src = data + (size_t)(v0 + p * sp) * W;
+        }
+        for (int w = tid; w < 32 * W; w += 256) {
+            int idx = w / W, word = w - idx * W;
+            uint32_t val = (idx < run_len) ? src[w] : (word == 2 * n ? (uint32_t)KIND_ZERO : 0u);
+            int slot = runs_along_p ? (r * 32 + idx) : (idx * 32 + r);
+            lds[slot * W + word] = val;
+        }
+    }
+    __syncthreads();
+    const int pp = tid & 31;
+    for (int u = 0; u < 4; u++) {
+        const int vv = (tid >> 5) + 8 * u;
+        const int64_t v = v0 + vv, p = p0 + pp;
+        const uint32_t *rec = lds + (vv * 32 + pp) * W;
+        const int kind = (int)rec[2 * n];
+        const bool nz = (kind == KIND_REGULAR || kind == -KIND_REGULAR);
+        const bool negative = kind < 0;
+        int s = 0;
+        if (nz) s = (int)rec[2 * n + 1] - 64 * n - scan[v].x;
+        uint32_t ch[TJMAX];
+#pragma unroll
+        for (int j = 0; j < TJMAX; j++) ch[j] = (nz && j < TJ) ? bits24(rec, 2 * n, 24 * j - s) : 0u;
+        const int64_t dst = v * k_pad + p;
+        for (int l = 0; l < L; l++) {
+            double acc = 0.0;
+            const double *pw = pow2 + (size_t)l * TJ;
+#pragma unroll
+            for (int j = 0; j < TJMAX; j++)
+                if (j < TJ) acc = fma((double)ch[j], pw[j], acc);
+            const int q = (int)mq[l];
+            const double qd = (double)q;
+            double quo = floor(acc * mqinv[l]);
+            double r = fma(-quo, qd, acc);
+            if (r < 0) r += qd;
+            if (r >= qd) r -= qd;
+            int ri = (int)r;
+            if (negative && ri) ri = q - ri;
+            if (ri >= ((q + 1) >> 1)) ri -= q;
+            if (l < single) {
+                planes[(size_t)l * plane_stride + dst] = (int8_t)ri;
+            } else {
+                int g = single + 2 * (l - single);
+                int lo8 = ((ri + 128) & 255) - 128;
+                int hi8 = (ri - lo8) >> 8;
+                planes[(size_t)g * plane_stride + dst] = (int8_t)hi8;
+                planes[(size_t)(g + 1) * plane_stride + dst] = (int8_t)lo8;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 3. product
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int mod_q(double v, double qd, double qinv) {
+    double quo = floor(v * qinv);
+    double r = fma(-quo, qd, v);
+    if (r < 0) r += qd;
+    if (r >= qd) r -= qd;
+    return (int)r;
+}
+
+// One modulus x one 128x128 output tile per workgroup; 4 waves in 2x2, each
+// 64x64 = 2x2 MFMA blocks of 32x32.  D = digits per residue (1 or 2).
+// Planes of modulus l start at plane index plane_of(l); for D = 2 the high
+// digit plane comes first.
+template <int D>
+__global__ __launch_bounds__(256) void product_kernel(
+    const int8_t *__restrict__ A, const int8_t *__restrict__ B, int64_t a_plane, int64_t b_plane,
+    int64_t k_pad, int64_t k_begin, int64_t k_end, const uint32_t *__restrict__ mq,
+    const uint32_t *__restrict__ minv, const double *__restrict__ mqinv, int single,
+    int mod_first, int mod_count, const int2 *__restrict__ tiles, int ntiles, int tiles_m,
+    int tiles_n, int accumulate, uint16_t *__restrict__ X) {
+    constexpr int PLANE_TILE = TILE * LDS_ROW;  // bytes of one plane tile in LDS
+    constexpr int MAIN_BYTES = 2 * D * PLANE_TILE;
+    constexpr int XT_BYTES = TILE * XPAD * 2;
+    constexpr int LDS_BYTES = MAIN_BYTES > XT_BYTES ? MAIN_BYTES : XT_BYTES;
+    __shared__ __attribute__((aligned(16))) int8_t smem[LDS_BYTES];
+    int8_t *sA = smem;                      // [D][128][LDS_ROW]
+    int8_t *sB = smem + D * PLANE_TILE;     // [D][128][LDS_ROW]
+
+    // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs,
+    // so slot b runs on XCD b % 8.  The grid is padded to a multiple of 8 and
+    // each XCD takes a contiguous range of (modulus, tile) work, so tiles
+    // sharing operand panels meet in one L2.
+    const int total = mod_count * ntiles;
+    const int per = (int)(gridDim.x >> 3);
+    const int bid = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if (bid >= total) return;
+    const int l = mod_first + bid / ntiles;
+    const int2 tile = tiles[bid % ntiles];
+    const int tm = tile.x, tn = tile.y;
+    const int plane = l < single ? l : single + 2 * (l - single);
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+
+    const int8_t *gA[D], *gB[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        gA[d] = A + (size_t)(plane + d) * a_plane + (size_t)tm * TILE * k_pad;
+        gB[d] = B + (size_t)(plane + d) * b_plane + (size_t)tn * TILE * k_pad;
+    }
+    // global -> register staging: 2 x 16 B per thread per plane tile
+    const int c0 = tid, c1 = tid + 256;
+    const int r0 = c0 >> 2, o0 = (c0 & 3) * 16, r1 = c1 >> 2, o1 = (c1 & 3) * 16;
+    v4i ra[D][2], rb[D][2];
+    auto load = [&](int64_t k) {
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            ra[d][0] = *(const v4i *)(gA[d] + (size_t)r0 * k_pad + k + o0);
+            ra[d][1] = *(const v4i *)(gA[d] + (size_t)r1 * k_pad + k + o1);
+            rb[d][0] = *(const v4i *)(gB[d] + (size_t)r0 * k_pad + k + o0);
+            rb[d][1] = *(const v4i *)(gB[d] + (size_t)r1 * k_pad + k + o1);
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            *(v4i *)(sA + d * PLANE_TILE + r0 * LDS_ROW + o0) = ra[d][0];
+            *(v4i *)(sA + d * PLANE_TILE + r1 * LDS_ROW + o1) = ra[d][1];
+            *(v4i *)(sB + d * PLANE_TILE + r0 * LDS_ROW + o0) = rb[d][0];
+            *(v4i *)(sB + d * PLANE_TILE + r1 * LDS_ROW + o1) = rb[d][1];
+        }
+    };
+
+    constexpr int NACC = D == 1 ? 1 : 3;  // D=2: hh, hl+lh, ll
+    v16i acc[NACC][2][2];
+#pragma unroll
+    for (int a = 0; a < NACC; a++)
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) acc[a][i][j] = v16i{0};
+
+    const int frag_row = lane & 31, frag_k = (lane >> 5) * 16;
+    load(k_begin);
+    for (int64_t k = k_begin; k < k_end; k += KSTEP) {
+        __syncthreads();
+        store();
+        __syncthreads();
+        if (k + KSTEP < k_end) load(k + KSTEP);
+#pragma unroll
+        for (int kk = 0; kk < KSTEP; kk += 32) {
+            v4i fa[D][2], fb[D][2];
+#pragma unroll
+            for (int d = 0; d < D; d++)
+#pragma unroll
+                for (int i = 0; i < 2; i++) {
+                    fa[d][i] = *(const v4i *)(sA + d * PLANE_TILE +
+                                              (wm * 64 + i * 32 + frag_row) * LDS_ROW + kk + frag_k);
+                    fb[d][i] = *(const v4i *)(sB + d * PLANE_TILE +
+                                              (wn * 64 + i * 32 + frag_row) * LDS_ROW + kk + frag_k);
+                }
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++) {
+                    if constexpr (D == 1) {
+                        acc[0][i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[0][i], fb[0][j],
+                                                                             acc[0][i][j], 0, 0, 0);
+                    } else {
+                        acc[0][i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[0][i], fb[0][j],
+                                                                             acc[0][i][j], 0, 0, 0);
+                        acc[1][i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[0][i], fb[1][j],
+                                                                             acc[1][i][j], 0, 0, 0);
+                        acc[1][i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[1][i], fb[0][j],
+                                                                             acc[1][i][j], 0, 0, 0);
+                        acc[2][i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[1][i], fb[1][j],
+                                                                             acc[2][i][j], 0, 0, 0);
+                    }
+                }
+        }
+    }
+
+    // residues -> x_l = S (M/q)^-1 mod q, transposed through LDS
+    const int q = (int)mq[l];
+    const double qd = (double)q, qinv = mqinv[l];
+    const double inv = (double)minv[l];
+    const double r16 = (double)((65536 % q)), r8 = (double)(256 % q);
+    __syncthreads();
+    uint16_t *sX = (uint16_t *)smem;  // [128 cols][XPAD] (row index fastest)
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                int res;
+                if constexpr (D == 1) {
+                    res = mod_q((double)acc[0][i][j][r], qd, qinv);
+                } else {
+                    double hh = (double)mod_q((double)acc[0][i][j][r], qd, qinv);
+                    double mid = (double)mod_q((double)acc[1][i][j][r], qd, qinv);
+                    double ll = (double)mod_q((double)acc[2][i][j][r], qd, qinv);
+                    res = mod_q(fma(hh, r16, fma(mid, r8, ll)), qd, qinv);
+                }
+                int x = mod_q((double)res * inv, qd, qinv);
+                int row = wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                int col = wn * 64 + j * 32 + (lane & 31);
+                sX[col * XPAD + row] = (uint16_t)x;
+            }
+    __syncthreads();
+    uint16_t *gX = X + ((size_t)(l * tiles_n + tn) * tiles_m + tm) * (TILE * TILE);
+#pragma unroll
+    for (int it = 0; it < 8; it++) {
+        int e = (it * 256 + tid) * 8;
+        int col = e >> 7, row = e & 127;
+        const uint32_t *src = (const uint32_t *)(sX + col * XPAD + row);
+        uint32_t w0 = src[0], w1 = src[1], w2 = src[2], w3 = src[3];
+        if (accumulate) {
+            v4i old = *(const v4i *)(gX + e);
+            uint32_t o[4] = {(uint32_t)old.x, (uint32_t)old.y, (uint32_t)old.z, (uint32_t)old.w};
+            uint32_t nw[4] = {w0, w1, w2, w3};
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                uint32_t lo = (nw[t] & 0xFFFF) + (o[t] & 0xFFFF);
+                uint32_t hi = (nw[t] >> 16) + (o[t] >> 16);
+                if (lo >= (uint32_t)q) lo -= q;
+                if (hi >= (uint32_t)q) hi -= q;
+                nw[t] = lo | (hi << 16);
+            }
+            w0 = nw[0];
+            w1 = nw[1];
+            w2 = nw[2];
+            w3 = nw[3];
+        }
+        *(v4i *)(gX + e) = v4i{(int)w0, (int)w1, (int)w2, (int)w3};
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 4. crt + rounding
+// ---------------------------------------------------------------------------
+template <int JMAX, int BS>
+__global__ __launch_bounds__(BS) void crt_kernel(
+    const uint16_t *__restrict__ X, int L, const int2 *__restrict__ tiles, int tiles_m,
+    int tiles_n, const double *__restrict__ C, const double *__restrict__ Mt,
+    const double *__restrict__ qinv, int J, const int4 *__restrict__ scan_a,
+    const int4 *__restrict__ scan_b, int64_t m, int64_t nn, int upper, int packed, int n,
+    uint32_t *__restrict__ out) {
+    __shared__ uint32_t limbs_s[JMAX * BS];
+    const int tid = threadIdx.x;
+    const int2 tile = tiles[blockIdx.y];
+    const int e = blockIdx.x * BS + tid;
+    const int il = e & 127, jl = e >> 7;
+    const int64_t i = (int64_t)tile.x * TILE + il, j = (int64_t)tile.y * TILE + jl;
+    if (i >= m || j >= nn || (upper && i > j)) return;
+
+    const uint16_t *xp = X + ((size_t)tile.y * tiles_m + tile.x) * (TILE * TILE) + e;
+    const size_t xstride = (size_t)tiles_n * tiles_m * (TILE * TILE);
+    double acc[JMAX];
+#pragma unroll
+    for (int t = 0; t < JMAX; t++) acc[t] = 0.0;
+    double f = 0.0;
+    for (int l = 0; l < L; l++) {
+        const double x = (double)xp[(size_t)l * xstride];
+        f = fma(x, qinv[l], f);
+        const double *cl = C + (size_t)l * J;
+#pragma unroll
+        for (int t = 0; t < JMAX; t++)
+            if (t < J) acc[t] = fma(x, cl[t], acc[t]);
+    }
+    const double tq = rint(f);
+    const double two24 = 16777216.0, inv24 = 1.0 / 16777216.0;
+    double carry = 0.0;
+    bool any = false;
+#pragma unroll
+    for (int t = 0; t < JMAX; t++) {
+        if (t < J) {
+            double v = fma(-tq, Mt[t], acc[t]) + carry;
+            double hi = floor(v * inv24);
+            double lo = fma(-hi, two24, v);
+            carry = hi;
+            uint32_t li = (uint32_t)lo;
+            limbs_s[t * BS + tid] = li;
+            any |= li != 0;
+        }
+    }
+    const bool negative = carry < 0;
+    uint32_t *out_rec;
+    {
+        int64_t o = packed ? (j * (j + 1) / 2 + i) : (i + j * m);
+        out_rec = out + (size_t)o * (2 * n + 2);
+    }
+    if (!any) {
+        for (int w = 0; w < 2 * n; w++) out_rec[w] = 0;
+        out_rec[2 * n] = KIND_ZERO;
+        out_rec[2 * n + 1] = 0;
+        return;
+    }
+    if (negative) {  // two's complement negation over J limbs
+        uint32_t c = 1;
+        for (int t = 0; t < J; t++) {
+            uint32_t v = ((~limbs_s[t * BS + tid]) & 0xFFFFFFu) + c;
+            c = v >> 24;
+            limbs_s[t * BS + tid] = v & 0xFFFFFFu;
+        }
+    }
+    int top = J - 1;
+    while (top > 0 && limbs_s[top * BS + tid] == 0) top--;
+    const int bits = 24 * top + (32 - __builtin_clz(limbs_s[top * BS + tid]));
+    const int P = 64 * n;
+    int shift = bits - P;
+    auto bit_word = [&](int b) -> uint32_t {  // bits [b, b + 32)
+        uint32_t w = 0;
+        int t0 = b >= 0 ? b / 24 : -((-b + 23) / 24);
+        for (int t = t0; t <= t0 + 2; t++) {
+            if (t < 0 || t >= J) continue;
+            uint32_t v = limbs_s[t * BS + tid];
+            int pos = 24 * t - b;
+            if (pos >= 32 || pos <= -24) continue;
+            w |= pos >= 0 ? (v << pos) : (v >> (-pos));
+        }
+        return w;
+    };
+    uint32_t mant[2 * MAX_LIMBS];
+    for (int u = 0; u < 2 * n; u++) mant[u] = bit_word(shift + 32 * u);
+    if (shift > 0) {
+        const int rb = shift - 1;
+        const bool round = (limbs_s[(rb / 24) * BS + tid] >> (rb % 24)) & 1;
+        bool sticky = false;
+        for (int t = 0; t < rb / 24 && !sticky; t++) sticky = limbs_s[t * BS + tid] != 0;
+        if (!sticky && (rb % 24)) sticky = (limbs_s[(rb / 24) * BS + tid] & ((1u << (rb % 24)) - 1)) != 0;
+        if (round && (sticky || (mant[0] & 1))) {
+            uint32_t c = 1;
+            for (int u = 0; u < 2 * n && c; u++) {
+                mant[u] += 1;
+                c = mant[u] == 0;
+            }
+            if (c) {
+                mant[2 * n - 1] = 0x80000000u;
+                shift += 1;
+            }
+        }
+    }
+    const int64_t scale = (int64_t)scan_a[i].x + scan_b[j].x;
+    const int64_t expo = scale + shift + P;
+    if (expo > (int64_t)((1 << 30) - 1)) {
+        for (int w = 0; w < 2 * n; w++) out_rec[w] = 0;
+        out_rec[2 * n] = (uint32_t)(negative ? -KIND_INF : KIND_INF);
+        out_rec[2 * n + 1] = 0;
+        return;
+    }
+    if (expo < (int64_t)(1 - (1 << 30))) {
+        for (int w = 0; w < 2 * n; w++) out_rec[w] = 0;
+        out_rec[2 * n] = KIND_ZERO;
+        out_rec[2 * n + 1] = 0;
+        return;
+    }
+    for (int u = 0; u < 2 * n; u++) out_rec[u] = mant[u];
+    out_rec[2 * n] = (uint32_t)(negative ? -KIND_REGULAR : KIND_REGULAR);
+    out_rec[2 * n + 1] = (uint32_t)(int32_t)expo;
+}
+
+__global__ void zero_values(uint32_t *__restrict__ out, int n, int64_t m, int64_t nn,
+                            int upper, int packed) {
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= m * nn) return;
+    int64_t i = idx % m, j = idx / m;
+    if (upper && i > j) return;
+    int64_t o = packed ? (j * (j + 1) / 2 + i) : (i + j * m);
+    uint32_t *rec = out + (size_t)o * (2 * n + 2);
+    for (int w = 0; w < 2 * n; w++) rec[w] = 0;
+    rec[2 * n] = KIND_ZERO;
+    rec[2 * n + 1] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// host driver
+// ---------------------------------------------------------------------------
+namespace {
+
+enum Slot {
+    S_SCAN_A = 0, S_SCAN_B, S_PLANES_A, S_PLANES_B, S_X, S_POW_A, S_POW_B, S_MODS, S_CRT,
+    S_TILES, S_COUNT
+};
+
+struct Operand {
+    const uint32_t *data;
+    int64_t V, K, sv, sp;
+};
+
+void launch_scan(const Operand &op, int n, int4 *out, hipStream_t s) {
+    if (op.V == 0) return;
+    if (op.sv == 1 && op.sp != 1) {
+        hipLaunchKernelGGL(scan_vec_fast, dim3((unsigned)((op.V + 63) / 64)), dim3(256), 0, s,
+                           op.data, n, op.V, op.K, op.sv, op.sp, out);
+    } else {
+        hipLaunchKernelGGL(scan_pos_fast, dim3((unsigned)((op.V + 3) / 4)), dim3(256), 0, s,
+                           op.data, n, op.V, op.K, op.sv, op.sp, out);
+    }
+    HIP_CHECK(hipGetLastError());
+}
+
+struct Window {
+    int64_t width = 0;  // max_v (emax_v - lo_v)
+    int64_t emin = INT64_MAX, emax = INT64_MIN;
+    bool nonfinite = false, any = false;
+};
+
+Window reduce_scan(const std::vector<int4> &s) {
+    Window w;
+    for (const int4 &v : s) {
+        if (v.w & 1) w.nonfinite = true;
+        if (v.w & 2) {
+            w.any = true;
+            w.width = std::max<int64_t>(w.width, (int64_t)v.z - v.x);
+            w.emin = std::min<int64_t>(w.emin, v.y);
+            w.emax = std::max<int64_t>(w.emax, v.z);
+        }
+    }
+    return w;
+}
+
+template <int TJMAX>
+void launch_encode_t(const Operand &op, int n, const int4 *scan, int TJ, const double *pow2,
+                     const uint32_t *mq, const double *mqinv, int L, int single, int8_t *planes,
+                     int64_t plane_stride, int64_t k_pad, int64_t v_pad, hipStream_t s) {
+    size_t lds = 1024 * (2 * (size_t)n + 2) * 4;
+    if (lds > 65536)
+        HIP_CHECK(hipFuncSetAttribute((const void *)encode_kernel<TJMAX>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    dim3 grid((unsigned)(k_pad / 32), (unsigned)(v_pad / 32));
+    hipLaunchKernelGGL(encode_kernel<TJMAX>, grid, dim3(256), lds, s, op.data, n, op.V, op.K,
+                       op.sv, op.sp, scan, TJ, pow2, mq, mqinv, L, single, planes, plane_stride,
+                       k_pad);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_encode(const Operand &op, int n, const int4 *scan, int TJ, const double *pow2,
+                   const uint32_t *mq, const double *mqinv, int L, int single, int8_t *planes,
+                   int64_t plane_stride, int64_t k_pad, int64_t v_pad, hipStream_t s) {
+    if (TJ <= 16)
+        launch_encode_t<16>(op, n, scan, TJ, pow2, mq, mqinv, L, single, planes, plane_stride,
+                            k_pad, v_pad, s);
+    else if (TJ <= 32)
+        launch_encode_t<32>(op, n, scan, TJ, pow2, mq, mqinv, L, single, planes, plane_stride,
+                            k_pad, v_pad, s);
+    else
+        launch_encode_t<64>(op, n, scan, TJ, pow2, mq, mqinv, L, single, planes, plane_stride,
+                            k_pad, v_pad, s);
+}
+
+// 2^(24 j) mod q_l for j < TJ, as doubles (count x TJ).
+std::vector<double> pow2_table(const RnsPlan &plan, int TJ) {
+    std::vector<double> t((size_t)plan.count * TJ);
+    for (int l = 0; l < plan.count; l++) {
+        uint64_t q = plan.q[l], v = 1 % q, step = (1u << 24) % q;
+        for (int j = 0; j < TJ; j++) {
+            t[(size_t)l * TJ + j] = (double)v;
+            v = v * step % q;
+        }
+    }
+    return t;
+}
+
+thread_local RnsStats t_stats;
+bool g_profile = false;
+
+// Events bracketing the four stages (only when profiling).
+struct StageEvents {
+    hipEvent_t ev[5] = {};
+    bool on;
+    explicit StageEvents(bool enable) : on(enable) {
+        if (on)
+            for (auto &e : ev) HIP_CHECK(hipEventCreate(&e));
+    }
+    void mark(int i, hipStream_t s) {
+        if (on) HIP_CHECK(hipEventRecord(ev[i], s));
+    }
+    void collect(RnsStats &st) {
+        if (!on) return;
+        HIP_CHECK(hipEventSynchronize(ev[4]));
+        for (int i = 0; i < 4; i++) {
+            float ms = 0;
+            HIP_CHECK(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+            st.stage_ms[i] = ms;
+        }
+    }
+    ~StageEvents() {
+        if (on)
+            for (auto &e : ev) (void)hipEventDestroy(e);
+    }
+};
+
+unsigned padded_grid(int64_t total) { return (unsigned)((total + 7) / 8 * 8); }
+
+}  // namespace
+
+const RnsStats &rns_last_stats() { return t_stats; }
+void rns_set_profiling(bool on) { g_profile = on; }
+
+int rns_gemm_device(const RnsGemmArgs &g) {
+    Device &dev = device();
+    hipStream_t s = dev.stream;
+    Workspace &ws = dev.ws;
+    const int n = g.limbs;
+    const bool tan = (g.ta == 'N' || g.ta == 'n');
+    const bool tbn = (g.tb == 'N' || g.tb == 'n');
+    const int64_t m = g.m, nn = g.n, k = g.k;
+    const bool packed = g.upper && m == nn && g.out_len == (uint64_t)(nn * (nn + 1) / 2);
+    t_stats = RnsStats{};
+    if (m == 0 || nn == 0) return SDPX_GPU_OK;
+
+    Operand oa{(const uint32_t *)g.a, m, k, tan ? 1 : g.lda, tan ? g.lda : 1};
+    Operand ob{(const uint32_t *)g.b, nn, k, tbn ? g.ldb : 1, tbn ? 1 : g.ldb};
+    const bool same = g.a == g.b && oa.sv == ob.sv && oa.sp == ob.sp && m == nn && g.lda == g.ldb;
+
+    auto zero_fill = [&]() {
+        int64_t total = m * nn;
+        hipLaunchKernelGGL(zero_values, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+                           (uint32_t *)g.out, n, m, nn, g.upper ? 1 : 0, packed ? 1 : 0);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamSynchronize(s));
+    };
+    if (k == 0) {
+        zero_fill();
+        return SDPX_GPU_OK;
+    }
+
+    // 1. scan
+    StageEvents sev(g_profile);
+    sev.mark(0, s);
+    int4 *scan_a = (int4 *)ws.get(S_SCAN_A, sizeof(int4) * m);
+    int4 *scan_b = same ? scan_a : (int4 *)ws.get(S_SCAN_B, sizeof(int4) * nn);
+    launch_scan(oa, n, scan_a, s);
+    if (!same) launch_scan(ob, n, scan_b, s);
+    std::vector<int4> ha(m), hb;
+    HIP_CHECK(hipMemcpyAsync(ha.data(), scan_a, sizeof(int4) * m, hipMemcpyDeviceToHost, s));
+    if (!same) {
+        hb.resize(nn);
+        HIP_CHECK(hipMemcpyAsync(hb.data(), scan_b, sizeof(int4) * nn, hipMemcpyDeviceToHost, s));
+    }
+    HIP_CHECK(hipStreamSynchronize(s));
+    Window wa = reduce_scan(ha), wb = same ? wa : reduce_scan(hb);
+    if (wa.nonfinite || wb.nonfinite) return SDPX_GPU_DECLINED;
+    if ((wa.any && wa.emax - wa.emin > MAX_SPREAD) || (wb.any && wb.emax - wb.emin > MAX_SPREAD))
+        return SDPX_GPU_DECLINED;
+    if (!wa.any || !wb.any) {
+        zero_fill();
+        return SDPX_GPU_OK;
+    }
+    const int TJa = (int)((wa.width + 23) / 24) + 1, TJb = (int)((wb.width + 23) / 24) + 1;
+    if (TJa > 64 || TJb > 64) return SDPX_GPU_DECLINED;
+    const RnsPlan *plan = rns_plan_for_bits(rns_needed_bits(wa.width, wb.width, k));
+    if (!plan) return SDPX_GPU_DECLINED;
+    if (plan->limbs > 128) return SDPX_GPU_DECLINED;
+    const int L = plan->count, G = plan->planes, single = plan->single;
+    t_stats.moduli = L;
+    t_stats.planes = G;
+    t_stats.bits = plan->bits;
+    t_stats.width_a = wa.width;
+    t_stats.width_b = wb.width;
+
+    // tables: moduli, inverses, 1/q, pow2 (A, B), CRT
+    const int64_t k_pad = (k + KSTEP - 1) / KSTEP * KSTEP;
+    const int64_t m_pad = (m + TILE - 1) / TILE * TILE, n_pad = (nn + TILE - 1) / TILE * TILE;
+    const int tiles_m = (int)(m_pad / TILE), tiles_n = (int)(n_pad / TILE);
+    std::vector<int2> tl;
+    for (int tn = 0; tn < tiles_n; tn++)
+        for (int tm = 0; tm < tiles_m; tm++)
+            if (!g.upper || tm <= tn) tl.push_back(make_int2(tm, tn));
+    const int ntiles = (int)tl.size();
+
+    size_t mods_bytes = (size_t)L * (4 + 4 + 8);
+    char *mods = (char *)ws.get(S_MODS, mods_bytes);
+    uint32_t *d_q = (uint32_t *)mods;
+    uint32_t *d_inv = (uint32_t *)(mods + 4 * L);
+    double *d_qinv = (double *)(mods + 8 * L);
+    HIP_CHECK(hipMemcpyAsync(d_q, plan->q.data(), 4 * L, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(d_inv, plan->inv.data(), 4 * L, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(d_qinv, plan->qinv.data(), 8 * L, hipMemcpyHostToDevice, s));
+    std::vector<double> pa = pow2_table(*plan, TJa), pb = pow2_table(*plan, TJb);
+    double *d_pa = (double *)ws.get(S_POW_A, 8 * pa.size());
+    double *d_pb = (double *)ws.get(S_POW_B, 8 * pb.size());
+    HIP_CHECK(hipMemcpyAsync(d_pa, pa.data(), 8 * pa.size(), hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(d_pb, pb.data(), 8 * pb.size(), hipMemcpyHostToDevice, s));
+    const int J = plan->limbs;
+    double *d_crt = (double *)ws.get(S_CRT, 8 * ((size_t)L * J + J));
+    HIP_CHECK(hipMemcpyAsync(d_crt, plan->c.data(), 8 * (size_t)L * J, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(d_crt + (size_t)L * J, plan->m.data(), 8 * (size_t)J,
+                             hipMemcpyHostToDevice, s));
+    int2 *d_tiles = (int2 *)ws.get(S_TILES, sizeof(int2) * ntiles);
+    HIP_CHECK(hipMemcpyAsync(d_tiles, tl.data(), sizeof(int2) * ntiles, hipMemcpyHostToDevice, s));
+
+    // 2. encode
+    sev.mark(1, s);
+    const int64_t a_plane = m_pad * k_pad, b_plane = n_pad * k_pad;
+    int8_t *pla = (int8_t *)ws.get(S_PLANES_A, (size_t)G * a_plane);
+    int8_t *plb = same ? pla : (int8_t *)ws.get(S_PLANES_B, (size_t)G * b_plane);
+    launch_encode(oa, n, scan_a, TJa, d_pa, d_q, d_qinv, L, single, pla, a_plane, k_pad, m_pad, s);
+    if (!same)
+        launch_encode(ob, n, scan_b, TJb, d_pb, d_q, d_qinv, L, single, plb, b_plane, k_pad, n_pad,
+                      s);
+
+    // 3. products, in exact INT32 chunks of the inner dimension
+    sev.mark(2, s);
+    uint16_t *X = (uint16_t *)ws.get(S_X, (size_t)L * tiles_m * tiles_n * TILE * TILE * 2);
+    for (int64_t k0 = 0; k0 < k_pad; k0 += KCHUNK) {
+        const int64_t k1 = std::min<int64_t>(k_pad, k0 + KCHUNK);
+        const int acc = k0 > 0 ? 1 : 0;
+        if (single > 0) {
+            hipLaunchKernelGGL(product_kernel<1>, dim3(padded_grid(single * ntiles)), dim3(256), 0, s,
+                               pla, plb, a_plane, b_plane, k_pad, k0, k1, d_q, d_inv, d_qinv,
+                               single, 0, single, d_tiles, ntiles, tiles_m, tiles_n, acc, X);
+            HIP_CHECK(hipGetLastError());
+        }
+        if (L > single) {
+            hipLaunchKernelGGL(product_kernel<2>, dim3(padded_grid((L - single) * ntiles)),
+                               dim3(256), 0, s, pla, plb, a_plane, b_plane, k_pad, k0, k1, d_q,
+                               d_inv, d_qinv, single, single, L - single, d_tiles, ntiles, tiles_m,
+                               tiles_n, acc, X);
+            HIP_CHECK(hipGetLastError());
+        }
+    }
+
+    // 4. crt + rounding
+    sev.mark(3, s);
+    t_stats.product_launches = ((k_pad + KCHUNK - 1) / KCHUNK) * ((single > 0) + (L > single));
+    const double *d_c = d_crt, *d_m = d_crt + (size_t)L * J;
+    const int up = g.upper ? 1 : 0, pk = packed ? 1 : 0;
+    uint32_t *outw = (uint32_t *)g.out;
+    if (J <= 32)
+        hipLaunchKernelGGL((crt_kernel<32, 256>), dim3(TILE * TILE / 256, ntiles), dim3(256), 0, s,
+                           X, L, d_tiles, tiles_m, tiles_n, d_c, d_m, d_qinv, J, scan_a, scan_b,
+                           m, nn, up, pk, n, outw);
+    else if (J <= 64)
+        hipLaunchKernelGGL((crt_kernel<64, 256>), dim3(TILE * TILE / 256, ntiles), dim3(256), 0, s,
+                           X, L, d_tiles, tiles_m, tiles_n, d_c, d_m, d_qinv, J, scan_a, scan_b,
+                           m, nn, up, pk, n, outw);
+    else
+        hipLaunchKernelGGL((crt_kernel<128, 64>), dim3(TILE * TILE / 64, ntiles), dim3(64), 0, s,
+                           X, L, d_tiles, tiles_m, tiles_n, d_c, d_m, d_qinv, J, scan_a, scan_b,
+                           m, nn, up, pk, n, outw);
+    HIP_CHECK(hipGetLastError());
+    sev.mark(4, s);
+    HIP_CHECK(hipStreamSynchronize(s));
+    sev.collect(t_stats);
+    return SDPX_GPU_OK;
+}
+
+}  // namespace sdpx

@@ -1,0 +1,83 @@
+"""Test helpers: the C oracle through ctypes, and conversions between
+numpy records and the Python big-integer oracle."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+import mpexact
+from sdpx_gpu import mp
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+_oracle = None
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        so = os.path.join(ORACLE_DIR, "liboracle.so")
+        src = os.path.join(ORACLE_DIR, "oracle.c")
+        if not os.path.exists(so) or os.path.getmtime(so) < os.path.getmtime(src):
+            subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+        L = ctypes.CDLL(so)
+        vp, sz, i32, c = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_char
+        L.oracle_gemm_exact.argtypes = [i32, c, c, sz, sz, sz, vp, sz, vp, sz, i32, i32, vp, i32]
+        L.oracle_gemm_exact.restype = i32
+        L.oracle_gemm_entries.argtypes = [i32, c, c, sz, vp, sz, vp, sz, sz, vp, vp, vp, i32]
+        L.oracle_gemm_entries.restype = None
+        _oracle = L
+    return _oracle
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None and a.size else None
+
+
+def oracle_gemm(ta, tb, m, n, k, a, lda, b, ldb, upper=False, packed=None, threads=8):
+    """C oracle: returns None on decline, else the output record array."""
+    nl = mp.limbs_of(a if a.size else b)
+    if packed is None:
+        packed = upper and m == n
+    out = mp.zeros(n * (n + 1) // 2 if packed else m * n, nl)
+    ok = oracle().oracle_gemm_exact(nl, ta.encode(), tb.encode(), m, n, k, _p(a), lda, _p(b), ldb,
+                                    1 if upper else 0, 1 if packed else 0, _p(out), threads)
+    return out if ok else None
+
+
+def oracle_entries(ta, tb, k, a, lda, b, ldb, rows, cols, threads=8):
+    nl = mp.limbs_of(a)
+    rows = np.ascontiguousarray(rows, np.int64)
+    cols = np.ascontiguousarray(cols, np.int64)
+    out = mp.zeros(rows.size, nl)
+    oracle().oracle_gemm_entries(nl, ta.encode(), tb.encode(), k, _p(a), lda, _p(b), ldb,
+                                 rows.size, _p(rows), _p(cols), _p(out), threads)
+    return out
+
+
+def to_mp(rec):
+    """numpy records -> list of mpexact.Mp"""
+    n = mp.limbs_of(rec)
+    return [mpexact.Mp(n, r["limbs"], r["kind"], r["exponent"]) for r in rec]
+
+
+def from_mp(vals, n):
+    out = mp.zeros(len(vals), n)
+    for i, v in enumerate(vals):
+        if v is None:
+            continue
+        out[i]["limbs"] = np.array(v.limbs, dtype=np.uint64)
+        out[i]["kind"] = v.kind
+        out[i]["exponent"] = v.exponent
+    return out
+
+
+def lcg_operands(m, n, k, spread, seed, nl, ta, tb):
+    """The reference's test operands (rns_blas_tests.rs:62-70)."""
+    rng = mpexact.Lcg(seed)
+    ar, ac = (m, k) if ta == "N" else (k, m)
+    br, bc = (k, n) if tb == "N" else (n, k)
+    a = mpexact.lcg_random(rng, ar * ac, spread, nl)
+    b = mpexact.lcg_random(rng, br * bc, spread, nl)
+    return from_mp(a, nl), ar, from_mp(b, nl), br

@@ -1,0 +1,38 @@
+"""The C-ABI library builds, loads and exports every declared symbol
+(no compute calls: this container has no GPU)."""
+import os
+import re
+
+import sdpx_gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    names = []
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", fn)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            names += re.findall(r"\b(sdpx_gpu_\w+)\s*\(", text)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    L = sdpx_gpu.lib()
+    names = declared()
+    assert len(names) >= 6
+    for name in names:
+        assert hasattr(L, name), name
+
+
+def test_abi_version_and_error_channel():
+    L = sdpx_gpu.lib()
+    assert L.sdpx_gpu_abi_version() == 1
+    # invalid input is rejected before any device work
+    from sdpx_gpu import mp
+    a = mp.zeros(4, 4)
+    code = L.sdpx_gpu_rns_gemm(4, ord("X"), ord("N"), 2, 2, 2, sdpx_gpu._ptr(a), 2,
+                               sdpx_gpu._ptr(a), 2, 0, sdpx_gpu._ptr(a), 4)
+    assert code == sdpx_gpu.INVALID
+    assert "transpose" in sdpx_gpu.last_error()
