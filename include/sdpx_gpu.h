@@ -90,6 +90,75 @@ int32_t sdpx_gpu_rns_gemm_dev(uint32_t limbs, uint8_t transa, uint8_t transb, ui
                               uint64_t n, uint64_t k, const void *a, uint64_t lda, const void *b,
                               uint64_t ldb, uint32_t upper_only, void *out, uint64_t out_len);
 
+/* ---- Multiple-precision dense LDL^T -----------------------------------
+ *
+ * sdpx_gpu_mp_ldl replaces DenseLeaf::factor_leaf / factor_loaded of the
+ * arrow backend (crates/solver/src/solver/kkt/ldl/arrow.rs:808-926): from
+ * order 256 the recursive factor ldl_rec (255-415) with its panel solve
+ * solve_unit_lower_t (417-490), below it ldl_dots (156-253), bitwise as the
+ * reference.  The lower triangle of a (n x n, lda, values of 64*limbs bits)
+ * is overwritten by the unit lower factor (ones on the diagonal), dinv
+ * (n values) receives D^-1.  signs: 0 entries (all +1), 1 (shared) or n
+ * (per pivot, +1/-1); reg_eps / reg_delta (one value each, or both NULL)
+ * select the pivot regularization of DenseLeaf::pivot (618-634).  *ok = 0
+ * on a failed pivot (the reference's Err; contents unspecified).  Every
+ * exact product of the recursion runs as an RNS product on the GPU.  flags:
+ * SDPX_GPU_LDL_DOTS forces ldl_dots at any order (the reference's
+ * factor_dots fallback); SDPX_GPU_LDL_HOST_PRODUCTS keeps every product on
+ * host exact dots (same values; for testing). */
+#define SDPX_GPU_LDL_DOTS 1u
+#define SDPX_GPU_LDL_HOST_PRODUCTS 2u
+int32_t sdpx_gpu_mp_ldl(uint32_t limbs, uint64_t n, void *a, uint64_t lda, const int8_t *signs,
+                        uint64_t nsigns, const void *reg_eps, const void *reg_delta, void *dinv,
+                        uint32_t flags, int32_t *ok);
+/* Counters of this thread's last sdpx_gpu_mp_ldl (measurement only). */
+int32_t sdpx_gpu_mp_ldl_stats(int64_t *gpu_products, int64_t *host_products, int64_t *regularized,
+                              double *gpu_mp_macs);
+
+/* ---- Float64 dense factor / solve -------------------------------------
+ *
+ * A factor handle replaces the dense Float64 backend behind
+ * DirectLDLSolver (crates/solver/src/solver/kkt/direct/mod.rs:11-62),
+ * DenseBlockSolver::refactor / solve (kkt/ldl/dense_block.rs:784-851,
+ * tiled_potrf 515-594, tiled_trsv 604-669): refactor reads the upper
+ * triangle of a symmetric n x n matrix (column-major, lda) and factors it
+ * as L L^T on the GPU; info = 0 on success, else the 1-based column of the
+ * first non-positive pivot (LAPACK dpotrf convention; the caller falls back
+ * or escalates regularization, direct/solver.rs:838).  solve_many overwrites
+ * the n x nrhs column-major panel b with A^-1 b (solve_many,
+ * direct/mod.rs:29-41).  `_dev` variants take device pointers. */
+typedef struct sdpx_gpu_dense sdpx_gpu_dense;
+int32_t sdpx_gpu_dense_create(uint64_t n, sdpx_gpu_dense **out);
+int32_t sdpx_gpu_dense_destroy(sdpx_gpu_dense *h);
+/* Dynamic pivot rule of the dense factor (dense_block.rs:329-365, settings
+ * dynamic_regularization_eps / _delta): a pivot <= eps becomes delta unless
+ * it is below -64 * eps_f64 * 64 * |unfactored diagonal| (then info > 0). */
+int32_t sdpx_gpu_dense_set_dynamic_pivot(sdpx_gpu_dense *h, int32_t enable, double eps,
+                                         double delta);
+int32_t sdpx_gpu_dense_refactor(sdpx_gpu_dense *h, const double *a, uint64_t lda, int32_t *info);
+int32_t sdpx_gpu_dense_refactor_dev(sdpx_gpu_dense *h, const double *a, uint64_t lda,
+                                    int32_t *info);
+int32_t sdpx_gpu_dense_solve_many(sdpx_gpu_dense *h, double *b, uint64_t nrhs, uint64_t ldb);
+int32_t sdpx_gpu_dense_solve_many_dev(sdpx_gpu_dense *h, double *b, uint64_t nrhs, uint64_t ldb);
+/* Device address of the lower factor (n x n, ld n) after a successful
+ * refactor; valid until the next refactor or destroy. */
+int32_t sdpx_gpu_dense_factor_dev_ptr(sdpx_gpu_dense *h, const double **l);
+/* Copy the lower factor to host memory (n x n, ldo); the strict upper part
+ * of the copy is unspecified. */
+int32_t sdpx_gpu_dense_get_factor(sdpx_gpu_dense *h, double *out, uint64_t ldo);
+
+/* Float64 Schur assembly of one PSD block (replaces the dense-column path of
+ * PsdBlock::compute_schur_selected, kkt/condensed/psd.rs:1299-1456, with
+ * Ginv = Rinv^T Rinv from kkt.rs:1119-1127):
+ *   S_ab = <A_a, Ginv A_b Ginv> = svec(Rinv A_a Rinv^T) . svec(Rinv A_b Rinv^T)
+ * for m constraint columns given as svec vectors (s = n(n+1)/2 rows,
+ * upper-triangular column order, off-diagonals scaled by sqrt 2), Rinv n x n
+ * column-major.  Writes the upper triangle of S (m x m, lds). */
+int32_t sdpx_gpu_schur_psd_f64(uint64_t n, uint64_t m, const double *rinv, const double *a_svec,
+                               double *s, uint64_t lds);
+int32_t sdpx_gpu_schur_psd_f64_dev(uint64_t n, uint64_t m, const double *rinv,
+                                   const double *a_svec, double *s, uint64_t lds);
+
 #ifdef __cplusplus
 }
 #endif

@@ -306,3 +306,173 @@ def lcg_random(rng, length, spread, n):
             continue
         out.append(Mp(n, q.limbs, q.kind, q.exponent + shift))
     return out
+
+
+# ---------------------------------------------------------------------------
+# Dense LDL^T restatement (crates/solver/src/solver/kkt/ldl/arrow.rs):
+# ldl_dots 156-253, ldl_rec 255-415, solve_unit_lower_t 417-490,
+# DenseLeaf::pivot 618-634.  Matrices are dicts/lists indexed [i + j*ld].
+# ---------------------------------------------------------------------------
+REC_BASE = 128
+
+
+def _pivot(d, sign, reg, stats):
+    if not d.is_finite():
+        return None
+    if reg is not None:
+        eps, delta = reg
+        ds = d if sign > 0 else neg(d)
+        if ds.to_fraction() < eps.to_fraction():
+            d = delta if sign > 0 else neg(delta)
+            stats["regularized"] += 1
+    if d.is_zero():
+        return None
+    return d
+
+
+def ldl_dots(l, ld, n, first, sign, reg, stats, dinv, off=0):
+    """In place on the lower triangle of l[off + i + j*ld]; dinv[i]."""
+    nl = l[off].n
+    one = from_int(1, nl)
+    r = {}
+    w = {}
+    for j in range(n):
+        fj = first[j]
+        dot = exact_dot(((r[(j, q)], w[(j, q)]) for q in range(fj, j)), nl)
+        a_jj = l[off + j + j * ld]
+        dj = sub(a_jj, dot)
+        if dj.is_zero() and j > fj:
+            exact = exact_dot([(r[(j, q)], w[(j, q)]) for q in range(fj, j)] + [(a_jj, neg(one))], nl)
+            dj = neg(exact)
+        dj = _pivot(dj, sign(j), reg, stats)
+        if dj is None:
+            return False
+        dinv[j] = div(one, dj)
+        for i in range(j + 1, n):
+            if first[i] > j:
+                v = zero(nl)
+            else:
+                lo = max(first[i], first[j])
+                dot = exact_dot(((r[(i, q)], w[(j, q)]) for q in range(lo, j)), nl)
+                v = mul(sub(l[off + i + j * ld], dot), dinv[j])
+            r[(i, j)] = v
+            w[(i, j)] = mul(v, dj)
+    for j in range(n):
+        l[off + j + j * ld] = one
+        for i in range(j + 1, n):
+            l[off + i + j * ld] = r[(i, j)]
+    return True
+
+
+def exact_product_nt(m, n, k, a, aoff, lda, b, boff, ldb, nl, upper_packed=False):
+    """op = A * B^T (A m x k at a[aoff + i + p*lda], B n x k)."""
+    if upper_packed:
+        out = []
+        for j in range(n):
+            for i in range(j + 1):
+                out.append(exact_dot(((a[aoff + i + p * lda], b[boff + j + p * ldb]) for p in range(k)), nl))
+        return out
+    out = [None] * (m * n)
+    for j in range(n):
+        for i in range(m):
+            out[i + j * m] = exact_dot(((a[aoff + i + p * lda], b[boff + j + p * ldb]) for p in range(k)), nl)
+    return out
+
+
+def solve_unit_lower_t(m, n, L, loff, ldl, B, boff, ldb, low, nl):
+    if m == 0 or n == 0:
+        return
+    if n <= REC_BASE:
+        for i in range(m):
+            x = [B[boff + i + j * ldb] for j in range(n)]
+            lo = low[i]
+            for j in range(lo, n):
+                dot = exact_dot(((x[q], L[loff + j + q * ldl]) for q in range(lo, j)), nl)
+                x[j] = sub(x[j], dot)
+            for j in range(n):
+                B[boff + i + j * ldb] = x[j]
+        return
+    n1, n2 = n // 2, n - n // 2
+    solve_unit_lower_t(m, n1, L, loff, ldl, B, boff, ldb, [min(f, n1) for f in low], nl)
+    p = exact_product_nt(m, n2, n1, B, boff, ldb, L, loff + n1, ldl, nl)
+    for j in range(n2):
+        for i in range(m):
+            idx = boff + i + (n1 + j) * ldb
+            B[idx] = sub(B[idx], p[i + j * m])
+    solve_unit_lower_t(m, n2, L, loff + n1 + n1 * ldl, ldl, B, boff + n1 * ldb, ldb,
+                       [max(f - n1, 0) for f in low], nl)
+
+
+def ldl_rec(l, ld, n, first, sign, reg, stats, dinv, off=0, doff=0):
+    nl = l[off].n
+    if n <= REC_BASE:
+        d = {}
+        ok = ldl_dots(l, ld, n, first, sign, reg, stats, d, off)
+        for k, v in d.items():
+            dinv[doff + k] = v
+        return ok
+    low, cut = n, None
+    for c in range(n - 1, 0, -1):
+        low = min(low, first[c])
+        if low >= c and (cut is None or abs(c - n // 2) < abs(cut - n // 2)):
+            cut = c
+    if cut is not None:
+        c = cut
+        ok1 = ldl_rec(l, ld, c, first[:c], sign, reg, stats, dinv, off, doff)
+        ok2 = ldl_rec(l, ld, n - c, [f - c for f in first[c:]], lambda k: sign(c + k), reg, stats,
+                      dinv, off + c + c * ld, doff + c)
+        return ok1 and ok2
+    n1, n2 = n // 2, n - n // 2
+    if not ldl_rec(l, ld, n1, first[:n1], sign, reg, stats, dinv, off, doff):
+        return False
+    l11 = [l[off + i + j * ld] for j in range(n1) for i in range(n1)]
+    solve_unit_lower_t(n2, n1, l11, 0, n1, l, off + n1, ld, [min(f, n1) for f in first[n1:]], nl)
+    w = [l[off + n1 + i + j * ld] for j in range(n1) for i in range(n2)]
+    for j in range(n1):
+        for i in range(n2):
+            idx = off + n1 + i + j * ld
+            l[idx] = mul(l[idx], dinv[doff + j])
+    packed = exact_product_nt(n2, n2, n1, w, 0, n2, l, off + n1, ld, nl, upper_packed=True)
+    for a in range(n2):
+        for b in range(a, n2):
+            idx = off + n1 + b + (n1 + a) * ld
+            l[idx] = sub(l[idx], packed[b * (b + 1) // 2 + a])
+    return ldl_rec(l, ld, n2, [max(f - n1, 0) for f in first[n1:]], lambda k: sign(n1 + k), reg,
+                   stats, dinv, off + n1 + n1 * ld, doff + n1)
+
+
+def envelope(l, ld, n):
+    first = []
+    for i in range(n):
+        k = 0
+        while k < i and l[i + k * ld].is_zero():
+            k += 1
+        first.append(k)
+    return first
+
+
+def reference_leaf_matrix(n, nl):
+    """The input of the reference's recursive_leaf_factor test
+    (arrow.rs:4276-4300): signs, envelope with a cut and ragged rows, a tiny
+    pivot regularized by (1e-20, 1)."""
+    signs = [-1 if k % 5 == 4 else 1 for k in range(n)]
+
+    def first_of(i):
+        if i < 110:
+            return 0
+        if i < 230:
+            return 110
+        return 110 + i % 13
+
+    a = [zero(nl)] * (n * n)
+    for j in range(n):
+        for i in range(j, n):
+            if j < first_of(i):
+                continue
+            if i == j:
+                v = float(signs[i]) * (1e-40 if i % 37 == 3 else float(n + i % 7))
+            else:
+                v = ((i * 29 + j * 13) % 17) / 17.0 - 0.5
+            a[i + j * n] = round_fraction(Fraction(v), nl)
+    reg = (round_fraction(Fraction(1e-20), nl), from_int(1, nl))
+    return a, signs, reg

@@ -277,4 +277,381 @@ void oracle_gemm_entries(int n, char ta, char tb, size_t k, const void *a, size_
     }
 }
 
+/* ------------------------------------------------------------------------
+ * Scalar operations, each one exact value rounded once.
+ * ---------------------------------------------------------------------- */
+typedef struct {
+    int n;
+    size_t stride;
+    uint8_t one[8 * 32 + 8], minus_one[8 * 32 + 8];
+} ora_ctx;
+
+static void ora_init(ora_ctx *c, int n) {
+    c->n = n;
+    c->stride = 8 * (size_t)n + 8;
+    memset(c->one, 0, sizeof c->one);
+    ((uint64_t *)c->one)[n - 1] = 1ull << 63;
+    *(int32_t *)(c->one + 8 * n) = KIND_REG;
+    *(int32_t *)(c->one + 8 * n + 4) = 1;
+    memcpy(c->minus_one, c->one, c->stride);
+    *(int32_t *)(c->minus_one + 8 * n) = -KIND_REG;
+}
+
+/* out = RN(sum_i x[i] * y[i]) over pointer lists */
+static void ora_dot_ptrs(const ora_ctx *c, int k, const uint8_t **x, const uint8_t **y, uint8_t *out) {
+    /* gather into a contiguous pair of temporaries for exact_dot */
+    size_t st = c->stride;
+    uint8_t *xa = malloc(st * (k ? k : 1)), *ya = malloc(st * (k ? k : 1));
+    size_t *idx = malloc(sizeof(size_t) * (k ? k : 1));
+    for (int p = 0; p < k; p++) {
+        memcpy(xa + p * st, x[p], st);
+        memcpy(ya + p * st, y[p], st);
+        idx[p] = (size_t)p;
+    }
+    exact_dot(c->n, k, xa, st, idx, ya, st, idx, out, st, 0);
+    free(xa);
+    free(ya);
+    free(idx);
+}
+
+static void ora_sub(const ora_ctx *c, const uint8_t *a, const uint8_t *b, uint8_t *out) {
+    const uint8_t *x[2] = {a, b}, *y[2] = {c->one, c->minus_one};
+    ora_dot_ptrs(c, 2, x, y, out);
+}
+static void ora_mul(const ora_ctx *c, const uint8_t *a, const uint8_t *b, uint8_t *out) {
+    const uint8_t *x[1] = {a}, *y[1] = {b};
+    ora_dot_ptrs(c, 1, x, y, out);
+}
+static int ora_is_zero(const ora_ctx *c, const uint8_t *v) { return abs(*(const int32_t *)(v + 8 * c->n)) == KIND_ZERO; }
+static int ora_is_finite(const ora_ctx *c, const uint8_t *v) {
+    int k = abs(*(const int32_t *)(v + 8 * c->n));
+    return k == KIND_ZERO || k == KIND_REG;
+}
+static int ora_is_neg(const ora_ctx *c, const uint8_t *v) { return *(const int32_t *)(v + 8 * c->n) < 0; }
+static void ora_neg(const ora_ctx *c, uint8_t *v) {
+    int32_t *k = (int32_t *)(v + 8 * c->n);
+    if (abs(*k) == KIND_REG || abs(*k) == KIND_INF) *k = -*k;
+}
+
+/* RN(1/d): quotient digits of 2^(2P+3) / M by shift-and-subtract over
+ * 32-bit words, remainder as the sticky bit. */
+static void ora_recip(const ora_ctx *c, const uint8_t *d, uint8_t *out) {
+    int n = c->n, P = 64 * n;
+    const uint64_t *m = (const uint64_t *)d;
+    int32_t e = *(const int32_t *)(d + 8 * n + 4);
+    int words = 2 * n + 2; /* remainder words (32-bit) */
+    uint32_t rem[70] = {0}, mw[70] = {0};
+    for (int i = 0; i < n; i++) {
+        mw[2 * i] = (uint32_t)m[i];
+        mw[2 * i + 1] = (uint32_t)(m[i] >> 32);
+    }
+    int qbits = P + 4;
+    uint64_t q[40] = {0};
+    /* numerator 2^(2P+3): bits 2P+3 .. 0; quotient bits below qbits only */
+    for (int bit = 2 * P + 3; bit >= 0; bit--) {
+        uint32_t carry = (bit == 2 * P + 3);
+        for (int i = 0; i < words; i++) {
+            uint32_t v = rem[i];
+            rem[i] = (v << 1) | carry;
+            carry = v >> 31;
+        }
+        int ge = 1;
+        for (int i = words - 1; i >= 0; i--) {
+            if (rem[i] != mw[i]) {
+                ge = rem[i] > mw[i];
+                break;
+            }
+        }
+        if (ge) {
+            int64_t br = 0;
+            for (int i = 0; i < words; i++) {
+                int64_t t = (int64_t)rem[i] - mw[i] - br;
+                br = t < 0;
+                rem[i] = (uint32_t)t;
+            }
+            if (bit < qbits + 2) q[bit / 64] |= 1ull << (bit % 64);
+        }
+    }
+    int sticky = 0;
+    for (int i = 0; i < words; i++) sticky |= rem[i] != 0;
+    /* 1/d = 2^(P-e)/M = (2^(2P+3)/M) 2^(-P-3-e); append sticky below */
+    uint64_t mag[41] = {0};
+    uint64_t carry = (uint64_t)sticky;
+    for (int i = 0; i < 40; i++) {
+        mag[i] = (q[i] << 1) | carry;
+        carry = q[i] >> 63;
+    }
+    mag[40] = carry;
+    from_scaled_integer(ora_is_neg(c, d), mag, 41, -(int64_t)P - 3 - e - 1, n, out, c->stride, 0);
+}
+
+/* ------------------------------------------------------------------------
+ * LDL^T restatement (arrow.rs:156-490): ldl_dots, solve_unit_lower_t,
+ * ldl_rec with envelope cuts.  Products are exact dots rounded once.
+ * ---------------------------------------------------------------------- */
+#define AT(base, idx) ((base) + (size_t)(idx) * c->stride)
+#define TRI(i) ((size_t)(i) * ((size_t)(i) + 1) / 2)
+
+typedef struct {
+    int reg;
+    const uint8_t *eps, *delta;
+    const int8_t *signs;
+    size_t nsigns, offset;
+    long long regularized;
+    int threads;
+} ora_ldl_opts;
+
+static int ora_sign(const ora_ldl_opts *o, size_t k) {
+    if (o->nsigns == 0) return 1;
+    return o->signs[o->nsigns == 1 ? 0 : o->offset + k];
+}
+
+static int ora_pivot(const ora_ctx *c, uint8_t *d, int sign, ora_ldl_opts *o) {
+    if (!ora_is_finite(c, d)) return 0;
+    if (o->reg) {
+        uint8_t t[8 * 32 + 8], diff[8 * 32 + 8];
+        memcpy(t, d, c->stride);
+        if (sign < 0) ora_neg(c, t);
+        ora_sub(c, t, o->eps, diff);
+        if (!ora_is_zero(c, diff) && ora_is_neg(c, diff)) {
+            memcpy(d, o->delta, c->stride);
+            if (sign < 0) ora_neg(c, d);
+            o->regularized++;
+        }
+    }
+    return !ora_is_zero(c, d);
+}
+
+static int ora_ldl_dots(const ora_ctx *c, uint8_t *l, size_t ld, size_t n, const size_t *first,
+                        ora_ldl_opts *o, uint8_t *dinv) {
+    size_t st = c->stride;
+    uint8_t *r = calloc(TRI(n) + 1, st), *w = calloc(TRI(n) + 1, st), *d = calloc(n + 1, st);
+    int ok = 1;
+    for (size_t j = 0; j < n && ok; j++) {
+        size_t fj = first[j];
+        size_t *idx = malloc(sizeof(size_t) * (j + 1));
+        for (size_t q = 0; q < j; q++) idx[q] = TRI(j) + q;
+        uint8_t t[8 * 32 + 8];
+        exact_dot(c->n, (int)(j - fj), r, st, idx + fj, w, st, idx + fj, t, st, 0);
+        uint8_t *dj = AT(d, j);
+        ora_sub(c, AT(l, j + j * ld), t, dj);
+        if (ora_is_zero(c, dj) && j > fj) {
+            /* exact a_jj - sum, rounded once */
+            size_t k = j - fj + 1;
+            const uint8_t **xs = malloc(sizeof(uint8_t *) * k), **ys = malloc(sizeof(uint8_t *) * k);
+            for (size_t q = 0; q < j - fj; q++) {
+                xs[q] = AT(r, TRI(j) + fj + q);
+                ys[q] = AT(w, TRI(j) + fj + q);
+            }
+            xs[k - 1] = AT(l, j + j * ld);
+            ys[k - 1] = c->minus_one;
+            ora_dot_ptrs(c, (int)k, xs, ys, dj);
+            ora_neg(c, dj);
+            free(xs);
+            free(ys);
+        }
+        free(idx);
+        if (!ora_pivot(c, dj, ora_sign(o, j), o)) {
+            ok = 0;
+            break;
+        }
+        ora_recip(c, dj, AT(dinv, j));
+#pragma omp parallel for schedule(dynamic, 8) num_threads(o->threads > 0 ? o->threads : 1)
+        for (long long ii = (long long)j + 1; ii < (long long)n; ii++) {
+            size_t i = (size_t)ii;
+            uint8_t *rij = AT(r, TRI(i) + j);
+            if (first[i] > j) {
+                write_zero(rij, st, 0, c->n);
+            } else {
+                size_t lo = first[i] > first[j] ? first[i] : first[j];
+                size_t *xi = malloc(sizeof(size_t) * (j + 1)), *yi = malloc(sizeof(size_t) * (j + 1));
+                for (size_t q = lo; q < j; q++) {
+                    xi[q - lo] = TRI(i) + q;
+                    yi[q - lo] = TRI(j) + q;
+                }
+                uint8_t t2[8 * 32 + 8], diff[8 * 32 + 8];
+                exact_dot(c->n, (int)(j - lo), r, st, xi, w, st, yi, t2, st, 0);
+                ora_sub(c, AT(l, i + j * ld), t2, diff);
+                ora_mul(c, diff, AT(dinv, j), rij);
+                free(xi);
+                free(yi);
+            }
+            ora_mul(c, rij, dj, AT(w, TRI(i) + j));
+        }
+    }
+    if (ok) {
+        for (size_t j = 0; j < n; j++) {
+            memcpy(AT(l, j + j * ld), c->one, st);
+            for (size_t i = j + 1; i < n; i++) memcpy(AT(l, i + j * ld), AT(r, TRI(i) + j), st);
+        }
+    }
+    free(r);
+    free(w);
+    free(d);
+    return ok;
+}
+
+/* c_out (m x n, ld m) = A (m x k, lda) * B^T where B is n x k (ldb). */
+static void ora_product_nt(const ora_ctx *c, size_t m, size_t n, size_t k, const uint8_t *a,
+                           size_t lda, const uint8_t *b, size_t ldb, uint8_t *out, int upper_packed,
+                           int threads) {
+    oracle_gemm_exact(c->n, 'N', 'T', m, n, k, a, lda, b, ldb, upper_packed, upper_packed, out, threads);
+}
+
+static void ora_solve_unit_lower_t(const ora_ctx *c, size_t m, size_t n, const uint8_t *L, size_t ldl,
+                                   uint8_t *B, size_t ldb, const size_t *low, int threads) {
+    size_t st = c->stride;
+    if (m == 0 || n == 0) return;
+    if (n <= 128) {
+#pragma omp parallel for schedule(dynamic, 4) num_threads(threads > 0 ? threads : 1)
+        for (long long ii = 0; ii < (long long)m; ii++) {
+            size_t i = (size_t)ii;
+            uint8_t *x = malloc(st * n);
+            for (size_t j = 0; j < n; j++) memcpy(x + j * st, AT(B, i + j * ldb), st);
+            size_t lo = low[i];
+            size_t *xi = malloc(sizeof(size_t) * (n + 1)), *yi = malloc(sizeof(size_t) * (n + 1));
+            for (size_t j = lo; j < n; j++) {
+                for (size_t q = lo; q < j; q++) {
+                    xi[q - lo] = q;
+                    yi[q - lo] = j + q * ldl;
+                }
+                uint8_t t[8 * 32 + 8], u[8 * 32 + 8];
+                exact_dot(c->n, (int)(j - lo), x, st, xi, L, st, yi, t, st, 0);
+                ora_sub(c, x + j * st, t, u);
+                memcpy(x + j * st, u, st);
+            }
+            for (size_t j = 0; j < n; j++) memcpy(AT(B, i + j * ldb), x + j * st, st);
+            free(x);
+            free(xi);
+            free(yi);
+        }
+        return;
+    }
+    size_t n1 = n / 2, n2 = n - n / 2;
+    size_t *low1 = malloc(sizeof(size_t) * m), *low2 = malloc(sizeof(size_t) * m);
+    for (size_t i = 0; i < m; i++) {
+        low1[i] = low[i] < n1 ? low[i] : n1;
+        low2[i] = low[i] > n1 ? low[i] - n1 : 0;
+    }
+    ora_solve_unit_lower_t(c, m, n1, L, ldl, B, ldb, low1, threads);
+    uint8_t *p = malloc(st * m * n2);
+    ora_product_nt(c, m, n2, n1, B, ldb, AT(L, n1), ldl, p, 0, threads);
+    uint8_t *B2 = AT(B, n1 * ldb);
+    for (size_t j = 0; j < n2; j++)
+        for (size_t i = 0; i < m; i++) {
+            uint8_t t[8 * 32 + 8];
+            ora_sub(c, AT(B2, i + j * ldb), AT(p, i + j * m), t);
+            memcpy(AT(B2, i + j * ldb), t, st);
+        }
+    free(p);
+    ora_solve_unit_lower_t(c, m, n2, AT(L, n1 + n1 * ldl), ldl, B2, ldb, low2, threads);
+    free(low1);
+    free(low2);
+}
+
+static int ora_ldl_rec(const ora_ctx *c, uint8_t *l, size_t ld, size_t n, const size_t *first,
+                       ora_ldl_opts *o, uint8_t *dinv) {
+    size_t st = c->stride;
+    if (n <= 128) return ora_ldl_dots(c, l, ld, n, first, o, dinv);
+    size_t low = n, cut = 0;
+    int have = 0;
+    for (size_t cc = n - 1; cc >= 1; cc--) {
+        if (first[cc] < low) low = first[cc];
+        size_t dc = cc > n / 2 ? cc - n / 2 : n / 2 - cc;
+        size_t dk = cut > n / 2 ? cut - n / 2 : n / 2 - cut;
+        if (low >= cc && (!have || dc < dk)) {
+            cut = cc;
+            have = 1;
+        }
+    }
+    if (have) {
+        size_t *tail = malloc(sizeof(size_t) * (n - cut));
+        for (size_t i = cut; i < n; i++) tail[i - cut] = first[i] - cut;
+        int ok1 = ora_ldl_rec(c, l, ld, cut, first, o, dinv);
+        size_t save = o->offset;
+        o->offset = save + cut;
+        int ok2 = ora_ldl_rec(c, AT(l, cut + cut * ld), ld, n - cut, tail, o, AT(dinv, cut));
+        o->offset = save;
+        free(tail);
+        return ok1 && ok2;
+    }
+    size_t n1 = n / 2, n2 = n - n / 2;
+    if (!ora_ldl_rec(c, l, ld, n1, first, o, dinv)) return 0;
+    uint8_t *l11 = malloc(st * n1 * n1);
+    for (size_t j = 0; j < n1; j++) memcpy(l11 + j * n1 * st, AT(l, j * ld), n1 * st);
+    size_t *lowv = malloc(sizeof(size_t) * n2);
+    for (size_t i = 0; i < n2; i++) lowv[i] = first[n1 + i] < n1 ? first[n1 + i] : n1;
+    uint8_t *a21 = AT(l, n1);
+    ora_solve_unit_lower_t(c, n2, n1, l11, n1, a21, ld, lowv, o->threads);
+    free(l11);
+    free(lowv);
+    uint8_t *w = malloc(st * n2 * n1);
+    for (size_t j = 0; j < n1; j++) memcpy(w + j * n2 * st, AT(a21, j * ld), n2 * st);
+    for (size_t j = 0; j < n1; j++)
+        for (size_t i = 0; i < n2; i++) {
+            uint8_t t[8 * 32 + 8];
+            ora_mul(c, AT(a21, i + j * ld), AT(dinv, j), t);
+            memcpy(AT(a21, i + j * ld), t, st);
+        }
+    uint8_t *packed = malloc(st * (n2 * (n2 + 1) / 2));
+    ora_product_nt(c, n2, n2, n1, w, n2, a21, ld, packed, 1, o->threads);
+    free(w);
+    uint8_t *a22 = AT(l, n1 + n1 * ld);
+    for (size_t a = 0; a < n2; a++)
+        for (size_t b = a; b < n2; b++) {
+            uint8_t t[8 * 32 + 8];
+            ora_sub(c, AT(a22, b + a * ld), AT(packed, b * (b + 1) / 2 + a), t);
+            memcpy(AT(a22, b + a * ld), t, st);
+        }
+    free(packed);
+    size_t *tail = malloc(sizeof(size_t) * n2);
+    for (size_t i = 0; i < n2; i++) tail[i] = first[n1 + i] > n1 ? first[n1 + i] - n1 : 0;
+    size_t save = o->offset;
+    o->offset = save + n1;
+    int ok = ora_ldl_rec(c, a22, ld, n2, tail, o, AT(dinv, n1));
+    o->offset = save;
+    free(tail);
+    return ok;
+}
+
+/* Entry point: factor the lower triangle of l (n x n, ld) in place.
+ * recursive != 0: ldl_rec, else ldl_dots.  reg_eps/reg_delta NULL: no
+ * regularization.  Returns 1 on success; *regularized counts replaced
+ * pivots. */
+int oracle_ldl(int nl, uint8_t *l, size_t ld, size_t n, const int8_t *signs, size_t nsigns,
+               const uint8_t *reg_eps, const uint8_t *reg_delta, uint8_t *dinv, int recursive,
+               long long *regularized, int threads) {
+    ora_ctx c;
+    ora_init(&c, nl);
+    size_t *first = malloc(sizeof(size_t) * (n ? n : 1));
+    for (size_t i = 0; i < n; i++) {
+        size_t k = 0;
+        while (k < i && abs(*(int32_t *)(l + (i + k * ld) * c.stride + 8 * nl)) == KIND_ZERO) k++;
+        first[i] = k;
+    }
+    ora_ldl_opts o = {reg_eps != NULL, reg_eps, reg_delta, signs, nsigns, 0, 0, threads};
+    int ok = recursive ? ora_ldl_rec(&c, l, ld, n, first, &o, dinv)
+                       : ora_ldl_dots(&c, l, ld, n, first, &o, dinv);
+    if (regularized) *regularized = o.regularized;
+    free(first);
+    return ok;
+}
+
+/* Scalar checks for the tests. */
+void oracle_recip(int nl, const uint8_t *d, uint8_t *out) {
+    ora_ctx c;
+    ora_init(&c, nl);
+    ora_recip(&c, d, out);
+}
+void oracle_sub(int nl, const uint8_t *a, const uint8_t *b, uint8_t *out) {
+    ora_ctx c;
+    ora_init(&c, nl);
+    ora_sub(&c, a, b, out);
+}
+void oracle_mul(int nl, const uint8_t *a, const uint8_t *b, uint8_t *out) {
+    ora_ctx c;
+    ora_init(&c, nl);
+    ora_mul(&c, a, b, out);
+}
+
 int oracle_abi(void) { return 1; }

@@ -2,6 +2,7 @@
 // Mirrors crates/ffi/src/lib.rs:30-43 (boundary(): errors become codes and a
 // thread-local message; nothing unwinds across the ABI).
 #include <cstring>
+#include <functional>
 #include <string>
 
 #include "common.h"
@@ -34,6 +35,14 @@ int32_t invalid(const char *what) {
     t_error = what;
     return SDPX_GPU_INVALID;
 }
+}  // namespace
+
+namespace sdpx {
+int32_t abi_boundary(const std::function<int32_t()> &f) { return boundary(f); }
+int32_t abi_invalid(const char *what) { return invalid(what); }
+}  // namespace sdpx
+
+namespace {
 
 bool trans_ok(uint8_t t) { return t == 'N' || t == 'n' || t == 'T' || t == 't' || t == 'C' || t == 'c'; }
 char trans_norm(uint8_t t) { return (t == 'N' || t == 'n') ? 'N' : 'T'; }

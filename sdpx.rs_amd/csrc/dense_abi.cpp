@@ -1,0 +1,210 @@
+// C ABI of the Float64 dense factor/solve and PSD Schur assembly.
+#include <cstring>
+#include <functional>
+
+#include "common.h"
+#include "fp64_dense.h"
+
+struct sdpx_gpu_dense {
+    int64_t n = 0;
+    double *L = nullptr;   // device n x n, lower factor after refactor
+    int *info = nullptr;   // device
+    double *ws = nullptr;  // split-K workspace
+    size_t ws_bytes = 0;
+    double *diag = nullptr;  // device copy of the unfactored diagonal
+    bool factored = false;
+    bool rule = false;
+    double eps = 0, delta = 0;
+};
+
+namespace sdpx {
+int32_t abi_boundary(const std::function<int32_t()> &f);
+int32_t abi_invalid(const char *what);
+}  // namespace sdpx
+
+using sdpx::abi_boundary;
+using sdpx::abi_invalid;
+
+namespace {
+
+int32_t refactor_impl(sdpx_gpu_dense *h, const double *a, uint64_t lda, bool dev, int32_t *info) {
+    if (!h) return abi_invalid("null handle");
+    if (lda < (uint64_t)h->n) return abi_invalid("lda below n");
+    sdpx::Device &d = sdpx::device();
+    std::lock_guard<std::recursive_mutex> g(d.mu);
+    hipStream_t s = d.stream;
+    const int64_t n = h->n;
+    if (n == 0) {
+        if (info) *info = 0;
+        h->factored = true;
+        return SDPX_GPU_OK;
+    }
+    HIP_CHECK(hipMemcpy2DAsync(h->L, n * 8, a, lda * 8, n * 8, n,
+                               dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+    sdpx::mirror_upper(h->L, n, n, s);
+    HIP_CHECK(hipMemsetAsync(h->info, 0, sizeof(int), s));
+    sdpx::PivotRule rule{h->eps, h->delta, h->diag};
+    if (h->rule)
+        HIP_CHECK(hipMemcpy2DAsync(h->diag, 8, h->L, (n + 1) * 8, 8, n, hipMemcpyDeviceToDevice, s));
+    sdpx::potrf_f64(h->L, n, n, h->info, h->ws, h->ws_bytes, s, h->rule ? &rule : nullptr);
+    int hinfo = 0;
+    HIP_CHECK(hipMemcpyAsync(&hinfo, h->info, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (info) *info = hinfo;
+    h->factored = hinfo == 0;
+    return SDPX_GPU_OK;
+}
+
+int32_t solve_impl(sdpx_gpu_dense *h, double *b, uint64_t nrhs, uint64_t ldb, bool dev) {
+    if (!h) return abi_invalid("null handle");
+    if (!h->factored) return abi_invalid("no successful factorization");
+    if (ldb < (uint64_t)h->n) return abi_invalid("ldb below n");
+    sdpx::Device &d = sdpx::device();
+    std::lock_guard<std::recursive_mutex> g(d.mu);
+    hipStream_t s = d.stream;
+    const int64_t n = h->n;
+    if (n == 0 || nrhs == 0) return SDPX_GPU_OK;
+    double *db = b;
+    int64_t ld = (int64_t)ldb;
+    if (!dev) {
+        db = (double *)d.ws.get(40, sizeof(double) * n * nrhs);
+        ld = n;
+        HIP_CHECK(hipMemcpy2DAsync(db, n * 8, b, ldb * 8, n * 8, nrhs, hipMemcpyHostToDevice, s));
+    }
+    sdpx::potrs_f64(h->L, n, n, db, (int64_t)nrhs, ld, h->ws, h->ws_bytes, s);
+    if (!dev)
+        HIP_CHECK(hipMemcpy2DAsync(b, ldb * 8, db, n * 8, n * 8, nrhs, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    return SDPX_GPU_OK;
+}
+
+int32_t schur_impl(uint64_t n, uint64_t m, const double *rinv, const double *a_svec, double *S,
+                   uint64_t lds, bool dev) {
+    if (lds < m) return abi_invalid("lds below m");
+    if (n == 0 || m == 0) return SDPX_GPU_OK;
+    sdpx::Device &d = sdpx::device();
+    std::lock_guard<std::recursive_mutex> g(d.mu);
+    hipStream_t s = d.stream;
+    const uint64_t sv = n * (n + 1) / 2;
+    const double *dr = rinv, *da = a_svec;
+    double *dS = S;
+    int64_t ld = (int64_t)lds;
+    if (!dev) {
+        double *r = (double *)d.ws.get(41, 8 * n * n);
+        double *a = (double *)d.ws.get(42, 8 * sv * m);
+        dS = (double *)d.ws.get(43, 8 * m * m);
+        ld = (int64_t)m;
+        HIP_CHECK(hipMemcpyAsync(r, rinv, 8 * n * n, hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(a, a_svec, 8 * sv * m, hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipMemcpy2DAsync(dS, m * 8, S, lds * 8, m * 8, m, hipMemcpyHostToDevice, s));
+        dr = r;
+        da = a;
+    }
+    sdpx::schur_psd_f64((int64_t)n, (int64_t)m, dr, da, dS, ld, s);
+    if (!dev)
+        HIP_CHECK(hipMemcpy2DAsync(S, lds * 8, dS, m * 8, m * 8, m, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    return SDPX_GPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t sdpx_gpu_dense_create(uint64_t n, sdpx_gpu_dense **out) {
+    return abi_boundary([&]() -> int32_t {
+        if (!out) return abi_invalid("null output handle");
+        if (n > (1u << 20)) return abi_invalid("order too large");
+        sdpx::Device &d = sdpx::device();
+        std::lock_guard<std::recursive_mutex> g(d.mu);
+        auto *h = new sdpx_gpu_dense();
+        h->n = (int64_t)n;
+        try {
+            HIP_CHECK(hipMalloc(&h->L, 8 * std::max<uint64_t>(n * n, 1)));
+            HIP_CHECK(hipMalloc(&h->info, sizeof(int)));
+            h->ws_bytes = 8 * std::max<uint64_t>(n * n, 1) * 8;
+            HIP_CHECK(hipMalloc(&h->ws, h->ws_bytes));
+            HIP_CHECK(hipMalloc(&h->diag, 8 * std::max<uint64_t>(n, 1)));
+        } catch (...) {
+            if (h->L) (void)hipFree(h->L);
+            if (h->info) (void)hipFree(h->info);
+            delete h;
+            throw;
+        }
+        *out = h;
+        return SDPX_GPU_OK;
+    });
+}
+
+int32_t sdpx_gpu_dense_destroy(sdpx_gpu_dense *h) {
+    return abi_boundary([&]() -> int32_t {
+        if (!h) return SDPX_GPU_OK;
+        sdpx::Device &d = sdpx::device();
+        std::lock_guard<std::recursive_mutex> g(d.mu);
+        HIP_CHECK(hipStreamSynchronize(d.stream));
+        (void)hipFree(h->L);
+        (void)hipFree(h->info);
+        (void)hipFree(h->ws);
+        (void)hipFree(h->diag);
+        delete h;
+        return SDPX_GPU_OK;
+    });
+}
+
+int32_t sdpx_gpu_dense_set_dynamic_pivot(sdpx_gpu_dense *h, int32_t enable, double eps,
+                                         double delta) {
+    return abi_boundary([&]() -> int32_t {
+        if (!h) return abi_invalid("null handle");
+        if (enable && !(delta > 0.0)) return abi_invalid("delta must be positive");
+        h->rule = enable != 0;
+        h->eps = eps;
+        h->delta = delta;
+        return SDPX_GPU_OK;
+    });
+}
+
+int32_t sdpx_gpu_dense_refactor(sdpx_gpu_dense *h, const double *a, uint64_t lda, int32_t *info) {
+    return abi_boundary([&] { return refactor_impl(h, a, lda, false, info); });
+}
+int32_t sdpx_gpu_dense_refactor_dev(sdpx_gpu_dense *h, const double *a, uint64_t lda,
+                                    int32_t *info) {
+    return abi_boundary([&] { return refactor_impl(h, a, lda, true, info); });
+}
+int32_t sdpx_gpu_dense_solve_many(sdpx_gpu_dense *h, double *b, uint64_t nrhs, uint64_t ldb) {
+    return abi_boundary([&] { return solve_impl(h, b, nrhs, ldb, false); });
+}
+int32_t sdpx_gpu_dense_solve_many_dev(sdpx_gpu_dense *h, double *b, uint64_t nrhs, uint64_t ldb) {
+    return abi_boundary([&] { return solve_impl(h, b, nrhs, ldb, true); });
+}
+int32_t sdpx_gpu_dense_factor_dev_ptr(sdpx_gpu_dense *h, const double **l) {
+    return abi_boundary([&]() -> int32_t {
+        if (!h || !l) return abi_invalid("null argument");
+        *l = h->L;
+        return SDPX_GPU_OK;
+    });
+}
+int32_t sdpx_gpu_dense_get_factor(sdpx_gpu_dense *h, double *out, uint64_t ldo) {
+    return abi_boundary([&]() -> int32_t {
+        if (!h || !out) return abi_invalid("null argument");
+        if (!h->factored) return abi_invalid("no successful factorization");
+        if (ldo < (uint64_t)h->n) return abi_invalid("ldo below n");
+        sdpx::Device &d = sdpx::device();
+        std::lock_guard<std::recursive_mutex> g(d.mu);
+        if (h->n)
+            HIP_CHECK(hipMemcpy2DAsync(out, ldo * 8, h->L, h->n * 8, h->n * 8, h->n,
+                                       hipMemcpyDeviceToHost, d.stream));
+        HIP_CHECK(hipStreamSynchronize(d.stream));
+        return SDPX_GPU_OK;
+    });
+}
+
+int32_t sdpx_gpu_schur_psd_f64(uint64_t n, uint64_t m, const double *rinv, const double *a_svec,
+                               double *s, uint64_t lds) {
+    return abi_boundary([&] { return schur_impl(n, m, rinv, a_svec, s, lds, false); });
+}
+int32_t sdpx_gpu_schur_psd_f64_dev(uint64_t n, uint64_t m, const double *rinv,
+                                   const double *a_svec, double *s, uint64_t lds) {
+    return abi_boundary([&] { return schur_impl(n, m, rinv, a_svec, s, lds, true); });
+}
+
+}  // extern "C"

@@ -1,0 +1,46 @@
+// Float64 dense kernels (see fp64_dense.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace sdpx {
+
+struct Fp64Gemm {
+    const double *A = nullptr, *B = nullptr;
+    double *C = nullptr;
+    int64_t m = 0, n = 0, k = 0, lda = 1, ldb = 1, ldc = 1;
+    int64_t sa = 0, sb = 0, sc = 0, batch = 1;
+    double alpha = 1.0, beta = 0.0;
+    bool ta = false, tb = false;
+    int tri = 0;                // 0 all, 1 upper (i <= j), 2 lower (i >= j)
+    double *workspace = nullptr;  // split-K partials (optional)
+    size_t workspace_bytes = 0;
+};
+
+// C = alpha op(A) op(B) + beta C (column-major, batched).
+void gemm_f64(const Fp64Gemm &g, hipStream_t s);
+// Dynamic pivot rule of the reference's dense factor (dense_block.rs:329-365):
+// diag is the unfactored diagonal on the device.
+struct PivotRule {
+    double eps, delta;
+    const double *diag;
+};
+// In-place lower Cholesky; *d_info (device) = 0 or the failing column + 1.
+void potrf_f64(double *A, int64_t n, int64_t lda, int *d_info, double *ws, size_t ws_bytes,
+               hipStream_t s, const PivotRule *rule = nullptr);
+// B <- (L L^T)^-1 B with the lower factor L.
+void potrs_f64(const double *L, int64_t n, int64_t ldl, double *B, int64_t nrhs, int64_t ldb,
+               double *ws, size_t ws_bytes, hipStream_t s);
+// S (m x m, upper triangle, lds) = sum over svec coordinates of
+// svec(Rinv A_a Rinv^T) svec(Rinv A_b Rinv^T); a_svec is s x m (s = n(n+1)/2).
+void schur_psd_f64(int64_t n, int64_t m, const double *rinv, const double *a_svec, double *S,
+                   int64_t lds, hipStream_t st);
+
+// Copy the upper triangle onto the lower one.
+void mirror_upper(double *A, int64_t n, int64_t lda, hipStream_t s);
+
+__global__ void svec_unpack(const double *svec, int64_t s, int64_t n, int64_t m, double *out);
+__global__ void svec_pack(const double *U, int64_t s, int64_t n, int64_t m, double *M);
+
+}  // namespace sdpx

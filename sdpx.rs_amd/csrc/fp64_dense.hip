@@ -1,0 +1,526 @@
+// Float64 dense kernels of the Schur path on gfx950: FP64 MFMA GEMM/SYRK
+// (v_mfma_f64_16x16x4_f64), blocked Cholesky, triangular solves, and the
+// PSD-block Schur assembly S = M^T M with M_b = svec(Rinv A_b Rinv^T).
+//
+// Reference path: crates/solver/src/solver/kkt/ldl/dense_block.rs
+// (tiled_potrf 515-594, tiled_trsv 604-669, solve 795-851: LAPACK/BLAS on
+// CPU) and crates/solver/src/solver/kkt/condensed/psd.rs:1432-1456
+// (svec(Ginv A Ginv) with Ginv = Rinv^T Rinv, kkt.rs:1119-1127, dotted with
+// svec(A_a)).  Float64 results are pinned by tolerance (1e-12 relative),
+// not bits: the reference's own BLAS fixes no summation order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "common.h"
+#include "fp64_dense.h"
+
+namespace sdpx {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// GEMM: C = alpha op(A) op(B) + beta C, batched, optional triangle.
+// 64x64 output tile per workgroup, 4 waves of 32x32 (2x2 MFMA 16x16 blocks),
+// k staged 16 at a time through LDS with a register prefetch.
+// tri: 0 all, 1 only i <= j (upper), 2 only i >= j (lower).
+// ---------------------------------------------------------------------------
+constexpr int GT = 64, GK = 16, GPAD = 68;
+
+struct GemmDesc {
+    const double *A, *B;
+    double *C;
+    int64_t m, n, k, lda, ldb, ldc;
+    int64_t sa, sb, sc;  // batch strides (elements)
+    double alpha, beta;
+    int ta, tb;          // 1 when transposed
+    int tri;
+    int64_t k_split;     // inner elements per split (split-K partials)
+    double *partial;     // when splits > 1: [split][m*n] partial products
+};
+
+__global__ __launch_bounds__(256) void gemm_f64_kernel(GemmDesc d, int tiles_m, int tiles_n,
+                                                       int splits) {
+    __shared__ double sA[GK][GPAD];
+    __shared__ double sB[GK][GPAD];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int t = blockIdx.x;
+    const int tm = t % tiles_m;
+    t /= tiles_m;
+    const int tn = t % tiles_n;
+    const int split = t / tiles_n;
+    const int batch = blockIdx.y;
+    const int64_t i0 = (int64_t)tm * GT, j0 = (int64_t)tn * GT;
+    if (d.tri == 1 && i0 > j0 + GT - 1) return;
+    if (d.tri == 2 && j0 > i0 + GT - 1) return;
+    const double *A = d.A + batch * d.sa;
+    const double *B = d.B + batch * d.sb;
+    const int64_t kb = (int64_t)split * d.k_split;
+    const int64_t ke = min(d.k, kb + d.k_split);
+
+    double ra[4], rb[4];
+    auto load = [&](int64_t k0) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            int e = tid + 256 * u;
+            int ii, pp;
+            if (!d.ta) { ii = e & 63; pp = e >> 6; } else { pp = e & 15; ii = e >> 4; }
+            int64_t gi = i0 + ii, gp = k0 + pp;
+            double v = 0.0;
+            if (gi < d.m && gp < ke) v = d.ta ? A[gp + gi * d.lda] : A[gi + gp * d.lda];
+            ra[u] = v;
+            int jj;
+            if (!d.tb) { pp = e & 15; jj = e >> 4; } else { jj = e & 63; pp = e >> 6; }
+            int64_t gj = j0 + jj;
+            gp = k0 + pp;
+            v = 0.0;
+            if (gj < d.n && gp < ke) v = d.tb ? B[gj + gp * d.ldb] : B[gp + gj * d.ldb];
+            rb[u] = v;
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            int e = tid + 256 * u;
+            int ii, pp, jj, qq;
+            if (!d.ta) { ii = e & 63; pp = e >> 6; } else { pp = e & 15; ii = e >> 4; }
+            sA[pp][ii] = ra[u];
+            if (!d.tb) { qq = e & 15; jj = e >> 4; } else { jj = e & 63; qq = e >> 6; }
+            sB[qq][jj] = rb[u];
+        }
+    };
+    const int wm = wave >> 1, wn = wave & 1;
+    v4d acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) acc[a][b] = v4d{0, 0, 0, 0};
+    const int frow = lane & 15, fk = lane >> 4;
+    load(kb);
+    for (int64_t k0 = kb; k0 < ke; k0 += GK) {
+        __syncthreads();
+        store();
+        __syncthreads();
+        if (k0 + GK < ke) load(k0 + GK);
+#pragma unroll
+        for (int kk = 0; kk < GK; kk += 4) {
+            double fa[2], fb[2];
+#pragma unroll
+            for (int a = 0; a < 2; a++) fa[a] = sA[kk + fk][wm * 32 + a * 16 + frow];
+#pragma unroll
+            for (int b = 0; b < 2; b++) fb[b] = sB[kk + fk][wn * 32 + b * 16 + frow];
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++)
+                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[a], fb[b], acc[a][b], 0, 0, 0);
+        }
+    }
+    // C/D layout: lane l holds rows 4*(l/16) + r, column l % 16
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                int64_t gi = i0 + wm * 32 + a * 16 + 4 * (lane >> 4) + r;
+                int64_t gj = j0 + wn * 32 + b * 16 + (lane & 15);
+                if (gi >= d.m || gj >= d.n) continue;
+                if (d.tri == 1 && gi > gj) continue;
+                if (d.tri == 2 && gi < gj) continue;
+                double v = acc[a][b][r];
+                if (splits > 1) {
+                    d.partial[(size_t)split * d.m * d.n + gi + gj * d.m] = v;
+                } else {
+                    double *c = d.C + batch * d.sc + gi + gj * d.ldc;
+                    *c = d.beta == 0.0 ? d.alpha * v : fma(d.alpha, v, d.beta * *c);
+                }
+            }
+}
+
+// Deterministic split-K reduction: C = alpha sum_s partial[s] + beta C.
+__global__ void splitk_reduce(GemmDesc d, int splits) {
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= d.m * d.n) return;
+    int64_t i = idx % d.m, j = idx / d.m;
+    if (d.tri == 1 && i > j) return;
+    if (d.tri == 2 && i < j) return;
+    double s = 0.0;
+    for (int t = 0; t < splits; t++) s += d.partial[(size_t)t * d.m * d.n + idx];
+    double *c = d.C + i + j * d.ldc;
+    *c = d.beta == 0.0 ? d.alpha * s : fma(d.alpha, s, d.beta * *c);
+}
+
+void gemm_f64(const Fp64Gemm &g, hipStream_t s) {
+    if (g.m <= 0 || g.n <= 0 || g.batch <= 0) return;
+    GemmDesc d{g.A, g.B, g.C, g.m, g.n, g.k, g.lda, g.ldb, g.ldc, g.sa, g.sb, g.sc,
+               g.alpha, g.beta, g.ta ? 1 : 0, g.tb ? 1 : 0, g.tri, 0, nullptr};
+    const int tiles_m = (int)((g.m + GT - 1) / GT), tiles_n = (int)((g.n + GT - 1) / GT);
+    // split the inner dimension when the tile grid cannot fill the chip
+    int64_t tiles = (int64_t)tiles_m * tiles_n * g.batch;
+    int splits = 1;
+    if (g.batch == 1 && g.workspace && g.k >= 1024) {
+        while (tiles * splits < 512 && g.k / (splits * 2) >= 256 && splits < 32) splits *= 2;
+        int64_t need = (int64_t)splits * g.m * g.n;
+        if ((size_t)need * 8 > g.workspace_bytes) splits = 1;
+    }
+    if (g.k == 0) splits = 1;
+    d.k_split = splits > 1 ? ((g.k + splits - 1) / splits + GK - 1) / GK * GK : std::max<int64_t>(g.k, 1);
+    d.partial = splits > 1 ? g.workspace : nullptr;
+    dim3 grid((unsigned)(tiles_m * tiles_n * splits), (unsigned)g.batch);
+    hipLaunchKernelGGL(gemm_f64_kernel, grid, dim3(256), 0, s, d, tiles_m, tiles_n, splits);
+    HIP_CHECK(hipGetLastError());
+    if (splits > 1) {
+        int64_t total = g.m * g.n;
+        hipLaunchKernelGGL(splitk_reduce, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, d,
+                           splits);
+        HIP_CHECK(hipGetLastError());
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Cholesky A = L L^T (lower, column-major), blocked right-looking.
+// ---------------------------------------------------------------------------
+constexpr int NB = 64;
+
+// Unblocked Cholesky of one nb x nb diagonal block in LDS (nb <= 64).  On a
+// non-positive pivot writes info = global column + 1 and stops.
+// With the dynamic pivot rule (dense_block.rs:329-365): a pivot d <= eps
+// becomes delta unless d < -64 eps_f64 nb |diag_k| (clearly indefinite).
+__global__ __launch_bounds__(256) void potrf_diag(double *A, int64_t lda, int64_t j0, int nb,
+                                                  int *info, int rule, double eps, double delta,
+                                                  const double *diag) {
+    __shared__ double s[NB][NB + 1];
+    __shared__ int bad;
+    const int tid = threadIdx.x;
+    if (tid == 0) bad = 0;
+    for (int e = tid; e < nb * nb; e += 256) {
+        int i = e % nb, j = e / nb;
+        s[i][j] = A[(j0 + i) + (j0 + j) * lda];
+    }
+    __syncthreads();
+    if (*info != 0) return;
+    for (int j = 0; j < nb; j++) {
+        if (tid == 0) {
+            double d = s[j][j];
+            if (!std::isfinite(d)) {
+                bad = (int)(j0 + j + 1);
+            } else if (rule && d <= eps) {
+                if (d < -64.0 * 2.220446049250313e-16 * (double)nb * fabs(diag[j0 + j]))
+                    bad = (int)(j0 + j + 1);
+                else
+                    s[j][j] = sqrt(delta);
+            } else if (!(d > 0.0)) {
+                bad = (int)(j0 + j + 1);
+            } else {
+                s[j][j] = sqrt(d);
+            }
+        }
+        __syncthreads();
+        if (bad) break;
+        const double djj = s[j][j];
+        for (int i = j + 1 + tid; i < nb; i += 256) s[i][j] /= djj;
+        __syncthreads();
+        // trailing rank-1 update of the lower triangle
+        for (int e = tid; e < (nb - j - 1) * (nb - j - 1); e += 256) {
+            int i = j + 1 + e % (nb - j - 1), c = j + 1 + e / (nb - j - 1);
+            if (i >= c) s[i][c] -= s[i][j] * s[c][j];
+        }
+        __syncthreads();
+    }
+    if (bad) {
+        if (tid == 0) *info = bad;
+        return;
+    }
+    for (int e = tid; e < nb * nb; e += 256) {
+        int i = e % nb, j = e / nb;
+        if (i >= j) A[(j0 + i) + (j0 + j) * lda] = s[i][j];
+    }
+}
+
+// X L^T = B for the rows below a diagonal block: row panel rows r0..r1,
+// columns j0..j0+nb; one thread per row, forward substitution.
+__global__ __launch_bounds__(256) void trsm_panel(double *A, int64_t lda, int64_t j0, int nb,
+                                                  int64_t r0, int64_t r1, const int *info) {
+    __shared__ double L[NB][NB + 1];
+    if (*info != 0) return;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < nb * nb; e += 256) {
+        int i = e % nb, j = e / nb;
+        L[i][j] = i >= j ? A[(j0 + i) + (j0 + j) * lda] : 0.0;
+    }
+    __syncthreads();
+    int64_t row = r0 + (int64_t)blockIdx.x * 256 + tid;
+    if (row >= r1) return;
+    double x[NB];
+#pragma unroll
+    for (int j = 0; j < NB; j++) x[j] = j < nb ? A[row + (j0 + j) * lda] : 0.0;
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+        if (j < nb) {
+            double v = x[j];
+#pragma unroll
+            for (int l = 0; l < NB; l++)
+                if (l < j) v -= x[l] * L[j][l];
+            x[j] = v / L[j][j];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NB; j++)
+        if (j < nb) A[row + (j0 + j) * lda] = x[j];
+}
+
+void potrf_f64(double *A, int64_t n, int64_t lda, int *d_info, double *ws, size_t ws_bytes,
+               hipStream_t s, const PivotRule *rule) {
+    for (int64_t j0 = 0; j0 < n; j0 += NB) {
+        const int nb = (int)std::min<int64_t>(NB, n - j0);
+        hipLaunchKernelGGL(potrf_diag, dim3(1), dim3(256), 0, s, A, lda, j0, nb, d_info,
+                           rule ? 1 : 0, rule ? rule->eps : 0.0, rule ? rule->delta : 0.0,
+                           rule ? rule->diag : nullptr);
+        HIP_CHECK(hipGetLastError());
+        const int64_t r0 = j0 + nb;
+        if (r0 >= n) break;
+        hipLaunchKernelGGL(trsm_panel, dim3((unsigned)((n - r0 + 255) / 256)), dim3(256), 0, s, A,
+                           lda, j0, nb, r0, n, d_info);
+        HIP_CHECK(hipGetLastError());
+        // A22 -= P P^T (lower triangle)
+        Fp64Gemm g;
+        g.m = g.n = n - r0;
+        g.k = nb;
+        g.A = A + r0 + j0 * lda;
+        g.lda = lda;
+        g.B = A + r0 + j0 * lda;
+        g.ldb = lda;
+        g.tb = true;
+        g.C = A + r0 + r0 * lda;
+        g.ldc = lda;
+        g.alpha = -1.0;
+        g.beta = 1.0;
+        g.tri = 2;
+        g.workspace = ws;
+        g.workspace_bytes = ws_bytes;
+        gemm_f64(g, s);
+    }
+}
+
+// Forward/back substitution with the factor for nrhs columns:
+// lower: solve L Y = B; upper (trans): solve L^T X = Y.  One workgroup per
+// 64-row diagonal block step, RHS columns spread over threads, then a GEMM
+// update of the remaining rows.
+__global__ __launch_bounds__(256) void trsv_block(const double *L, int64_t ldl, double *B,
+                                                  int64_t ldb, int64_t j0, int nb, int64_t nrhs,
+                                                  int trans) {
+    __shared__ double s[NB][NB + 1];
+    const int tid = threadIdx.x;
+    for (int e = tid; e < nb * nb; e += 256) {
+        int i = e % nb, j = e / nb;
+        s[i][j] = i >= j ? L[(j0 + i) + (j0 + j) * ldl] : 0.0;
+    }
+    __syncthreads();
+    int64_t c = (int64_t)blockIdx.x * 256 + tid;
+    if (c >= nrhs) return;
+    double x[NB];
+#pragma unroll
+    for (int i = 0; i < NB; i++) x[i] = i < nb ? B[(j0 + i) + c * ldb] : 0.0;
+    if (!trans) {
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            if (i < nb) {
+                double v = x[i];
+#pragma unroll
+                for (int l = 0; l < NB; l++)
+                    if (l < i) v -= s[i][l] * x[l];
+                x[i] = v / s[i][i];
+            }
+        }
+    } else {
+#pragma unroll
+        for (int ii = NB - 1; ii >= 0; ii--) {
+            if (ii < nb) {
+                double v = x[ii];
+#pragma unroll
+                for (int l = 0; l < NB; l++)
+                    if (l > ii && l < nb) v -= s[l][ii] * x[l];
+                x[ii] = v / s[ii][ii];
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NB; i++)
+        if (i < nb) B[(j0 + i) + c * ldb] = x[i];
+}
+
+void potrs_f64(const double *L, int64_t n, int64_t ldl, double *B, int64_t nrhs, int64_t ldb,
+               double *ws, size_t ws_bytes, hipStream_t s) {
+    const unsigned cgrid = (unsigned)((nrhs + 255) / 256);
+    // forward: L Y = B
+    for (int64_t j0 = 0; j0 < n; j0 += NB) {
+        const int nb = (int)std::min<int64_t>(NB, n - j0);
+        hipLaunchKernelGGL(trsv_block, dim3(cgrid), dim3(256), 0, s, L, ldl, B, ldb, j0, nb, nrhs, 0);
+        HIP_CHECK(hipGetLastError());
+        if (j0 + nb < n) {
+            Fp64Gemm g;  // B[j0+nb:] -= L[j0+nb:, j0:j0+nb] B[j0:j0+nb]
+            g.m = n - j0 - nb;
+            g.n = nrhs;
+            g.k = nb;
+            g.A = L + (j0 + nb) + j0 * ldl;
+            g.lda = ldl;
+            g.B = B + j0;
+            g.ldb = ldb;
+            g.C = B + j0 + nb;
+            g.ldc = ldb;
+            g.alpha = -1.0;
+            g.beta = 1.0;
+            gemm_f64(g, s);
+        }
+    }
+    // backward: L^T X = Y
+    int64_t last = ((n - 1) / NB) * NB;
+    for (int64_t j0 = last; j0 >= 0; j0 -= NB) {
+        const int nb = (int)std::min<int64_t>(NB, n - j0);
+        hipLaunchKernelGGL(trsv_block, dim3(cgrid), dim3(256), 0, s, L, ldl, B, ldb, j0, nb, nrhs, 1);
+        HIP_CHECK(hipGetLastError());
+        if (j0 > 0) {
+            Fp64Gemm g;  // B[0:j0] -= L[j0:j0+nb, 0:j0]^T B[j0:j0+nb]
+            g.m = j0;
+            g.n = nrhs;
+            g.k = nb;
+            g.A = L + j0;
+            g.lda = ldl;
+            g.ta = true;
+            g.B = B + j0;
+            g.ldb = ldb;
+            g.C = B;
+            g.ldc = ldb;
+            g.alpha = -1.0;
+            g.beta = 1.0;
+            gemm_f64(g, s);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// PSD-block Schur assembly.
+// ---------------------------------------------------------------------------
+// svec columns (upper-triangular column order, off-diagonals scaled by
+// sqrt 2) -> dense symmetric matrices stacked vertically: Astack[b*n + i, j].
+__global__ void svec_unpack(const double *svec, int64_t s, int64_t n, int64_t m, double *out) {
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= s * m) return;
+    int64_t b = idx / s, e = idx % s;
+    // e -> (i, j), i <= j, column-major upper
+    int64_t j = (int64_t)((sqrt(8.0 * (double)e + 1.0) - 1.0) / 2.0);
+    while (j * (j + 1) / 2 > e) j--;
+    while ((j + 1) * (j + 2) / 2 <= e) j++;
+    int64_t i = e - j * (j + 1) / 2;
+    double v = svec[e + b * s];
+    if (i != j) v *= 0.70710678118654752440;
+    const int64_t ld = n * m;
+    out[(b * n + i) + j * ld] = v;
+    out[(b * n + j) + i * ld] = v;
+}
+
+// U_b (stacked, ld = n*m) -> svec columns M[:, b] (sqrt 2 off-diagonal),
+// symmetrised as (U_ij + U_ji)/2.
+__global__ void svec_pack(const double *U, int64_t s, int64_t n, int64_t m, double *M) {
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= s * m) return;
+    int64_t b = idx / s, e = idx % s;
+    int64_t j = (int64_t)((sqrt(8.0 * (double)e + 1.0) - 1.0) / 2.0);
+    while (j * (j + 1) / 2 > e) j--;
+    while ((j + 1) * (j + 2) / 2 <= e) j++;
+    int64_t i = e - j * (j + 1) / 2;
+    const int64_t ld = n * m;
+    double v = i == j ? U[(b * n + i) + j * ld]
+                      : (U[(b * n + i) + j * ld] + U[(b * n + j) + i * ld]) * 0.70710678118654752440;
+    M[e + b * s] = v;
+}
+
+}  // namespace sdpx
+
+namespace sdpx {
+
+// Workspace slots 30..35 are this file's.
+void schur_psd_f64(int64_t n, int64_t m, const double *rinv, const double *a_svec, double *S,
+                   int64_t lds, hipStream_t st) {
+    Workspace &ws = device().ws;
+    const int64_t s = n * (n + 1) / 2;
+    double *stack = (double *)ws.get(30, sizeof(double) * n * n * m);
+    double *x = (double *)ws.get(31, sizeof(double) * n * n * m);
+    double *M = (double *)ws.get(32, sizeof(double) * s * m);
+    const int64_t total = s * m;
+    hipLaunchKernelGGL(svec_unpack, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, a_svec,
+                       s, n, m, stack);
+    HIP_CHECK(hipGetLastError());
+    // X = Astack Rinv^T  (nm x n): X_b = A_b Rinv^T
+    Fp64Gemm g1;
+    g1.m = n * m;
+    g1.n = n;
+    g1.k = n;
+    g1.A = stack;
+    g1.lda = n * m;
+    g1.B = rinv;
+    g1.ldb = n;
+    g1.tb = true;
+    g1.C = x;
+    g1.ldc = n * m;
+    gemm_f64(g1, st);
+    // U_b = Rinv X_b (batched; written over the stack buffer)
+    Fp64Gemm g2;
+    g2.m = n;
+    g2.n = n;
+    g2.k = n;
+    g2.A = rinv;
+    g2.lda = n;
+    g2.B = x;
+    g2.ldb = n * m;
+    g2.sb = n;
+    g2.C = stack;
+    g2.ldc = n * m;
+    g2.sc = n;
+    g2.batch = m;
+    gemm_f64(g2, st);
+    hipLaunchKernelGGL(svec_pack, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, stack, s,
+                       n, m, M);
+    HIP_CHECK(hipGetLastError());
+    // S = M^T M (upper)
+    size_t wsb = sizeof(double) * (size_t)m * m * 16;
+    double *part = (double *)ws.get(33, wsb);
+    Fp64Gemm g3;
+    g3.m = m;
+    g3.n = m;
+    g3.k = s;
+    g3.A = M;
+    g3.lda = s;
+    g3.ta = true;
+    g3.B = M;
+    g3.ldb = s;
+    g3.C = S;
+    g3.ldc = lds;
+    g3.tri = 1;
+    g3.workspace = part;
+    g3.workspace_bytes = wsb;
+    gemm_f64(g3, st);
+}
+
+}  // namespace sdpx
+
+namespace sdpx {
+
+// Lower triangle <- upper triangle (the factor reads the upper triangle, as
+// the reference's dense factor takes the KKT's upper CSC).
+__global__ void mirror_upper_to_lower(double *A, int64_t n, int64_t lda) {
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n * n) return;
+    int64_t i = idx % n, j = idx / n;
+    if (i > j) A[i + j * lda] = A[j + i * lda];
+}
+
+void mirror_upper(double *A, int64_t n, int64_t lda, hipStream_t s) {
+    hipLaunchKernelGGL(mirror_upper_to_lower, dim3((unsigned)((n * n + 255) / 256)), dim3(256), 0, s,
+                       A, n, lda);
+    HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace sdpx

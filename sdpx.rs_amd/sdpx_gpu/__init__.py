@@ -67,6 +67,29 @@ def _declare(L):
     L.sdpx_gpu_set_profiling.restype = i32
     L.sdpx_gpu_last_stage_times.argtypes = [dp, u32]
     L.sdpx_gpu_last_stage_times.restype = i32
+    L.sdpx_gpu_dense_create.argtypes = [u64, ctypes.POINTER(vp)]
+    L.sdpx_gpu_dense_destroy.argtypes = [vp]
+    L.sdpx_gpu_dense_set_dynamic_pivot.argtypes = [vp, i32, ctypes.c_double, ctypes.c_double]
+    for name in ("sdpx_gpu_dense_refactor", "sdpx_gpu_dense_refactor_dev"):
+        getattr(L, name).argtypes = [vp, vp, u64, ctypes.POINTER(i32)]
+    for name in ("sdpx_gpu_dense_solve_many", "sdpx_gpu_dense_solve_many_dev"):
+        getattr(L, name).argtypes = [vp, vp, u64, u64]
+    L.sdpx_gpu_dense_factor_dev_ptr.argtypes = [vp, ctypes.POINTER(vp)]
+    L.sdpx_gpu_dense_get_factor.argtypes = [vp, vp, u64]
+    L.sdpx_gpu_dense_get_factor.restype = i32
+    for name in ("sdpx_gpu_schur_psd_f64", "sdpx_gpu_schur_psd_f64_dev"):
+        getattr(L, name).argtypes = [u64, u64, vp, vp, vp, u64]
+    for name in ("sdpx_gpu_dense_create", "sdpx_gpu_dense_destroy", "sdpx_gpu_dense_set_dynamic_pivot",
+                 "sdpx_gpu_dense_refactor", "sdpx_gpu_dense_refactor_dev", "sdpx_gpu_dense_solve_many",
+                 "sdpx_gpu_dense_solve_many_dev", "sdpx_gpu_dense_factor_dev_ptr",
+                 "sdpx_gpu_schur_psd_f64", "sdpx_gpu_schur_psd_f64_dev"):
+        getattr(L, name).restype = i32
+    L.sdpx_gpu_mp_ldl.argtypes = [u32, u64, vp, u64, vp, u64, vp, vp, vp, u32,
+                                  ctypes.POINTER(i32)]
+    L.sdpx_gpu_mp_ldl.restype = i32
+    i64p = ctypes.POINTER(ctypes.c_int64)
+    L.sdpx_gpu_mp_ldl_stats.argtypes = [i64p, i64p, i64p, dp]
+    L.sdpx_gpu_mp_ldl_stats.restype = i32
     gemm_args = [u32, u8, u8, u64, u64, u64, vp, u64, vp, u64, u32, vp, u64]
     for name in ("sdpx_gpu_rns_gemm", "sdpx_gpu_rns_gemm_dev"):
         f = getattr(L, name)
@@ -143,3 +166,110 @@ def rns_gemm_dev(ta, tb, m, n, k, a_ptr, lda, b_ptr, ldb, limbs, upper_only, out
     code = lib().sdpx_gpu_rns_gemm_dev(limbs, ord(ta), ord(tb), m, n, k, a_ptr, lda, b_ptr, ldb,
                                        1 if upper_only else 0, out_ptr, out_len)
     return check(code, allow_decline=True)
+
+
+class DenseFactor:
+    """Float64 dense factor/solve on the GPU, mirroring DirectLDLSolver's
+    refactor / solve_many (crates/solver/src/solver/kkt/direct/mod.rs:11-62)
+    for DenseBlockSolver (kkt/ldl/dense_block.rs:784-851)."""
+
+    def __init__(self, n):
+        self.n = n
+        h = ctypes.c_void_p()
+        check(lib().sdpx_gpu_dense_create(n, ctypes.byref(h)))
+        self._h = h
+        self.info = 0
+
+    def set_dynamic_pivot(self, eps, delta, enable=True):
+        check(lib().sdpx_gpu_dense_set_dynamic_pivot(self._h, 1 if enable else 0, eps, delta))
+
+    def refactor(self, a):
+        """Factor the symmetric matrix whose upper triangle ``a`` holds
+        (column-major n x n).  Returns False on a failed pivot (info > 0)."""
+        a = np.asfortranarray(a, dtype=np.float64)
+        info = ctypes.c_int32()
+        check(lib().sdpx_gpu_dense_refactor(self._h, a.ctypes.data_as(ctypes.c_void_p), self.n,
+                                            ctypes.byref(info)))
+        self.info = info.value
+        return self.info == 0
+
+    def refactor_dev(self, ptr, lda):
+        info = ctypes.c_int32()
+        check(lib().sdpx_gpu_dense_refactor_dev(self._h, ptr, lda, ctypes.byref(info)))
+        self.info = info.value
+        return self.info == 0
+
+    def solve_many(self, b):
+        """A^-1 b for an n x nrhs panel (returns a new array)."""
+        b = np.array(b, dtype=np.float64, order="F", copy=True)
+        nrhs = 1 if b.ndim == 1 else b.shape[1]
+        check(lib().sdpx_gpu_dense_solve_many(self._h, b.ctypes.data_as(ctypes.c_void_p), nrhs,
+                                              self.n))
+        return b
+
+    def solve_many_dev(self, ptr, nrhs, ldb):
+        check(lib().sdpx_gpu_dense_solve_many_dev(self._h, ptr, nrhs, ldb))
+
+    def factor(self):
+        """The lower factor L (A = L L^T) copied to the host."""
+        out = np.zeros((self.n, self.n), order="F")
+        check(lib().sdpx_gpu_dense_get_factor(self._h, out.ctypes.data_as(ctypes.c_void_p), self.n))
+        return np.tril(out)
+
+    def factor_ptr(self):
+        p = ctypes.c_void_p()
+        check(lib().sdpx_gpu_dense_factor_dev_ptr(self._h, ctypes.byref(p)))
+        return p.value
+
+    def close(self):
+        if self._h:
+            lib().sdpx_gpu_dense_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def schur_psd_f64(rinv, a_svec):
+    """Upper triangle of S_ab = svec(Rinv A_a Rinv^T) . svec(Rinv A_b Rinv^T)
+    (kkt/condensed/psd.rs:1299-1456).  ``a_svec`` is s x m."""
+    rinv = np.asfortranarray(rinv, dtype=np.float64)
+    a_svec = np.asfortranarray(a_svec, dtype=np.float64)
+    n = rinv.shape[0]
+    m = a_svec.shape[1]
+    S = np.zeros((m, m), order="F")
+    check(lib().sdpx_gpu_schur_psd_f64(n, m, rinv.ctypes.data_as(ctypes.c_void_p),
+                                       a_svec.ctypes.data_as(ctypes.c_void_p),
+                                       S.ctypes.data_as(ctypes.c_void_p), m))
+    return S
+
+
+LDL_DOTS, LDL_HOST_PRODUCTS = 1, 2
+
+
+def mp_ldl(a, n, signs=None, reg=None, flags=0):
+    """DenseLeaf::factor_leaf (arrow.rs:808-926) on the GPU path: returns
+    (ok, l, dinv) with l the unit lower factor in the lower triangle."""
+    nl = mp.limbs_of(a)
+    l = a.copy()
+    dinv = mp.zeros(n, nl)
+    sg = None if signs is None else np.ascontiguousarray(signs, np.int8)
+    eps = delta = None
+    if reg is not None:
+        eps, delta = reg
+    ok = ctypes.c_int32()
+    check(lib().sdpx_gpu_mp_ldl(nl, n, _ptr(l), n, _ptr(sg) if sg is not None else None,
+                                0 if sg is None else sg.size, _ptr(eps), _ptr(delta), _ptr(dinv),
+                                flags, ctypes.byref(ok)))
+    return bool(ok.value), l, dinv
+
+
+def mp_ldl_stats():
+    a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    d = ctypes.c_double()
+    lib().sdpx_gpu_mp_ldl_stats(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(d))
+    return {"gpu_products": a.value, "host_products": b.value, "regularized": c.value,
+            "gpu_mp_macs": d.value}

@@ -27,6 +27,13 @@ def oracle():
         L.oracle_gemm_exact.restype = i32
         L.oracle_gemm_entries.argtypes = [i32, c, c, sz, vp, sz, vp, sz, sz, vp, vp, vp, i32]
         L.oracle_gemm_entries.restype = None
+        L.oracle_ldl.argtypes = [i32, vp, sz, sz, vp, sz, vp, vp, vp, i32,
+                                 ctypes.POINTER(ctypes.c_longlong), i32]
+        L.oracle_ldl.restype = i32
+        for name in ("oracle_recip",):
+            getattr(L, name).argtypes = [i32, vp, vp]
+        for name in ("oracle_sub", "oracle_mul"):
+            getattr(L, name).argtypes = [i32, vp, vp, vp]
         _oracle = L
     return _oracle
 
@@ -81,3 +88,20 @@ def lcg_operands(m, n, k, spread, seed, nl, ta, tb):
     a = mpexact.lcg_random(rng, ar * ac, spread, nl)
     b = mpexact.lcg_random(rng, br * bc, spread, nl)
     return from_mp(a, nl), ar, from_mp(b, nl), br
+
+
+def oracle_ldl(a, n, signs=None, reg=None, recursive=True, threads=8):
+    """C oracle LDL^T of the lower triangle of a (n x n records, ld n).
+    Returns (ok, l, dinv, regularized)."""
+    nl = mp.limbs_of(a)
+    l = a.copy()
+    dinv = mp.zeros(n, nl)
+    sg = None if signs is None else np.ascontiguousarray(signs, np.int8)
+    eps = delta = None
+    if reg is not None:
+        eps, delta = reg
+    cnt = ctypes.c_longlong()
+    ok = oracle().oracle_ldl(nl, _p(l), n, n, _p(sg) if sg is not None else None,
+                             0 if sg is None else sg.size, _p(eps), _p(delta), _p(dinv),
+                             1 if recursive else 0, ctypes.byref(cnt), threads)
+    return bool(ok), l, dinv, cnt.value
