@@ -115,6 +115,42 @@ int32_t sdpx_gpu_mp_ldl(uint32_t limbs, uint64_t n, void *a, uint64_t lda, const
 int32_t sdpx_gpu_mp_ldl_stats(int64_t *gpu_products, int64_t *host_products, int64_t *regularized,
                               double *gpu_mp_macs);
 
+/* ---- Sampled Schur build + arrow factor (multiple precision) ------------
+ *
+ * One IPM iteration's Schur build and factor of a fully sampled condensed
+ * KKT (factor-authoritative dim-1 sampled PSD blocks, sdpx_sampled_block of
+ * include/sdpx.h, coupled to t border rows), bitwise as the reference:
+ *   V = Rinv Q, Gram = V^T V            (sampled/mod.rs:1836-1868, 1990-2143)
+ *   H_pq = w_p w_q 1/2 (K_pq^2 + K_pq^2) (entry, sampled/mod.rs:1973-1988)
+ *   H = L D L^T, Y = L^-1 B             (Leaf::refactor, arrow.rs:1522-1640;
+ *                                        ldl_dots 156-253, forward_suffix_rows
+ *                                        985-1019)
+ *   S -= Y^T RN(D^-1 Y) per leaf in order (contribution_product /
+ *                                        subtract_contributions, 1935-2074)
+ *   S = L_b D_b L_b^T                   (factor_border 3378-3404, ldl_rec)
+ * Leaf arrays are column-major values of 64*limbs bits.  border (t x t,
+ * ld t) holds the border block C in its upper triangle on entry and the
+ * border factor in its lower triangle on return; border_dinv receives D_b^-1.
+ * factors (optional, one per leaf) receive L (k x k), D^-1 (k), Y (k x t).
+ * Every exact product runs as an RNS product on the GPU. */
+typedef struct {
+    uint64_t side, samples;      /* h (basis rows), k (samples) */
+    const void *rinv;            /* h x h */
+    const void *basis;           /* h x k */
+    const void *weights;         /* k */
+    const void *coupling;        /* k x t */
+} sdpx_gpu_sampled_leaf;
+typedef struct {
+    void *l, *dinv, *y;          /* outputs, each optional */
+} sdpx_gpu_sampled_factor;
+int32_t sdpx_gpu_sampled_arrow(uint32_t limbs, uint64_t nleaves, const sdpx_gpu_sampled_leaf *leaves,
+                               uint64_t t, void *border, const int8_t *border_signs,
+                               uint64_t nborder_signs, void *border_dinv,
+                               sdpx_gpu_sampled_factor *factors, uint32_t flags, int32_t *ok);
+/* Phase times (s: build, leaves, border) and product counts of this
+ * thread's last sdpx_gpu_sampled_arrow (measurement only). */
+int32_t sdpx_gpu_sampled_stats(double *seconds3, int64_t *gpu_products, int64_t *host_products);
+
 /* ---- Float64 dense factor / solve -------------------------------------
  *
  * A factor handle replaces the dense Float64 backend behind

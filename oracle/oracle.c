@@ -637,6 +637,123 @@ int oracle_ldl(int nl, uint8_t *l, size_t ld, size_t n, const int8_t *signs, siz
     return ok;
 }
 
+int oracle_ldl(int nl, uint8_t *l, size_t ld, size_t n, const int8_t *signs, size_t nsigns,
+               const uint8_t *reg_eps, const uint8_t *reg_delta, uint8_t *dinv, int recursive,
+               long long *regularized, int threads);
+
+/* ------------------------------------------------------------------------
+ * Sampled Schur build + arrow factor restatement (sampled/mod.rs:1836-2143,
+ * 1973-1988; arrow.rs:985-1019, 1522-1640, 1935-2074, 3378-3404).
+ * ---------------------------------------------------------------------- */
+typedef struct {
+    uint64_t side, samples;
+    const void *rinv, *basis, *weights, *coupling;
+} ora_leaf;
+
+int oracle_sampled_arrow(int nl, size_t nleaves, const ora_leaf *leaves, size_t t, uint8_t *border,
+                         const int8_t *bsigns, size_t nbsigns, uint8_t *border_dinv, uint8_t **lout,
+                         uint8_t **dout, uint8_t **yout, int threads) {
+    ora_ctx cc;
+    ora_init(&cc, nl);
+    const ora_ctx *c = &cc;
+    size_t st = c->stride;
+    uint8_t half[8 * 32 + 8];
+    memset(half, 0, st);
+    ((uint64_t *)half)[nl - 1] = 1ull << 63;
+    *(int32_t *)(half + 8 * nl) = KIND_REG;
+    *(int32_t *)(half + 8 * nl + 4) = 0;
+    for (size_t b = 0; b < nleaves; b++) {
+        const ora_leaf *lf = &leaves[b];
+        size_t h = lf->side, K = lf->samples;
+        uint8_t *V = malloc(st * h * K), *G = malloc(st * (K * (K + 1) / 2));
+        oracle_gemm_exact(nl, 'N', 'N', h, K, h, lf->rinv, h, lf->basis, h, 0, 0, V, threads);
+        oracle_gemm_exact(nl, 'T', 'N', K, K, h, V, h, V, h, 1, 1, G, threads);
+        uint8_t *H = calloc(K * K, st);
+        for (size_t e = 0; e < K * K; e++) write_zero(H, st, e, nl);
+        for (size_t q = 0; q < K; q++)
+            for (size_t p = q; p < K; p++) {
+                uint8_t t1[8 * 32 + 8], t2[8 * 32 + 8], u1[8 * 32 + 8], u[8 * 32 + 8];
+                const uint8_t *kpq = AT(G, p * (p + 1) / 2 + q);
+                ora_mul(c, AT((const uint8_t *)lf->weights, p), AT((const uint8_t *)lf->weights, q), t1);
+                ora_mul(c, t1, half, t2);
+                ora_mul(c, kpq, kpq, u1);
+                {
+                    const uint8_t *xs[2] = {u1, u1}, *ys[2] = {c->one, c->one};
+                    ora_dot_ptrs(c, 2, xs, ys, u);
+                }
+                ora_mul(c, t2, u, AT(H, p + q * K));
+            }
+        free(V);
+        free(G);
+        uint8_t *dinv = malloc(st * K);
+        long long reg = 0;
+        int8_t plus = 1;
+        if (!oracle_ldl(nl, H, K, K, &plus, 1, NULL, NULL, dinv, 0, &reg, threads)) {
+            free(H);
+            free(dinv);
+            return 0;
+        }
+        /* envelope of the factor's lower triangle (as the reference keeps it) */
+        size_t *first = malloc(sizeof(size_t) * K);
+        for (size_t i = 0; i < K; i++) {
+            size_t k = 0;
+            while (k < i && ora_is_zero(c, AT(H, i + k * K))) k++;
+            first[i] = k;
+        }
+        uint8_t *Y = malloc(st * K * t);
+        memcpy(Y, lf->coupling, st * K * t);
+#pragma omp parallel for schedule(dynamic, 8) num_threads(threads > 0 ? threads : 1)
+        for (long long ci = 0; ci < (long long)t; ci++) {
+            uint8_t *col = Y + (size_t)ci * K * st;
+            const uint8_t **xs = malloc(sizeof(uint8_t *) * (K + 1)), **ys = malloc(sizeof(uint8_t *) * (K + 1));
+            uint8_t *negs = malloc(st * (K + 1));
+            for (size_t i = 0; i < K; i++) {
+                size_t lo = first[i], m = 0;
+                for (size_t k = lo; k < i; k++, m++) {
+                    memcpy(negs + m * st, AT(H, i + k * K), st);
+                    ora_neg(c, negs + m * st);
+                    xs[m] = negs + m * st;
+                    ys[m] = col + k * st;
+                }
+                xs[m] = c->one;
+                ys[m] = col + i * st;
+                m++;
+                uint8_t r[8 * 32 + 8];
+                ora_dot_ptrs(c, (int)m, xs, ys, r);
+                memcpy(col + i * st, r, st);
+            }
+            free(xs);
+            free(ys);
+            free(negs);
+        }
+        free(first);
+        uint8_t *Z = malloc(st * K * t), *P = malloc(st * (t * (t + 1) / 2));
+        for (size_t e = 0; e < K * t; e++) ora_mul(c, AT(Y, e), AT(dinv, e % K), AT(Z, e));
+        oracle_gemm_exact(nl, 'T', 'N', t, t, K, Y, K, Z, K, 1, 1, P, threads);
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads > 0 ? threads : 1)
+        for (long long jj = 0; jj < (long long)t; jj++) {
+            size_t j = (size_t)jj;
+            for (size_t i = 0; i <= j; i++) {
+                uint8_t r[8 * 32 + 8];
+                ora_sub(c, AT(border, i + j * t), AT(P, j * (j + 1) / 2 + i), r);
+                memcpy(AT(border, i + j * t), r, st);
+            }
+        }
+        free(Z);
+        free(P);
+        if (lout && lout[b]) memcpy(lout[b], H, st * K * K);
+        if (dout && dout[b]) memcpy(dout[b], dinv, st * K);
+        if (yout && yout[b]) memcpy(yout[b], Y, st * K * t);
+        free(H);
+        free(dinv);
+        free(Y);
+    }
+    for (size_t j = 0; j < t; j++)
+        for (size_t i = j + 1; i < t; i++) memcpy(AT(border, i + j * t), AT(border, j + i * t), st);
+    long long reg = 0;
+    return oracle_ldl(nl, border, t, t, bsigns, nbsigns, NULL, NULL, border_dinv, t >= 256, &reg, threads);
+}
+
 /* Scalar checks for the tests. */
 void oracle_recip(int nl, const uint8_t *d, uint8_t *out) {
     ora_ctx c;

@@ -24,6 +24,7 @@
 #include "ldl.h"
 #include "mpx.h"
 #include "rns_gemm.h"
+#include "sampled.h"
 
 namespace sdpx {
 namespace {
@@ -325,6 +326,13 @@ bool ldl_rec(const Ctx &cx, uint8_t *l, size_t ld, size_t n, const size_t *first
 }
 
 }  // namespace
+
+void exact_product_public(int limbs, const LdlOptions &opt, char ta, char tb, size_t m, size_t n,
+                          size_t k, const uint8_t *a, size_t lda, const uint8_t *b, size_t ldb,
+                          uint8_t *c, bool packed, LdlStats &stats) {
+    Ctx cx{limbs, value_bytes(limbs), &opt, &stats};
+    exact_product(cx, ta, tb, m, n, k, a, lda, b, ldb, c, packed);
+}
 
 // Row envelope: first nonzero column of each row of the lower triangle.
 std::vector<size_t> envelope(int limbs, const uint8_t *l, size_t ld, size_t n) {

@@ -90,6 +90,12 @@ def _declare(L):
     i64p = ctypes.POINTER(ctypes.c_int64)
     L.sdpx_gpu_mp_ldl_stats.argtypes = [i64p, i64p, i64p, dp]
     L.sdpx_gpu_mp_ldl_stats.restype = i32
+    L.sdpx_gpu_sampled_arrow.argtypes = [u32, u64, vp, u64, vp, vp, u64, vp, vp, u32,
+                                         ctypes.POINTER(i32)]
+    L.sdpx_gpu_sampled_arrow.restype = i32
+    L.sdpx_gpu_sampled_stats.argtypes = [dp, ctypes.POINTER(ctypes.c_int64),
+                                         ctypes.POINTER(ctypes.c_int64)]
+    L.sdpx_gpu_sampled_stats.restype = i32
     gemm_args = [u32, u8, u8, u64, u64, u64, vp, u64, vp, u64, u32, vp, u64]
     for name in ("sdpx_gpu_rns_gemm", "sdpx_gpu_rns_gemm_dev"):
         f = getattr(L, name)
@@ -273,3 +279,47 @@ def mp_ldl_stats():
     lib().sdpx_gpu_mp_ldl_stats(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(d))
     return {"gpu_products": a.value, "host_products": b.value, "regularized": c.value,
             "gpu_mp_macs": d.value}
+
+
+class SampledLeaf(ctypes.Structure):
+    _fields_ = [("side", ctypes.c_uint64), ("samples", ctypes.c_uint64),
+                ("rinv", ctypes.c_void_p), ("basis", ctypes.c_void_p),
+                ("weights", ctypes.c_void_p), ("coupling", ctypes.c_void_p)]
+
+
+class SampledFactor(ctypes.Structure):
+    _fields_ = [("l", ctypes.c_void_p), ("dinv", ctypes.c_void_p), ("y", ctypes.c_void_p)]
+
+
+def sampled_arrow(problem, nl, border_signs=(-1,), flags=0, want_factors=False):
+    """One Schur build + arrow factor of a sampled problem
+    (workloads.sampled_problem).  Returns (ok, border, border_dinv, factors)."""
+    blocks = problem["blocks"]
+    t = problem["t"]
+    leaves = (SampledLeaf * len(blocks))()
+    for i, b in enumerate(blocks):
+        leaves[i] = SampledLeaf(b["h"], b["k"], _ptr(b["rinv"]), _ptr(b["q"]), _ptr(b["w"]),
+                                _ptr(b["b"]))
+    border = problem["border"].copy()
+    bdinv = mp.zeros(t, nl)
+    signs = np.ascontiguousarray(border_signs, np.int8)
+    facs = None
+    outs = []
+    if want_factors:
+        facs = (SampledFactor * len(blocks))()
+        for i, b in enumerate(blocks):
+            l, d, y = mp.zeros(b["k"] ** 2, nl), mp.zeros(b["k"], nl), mp.zeros(b["k"] * t, nl)
+            outs.append((l, d, y))
+            facs[i] = SampledFactor(_ptr(l), _ptr(d), _ptr(y))
+    ok = ctypes.c_int32()
+    check(lib().sdpx_gpu_sampled_arrow(nl, len(blocks), leaves, t, _ptr(border), _ptr(signs),
+                                       signs.size, _ptr(bdinv), facs, flags, ctypes.byref(ok)))
+    return bool(ok.value), border, bdinv, outs
+
+
+def sampled_stats():
+    secs = (ctypes.c_double * 3)()
+    g, h = ctypes.c_int64(), ctypes.c_int64()
+    lib().sdpx_gpu_sampled_stats(secs, ctypes.byref(g), ctypes.byref(h))
+    return {"build_s": secs[0], "leaves_s": secs[1], "border_s": secs[2],
+            "gpu_products": g.value, "host_products": h.value}

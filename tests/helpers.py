@@ -105,3 +105,38 @@ def oracle_ldl(a, n, signs=None, reg=None, recursive=True, threads=8):
                              0 if sg is None else sg.size, _p(eps), _p(delta), _p(dinv),
                              1 if recursive else 0, ctypes.byref(cnt), threads)
     return bool(ok), l, dinv, cnt.value
+
+
+class OraLeaf(ctypes.Structure):
+    _fields_ = [("side", ctypes.c_uint64), ("samples", ctypes.c_uint64),
+                ("rinv", ctypes.c_void_p), ("basis", ctypes.c_void_p),
+                ("weights", ctypes.c_void_p), ("coupling", ctypes.c_void_p)]
+
+
+def oracle_sampled_arrow(problem, nl, border_signs=(-1,), threads=8):
+    """C oracle of the sampled Schur build + arrow factor.  Returns
+    (ok, border, border_dinv, [(l, dinv, y) per leaf])."""
+    blocks = problem["blocks"]
+    t = problem["t"]
+    L = oracle()
+    L.oracle_sampled_arrow.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_void_p,
+                                       ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+    L.oracle_sampled_arrow.restype = ctypes.c_int
+    leaves = (OraLeaf * len(blocks))()
+    outs = []
+    lp = (ctypes.c_void_p * len(blocks))()
+    dp = (ctypes.c_void_p * len(blocks))()
+    yp = (ctypes.c_void_p * len(blocks))()
+    for i, b in enumerate(blocks):
+        leaves[i] = OraLeaf(b["h"], b["k"], _p(b["rinv"]), _p(b["q"]), _p(b["w"]), _p(b["b"]))
+        l, d, y = mp.zeros(b["k"] ** 2, nl), mp.zeros(b["k"], nl), mp.zeros(b["k"] * t, nl)
+        outs.append((l, d, y))
+        lp[i], dp[i], yp[i] = _p(l), _p(d), _p(y)
+    border = problem["border"].copy()
+    bdinv = mp.zeros(t, nl)
+    signs = np.ascontiguousarray(border_signs, np.int8)
+    ok = L.oracle_sampled_arrow(nl, len(blocks), leaves, t, _p(border), _p(signs), signs.size,
+                                _p(bdinv), lp, dp, yp, threads)
+    return bool(ok), border, bdinv, outs
