@@ -115,15 +115,97 @@ def cpu_baseline_gemm(a, b, m, n, k, nl):
                       f"{dt:.2f} s, scaled by {m * n}/{count}"}
 
 
-WORKLOADS = {"rns_gemm": rns_gemm_workload}
+def sampled_workload(args):
+    """One IPM iteration's Schur build + arrow factor of the sampled
+    bootstrap-style SDP at 256 bits (40 PSD blocks of side U{10..60},
+    K = 2h - 1 Chebyshev samples, log-uniform weights in [1e-6, 1], 2000
+    border rows), with every exact product as INT8-MFMA RNS products; the
+    same problem in float64 on the same GPU beside it."""
+    import torch
+
+    import sdpx_gpu
+    from sdpx_gpu import workloads
+
+    nl = 4
+    sdpx_gpu.check(sdpx_gpu.lib().sdpx_gpu_init(0))
+    prob = workloads.sampled_problem(nl, seed=7)
+    t = prob["t"]
+
+    def step():
+        ok, border, bdinv, _ = sdpx_gpu.sampled_arrow(prob, nl)
+        assert ok
+        return border, bdinv
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    dt = (time.perf_counter() - t0) / args.steps
+    phases = sdpx_gpu.sampled_stats()
+    # one profiled pass for the kernel roofline (HIP events on the library stream)
+    sdpx_gpu.set_profiling(True)
+    sdpx_gpu.rns_totals(reset=True)
+    step()
+    tot = sdpx_gpu.rns_totals(reset=True)
+    sdpx_gpu.set_profiling(False)
+    # float64 on the same GPU, same problem (dynamic pivot rule: the 256-bit
+    # leaves are singular in binary64)
+    dev = sdpx_gpu.SampledF64(prob)
+    for _ in range(max(1, args.warmup)):
+        dev.run(dynamic=(1e-13, 2e-7))
+    torch.cuda.synchronize()
+    f0 = time.perf_counter()
+    fsteps = max(5, args.steps)
+    for _ in range(fsteps):
+        info, ftimes = dev.run(dynamic=(1e-13, 2e-7))
+    torch.cuda.synchronize()
+    fdt = (time.perf_counter() - f0) / fsteps
+    product_s = tot["stage_ms"][2] / 1e3
+    achieved = tot["int8_ops"] / product_s if product_s > 0 else 0.0
+    result = {
+        "value": dt, "unit": "s/IPM-iter", "ms_per_step": dt * 1e3,
+        "dtype": "mp256 (int8 MFMA residues, int32 accumulation)",
+        "config": {"workload": "sampled bootstrap SDP, 40 PSD blocks h~U{10..60}, K=2h-1, "
+                               "2000 border rows, MPFR-256: Schur build + arrow LDL^T",
+                   "precision_bits": 256, "blocks": len(prob["blocks"]), "border": t,
+                   "samples": int(sum(b["k"] for b in prob["blocks"])),
+                   "phase_s": {k: phases[k] for k in ("build_s", "leaves_s", "border_s")},
+                   "gpu_products": phases["gpu_products"], "host_products": phases["host_products"],
+                   "rns_stage_ms": dict(zip(("scan", "encode", "product", "crt"), tot["stage_ms"])),
+                   "rns_calls": tot["calls"], "mp_macs": tot["mp_macs"],
+                   "fp64_gpu_s_per_iter": fdt, "fp64_phase_ms": ftimes,
+                   "mp256_over_fp64": dt / fdt,
+                   "host_staged": True},
+        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": INT8_PEAK_OPS / 1e12,
+                     "unit": "TOP/s (INT8)", "frac": achieved / INT8_PEAK_OPS, "traffic": None,
+                     "kernel": "product_kernel (INT8 MFMA residue products)",
+                     "int8_ops_per_step": tot["int8_ops"]},
+    }
+    if args.cpu_baseline:
+        sys.path[:0] = [os.path.join(ROOT, "tests"), os.path.join(ROOT, "oracle")]
+        from helpers import oracle_sampled_arrow
+
+        cores = os.cpu_count() or 1
+        c0 = time.perf_counter()
+        ok, *_ = oracle_sampled_arrow(prob, nl, threads=cores)
+        cdt = time.perf_counter() - c0
+        result["cpu_baseline"] = {"value": cdt, "unit": "s/IPM-iter", "cores": cores, "kind": "port",
+                                  "sample": "the whole workload (one iteration) through the C "
+                                            "restatement with exact dots, OpenMP on all cores"}
+    return result
+
+
+WORKLOADS = {"rns_gemm": rns_gemm_workload, "sampled": sampled_workload}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="rns_gemm", choices=sorted(WORKLOADS))
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="sampled", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
     args = ap.parse_args()
     if args.gpus != 1:

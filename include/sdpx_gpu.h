@@ -65,6 +65,12 @@ int32_t sdpx_gpu_last_plan(uint32_t *moduli, uint32_t *planes, double *bits);
  * product (all INT8 MFMA launches), crt. */
 int32_t sdpx_gpu_set_profiling(int32_t on);
 int32_t sdpx_gpu_last_stage_times(double *ms, uint32_t count);
+/* Totals over this thread's profiled exact products since the last reset:
+ * stage ms (4, as above), algorithmic INT8 operations (2 x outputs x k x
+ * INT8 GEMMs per exact product), multiple-precision MACs, calls; reset != 0
+ * clears them after reading. */
+int32_t sdpx_gpu_rns_totals(int32_t reset, double *stage_ms4, double *int8_ops, double *mp_macs,
+                            int64_t *calls);
 
 /* ---- Exact residue (RNS) products on INT8 matrix cores ----------------
  *
@@ -151,6 +157,23 @@ int32_t sdpx_gpu_sampled_arrow(uint32_t limbs, uint64_t nleaves, const sdpx_gpu_
  * thread's last sdpx_gpu_sampled_arrow (measurement only). */
 int32_t sdpx_gpu_sampled_stats(double *seconds3, int64_t *gpu_products, int64_t *host_products);
 
+/* Float64 form of the same Schur build + arrow factor on device pointers
+ * (the binary64 dense elimination H = L L^T, Y = L^-1 B, S = C + Y^T Y of
+ * kkt/ldl/dense_block.rs:95-229, with the leaves H_pq = w_p w_q K_pq^2):
+ * s (t x t, upper) = c - sum Y^T Y, border_factor (t x t) = chol(-s) lower.
+ * dyn_delta > 0 applies the dynamic pivot rule (pivots <= dyn_eps become
+ * dyn_delta, as the reference's dynamic regularization) to every factor.
+ * info: 0, or the failing pivot column + 1 of the last failing factor.
+ * times_ms (optional, 3): build, leaves, border. */
+typedef struct {
+    uint64_t side, samples;
+    const double *rinv, *basis, *weights, *coupling;  /* device */
+} sdpx_gpu_sampled_leaf_f64;
+int32_t sdpx_gpu_sampled_arrow_f64_dev(uint64_t nleaves, const sdpx_gpu_sampled_leaf_f64 *leaves,
+                                       uint64_t t, const double *c, double *s, double *border_factor,
+                                       double dyn_eps, double dyn_delta, int32_t *info,
+                                       double *times_ms);
+
 /* ---- Float64 dense factor / solve -------------------------------------
  *
  * A factor handle replaces the dense Float64 backend behind
@@ -194,6 +217,25 @@ int32_t sdpx_gpu_schur_psd_f64(uint64_t n, uint64_t m, const double *rinv, const
                                double *s, uint64_t lds);
 int32_t sdpx_gpu_schur_psd_f64_dev(uint64_t n, uint64_t m, const double *rinv,
                                    const double *a_svec, double *s, uint64_t lds);
+
+/* ---- Adjoint (differentiation) ---------------------------------------
+ *
+ * No counterpart in the reference (it has no differentiation); built on its
+ * KKT solve_many (kkt/mod.rs:59, direct/mod.rs:29-41) and the symmetric
+ * quasi-definite KKT (paper/sdpx.tex:389-396), whose transpose is itself.
+ * At an IPM solution of the PSD-block problem with the Schur factor in h
+ * (M_ij = <A_i, W A_j W>, W = Rinv^T Rinv), for nrhs cotangents ybar of the
+ * free variables y* (m x nrhs, column-major):
+ *   bgrad = M^-1 ybar                      (m x nrhs) = d(ybar . y*)/db
+ *   cgrad = svec(W smat(A bgrad) W)        (s x nrhs) = d(ybar . y*)/d svec(C)
+ * with the same GPU factor (one transposed-KKT solve per cotangent).
+ * rinv n x n, a_svec s x m (s = n(n+1)/2) as sdpx_gpu_schur_psd_f64. */
+int32_t sdpx_gpu_adjoint_psd_f64(sdpx_gpu_dense *h, uint64_t n, const double *rinv,
+                                 const double *a_svec, uint64_t nrhs, const double *ybar,
+                                 double *bgrad, double *cgrad);
+int32_t sdpx_gpu_adjoint_psd_f64_dev(sdpx_gpu_dense *h, uint64_t n, const double *rinv,
+                                     const double *a_svec, uint64_t nrhs, const double *ybar,
+                                     double *bgrad, double *cgrad);
 
 #ifdef __cplusplus
 }

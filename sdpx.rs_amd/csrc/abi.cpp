@@ -116,6 +116,18 @@ int32_t sdpx_gpu_last_stage_times(double *ms, uint32_t count) {
     return SDPX_GPU_OK;
 }
 
+int32_t sdpx_gpu_rns_totals(int32_t reset, double *stage_ms4, double *int8_ops, double *mp_macs,
+                            int64_t *calls) {
+    sdpx::RnsTotals &t = sdpx::rns_totals();
+    if (stage_ms4)
+        for (int i = 0; i < 4; i++) stage_ms4[i] = t.stage_ms[i];
+    if (int8_ops) *int8_ops = t.int8_ops;
+    if (mp_macs) *mp_macs = t.mp_macs;
+    if (calls) *calls = t.calls;
+    if (reset) t = sdpx::RnsTotals{};
+    return SDPX_GPU_OK;
+}
+
 int32_t sdpx_gpu_rns_gemm_dev(uint32_t limbs, uint8_t ta, uint8_t tb, uint64_t m, uint64_t n,
                               uint64_t k, const void *a, uint64_t lda, const void *b, uint64_t ldb,
                               uint32_t upper, void *out, uint64_t out_len) {

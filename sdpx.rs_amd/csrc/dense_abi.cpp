@@ -208,3 +208,93 @@ int32_t sdpx_gpu_schur_psd_f64_dev(uint64_t n, uint64_t m, const double *rinv,
 }
 
 }  // extern "C"
+
+#include "fp64_sampled.h"
+
+extern "C" {
+
+int32_t sdpx_gpu_sampled_arrow_f64_dev(uint64_t nleaves, const sdpx_gpu_sampled_leaf_f64 *leaves,
+                                       uint64_t t, const double *c, double *s, double *border_factor,
+                                       double dyn_eps, double dyn_delta, int32_t *info,
+                                       double *times_ms) {
+    return abi_boundary([&]() -> int32_t {
+        if (nleaves && !leaves) return abi_invalid("null leaves");
+        if (t && (!c || !s || !border_factor)) return abi_invalid("null border");
+        sdpx::Device &d = sdpx::device();
+        std::lock_guard<std::recursive_mutex> g(d.mu);
+        std::vector<sdpx::Fp64Leaf> in(nleaves);
+        size_t hb = 0;
+        for (uint64_t i = 0; i < nleaves; i++) hb += leaves[i].samples * leaves[i].samples;
+        double *hbuf = (double *)d.ws.get(54, 8 * std::max<size_t>(hb, 1));
+        size_t off = 0;
+        for (uint64_t i = 0; i < nleaves; i++) {
+            const auto &l = leaves[i];
+            in[i] = sdpx::Fp64Leaf{(int64_t)l.side, (int64_t)l.samples, l.rinv, l.basis,
+                                   l.weights, l.coupling, hbuf + off};
+            off += l.samples * l.samples;
+        }
+        int *dinfo = (int *)d.ws.get(55, sizeof(int));
+        HIP_CHECK(hipMemsetAsync(dinfo, 0, sizeof(int), d.stream));
+        sdpx::Fp64SampledTimes tm;
+        sdpx::PivotRule rule{dyn_eps, dyn_delta, nullptr};
+        sdpx::sampled_arrow_f64(in, (int64_t)t, c, s, border_factor, dinfo, &tm, d.stream,
+                                dyn_delta > 0 ? &rule : nullptr);
+        int hinfo = 0;
+        HIP_CHECK(hipMemcpyAsync(&hinfo, dinfo, sizeof(int), hipMemcpyDeviceToHost, d.stream));
+        HIP_CHECK(hipStreamSynchronize(d.stream));
+        if (info) *info = hinfo;
+        if (times_ms) {
+            times_ms[0] = tm.build_ms;
+            times_ms[1] = tm.leaves_ms;
+            times_ms[2] = tm.border_ms;
+        }
+        return SDPX_GPU_OK;
+    });
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int32_t sdpx_gpu_adjoint_psd_f64(sdpx_gpu_dense *h, uint64_t n, const double *rinv,
+                                 const double *a_svec, uint64_t nrhs, const double *ybar,
+                                 double *bgrad, double *cgrad) {
+    return abi_boundary([&]() -> int32_t {
+        if (!h || !h->factored) return abi_invalid("no successful factorization");
+        const uint64_t m = (uint64_t)h->n, s = n * (n + 1) / 2;
+        if (m == 0 || nrhs == 0) return SDPX_GPU_OK;
+        sdpx::Device &d = sdpx::device();
+        std::lock_guard<std::recursive_mutex> g(d.mu);
+        hipStream_t st = d.stream;
+        double *dr = (double *)d.ws.get(63, 8 * n * n);
+        double *da = (double *)d.ws.get(64, 8 * s * m);
+        double *dy = (double *)d.ws.get(65, 8 * m * nrhs);
+        double *db = (double *)d.ws.get(66, 8 * m * nrhs);
+        double *dc = (double *)d.ws.get(67, 8 * s * nrhs);
+        HIP_CHECK(hipMemcpyAsync(dr, rinv, 8 * n * n, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(da, a_svec, 8 * s * m, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(dy, ybar, 8 * m * nrhs, hipMemcpyHostToDevice, st));
+        sdpx::adjoint_psd_f64(h->L, (int64_t)m, (int64_t)n, dr, da, (int64_t)nrhs, dy, db, dc, st);
+        HIP_CHECK(hipMemcpyAsync(bgrad, db, 8 * m * nrhs, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(cgrad, dc, 8 * s * nrhs, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        return SDPX_GPU_OK;
+    });
+}
+
+int32_t sdpx_gpu_adjoint_psd_f64_dev(sdpx_gpu_dense *h, uint64_t n, const double *rinv,
+                                     const double *a_svec, uint64_t nrhs, const double *ybar,
+                                     double *bgrad, double *cgrad) {
+    return abi_boundary([&]() -> int32_t {
+        if (!h || !h->factored) return abi_invalid("no successful factorization");
+        if (h->n == 0 || nrhs == 0) return SDPX_GPU_OK;
+        sdpx::Device &d = sdpx::device();
+        std::lock_guard<std::recursive_mutex> g(d.mu);
+        sdpx::adjoint_psd_f64(h->L, h->n, (int64_t)n, rinv, a_svec, (int64_t)nrhs, ybar, bgrad,
+                              cgrad, d.stream);
+        HIP_CHECK(hipStreamSynchronize(d.stream));
+        return SDPX_GPU_OK;
+    });
+}
+
+}  // extern "C"

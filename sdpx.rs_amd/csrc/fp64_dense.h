@@ -29,6 +29,11 @@ struct PivotRule {
 // In-place lower Cholesky; *d_info (device) = 0 or the failing column + 1.
 void potrf_f64(double *A, int64_t n, int64_t lda, int *d_info, double *ws, size_t ws_bytes,
                hipStream_t s, const PivotRule *rule = nullptr);
+// B <- L^-1 B and B <- L^-T B with the lower factor L.
+void trsm_lower_f64(const double *L, int64_t n, int64_t ldl, double *B, int64_t nrhs, int64_t ldb,
+                    hipStream_t s);
+void trsm_lower_t_f64(const double *L, int64_t n, int64_t ldl, double *B, int64_t nrhs,
+                      int64_t ldb, hipStream_t s);
 // B <- (L L^T)^-1 B with the lower factor L.
 void potrs_f64(const double *L, int64_t n, int64_t ldl, double *B, int64_t nrhs, int64_t ldb,
                double *ws, size_t ws_bytes, hipStream_t s);
@@ -39,6 +44,12 @@ void schur_psd_f64(int64_t n, int64_t m, const double *rinv, const double *a_sve
 
 // Copy the upper triangle onto the lower one.
 void mirror_upper(double *A, int64_t n, int64_t lda, hipStream_t s);
+
+// Adjoint of the PSD Schur solve (device pointers): bgrad = M^-1 ybar,
+// cgrad = svec(W smat(A_svec bgrad) W), W = Rinv^T Rinv; Lm the lower factor of M.
+void adjoint_psd_f64(const double *Lm, int64_t m, int64_t n, const double *rinv,
+                     const double *a_svec, int64_t nrhs, const double *ybar, double *bgrad,
+                     double *cgrad, hipStream_t st);
 
 __global__ void svec_unpack(const double *svec, int64_t s, int64_t n, int64_t m, double *out);
 __global__ void svec_pack(const double *U, int64_t s, int64_t n, int64_t m, double *M);

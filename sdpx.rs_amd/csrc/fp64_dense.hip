@@ -207,7 +207,7 @@ __global__ __launch_bounds__(256) void potrf_diag(double *A, int64_t lda, int64_
             if (!std::isfinite(d)) {
                 bad = (int)(j0 + j + 1);
             } else if (rule && d <= eps) {
-                if (d < -64.0 * 2.220446049250313e-16 * (double)nb * fabs(diag[j0 + j]))
+                if (diag && d < -64.0 * 2.220446049250313e-16 * (double)nb * fabs(diag[j0 + j]))
                     bad = (int)(j0 + j + 1);
                 else
                     s[j][j] = sqrt(delta);
@@ -351,10 +351,9 @@ __global__ __launch_bounds__(256) void trsv_block(const double *L, int64_t ldl, 
         if (i < nb) B[(j0 + i) + c * ldb] = x[i];
 }
 
-void potrs_f64(const double *L, int64_t n, int64_t ldl, double *B, int64_t nrhs, int64_t ldb,
-               double *ws, size_t ws_bytes, hipStream_t s) {
+void trsm_lower_f64(const double *L, int64_t n, int64_t ldl, double *B, int64_t nrhs, int64_t ldb,
+                    hipStream_t s) {
     const unsigned cgrid = (unsigned)((nrhs + 255) / 256);
-    // forward: L Y = B
     for (int64_t j0 = 0; j0 < n; j0 += NB) {
         const int nb = (int)std::min<int64_t>(NB, n - j0);
         hipLaunchKernelGGL(trsv_block, dim3(cgrid), dim3(256), 0, s, L, ldl, B, ldb, j0, nb, nrhs, 0);
@@ -375,7 +374,11 @@ void potrs_f64(const double *L, int64_t n, int64_t ldl, double *B, int64_t nrhs,
             gemm_f64(g, s);
         }
     }
-    // backward: L^T X = Y
+}
+
+void trsm_lower_t_f64(const double *L, int64_t n, int64_t ldl, double *B, int64_t nrhs,
+                      int64_t ldb, hipStream_t s) {
+    const unsigned cgrid = (unsigned)((nrhs + 255) / 256);
     int64_t last = ((n - 1) / NB) * NB;
     for (int64_t j0 = last; j0 >= 0; j0 -= NB) {
         const int nb = (int)std::min<int64_t>(NB, n - j0);
@@ -398,6 +401,15 @@ void potrs_f64(const double *L, int64_t n, int64_t ldl, double *B, int64_t nrhs,
             gemm_f64(g, s);
         }
     }
+}
+
+void potrs_f64(const double *L, int64_t n, int64_t ldl, double *B, int64_t nrhs, int64_t ldb,
+               double *ws, size_t ws_bytes, hipStream_t s) {
+    (void)ws;
+    (void)ws_bytes;
+    if (n == 0 || nrhs == 0) return;
+    trsm_lower_f64(L, n, ldl, B, nrhs, ldb, s);
+    trsm_lower_t_f64(L, n, ldl, B, nrhs, ldb, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -520,6 +532,55 @@ __global__ void mirror_upper_to_lower(double *A, int64_t n, int64_t lda) {
 void mirror_upper(double *A, int64_t n, int64_t lda, hipStream_t s) {
     hipLaunchKernelGGL(mirror_upper_to_lower, dim3((unsigned)((n * n + 255) / 256)), dim3(256), 0, s,
                        A, n, lda);
+    HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace sdpx
+
+namespace sdpx {
+
+// Adjoint of the Schur solve at a PSD block (no reference counterpart; the
+// reference has no differentiation).  At the IPM solution the condensed
+// Newton system is M dy = db + A(W dC W) with M_ij = <A_i, W A_j W>,
+// W = Rinv^T Rinv.  For cotangents ybar of y*: v = M^-T ybar = M^-1 ybar
+// (the same factor: M is symmetric), bbar = v, Cbar = W A*(v) W.
+// ybar/bgrad: m x nrhs; cgrad: s x nrhs svec columns (device pointers).
+void adjoint_psd_f64(const double *Lm, int64_t m, int64_t n, const double *rinv,
+                     const double *a_svec, int64_t nrhs, const double *ybar, double *bgrad,
+                     double *cgrad, hipStream_t st) {
+    Workspace &ws = device().ws;
+    const int64_t s = n * (n + 1) / 2;
+    // v = M^-1 ybar (into bgrad)
+    HIP_CHECK(hipMemcpyAsync(bgrad, ybar, 8 * (size_t)m * nrhs, hipMemcpyDeviceToDevice, st));
+    potrs_f64(Lm, m, m, bgrad, nrhs, m, nullptr, 0, st);
+    // Z = A_svec v  (s x nrhs)
+    double *z = (double *)ws.get(60, 8 * (size_t)s * nrhs);
+    Fp64Gemm g0;
+    g0.m = s; g0.n = nrhs; g0.k = m;
+    g0.A = a_svec; g0.lda = s; g0.B = bgrad; g0.ldb = m; g0.C = z; g0.ldc = s;
+    gemm_f64(g0, st);
+    double *stack = (double *)ws.get(61, 8 * (size_t)n * n * nrhs);
+    double *x = (double *)ws.get(62, 8 * (size_t)n * n * nrhs);
+    const int64_t total = s * nrhs;
+    hipLaunchKernelGGL(svec_unpack, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, z, s, n,
+                       nrhs, stack);
+    HIP_CHECK(hipGetLastError());
+    // W Z W = Rinv^T (Rinv Z Rinv^T) Rinv, two congruences on the stack
+    for (int pass = 0; pass < 2; pass++) {
+        const bool t = pass == 1;  // second pass uses Rinv^T
+        Fp64Gemm g1;  // X_k = Z_k F^T  (F = Rinv, then Rinv^T)
+        g1.m = n * nrhs; g1.n = n; g1.k = n;
+        g1.A = stack; g1.lda = n * nrhs; g1.B = rinv; g1.ldb = n; g1.tb = !t;
+        g1.C = x; g1.ldc = n * nrhs;
+        gemm_f64(g1, st);
+        Fp64Gemm g2;  // Z_k = F X_k
+        g2.m = n; g2.n = n; g2.k = n;
+        g2.A = rinv; g2.lda = n; g2.ta = t; g2.B = x; g2.ldb = n * nrhs; g2.sb = n;
+        g2.C = stack; g2.ldc = n * nrhs; g2.sc = n; g2.batch = nrhs;
+        gemm_f64(g2, st);
+    }
+    hipLaunchKernelGGL(svec_pack, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, stack, s, n,
+                       nrhs, cgrad);
     HIP_CHECK(hipGetLastError());
 }
 

@@ -29,6 +29,15 @@ struct RnsStats {
 // Record HIP events around the pipeline stages of later calls.
 void rns_set_profiling(bool on);
 
+// Totals over this thread's profiled calls since the last reset: stage ms
+// (scan, encode, product, crt), algorithmic INT8 ops of the products, calls.
+struct RnsTotals {
+    double stage_ms[4] = {0, 0, 0, 0};
+    double int8_ops = 0, mp_macs = 0;
+    int64_t calls = 0;
+};
+RnsTotals &rns_totals();
+
 // Runs on the library stream and returns after the results are complete.
 // Returns an SDPX_GPU_* code; throws sdpx::Error on device failures.
 int rns_gemm_device(const RnsGemmArgs &args);

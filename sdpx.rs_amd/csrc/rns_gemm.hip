@@ -646,6 +646,8 @@ unsigned padded_grid(int64_t total) { return (unsigned)((total + 7) / 8 * 8); }
 }  // namespace
 
 const RnsStats &rns_last_stats() { return t_stats; }
+thread_local RnsTotals t_totals;
+RnsTotals &rns_totals() { return t_totals; }
 void rns_set_profiling(bool on) { g_profile = on; }
 
 int rns_gemm_device(const RnsGemmArgs &g) {
@@ -794,6 +796,16 @@ int rns_gemm_device(const RnsGemmArgs &g) {
     sev.mark(4, s);
     HIP_CHECK(hipStreamSynchronize(s));
     sev.collect(t_stats);
+    if (g_profile) {
+        // algorithmic work: one exact dot of k terms per output; one INT8
+        // GEMM per single-digit modulus and four per two-digit modulus
+        const double outs = packed || g.upper ? (double)m * (m + 1) / 2.0 : (double)m * nn;
+        const double gemms = (double)single + 4.0 * (L - single);
+        t_totals.mp_macs += outs * (double)k;
+        t_totals.int8_ops += 2.0 * outs * (double)k * gemms;
+        for (int i = 0; i < 4; i++) t_totals.stage_ms[i] += t_stats.stage_ms[i];
+        t_totals.calls++;
+    }
     return SDPX_GPU_OK;
 }
 

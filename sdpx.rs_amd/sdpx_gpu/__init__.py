@@ -96,6 +96,14 @@ def _declare(L):
     L.sdpx_gpu_sampled_stats.argtypes = [dp, ctypes.POINTER(ctypes.c_int64),
                                          ctypes.POINTER(ctypes.c_int64)]
     L.sdpx_gpu_sampled_stats.restype = i32
+    L.sdpx_gpu_sampled_arrow_f64_dev.argtypes = [u64, vp, u64, vp, vp, vp, ctypes.c_double,
+                                                 ctypes.c_double, ctypes.POINTER(i32), dp]
+    L.sdpx_gpu_sampled_arrow_f64_dev.restype = i32
+    L.sdpx_gpu_rns_totals.argtypes = [i32, dp, dp, dp, ctypes.POINTER(ctypes.c_int64)]
+    L.sdpx_gpu_rns_totals.restype = i32
+    for name in ("sdpx_gpu_adjoint_psd_f64", "sdpx_gpu_adjoint_psd_f64_dev"):
+        getattr(L, name).argtypes = [vp, u64, vp, vp, u64, vp, vp, vp]
+        getattr(L, name).restype = i32
     gemm_args = [u32, u8, u8, u64, u64, u64, vp, u64, vp, u64, u32, vp, u64]
     for name in ("sdpx_gpu_rns_gemm", "sdpx_gpu_rns_gemm_dev"):
         f = getattr(L, name)
@@ -138,6 +146,18 @@ def last_stage_times():
     buf = (ctypes.c_double * 4)()
     lib().sdpx_gpu_last_stage_times(buf, 4)
     return list(buf)
+
+
+def rns_totals(reset=False):
+    """Totals over this thread's profiled exact products (see
+    sdpx_gpu_rns_totals)."""
+    ms = (ctypes.c_double * 4)()
+    ops, macs = ctypes.c_double(), ctypes.c_double()
+    calls = ctypes.c_int64()
+    lib().sdpx_gpu_rns_totals(1 if reset else 0, ms, ctypes.byref(ops), ctypes.byref(macs),
+                              ctypes.byref(calls))
+    return {"stage_ms": list(ms), "int8_ops": ops.value, "mp_macs": macs.value,
+            "calls": calls.value}
 
 
 def _ptr(arr):
@@ -323,3 +343,61 @@ def sampled_stats():
     lib().sdpx_gpu_sampled_stats(secs, ctypes.byref(g), ctypes.byref(h))
     return {"build_s": secs[0], "leaves_s": secs[1], "border_s": secs[2],
             "gpu_products": g.value, "host_products": h.value}
+
+
+class SampledLeafF64(ctypes.Structure):
+    _fields_ = [("side", ctypes.c_uint64), ("samples", ctypes.c_uint64),
+                ("rinv", ctypes.c_void_p), ("basis", ctypes.c_void_p),
+                ("weights", ctypes.c_void_p), ("coupling", ctypes.c_void_p)]
+
+
+class SampledF64:
+    """Device-resident float64 form of a sampled problem (torch tensors on
+    cuda) for sdpx_gpu_sampled_arrow_f64_dev."""
+
+    def __init__(self, problem):
+        import torch
+
+        dev = "cuda"
+        self.t = problem["t"]
+        self.keep = []
+        self.leaves = (SampledLeafF64 * len(problem["blocks"]))()
+        for i, b in enumerate(problem["blocks"]):
+            arrs = [torch.tensor(np.asfortranarray(b[k]).ravel(order="F"), dtype=torch.float64,
+                                 device=dev) for k in ("rinv_f64", "q_f64", "w_f64", "b_f64")]
+            self.keep.append(arrs)
+            self.leaves[i] = SampledLeafF64(b["h"], b["k"], *[a.data_ptr() for a in arrs])
+        self.c = torch.tensor(np.asfortranarray(problem["border_f64"]).ravel(order="F"),
+                              dtype=torch.float64, device=dev)
+        self.s = torch.zeros_like(self.c)
+        self.factor = torch.zeros_like(self.c)
+        torch.cuda.synchronize()
+
+    def run(self, dynamic=None):
+        """dynamic: (eps, delta) of the dynamic pivot rule, or None."""
+        info = ctypes.c_int32()
+        times = (ctypes.c_double * 3)()
+        eps, delta = dynamic if dynamic else (0.0, 0.0)
+        check(lib().sdpx_gpu_sampled_arrow_f64_dev(len(self.leaves), self.leaves, self.t,
+                                                   self.c.data_ptr(), self.s.data_ptr(),
+                                                   self.factor.data_ptr(), eps, delta,
+                                                   ctypes.byref(info), times))
+        return info.value, list(times)
+
+
+def adjoint_psd_f64(factor, rinv, a_svec, ybar):
+    """Adjoint of the PSD Schur solve with the GPU factor of M: returns
+    (bgrad (m x r), cgrad (s x r)) for cotangents ybar (m x r) of y*."""
+    rinv = np.asfortranarray(rinv, dtype=np.float64)
+    a_svec = np.asfortranarray(a_svec, dtype=np.float64)
+    ybar = np.asfortranarray(np.atleast_2d(ybar.T).T if ybar.ndim == 1 else ybar, dtype=np.float64)
+    n = rinv.shape[0]
+    m, r = ybar.shape
+    s = n * (n + 1) // 2
+    bgrad = np.zeros((m, r), order="F")
+    cgrad = np.zeros((s, r), order="F")
+    vp = ctypes.c_void_p
+    check(lib().sdpx_gpu_adjoint_psd_f64(factor._h, n, rinv.ctypes.data_as(vp),
+                                         a_svec.ctypes.data_as(vp), r, ybar.ctypes.data_as(vp),
+                                         bgrad.ctypes.data_as(vp), cgrad.ctypes.data_as(vp)))
+    return bgrad, cgrad

@@ -32,7 +32,8 @@ def chebyshev(x, count):
     return np.cos(np.outer(np.arange(count), np.arccos(x)))  # count x len(x)
 
 
-def sampled_problem(n, nblocks=40, hmin=10, hmax=60, t=2000, eps=1e-8, seed=7, lift=True):
+def sampled_problem(n, nblocks=40, hmin=10, hmax=60, t=2000, eps=1e-8, seed=7, lift=True,
+                    weight_range=(1e-6, 1.0), samples=lambda h: 2 * h - 1):
     """Returns dict with per-block records and float64 views:
     blocks: list of {h, k, rinv, q, w, b (records), and *_f64 arrays};
     border (t x t records, upper = -eps I), border_f64."""
@@ -40,10 +41,10 @@ def sampled_problem(n, nblocks=40, hmin=10, hmax=60, t=2000, eps=1e-8, seed=7, l
     blocks = []
     for _ in range(nblocks):
         h = int(rng.integers(hmin, hmax + 1))
-        k = 2 * h - 1
+        k = int(samples(h))
         x = rng.uniform(-1.0, 1.0, k)
         q = chebyshev(x, h)                          # h x k
-        w = np.exp(rng.uniform(np.log(1e-6), 0.0, k))
+        w = np.exp(rng.uniform(np.log(weight_range[0]), np.log(weight_range[1]), k))
         g = rng.standard_normal((h, h))
         rinv = np.eye(h) + g / (4.0 * np.sqrt(h))
         b = chebyshev(x, t).T                         # k x t
