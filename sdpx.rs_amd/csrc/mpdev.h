@@ -1,0 +1,415 @@
+// Multiple-precision arithmetic for GPU panel kernels, on the MpFloat<N>
+// interchange layout held as 32-bit words: correctly rounded (nearest-even)
+// products, sums, reciprocals and exact dot products.  Everything is
+// __host__ __device__ so the host instantiation can be checked bit for bit
+// against the host library on a machine without a GPU.
+//
+// Exact dots accumulate every partial product word into signed 64-bit slots
+// (one per 32-bit position of a fixed window, strided so per-thread slices
+// of LDS are bank-conflict free) and resolve carries once at the end, then
+// round once - the contract of crates/arithmetic/src/exactdot.rs:1-13.  A
+// dot whose exponent spread does not fit the window reports failure and the
+// caller takes the host path for it.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace sdpx {
+namespace mpd {
+
+constexpr int KIND_ZERO = 2, KIND_REG = 3, KIND_INF = 1, KIND_NAN = 0;
+constexpr int64_t EMAX = (1LL << 30) - 1, EMIN = 1 - (1LL << 30);
+
+// A value in registers: W = 2N mantissa words (LSB first), kind, exponent.
+template <int W>
+struct Val {
+    uint32_t m[W];
+    int32_t kind, e;
+};
+
+template <int W>
+__host__ __device__ inline void load(const uint32_t *rec, Val<W> &v) {
+#pragma unroll
+    for (int i = 0; i < W; i++) v.m[i] = rec[i];
+    v.kind = (int32_t)rec[W];
+    v.e = (int32_t)rec[W + 1];
+}
+template <int W>
+__host__ __device__ inline void store(uint32_t *rec, const Val<W> &v) {
+#pragma unroll
+    for (int i = 0; i < W; i++) rec[i] = v.m[i];
+    rec[W] = (uint32_t)v.kind;
+    rec[W + 1] = (uint32_t)v.e;
+}
+template <int W>
+__host__ __device__ inline void set_zero(Val<W> &v) {
+#pragma unroll
+    for (int i = 0; i < W; i++) v.m[i] = 0;
+    v.kind = KIND_ZERO;
+    v.e = 0;
+}
+template <int W>
+__host__ __device__ inline bool is_zero(const Val<W> &v) {
+    return v.kind == KIND_ZERO || v.kind == -KIND_ZERO;
+}
+template <int W>
+__host__ __device__ inline bool is_regular(const Val<W> &v) {
+    return v.kind == KIND_REG || v.kind == -KIND_REG;
+}
+__host__ __device__ inline int bitlen32(uint32_t x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return x ? 32 - __clz((int)x) : 0;
+#else
+    return x ? 32 - __builtin_clz(x) : 0;
+#endif
+}
+
+// Round (-1)^neg * mag * 2^scale once (mag: L words, LSB first) into out.
+// Indexing is static (the word selection loops over all words), so mag may
+// live in registers.
+template <int W, int L>
+__host__ __device__ inline void round_words(bool neg, const uint32_t (&mag)[L], int64_t scale,
+                                            Val<W> &out) {
+    int top = -1;
+    uint32_t topw = 0;
+#pragma unroll
+    for (int i = 0; i < L; i++)
+        if (mag[i]) {
+            top = i;
+            topw = mag[i];
+        }
+    if (top < 0) {
+        set_zero(out);
+        return;
+    }
+    const int P = 32 * W;
+    const int bits = 32 * top + bitlen32(topw);
+    int shift = bits - P;
+    // mantissa word u = bits [shift + 32u, +32)
+#pragma unroll
+    for (int u = 0; u < W; u++) {
+        const int b = shift + 32 * u;
+        uint32_t w = 0;
+#pragma unroll
+        for (int i = 0; i < L; i++) {
+            const int pos = 32 * i - b;  // bit position of mag[i]'s LSB in the word
+            if (pos > -32 && pos < 32) w |= pos >= 0 ? (mag[i] << pos) : (mag[i] >> (-pos));
+        }
+        out.m[u] = w;
+    }
+    if (shift > 0) {
+        const int rb = shift - 1;
+        bool round = false, sticky = false;
+#pragma unroll
+        for (int i = 0; i < L; i++) {
+            const int lo = 32 * i;
+            if (rb >= lo && rb < lo + 32) {
+                const int off = rb - lo;
+                round = (mag[i] >> off) & 1;
+                if (off && (mag[i] & ((1u << off) - 1))) sticky = true;
+            } else if (lo + 31 < rb && mag[i]) {
+                sticky = true;
+            }
+        }
+        if (round && (sticky || (out.m[0] & 1))) {
+            uint32_t c = 1;
+#pragma unroll
+            for (int u = 0; u < W; u++) {
+                const uint32_t v = out.m[u] + c;
+                c = (c && v == 0) ? 1 : 0;
+                out.m[u] = v;
+            }
+            if (c) {
+                out.m[W - 1] = 0x80000000u;
+                shift++;
+            }
+        }
+    }
+    const int64_t e = scale + shift + P;
+    if (e > EMAX) {
+#pragma unroll
+        for (int u = 0; u < W; u++) out.m[u] = 0;
+        out.kind = neg ? -KIND_INF : KIND_INF;
+        out.e = 0;
+        return;
+    }
+    if (e < EMIN) {
+        set_zero(out);
+        return;
+    }
+    out.kind = neg ? -KIND_REG : KIND_REG;
+    out.e = (int32_t)e;
+}
+
+// RN(a * b)
+template <int W>
+__host__ __device__ inline void mul(const Val<W> &a, const Val<W> &b, Val<W> &out) {
+    if (is_zero(a) || is_zero(b)) {
+        set_zero(out);
+        return;
+    }
+    uint32_t p[2 * W];
+#pragma unroll
+    for (int i = 0; i < 2 * W; i++) p[i] = 0;
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+        uint64_t carry = 0;
+#pragma unroll
+        for (int j = 0; j < W; j++) {
+            const uint64_t t = (uint64_t)a.m[i] * b.m[j] + p[i + j] + carry;
+            p[i + j] = (uint32_t)t;
+            carry = t >> 32;
+        }
+        p[i + W] = (uint32_t)carry;
+    }
+    round_words<W, 2 * W>((a.kind < 0) != (b.kind < 0), p,
+                          (int64_t)a.e + b.e - 2 * 32 * (int64_t)W, out);
+}
+
+// RN(a + s*b) with s = +1 (add) or -1 (sub).
+template <int W>
+__host__ __device__ inline void add_signed(const Val<W> &a, const Val<W> &b, bool negate_b,
+                                           Val<W> &out) {
+    const bool na = a.kind < 0, nb = (b.kind < 0) != negate_b;
+    if (is_zero(b)) {
+        out = a;
+        return;
+    }
+    if (is_zero(a)) {
+        out = b;
+        if (negate_b && is_regular(out)) out.kind = -out.kind;
+        return;
+    }
+    const bool swap = b.e > a.e;
+    Val<W> xv, yv;
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+        xv.m[i] = swap ? b.m[i] : a.m[i];
+        yv.m[i] = swap ? a.m[i] : b.m[i];
+    }
+    xv.e = swap ? b.e : a.e;
+    yv.e = swap ? a.e : b.e;
+    const bool nx = swap ? nb : na, ny = swap ? na : nb;
+    const Val<W> *x = &xv, *y = &yv;
+    const int64_t P = 32 * W;
+    const int64_t gap = (int64_t)x->e - y->e;
+    if (gap >= P + 2) {
+        out = xv;
+        out.kind = nx ? -KIND_REG : KIND_REG;
+        return;
+    }
+    // X = Mx << gap (gap <= P + 1), Y = My; both at scale e_y - P
+    constexpr int L = 2 * W + 2;
+    uint32_t X[L], Y[L];
+#pragma unroll
+    for (int i = 0; i < L; i++) {
+        X[i] = 0;
+        Y[i] = i < W ? y->m[i] : 0;
+    }
+    const int wsh = (int)(gap >> 5), bsh = (int)(gap & 31);
+#pragma unroll
+    for (int i = 0; i < L; i++) {
+        // X[i] = bits of Mx << gap: word i takes Mx words i - wsh and i - wsh - 1
+        uint32_t v = 0;
+#pragma unroll
+        for (int k = 0; k < W; k++) {
+            if (k == i - wsh) v |= x->m[k] << bsh;
+            if (bsh && k == i - wsh - 1) v |= x->m[k] >> (32 - bsh);
+        }
+        X[i] = v;
+    }
+    if (nx == ny) {
+        uint64_t c = 0;
+#pragma unroll
+        for (int i = 0; i < L; i++) {
+            const uint64_t t = (uint64_t)X[i] + Y[i] + c;
+            X[i] = (uint32_t)t;
+            c = t >> 32;
+        }
+        round_words<W, L>(nx, X, (int64_t)y->e - P, out);
+        return;
+    }
+    int cmp = 0;
+#pragma unroll
+    for (int i = L - 1; i >= 0; i--)
+        if (cmp == 0 && X[i] != Y[i]) cmp = X[i] > Y[i] ? 1 : -1;
+    if (cmp == 0) {
+        set_zero(out);
+        return;
+    }
+    uint32_t D[L];
+    uint64_t borrow = 0;
+#pragma unroll
+    for (int i = 0; i < L; i++) {
+        const uint64_t big = cmp > 0 ? X[i] : Y[i], small = cmp > 0 ? Y[i] : X[i];
+        const uint64_t d = big - small - borrow;
+        D[i] = (uint32_t)d;
+        borrow = (d >> 63) & 1;
+    }
+    round_words<W, L>(cmp > 0 ? nx : ny, D, (int64_t)y->e - P, out);
+}
+
+// ---------------------------------------------------------------------------
+// Exact dot with a slot window: slots[k * stride] for k < SLOTS, each a
+// signed 64-bit sum of 32-bit partial words at position k (bit 32k above
+// the window base).
+// ---------------------------------------------------------------------------
+template <int W, int SLOTS>
+struct DotWindow {
+    int64_t *slots;
+    int stride;
+    int64_t base;  // scale of slot 0's bit 0
+    __host__ __device__ void clear() {
+        for (int k = 0; k < SLOTS; k++) slots[k * stride] = 0;
+    }
+    // Add s * (a*b) where the product's scale (value = Ma*Mb*2^scale) is
+    // scale >= base and the product fits the window.
+    __host__ __device__ void add_product(const Val<W> &a, const Val<W> &b, bool negate) {
+        uint32_t p[2 * W];
+#pragma unroll
+        for (int i = 0; i < 2 * W; i++) p[i] = 0;
+#pragma unroll
+        for (int i = 0; i < W; i++) {
+            uint64_t carry = 0;
+#pragma unroll
+            for (int j = 0; j < W; j++) {
+                const uint64_t t = (uint64_t)a.m[i] * b.m[j] + p[i + j] + carry;
+                p[i + j] = (uint32_t)t;
+                carry = t >> 32;
+            }
+            p[i + W] = (uint32_t)carry;
+        }
+        const bool neg = ((a.kind < 0) != (b.kind < 0)) != negate;
+        const int64_t scale = (int64_t)a.e + b.e - 2 * 32 * (int64_t)W;
+        add_words<2 * W>(p, scale, neg);
+    }
+    // Add s * (value) of one operand (its mantissa at scale e - P).
+    __host__ __device__ void add_value(const Val<W> &a, bool negate) {
+        const bool neg = (a.kind < 0) != negate;
+        add_words<W>(a.m, (int64_t)a.e - 32 * (int64_t)W, neg);
+    }
+    template <int L>
+    __host__ __device__ void add_words(const uint32_t (&p)[L], int64_t scale, bool neg) {
+        const int64_t off = scale - base;
+        const int wo = (int)(off >> 5), bo = (int)(off & 31);
+        // shifted words p << bo occupy L + 1 slots from wo
+        uint32_t prev = 0;
+#pragma unroll
+        for (int i = 0; i <= L; i++) {
+            const uint32_t cur = i < L ? p[i] : 0;
+            const uint32_t v = bo ? ((cur << bo) | (prev >> (32 - bo))) : cur;
+            prev = cur;
+            int64_t *s = slots + (int64_t)(wo + i) * stride;
+            *s = neg ? *s - (int64_t)v : *s + (int64_t)v;
+        }
+    }
+    // Resolve carries and round once into out.  Returns false when the
+    // value does not fit (never for admitted windows).
+    __host__ __device__ void finish(Val<W> &out) {
+        // carry-resolve into 32-bit words (two's complement), kept in slots
+        int64_t carry = 0;
+        for (int k = 0; k < SLOTS; k++) {
+            int64_t *s = slots + (int64_t)k * stride;
+            const int64_t v = *s + carry;
+            // floor division by 2^32 (arithmetic shift)
+            carry = v >> 32;
+            *s = (int64_t)(uint32_t)(v & 0xFFFFFFFFLL);
+        }
+        const bool neg = carry < 0;
+        if (neg) {
+            uint64_t c = 1;
+            for (int k = 0; k < SLOTS; k++) {
+                int64_t *s = slots + (int64_t)k * stride;
+                const uint64_t v = (uint64_t)(~(uint32_t)*s) + c;
+                *s = (int64_t)(uint32_t)v;
+                c = v >> 32;
+            }
+        }
+        // top word, then gather the top W + 2 words around it into registers
+        int top = -1;
+        for (int k = SLOTS - 1; k >= 0 && top < 0; k--)
+            if (slots[(int64_t)k * stride] != 0) top = k;
+        if (top < 0) {
+            set_zero(out);
+            return;
+        }
+        // words [top - W - 1, top]; anything below is folded into a sticky word
+        constexpr int G = W + 2;
+        uint32_t g[G + 1];
+        const int lo = top - G + 1;
+        bool sticky = false;
+        for (int k = 0; k < lo && k < SLOTS; k++)
+            if (slots[(int64_t)k * stride] != 0) sticky = true;
+#pragma unroll
+        for (int i = 0; i < G; i++) {
+            const int k = lo + i;
+            g[i + 1] = k >= 0 ? (uint32_t)slots[(int64_t)k * stride] : 0u;
+        }
+        g[0] = sticky ? 1u : 0u;  // one sticky bit below the gathered words
+        round_words<W, G + 1>(neg, g, base + 32 * (int64_t)(lo - 1), out);
+    }
+};
+
+// RN(1 / d) by restoring division of 2^(2P+1) by the mantissa, quotient
+// bits plus a sticky bit (matches the host library's reciprocal).
+template <int W>
+__host__ __device__ inline void recip(const Val<W> &d, Val<W> &out) {
+    if (is_zero(d)) {
+        set_zero(out);
+        out.kind = d.kind < 0 ? -KIND_INF : KIND_INF;
+        return;
+    }
+    const int P = 32 * W;
+    uint32_t rem[W + 1];
+#pragma unroll
+    for (int i = 0; i <= W; i++) rem[i] = 0;
+    constexpr int QW = W + 2;  // quotient words (P + 3 bits suffice)
+    uint32_t q[QW + 1];
+#pragma unroll
+    for (int i = 0; i <= QW; i++) q[i] = 0;
+    for (int bit = 2 * P + 1; bit >= 0; bit--) {
+        uint32_t carry = (bit == 2 * P + 1) ? 1u : 0u;
+#pragma unroll
+        for (int i = 0; i <= W; i++) {
+            const uint32_t v = rem[i];
+            rem[i] = (v << 1) | carry;
+            carry = v >> 31;
+        }
+        int c = rem[W] ? 1 : 0;
+#pragma unroll
+        for (int i = W - 1; i >= 0; i--)
+            if (c == 0 && rem[i] != d.m[i]) c = rem[i] > d.m[i] ? 1 : -1;
+        if (c >= 0) {
+            uint64_t borrow = 0;
+#pragma unroll
+            for (int i = 0; i <= W; i++) {
+                const uint64_t mi = i < W ? d.m[i] : 0;
+                const uint64_t t = (uint64_t)rem[i] - mi - borrow;
+                rem[i] = (uint32_t)t;
+                borrow = (t >> 63) & 1;
+            }
+            if (bit < 32 * QW) {
+#pragma unroll
+                for (int i = 0; i < QW; i++)
+                    if (i == (bit >> 5)) q[i] |= 1u << (bit & 31);
+            }
+        }
+    }
+    bool sticky = false;
+#pragma unroll
+    for (int i = 0; i <= W; i++) sticky |= rem[i] != 0;
+    // mag = (q << 1) | sticky
+    uint32_t mag[QW + 1];
+    uint32_t c = sticky ? 1u : 0u;
+#pragma unroll
+    for (int i = 0; i < QW; i++) {
+        mag[i] = (q[i] << 1) | c;
+        c = q[i] >> 31;
+    }
+    mag[QW] = c;
+    round_words<W, QW + 1>(d.kind < 0, mag, -(int64_t)P - 2 - d.e, out);
+}
+
+}  // namespace mpd
+}  // namespace sdpx

@@ -280,8 +280,9 @@ inline void DotAcc::dot(const uint8_t *x, size_t xs, const uint8_t *y, size_t ys
         hi = std::max(hi, s);
     }
     View c{n, sub_from};
-    const bool with_c = sub_from && !c.zero();
-    if (with_c) {
+    const bool with_c = sub_from != nullptr;  // result c - sum (c may be zero)
+    const bool c_nz = with_c && !c.zero();
+    if (c_nz) {
         int64_t s = (int64_t)c.exponent() - p;
         lo = std::min(lo, s);
         hi = std::max(hi, s - p);  // treat as a 2P-bit term at scale s - P
@@ -354,7 +355,7 @@ inline void DotAcc::dot(const uint8_t *x, size_t xs, const uint8_t *y, size_t ys
             if (!carry) break;
             carry = ++v == 0;
         }
-        add_shift(c.limbs(), n, (int64_t)c.exponent() - p - lo, c.negative());
+        if (c_nz) add_shift(c.limbs(), n, (int64_t)c.exponent() - p - lo, c.negative());
     }
     const bool neg = (acc[len - 1] >> 63) != 0;
     if (neg) {

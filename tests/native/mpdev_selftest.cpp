@@ -1,0 +1,139 @@
+// Host instantiation of the GPU panel arithmetic (mpdev.h) against the host
+// library's arithmetic (mpx.h), bit for bit, on random values.
+// Build: hipcc -x hip --offload-host-only -O2 -std=c++17 (see test_mpdev.py).
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "../../sdpx.rs_amd/csrc/mpdev.h"
+#include "../../sdpx.rs_amd/csrc/mpx.h"
+
+using namespace sdpx;
+
+template <int N>
+void random_value(std::mt19937_64 &rng, uint8_t *rec, int spread) {
+    uint64_t *l = (uint64_t *)rec;
+    for (int i = 0; i < N; i++) l[i] = rng();
+    l[N - 1] |= 1ULL << 63;
+    if (rng() % 7 == 0) l[0] &= ~0xFFFFULL;  // some trailing zeros
+    int32_t kind = (rng() & 1) ? -3 : 3;
+    if (rng() % 23 == 0) {
+        std::memset(rec, 0, 8 * N);
+        kind = 2;
+    }
+    *(int32_t *)(rec + 8 * N) = kind;
+    *(int32_t *)(rec + 8 * N + 4) = kind == 2 ? 0 : (int32_t)(rng() % (2 * spread + 1)) - spread;
+}
+
+template <int N>
+int run(int count, int spread, uint64_t seed) {
+    constexpr int W = 2 * N;
+    const size_t vb = 8 * N + 8;
+    std::mt19937_64 rng(seed);
+    std::vector<uint8_t> a(vb), b(vb), r1(vb), r2(vb);
+    int bad = 0;
+    for (int t = 0; t < count; t++) {
+        random_value<N>(rng, a.data(), spread);
+        random_value<N>(rng, b.data(), rng() % 4 == 0 ? 2 * 32 * N + 40 : spread);
+        mpd::Val<W> va, vb2, vr;
+        mpd::load((const uint32_t *)a.data(), va);
+        mpd::load((const uint32_t *)b.data(), vb2);
+        mpx::View xa{N, a.data()}, xb{N, b.data()};
+        // mul
+        mpd::mul(va, vb2, vr);
+        mpd::store((uint32_t *)r1.data(), vr);
+        mpx::mul(xa, xb, r2.data());
+        if (std::memcmp(r1.data(), r2.data(), vb)) bad++, std::printf("mul mismatch %d\n", t);
+        // sub / add
+        mpd::add_signed(va, vb2, true, vr);
+        mpd::store((uint32_t *)r1.data(), vr);
+        mpx::sub(xa, xb, r2.data());
+        if (std::memcmp(r1.data(), r2.data(), vb)) bad++, std::printf("sub mismatch %d\n", t);
+        mpd::add_signed(va, vb2, false, vr);
+        mpd::store((uint32_t *)r1.data(), vr);
+        mpx::add(xa, xb, r2.data());
+        if (std::memcmp(r1.data(), r2.data(), vb)) bad++, std::printf("add mismatch %d\n", t);
+        // recip
+        if (!mpd::is_zero(va)) {
+            mpd::recip(va, vr);
+            mpd::store((uint32_t *)r1.data(), vr);
+            mpx::recip(xa, r2.data());
+            if (std::memcmp(r1.data(), r2.data(), vb)) bad++, std::printf("recip mismatch %d\n", t);
+        }
+    }
+    // exact dots with the slot window, against mpx's accumulator
+    constexpr int SLOTS = 4 * N + 24;
+    mpd::Val<W> vr;
+    std::vector<int64_t> slots(SLOTS * 3);
+    for (int t = 0; t < count / 4; t++) {
+        const int k = 1 + (int)(rng() % 64);
+        std::vector<uint8_t> xs(k * vb), ys(k * vb), c(vb);
+        for (int i = 0; i < k; i++) {
+            random_value<N>(rng, &xs[i * vb], 40);
+            random_value<N>(rng, &ys[i * vb], 40);
+        }
+        random_value<N>(rng, c.data(), 60);
+        for (int with_c = 0; with_c < 2; with_c++) {
+            mpx::DotAcc acc;
+            acc.dot(xs.data(), vb, ys.data(), vb, k, N, r2.data(), with_c ? c.data() : nullptr);
+            // window base: min scale of the terms
+            int64_t lo = INT64_MAX, hi = INT64_MIN;
+            for (int i = 0; i < k; i++) {
+                mpx::View x{N, &xs[i * vb]}, y{N, &ys[i * vb]};
+                if (x.zero() || y.zero()) continue;
+                int64_t s = (int64_t)x.exponent() + y.exponent() - 2 * 64 * N;
+                lo = std::min(lo, s);
+                hi = std::max(hi, s);
+            }
+            mpx::View cv{N, c.data()};
+            if (with_c && !cv.zero()) {
+                lo = std::min<int64_t>(lo, (int64_t)cv.exponent() - 64 * N);
+                hi = std::max<int64_t>(hi, (int64_t)cv.exponent() - 2 * 64 * N);
+            }
+            if (lo == INT64_MAX) continue;
+            if ((hi - lo) / 32 + 2 * W + 4 > SLOTS) continue;  // outside the window: host path
+            mpd::DotWindow<W, SLOTS> dw{slots.data(), 3, lo};
+            dw.clear();
+            for (int i = 0; i < k; i++) {
+                mpd::Val<W> x, y;
+                mpd::load((const uint32_t *)&xs[i * vb], x);
+                mpd::load((const uint32_t *)&ys[i * vb], y);
+                if (mpd::is_zero(x) || mpd::is_zero(y)) continue;
+                dw.add_product(x, y, with_c != 0);
+            }
+            if (with_c) {
+                mpd::Val<W> cvv;
+                mpd::load((const uint32_t *)c.data(), cvv);
+                if (!mpd::is_zero(cvv)) dw.add_value(cvv, false);
+            }
+            dw.finish(vr);
+            mpd::store((uint32_t *)r1.data(), vr);
+            if (std::memcmp(r1.data(), r2.data(), vb)) {
+                bad++;
+                std::printf("dot mismatch %d k=%d c=%d\n", t, k, with_c);
+                if (std::getenv("DUMP") && bad == 1) {
+                    FILE *f = fopen("/tmp/dotcase.bin", "wb");
+                    int32_t hdr[2] = {N, k};
+                    fwrite(hdr, 4, 2, f);
+                    fwrite(xs.data(), 1, k * vb, f);
+                    fwrite(ys.data(), 1, k * vb, f);
+                    fwrite(c.data(), 1, vb, f);
+                    fwrite(r1.data(), 1, vb, f);
+                    fwrite(r2.data(), 1, vb, f);
+                    fclose(f);
+                }
+            }
+        }
+    }
+    return bad;
+}
+
+int main(int argc, char **argv) {
+    int count = argc > 1 ? std::atoi(argv[1]) : 20000;
+    int bad = run<1>(count, 200, 1) + run<2>(count, 200, 2) + run<4>(count, 300, 3) +
+              run<8>(count, 600, 4);
+    std::printf("mismatches %d\n", bad);
+    return bad ? 1 : 0;
+}
