@@ -153,6 +153,16 @@ int32_t sdpx_gpu_sampled_arrow(uint32_t limbs, uint64_t nleaves, const sdpx_gpu_
                                uint64_t t, void *border, const int8_t *border_signs,
                                uint64_t nborder_signs, void *border_dinv,
                                sdpx_gpu_sampled_factor *factors, uint32_t flags, int32_t *ok);
+/* Device-resident form (all pointers device pointers; 128/256/512 bits):
+ * leaf factors, D^-1 and Y land in l_all (sum k^2 values), dinv_all (sum k)
+ * and y_all (sum k*t) in leaf order.  Returns 2 (declined, nothing
+ * guaranteed) when a dot's exponent spread or a product exceeds the GPU
+ * kernels' limits; the host entry point above then finishes on the host. */
+int32_t sdpx_gpu_sampled_arrow_dev(uint32_t limbs, uint64_t nleaves,
+                                   const sdpx_gpu_sampled_leaf *leaves, uint64_t t, void *l_all,
+                                   void *dinv_all, void *y_all, void *border,
+                                   const int8_t *border_signs, uint64_t nborder_signs,
+                                   void *border_dinv, int32_t *ok);
 /* Phase times (s: build, leaves, border) and product counts of this
  * thread's last sdpx_gpu_sampled_arrow (measurement only). */
 int32_t sdpx_gpu_sampled_stats(double *seconds3, int64_t *gpu_products, int64_t *host_products);

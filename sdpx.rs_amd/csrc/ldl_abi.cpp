@@ -6,6 +6,8 @@
 
 #include "common.h"
 #include "ldl.h"
+#include "ldl_device.h"
+#include "mp_kernels.h"
 
 namespace sdpx {
 int32_t abi_boundary(const std::function<int32_t()> &f);
@@ -36,6 +38,29 @@ int32_t sdpx_gpu_mp_ldl(uint32_t limbs, uint64_t n, void *a, uint64_t lda, const
         opt.reg_delta = (const uint8_t *)reg_delta;
         bool recursive = (flags & SDPX_GPU_LDL_DOTS) == 0 && n >= 256;
         t_ldl_stats = sdpx::LdlStats{};
+        if (opt.gpu_products && sdpx::mp_kernels_support((int)limbs) && n > 0) {
+            // device-resident factor: stage in, factor in HBM, stage out
+            sdpx::Device &d = sdpx::device();
+            std::lock_guard<std::recursive_mutex> g(d.mu);
+            const size_t vb = sdpx::value_bytes((int)limbs);
+            uint8_t *dl = (uint8_t *)d.ws.get(72, n * n * vb);
+            uint8_t *dd = (uint8_t *)d.ws.get(73, n * vb);
+            HIP_CHECK(hipMemcpy2DAsync(dl, n * vb, a, lda * vb, n * vb, n, hipMemcpyHostToDevice, d.stream));
+            HIP_CHECK(hipStreamSynchronize(d.stream));
+            int r = sdpx::ldl_factor_device((int)limbs, dl, n, n, signs, nsigns, reg_eps, reg_delta,
+                                            dd, t_ldl_stats, recursive);
+            if (r >= 0) {
+                if (r == 1) {
+                    HIP_CHECK(hipMemcpy2DAsync(a, lda * vb, dl, n * vb, n * vb, n, hipMemcpyDeviceToHost,
+                                               d.stream));
+                    HIP_CHECK(hipMemcpyAsync(dinv, dd, n * vb, hipMemcpyDeviceToHost, d.stream));
+                    HIP_CHECK(hipStreamSynchronize(d.stream));
+                }
+                if (ok) *ok = r;
+                return SDPX_GPU_OK;
+            }
+            t_ldl_stats.declined++;  // not handled on the GPU: host path below
+        }
         bool good = sdpx::ldl_factor((int)limbs, (uint8_t *)a, lda, n, signs, nsigns, opt,
                                      (uint8_t *)dinv, t_ldl_stats, recursive);
         if (ok) *ok = good ? 1 : 0;
@@ -58,7 +83,102 @@ int32_t sdpx_gpu_mp_ldl_stats(int64_t *gpu_products, int64_t *host_products, int
 
 namespace {
 thread_local sdpx::SampledStats t_sampled_stats;
+
+// Stage a host-resident sampled problem through HBM and run the device
+// pipeline; -1 when the GPU path declines (then the host path runs).
+int sampled_host_via_device(int limbs, uint64_t nleaves, const sdpx_gpu_sampled_leaf *leaves,
+                            uint64_t t, uint8_t *border, const int8_t *bsigns, uint64_t nbsigns,
+                            uint8_t *border_dinv, sdpx_gpu_sampled_factor *factors,
+                            sdpx::SampledStats &st) {
+    sdpx::Device &d = sdpx::device();
+    std::lock_guard<std::recursive_mutex> g(d.mu);
+    hipStream_t s = d.stream;
+    const size_t vb = sdpx::value_bytes(limbs);
+    size_t in_vals = 0, lsum = 0, ksum = 0;
+    for (uint64_t i = 0; i < nleaves; i++) {
+        const auto &l = leaves[i];
+        in_vals += l.side * l.side + l.side * l.samples + l.samples + l.samples * t;
+        lsum += l.samples * l.samples;
+        ksum += l.samples;
+    }
+    uint8_t *din = (uint8_t *)d.ws.get(74, std::max<size_t>(in_vals, 1) * vb);
+    uint8_t *dl = (uint8_t *)d.ws.get(75, std::max<size_t>(lsum, 1) * vb);
+    uint8_t *dd = (uint8_t *)d.ws.get(76, std::max<size_t>(ksum, 1) * vb);
+    uint8_t *dy = (uint8_t *)d.ws.get(77, std::max<size_t>(ksum * t, 1) * vb);
+    uint8_t *db = (uint8_t *)d.ws.get(78, std::max<size_t>(t * t, 1) * vb);
+    uint8_t *dbd = (uint8_t *)d.ws.get(79, std::max<size_t>(t, 1) * vb);
+    std::vector<sdpx::SampledLeafDev> dev(nleaves);
+    size_t off = 0;
+    auto put = [&](const void *src, size_t count) {
+        uint8_t *p = din + off * vb;
+        if (count) HIP_CHECK(hipMemcpyAsync(p, src, count * vb, hipMemcpyHostToDevice, s));
+        off += count;
+        return p;
+    };
+    for (uint64_t i = 0; i < nleaves; i++) {
+        const auto &l = leaves[i];
+        dev[i].h = l.side;
+        dev[i].k = l.samples;
+        dev[i].rinv = put(l.rinv, l.side * l.side);
+        dev[i].q = put(l.basis, l.side * l.samples);
+        dev[i].w = put(l.weights, l.samples);
+        dev[i].b = put(l.coupling, l.samples * t);
+    }
+    if (t) HIP_CHECK(hipMemcpyAsync(db, border, t * t * vb, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    int r = sdpx::sampled_arrow_device(limbs, dev, t, dl, dd, dy, db, bsigns, nbsigns, dbd, st);
+    if (r < 0) return -1;
+    if (t) {
+        HIP_CHECK(hipMemcpyAsync(border, db, t * t * vb, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipMemcpyAsync(border_dinv, dbd, t * vb, hipMemcpyDeviceToHost, s));
+    }
+    if (factors) {
+        size_t lo = 0, ko = 0;
+        for (uint64_t i = 0; i < nleaves; i++) {
+            const size_t K = leaves[i].samples;
+            auto &f = factors[i];
+            if (f.l) HIP_CHECK(hipMemcpyAsync(f.l, dl + lo * vb, K * K * vb, hipMemcpyDeviceToHost, s));
+            if (f.dinv) HIP_CHECK(hipMemcpyAsync(f.dinv, dd + ko * vb, K * vb, hipMemcpyDeviceToHost, s));
+            if (f.y) HIP_CHECK(hipMemcpyAsync(f.y, dy + ko * t * vb, K * t * vb, hipMemcpyDeviceToHost, s));
+            lo += K * K;
+            ko += K;
+        }
+    }
+    HIP_CHECK(hipStreamSynchronize(s));
+    return r;
 }
+}  // namespace
+
+extern "C" {
+
+int32_t sdpx_gpu_sampled_arrow_dev(uint32_t limbs, uint64_t nleaves,
+                                   const sdpx_gpu_sampled_leaf *leaves, uint64_t t, void *l_all,
+                                   void *dinv_all, void *y_all, void *border,
+                                   const int8_t *border_signs, uint64_t nborder_signs,
+                                   void *border_dinv, int32_t *ok) {
+    return sdpx::abi_boundary([&]() -> int32_t {
+        if (!sdpx::mp_kernels_support((int)limbs))
+            return sdpx::abi_invalid("device pipeline supports 128, 256 and 512 bits");
+        sdpx::Device &d = sdpx::device();
+        std::lock_guard<std::recursive_mutex> g(d.mu);
+        std::vector<sdpx::SampledLeafDev> dev(nleaves);
+        for (uint64_t i = 0; i < nleaves; i++)
+            dev[i] = sdpx::SampledLeafDev{leaves[i].side, leaves[i].samples,
+                                          (const uint8_t *)leaves[i].rinv,
+                                          (const uint8_t *)leaves[i].basis,
+                                          (const uint8_t *)leaves[i].weights,
+                                          (const uint8_t *)leaves[i].coupling};
+        t_sampled_stats = sdpx::SampledStats{};
+        int r = sdpx::sampled_arrow_device((int)limbs, dev, t, (uint8_t *)l_all, (uint8_t *)dinv_all,
+                                           (uint8_t *)y_all, (uint8_t *)border, border_signs,
+                                           nborder_signs, (uint8_t *)border_dinv, t_sampled_stats);
+        if (r < 0) return SDPX_GPU_DECLINED;
+        if (ok) *ok = r;
+        return SDPX_GPU_OK;
+    });
+}
+
+}  // extern "C"
 
 extern "C" {
 
@@ -86,6 +206,15 @@ int32_t sdpx_gpu_sampled_arrow(uint32_t limbs, uint64_t nleaves, const sdpx_gpu_
         opt.gpu_products = (flags & SDPX_GPU_LDL_HOST_PRODUCTS) == 0;
         std::vector<sdpx::SampledLeafOut> out;
         t_sampled_stats = sdpx::SampledStats{};
+        if (opt.gpu_products && sdpx::mp_kernels_support((int)limbs)) {
+            int r = sampled_host_via_device((int)limbs, nleaves, leaves, t, (uint8_t *)border,
+                                            border_signs, nborder_signs, (uint8_t *)border_dinv,
+                                            factors, t_sampled_stats);
+            if (r >= 0) {
+                if (ok) *ok = r;
+                return SDPX_GPU_OK;
+            }
+        }
         bool good = sdpx::sampled_arrow_factor((int)limbs, in, t, (uint8_t *)border, border_signs,
                                                nborder_signs, opt, out, (uint8_t *)border_dinv,
                                                t_sampled_stats);

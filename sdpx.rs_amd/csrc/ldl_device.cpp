@@ -1,0 +1,383 @@
+// Device-resident multiple-precision LDL^T and sampled arrow factor: the
+// same recursion and roundings as ldl.cpp / sampled.cpp (and so as the
+// reference, arrow.rs:156-490, 1935-2074; sampled/mod.rs:1836-2143), with
+// every panel step as a GPU kernel (mp_kernels.hip) and every exact product
+// as an RNS product on device pointers.  The host only walks the recursion
+// and launches; values stay in HBM.  A dot whose exponent spread overflows
+// the GPU window, or a declined product, sends the whole factorization to the
+// host path, which gives the same bits.
+#include <cstring>
+#include <functional>
+#include <vector>
+
+#include "common.h"
+#include "ldl.h"
+#include "ldl_device.h"
+#include "mp_kernels.h"
+#include "rns_gemm.h"
+#include "sampled.h"
+
+namespace sdpx {
+namespace {
+
+// Bump allocator over one workspace slot; stream order makes reuse after a
+// reset safe (all users are on the library stream).
+struct Arena {
+    uint8_t *base = nullptr;
+    size_t size = 0, top = 0;
+    void *alloc(size_t bytes) {
+        bytes = (bytes + 255) & ~size_t(255);
+        if (top + bytes > size) throw Error(SDPX_GPU_DEVICE_ERROR, "device arena exhausted");
+        void *p = base + top;
+        top += bytes;
+        return p;
+    }
+};
+
+struct DCtx {
+    int n;
+    size_t vb;
+    hipStream_t st;
+    Arena *arena;
+    const int8_t *d_signs;  // device
+    int sign_mode;          // 0 all +1, 1 shared, 2 per pivot
+    const void *d_reg_eps, *d_reg_delta;
+    int *d_status;  // [0] worst ldl status, [1] regularized, [2] window/decline flag
+    uint8_t *ldl_scratch;
+    size_t ldl_scratch_bytes;
+    LdlStats *stats;
+    bool declined = false;
+    uint8_t *at(uint8_t *p, size_t idx) const { return p + idx * vb; }
+};
+
+int32_t *upload_ints(DCtx &c, const std::vector<size_t> &v) {
+    std::vector<int32_t> h(v.begin(), v.end());
+    int32_t *d = (int32_t *)c.arena->alloc(sizeof(int32_t) * std::max<size_t>(h.size(), 1));
+    if (!h.empty())
+        HIP_CHECK(hipMemcpyAsync(d, h.data(), sizeof(int32_t) * h.size(), hipMemcpyHostToDevice, c.st));
+    // the host vector dies before the copy completes: wait
+    HIP_CHECK(hipStreamSynchronize(c.st));
+    return d;
+}
+
+void product(DCtx &c, char ta, char tb, size_t m, size_t n, size_t k, const uint8_t *a, size_t lda,
+             const uint8_t *b, size_t ldb, uint8_t *out, bool packed) {
+    if (m == 0 || n == 0) return;
+    RnsGemmArgs g{c.n, ta, tb, (int64_t)m, (int64_t)n, (int64_t)k, a, (int64_t)lda, b,
+                  (int64_t)ldb, packed, packed ? n * (n + 1) / 2 : m * n, out};
+    int r = rns_gemm_device(g);
+    if (r == SDPX_GPU_DECLINED) {
+        c.declined = true;
+        return;
+    }
+    if (r != SDPX_GPU_OK) throw Error(r, "device exact product failed");
+    c.stats->gpu_products++;
+    c.stats->gpu_mp_macs += (double)k * (packed ? n * (n + 1) / 2.0 : (double)m * n);
+}
+
+// ldl_dots of one block on the GPU; returns 1 ok, 0 failed pivot.
+int dots_device(DCtx &c, uint8_t *l, size_t ld, size_t n, const std::vector<size_t> &first,
+                size_t sign_off, uint8_t *dinv) {
+    size_t mark = c.arena->top;
+    int32_t *d_first = upload_ints(c, first);
+    LdlBlockDesc bd{(int64_t)0, (int64_t)ld, (int64_t)n, 0, 0, (int64_t)sign_off, 0};
+    LdlBlockDesc *d_bd = (LdlBlockDesc *)c.arena->alloc(sizeof(LdlBlockDesc));
+    HIP_CHECK(hipMemcpyAsync(d_bd, &bd, sizeof bd, hipMemcpyHostToDevice, c.st));
+    const size_t need = (n * (n + 1) + n) * c.vb;
+    if (need > c.ldl_scratch_bytes) throw Error(SDPX_GPU_DEVICE_ERROR, "ldl scratch too small");
+    HIP_CHECK(hipMemsetAsync(c.d_status, 0, sizeof(int), c.st));
+    mp_ldl_dots(c.n, l, dinv, d_first, c.d_signs, c.sign_mode, d_bd, 1, c.ldl_scratch, c.d_reg_eps,
+                c.d_reg_delta, c.d_status, c.st);
+    int status[2] = {0, 0};
+    HIP_CHECK(hipMemcpyAsync(status, c.d_status, sizeof(int) * 2, hipMemcpyDeviceToHost, c.st));
+    HIP_CHECK(hipStreamSynchronize(c.st));
+    c.arena->top = mark;
+    if (status[0] == 2) {
+        c.declined = true;  // window overflow: host path
+        return 1;
+    }
+    return status[0] == 0 ? 1 : 0;
+}
+
+void solve_t_device(DCtx &c, size_t m, size_t n, uint8_t *L, size_t ldl, uint8_t *B, size_t ldb,
+                    const std::vector<size_t> &low) {
+    if (m == 0 || n == 0 || c.declined) return;
+    if (n <= 128) {
+        size_t mark = c.arena->top;
+        int32_t *d_low = upload_ints(c, low);
+        mp_trsm_base(c.n, L, ldl, n, B, ldb, m, d_low, c.d_status + 2, c.st);
+        HIP_CHECK(hipStreamSynchronize(c.st));
+        c.arena->top = mark;
+        c.stats->base_rows += m;
+        return;
+    }
+    const size_t n1 = n / 2, n2 = n - n / 2;
+    std::vector<size_t> low1(m), low2(m);
+    for (size_t i = 0; i < m; i++) {
+        low1[i] = std::min(low[i], n1);
+        low2[i] = low[i] > n1 ? low[i] - n1 : 0;
+    }
+    solve_t_device(c, m, n1, L, ldl, B, ldb, low1);
+    size_t mark = c.arena->top;
+    uint8_t *p = (uint8_t *)c.arena->alloc(m * n2 * c.vb);
+    product(c, 'N', 'T', m, n2, n1, B, ldb, c.at(L, n1), ldl, p, false);
+    if (c.declined) return;
+    uint8_t *B2 = c.at(B, n1 * ldb);
+    mp_ew_sub_rect(c.n, B2, ldb, p, m, m, n2, c.st);
+    HIP_CHECK(hipStreamSynchronize(c.st));
+    c.arena->top = mark;
+    solve_t_device(c, m, n2, c.at(L, n1 + n1 * ldl), ldl, B2, ldb, low2);
+}
+
+int rec_device(DCtx &c, uint8_t *l, size_t ld, size_t n, const std::vector<size_t> &first,
+               size_t sign_off, uint8_t *dinv) {
+    if (c.declined) return 1;
+    if (n <= 128) return dots_device(c, l, ld, n, first, sign_off, dinv);
+    size_t low = n, cut = 0;
+    bool have = false;
+    for (size_t cc = n - 1; cc >= 1; cc--) {
+        low = std::min(low, first[cc]);
+        auto dist = [&](size_t x) { return x > n / 2 ? x - n / 2 : n / 2 - x; };
+        if (low >= cc && (!have || dist(cc) < dist(cut))) {
+            cut = cc;
+            have = true;
+        }
+    }
+    if (have) {
+        std::vector<size_t> f1(first.begin(), first.begin() + cut), tail(n - cut);
+        for (size_t i = cut; i < n; i++) tail[i - cut] = first[i] - cut;
+        int ok1 = rec_device(c, l, ld, cut, f1, sign_off, dinv);
+        int ok2 = rec_device(c, c.at(l, cut + cut * ld), ld, n - cut, tail, sign_off + cut,
+                             c.at(dinv, cut));
+        return ok1 && ok2;
+    }
+    const size_t n1 = n / 2, n2 = n - n / 2;
+    std::vector<size_t> f1(first.begin(), first.begin() + n1);
+    if (!rec_device(c, l, ld, n1, f1, sign_off, dinv)) return 0;
+    if (c.declined) return 1;
+    size_t mark = c.arena->top;
+    uint8_t *l11 = (uint8_t *)c.arena->alloc(n1 * n1 * c.vb);
+    HIP_CHECK(hipMemcpy2DAsync(l11, n1 * c.vb, l, ld * c.vb, n1 * c.vb, n1, hipMemcpyDeviceToDevice, c.st));
+    std::vector<size_t> lowv(n2);
+    for (size_t i = 0; i < n2; i++) lowv[i] = std::min(first[n1 + i], n1);
+    uint8_t *a21 = c.at(l, n1);
+    solve_t_device(c, n2, n1, l11, n1, a21, ld, lowv);
+    if (c.declined) return 1;
+    c.arena->top = mark;  // l11 no longer read (stream order)
+    uint8_t *w = (uint8_t *)c.arena->alloc(n2 * n1 * c.vb);
+    HIP_CHECK(hipMemcpy2DAsync(w, n2 * c.vb, a21, ld * c.vb, n2 * c.vb, n1, hipMemcpyDeviceToDevice, c.st));
+    mp_ew_mul_diag(c.n, a21, ld, dinv, a21, ld, n2, n1, false, c.st);
+    uint8_t *packed = (uint8_t *)c.arena->alloc(n2 * (n2 + 1) / 2 * c.vb);
+    product(c, 'N', 'T', n2, n2, n1, w, n2, a21, ld, packed, true);
+    if (c.declined) return 1;
+    uint8_t *a22 = c.at(l, n1 + n1 * ld);
+    mp_ew_sub_packed(c.n, a22, ld, packed, n2, 1, c.st);
+    HIP_CHECK(hipStreamSynchronize(c.st));
+    c.arena->top = mark;
+    std::vector<size_t> tail(n2);
+    for (size_t i = 0; i < n2; i++) tail[i] = first[n1 + i] > n1 ? first[n1 + i] - n1 : 0;
+    return rec_device(c, a22, ld, n2, tail, sign_off + n1, c.at(dinv, n1));
+}
+
+std::vector<size_t> envelope_device(DCtx &c, const uint8_t *l, size_t ld, size_t n) {
+    size_t mark = c.arena->top;
+    int32_t *d = (int32_t *)c.arena->alloc(sizeof(int32_t) * std::max<size_t>(n, 1));
+    mp_envelope(c.n, l, ld, n, d, c.st);
+    std::vector<int32_t> h(n);
+    if (n) HIP_CHECK(hipMemcpyAsync(h.data(), d, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c.st));
+    HIP_CHECK(hipStreamSynchronize(c.st));
+    c.arena->top = mark;
+    return std::vector<size_t>(h.begin(), h.end());
+}
+
+struct DeviceSetup {
+    Device &dev;
+    Arena arena;
+    DCtx ctx;
+    DeviceSetup(int limbs, size_t arena_bytes, size_t scratch_bytes, const int8_t *signs_host,
+                size_t nsigns, const void *reg_eps_host, const void *reg_delta_host, LdlStats *st)
+        : dev(device()) {
+        const size_t vb = value_bytes(limbs);
+        arena.base = (uint8_t *)dev.ws.get(70, arena_bytes);
+        arena.size = arena_bytes;
+        arena.top = 0;
+        ctx.n = limbs;
+        ctx.vb = vb;
+        ctx.st = dev.stream;
+        ctx.arena = &arena;
+        ctx.stats = st;
+        ctx.ldl_scratch = (uint8_t *)dev.ws.get(71, scratch_bytes);
+        ctx.ldl_scratch_bytes = scratch_bytes;
+        // constants: status, signs, regularization values
+        ctx.d_status = (int *)arena.alloc(sizeof(int) * 4);
+        HIP_CHECK(hipMemsetAsync(ctx.d_status, 0, sizeof(int) * 4, ctx.st));
+        ctx.sign_mode = nsigns == 0 ? 0 : (nsigns == 1 ? 1 : 2);
+        int8_t *ds = (int8_t *)arena.alloc(std::max<size_t>(nsigns, 1));
+        if (nsigns) HIP_CHECK(hipMemcpyAsync(ds, signs_host, nsigns, hipMemcpyHostToDevice, ctx.st));
+        ctx.d_signs = ds;
+        ctx.d_reg_eps = ctx.d_reg_delta = nullptr;
+        if (reg_eps_host) {
+            void *e = arena.alloc(vb), *d = arena.alloc(vb);
+            HIP_CHECK(hipMemcpyAsync(e, reg_eps_host, vb, hipMemcpyHostToDevice, ctx.st));
+            HIP_CHECK(hipMemcpyAsync(d, reg_delta_host, vb, hipMemcpyHostToDevice, ctx.st));
+            ctx.d_reg_eps = e;
+            ctx.d_reg_delta = d;
+        }
+        HIP_CHECK(hipStreamSynchronize(ctx.st));
+    }
+    int regularized() {
+        int s[4];
+        HIP_CHECK(hipMemcpyAsync(s, ctx.d_status, sizeof s, hipMemcpyDeviceToHost, ctx.st));
+        HIP_CHECK(hipStreamSynchronize(ctx.st));
+        return s[1];
+    }
+    bool window_overflow() {
+        int s[4];
+        HIP_CHECK(hipMemcpyAsync(s, ctx.d_status, sizeof s, hipMemcpyDeviceToHost, ctx.st));
+        HIP_CHECK(hipStreamSynchronize(ctx.st));
+        return s[2] != 0;
+    }
+};
+
+size_t ldl_arena_bytes(size_t n, size_t vb) {
+    // l11 or w copy plus the packed product, nested over halving levels
+    return (size_t)(2.5 * (double)n * n * vb) + (64u << 20);
+}
+
+}  // namespace
+
+int ldl_factor_device(int limbs, uint8_t *d_l, size_t ld, size_t n, const int8_t *signs,
+                      size_t nsigns, const void *reg_eps, const void *reg_delta, uint8_t *d_dinv,
+                      LdlStats &stats, bool recursive) {
+    const size_t vb = value_bytes(limbs);
+    const size_t blk = recursive ? std::min<size_t>(n, 128) : n;
+    DeviceSetup ds(limbs, ldl_arena_bytes(n, vb), (blk * (blk + 1) + blk) * vb + 4096, signs, nsigns,
+                   reg_eps, reg_delta, &stats);
+    DCtx &c = ds.ctx;
+    std::vector<size_t> first = envelope_device(c, d_l, ld, n);
+    int ok;
+    if (recursive)
+        ok = rec_device(c, d_l, ld, n, first, 0, d_dinv);
+    else
+        ok = dots_device(c, d_l, ld, n, first, 0, d_dinv);
+    if (c.declined || ds.window_overflow()) return -1;  // caller reruns on the host
+    stats.regularized += ds.regularized();
+    return ok;
+}
+
+int sampled_arrow_device(int limbs, const std::vector<SampledLeafDev> &leaves, size_t t,
+                         uint8_t *d_l_all, uint8_t *d_dinv_all, uint8_t *d_y_all, uint8_t *d_border,
+                         const int8_t *border_signs, size_t nborder_signs, uint8_t *d_border_dinv,
+                         SampledStats &st) {
+    const size_t vb = value_bytes(limbs);
+    size_t kmax = 0, hmax = 0, ksum = 0;
+    std::vector<size_t> loff, doff, yoff;
+    for (auto &l : leaves) {
+        kmax = std::max(kmax, l.k);
+        hmax = std::max(hmax, l.h);
+        loff.push_back(loff.empty() ? 0 : loff.back() + leaves[loff.size() - 1].k * leaves[loff.size() - 1].k);
+        doff.push_back(ksum);
+        yoff.push_back(ksum * t);
+        ksum += l.k;
+    }
+    size_t arena = ldl_arena_bytes(t, vb) + (hmax * kmax + kmax * (kmax + 1) / 2 + kmax * t) * vb +
+                   t * (t + 1) / 2 * vb + (64u << 20);
+    size_t per_leaf = (kmax * (kmax + 1) + kmax);
+    size_t scratch = std::max<size_t>(leaves.size() * per_leaf * vb, (128 * 129 + 128) * vb) + 4096;
+    LdlStats ls;
+    DeviceSetup ds(limbs, arena, scratch, border_signs, nborder_signs, nullptr, nullptr, &ls);
+    DCtx &c = ds.ctx;
+    hipStream_t s = c.st;
+    hipEvent_t ev[4];
+    for (auto &e : ev) HIP_CHECK(hipEventCreate(&e));
+    HIP_CHECK(hipEventRecord(ev[0], s));
+    // 1. Schur build of every leaf: V = Rinv Q, Gram = V^T V, entries
+    for (size_t i = 0; i < leaves.size() && !c.declined; i++) {
+        const auto &lf = leaves[i];
+        size_t mark = c.arena->top;
+        uint8_t *V = (uint8_t *)c.arena->alloc(lf.h * lf.k * vb);
+        uint8_t *G = (uint8_t *)c.arena->alloc(lf.k * (lf.k + 1) / 2 * vb);
+        product(c, 'N', 'N', lf.h, lf.k, lf.h, lf.rinv, lf.h, lf.q, lf.h, V, false);
+        product(c, 'T', 'N', lf.k, lf.k, lf.h, V, lf.h, V, lf.h, G, true);
+        if (c.declined) break;
+        mp_sampled_entries(limbs, G, lf.w, (int64_t)lf.k, c.at(d_l_all, loff[i]), s);
+        HIP_CHECK(hipStreamSynchronize(s));
+        c.arena->top = mark;
+    }
+    HIP_CHECK(hipEventRecord(ev[1], s));
+    if (!c.declined) {
+        // 2. every leaf factor in one launch, one workgroup per leaf
+        std::vector<size_t> first_cat;
+        std::vector<std::vector<size_t>> fl;
+        std::vector<LdlBlockDesc> bds;
+        for (size_t i = 0; i < leaves.size(); i++) {
+            const size_t K = leaves[i].k;
+            fl.push_back(envelope_device(c, c.at(d_l_all, loff[i]), K, K));
+            bds.push_back(LdlBlockDesc{(int64_t)loff[i], (int64_t)K, (int64_t)K, (int64_t)doff[i],
+                                       (int64_t)first_cat.size(), 0, (int64_t)(i * per_leaf)});
+            first_cat.insert(first_cat.end(), fl.back().begin(), fl.back().end());
+        }
+        size_t mark = c.arena->top;
+        int32_t *d_first = upload_ints(c, first_cat);
+        LdlBlockDesc *d_bd = (LdlBlockDesc *)c.arena->alloc(sizeof(LdlBlockDesc) * bds.size());
+        HIP_CHECK(hipMemcpyAsync(d_bd, bds.data(), sizeof(LdlBlockDesc) * bds.size(),
+                                 hipMemcpyHostToDevice, s));
+        int8_t *plus = (int8_t *)c.arena->alloc(1);
+        const int8_t one = 1;
+        HIP_CHECK(hipMemcpyAsync(plus, &one, 1, hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipMemsetAsync(c.d_status, 0, sizeof(int) * 2, s));
+        mp_ldl_dots(limbs, d_l_all, d_dinv_all, d_first, plus, 1, d_bd, (int)bds.size(),
+                    c.ldl_scratch, nullptr, nullptr, c.d_status, s);
+        int status[2];
+        HIP_CHECK(hipMemcpyAsync(status, c.d_status, sizeof status, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        c.arena->top = mark;
+        if (status[0] == 2) c.declined = true;
+        if (status[0] == 1) {
+            st.failed_leaf = 0;
+            for (auto &e : ev) (void)hipEventDestroy(e);
+            return 0;
+        }
+        // 3. Y, Z, contribution and border update, in leaf order
+        for (size_t i = 0; i < leaves.size() && !c.declined; i++) {
+            const auto &lf = leaves[i];
+            size_t m2 = c.arena->top;
+            int32_t *d_f = upload_ints(c, fl[i]);
+            uint8_t *Y = c.at(d_y_all, yoff[i]);
+            HIP_CHECK(hipMemcpyAsync(Y, lf.b, lf.k * t * vb, hipMemcpyDeviceToDevice, s));
+            mp_forward_rows(limbs, c.at(d_l_all, loff[i]), d_f, (int64_t)lf.k, Y, (int64_t)t,
+                            c.d_status + 2, s);
+            uint8_t *Z = (uint8_t *)c.arena->alloc(lf.k * t * vb);
+            mp_ew_mul_diag(limbs, Y, lf.k, c.at(d_dinv_all, doff[i]), Z, lf.k, lf.k, t, true, s);
+            uint8_t *P = (uint8_t *)c.arena->alloc(t * (t + 1) / 2 * vb);
+            product(c, 'T', 'N', t, t, lf.k, Y, lf.k, Z, lf.k, P, true);
+            if (c.declined) break;
+            mp_ew_sub_packed(limbs, d_border, t, P, t, 0, s);
+            HIP_CHECK(hipStreamSynchronize(s));
+            c.arena->top = m2;
+        }
+    }
+    HIP_CHECK(hipEventRecord(ev[2], s));
+    int ok = 1;
+    if (!c.declined && !ds.window_overflow()) {
+        mp_mirror_upper(limbs, d_border, t, t, s);
+        std::vector<size_t> first = envelope_device(c, d_border, t, t);
+        ok = t >= 256 ? rec_device(c, d_border, t, t, first, 0, d_border_dinv)
+                      : dots_device(c, d_border, t, t, first, 0, d_border_dinv);
+    }
+    HIP_CHECK(hipEventRecord(ev[3], s));
+    HIP_CHECK(hipEventSynchronize(ev[3]));
+    float a = 0, b = 0, d = 0;
+    HIP_CHECK(hipEventElapsedTime(&a, ev[0], ev[1]));
+    HIP_CHECK(hipEventElapsedTime(&b, ev[1], ev[2]));
+    HIP_CHECK(hipEventElapsedTime(&d, ev[2], ev[3]));
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    st.build_s = a / 1e3;
+    st.leaves_s = b / 1e3;
+    st.border_s = d / 1e3;
+    st.gpu_products += ls.gpu_products;
+    if (c.declined || ds.window_overflow()) return -1;
+    return ok;
+}
+
+}  // namespace sdpx

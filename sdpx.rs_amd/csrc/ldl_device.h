@@ -1,0 +1,30 @@
+// Device-resident multiple-precision LDL^T and sampled arrow factor (see
+// ldl_device.cpp).  All pointers are device pointers.
+#pragma once
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+#include "ldl.h"
+#include "sampled.h"
+
+namespace sdpx {
+
+struct SampledLeafDev {
+    size_t h = 0, k = 0;
+    const uint8_t *rinv = nullptr, *q = nullptr, *w = nullptr, *b = nullptr;
+};
+
+// 1 ok, 0 failed pivot, -1 not handled on the GPU (rerun on the host)
+int ldl_factor_device(int limbs, uint8_t *d_l, size_t ld, size_t n, const int8_t *signs,
+                      size_t nsigns, const void *reg_eps, const void *reg_delta, uint8_t *d_dinv,
+                      LdlStats &stats, bool recursive);
+
+// leaf factors / D^-1 / Y land in contiguous arrays in leaf order
+// (sum k^2, sum k, sum k*t values)
+int sampled_arrow_device(int limbs, const std::vector<SampledLeafDev> &leaves, size_t t,
+                         uint8_t *d_l_all, uint8_t *d_dinv_all, uint8_t *d_y_all, uint8_t *d_border,
+                         const int8_t *border_signs, size_t nborder_signs, uint8_t *d_border_dinv,
+                         SampledStats &st);
+
+}  // namespace sdpx

@@ -1,0 +1,34 @@
+// GPU panel kernels of the multiple-precision LDL^T (see mp_kernels.hip).
+// All pointers are device pointers to value arrays; n-limb values.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace sdpx {
+
+struct LdlBlockDesc {  // mirrors the kernel's LdlBlock
+    int64_t a_off, ld, n, dinv_off, first_off, sign_off, scratch_off;
+};
+
+bool mp_kernels_support(int limbs);
+void mp_ew_sub_packed(int limbs, void *S, int64_t ld, const void *P, int64_t n, int lower,
+                      hipStream_t st);
+void mp_ew_sub_rect(int limbs, void *S, int64_t lds, const void *P, int64_t ldp, int64_t m,
+                    int64_t n, hipStream_t st);
+void mp_ew_mul_diag(int limbs, const void *Y, int64_t ldy, const void *D, void *Z, int64_t ldz,
+                    int64_t m, int64_t n, bool per_row, hipStream_t st);
+void mp_sampled_entries(int limbs, const void *G, const void *w, int64_t K, void *H,
+                        hipStream_t st);
+void mp_forward_rows(int limbs, const void *L, const int32_t *first, int64_t K, void *Y, int64_t t,
+                     int *fail, hipStream_t st);
+void mp_trsm_base(int limbs, const void *L, int64_t ldl, int64_t n, void *B, int64_t ldb,
+                  int64_t m, const int32_t *low, int *fail, hipStream_t st);
+// status[0] = 0 ok, 1 failed pivot, 2 window overflow; status[1] += regularized
+void mp_ldl_dots(int limbs, void *A, void *dinv, const int32_t *first, const int8_t *signs,
+                 int sign_mode, const void *blocks, int nblocks, void *scratch, const void *reg_eps,
+                 const void *reg_delta, int *status, hipStream_t st);
+void mp_envelope(int limbs, const void *A, int64_t ld, int64_t n, int32_t *first, hipStream_t st);
+void mp_mirror_upper(int limbs, void *S, int64_t n, int64_t ld, hipStream_t st);
+
+}  // namespace sdpx

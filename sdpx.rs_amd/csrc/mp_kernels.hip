@@ -1,0 +1,479 @@
+// GPU panel kernels of the multiple-precision LDL^T and sampled arrow
+// factor (arithmetic in mpdev.h).  Each kernel computes exactly the
+// roundings of the corresponding host step (ldl.cpp / sampled.cpp), so
+// moving a step to the GPU never changes a bit:
+//   ew_sub        S = RN(S - P) over a packed upper triangle
+//                 (subtract_contributions, arrow.rs:1975-2074; A22 update,
+//                 arrow.rs:376-392)
+//   ew_mul_diag   Z = RN(Y D) with D per row or column (contribution_product
+//                 Z = D^-1 Y, arrow.rs:1940-1956; L21 = W21 D1^-1, 347-351)
+//   forward_rows  Y = L^-1 B, y_i = RN(b_i - sum_k L_ik y_k) with one
+//                 rounding (forward_suffix_rows, arrow.rs:1000-1019)
+//   trsm_base     X L^T = B rows, x_j = RN(x_j - RN(dot)) (solve_unit_lower_t
+//                 base case, arrow.rs:425-447)
+//   ldl_dots      left-looking LDL^T of a block (arrow.rs:156-253), one
+//                 workgroup per block
+//   sampled_entries  H_pq = RN(RN(w_p w_q) RN(K_pq^2)) (entry,
+//                 sampled/mod.rs:1973-1988, whose halving and doubling are exact)
+// Exact dots use a slot window in LDS; a dot whose exponent spread exceeds
+// the window raises a flag and the host recomputes the affected output.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+
+#include "common.h"
+#include "mp_kernels.h"
+#include "mpdev.h"
+
+namespace sdpx {
+
+using namespace mpd;
+
+template <int W>
+__device__ __forceinline__ const uint32_t *rec_at(const uint32_t *base, int64_t idx) {
+    return base + idx * (W + 2);
+}
+template <int W>
+__device__ __forceinline__ uint32_t *rec_at(uint32_t *base, int64_t idx) {
+    return base + idx * (W + 2);
+}
+
+// ---------------------------------------------------------------------------
+// elementwise
+// ---------------------------------------------------------------------------
+// mode 0: S(i, j) (i <= j, upper, ld) -= P[j(j+1)/2 + i]
+// mode 1: S(j, i) (lower mirror, b = j >= a = i) -= P[j(j+1)/2 + i]
+template <int W>
+__global__ void ew_sub_packed_kernel(uint32_t *S, int64_t ld, const uint32_t *P, int64_t n,
+                                     int mode) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = n * (n + 1) / 2;
+    if (e >= total) return;
+    int64_t j = (int64_t)((sqrt(8.0 * (double)e + 1.0) - 1.0) / 2.0);
+    while (j * (j + 1) / 2 > e) j--;
+    while ((j + 1) * (j + 2) / 2 <= e) j++;
+    const int64_t i = e - j * (j + 1) / 2;
+    const int64_t o = mode == 0 ? i + j * ld : j + i * ld;
+    Val<W> s, p, r;
+    load(rec_at<W>(S, o), s);
+    load(rec_at<W>(P, e), p);
+    add_signed(s, p, true, r);
+    store(rec_at<W>(S, o), r);
+}
+
+// Z[i + j*ldz] = RN(Y[i + j*ldy] * D[per_row ? i : j]) for an m x n block
+template <int W>
+__global__ void ew_mul_diag_kernel(const uint32_t *Y, int64_t ldy, const uint32_t *D,
+                                   uint32_t *Z, int64_t ldz, int64_t m, int64_t n, int per_row) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m * n) return;
+    const int64_t i = e % m, j = e / m;
+    Val<W> y, d, r;
+    load(rec_at<W>(Y, i + j * ldy), y);
+    load(rec_at<W>(D, per_row ? i : j), d);
+    mul(y, d, r);
+    store(rec_at<W>(Z, i + j * ldz), r);
+}
+
+// H (K x K lower, ld K) from the packed-upper Gram and the weights
+template <int W>
+__global__ void sampled_entries_kernel(const uint32_t *G, const uint32_t *w, int64_t K,
+                                       uint32_t *H) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= K * K) return;
+    const int64_t p = e % K, q = e / K;
+    Val<W> r;
+    if (p < q) {
+        set_zero(r);
+        store(rec_at<W>(H, e), r);
+        return;
+    }
+    Val<W> wp, wq, g, t1, u1;
+    load(rec_at<W>(w, p), wp);
+    load(rec_at<W>(w, q), wq);
+    load(rec_at<W>(G, p * (p + 1) / 2 + q), g);
+    mul(wp, wq, t1);
+    mul(g, g, u1);
+    // RN(RN(t1 / 2) RN(u1 + u1)) = RN(t1 u1): both steps are exact scalings
+    // by powers of two (entry(), sampled/mod.rs:1973-1988)
+    mul(t1, u1, r);
+    store(rec_at<W>(H, e), r);
+}
+
+// ---------------------------------------------------------------------------
+// dots
+// ---------------------------------------------------------------------------
+// Window of SLOTS 32-bit positions: products of 2W words plus 16 words of
+// exponent spread and carry headroom.
+template <int W>
+struct Win {
+    static constexpr int SLOTS = 2 * W + 20;
+};
+
+// exponent scale range of the products x_k y_k (zeros skipped)
+template <int W, class FX, class FY>
+__device__ inline bool scale_range(int64_t k, FX fx, FY fy, int64_t &lo, int64_t &hi) {
+    lo = INT64_MAX;
+    hi = INT64_MIN;
+    for (int64_t t = 0; t < k; t++) {
+        const uint32_t *x = fx(t), *y = fy(t);
+        const int32_t kx = (int32_t)x[W], ky = (int32_t)y[W];
+        if (kx == mpd::KIND_ZERO || kx == -mpd::KIND_ZERO || ky == mpd::KIND_ZERO || ky == -mpd::KIND_ZERO) continue;
+        const int64_t s = (int64_t)(int32_t)x[W + 1] + (int32_t)y[W + 1] - 64 * (int64_t)W;
+        lo = min(lo, s);
+        hi = max(hi, s);
+    }
+    return lo != INT64_MAX;
+}
+
+// RN(c - sum x_t y_t) (with_c) or RN(sum x_t y_t), one rounding; false when
+// the spread does not fit the window.
+template <int W, class FX, class FY>
+__device__ inline bool window_dot(int64_t k, FX fx, FY fy, const Val<W> *c, bool negate_terms,
+                                  int64_t *slots, int stride, Val<W> &out) {
+    constexpr int SLOTS = Win<W>::SLOTS;
+    int64_t lo, hi;
+    bool any = scale_range<W>(k, fx, fy, lo, hi);
+    const bool cnz = c && !is_zero(*c);
+    if (cnz) {
+        const int64_t s = (int64_t)c->e - 32 * W;
+        lo = any ? min(lo, s) : s;
+        hi = any ? max(hi, s - 32 * W) : s - 32 * W;
+        any = true;
+    }
+    if (!any) {
+        set_zero(out);
+        return true;
+    }
+    if ((hi - lo) / 32 + 2 * W + 4 > SLOTS) return false;
+    DotWindow<W, SLOTS> dw{slots, stride, lo};
+    dw.clear();
+    for (int64_t t = 0; t < k; t++) {
+        Val<W> x, y;
+        load(fx(t), x);
+        load(fy(t), y);
+        if (is_zero(x) || is_zero(y)) continue;
+        dw.add_product(x, y, negate_terms);
+    }
+    if (cnz) dw.add_value(*c, false);
+    dw.finish(out);
+    return true;
+}
+
+// Y = L^-1 B per column (forward_suffix_rows): L K x K lower (ld K),
+// first[K] the row envelope, Y K x t (ld K) in place.  One thread per column.
+template <int W>
+__global__ __launch_bounds__(128) void forward_rows_kernel(const uint32_t *L, const int32_t *first,
+                                                           int64_t K, uint32_t *Y, int64_t t,
+                                                           int *fail) {
+    extern __shared__ int64_t slots_sh[];
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= t) return;
+    int64_t *slots = slots_sh + threadIdx.x;
+    const int stride = blockDim.x;
+    uint32_t *col = Y + c * K * (W + 2);
+    for (int64_t i = 0; i < K; i++) {
+        const int64_t lo = first[i];
+        Val<W> b, r;
+        load(rec_at<W>(col, i), b);
+        // y_i = RN(b_i - sum_{k in [lo, i)} L_ik y_k)
+        auto fx = [&](int64_t q) { return rec_at<W>(L, i + (lo + q) * K); };
+        auto fy = [&](int64_t q) { return rec_at<W>((const uint32_t *)col, lo + q); };
+        if (!window_dot<W>(i - lo, fx, fy, &b, true, slots, stride, r)) {
+            atomicOr(fail, 1);
+            return;
+        }
+        store(rec_at<W>(col, i), r);
+    }
+}
+
+// X L^T = B rows (solve_unit_lower_t base): B m x n (ldb), L n x n (ldl,
+// unit lower, diagonal unread), low[m]; one thread per row.
+template <int W>
+__global__ __launch_bounds__(128) void trsm_base_kernel(const uint32_t *L, int64_t ldl, int64_t n,
+                                                        uint32_t *B, int64_t ldb, int64_t m,
+                                                        const int32_t *low, int *fail) {
+    extern __shared__ int64_t slots_sh[];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    int64_t *slots = slots_sh + threadIdx.x;
+    const int stride = blockDim.x;
+    const int64_t lo = low[i];
+    for (int64_t j = lo; j < n; j++) {
+        auto fx = [&](int64_t q) { return rec_at<W>((const uint32_t *)B, i + (lo + q) * ldb); };
+        auto fy = [&](int64_t q) { return rec_at<W>(L, j + (lo + q) * ldl); };
+        Val<W> t, x, r;
+        if (!window_dot<W>(j - lo, fx, fy, nullptr, false, slots, stride, t)) {
+            atomicOr(fail, 1);
+            return;
+        }
+        load(rec_at<W>(B, i + j * ldb), x);
+        add_signed(x, t, true, r);
+        store(rec_at<W>(B, i + j * ldb), r);
+    }
+}
+
+// ldl_dots on one block per workgroup.  Blocks: (offset into A, ld, n,
+// offset into dinv, first[] offset, sign offset); r and w are per-block
+// packed-lower scratch (n(n+1)/2 values each, at scratch + 2*tri(n)*block).
+struct LdlBlock {
+    int64_t a_off, ld, n, dinv_off, first_off, sign_off, scratch_off;
+};
+
+template <int W>
+__global__ __launch_bounds__(128) void ldl_dots_kernel(uint32_t *A, uint32_t *dinv,
+                                                       const int32_t *first_all,
+                                                       const int8_t *signs, int sign_mode,
+                                                       const LdlBlock *blocks, uint32_t *scratch,
+                                                       const uint32_t *reg_eps,
+                                                       const uint32_t *reg_delta, int *status) {
+    extern __shared__ int64_t slots_sh[];
+    __shared__ int bad;
+    const LdlBlock B = blocks[blockIdx.x];
+    uint32_t *a = A + B.a_off * (W + 2);
+    uint32_t *dv = dinv + B.dinv_off * (W + 2);
+    const int32_t *first = first_all + B.first_off;
+    const int64_t n = B.n, ld = B.ld;
+    auto tri = [](int64_t i) { return i * (i + 1) / 2; };
+    uint32_t *r = scratch + B.scratch_off * (W + 2);
+    uint32_t *w = r + tri(n) * (W + 2);
+    uint32_t *dstore = w + tri(n) * (W + 2);  // n pivots
+    int64_t *slots = slots_sh + threadIdx.x;
+    const int stride = blockDim.x;
+    if (threadIdx.x == 0) bad = 0;
+    __syncthreads();
+    for (int64_t j = 0; j < n; j++) {
+        if (threadIdx.x == 0) {
+            const int64_t fj = first[j];
+            auto fx = [&](int64_t q) { return rec_at<W>((const uint32_t *)r, tri(j) + fj + q); };
+            auto fy = [&](int64_t q) { return rec_at<W>((const uint32_t *)w, tri(j) + fj + q); };
+            Val<W> ajj, t, d;
+            load(rec_at<W>(a, j + j * ld), ajj);
+            bool ok = window_dot<W>(j - fj, fx, fy, nullptr, false, slots, stride, t);
+            if (ok) {
+                add_signed(ajj, t, true, d);
+                if (is_zero(d) && j > fj) ok = window_dot<W>(j - fj, fx, fy, &ajj, true, slots, stride, d);
+            }
+            if (!ok) {
+                bad = 2;  // window overflow: host path
+            } else {
+                // DenseLeaf::pivot
+                const int sg = sign_mode == 0 ? 1 : (sign_mode == 1 ? signs[0] : signs[B.sign_off + j]);
+                if (!is_regular(d) && !is_zero(d)) bad = 1;
+                if (!bad && reg_eps) {
+                    Val<W> ds = d, e, diff;
+                    if (sg < 0 && is_regular(ds)) ds.kind = -ds.kind;
+                    load(reg_eps, e);
+                    add_signed(ds, e, true, diff);
+                    if (!is_zero(diff) && diff.kind < 0) {
+                        load(reg_delta, d);
+                        if (sg < 0 && is_regular(d)) d.kind = -d.kind;
+                        atomicAdd(status + 1, 1);  // regularized count
+                    }
+                }
+                if (!bad && is_zero(d)) bad = 1;
+                if (!bad) {
+                    store(rec_at<W>(dstore, j), d);
+                    Val<W> di;
+                    recip(d, di);
+                    store(rec_at<W>(dv, j), di);
+                }
+            }
+        }
+        __syncthreads();
+        if (bad) break;
+        Val<W> dj, dij;
+        load(rec_at<W>((const uint32_t *)dstore, j), dj);
+        load(rec_at<W>((const uint32_t *)dv, j), dij);
+        for (int64_t i = j + 1 + threadIdx.x; i < n; i += blockDim.x) {
+            Val<W> v;
+            if (first[i] > j) {
+                set_zero(v);
+            } else {
+                const int64_t lo = max(first[i], first[j]);
+                auto fx = [&](int64_t q) { return rec_at<W>((const uint32_t *)r, tri(i) + lo + q); };
+                auto fy = [&](int64_t q) { return rec_at<W>((const uint32_t *)w, tri(j) + lo + q); };
+                Val<W> t, aij, diff;
+                if (!window_dot<W>(j - lo, fx, fy, nullptr, false, slots, stride, t)) {
+                    atomicMax(&bad, 2);
+                    continue;
+                }
+                load(rec_at<W>(a, i + j * ld), aij);
+                add_signed(aij, t, true, diff);
+                mul(diff, dij, v);
+            }
+            store(rec_at<W>(r, tri(i) + j), v);
+            Val<W> wv;
+            mul(v, dj, wv);
+            store(rec_at<W>(w, tri(i) + j), wv);
+        }
+        __syncthreads();
+        if (bad) break;
+    }
+    if (bad) {
+        if (threadIdx.x == 0) atomicMax(status, bad);
+        return;
+    }
+    // write the unit lower factor
+    for (int64_t e = threadIdx.x; e < n * n; e += blockDim.x) {
+        const int64_t i = e % n, j = e / n;
+        if (i < j) continue;
+        Val<W> v;
+        if (i == j) {
+            for (int u = 0; u < W; u++) v.m[u] = 0;
+            v.m[W - 1] = 0x80000000u;
+            v.kind = mpd::KIND_REG;
+            v.e = 1;
+        } else {
+            load(rec_at<W>((const uint32_t *)r, tri(i) + j), v);
+        }
+        store(rec_at<W>(a, i + j * ld), v);
+    }
+}
+
+// S[i + j*lds] = RN(S - P[i + j*ldp]) for an m x n block
+template <int W>
+__global__ void ew_sub_rect_kernel(uint32_t *S, int64_t lds, const uint32_t *P, int64_t ldp,
+                                   int64_t m, int64_t n) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m * n) return;
+    const int64_t i = e % m, j = e / m;
+    Val<W> a, b, r;
+    load(rec_at<W>((const uint32_t *)S, i + j * lds), a);
+    load(rec_at<W>(P, i + j * ldp), b);
+    add_signed(a, b, true, r);
+    store(rec_at<W>(S, i + j * lds), r);
+}
+
+// first nonzero column of each row of the lower triangle
+template <int W>
+__global__ void envelope_kernel(const uint32_t *A, int64_t ld, int64_t n, int32_t *first) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int64_t k = 0;
+    while (k < i) {
+        const int32_t kind = (int32_t)rec_at<W>(A, i + k * ld)[W];
+        if (kind != mpd::KIND_ZERO && kind != -mpd::KIND_ZERO) break;
+        k++;
+    }
+    first[i] = (int32_t)k;
+}
+
+// mirror the upper triangle onto the lower one (values)
+template <int W>
+__global__ void mirror_upper_kernel(uint32_t *S, int64_t n, int64_t ld) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * n) return;
+    const int64_t i = e % n, j = e / n;
+    if (i <= j) return;
+    const uint32_t *src = rec_at<W>((const uint32_t *)S, j + i * ld);
+    uint32_t *dst = rec_at<W>(S, i + j * ld);
+    for (int u = 0; u < W + 2; u++) dst[u] = src[u];
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+namespace {
+template <int W>
+size_t slot_bytes(int threads) {
+    return (size_t)Win<W>::SLOTS * threads * sizeof(int64_t);
+}
+inline unsigned blocks_for(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+}  // namespace
+
+#define SDPX_DISPATCH_W(limbs, CALL)                                    \
+    switch (limbs) {                                                    \
+        case 2: { constexpr int W = 4; CALL; break; }                   \
+        case 4: { constexpr int W = 8; CALL; break; }                   \
+        case 8: { constexpr int W = 16; CALL; break; }                  \
+        default: throw Error(SDPX_GPU_INVALID, "GPU panels support 128, 256 and 512 bits"); \
+    }
+
+bool mp_kernels_support(int limbs) { return limbs == 2 || limbs == 4 || limbs == 8; }
+
+void mp_ew_sub_packed(int limbs, void *S, int64_t ld, const void *P, int64_t n, int lower,
+                      hipStream_t st) {
+    const int64_t total = n * (n + 1) / 2;
+    if (total == 0) return;
+    SDPX_DISPATCH_W(limbs, hipLaunchKernelGGL(ew_sub_packed_kernel<W>, dim3(blocks_for(total, 256)),
+                                              dim3(256), 0, st, (uint32_t *)S, ld,
+                                              (const uint32_t *)P, n, lower));
+    HIP_CHECK(hipGetLastError());
+}
+
+void mp_ew_mul_diag(int limbs, const void *Y, int64_t ldy, const void *D, void *Z, int64_t ldz,
+                    int64_t m, int64_t n, bool per_row, hipStream_t st) {
+    if (m * n == 0) return;
+    SDPX_DISPATCH_W(limbs, hipLaunchKernelGGL(ew_mul_diag_kernel<W>, dim3(blocks_for(m * n, 256)),
+                                              dim3(256), 0, st, (const uint32_t *)Y, ldy,
+                                              (const uint32_t *)D, (uint32_t *)Z, ldz, m, n,
+                                              per_row ? 1 : 0));
+    HIP_CHECK(hipGetLastError());
+}
+
+void mp_sampled_entries(int limbs, const void *G, const void *w, int64_t K, void *H,
+                        hipStream_t st) {
+    if (K == 0) return;
+    SDPX_DISPATCH_W(limbs, hipLaunchKernelGGL(sampled_entries_kernel<W>, dim3(blocks_for(K * K, 256)),
+                                              dim3(256), 0, st, (const uint32_t *)G,
+                                              (const uint32_t *)w, K, (uint32_t *)H));
+    HIP_CHECK(hipGetLastError());
+}
+
+void mp_forward_rows(int limbs, const void *L, const int32_t *first, int64_t K, void *Y, int64_t t,
+                     int *fail, hipStream_t st) {
+    if (K == 0 || t == 0) return;
+    SDPX_DISPATCH_W(limbs, hipLaunchKernelGGL(forward_rows_kernel<W>, dim3(blocks_for(t, 128)),
+                                              dim3(128), slot_bytes<W>(128), st, (const uint32_t *)L,
+                                              first, K, (uint32_t *)Y, t, fail));
+    HIP_CHECK(hipGetLastError());
+}
+
+void mp_trsm_base(int limbs, const void *L, int64_t ldl, int64_t n, void *B, int64_t ldb,
+                  int64_t m, const int32_t *low, int *fail, hipStream_t st) {
+    if (m == 0 || n == 0) return;
+    SDPX_DISPATCH_W(limbs, hipLaunchKernelGGL(trsm_base_kernel<W>, dim3(blocks_for(m, 128)),
+                                              dim3(128), slot_bytes<W>(128), st, (const uint32_t *)L,
+                                              ldl, n, (uint32_t *)B, ldb, m, low, fail));
+    HIP_CHECK(hipGetLastError());
+}
+
+void mp_ldl_dots(int limbs, void *A, void *dinv, const int32_t *first, const int8_t *signs,
+                 int sign_mode, const void *blocks, int nblocks, void *scratch, const void *reg_eps,
+                 const void *reg_delta, int *status, hipStream_t st) {
+    if (nblocks == 0) return;
+    SDPX_DISPATCH_W(limbs, hipLaunchKernelGGL(ldl_dots_kernel<W>, dim3(nblocks), dim3(128),
+                                              slot_bytes<W>(128), st, (uint32_t *)A,
+                                              (uint32_t *)dinv, first, signs, sign_mode,
+                                              (const LdlBlock *)blocks, (uint32_t *)scratch,
+                                              (const uint32_t *)reg_eps,
+                                              (const uint32_t *)reg_delta, status));
+    HIP_CHECK(hipGetLastError());
+}
+
+void mp_ew_sub_rect(int limbs, void *S, int64_t lds, const void *P, int64_t ldp, int64_t m,
+                    int64_t n, hipStream_t st) {
+    if (m * n == 0) return;
+    SDPX_DISPATCH_W(limbs, hipLaunchKernelGGL(ew_sub_rect_kernel<W>, dim3(blocks_for(m * n, 256)),
+                                              dim3(256), 0, st, (uint32_t *)S, lds,
+                                              (const uint32_t *)P, ldp, m, n));
+    HIP_CHECK(hipGetLastError());
+}
+
+void mp_envelope(int limbs, const void *A, int64_t ld, int64_t n, int32_t *first, hipStream_t st) {
+    if (n == 0) return;
+    SDPX_DISPATCH_W(limbs, hipLaunchKernelGGL(envelope_kernel<W>, dim3(blocks_for(n, 256)), dim3(256),
+                                              0, st, (const uint32_t *)A, ld, n, first));
+    HIP_CHECK(hipGetLastError());
+}
+
+void mp_mirror_upper(int limbs, void *S, int64_t n, int64_t ld, hipStream_t st) {
+    if (n == 0) return;
+    SDPX_DISPATCH_W(limbs, hipLaunchKernelGGL(mirror_upper_kernel<W>, dim3(blocks_for(n * n, 256)),
+                                              dim3(256), 0, st, (uint32_t *)S, n, ld));
+    HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace sdpx
