@@ -96,6 +96,22 @@ int32_t sdpx_gpu_rns_gemm_dev(uint32_t limbs, uint8_t transa, uint8_t transb, ui
                               uint64_t n, uint64_t k, const void *a, uint64_t lda, const void *b,
                               uint64_t ldb, uint32_t upper_only, void *out, uint64_t out_len);
 
+/* Grouped exact blocks: replaces rns_blas::gemm_blocks_upper
+ * (rns_blas.rs:2671-2922) behind XgemmScalar::xgemm_blocks_upper_exact
+ * (traits.rs:250-258): c[i + j*m] for i <= j receives the upper triangle of
+ * sum_b Z_b^T Y_b over blocks of uneven height on sorted global columns,
+ * Z_b = RN(diag(d_b) Y_b) entrywise, each entry the exact sum over all
+ * blocks rounded once.  y is column-major rows x ncols.  Returns 2 on a
+ * decline (outputs may be partially written, as the reference allows). */
+typedef struct {
+    uint64_t rows, ncols;
+    const void *y;              /* rows x ncols values */
+    const void *d;              /* rows values */
+    const uint64_t *columns;    /* ncols ascending global columns < m */
+} sdpx_gpu_exact_block;
+int32_t sdpx_gpu_rns_gemm_blocks_upper(uint32_t limbs, uint64_t m, uint64_t nblocks,
+                                       const sdpx_gpu_exact_block *blocks, void *out);
+
 /* ---- Multiple-precision dense LDL^T -----------------------------------
  *
  * sdpx_gpu_mp_ldl replaces DenseLeaf::factor_leaf / factor_loaded of the

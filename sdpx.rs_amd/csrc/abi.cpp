@@ -6,6 +6,7 @@
 #include <string>
 
 #include "common.h"
+#include "mpx.h"
 #include "rns_gemm.h"
 
 namespace {
@@ -175,6 +176,51 @@ int32_t sdpx_gpu_rns_gemm(uint32_t limbs, uint8_t ta, uint8_t tb, uint64_t m, ui
             HIP_CHECK(hipStreamSynchronize(s));
         }
         return r;
+    });
+}
+
+int32_t sdpx_gpu_rns_gemm_blocks_upper(uint32_t limbs, uint64_t m, uint64_t nblocks,
+                                       const sdpx_gpu_exact_block *blocks, void *out) {
+    return boundary([&]() -> int32_t {
+        if (limbs < 1 || limbs > (uint32_t)sdpx::MAX_LIMBS) return invalid("limbs must be 1..16");
+        if (nblocks && !blocks) return invalid("null blocks");
+        const size_t vb = sdpx::value_bytes((int)limbs);
+        uint64_t K = 0;
+        for (uint64_t b = 0; b < nblocks; b++) {
+            const auto &bl = blocks[b];
+            if (bl.rows && (!bl.y || !bl.d || (bl.ncols && !bl.columns))) return invalid("null block array");
+            for (uint64_t c = 0; c < bl.ncols; c++) {
+                if (bl.columns[c] >= m) return invalid("block column out of range");
+                if (c && bl.columns[c] <= bl.columns[c - 1]) return invalid("block columns must ascend");
+            }
+            K += bl.rows;
+        }
+        if (m == 0) return SDPX_GPU_OK;
+        // scatter every block into dense K x m operands Z (= RN(diag(d) y)) and Y;
+        // columns a block does not touch are exact zeros, which leave every
+        // exact sum unchanged (rns_blas.rs:2671-2922 sums the blocks exactly
+        // and rounds once)
+        std::vector<uint8_t> Z(K * m * vb), Y(K * m * vb);
+        for (size_t e = 0; e < K * m; e++) {
+            sdpx::mpx::set_zero(&Z[e * vb], (int)limbs);
+            sdpx::mpx::set_zero(&Y[e * vb], (int)limbs);
+        }
+        uint64_t r0 = 0;
+        for (uint64_t b = 0; b < nblocks; b++) {
+            const auto &bl = blocks[b];
+            const uint8_t *y = (const uint8_t *)bl.y, *d = (const uint8_t *)bl.d;
+            for (uint64_t c = 0; c < bl.ncols; c++)
+                for (uint64_t r = 0; r < bl.rows; r++) {
+                    const uint8_t *src = y + (r + c * bl.rows) * vb;
+                    const size_t dst = (r0 + r) + bl.columns[c] * K;
+                    sdpx::mpx::copy_value(&Y[dst * vb], src, (int)limbs);
+                    sdpx::mpx::mul(sdpx::mpx::View{(int)limbs, d + r * vb},
+                                   sdpx::mpx::View{(int)limbs, src}, &Z[dst * vb]);
+                }
+            r0 += bl.rows;
+        }
+        // c[i + j m] (i <= j) = sum_r Z[r, i] Y[r, j]
+        return sdpx_gpu_rns_gemm(limbs, 'T', 'N', m, m, K, Z.data(), K, Y.data(), K, 1, out, m * m);
     });
 }
 

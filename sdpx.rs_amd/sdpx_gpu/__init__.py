@@ -104,6 +104,8 @@ def _declare(L):
     for name in ("sdpx_gpu_adjoint_psd_f64", "sdpx_gpu_adjoint_psd_f64_dev"):
         getattr(L, name).argtypes = [vp, u64, vp, vp, u64, vp, vp, vp]
         getattr(L, name).restype = i32
+    L.sdpx_gpu_rns_gemm_blocks_upper.argtypes = [u32, u64, u64, vp, vp]
+    L.sdpx_gpu_rns_gemm_blocks_upper.restype = i32
     gemm_args = [u32, u8, u8, u64, u64, u64, vp, u64, vp, u64, u32, vp, u64]
     for name in ("sdpx_gpu_rns_gemm", "sdpx_gpu_rns_gemm_dev"):
         f = getattr(L, name)
@@ -401,3 +403,24 @@ def adjoint_psd_f64(factor, rinv, a_svec, ybar):
                                          a_svec.ctypes.data_as(vp), r, ybar.ctypes.data_as(vp),
                                          bgrad.ctypes.data_as(vp), cgrad.ctypes.data_as(vp)))
     return bgrad, cgrad
+
+
+class ExactBlock(ctypes.Structure):
+    _fields_ = [("rows", ctypes.c_uint64), ("ncols", ctypes.c_uint64), ("y", ctypes.c_void_p),
+                ("d", ctypes.c_void_p), ("columns", ctypes.c_void_p)]
+
+
+def xgemm_blocks_upper_exact(m, blocks, out):
+    """XgemmScalar::xgemm_blocks_upper_exact (traits.rs:250-258): blocks are
+    (y records rows x ncols column-major, d records (rows), columns (ascending
+    uint64)).  Writes c[i + j m], i <= j; returns False on a decline."""
+    arr = (ExactBlock * len(blocks))()
+    keep = []
+    for i, (y, d, cols) in enumerate(blocks):
+        cols = np.ascontiguousarray(cols, dtype=np.uint64)
+        keep.append(cols)
+        arr[i] = ExactBlock(d.size, cols.size, _ptr(y), _ptr(d), _ptr(cols))
+    nl = mp.limbs_of(out)
+    code = lib().sdpx_gpu_rns_gemm_blocks_upper(nl, m, len(blocks), arr, _ptr(out))
+    check(code, allow_decline=True)
+    return code == OK

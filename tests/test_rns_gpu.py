@@ -182,3 +182,36 @@ def test_full_size_fixed_point_spot_check(gpu):
     assert np.all(mp.same_bits(out[rows + cols * m], ref))
     ok, out2 = gpu.rns_gemm("N", "N", m, n, k, a, m, b, k)
     assert ok and out2.tobytes() == out.tobytes()
+
+
+def test_grouped_blocks_upper(gpu):
+    # xgemm_blocks_upper_exact: uneven blocks on sorted global columns, the
+    # exact sum over all blocks rounded once (rns_blas.rs:2671-2922)
+    import mpexact as M
+    from helpers import to_mp, from_mp
+    rng = np.random.default_rng(17)
+    nl, m = 4, 40
+    blocks = []
+    for rows, ncols in ((7, 12), (30, 40), (3, 5)):
+        cols = np.sort(rng.choice(m, ncols, replace=False)).astype(np.uint64)
+        y = mp.random_full(rows * ncols, nl, 20, rng)
+        d = mp.random_full(rows, nl, 10, rng)
+        blocks.append((y, d, cols))
+    out = mp.zeros(m * m, nl)
+    assert gpu.xgemm_blocks_upper_exact(m, blocks, out)
+    # oracle: exact sums with z = RN(d y)
+    for i in range(m):
+        for j in range(i, m, 7):
+            pairs = []
+            for y, d, cols in blocks:
+                cl = list(cols)
+                if i in cl and j in cl:
+                    ci, cj = cl.index(i), cl.index(j)
+                    rows = d.size
+                    for r in range(rows):
+                        yi = to_mp(y[r + ci * rows: r + ci * rows + 1])[0]
+                        yj = to_mp(y[r + cj * rows: r + cj * rows + 1])[0]
+                        dr = to_mp(d[r:r + 1])[0]
+                        pairs.append((M.mul(dr, yi), yj))
+            ref = M.exact_dot(pairs, nl)
+            assert mp.same_bits(out[i + j * m: i + j * m + 1], from_mp([ref], nl))[0], (i, j)
