@@ -197,7 +197,150 @@ def sampled_workload(args):
     return result
 
 
-WORKLOADS = {"rns_gemm": rns_gemm_workload, "sampled": sampled_workload}
+def chol_workload(args):
+    """Blocked MPFR-512 LDL^T of a synthetic SPD Schur matrix of order 8192,
+    A = Q diag(s) Q^T with Q a seeded Haar orthogonal matrix (binary64,
+    lifted exactly) and s = 2^-100 ... 1 log-spaced (powers of two, so
+    Q diag(s) is exact in binary64 and A is formed exactly rounded at 512
+    bits by one RNS product on the GPU), device-resident; the same matrix's
+    float64 Cholesky on the same GPU beside it."""
+    import torch
+
+    import sdpx_gpu
+    from sdpx_gpu import mp
+
+    nl, n = 8, args.order
+    rng = np.random.default_rng(11)
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    s = 2.0 ** np.round(np.linspace(-100, 0, n))
+    bq = q * s  # exact: power-of-two scaling
+    sdpx_gpu.check(sdpx_gpu.lib().sdpx_gpu_init(0))
+    dq = to_device(mp.from_float64(q.T.reshape(-1), nl))       # Q column-major
+    dbq = to_device(mp.from_float64(bq.T.reshape(-1), nl))     # Q diag(s)
+    vb = mp.dtype(nl).itemsize
+    da = torch.empty(n * n * vb, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    code = sdpx_gpu.rns_gemm_dev("N", "T", n, n, n, dbq.data_ptr(), n, dq.data_ptr(), n, nl, False,
+                                 da.data_ptr(), n * n)
+    assert code == sdpx_gpu.OK
+    del dq, dbq
+    work = torch.empty_like(da)
+    dinv = torch.empty(n * vb, dtype=torch.uint8, device="cuda")
+    L = sdpx_gpu.lib()
+    import ctypes
+
+    def step():
+        work.copy_(da)
+        torch.cuda.synchronize()
+        ok = ctypes.c_int32()
+        t0 = time.perf_counter()
+        sdpx_gpu.check(L.sdpx_gpu_mp_ldl_dev(nl, n, work.data_ptr(), n, None, 0, None, None,
+                                             dinv.data_ptr(), 0, ctypes.byref(ok)))
+        dt = time.perf_counter() - t0
+        assert ok.value == 1
+        return dt
+
+    for _ in range(args.warmup):
+        step()
+    sdpx_gpu.set_profiling(True)
+    sdpx_gpu.rns_totals(reset=True)
+    times = [step() for _ in range(args.steps)]
+    tot = sdpx_gpu.rns_totals(reset=True)
+    sdpx_gpu.set_profiling(False)
+    dt = sum(times) / len(times)
+    st = sdpx_gpu.mp_ldl_stats()
+    # float64 Cholesky of the same matrix (dynamic pivot rule: cond 2^100)
+    a64 = torch.tensor((q * s) @ q.T, dtype=torch.float64, device="cuda")
+    f = sdpx_gpu.DenseFactor(n)
+    f.set_dynamic_pivot(1e-13, 2e-7)
+    f.refactor_dev(a64.data_ptr(), n)
+    torch.cuda.synchronize()
+    f0 = time.perf_counter()
+    for _ in range(3):
+        f.refactor_dev(a64.data_ptr(), n)
+    fdt = (time.perf_counter() - f0) / 3
+    product_s = tot["stage_ms"][2] / 1e3
+    achieved = tot["int8_ops"] / product_s if product_s > 0 else 0.0
+    result = {
+        "value": dt, "unit": "s/factor", "ms_per_step": dt * 1e3,
+        "dtype": "mp512 (int8 MFMA residues, int32 accumulation)",
+        "config": {"workload": f"MPFR-512 LDL^T of Q diag(s) Q^T, order {n}, s in [2^-100, 1]",
+                   "order": n, "precision_bits": 512, "gpu_products": st["gpu_products"],
+                   "rns_stage_ms_per_factor": {k: v / args.steps for k, v in
+                                               zip(("scan", "encode", "product", "crt"),
+                                                   tot["stage_ms"])},
+                   "mp_macs_per_factor": tot["mp_macs"] / args.steps,
+                   "fp64_gpu_s_per_factor": fdt, "mp512_over_fp64": dt / fdt},
+        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": INT8_PEAK_OPS / 1e12,
+                     "unit": "TOP/s (INT8)", "frac": achieved / INT8_PEAK_OPS, "traffic": None,
+                     "kernel": "product_kernel (INT8 MFMA residue products)",
+                     "int8_ops_per_step": tot["int8_ops"] / args.steps},
+    }
+    if args.cpu_baseline:
+        sys.path[:0] = [os.path.join(ROOT, "tests"), os.path.join(ROOT, "oracle")]
+        from helpers import oracle_ldl
+
+        sub = 1024
+        a_sub = mp.from_float64(((q[:sub] * s) @ q[:sub].T).T.reshape(-1), nl)
+        cores = os.cpu_count() or 1
+        c0 = time.perf_counter()
+        oracle_ldl(a_sub, sub, threads=cores)
+        cdt = time.perf_counter() - c0
+        result["cpu_baseline"] = {"value": cdt * (n / sub) ** 3, "unit": "s/factor", "cores": cores,
+                                  "kind": "port",
+                                  "sample": f"C restatement (exact dots) on the leading {sub}x{sub} "
+                                            f"block ({cdt:.1f} s), scaled by (n/{sub})^3"}
+    return result
+
+
+def rns_syrk_workload(args):
+    """Exact 256-bit SYRK B^T B, B 16384 x 4096 fixed-point (packed upper)."""
+    import torch
+
+    import sdpx_gpu
+    from sdpx_gpu import mp
+
+    nl, k, n = 4, 16384, 4096
+    rng = np.random.default_rng(43)
+    b = mp.random_fixed(k * n, nl, 256, rng)
+    db = to_device(b)
+    out_len = n * (n + 1) // 2
+    dout = torch.empty(out_len * mp.dtype(nl).itemsize, dtype=torch.uint8, device="cuda")
+    sdpx_gpu.check(sdpx_gpu.lib().sdpx_gpu_init(0))
+    torch.cuda.synchronize()
+
+    def step():
+        code = sdpx_gpu.rns_gemm_dev("T", "N", n, n, k, db.data_ptr(), k, db.data_ptr(), k, nl,
+                                     True, dout.data_ptr(), out_len)
+        assert code == sdpx_gpu.OK
+
+    for _ in range(args.warmup):
+        step()
+    sdpx_gpu.set_profiling(True)
+    sdpx_gpu.rns_totals(reset=True)
+    step()
+    tot = sdpx_gpu.rns_totals(reset=True)
+    sdpx_gpu.set_profiling(False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    product_s = tot["stage_ms"][2] / 1e3
+    achieved = tot["int8_ops"] / product_s
+    return {"value": dt, "unit": "s/SYRK", "ms_per_step": dt * 1e3, "dtype": "int8->int32",
+            "config": {"workload": "rns_syrk B^T B, B 16384x4096 MP256 fixed-point",
+                       "stage_ms": dict(zip(("scan", "encode", "product", "crt"), tot["stage_ms"])),
+                       "mp_macs": tot["mp_macs"]},
+            "roofline": {"bound": "mfma", "achieved": achieved / 1e12,
+                         "peak": INT8_PEAK_OPS / 1e12, "unit": "TOP/s (INT8)",
+                         "frac": achieved / INT8_PEAK_OPS, "traffic": None,
+                         "kernel": "product_kernel", "int8_ops_per_step": tot["int8_ops"]}}
+
+
+WORKLOADS = {"rns_gemm": rns_gemm_workload, "sampled": sampled_workload,
+             "chol": chol_workload, "rns_syrk": rns_syrk_workload}
 
 
 def main():
@@ -207,6 +350,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="sampled", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
+    ap.add_argument("--order", type=int, default=8192, help="matrix order of --workload chol")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))

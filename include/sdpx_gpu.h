@@ -133,6 +133,22 @@ int32_t sdpx_gpu_rns_gemm_blocks_upper(uint32_t limbs, uint64_t m, uint64_t nblo
 int32_t sdpx_gpu_mp_ldl(uint32_t limbs, uint64_t n, void *a, uint64_t lda, const int8_t *signs,
                         uint64_t nsigns, const void *reg_eps, const void *reg_delta, void *dinv,
                         uint32_t flags, int32_t *ok);
+/* Device-pointer form of sdpx_gpu_mp_ldl (128/256/512 bits; signs and the
+ * regularization values stay host pointers).  Returns 2 (declined) when a
+ * dot's exponent spread or a product exceeds the GPU limits. */
+int32_t sdpx_gpu_mp_ldl_dev(uint32_t limbs, uint64_t n, void *a, uint64_t lda, const int8_t *signs,
+                            uint64_t nsigns, const void *reg_eps, const void *reg_delta, void *dinv,
+                            uint32_t flags, int32_t *ok);
+/* L D L^T x = b for nrhs columns (b: n x nrhs, ldb, overwritten by x) with
+ * the factor of sdpx_gpu_mp_ldl (unit lower l, dinv).  Replaces
+ * DenseLeaf::solve_many (arrow.rs:1155-1195) for a dense leaf; the rounding
+ * sequence is the recursive panel form (exact dots below 128 columns, one
+ * exact product per level, as forward_panel_rows, arrow.rs:1020-1100), not
+ * the reference's per-entry FMA chains, so results agree to the working
+ * precision rather than bitwise.  128/256/512 bits; returns 2 when a dot's
+ * exponent spread exceeds the GPU window. */
+int32_t sdpx_gpu_mp_ldl_solve(uint32_t limbs, uint64_t n, const void *l, uint64_t ldl,
+                              const void *dinv, uint64_t nrhs, void *b, uint64_t ldb);
 /* Counters of this thread's last sdpx_gpu_mp_ldl (measurement only). */
 int32_t sdpx_gpu_mp_ldl_stats(int64_t *gpu_products, int64_t *host_products, int64_t *regularized,
                               double *gpu_mp_macs);

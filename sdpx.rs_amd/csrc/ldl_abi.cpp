@@ -68,6 +68,50 @@ int32_t sdpx_gpu_mp_ldl(uint32_t limbs, uint64_t n, void *a, uint64_t lda, const
     });
 }
 
+int32_t sdpx_gpu_mp_ldl_dev(uint32_t limbs, uint64_t n, void *a, uint64_t lda, const int8_t *signs,
+                            uint64_t nsigns, const void *reg_eps, const void *reg_delta, void *dinv,
+                            uint32_t flags, int32_t *ok) {
+    return sdpx::abi_boundary([&]() -> int32_t {
+        if (!sdpx::mp_kernels_support((int)limbs))
+            return sdpx::abi_invalid("device factor supports 128, 256 and 512 bits");
+        if (lda < n) return sdpx::abi_invalid("lda below n");
+        sdpx::Device &d = sdpx::device();
+        std::lock_guard<std::recursive_mutex> g(d.mu);
+        t_ldl_stats = sdpx::LdlStats{};
+        bool recursive = (flags & SDPX_GPU_LDL_DOTS) == 0 && n >= 256;
+        int r = sdpx::ldl_factor_device((int)limbs, (uint8_t *)a, lda, n, signs, nsigns, reg_eps,
+                                        reg_delta, (uint8_t *)dinv, t_ldl_stats, recursive);
+        if (r < 0) return SDPX_GPU_DECLINED;
+        if (ok) *ok = r;
+        return SDPX_GPU_OK;
+    });
+}
+
+int32_t sdpx_gpu_mp_ldl_solve(uint32_t limbs, uint64_t n, const void *l, uint64_t ldl,
+                              const void *dinv, uint64_t nrhs, void *b, uint64_t ldb) {
+    return sdpx::abi_boundary([&]() -> int32_t {
+        if (!sdpx::mp_kernels_support((int)limbs))
+            return sdpx::abi_invalid("GPU solves support 128, 256 and 512 bits");
+        if (ldl < n || ldb < n) return sdpx::abi_invalid("leading dimension below n");
+        if (n == 0 || nrhs == 0) return SDPX_GPU_OK;
+        sdpx::Device &d = sdpx::device();
+        std::lock_guard<std::recursive_mutex> g(d.mu);
+        const size_t vb = sdpx::value_bytes((int)limbs);
+        uint8_t *dl = (uint8_t *)d.ws.get(72, n * n * vb);
+        uint8_t *dd = (uint8_t *)d.ws.get(73, n * vb);
+        uint8_t *db = (uint8_t *)d.ws.get(74, n * nrhs * vb);
+        HIP_CHECK(hipMemcpy2DAsync(dl, n * vb, l, ldl * vb, n * vb, n, hipMemcpyHostToDevice, d.stream));
+        HIP_CHECK(hipMemcpyAsync(dd, dinv, n * vb, hipMemcpyHostToDevice, d.stream));
+        HIP_CHECK(hipMemcpy2DAsync(db, n * vb, b, ldb * vb, n * vb, nrhs, hipMemcpyHostToDevice, d.stream));
+        HIP_CHECK(hipStreamSynchronize(d.stream));
+        int r = sdpx::ldl_solve_device((int)limbs, dl, n, n, dd, db, n, nrhs);
+        if (r < 0) return SDPX_GPU_DECLINED;
+        HIP_CHECK(hipMemcpy2DAsync(b, ldb * vb, db, n * vb, n * vb, nrhs, hipMemcpyDeviceToHost, d.stream));
+        HIP_CHECK(hipStreamSynchronize(d.stream));
+        return SDPX_GPU_OK;
+    });
+}
+
 int32_t sdpx_gpu_mp_ldl_stats(int64_t *gpu_products, int64_t *host_products, int64_t *regularized,
                               double *gpu_mp_macs) {
     if (gpu_products) *gpu_products = t_ldl_stats.gpu_products;

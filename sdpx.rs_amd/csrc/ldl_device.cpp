@@ -129,6 +129,28 @@ void solve_t_device(DCtx &c, size_t m, size_t n, uint8_t *L, size_t ldl, uint8_t
     solve_t_device(c, m, n2, c.at(L, n1 + n1 * ldl), ldl, B2, ldb, low2);
 }
 
+// X L = B (L unit lower n x n, X/B m x n): recursive halves from the right
+// with one exact product between them, rows by exact dots below 128 columns.
+void solve_n_device(DCtx &c, size_t m, size_t n, uint8_t *L, size_t ldl, uint8_t *B, size_t ldb) {
+    if (m == 0 || n == 0 || c.declined) return;
+    if (n <= 128) {
+        mp_trsm_base_n(c.n, L, ldl, n, B, ldb, m, c.d_status + 2, c.st);
+        return;
+    }
+    const size_t n1 = n / 2, n2 = n - n / 2;
+    uint8_t *B2 = c.at(B, n1 * ldb);
+    solve_n_device(c, m, n2, c.at(L, n1 + n1 * ldl), ldl, B2, ldb);
+    size_t mark = c.arena->top;
+    uint8_t *p = (uint8_t *)c.arena->alloc(m * n1 * c.vb);
+    // B1 -= X2 L21  (m x n2 times n2 x n1)
+    product(c, 'N', 'N', m, n1, n2, B2, ldb, c.at(L, n1), ldl, p, false);
+    if (c.declined) return;
+    mp_ew_sub_rect(c.n, B, ldb, p, m, m, n1, c.st);
+    HIP_CHECK(hipStreamSynchronize(c.st));
+    c.arena->top = mark;
+    solve_n_device(c, m, n1, L, ldl, B, ldb);
+}
+
 int rec_device(DCtx &c, uint8_t *l, size_t ld, size_t n, const std::vector<size_t> &first,
                size_t sign_off, uint8_t *dinv) {
     if (c.declined) return 1;
@@ -263,6 +285,28 @@ int ldl_factor_device(int limbs, uint8_t *d_l, size_t ld, size_t n, const int8_t
     if (c.declined || ds.window_overflow()) return -1;  // caller reruns on the host
     stats.regularized += ds.regularized();
     return ok;
+}
+
+int ldl_solve_device(int limbs, const uint8_t *d_l, size_t ldl, size_t n, const uint8_t *d_dinv,
+                     uint8_t *d_b, size_t ldb, size_t nrhs) {
+    const size_t vb = value_bytes(limbs);
+    LdlStats ls;
+    DeviceSetup ds(limbs, (size_t)(1.5 * (double)nrhs * n * vb) + (size_t)nrhs * n * vb + (64u << 20),
+                   4096, nullptr, 0, nullptr, nullptr, &ls);
+    DCtx &c = ds.ctx;
+    // X = B^T (nrhs x n): L y = b  <=>  y^T L^T = b^T;  L^T x = z  <=>  x^T L = z^T
+    uint8_t *X = (uint8_t *)c.arena->alloc(nrhs * n * vb);
+    mp_transpose(limbs, d_b, ldb, n, nrhs, X, nrhs, c.st);
+    std::vector<size_t> low(nrhs, 0);
+    solve_t_device(c, nrhs, n, (uint8_t *)d_l, ldl, X, nrhs, low);
+    if (c.declined) return -1;
+    mp_ew_mul_diag(limbs, X, nrhs, d_dinv, X, nrhs, nrhs, n, false, c.st);
+    solve_n_device(c, nrhs, n, (uint8_t *)d_l, ldl, X, nrhs);
+    if (c.declined) return -1;
+    mp_transpose(limbs, X, nrhs, nrhs, n, d_b, ldb, c.st);
+    HIP_CHECK(hipStreamSynchronize(c.st));
+    if (ds.window_overflow()) return -1;
+    return 1;
 }
 
 int sampled_arrow_device(int limbs, const std::vector<SampledLeafDev> &leaves, size_t t,

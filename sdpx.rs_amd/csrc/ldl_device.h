@@ -20,6 +20,11 @@ int ldl_factor_device(int limbs, uint8_t *d_l, size_t ld, size_t n, const int8_t
                       size_t nsigns, const void *reg_eps, const void *reg_delta, uint8_t *d_dinv,
                       LdlStats &stats, bool recursive);
 
+// L D L^T x = b for nrhs columns of b (n x nrhs, ldb) with a factor from
+// ldl_factor_device; 1 ok, -1 not handled on the GPU.
+int ldl_solve_device(int limbs, const uint8_t *d_l, size_t ldl, size_t n, const uint8_t *d_dinv,
+                     uint8_t *d_b, size_t ldb, size_t nrhs);
+
 // leaf factors / D^-1 / Y land in contiguous arrays in leaf order
 // (sum k^2, sum k, sum k*t values)
 int sampled_arrow_device(int limbs, const std::vector<SampledLeafDev> &leaves, size_t t,

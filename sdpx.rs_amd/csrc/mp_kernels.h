@@ -28,6 +28,10 @@ void mp_trsm_base(int limbs, const void *L, int64_t ldl, int64_t n, void *B, int
 void mp_ldl_dots(int limbs, void *A, void *dinv, const int32_t *first, const int8_t *signs,
                  int sign_mode, const void *blocks, int nblocks, void *scratch, const void *reg_eps,
                  const void *reg_delta, int *status, hipStream_t st);
+void mp_trsm_base_n(int limbs, const void *L, int64_t ldl, int64_t n, void *B, int64_t ldb,
+                    int64_t m, int *fail, hipStream_t st);
+void mp_transpose(int limbs, const void *in, int64_t ldi, int64_t m, int64_t n, void *out,
+                  int64_t ldo, hipStream_t st);
 void mp_envelope(int limbs, const void *A, int64_t ld, int64_t n, int32_t *first, hipStream_t st);
 void mp_mirror_upper(int limbs, void *S, int64_t n, int64_t ld, hipStream_t st);
 

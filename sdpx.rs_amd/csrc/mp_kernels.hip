@@ -214,6 +214,44 @@ __global__ __launch_bounds__(128) void trsm_base_kernel(const uint32_t *L, int64
     }
 }
 
+// X L = B rows for unit lower L, descending columns: x_j = RN(x_j -
+// RN(sum_{k > j} x_k L_kj)); B m x n (ldb), L n x n (ldl); one thread per row.
+template <int W>
+__global__ __launch_bounds__(128) void trsm_base_n_kernel(const uint32_t *L, int64_t ldl, int64_t n,
+                                                          uint32_t *B, int64_t ldb, int64_t m,
+                                                          int *fail) {
+    extern __shared__ int64_t slots_sh[];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    int64_t *slots = slots_sh + threadIdx.x;
+    const int stride = blockDim.x;
+    for (int64_t j = n - 1; j >= 0; j--) {
+        const int64_t k0 = j + 1;
+        auto fx = [&](int64_t q) { return rec_at<W>((const uint32_t *)B, i + (k0 + q) * ldb); };
+        auto fy = [&](int64_t q) { return rec_at<W>(L, (k0 + q) + j * ldl); };
+        Val<W> t, x, r;
+        if (!window_dot<W>(n - k0, fx, fy, nullptr, false, slots, stride, t)) {
+            atomicOr(fail, 1);
+            return;
+        }
+        load(rec_at<W>(B, i + j * ldb), x);
+        add_signed(x, t, true, r);
+        store(rec_at<W>(B, i + j * ldb), r);
+    }
+}
+
+// out (n x m, ldo) = in^T (in: m x n, ldi), values
+template <int W>
+__global__ void transpose_kernel(const uint32_t *in, int64_t ldi, int64_t m, int64_t n,
+                                 uint32_t *out, int64_t ldo) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m * n) return;
+    const int64_t i = e % m, j = e / m;
+    const uint32_t *src = rec_at<W>(in, i + j * ldi);
+    uint32_t *dst = rec_at<W>(out, j + i * ldo);
+    for (int u = 0; u < W + 2; u++) dst[u] = src[u];
+}
+
 // ldl_dots on one block per workgroup.  Blocks: (offset into A, ld, n,
 // offset into dinv, first[] offset, sign offset); r and w are per-block
 // packed-lower scratch (n(n+1)/2 values each, at scratch + 2*tri(n)*block).
@@ -466,6 +504,24 @@ void mp_envelope(int limbs, const void *A, int64_t ld, int64_t n, int32_t *first
     if (n == 0) return;
     SDPX_DISPATCH_W(limbs, hipLaunchKernelGGL(envelope_kernel<W>, dim3(blocks_for(n, 256)), dim3(256),
                                               0, st, (const uint32_t *)A, ld, n, first));
+    HIP_CHECK(hipGetLastError());
+}
+
+void mp_trsm_base_n(int limbs, const void *L, int64_t ldl, int64_t n, void *B, int64_t ldb,
+                    int64_t m, int *fail, hipStream_t st) {
+    if (m == 0 || n == 0) return;
+    SDPX_DISPATCH_W(limbs, hipLaunchKernelGGL(trsm_base_n_kernel<W>, dim3(blocks_for(m, 128)),
+                                              dim3(128), slot_bytes<W>(128), st, (const uint32_t *)L,
+                                              ldl, n, (uint32_t *)B, ldb, m, fail));
+    HIP_CHECK(hipGetLastError());
+}
+
+void mp_transpose(int limbs, const void *in, int64_t ldi, int64_t m, int64_t n, void *out,
+                  int64_t ldo, hipStream_t st) {
+    if (m * n == 0) return;
+    SDPX_DISPATCH_W(limbs, hipLaunchKernelGGL(transpose_kernel<W>, dim3(blocks_for(m * n, 256)),
+                                              dim3(256), 0, st, (const uint32_t *)in, ldi, m, n,
+                                              (uint32_t *)out, ldo));
     HIP_CHECK(hipGetLastError());
 }
 

@@ -106,6 +106,11 @@ def _declare(L):
         getattr(L, name).restype = i32
     L.sdpx_gpu_rns_gemm_blocks_upper.argtypes = [u32, u64, u64, vp, vp]
     L.sdpx_gpu_rns_gemm_blocks_upper.restype = i32
+    L.sdpx_gpu_mp_ldl_dev.argtypes = [u32, u64, vp, u64, vp, u64, vp, vp, vp, u32,
+                                      ctypes.POINTER(i32)]
+    L.sdpx_gpu_mp_ldl_dev.restype = i32
+    L.sdpx_gpu_mp_ldl_solve.argtypes = [u32, u64, vp, u64, vp, u64, vp, u64]
+    L.sdpx_gpu_mp_ldl_solve.restype = i32
     gemm_args = [u32, u8, u8, u64, u64, u64, vp, u64, vp, u64, u32, vp, u64]
     for name in ("sdpx_gpu_rns_gemm", "sdpx_gpu_rns_gemm_dev"):
         f = getattr(L, name)
@@ -424,3 +429,13 @@ def xgemm_blocks_upper_exact(m, blocks, out):
     code = lib().sdpx_gpu_rns_gemm_blocks_upper(nl, m, len(blocks), arr, _ptr(out))
     check(code, allow_decline=True)
     return code == OK
+
+
+def mp_ldl_solve(l, dinv, n, b):
+    """L D L^T x = b on the GPU (b: records, n x nrhs column-major).
+    Returns x (new array); raises on a decline."""
+    nl = mp.limbs_of(l)
+    x = b.copy()
+    nrhs = x.size // n
+    check(lib().sdpx_gpu_mp_ldl_solve(nl, n, _ptr(l), n, _ptr(dinv), nrhs, _ptr(x), n))
+    return x

@@ -46,3 +46,45 @@ def test_random_gpu_products(gpu, nl, n, kind):
     bad = ~mp.same_bits(Lg, L)
     assert not bad.any(), f"{bad.sum()} factor entries differ"
     assert np.all(mp.same_bits(dg, dinv))
+
+
+def widen(rec, nl_to):
+    """Exact widening of records to 64*nl_to bits (zero low limbs)."""
+    nl = mp.limbs_of(rec)
+    out = mp.zeros(rec.size, nl_to)
+    out["limbs"][:, nl_to - nl:] = rec["limbs"]
+    out["kind"] = rec["kind"]
+    out["exponent"] = rec["exponent"]
+    return out
+
+
+def test_solves_residual_vs_1024_bit_oracle(gpu):
+    # L D L^T x = b at 512 bits for 16 Gaussian right-hand sides; the
+    # residual b - A x is evaluated exactly at 1024 bits (C oracle exact
+    # dots on exactly widened values)
+    from helpers import oracle_gemm
+    rng = np.random.default_rng(8)
+    nl, n, nrhs = 8, 400, 16
+    s = spread_spd(n, rng, -30)
+    s = (s + s.T) / 2
+    A = mp.from_float64(s.T.reshape(-1), nl)
+    okg, L, dinv = sdpx_gpu.mp_ldl(A, n)
+    assert okg
+    b = mp.from_float64(rng.standard_normal(n * nrhs), nl)
+    x = sdpx_gpu.mp_ldl_solve(L, dinv, n, b)
+    Aw, xw, bw = widen(A, 16), widen(x, 16), widen(b, 16)
+    ax = oracle_gemm("N", "N", n, nrhs, n, Aw, n, xw, n, threads=16)
+    r = mp.to_float64(ax) - mp.to_float64(bw)  # |r| tiny: float64 views of 1024-bit values
+    scale = np.abs(s).max() * np.abs(mp.to_float64(x)).max() + np.abs(mp.to_float64(b)).max()
+    # Ax is exact to 1024 bits, so the float64 difference can only see
+    # residuals above 2^-53 relative; demand far less: every entry of
+    # b - Ax at 1024 bits below 2^-400 relative
+    from helpers import to_mp
+    import mpexact as M
+    worst = 0
+    axm, bm = to_mp(ax), to_mp(bw)
+    for t in range(0, n * nrhs, 37):
+        d = abs(axm[t].to_fraction() - bm[t].to_fraction())
+        worst = max(worst, float(d / M.Fraction(scale)) if d else 0.0)
+    assert worst < 2.0 ** -400, worst
+    assert np.abs(r).max() <= 1e-12 * scale
