@@ -117,14 +117,16 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmDesc d, int tiles_m, 
                     acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[a], fb[b], acc[a][b], 0, 0, 0);
         }
     }
-    // C/D layout: lane l holds rows 4*(l/16) + r, column l % 16
+    // C/D layout of v_mfma_f64_16x16x4f64: register r of lane l holds row
+    // l/16 + 4 r, column l % 16 (one row per register, unlike the
+    // four-consecutive-rows grouping of the f32 forms)
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                int64_t gi = i0 + wm * 32 + a * 16 + 4 * (lane >> 4) + r;
+                int64_t gi = i0 + wm * 32 + a * 16 + (lane >> 4) + 4 * r;
                 int64_t gj = j0 + wn * 32 + b * 16 + (lane & 15);
                 if (gi >= d.m || gj >= d.n) continue;
                 if (d.tri == 1 && gi > gj) continue;

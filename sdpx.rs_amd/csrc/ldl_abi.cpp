@@ -203,6 +203,17 @@ int32_t sdpx_gpu_sampled_arrow_dev(uint32_t limbs, uint64_t nleaves,
     return sdpx::abi_boundary([&]() -> int32_t {
         if (!sdpx::mp_kernels_support((int)limbs))
             return sdpx::abi_invalid("device pipeline supports 128, 256 and 512 bits");
+        if (nleaves && !leaves) return sdpx::abi_invalid("null leaves");
+        if (t && (!border || !border_dinv)) return sdpx::abi_invalid("null border");
+        if (nborder_signs != 0 && nborder_signs != 1 && nborder_signs != t)
+            return sdpx::abi_invalid("border signs: 0, 1 or t entries");
+        if (nborder_signs && !border_signs) return sdpx::abi_invalid("null border signs");
+        for (uint64_t i = 0; i < nleaves; i++)
+            if (!leaves[i].rinv || !leaves[i].basis || !leaves[i].weights ||
+                (t && !leaves[i].coupling))
+                return sdpx::abi_invalid("null leaf array");
+        if (nleaves && (!l_all || !dinv_all || (t && !y_all)))
+            return sdpx::abi_invalid("null leaf factor output");
         sdpx::Device &d = sdpx::device();
         std::lock_guard<std::recursive_mutex> g(d.mu);
         std::vector<sdpx::SampledLeafDev> dev(nleaves);

@@ -111,6 +111,9 @@ def _declare(L):
     L.sdpx_gpu_mp_ldl_dev.restype = i32
     L.sdpx_gpu_mp_ldl_solve.argtypes = [u32, u64, vp, u64, vp, u64, vp, u64]
     L.sdpx_gpu_mp_ldl_solve.restype = i32
+    L.sdpx_gpu_sampled_arrow_dev.argtypes = [u32, u64, vp, u64, vp, vp, vp, vp, vp, u64, vp,
+                                             ctypes.POINTER(i32)]
+    L.sdpx_gpu_sampled_arrow_dev.restype = i32
     gemm_args = [u32, u8, u8, u64, u64, u64, vp, u64, vp, u64, u32, vp, u64]
     for name in ("sdpx_gpu_rns_gemm", "sdpx_gpu_rns_gemm_dev"):
         f = getattr(L, name)
@@ -342,6 +345,55 @@ def sampled_arrow(problem, nl, border_signs=(-1,), flags=0, want_factors=False):
     check(lib().sdpx_gpu_sampled_arrow(nl, len(blocks), leaves, t, _ptr(border), _ptr(signs),
                                        signs.size, _ptr(bdinv), facs, flags, ctypes.byref(ok)))
     return bool(ok.value), border, bdinv, outs
+
+
+class SampledDev:
+    """Device-resident multiple-precision form of a sampled problem (torch
+    uint8 tensors on cuda holding the interchange records) for
+    sdpx_gpu_sampled_arrow_dev.  run() factors a fresh copy of the border;
+    the leaf factors land in l_all / dinv_all / y_all in leaf order."""
+
+    def __init__(self, problem, nl, border_signs=(-1,)):
+        import torch
+
+        def dev(a):
+            return torch.from_numpy(np.ascontiguousarray(a).view(np.uint8).reshape(-1)).to("cuda")
+
+        blocks = problem["blocks"]
+        self.nl, self.t = nl, problem["t"]
+        self.keep = []
+        self.leaves = (SampledLeaf * len(blocks))()
+        for i, b in enumerate(blocks):
+            arrs = [dev(b[k]) for k in ("rinv", "q", "w", "b")]
+            self.keep.append(arrs)
+            self.leaves[i] = SampledLeaf(b["h"], b["k"], *[a.data_ptr() for a in arrs])
+        vb = mp.dtype(nl).itemsize
+        ks = [b["k"] for b in blocks]
+        self.border0 = dev(problem["border"])
+        self.border = torch.empty_like(self.border0)
+        self.bdinv = torch.zeros(self.t * vb, dtype=torch.uint8, device="cuda")
+        self.l_all = torch.zeros(max(1, sum(k * k for k in ks)) * vb, dtype=torch.uint8, device="cuda")
+        self.dinv_all = torch.zeros(max(1, sum(ks)) * vb, dtype=torch.uint8, device="cuda")
+        self.y_all = torch.zeros(max(1, sum(ks) * self.t) * vb, dtype=torch.uint8, device="cuda")
+        self.signs = np.ascontiguousarray(border_signs, np.int8)
+        torch.cuda.synchronize()
+
+    def run(self):
+        """One Schur build + arrow factor; returns ok (False: a pivot failed).
+        Raises on a decline (the device limits were exceeded)."""
+        self.border.copy_(self.border0)
+        ok = ctypes.c_int32()
+        check(lib().sdpx_gpu_sampled_arrow_dev(self.nl, len(self.leaves), self.leaves, self.t,
+                                               self.l_all.data_ptr(), self.dinv_all.data_ptr(),
+                                               self.y_all.data_ptr(), self.border.data_ptr(),
+                                               _ptr(self.signs), self.signs.size,
+                                               self.bdinv.data_ptr(), ctypes.byref(ok)))
+        return bool(ok.value)
+
+    def result(self):
+        """(border factor, border dinv) as host record arrays."""
+        dt = mp.dtype(self.nl)
+        return (self.border.cpu().numpy().view(dt).copy(), self.bdinv.cpu().numpy().view(dt).copy())
 
 
 def sampled_stats():

@@ -88,8 +88,15 @@ def test_dynamic_pivot_rule(gpu):
     assert f.refactor(np.array([[1e-30]]))
     x = f.solve_many(np.array([[1.0]]))
     assert x[0, 0] == pytest.approx(1.0 / 2e-7, rel=1e-15)
-    assert f.refactor(np.array([[-1e-30]]))  # within rounding of zero: replaced
-    assert not f.refactor(np.array([[-1.0]]))  # clearly negative: rejected
+    # a pivot that cancels to rounding level relative to its diagonal
+    # (|d| <= 64 eps n |a_kk|) is replaced; a negative pivot of the size of
+    # its own diagonal is indefinite and rejected (dense_block.rs:341-345)
+    f2 = gpu.DenseFactor(2)
+    f2.set_dynamic_pivot(1e-13, 2e-7)
+    assert f2.refactor(np.array([[3.0, 1.0], [1.0, 1.0 / 3.0]]))
+    f2.close()
+    assert not f.refactor(np.array([[-1e-30]]))
+    assert not f.refactor(np.array([[-1.0]]))
     f.close()
 
 

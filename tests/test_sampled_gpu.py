@@ -36,3 +36,32 @@ def test_reduced_instance(gpu):
 
 def test_full_instance_256(gpu):
     compare(workloads.sampled_problem(4, seed=7), 4)
+
+
+def test_device_resident_entry_matches_host_entry(gpu):
+    """sdpx_gpu_sampled_arrow_dev on HBM-resident records gives the host
+    entry's bits (border factor, its D^-1, and every leaf factor), twice in a
+    row on the same buffers."""
+    prob = workloads.sampled_problem(4, nblocks=5, hmin=10, hmax=24, t=300, seed=5)
+    okh, bh, dh, fh = sdpx_gpu.sampled_arrow(prob, 4, want_factors=True)
+    run = sdpx_gpu.SampledDev(prob, 4)
+    t = prob["t"]
+    lower = np.tril_indices(t)
+    idx = lower[0] + lower[1] * t
+    dt = mp.dtype(4)
+    for rep in range(2):
+        print("sampled dev rep", rep, flush=True)
+        assert run.run() and okh
+        bg, dg = run.result()
+        assert np.all(mp.same_bits(bg[idx], bh[idx]))
+        assert np.all(mp.same_bits(dg, dh))
+        l_all = run.l_all.cpu().numpy().view(dt)
+        dinv_all = run.dinv_all.cpu().numpy().view(dt)
+        y_all = run.y_all.cpu().numpy().view(dt)
+        lo = dlo = ylo = 0
+        for b, (l, d, y) in zip(prob["blocks"], fh):
+            k = b["k"]
+            assert np.all(mp.same_bits(l_all[lo:lo + k * k], l))
+            assert np.all(mp.same_bits(dinv_all[dlo:dlo + k], d))
+            assert np.all(mp.same_bits(y_all[ylo:ylo + k * t], y))
+            lo, dlo, ylo = lo + k * k, dlo + k, ylo + k * t
