@@ -111,35 +111,50 @@ struct Win {
     static constexpr int SLOTS = 2 * W + 20;
 };
 
-// exponent scale range of the products x_k y_k (zeros skipped)
-template <int W, class FX, class FY>
-__device__ inline bool scale_range(int64_t k, FX fx, FY fy, int64_t &lo, int64_t &hi) {
-    lo = INT64_MAX;
-    hi = INT64_MIN;
-    for (int64_t t = 0; t < k; t++) {
-        const uint32_t *x = fx(t), *y = fy(t);
-        const int32_t kx = (int32_t)x[W], ky = (int32_t)y[W];
-        if (kx == mpd::KIND_ZERO || kx == -mpd::KIND_ZERO || ky == mpd::KIND_ZERO || ky == -mpd::KIND_ZERO) continue;
-        const int64_t s = (int64_t)(int32_t)x[W + 1] + (int32_t)y[W + 1] - 64 * (int64_t)W;
-        lo = min(lo, s);
-        hi = max(hi, s);
-    }
-    return lo != INT64_MAX;
+// ---------------------------------------------------------------------------
+// group dots: one exact dot shared by the G lanes of an aligned lane group
+// ---------------------------------------------------------------------------
+// Terms are indexed by their absolute position k in [k0, k1); lane gl of the
+// group takes the k with k % G == gl, so a value written at position k by
+// lane k % G is only ever read back by that same lane.  Every lane builds a
+// partial slot window over the group's common base; the windows are summed
+// across the group (integer adds of the unresolved slots: the same sum a
+// single lane would hold after all k1 - k0 terms), and every lane then
+// carries and rounds the identical total, so all lanes hold the result.
+template <int G>
+__device__ __forceinline__ int64_t group_sum(int64_t v) {
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
 }
 
-// RN(c - sum x_t y_t) (with_c) or RN(sum x_t y_t), one rounding; false when
-// the spread does not fit the window.
-template <int W, class FX, class FY>
-__device__ inline bool window_dot(int64_t k, FX fx, FY fy, const Val<W> *c, bool negate_terms,
-                                  int64_t *slots, int stride, Val<W> &out) {
+template <int W, int G, class FX, class FY>
+__device__ inline bool group_dot(int64_t k0, int64_t k1, int gl, FX fx, FY fy, const Val<W> *c,
+                                 bool negate_terms, int64_t *slots, int stride, Val<W> &out) {
     constexpr int SLOTS = Win<W>::SLOTS;
-    int64_t lo, hi;
-    bool any = scale_range<W>(k, fx, fy, lo, hi);
+    const int64_t kf = k0 + (((int64_t)gl - k0) % G + G) % G;  // first k of this lane
+    int64_t lo = INT64_MAX, hi = INT64_MIN;
+    for (int64_t k = kf; k < k1; k += G) {
+        const uint32_t *x = fx(k), *y = fy(k);
+        const int32_t kx = (int32_t)x[W], ky = (int32_t)y[W];
+        if (kx == mpd::KIND_ZERO || kx == -mpd::KIND_ZERO || ky == mpd::KIND_ZERO ||
+            ky == -mpd::KIND_ZERO)
+            continue;
+        const int64_t sc = (int64_t)(int32_t)x[W + 1] + (int32_t)y[W + 1] - 64 * (int64_t)W;
+        lo = min(lo, sc);
+        hi = max(hi, sc);
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) {
+        lo = min(lo, (int64_t)__shfl_xor(lo, off, 64));
+        hi = max(hi, (int64_t)__shfl_xor(hi, off, 64));
+    }
+    bool any = lo != INT64_MAX;
     const bool cnz = c && !is_zero(*c);
     if (cnz) {
-        const int64_t s = (int64_t)c->e - 32 * W;
-        lo = any ? min(lo, s) : s;
-        hi = any ? max(hi, s - 32 * W) : s - 32 * W;
+        const int64_t sc = (int64_t)c->e - 32 * W;
+        lo = any ? min(lo, sc) : sc;
+        hi = any ? max(hi, sc - 32 * W) : sc - 32 * W;
         any = true;
     }
     if (!any) {
@@ -149,94 +164,117 @@ __device__ inline bool window_dot(int64_t k, FX fx, FY fy, const Val<W> *c, bool
     if ((hi - lo) / 32 + 2 * W + 4 > SLOTS) return false;
     DotWindow<W, SLOTS> dw{slots, stride, lo};
     dw.clear();
-    for (int64_t t = 0; t < k; t++) {
+    for (int64_t k = kf; k < k1; k += G) {
         Val<W> x, y;
-        load(fx(t), x);
-        load(fy(t), y);
+        load(fx(k), x);
+        load(fy(k), y);
         if (is_zero(x) || is_zero(y)) continue;
         dw.add_product(x, y, negate_terms);
     }
-    if (cnz) dw.add_value(*c, false);
+    if (cnz && gl == 0) dw.add_value(*c, false);
+    if (G > 1) {
+        // products land in slots [0, (hi - lo)/32 + 2W]; the rest stay zero
+        const int used = (int)min((int64_t)SLOTS, (hi - lo) / 32 + 2 * W + 1);
+        for (int sl = 0; sl < used; sl++) {
+            int64_t *p = slots + (int64_t)sl * stride;
+            *p = group_sum<G>(*p);
+        }
+    }
     dw.finish(out);
     return true;
 }
 
+// Lanes per exact dot in the panel kernels.  The row kernels (forward rows,
+// trsm bases) run one wave per workgroup so their many small workgroups
+// spread over all CUs; ldl_dots runs a whole block per workgroup.  The slot
+// windows live in LDS: SLOTS x 8 bytes per thread.
+constexpr int DOT_G = 16;
+constexpr int ROW_THREADS = 64;
+constexpr int LDL_THREADS = 256;
+
 // Y = L^-1 B per column (forward_suffix_rows): L K x K lower (ld K),
-// first[K] the row envelope, Y K x t (ld K) in place.  One thread per column.
-template <int W>
-__global__ __launch_bounds__(128) void forward_rows_kernel(const uint32_t *L, const int32_t *first,
-                                                           int64_t K, uint32_t *Y, int64_t t,
-                                                           int *fail) {
+// first[K] the row envelope, Y K x t (ld K) in place.  One lane group per
+// column; y_i is stored by lane i % G, the only lane that reads it back.
+template <int W, int G>
+__global__ __launch_bounds__(ROW_THREADS) void forward_rows_kernel(
+    const uint32_t *L, const int32_t *first, int64_t K, uint32_t *Y, int64_t t, int *fail) {
     extern __shared__ int64_t slots_sh[];
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= t) return;
+    const int gl = threadIdx.x & (G - 1);
+    const int64_t c = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    if (c >= t) return;  // whole group
     int64_t *slots = slots_sh + threadIdx.x;
     const int stride = blockDim.x;
     uint32_t *col = Y + c * K * (W + 2);
     for (int64_t i = 0; i < K; i++) {
-        const int64_t lo = first[i];
         Val<W> b, r;
         load(rec_at<W>(col, i), b);
-        // y_i = RN(b_i - sum_{k in [lo, i)} L_ik y_k)
-        auto fx = [&](int64_t q) { return rec_at<W>(L, i + (lo + q) * K); };
-        auto fy = [&](int64_t q) { return rec_at<W>((const uint32_t *)col, lo + q); };
-        if (!window_dot<W>(i - lo, fx, fy, &b, true, slots, stride, r)) {
-            atomicOr(fail, 1);
+        // y_i = RN(b_i - sum_{k in [first_i, i)} L_ik y_k)
+        auto fx = [&](int64_t k) { return rec_at<W>(L, i + k * K); };
+        auto fy = [&](int64_t k) { return rec_at<W>((const uint32_t *)col, k); };
+        if (!group_dot<W, G>(first[i], i, gl, fx, fy, &b, true, slots, stride, r)) {
+            if (gl == 0) atomicOr(fail, 1);
             return;
         }
-        store(rec_at<W>(col, i), r);
+        if (gl == (int)(i % G)) store(rec_at<W>(col, i), r);
     }
 }
 
 // X L^T = B rows (solve_unit_lower_t base): B m x n (ldb), L n x n (ldl,
-// unit lower, diagonal unread), low[m]; one thread per row.
-template <int W>
-__global__ __launch_bounds__(128) void trsm_base_kernel(const uint32_t *L, int64_t ldl, int64_t n,
-                                                        uint32_t *B, int64_t ldb, int64_t m,
-                                                        const int32_t *low, int *fail) {
+// unit lower, diagonal unread), low[m]; one lane group per row.
+template <int W, int G>
+__global__ __launch_bounds__(ROW_THREADS) void trsm_base_kernel(
+    const uint32_t *L, int64_t ldl, int64_t n, uint32_t *B, int64_t ldb, int64_t m,
+    const int32_t *low, int *fail) {
     extern __shared__ int64_t slots_sh[];
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int gl = threadIdx.x & (G - 1);
+    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
     if (i >= m) return;
     int64_t *slots = slots_sh + threadIdx.x;
     const int stride = blockDim.x;
     const int64_t lo = low[i];
     for (int64_t j = lo; j < n; j++) {
-        auto fx = [&](int64_t q) { return rec_at<W>((const uint32_t *)B, i + (lo + q) * ldb); };
-        auto fy = [&](int64_t q) { return rec_at<W>(L, j + (lo + q) * ldl); };
-        Val<W> t, x, r;
-        if (!window_dot<W>(j - lo, fx, fy, nullptr, false, slots, stride, t)) {
-            atomicOr(fail, 1);
+        auto fx = [&](int64_t k) { return rec_at<W>((const uint32_t *)B, i + k * ldb); };
+        auto fy = [&](int64_t k) { return rec_at<W>(L, j + k * ldl); };
+        Val<W> t;
+        if (!group_dot<W, G>(lo, j, gl, fx, fy, nullptr, false, slots, stride, t)) {
+            if (gl == 0) atomicOr(fail, 1);
             return;
         }
-        load(rec_at<W>(B, i + j * ldb), x);
-        add_signed(x, t, true, r);
-        store(rec_at<W>(B, i + j * ldb), r);
+        if (gl == (int)(j % G)) {
+            Val<W> x, r;
+            load(rec_at<W>(B, i + j * ldb), x);
+            add_signed(x, t, true, r);
+            store(rec_at<W>(B, i + j * ldb), r);
+        }
     }
 }
 
 // X L = B rows for unit lower L, descending columns: x_j = RN(x_j -
-// RN(sum_{k > j} x_k L_kj)); B m x n (ldb), L n x n (ldl); one thread per row.
-template <int W>
-__global__ __launch_bounds__(128) void trsm_base_n_kernel(const uint32_t *L, int64_t ldl, int64_t n,
-                                                          uint32_t *B, int64_t ldb, int64_t m,
-                                                          int *fail) {
+// RN(sum_{k > j} x_k L_kj)); B m x n (ldb), L n x n (ldl); one lane group
+// per row.
+template <int W, int G>
+__global__ __launch_bounds__(ROW_THREADS) void trsm_base_n_kernel(
+    const uint32_t *L, int64_t ldl, int64_t n, uint32_t *B, int64_t ldb, int64_t m, int *fail) {
     extern __shared__ int64_t slots_sh[];
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int gl = threadIdx.x & (G - 1);
+    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
     if (i >= m) return;
     int64_t *slots = slots_sh + threadIdx.x;
     const int stride = blockDim.x;
     for (int64_t j = n - 1; j >= 0; j--) {
-        const int64_t k0 = j + 1;
-        auto fx = [&](int64_t q) { return rec_at<W>((const uint32_t *)B, i + (k0 + q) * ldb); };
-        auto fy = [&](int64_t q) { return rec_at<W>(L, (k0 + q) + j * ldl); };
-        Val<W> t, x, r;
-        if (!window_dot<W>(n - k0, fx, fy, nullptr, false, slots, stride, t)) {
-            atomicOr(fail, 1);
+        auto fx = [&](int64_t k) { return rec_at<W>((const uint32_t *)B, i + k * ldb); };
+        auto fy = [&](int64_t k) { return rec_at<W>(L, k + j * ldl); };
+        Val<W> t;
+        if (!group_dot<W, G>(j + 1, n, gl, fx, fy, nullptr, false, slots, stride, t)) {
+            if (gl == 0) atomicOr(fail, 1);
             return;
         }
-        load(rec_at<W>(B, i + j * ldb), x);
-        add_signed(x, t, true, r);
-        store(rec_at<W>(B, i + j * ldb), r);
+        if (gl == (int)(j % G)) {
+            Val<W> x, r;
+            load(rec_at<W>(B, i + j * ldb), x);
+            add_signed(x, t, true, r);
+            store(rec_at<W>(B, i + j * ldb), r);
+        }
     }
 }
 
@@ -255,17 +293,17 @@ __global__ void transpose_kernel(const uint32_t *in, int64_t ldi, int64_t m, int
 // ldl_dots on one block per workgroup.  Blocks: (offset into A, ld, n,
 // offset into dinv, first[] offset, sign offset); r and w are per-block
 // packed-lower scratch (n(n+1)/2 values each, at scratch + 2*tri(n)*block).
+// Per column: lane group 0 forms the pivot, then the groups share the rows;
+// the barrier between the phases orders every scratch write before its reads.
 struct LdlBlock {
     int64_t a_off, ld, n, dinv_off, first_off, sign_off, scratch_off;
 };
 
-template <int W>
-__global__ __launch_bounds__(128) void ldl_dots_kernel(uint32_t *A, uint32_t *dinv,
-                                                       const int32_t *first_all,
-                                                       const int8_t *signs, int sign_mode,
-                                                       const LdlBlock *blocks, uint32_t *scratch,
-                                                       const uint32_t *reg_eps,
-                                                       const uint32_t *reg_delta, int *status) {
+template <int W, int G>
+__global__ __launch_bounds__(LDL_THREADS) void ldl_dots_kernel(
+    uint32_t *A, uint32_t *dinv, const int32_t *first_all, const int8_t *signs, int sign_mode,
+    const LdlBlock *blocks, uint32_t *scratch, const uint32_t *reg_eps, const uint32_t *reg_delta,
+    int *status) {
     extern __shared__ int64_t slots_sh[];
     __shared__ int bad;
     const LdlBlock B = blocks[blockIdx.x];
@@ -279,43 +317,48 @@ __global__ __launch_bounds__(128) void ldl_dots_kernel(uint32_t *A, uint32_t *di
     uint32_t *dstore = w + tri(n) * (W + 2);  // n pivots
     int64_t *slots = slots_sh + threadIdx.x;
     const int stride = blockDim.x;
+    const int gl = threadIdx.x & (G - 1), grp = threadIdx.x / G, groups = blockDim.x / G;
     if (threadIdx.x == 0) bad = 0;
     __syncthreads();
     for (int64_t j = 0; j < n; j++) {
-        if (threadIdx.x == 0) {
+        if (grp == 0) {
             const int64_t fj = first[j];
-            auto fx = [&](int64_t q) { return rec_at<W>((const uint32_t *)r, tri(j) + fj + q); };
-            auto fy = [&](int64_t q) { return rec_at<W>((const uint32_t *)w, tri(j) + fj + q); };
+            auto fx = [&](int64_t k) { return rec_at<W>((const uint32_t *)r, tri(j) + k); };
+            auto fy = [&](int64_t k) { return rec_at<W>((const uint32_t *)w, tri(j) + k); };
             Val<W> ajj, t, d;
             load(rec_at<W>(a, j + j * ld), ajj);
-            bool ok = window_dot<W>(j - fj, fx, fy, nullptr, false, slots, stride, t);
+            bool ok = group_dot<W, G>(fj, j, gl, fx, fy, nullptr, false, slots, stride, t);
             if (ok) {
                 add_signed(ajj, t, true, d);
-                if (is_zero(d) && j > fj) ok = window_dot<W>(j - fj, fx, fy, &ajj, true, slots, stride, d);
+                // exact retry of a cancelled pivot (uniform over the group)
+                if (is_zero(d) && j > fj)
+                    ok = group_dot<W, G>(fj, j, gl, fx, fy, &ajj, true, slots, stride, d);
             }
-            if (!ok) {
-                bad = 2;  // window overflow: host path
-            } else {
-                // DenseLeaf::pivot
-                const int sg = sign_mode == 0 ? 1 : (sign_mode == 1 ? signs[0] : signs[B.sign_off + j]);
-                if (!is_regular(d) && !is_zero(d)) bad = 1;
-                if (!bad && reg_eps) {
-                    Val<W> ds = d, e, diff;
-                    if (sg < 0 && is_regular(ds)) ds.kind = -ds.kind;
-                    load(reg_eps, e);
-                    add_signed(ds, e, true, diff);
-                    if (!is_zero(diff) && diff.kind < 0) {
-                        load(reg_delta, d);
-                        if (sg < 0 && is_regular(d)) d.kind = -d.kind;
-                        atomicAdd(status + 1, 1);  // regularized count
+            if (gl == 0) {
+                if (!ok) {
+                    bad = 2;  // window overflow: host path
+                } else {
+                    // DenseLeaf::pivot
+                    const int sg = sign_mode == 0 ? 1 : (sign_mode == 1 ? signs[0] : signs[B.sign_off + j]);
+                    if (!is_regular(d) && !is_zero(d)) bad = 1;
+                    if (!bad && reg_eps) {
+                        Val<W> ds = d, e, diff;
+                        if (sg < 0 && is_regular(ds)) ds.kind = -ds.kind;
+                        load(reg_eps, e);
+                        add_signed(ds, e, true, diff);
+                        if (!is_zero(diff) && diff.kind < 0) {
+                            load(reg_delta, d);
+                            if (sg < 0 && is_regular(d)) d.kind = -d.kind;
+                            atomicAdd(status + 1, 1);  // regularized count
+                        }
                     }
-                }
-                if (!bad && is_zero(d)) bad = 1;
-                if (!bad) {
-                    store(rec_at<W>(dstore, j), d);
-                    Val<W> di;
-                    recip(d, di);
-                    store(rec_at<W>(dv, j), di);
+                    if (!bad && is_zero(d)) bad = 1;
+                    if (!bad) {
+                        store(rec_at<W>(dstore, j), d);
+                        Val<W> di;
+                        recip(d, di);
+                        store(rec_at<W>(dv, j), di);
+                    }
                 }
             }
         }
@@ -324,27 +367,29 @@ __global__ __launch_bounds__(128) void ldl_dots_kernel(uint32_t *A, uint32_t *di
         Val<W> dj, dij;
         load(rec_at<W>((const uint32_t *)dstore, j), dj);
         load(rec_at<W>((const uint32_t *)dv, j), dij);
-        for (int64_t i = j + 1 + threadIdx.x; i < n; i += blockDim.x) {
+        for (int64_t i = j + 1 + grp; i < n; i += groups) {
             Val<W> v;
             if (first[i] > j) {
                 set_zero(v);
             } else {
                 const int64_t lo = max(first[i], first[j]);
-                auto fx = [&](int64_t q) { return rec_at<W>((const uint32_t *)r, tri(i) + lo + q); };
-                auto fy = [&](int64_t q) { return rec_at<W>((const uint32_t *)w, tri(j) + lo + q); };
+                auto fx = [&](int64_t k) { return rec_at<W>((const uint32_t *)r, tri(i) + k); };
+                auto fy = [&](int64_t k) { return rec_at<W>((const uint32_t *)w, tri(j) + k); };
                 Val<W> t, aij, diff;
-                if (!window_dot<W>(j - lo, fx, fy, nullptr, false, slots, stride, t)) {
-                    atomicMax(&bad, 2);
+                if (!group_dot<W, G>(lo, j, gl, fx, fy, nullptr, false, slots, stride, t)) {
+                    if (gl == 0) atomicMax(&bad, 2);
                     continue;
                 }
                 load(rec_at<W>(a, i + j * ld), aij);
                 add_signed(aij, t, true, diff);
                 mul(diff, dij, v);
             }
-            store(rec_at<W>(r, tri(i) + j), v);
-            Val<W> wv;
-            mul(v, dj, wv);
-            store(rec_at<W>(w, tri(i) + j), wv);
+            if (gl == 0) {
+                store(rec_at<W>(r, tri(i) + j), v);
+                Val<W> wv;
+                mul(v, dj, wv);
+                store(rec_at<W>(w, tri(i) + j), wv);
+            }
         }
         __syncthreads();
         if (bad) break;
@@ -418,14 +463,23 @@ template <int W>
 size_t slot_bytes(int threads) {
     return (size_t)Win<W>::SLOTS * threads * sizeof(int64_t);
 }
+// dynamic LDS of a panel kernel (its slot windows), raising the kernel's
+// limit above the 64 KiB default where needed
+template <int W>
+size_t panel_lds(const void *kernel, int threads) {
+    const size_t bytes = slot_bytes<W>(threads);
+    if (bytes > 65536)
+        HIP_CHECK(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    return bytes;
+}
 inline unsigned blocks_for(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 }  // namespace
 
-#define SDPX_DISPATCH_W(limbs, CALL)                                    \
+#define SDPX_DISPATCH_W(limbs, ...)                                     \
     switch (limbs) {                                                    \
-        case 2: { constexpr int W = 4; CALL; break; }                   \
-        case 4: { constexpr int W = 8; CALL; break; }                   \
-        case 8: { constexpr int W = 16; CALL; break; }                  \
+        case 2: { constexpr int W = 4; __VA_ARGS__; break; }            \
+        case 4: { constexpr int W = 8; __VA_ARGS__; break; }            \
+        case 8: { constexpr int W = 16; __VA_ARGS__; break; }           \
         default: throw Error(SDPX_GPU_INVALID, "GPU panels support 128, 256 and 512 bits"); \
     }
 
@@ -463,18 +517,26 @@ void mp_sampled_entries(int limbs, const void *G, const void *w, int64_t K, void
 void mp_forward_rows(int limbs, const void *L, const int32_t *first, int64_t K, void *Y, int64_t t,
                      int *fail, hipStream_t st) {
     if (K == 0 || t == 0) return;
-    SDPX_DISPATCH_W(limbs, hipLaunchKernelGGL(forward_rows_kernel<W>, dim3(blocks_for(t, 128)),
-                                              dim3(128), slot_bytes<W>(128), st, (const uint32_t *)L,
-                                              first, K, (uint32_t *)Y, t, fail));
+    SDPX_DISPATCH_W(limbs, {
+        auto kern = forward_rows_kernel<W, DOT_G>;
+        const size_t lds = panel_lds<W>((const void *)kern, ROW_THREADS);
+        hipLaunchKernelGGL(kern, dim3(blocks_for(t * DOT_G, ROW_THREADS)),
+                           dim3(ROW_THREADS), lds, st, (const uint32_t *)L, first, K,
+                           (uint32_t *)Y, t, fail);
+    });
     HIP_CHECK(hipGetLastError());
 }
 
 void mp_trsm_base(int limbs, const void *L, int64_t ldl, int64_t n, void *B, int64_t ldb,
                   int64_t m, const int32_t *low, int *fail, hipStream_t st) {
     if (m == 0 || n == 0) return;
-    SDPX_DISPATCH_W(limbs, hipLaunchKernelGGL(trsm_base_kernel<W>, dim3(blocks_for(m, 128)),
-                                              dim3(128), slot_bytes<W>(128), st, (const uint32_t *)L,
-                                              ldl, n, (uint32_t *)B, ldb, m, low, fail));
+    SDPX_DISPATCH_W(limbs, {
+        auto kern = trsm_base_kernel<W, DOT_G>;
+        const size_t lds = panel_lds<W>((const void *)kern, ROW_THREADS);
+        hipLaunchKernelGGL(kern, dim3(blocks_for(m * DOT_G, ROW_THREADS)),
+                           dim3(ROW_THREADS), lds, st, (const uint32_t *)L, ldl, n,
+                           (uint32_t *)B, ldb, m, low, fail);
+    });
     HIP_CHECK(hipGetLastError());
 }
 
@@ -482,12 +544,14 @@ void mp_ldl_dots(int limbs, void *A, void *dinv, const int32_t *first, const int
                  int sign_mode, const void *blocks, int nblocks, void *scratch, const void *reg_eps,
                  const void *reg_delta, int *status, hipStream_t st) {
     if (nblocks == 0) return;
-    SDPX_DISPATCH_W(limbs, hipLaunchKernelGGL(ldl_dots_kernel<W>, dim3(nblocks), dim3(128),
-                                              slot_bytes<W>(128), st, (uint32_t *)A,
-                                              (uint32_t *)dinv, first, signs, sign_mode,
-                                              (const LdlBlock *)blocks, (uint32_t *)scratch,
-                                              (const uint32_t *)reg_eps,
-                                              (const uint32_t *)reg_delta, status));
+    SDPX_DISPATCH_W(limbs, {
+        auto kern = ldl_dots_kernel<W, DOT_G>;
+        const size_t lds = panel_lds<W>((const void *)kern, LDL_THREADS);
+        hipLaunchKernelGGL(kern, dim3(nblocks), dim3(LDL_THREADS), lds, st, (uint32_t *)A,
+                           (uint32_t *)dinv, first, signs, sign_mode, (const LdlBlock *)blocks,
+                           (uint32_t *)scratch, (const uint32_t *)reg_eps,
+                           (const uint32_t *)reg_delta, status);
+    });
     HIP_CHECK(hipGetLastError());
 }
 
@@ -510,9 +574,13 @@ void mp_envelope(int limbs, const void *A, int64_t ld, int64_t n, int32_t *first
 void mp_trsm_base_n(int limbs, const void *L, int64_t ldl, int64_t n, void *B, int64_t ldb,
                     int64_t m, int *fail, hipStream_t st) {
     if (m == 0 || n == 0) return;
-    SDPX_DISPATCH_W(limbs, hipLaunchKernelGGL(trsm_base_n_kernel<W>, dim3(blocks_for(m, 128)),
-                                              dim3(128), slot_bytes<W>(128), st, (const uint32_t *)L,
-                                              ldl, n, (uint32_t *)B, ldb, m, fail));
+    SDPX_DISPATCH_W(limbs, {
+        auto kern = trsm_base_n_kernel<W, DOT_G>;
+        const size_t lds = panel_lds<W>((const void *)kern, ROW_THREADS);
+        hipLaunchKernelGGL(kern, dim3(blocks_for(m * DOT_G, ROW_THREADS)),
+                           dim3(ROW_THREADS), lds, st, (const uint32_t *)L, ldl, n,
+                           (uint32_t *)B, ldb, m, fail);
+    });
     HIP_CHECK(hipGetLastError());
 }
 
