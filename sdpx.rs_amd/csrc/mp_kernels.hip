@@ -128,12 +128,22 @@ __device__ __forceinline__ int64_t group_sum(int64_t v) {
     return v;
 }
 
+// hint_lo <= hint_hi, when given, bound the scales of every non-zero term
+// (from running exponent ranges); a window that fits them skips the exact
+// range pass.  Any window holding all terms gives the same exact sum, and a
+// hint too wide for the window falls back to the exact pass, so the result
+// and the overflow decision never depend on the hint.
 template <int W, int G, class FX, class FY>
 __device__ inline bool group_dot(int64_t k0, int64_t k1, int gl, FX fx, FY fy, const Val<W> *c,
-                                 bool negate_terms, int64_t *slots, int stride, Val<W> &out) {
+                                 bool negate_terms, int64_t *slots, int stride, Val<W> &out,
+                                 int64_t hint_lo = INT64_MAX, int64_t hint_hi = INT64_MIN) {
     constexpr int SLOTS = Win<W>::SLOTS;
     const int64_t kf = k0 + (((int64_t)gl - k0) % G + G) % G;  // first k of this lane
     int64_t lo = INT64_MAX, hi = INT64_MIN;
+    if (hint_lo <= hint_hi && (hint_hi - hint_lo) / 32 + 2 * W + 4 <= SLOTS) {
+        lo = hint_lo;
+        hi = hint_hi;
+    } else {
     for (int64_t k = kf; k < k1; k += G) {
         const uint32_t *x = fx(k), *y = fy(k);
         const int32_t kx = (int32_t)x[W], ky = (int32_t)y[W];
@@ -148,6 +158,7 @@ __device__ inline bool group_dot(int64_t k0, int64_t k1, int gl, FX fx, FY fy, c
     for (int off = G / 2; off > 0; off >>= 1) {
         lo = min(lo, (int64_t)__shfl_xor(lo, off, 64));
         hi = max(hi, (int64_t)__shfl_xor(hi, off, 64));
+    }
     }
     bool any = lo != INT64_MAX;
     const bool cnz = c && !is_zero(*c);
@@ -186,11 +197,11 @@ __device__ inline bool group_dot(int64_t k0, int64_t k1, int gl, FX fx, FY fy, c
 
 // Lanes per exact dot in the panel kernels.  The row kernels (forward rows,
 // trsm bases) run one wave per workgroup so their many small workgroups
-// spread over all CUs; ldl_dots runs a whole block per workgroup.  The slot
-// windows live in LDS: SLOTS x 8 bytes per thread.
+// spread over all CUs; ldl_dots runs a whole block per workgroup (512
+// threads, 256 at 512 bits).  The slot windows live in LDS: SLOTS x 8 bytes
+// per thread.
 constexpr int DOT_G = 16;
 constexpr int ROW_THREADS = 64;
-constexpr int LDL_THREADS = 256;
 
 // Y = L^-1 B per column (forward_suffix_rows): L K x K lower (ld K),
 // first[K] the row envelope, Y K x t (ld K) in place.  One lane group per
@@ -293,17 +304,27 @@ __global__ void transpose_kernel(const uint32_t *in, int64_t ldi, int64_t m, int
 // ldl_dots on one block per workgroup.  Blocks: (offset into A, ld, n,
 // offset into dinv, first[] offset, sign offset); r and w are per-block
 // packed-lower scratch (n(n+1)/2 values each, at scratch + 2*tri(n)*block).
-// Per column: lane group 0 forms the pivot, then the groups share the rows;
-// the barrier between the phases orders every scratch write before its reads.
+// Column j: lane group 0 computes row j + 1 and then the pivot of column
+// j + 1 (it needs only that row), while the other groups compute rows
+// j + 2 .. n - 1; one barrier per column orders every scratch write before
+// its reads.  r_ij and w_ij are stored by lane j % G of their group, the lane
+// that reads position j back in a later dot of the same group.  Running
+// exponent ranges per row of r and w (LDS) give each dot its window without
+// a range pass when they fit.
 struct LdlBlock {
     int64_t a_off, ld, n, dinv_off, first_off, sign_off, scratch_off;
 };
 
+constexpr int LDL_STATS_MAX = 256;  // rows with exponent ranges in LDS
+template <int W>
+constexpr int ldl_threads() { return W >= 16 ? 256 : 512; }
+
 template <int W, int G>
-__global__ __launch_bounds__(LDL_THREADS) void ldl_dots_kernel(
+__global__ __launch_bounds__(ldl_threads<W>()) void ldl_dots_kernel(
     uint32_t *A, uint32_t *dinv, const int32_t *first_all, const int8_t *signs, int sign_mode,
     const LdlBlock *blocks, uint32_t *scratch, const uint32_t *reg_eps, const uint32_t *reg_delta,
     int *status) {
+    constexpr int SLOTS = Win<W>::SLOTS;
     extern __shared__ int64_t slots_sh[];
     __shared__ int bad;
     const LdlBlock B = blocks[blockIdx.x];
@@ -315,91 +336,164 @@ __global__ __launch_bounds__(LDL_THREADS) void ldl_dots_kernel(
     uint32_t *r = scratch + B.scratch_off * (W + 2);
     uint32_t *w = r + tri(n) * (W + 2);
     uint32_t *dstore = w + tri(n) * (W + 2);  // n pivots
+    const int nthreads = blockDim.x;
     int64_t *slots = slots_sh + threadIdx.x;
-    const int stride = blockDim.x;
-    const int gl = threadIdx.x & (G - 1), grp = threadIdx.x / G, groups = blockDim.x / G;
+    const int stride = nthreads;
+    const int gl = threadIdx.x & (G - 1), grp = threadIdx.x / G, groups = nthreads / G;
+    int32_t *ex = (int32_t *)(slots_sh + (size_t)SLOTS * nthreads);
+    int32_t *rmin = ex, *rmax = ex + LDL_STATS_MAX, *wmin = ex + 2 * LDL_STATS_MAX,
+            *wmax = ex + 3 * LDL_STATS_MAX;
+    const bool stats = n <= LDL_STATS_MAX;
+    if (stats)
+        for (int64_t i = threadIdx.x; i < n; i += nthreads) {
+            rmin[i] = wmin[i] = INT32_MAX;
+            rmax[i] = wmax[i] = INT32_MIN;
+        }
     if (threadIdx.x == 0) bad = 0;
     __syncthreads();
-    for (int64_t j = 0; j < n; j++) {
-        if (grp == 0) {
-            const int64_t fj = first[j];
-            auto fx = [&](int64_t k) { return rec_at<W>((const uint32_t *)r, tri(j) + k); };
-            auto fy = [&](int64_t k) { return rec_at<W>((const uint32_t *)w, tri(j) + k); };
-            Val<W> ajj, t, d;
-            load(rec_at<W>(a, j + j * ld), ajj);
-            bool ok = group_dot<W, G>(fj, j, gl, fx, fy, nullptr, false, slots, stride, t);
-            if (ok) {
-                add_signed(ajj, t, true, d);
-                // exact retry of a cancelled pivot (uniform over the group)
-                if (is_zero(d) && j > fj)
-                    ok = group_dot<W, G>(fj, j, gl, fx, fy, &ajj, true, slots, stride, d);
-            }
-            if (gl == 0) {
-                if (!ok) {
-                    bad = 2;  // window overflow: host path
-                } else {
-                    // DenseLeaf::pivot
-                    const int sg = sign_mode == 0 ? 1 : (sign_mode == 1 ? signs[0] : signs[B.sign_off + j]);
-                    if (!is_regular(d) && !is_zero(d)) bad = 1;
-                    if (!bad && reg_eps) {
-                        Val<W> ds = d, e, diff;
-                        if (sg < 0 && is_regular(ds)) ds.kind = -ds.kind;
-                        load(reg_eps, e);
-                        add_signed(ds, e, true, diff);
-                        if (!is_zero(diff) && diff.kind < 0) {
-                            load(reg_delta, d);
-                            if (sg < 0 && is_regular(d)) d.kind = -d.kind;
-                            atomicAdd(status + 1, 1);  // regularized count
-                        }
-                    }
-                    if (!bad && is_zero(d)) bad = 1;
-                    if (!bad) {
-                        store(rec_at<W>(dstore, j), d);
-                        Val<W> di;
-                        recip(d, di);
-                        store(rec_at<W>(dv, j), di);
-                    }
-                }
+
+    // window hint of the dot of r row i with w row j from the ranges so far
+    auto hint = [&](int64_t i, int64_t j, int32_t rmn, int32_t rmx, int32_t wmn, int32_t wmx,
+                    int64_t &hl, int64_t &hh) {
+        hl = INT64_MAX;
+        hh = INT64_MIN;
+        if (stats && rmn <= rmx && wmn <= wmx) {
+            hl = (int64_t)rmn + wmn - 64 * W;
+            hh = (int64_t)rmx + wmx - 64 * W;
+        }
+    };
+    // DenseLeaf::pivot of column j from its exact dot (group 0; lane 0 writes)
+    auto pivot = [&](int64_t j, int64_t hl, int64_t hh) {
+        const int64_t fj = first[j];
+        auto fx = [&](int64_t k) { return rec_at<W>((const uint32_t *)r, tri(j) + k); };
+        auto fy = [&](int64_t k) { return rec_at<W>((const uint32_t *)w, tri(j) + k); };
+        Val<W> ajj, t, d;
+        load(rec_at<W>(a, j + j * ld), ajj);
+        bool ok = group_dot<W, G>(fj, j, gl, fx, fy, nullptr, false, slots, stride, t, hl, hh);
+        if (ok) {
+            add_signed(ajj, t, true, d);
+            // exact retry of a cancelled pivot (uniform over the group)
+            if (is_zero(d) && j > fj)
+                ok = group_dot<W, G>(fj, j, gl, fx, fy, &ajj, true, slots, stride, d, hl, hh);
+        }
+        if (gl != 0) return;
+        if (!ok) {
+            bad = 2;  // window overflow: host path
+            return;
+        }
+        const int sg = sign_mode == 0 ? 1 : (sign_mode == 1 ? signs[0] : signs[B.sign_off + j]);
+        int fail = 0;
+        if (!is_regular(d) && !is_zero(d)) fail = 1;
+        if (!fail && reg_eps) {
+            Val<W> ds = d, e, diff;
+            if (sg < 0 && is_regular(ds)) ds.kind = -ds.kind;
+            load(reg_eps, e);
+            add_signed(ds, e, true, diff);
+            if (!is_zero(diff) && diff.kind < 0) {
+                load(reg_delta, d);
+                if (sg < 0 && is_regular(d)) d.kind = -d.kind;
+                atomicAdd(status + 1, 1);  // regularized count
             }
         }
-        __syncthreads();
-        if (bad) break;
+        if (!fail && is_zero(d)) fail = 1;
+        if (fail) {
+            bad = 1;
+            return;
+        }
+        store(rec_at<W>(dstore, j), d);
+        Val<W> di;
+        recip(d, di);
+        store(rec_at<W>(dv, j), di);
+    };
+    // r_ij, w_ij of column j (a lane group); returns them in every lane
+    auto entry = [&](int64_t i, int64_t j, const Val<W> &dj, const Val<W> &dij, Val<W> &v,
+                     Val<W> &wv) -> bool {
+        if (first[i] > j) {
+            set_zero(v);
+        } else {
+            const int64_t lo = max(first[i], first[j]);
+            int64_t hl, hh;
+            if (stats)
+                hint(i, j, rmin[i], rmax[i], wmin[j], wmax[j], hl, hh);
+            else
+                hint(i, j, 1, 0, 1, 0, hl, hh);
+            auto fx = [&](int64_t k) { return rec_at<W>((const uint32_t *)r, tri(i) + k); };
+            auto fy = [&](int64_t k) { return rec_at<W>((const uint32_t *)w, tri(j) + k); };
+            Val<W> t, aij, diff;
+            if (!group_dot<W, G>(lo, j, gl, fx, fy, nullptr, false, slots, stride, t, hl, hh))
+                return false;
+            load(rec_at<W>(a, i + j * ld), aij);
+            add_signed(aij, t, true, diff);
+            mul(diff, dij, v);
+        }
+        mul(v, dj, wv);
+        if (gl == (int)(j % G)) {
+            store(rec_at<W>(r, tri(i) + j), v);
+            store(rec_at<W>(w, tri(i) + j), wv);
+        }
+        if (stats && gl == 0) {
+            if (!is_zero(v)) {
+                rmin[i] = min(rmin[i], v.e);
+                rmax[i] = max(rmax[i], v.e);
+            }
+            if (!is_zero(wv)) {
+                wmin[i] = min(wmin[i], wv.e);
+                wmax[i] = max(wmax[i], wv.e);
+            }
+        }
+        return true;
+    };
+
+    if (grp == 0 && n > 0) pivot(0, INT64_MAX, INT64_MIN);
+    __syncthreads();
+    for (int64_t j = 0; j < n && !bad; j++) {
         Val<W> dj, dij;
         load(rec_at<W>((const uint32_t *)dstore, j), dj);
         load(rec_at<W>((const uint32_t *)dv, j), dij);
-        for (int64_t i = j + 1 + grp; i < n; i += groups) {
-            Val<W> v;
-            if (first[i] > j) {
-                set_zero(v);
-            } else {
-                const int64_t lo = max(first[i], first[j]);
-                auto fx = [&](int64_t k) { return rec_at<W>((const uint32_t *)r, tri(i) + k); };
-                auto fy = [&](int64_t k) { return rec_at<W>((const uint32_t *)w, tri(j) + k); };
-                Val<W> t, aij, diff;
-                if (!group_dot<W, G>(lo, j, gl, fx, fy, nullptr, false, slots, stride, t)) {
-                    if (gl == 0) atomicMax(&bad, 2);
-                    continue;
+        if (grp == 0) {
+            if (j + 1 < n) {
+                const int64_t i = j + 1;
+                int32_t rmn = 1, rmx = 0, wmn = 1, wmx = 0;
+                if (stats) {
+                    rmn = rmin[i];
+                    rmx = rmax[i];
+                    wmn = wmin[i];
+                    wmx = wmax[i];
                 }
-                load(rec_at<W>(a, i + j * ld), aij);
-                add_signed(aij, t, true, diff);
-                mul(diff, dij, v);
+                Val<W> v, wv;
+                if (!entry(i, j, dj, dij, v, wv)) {
+                    if (gl == 0) atomicMax(&bad, 2);
+                } else {
+                    // the pivot's range: row i so far plus the entry just made
+                    if (!is_zero(v)) {
+                        rmn = min(rmn, v.e);
+                        rmx = max(rmx, v.e);
+                    }
+                    if (!is_zero(wv)) {
+                        wmn = min(wmn, wv.e);
+                        wmx = max(wmx, wv.e);
+                    }
+                    int64_t hl, hh;
+                    hint(i, i, rmn, rmx, wmn, wmx, hl, hh);
+                    pivot(i, hl, hh);
+                }
             }
-            if (gl == 0) {
-                store(rec_at<W>(r, tri(i) + j), v);
-                Val<W> wv;
-                mul(v, dj, wv);
-                store(rec_at<W>(w, tri(i) + j), wv);
+        } else {
+            for (int64_t i = j + 2 + (grp - 1); i < n; i += groups - 1) {
+                Val<W> v, wv;
+                if (!entry(i, j, dj, dij, v, wv)) {
+                    if (gl == 0) atomicMax(&bad, 2);
+                }
             }
         }
         __syncthreads();
-        if (bad) break;
     }
     if (bad) {
         if (threadIdx.x == 0) atomicMax(status, bad);
         return;
     }
     // write the unit lower factor
-    for (int64_t e = threadIdx.x; e < n * n; e += blockDim.x) {
+    for (int64_t e = threadIdx.x; e < n * n; e += nthreads) {
         const int64_t i = e % n, j = e / n;
         if (i < j) continue;
         Val<W> v;
@@ -546,8 +640,12 @@ void mp_ldl_dots(int limbs, void *A, void *dinv, const int32_t *first, const int
     if (nblocks == 0) return;
     SDPX_DISPATCH_W(limbs, {
         auto kern = ldl_dots_kernel<W, DOT_G>;
-        const size_t lds = panel_lds<W>((const void *)kern, LDL_THREADS);
-        hipLaunchKernelGGL(kern, dim3(nblocks), dim3(LDL_THREADS), lds, st, (uint32_t *)A,
+        constexpr int threads = ldl_threads<W>();
+        const size_t lds = slot_bytes<W>(threads) + 4 * LDL_STATS_MAX * sizeof(int32_t);
+        if (lds > 65536)
+            HIP_CHECK(hipFuncSetAttribute((const void *)kern,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(kern, dim3(nblocks), dim3(threads), lds, st, (uint32_t *)A,
                            (uint32_t *)dinv, first, signs, sign_mode, (const LdlBlock *)blocks,
                            (uint32_t *)scratch, (const uint32_t *)reg_eps,
                            (const uint32_t *)reg_delta, status);

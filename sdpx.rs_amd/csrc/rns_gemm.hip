@@ -206,6 +206,164 @@ __global__ __launch_bounds__(256) void encode_kernel(
     }
 }
 
+// Encode on the INT8 matrix cores.  With X = sum_b x_b 256^b the aligned
+// magnitude of a value (B_pad bytes) and T_bl = 256^b mod q_l taken
+// symmetric and split into signed bytes T = 256 th + tl,
+//   S_l = sum_b (x_b - 128) (256 th_bl + tl_bl)  ==  X - C_l  (mod q_l),
+// C_l = 128 sum_b T_bl.  Per 32 values x 32 moduli that is two 32x32x32
+// MFMAs per 32 bytes; E_l in [2^30, 2^30 + q_l) with E_l == C_l makes
+// 256 GH + GL + E_l a positive 31-bit integer, reduced by a Barrett step
+// (floor(2^32 / q) estimate, one correction).  A workgroup encodes a tile of
+// 4 vectors x 64 positions (wave w: vector v0 + w, two 32-position M-tiles),
+// one block of 32 moduli at a time, staging the digit bytes in LDS so the
+// plane writes are 16-byte runs along k.
+constexpr int ENC_TV = 4, ENC_TP = 64;
+constexpr int ENC_ROW = ENC_TV * ENC_TP + 16;  // LDS bytes per staged plane
+
+__global__ __launch_bounds__(256) void encode_mfma_kernel(
+    const uint32_t *__restrict__ data, int n, int64_t V, int64_t K, int64_t sv, int64_t sp,
+    const int4 *__restrict__ scan, int nk, const int8_t *__restrict__ th, const int8_t *__restrict__ tl,
+    const uint32_t *__restrict__ econst, const uint32_t *__restrict__ mq, const uint32_t *__restrict__ mbar,
+    int L, int single, int8_t *__restrict__ planes, int64_t plane_stride, int64_t k_pad,
+    int64_t tiles_v, int64_t tiles_p) {
+    extern __shared__ __attribute__((aligned(16))) int8_t esm[];
+    const int bpad = 32 * nk, trow = bpad + 16;  // table row: padded for b128 reads
+    const int nblocks = (L + 31) / 32, lpad = 32 * nblocks;
+    int8_t *sth = esm, *stl = esm + lpad * trow;
+    int8_t *sout = esm + 2 * lpad * trow;  // [64 staged planes][ENC_ROW]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int h = lane >> 5, col = lane & 31;
+    const int W32 = 2 * n + 2;
+    // tables -> LDS (zero rows for padded moduli)
+    for (int e = tid; e < lpad * (bpad / 4); e += 256) {
+        const int l = e / (bpad / 4), w4 = e - l * (bpad / 4);
+        uint32_t vh = 0, vl = 0;
+        if (l < L) {
+            vh = ((const uint32_t *)th)[(size_t)l * (bpad / 4) + w4];
+            vl = ((const uint32_t *)tl)[(size_t)l * (bpad / 4) + w4];
+        }
+        *(uint32_t *)(sth + l * trow + 4 * w4) = vh;
+        *(uint32_t *)(stl + l * trow + 4 * w4) = vl;
+    }
+    __syncthreads();
+    const int64_t ntiles = tiles_v * tiles_p;
+    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int64_t v0 = (tile / tiles_p) * ENC_TV, p0 = (tile % tiles_p) * ENC_TP;
+        const int64_t v = v0 + wave;
+        // A fragments (value bytes ^ 0x80) and sign masks of the two M-tiles
+        v4i af[2][3];
+        uint32_t negmask[2];
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            const int64_t p = p0 + 32 * t + col;
+            const uint32_t *rec = nullptr;
+            int kind = KIND_ZERO, s = 0;
+            if (v < V && p < K) {
+                rec = data + (size_t)(v * sv + p * sp) * W32;
+                kind = (int)rec[2 * n];
+                if (kind == KIND_REGULAR || kind == -KIND_REGULAR)
+                    s = (int)rec[2 * n + 1] - 64 * n - scan[v].x;
+                else
+                    rec = nullptr;
+            }
+            negmask[t] = (uint32_t)__ballot(rec != nullptr && kind < 0);
+            // X = M 2^s: dword w of X is bits [32 w - s, 32 w - s + 32) of M
+            const int sw = s >= 0 ? s / 32 : -((-s + 31) / 32);  // floor(s / 32)
+            const int sb = s - 32 * sw;                            // [0, 32)
+            const int off = (32 - sb) & 31, adj = sb > 0 ? 1 : 0;
+            auto word = [&](int idx) -> uint32_t {
+                return (rec && idx >= 0 && idx < 2 * n) ? rec[idx] : 0u;
+            };
+#pragma unroll
+            for (int ks = 0; ks < 3; ks++) {
+                if (ks < nk) {
+                    const int w0 = 8 * ks + 4 * h;
+                    const int i0 = w0 - sw - adj;
+                    uint32_t m[5];
+#pragma unroll
+                    for (int u = 0; u < 5; u++) m[u] = word(i0 + u);
+                    uint32_t d[4];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) d[u] = __builtin_amdgcn_alignbit(m[u + 1], m[u], off) ^ 0x80808080u;
+                    af[t][ks] = v4i{(int)d[0], (int)d[1], (int)d[2], (int)d[3]};
+                } else {
+                    af[t][ks] = v4i{0, 0, 0, 0};
+                }
+            }
+        }
+        for (int nb = 0; nb < nblocks; nb++) {
+            const int c = 32 * nb + col;  // this lane's modulus
+            const int pfirst = 32 * nb <= single ? 32 * nb : single + 2 * (32 * nb - single);
+            const int clast = min(L, 32 * nb + 32);
+            const int pend = clast <= single ? clast : single + 2 * (clast - single);
+            uint32_t q = 1, mqc = 0, ec = 0;
+            if (c < L) {
+                q = mq[c];
+                mqc = mbar[c];
+                ec = econst[c];
+            }
+            const int8_t *bh = sth + c * trow + 16 * h, *bl = stl + c * trow + 16 * h;
+            __syncthreads();  // staged planes of the previous block are out
+#pragma unroll
+            for (int t = 0; t < 2; t++) {
+                v16i gh = v16i{0}, gl = v16i{0};
+#pragma unroll
+                for (int ks = 0; ks < 3; ks++) {
+                    if (ks < nk) {
+                        const v4i fh = *(const v4i *)(bh + 32 * ks);
+                        const v4i fl = *(const v4i *)(bl + 32 * ks);
+                        gh = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[t][ks], fh, gh, 0, 0, 0);
+                        gl = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[t][ks], fl, gl, 0, 0, 0);
+                    }
+                }
+                if (c < L) {
+                    uint32_t lo4[4] = {0, 0, 0, 0}, hi4[4] = {0, 0, 0, 0};
+#pragma unroll
+                    for (int r = 0; r < 16; r++) {
+                        const int row = 8 * (r >> 2) + 4 * h + (r & 3);
+                        const uint32_t x = (uint32_t)(256 * gh[r] + gl[r]) + ec;
+                        uint32_t rr = x - __umulhi(x, mqc) * q;
+                        if (rr >= q) rr -= q;
+                        if (((negmask[t] >> row) & 1u) && rr) rr = q - rr;
+                        int ri = (int)rr;
+                        if (ri >= (int)((q + 1) >> 1)) ri -= (int)q;
+                        const int sh = 8 * (r & 3);
+                        if (c < single) {
+                            lo4[r >> 2] |= (uint32_t)(ri & 255) << sh;
+                        } else {
+                            const int l8 = ((ri + 128) & 255) - 128;
+                            hi4[r >> 2] |= (uint32_t)(((ri - l8) >> 8) & 255) << sh;
+                            lo4[r >> 2] |= (uint32_t)(l8 & 255) << sh;
+                        }
+                    }
+                    const int lp = (c < single ? c : single + 2 * (c - single)) - pfirst;
+#pragma unroll
+                    for (int g = 0; g < 4; g++) {
+                        const int pos = 32 * t + 8 * g + 4 * h;
+                        if (c < single) {
+                            *(uint32_t *)(sout + lp * ENC_ROW + wave * ENC_TP + pos) = lo4[g];
+                        } else {
+                            *(uint32_t *)(sout + lp * ENC_ROW + wave * ENC_TP + pos) = hi4[g];
+                            *(uint32_t *)(sout + (lp + 1) * ENC_ROW + wave * ENC_TP + pos) = lo4[g];
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            // staged planes [pfirst, pend) -> HBM, 16-byte runs along k
+            const int chunks = (pend - pfirst) * ENC_TV * (ENC_TP / 16);
+            for (int e = tid; e < chunks; e += 256) {
+                const int lp = e / (ENC_TV * 4), rest = e - lp * ENC_TV * 4;
+                const int tv = rest >> 2, part = rest & 3;
+                const v4i val = *(const v4i *)(sout + lp * ENC_ROW + tv * ENC_TP + 16 * part);
+                *(v4i *)(planes + (size_t)(pfirst + lp) * plane_stride + (size_t)(v0 + tv) * k_pad +
+                         p0 + 16 * part) = val;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------------
 // 3. product
 // ---------------------------------------------------------------------------
@@ -215,6 +373,22 @@ __device__ __forceinline__ int mod_q(double v, double qd, double qinv) {
     if (r < 0) r += qd;
     if (r >= qd) r -= qd;
     return (int)r;
+}
+
+// x_l in [0, q) is stored as its symmetric representative x' (x' == x mod
+// q, -q/2 < x' <= q/2) in balanced base-256 digits, lo = low byte, hi = high
+// byte, x' = 256 hi + lo with both digits in [-128, 127]: the CRT's matrix
+// product consumes the two bytes directly as INT8 operands.
+__device__ __forceinline__ uint16_t x_encode(int x, int q) {
+    const int xs = x >= ((q + 1) >> 1) ? x - q : x;
+    const int lo = ((xs + 128) & 255) - 128;
+    const int hi = (xs - lo) >> 8;
+    return (uint16_t)((lo & 255) | ((hi & 255) << 8));
+}
+// back to [0, q)
+__device__ __forceinline__ int x_decode(uint16_t u, int q) {
+    const int x = (int)(int8_t)(u & 255) + 256 * (int)(int8_t)(u >> 8);
+    return x < 0 ? x + q : x;
 }
 
 // One modulus x one 128x128 output tile per workgroup; 4 waves in 2x2, each
@@ -355,7 +529,7 @@ __global__ __launch_bounds__(256) void product_kernel(
                 int x = mod_q((double)res * inv, qd, qinv);
                 int row = wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
                 int col = wn * 64 + j * 32 + (lane & 31);
-                sX[col * XPAD + row] = (uint16_t)x;
+                sX[col * XPAD + row] = x_encode(x, q);
             }
     __syncthreads();
     uint16_t *gX = X + ((size_t)(l * tiles_n + tn) * tiles_m + tm) * (TILE * TILE);
@@ -371,11 +545,10 @@ __global__ __launch_bounds__(256) void product_kernel(
             uint32_t nw[4] = {w0, w1, w2, w3};
 #pragma unroll
             for (int t = 0; t < 4; t++) {
-                uint32_t lo = (nw[t] & 0xFFFF) + (o[t] & 0xFFFF);
-                uint32_t hi = (nw[t] >> 16) + (o[t] >> 16);
-                if (lo >= (uint32_t)q) lo -= q;
-                if (hi >= (uint32_t)q) hi -= q;
-                nw[t] = lo | (hi << 16);
+                const int lo = x_decode((uint16_t)(nw[t] & 0xFFFF), q) + x_decode((uint16_t)(o[t] & 0xFFFF), q);
+                const int hi = x_decode((uint16_t)(nw[t] >> 16), q) + x_decode((uint16_t)(o[t] >> 16), q);
+                nw[t] = (uint32_t)x_encode(lo >= q ? lo - q : lo, q) |
+                        ((uint32_t)x_encode(hi >= q ? hi - q : hi, q) << 16);
             }
             w0 = nw[0];
             w1 = nw[1];
@@ -393,7 +566,8 @@ template <int JMAX, int BS>
 __global__ __launch_bounds__(BS) void crt_kernel(
     const uint16_t *__restrict__ X, int L, const int2 *__restrict__ tiles, int tiles_m,
     int tiles_n, const double *__restrict__ C, const double *__restrict__ Mt,
-    const double *__restrict__ qinv, int J, const int4 *__restrict__ scan_a,
+    const double *__restrict__ qinv, const uint32_t *__restrict__ mq, int J,
+    const int4 *__restrict__ scan_a,
     const int4 *__restrict__ scan_b, int64_t m, int64_t nn, int upper, int packed, int n,
     uint32_t *__restrict__ out) {
     __shared__ uint32_t limbs_s[JMAX * BS];
@@ -411,7 +585,7 @@ __global__ __launch_bounds__(BS) void crt_kernel(
     for (int t = 0; t < JMAX; t++) acc[t] = 0.0;
     double f = 0.0;
     for (int l = 0; l < L; l++) {
-        const double x = (double)xp[(size_t)l * xstride];
+        const double x = (double)x_decode(xp[(size_t)l * xstride], (int)mq[l]);
         f = fma(x, qinv[l], f);
         const double *cl = C + (size_t)l * J;
 #pragma unroll
@@ -530,7 +704,7 @@ namespace {
 
 enum Slot {
     S_SCAN_A = 0, S_SCAN_B, S_PLANES_A, S_PLANES_B, S_X, S_POW_A, S_POW_B, S_MODS, S_CRT,
-    S_TILES, S_COUNT
+    S_TILES, S_ENC_A, S_ENC_B, S_COUNT
 };
 
 struct Operand {
@@ -599,6 +773,67 @@ void launch_encode(const Operand &op, int n, const int4 *scan, int TJ, const dou
                             k_pad, v_pad, s);
 }
 
+// Tables of the matrix-core encode for inner byte count 32 nk: th, tl
+// (count x 32 nk signed bytes), E_l, q_l, floor(2^32 / q_l).  Laid out in one
+// buffer: th | tl | E | q | mbar.
+struct EncodeTables {
+    std::vector<uint8_t> bytes;
+    size_t th = 0, tl = 0, e = 0, q = 0, mbar = 0;
+};
+
+EncodeTables encode_tables(const RnsPlan &plan, int nk) {
+    const int L = plan.count, bpad = 32 * nk;
+    EncodeTables t;
+    t.th = 0;
+    t.tl = (size_t)L * bpad;
+    t.e = 2 * (size_t)L * bpad;
+    t.q = t.e + 4 * (size_t)L;
+    t.mbar = t.q + 4 * (size_t)L;
+    t.bytes.assign(t.mbar + 4 * (size_t)L, 0);
+    for (int l = 0; l < L; l++) {
+        const int64_t q = plan.q[l];
+        int64_t pw = 1 % q, sum = 0;
+        for (int b = 0; b < bpad; b++) {
+            const int64_t sym = pw >= (q + 1) / 2 ? pw - q : pw;
+            const int64_t lo = ((sym + 128) & 255) - 128, hi = (sym - lo) / 256;
+            t.bytes[t.th + (size_t)l * bpad + b] = (uint8_t)(int8_t)hi;
+            t.bytes[t.tl + (size_t)l * bpad + b] = (uint8_t)(int8_t)lo;
+            sum += sym;
+            pw = pw * 256 % q;
+        }
+        const int64_t c = 128 * sum, base = int64_t(1) << 30;
+        const uint32_t e = (uint32_t)(base + (((c - base) % q) + q) % q);
+        const uint32_t qq = (uint32_t)q, mb = (uint32_t)((uint64_t(1) << 32) / (uint64_t)q);
+        std::memcpy(&t.bytes[t.e + 4 * (size_t)l], &e, 4);
+        std::memcpy(&t.bytes[t.q + 4 * (size_t)l], &qq, 4);
+        std::memcpy(&t.bytes[t.mbar + 4 * (size_t)l], &mb, 4);
+    }
+    return t;
+}
+
+// Inner bytes the matrix-core encode handles: 3 MFMA k-steps (768 bits)
+// keep every partial sum inside 31 bits.
+constexpr int ENC_MAX_NK = 3;
+
+void launch_encode_mfma(const Operand &op, int n, const int4 *scan, int nk, const uint8_t *d_tab,
+                        const EncodeTables &t, int L, int single, int8_t *planes,
+                        int64_t plane_stride, int64_t k_pad, int64_t v_pad, hipStream_t s) {
+    const int lpad = (L + 31) / 32 * 32, trow = 32 * nk + 16;
+    const size_t lds = 2 * (size_t)lpad * trow + 64 * (size_t)ENC_ROW;
+    if (lds > 65536)
+        HIP_CHECK(hipFuncSetAttribute((const void *)encode_mfma_kernel,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const int64_t tiles_v = v_pad / ENC_TV, tiles_p = k_pad / ENC_TP;
+    const int64_t tiles = tiles_v * tiles_p;
+    const unsigned grid = (unsigned)std::min<int64_t>(tiles, 2048);
+    hipLaunchKernelGGL(encode_mfma_kernel, dim3(grid), dim3(256), lds, s, op.data, n, op.V, op.K,
+                       op.sv, op.sp, scan, nk, (const int8_t *)(d_tab + t.th),
+                       (const int8_t *)(d_tab + t.tl), (const uint32_t *)(d_tab + t.e),
+                       (const uint32_t *)(d_tab + t.q), (const uint32_t *)(d_tab + t.mbar), L,
+                       single, planes, plane_stride, k_pad, tiles_v, tiles_p);
+    HIP_CHECK(hipGetLastError());
+}
+
 // 2^(24 j) mod q_l for j < TJ, as doubles (count x TJ).
 std::vector<double> pow2_table(const RnsPlan &plan, int TJ) {
     std::vector<double> t((size_t)plan.count * TJ);
@@ -614,6 +849,7 @@ std::vector<double> pow2_table(const RnsPlan &plan, int TJ) {
 
 thread_local RnsStats t_stats;
 bool g_profile = false;
+bool g_encode_mfma = true;  // matrix-core encode where the window allows
 
 // Events bracketing the four stages (only when profiling).
 struct StageEvents {
@@ -730,11 +966,34 @@ int rns_gemm_device(const RnsGemmArgs &g) {
     HIP_CHECK(hipMemcpyAsync(d_q, plan->q.data(), 4 * L, hipMemcpyHostToDevice, s));
     HIP_CHECK(hipMemcpyAsync(d_inv, plan->inv.data(), 4 * L, hipMemcpyHostToDevice, s));
     HIP_CHECK(hipMemcpyAsync(d_qinv, plan->qinv.data(), 8 * L, hipMemcpyHostToDevice, s));
-    std::vector<double> pa = pow2_table(*plan, TJa), pb = pow2_table(*plan, TJb);
-    double *d_pa = (double *)ws.get(S_POW_A, 8 * pa.size());
-    double *d_pb = (double *)ws.get(S_POW_B, 8 * pb.size());
-    HIP_CHECK(hipMemcpyAsync(d_pa, pa.data(), 8 * pa.size(), hipMemcpyHostToDevice, s));
-    HIP_CHECK(hipMemcpyAsync(d_pb, pb.data(), 8 * pb.size(), hipMemcpyHostToDevice, s));
+    // encode tables: the matrix-core form when the window fits 3 k-steps,
+    // else 24-bit chunk powers for the VALU form
+    const int nka = (int)(((wa.width + 7) / 8 + 31) / 32), nkb = (int)(((wb.width + 7) / 8 + 31) / 32);
+    const bool mfma_a = g_encode_mfma && nka <= ENC_MAX_NK, mfma_b = g_encode_mfma && nkb <= ENC_MAX_NK;
+    EncodeTables eta, etb;
+    uint8_t *d_eta = nullptr, *d_etb = nullptr;
+    double *d_pa = nullptr, *d_pb = nullptr;
+    std::vector<double> pa, pb;
+    if (mfma_a) {
+        eta = encode_tables(*plan, nka);
+        d_eta = (uint8_t *)ws.get(S_ENC_A, eta.bytes.size());
+        HIP_CHECK(hipMemcpyAsync(d_eta, eta.bytes.data(), eta.bytes.size(), hipMemcpyHostToDevice, s));
+    } else {
+        pa = pow2_table(*plan, TJa);
+        d_pa = (double *)ws.get(S_POW_A, 8 * pa.size());
+        HIP_CHECK(hipMemcpyAsync(d_pa, pa.data(), 8 * pa.size(), hipMemcpyHostToDevice, s));
+    }
+    if (!same) {
+        if (mfma_b) {
+            etb = encode_tables(*plan, nkb);
+            d_etb = (uint8_t *)ws.get(S_ENC_B, etb.bytes.size());
+            HIP_CHECK(hipMemcpyAsync(d_etb, etb.bytes.data(), etb.bytes.size(), hipMemcpyHostToDevice, s));
+        } else {
+            pb = pow2_table(*plan, TJb);
+            d_pb = (double *)ws.get(S_POW_B, 8 * pb.size());
+            HIP_CHECK(hipMemcpyAsync(d_pb, pb.data(), 8 * pb.size(), hipMemcpyHostToDevice, s));
+        }
+    }
     const int J = plan->limbs;
     double *d_crt = (double *)ws.get(S_CRT, 8 * ((size_t)L * J + J));
     HIP_CHECK(hipMemcpyAsync(d_crt, plan->c.data(), 8 * (size_t)L * J, hipMemcpyHostToDevice, s));
@@ -748,10 +1007,17 @@ int rns_gemm_device(const RnsGemmArgs &g) {
     const int64_t a_plane = m_pad * k_pad, b_plane = n_pad * k_pad;
     int8_t *pla = (int8_t *)ws.get(S_PLANES_A, (size_t)G * a_plane);
     int8_t *plb = same ? pla : (int8_t *)ws.get(S_PLANES_B, (size_t)G * b_plane);
-    launch_encode(oa, n, scan_a, TJa, d_pa, d_q, d_qinv, L, single, pla, a_plane, k_pad, m_pad, s);
-    if (!same)
-        launch_encode(ob, n, scan_b, TJb, d_pb, d_q, d_qinv, L, single, plb, b_plane, k_pad, n_pad,
-                      s);
+    if (mfma_a)
+        launch_encode_mfma(oa, n, scan_a, nka, d_eta, eta, L, single, pla, a_plane, k_pad, m_pad, s);
+    else
+        launch_encode(oa, n, scan_a, TJa, d_pa, d_q, d_qinv, L, single, pla, a_plane, k_pad, m_pad, s);
+    if (!same) {
+        if (mfma_b)
+            launch_encode_mfma(ob, n, scan_b, nkb, d_etb, etb, L, single, plb, b_plane, k_pad, n_pad, s);
+        else
+            launch_encode(ob, n, scan_b, TJb, d_pb, d_q, d_qinv, L, single, plb, b_plane, k_pad,
+                          n_pad, s);
+    }
 
     // 3. products, in exact INT32 chunks of the inner dimension
     sev.mark(2, s);
@@ -782,15 +1048,15 @@ int rns_gemm_device(const RnsGemmArgs &g) {
     uint32_t *outw = (uint32_t *)g.out;
     if (J <= 32)
         hipLaunchKernelGGL((crt_kernel<32, 256>), dim3(TILE * TILE / 256, ntiles), dim3(256), 0, s,
-                           X, L, d_tiles, tiles_m, tiles_n, d_c, d_m, d_qinv, J, scan_a, scan_b,
+                           X, L, d_tiles, tiles_m, tiles_n, d_c, d_m, d_qinv, d_q, J, scan_a, scan_b,
                            m, nn, up, pk, n, outw);
     else if (J <= 64)
         hipLaunchKernelGGL((crt_kernel<64, 256>), dim3(TILE * TILE / 256, ntiles), dim3(256), 0, s,
-                           X, L, d_tiles, tiles_m, tiles_n, d_c, d_m, d_qinv, J, scan_a, scan_b,
+                           X, L, d_tiles, tiles_m, tiles_n, d_c, d_m, d_qinv, d_q, J, scan_a, scan_b,
                            m, nn, up, pk, n, outw);
     else
         hipLaunchKernelGGL((crt_kernel<128, 64>), dim3(TILE * TILE / 64, ntiles), dim3(64), 0, s,
-                           X, L, d_tiles, tiles_m, tiles_n, d_c, d_m, d_qinv, J, scan_a, scan_b,
+                           X, L, d_tiles, tiles_m, tiles_n, d_c, d_m, d_qinv, d_q, J, scan_a, scan_b,
                            m, nn, up, pk, n, outw);
     HIP_CHECK(hipGetLastError());
     sev.mark(4, s);

@@ -50,7 +50,9 @@ def test_golden_cases(gpu):
 
 @pytest.mark.parametrize("nl,m,n,k,spread", [
     (4, 300, 257, 1000, 60), (8, 129, 130, 131, 200), (2, 64, 200, 513, 8),
-    (12, 40, 33, 77, 90), (16, 35, 37, 45, 90), (1, 128, 128, 128, 3)])
+    (12, 40, 33, 77, 90), (16, 35, 37, 45, 90), (1, 128, 128, 128, 3),
+    # operand windows just inside / outside the 768-bit matrix-core encode
+    (8, 64, 65, 300, 120), (8, 64, 65, 300, 130)])
 def test_random_vs_oracle(gpu, nl, m, n, k, spread):
     rng = np.random.default_rng(nl * 1000 + m)
     for ta, tb in TRANS:
