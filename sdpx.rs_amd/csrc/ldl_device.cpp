@@ -105,7 +105,9 @@ void solve_t_device(DCtx &c, size_t m, size_t n, uint8_t *L, size_t ldl, uint8_t
     if (n <= 128) {
         size_t mark = c.arena->top;
         int32_t *d_low = upload_ints(c, low);
-        mp_trsm_base(c.n, L, ldl, n, B, ldb, m, d_low, c.d_status + 2, c.st);
+        void *ranges = c.arena->alloc(8 * n);
+        mp_exp_ranges(c.n, L, ldl, n, 0, ranges, c.st);
+        mp_trsm_base(c.n, L, ldl, n, B, ldb, m, d_low, ranges, c.d_status + 2, c.st);
         HIP_CHECK(hipStreamSynchronize(c.st));
         c.arena->top = mark;
         c.stats->base_rows += m;
@@ -134,7 +136,11 @@ void solve_t_device(DCtx &c, size_t m, size_t n, uint8_t *L, size_t ldl, uint8_t
 void solve_n_device(DCtx &c, size_t m, size_t n, uint8_t *L, size_t ldl, uint8_t *B, size_t ldb) {
     if (m == 0 || n == 0 || c.declined) return;
     if (n <= 128) {
-        mp_trsm_base_n(c.n, L, ldl, n, B, ldb, m, c.d_status + 2, c.st);
+        // the ranges buffer is released at the caller's next arena reset,
+        // after this launch in stream order
+        void *ranges = c.arena->alloc(8 * n);
+        mp_exp_ranges(c.n, L, ldl, n, 1, ranges, c.st);
+        mp_trsm_base_n(c.n, L, ldl, n, B, ldb, m, ranges, c.d_status + 2, c.st);
         return;
     }
     const size_t n1 = n / 2, n2 = n - n / 2;
@@ -389,7 +395,9 @@ int sampled_arrow_device(int limbs, const std::vector<SampledLeafDev> &leaves, s
             int32_t *d_f = upload_ints(c, fl[i]);
             uint8_t *Y = c.at(d_y_all, yoff[i]);
             HIP_CHECK(hipMemcpyAsync(Y, lf.b, lf.k * t * vb, hipMemcpyDeviceToDevice, s));
-            mp_forward_rows(limbs, c.at(d_l_all, loff[i]), d_f, (int64_t)lf.k, Y, (int64_t)t,
+            void *ranges = c.arena->alloc(8 * lf.k);
+            mp_exp_ranges(limbs, c.at(d_l_all, loff[i]), (int64_t)lf.k, (int64_t)lf.k, 0, ranges, s);
+            mp_forward_rows(limbs, c.at(d_l_all, loff[i]), d_f, (int64_t)lf.k, Y, (int64_t)t, ranges,
                             c.d_status + 2, s);
             uint8_t *Z = (uint8_t *)c.arena->alloc(lf.k * t * vb);
             mp_ew_mul_diag(limbs, Y, lf.k, c.at(d_dinv_all, doff[i]), Z, lf.k, lf.k, t, true, s);

@@ -20,16 +20,22 @@ void mp_ew_mul_diag(int limbs, const void *Y, int64_t ldy, const void *D, void *
                     int64_t m, int64_t n, bool per_row, hipStream_t st);
 void mp_sampled_entries(int limbs, const void *G, const void *w, int64_t K, void *H,
                         hipStream_t st);
+// Exponent ranges (int2 min, max per row/column) of L's strictly lower
+// triangle: mode 0 rows (k < j), mode 1 columns (k > j).  The row kernels
+// take them (optional, may be null) as window hints; results never depend
+// on them.
+void mp_exp_ranges(int limbs, const void *L, int64_t ldl, int64_t n, int mode, void *out,
+                   hipStream_t st);
 void mp_forward_rows(int limbs, const void *L, const int32_t *first, int64_t K, void *Y, int64_t t,
-                     int *fail, hipStream_t st);
+                     const void *lranges, int *fail, hipStream_t st);
 void mp_trsm_base(int limbs, const void *L, int64_t ldl, int64_t n, void *B, int64_t ldb,
-                  int64_t m, const int32_t *low, int *fail, hipStream_t st);
+                  int64_t m, const int32_t *low, const void *lranges, int *fail, hipStream_t st);
 // status[0] = 0 ok, 1 failed pivot, 2 window overflow; status[1] += regularized
 void mp_ldl_dots(int limbs, void *A, void *dinv, const int32_t *first, const int8_t *signs,
                  int sign_mode, const void *blocks, int nblocks, void *scratch, const void *reg_eps,
                  const void *reg_delta, int *status, hipStream_t st);
 void mp_trsm_base_n(int limbs, const void *L, int64_t ldl, int64_t n, void *B, int64_t ldb,
-                    int64_t m, int *fail, hipStream_t st);
+                    int64_t m, const void *lranges, int *fail, hipStream_t st);
 void mp_transpose(int limbs, const void *in, int64_t ldi, int64_t m, int64_t n, void *out,
                   int64_t ldo, hipStream_t st);
 void mp_envelope(int limbs, const void *A, int64_t ld, int64_t n, int32_t *first, hipStream_t st);

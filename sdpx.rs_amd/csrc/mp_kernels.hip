@@ -206,9 +206,44 @@ constexpr int ROW_THREADS = 64;
 // Y = L^-1 B per column (forward_suffix_rows): L K x K lower (ld K),
 // first[K] the row envelope, Y K x t (ld K) in place.  One lane group per
 // column; y_i is stored by lane i % G, the only lane that reads it back.
+// window hint from two exponent ranges (min > max: empty)
+template <int W>
+__device__ __forceinline__ void range_hint(int32_t amn, int32_t amx, int32_t bmn, int32_t bmx,
+                                           int64_t &hl, int64_t &hh) {
+    hl = INT64_MAX;
+    hh = INT64_MIN;
+    if (amn <= amx && bmn <= bmx) {
+        hl = (int64_t)amn + bmn - 64 * W;
+        hh = (int64_t)amx + bmx - 64 * W;
+    }
+}
+__device__ __forceinline__ void range_add(int32_t &mn, int32_t &mx, int32_t kind, int32_t e) {
+    if (kind != mpd::KIND_ZERO && kind != -mpd::KIND_ZERO) {
+        mn = min(mn, e);
+        mx = max(mx, e);
+    }
+}
+
+// exponent range (min, max) of the non-zero entries of each row of the
+// strictly lower triangle of L (mode 0: row j over columns k < j) or of each
+// column (mode 1: column j over rows k > j); empty gives (INT32_MAX, INT32_MIN)
+template <int W>
+__global__ void exp_ranges_kernel(const uint32_t *L, int64_t ldl, int64_t n, int mode, int2 *out) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    int32_t mn = INT32_MAX, mx = INT32_MIN;
+    const int64_t k0 = mode == 0 ? 0 : j + 1, k1 = mode == 0 ? j : n;
+    for (int64_t k = k0; k < k1; k++) {
+        const uint32_t *rec = rec_at<W>(L, mode == 0 ? j + k * ldl : k + j * ldl);
+        range_add(mn, mx, (int32_t)rec[W], (int32_t)rec[W + 1]);
+    }
+    out[j] = make_int2(mn, mx);
+}
+
 template <int W, int G>
 __global__ __launch_bounds__(ROW_THREADS) void forward_rows_kernel(
-    const uint32_t *L, const int32_t *first, int64_t K, uint32_t *Y, int64_t t, int *fail) {
+    const uint32_t *L, const int32_t *first, int64_t K, uint32_t *Y, int64_t t, const int2 *lr,
+    int *fail) {
     extern __shared__ int64_t slots_sh[];
     const int gl = threadIdx.x & (G - 1);
     const int64_t c = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
@@ -216,16 +251,20 @@ __global__ __launch_bounds__(ROW_THREADS) void forward_rows_kernel(
     int64_t *slots = slots_sh + threadIdx.x;
     const int stride = blockDim.x;
     uint32_t *col = Y + c * K * (W + 2);
+    int32_t ymn = INT32_MAX, ymx = INT32_MIN;  // y_k so far
     for (int64_t i = 0; i < K; i++) {
         Val<W> b, r;
         load(rec_at<W>(col, i), b);
         // y_i = RN(b_i - sum_{k in [first_i, i)} L_ik y_k)
         auto fx = [&](int64_t k) { return rec_at<W>(L, i + k * K); };
         auto fy = [&](int64_t k) { return rec_at<W>((const uint32_t *)col, k); };
-        if (!group_dot<W, G>(first[i], i, gl, fx, fy, &b, true, slots, stride, r)) {
+        int64_t hl = INT64_MAX, hh = INT64_MIN;
+        if (lr) range_hint<W>(lr[i].x, lr[i].y, ymn, ymx, hl, hh);
+        if (!group_dot<W, G>(first[i], i, gl, fx, fy, &b, true, slots, stride, r, hl, hh)) {
             if (gl == 0) atomicOr(fail, 1);
             return;
         }
+        range_add(ymn, ymx, r.kind, r.e);
         if (gl == (int)(i % G)) store(rec_at<W>(col, i), r);
     }
 }
@@ -235,7 +274,7 @@ __global__ __launch_bounds__(ROW_THREADS) void forward_rows_kernel(
 template <int W, int G>
 __global__ __launch_bounds__(ROW_THREADS) void trsm_base_kernel(
     const uint32_t *L, int64_t ldl, int64_t n, uint32_t *B, int64_t ldb, int64_t m,
-    const int32_t *low, int *fail) {
+    const int32_t *low, const int2 *lr, int *fail) {
     extern __shared__ int64_t slots_sh[];
     const int gl = threadIdx.x & (G - 1);
     const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
@@ -243,20 +282,21 @@ __global__ __launch_bounds__(ROW_THREADS) void trsm_base_kernel(
     int64_t *slots = slots_sh + threadIdx.x;
     const int stride = blockDim.x;
     const int64_t lo = low[i];
+    int32_t xmn = INT32_MAX, xmx = INT32_MIN;  // x_ik so far
     for (int64_t j = lo; j < n; j++) {
         auto fx = [&](int64_t k) { return rec_at<W>((const uint32_t *)B, i + k * ldb); };
         auto fy = [&](int64_t k) { return rec_at<W>(L, j + k * ldl); };
-        Val<W> t;
-        if (!group_dot<W, G>(lo, j, gl, fx, fy, nullptr, false, slots, stride, t)) {
+        Val<W> t, x, r;
+        int64_t hl = INT64_MAX, hh = INT64_MIN;
+        if (lr) range_hint<W>(xmn, xmx, lr[j].x, lr[j].y, hl, hh);
+        if (!group_dot<W, G>(lo, j, gl, fx, fy, nullptr, false, slots, stride, t, hl, hh)) {
             if (gl == 0) atomicOr(fail, 1);
             return;
         }
-        if (gl == (int)(j % G)) {
-            Val<W> x, r;
-            load(rec_at<W>(B, i + j * ldb), x);
-            add_signed(x, t, true, r);
-            store(rec_at<W>(B, i + j * ldb), r);
-        }
+        load(rec_at<W>(B, i + j * ldb), x);
+        add_signed(x, t, true, r);
+        range_add(xmn, xmx, r.kind, r.e);
+        if (gl == (int)(j % G)) store(rec_at<W>(B, i + j * ldb), r);
     }
 }
 
@@ -265,27 +305,29 @@ __global__ __launch_bounds__(ROW_THREADS) void trsm_base_kernel(
 // per row.
 template <int W, int G>
 __global__ __launch_bounds__(ROW_THREADS) void trsm_base_n_kernel(
-    const uint32_t *L, int64_t ldl, int64_t n, uint32_t *B, int64_t ldb, int64_t m, int *fail) {
+    const uint32_t *L, int64_t ldl, int64_t n, uint32_t *B, int64_t ldb, int64_t m,
+    const int2 *lc, int *fail) {
     extern __shared__ int64_t slots_sh[];
     const int gl = threadIdx.x & (G - 1);
     const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
     if (i >= m) return;
     int64_t *slots = slots_sh + threadIdx.x;
     const int stride = blockDim.x;
+    int32_t xmn = INT32_MAX, xmx = INT32_MIN;  // x_ik so far (k > j)
     for (int64_t j = n - 1; j >= 0; j--) {
         auto fx = [&](int64_t k) { return rec_at<W>((const uint32_t *)B, i + k * ldb); };
         auto fy = [&](int64_t k) { return rec_at<W>(L, k + j * ldl); };
-        Val<W> t;
-        if (!group_dot<W, G>(j + 1, n, gl, fx, fy, nullptr, false, slots, stride, t)) {
+        Val<W> t, x, r;
+        int64_t hl = INT64_MAX, hh = INT64_MIN;
+        if (lc) range_hint<W>(xmn, xmx, lc[j].x, lc[j].y, hl, hh);
+        if (!group_dot<W, G>(j + 1, n, gl, fx, fy, nullptr, false, slots, stride, t, hl, hh)) {
             if (gl == 0) atomicOr(fail, 1);
             return;
         }
-        if (gl == (int)(j % G)) {
-            Val<W> x, r;
-            load(rec_at<W>(B, i + j * ldb), x);
-            add_signed(x, t, true, r);
-            store(rec_at<W>(B, i + j * ldb), r);
-        }
+        load(rec_at<W>(B, i + j * ldb), x);
+        add_signed(x, t, true, r);
+        range_add(xmn, xmx, r.kind, r.e);
+        if (gl == (int)(j % G)) store(rec_at<W>(B, i + j * ldb), r);
     }
 }
 
@@ -608,28 +650,36 @@ void mp_sampled_entries(int limbs, const void *G, const void *w, int64_t K, void
     HIP_CHECK(hipGetLastError());
 }
 
+void mp_exp_ranges(int limbs, const void *L, int64_t ldl, int64_t n, int mode, void *out,
+                   hipStream_t st) {
+    if (n == 0) return;
+    SDPX_DISPATCH_W(limbs, hipLaunchKernelGGL(exp_ranges_kernel<W>, dim3(blocks_for(n, 64)), dim3(64),
+                                              0, st, (const uint32_t *)L, ldl, n, mode, (int2 *)out));
+    HIP_CHECK(hipGetLastError());
+}
+
 void mp_forward_rows(int limbs, const void *L, const int32_t *first, int64_t K, void *Y, int64_t t,
-                     int *fail, hipStream_t st) {
+                     const void *lranges, int *fail, hipStream_t st) {
     if (K == 0 || t == 0) return;
     SDPX_DISPATCH_W(limbs, {
         auto kern = forward_rows_kernel<W, DOT_G>;
         const size_t lds = panel_lds<W>((const void *)kern, ROW_THREADS);
         hipLaunchKernelGGL(kern, dim3(blocks_for(t * DOT_G, ROW_THREADS)),
                            dim3(ROW_THREADS), lds, st, (const uint32_t *)L, first, K,
-                           (uint32_t *)Y, t, fail);
+                           (uint32_t *)Y, t, (const int2 *)lranges, fail);
     });
     HIP_CHECK(hipGetLastError());
 }
 
 void mp_trsm_base(int limbs, const void *L, int64_t ldl, int64_t n, void *B, int64_t ldb,
-                  int64_t m, const int32_t *low, int *fail, hipStream_t st) {
+                  int64_t m, const int32_t *low, const void *lranges, int *fail, hipStream_t st) {
     if (m == 0 || n == 0) return;
     SDPX_DISPATCH_W(limbs, {
         auto kern = trsm_base_kernel<W, DOT_G>;
         const size_t lds = panel_lds<W>((const void *)kern, ROW_THREADS);
         hipLaunchKernelGGL(kern, dim3(blocks_for(m * DOT_G, ROW_THREADS)),
                            dim3(ROW_THREADS), lds, st, (const uint32_t *)L, ldl, n,
-                           (uint32_t *)B, ldb, m, low, fail);
+                           (uint32_t *)B, ldb, m, low, (const int2 *)lranges, fail);
     });
     HIP_CHECK(hipGetLastError());
 }
@@ -670,14 +720,14 @@ void mp_envelope(int limbs, const void *A, int64_t ld, int64_t n, int32_t *first
 }
 
 void mp_trsm_base_n(int limbs, const void *L, int64_t ldl, int64_t n, void *B, int64_t ldb,
-                    int64_t m, int *fail, hipStream_t st) {
+                    int64_t m, const void *lranges, int *fail, hipStream_t st) {
     if (m == 0 || n == 0) return;
     SDPX_DISPATCH_W(limbs, {
         auto kern = trsm_base_n_kernel<W, DOT_G>;
         const size_t lds = panel_lds<W>((const void *)kern, ROW_THREADS);
         hipLaunchKernelGGL(kern, dim3(blocks_for(m * DOT_G, ROW_THREADS)),
                            dim3(ROW_THREADS), lds, st, (const uint32_t *)L, ldl, n,
-                           (uint32_t *)B, ldb, m, fail);
+                           (uint32_t *)B, ldb, m, (const int2 *)lranges, fail);
     });
     HIP_CHECK(hipGetLastError());
 }

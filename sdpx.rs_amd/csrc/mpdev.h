@@ -351,8 +351,11 @@ struct DotWindow {
     }
 };
 
-// RN(1 / d) by restoring division of 2^(2P+1) by the mantissa, quotient
-// bits plus a sticky bit (matches the host library's reciprocal).
+// RN(1 / d): Q = floor(2^(2P+1) / D) for the P-bit mantissa D (top bit set)
+// and the remainder's sticky bit, then one rounding of (Q << 1 | sticky).
+// Q comes from schoolbook division in base 2^32 (Knuth, TAOCP vol. 2,
+// 4.3.1, algorithm D: two-word quotient-digit estimate, at most two
+// corrections, multiply-subtract, add back), W + 2 digit steps.
 template <int W>
 __host__ __device__ inline void recip(const Val<W> &d, Val<W> &out) {
     if (is_zero(d)) {
@@ -361,44 +364,55 @@ __host__ __device__ inline void recip(const Val<W> &d, Val<W> &out) {
         return;
     }
     const int P = 32 * W;
-    uint32_t rem[W + 1];
+    constexpr int QW = W + 2;  // quotient digits
+    // numerator 2^(2P+1): digit 2W holds 2; one spare top digit
+    uint32_t u[2 * W + 2];
 #pragma unroll
-    for (int i = 0; i <= W; i++) rem[i] = 0;
-    constexpr int QW = W + 2;  // quotient words (P + 3 bits suffice)
+    for (int i = 0; i < 2 * W + 2; i++) u[i] = 0;
+    u[2 * W] = 2;
+    const uint32_t *v = d.m;
+    const uint64_t vt = v[W - 1], vs = W >= 2 ? v[W - 2] : 0;
     uint32_t q[QW + 1];
+    q[QW] = 0;
 #pragma unroll
-    for (int i = 0; i <= QW; i++) q[i] = 0;
-    for (int bit = 2 * P + 1; bit >= 0; bit--) {
-        uint32_t carry = (bit == 2 * P + 1) ? 1u : 0u;
-#pragma unroll
-        for (int i = 0; i <= W; i++) {
-            const uint32_t v = rem[i];
-            rem[i] = (v << 1) | carry;
-            carry = v >> 31;
+    for (int j = QW - 1; j >= 0; j--) {
+        const uint64_t num = ((uint64_t)u[j + W] << 32) | u[j + W - 1];
+        uint64_t qh = num / vt, rh = num % vt;
+        const uint64_t ulow = (W >= 2 && j + W - 2 >= 0) ? u[j + W - 2 >= 0 ? j + W - 2 : 0] : 0;
+        while (qh >> 32 || qh * vs > ((rh << 32) | ulow)) {
+            qh--;
+            rh += vt;
+            if (rh >> 32) break;
         }
-        int c = rem[W] ? 1 : 0;
+        // u[j .. j+W] -= qh * v
+        uint64_t mc = 0;
+        int64_t br = 0;
 #pragma unroll
-        for (int i = W - 1; i >= 0; i--)
-            if (c == 0 && rem[i] != d.m[i]) c = rem[i] > d.m[i] ? 1 : -1;
-        if (c >= 0) {
-            uint64_t borrow = 0;
-#pragma unroll
-            for (int i = 0; i <= W; i++) {
-                const uint64_t mi = i < W ? d.m[i] : 0;
-                const uint64_t t = (uint64_t)rem[i] - mi - borrow;
-                rem[i] = (uint32_t)t;
-                borrow = (t >> 63) & 1;
-            }
-            if (bit < 32 * QW) {
-#pragma unroll
-                for (int i = 0; i < QW; i++)
-                    if (i == (bit >> 5)) q[i] |= 1u << (bit & 31);
-            }
+        for (int i = 0; i < W; i++) {
+            const uint64_t p = qh * v[i] + mc;
+            mc = p >> 32;
+            const int64_t t = (int64_t)u[j + i] - (int64_t)(uint32_t)p + br;
+            u[j + i] = (uint32_t)t;
+            br = t >> 32;
         }
+        const int64_t t = (int64_t)u[j + W] - (int64_t)mc + br;
+        u[j + W] = (uint32_t)t;
+        if (t < 0) {  // qh was one too large: add v back
+            qh--;
+            uint64_t c = 0;
+#pragma unroll
+            for (int i = 0; i < W; i++) {
+                const uint64_t s2 = (uint64_t)u[j + i] + v[i] + c;
+                u[j + i] = (uint32_t)s2;
+                c = s2 >> 32;
+            }
+            u[j + W] += (uint32_t)c;
+        }
+        q[j] = (uint32_t)qh;
     }
     bool sticky = false;
 #pragma unroll
-    for (int i = 0; i <= W; i++) sticky |= rem[i] != 0;
+    for (int i = 0; i < W; i++) sticky |= u[i] != 0;
     // mag = (q << 1) | sticky
     uint32_t mag[QW + 1];
     uint32_t c = sticky ? 1u : 0u;

@@ -684,6 +684,221 @@ __global__ __launch_bounds__(BS) void crt_kernel(
     out_rec[2 * n + 1] = (uint32_t)(int32_t)expo;
 }
 
+// CRT on the INT8 matrix cores.  The product kernel leaves x'_l as balanced
+// digit pairs (lo, hi), so V = sum_l x'_l (M/q_l) = sum_t 256^t R_t with
+//   R_t = sum_l lo_l c_{l,t} + hi_l c_{l,t-1}
+// (c the balanced base-256 digits of M/q_l): one INT8 product of the
+// (outputs x 2L) digit matrix with the (2L x nt) table, exact in INT32.  A
+// workgroup (2 waves) takes one 128-row column of an output tile: stage the
+// x digits through LDS as [output][2 lpad], 2 x nbl 32x32x32 MFMA blocks per
+// wave, R through LDS, then one thread per output carries R into 32-bit
+// limbs, removes k M with k = round(V / M) from the top limbs, and rounds
+// the signed result once, nearest-even, to 64N bits.
+template <int NUMAX, int NBMAX>
+__global__ __launch_bounds__(128) void crt_mfma_kernel(
+    const uint16_t *__restrict__ X, int L, int lpad, const int2 *__restrict__ tiles, int ntiles,
+    int tiles_m, int tiles_n, const int8_t *__restrict__ table, int nt, int npad,
+    const uint32_t *__restrict__ mlimbs, int nu, double m_top, const int4 *__restrict__ scan_a,
+    const int4 *__restrict__ scan_b, int64_t m, int64_t nn, int upper, int packed, int n,
+    uint32_t *__restrict__ out) {
+    extern __shared__ __attribute__((aligned(16))) int8_t csm[];
+    const int trow = 2 * lpad + 16;  // bytes per table row and per staged x row
+    const int rrow = npad + 4;       // int32 per R row
+    int8_t *stab = csm;
+    int8_t *uni = csm + (size_t)npad * trow;  // x staging [128][trow] or R [128][rrow]
+    int32_t *sR = (int32_t *)uni;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, h = lane >> 5, col = lane & 31;
+    const int nks = lpad / 16, nbl = npad / 32;
+    const int c16 = 2 * lpad / 16;
+    for (int e = tid; e < npad * c16; e += 128) {
+        const int t = e / c16, c = e - t * c16;
+        *(v4i *)(stab + t * trow + 16 * c) = *(const v4i *)(table + (size_t)t * 2 * lpad + 16 * c);
+    }
+    const size_t xstride = (size_t)tiles_n * tiles_m * (TILE * TILE);
+    const int64_t items = (int64_t)ntiles * TILE;
+    const int P = 64 * n;
+    for (int64_t item = blockIdx.x; item < items; item += gridDim.x) {
+        const int2 tile = tiles[item / TILE];
+        const int jl = (int)(item % TILE);
+        const int64_t j = (int64_t)tile.y * TILE + jl, i0 = (int64_t)tile.x * TILE;
+        if (j >= nn || (upper && i0 > j)) continue;  // uniform over the workgroup
+        __syncthreads();  // the table is in; the previous item's R is read
+        const uint16_t *xb = X + ((size_t)tile.y * tiles_m + tile.x) * (TILE * TILE) + (size_t)jl * TILE;
+        for (int e = tid; e < lpad * 64; e += 128) {
+            const int l = e >> 6, o2 = (e & 63) * 2;
+            uint32_t pr = 0;
+            if (l < L) pr = *(const uint32_t *)(xb + (size_t)l * xstride + o2);
+            *(uint16_t *)(uni + o2 * trow + 2 * l) = (uint16_t)(pr & 0xFFFF);
+            *(uint16_t *)(uni + (o2 + 1) * trow + 2 * l) = (uint16_t)(pr >> 16);
+        }
+        __syncthreads();
+        v16i acc[2][NBMAX];
+#pragma unroll
+        for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+            for (int nb = 0; nb < NBMAX; nb++) {
+                acc[mt][nb] = v16i{0};
+                if (nb < nbl) {
+                    const int8_t *pa = uni + (64 * wave + 32 * mt + col) * trow + 16 * h;
+                    const int8_t *pb = stab + (32 * nb + col) * trow + 16 * h;
+                    for (int ks = 0; ks < nks; ks++)
+                        acc[mt][nb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
+                            *(const v4i *)(pa + 32 * ks), *(const v4i *)(pb + 32 * ks), acc[mt][nb], 0, 0, 0);
+                }
+            }
+        __syncthreads();  // x staging fully read: R overwrites it
+#pragma unroll
+        for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+            for (int nb = 0; nb < NBMAX; nb++)
+                if (nb < nbl)
+#pragma unroll
+                    for (int r = 0; r < 16; r++) {
+                        const int o = 64 * wave + 32 * mt + 8 * (r >> 2) + 4 * h + (r & 3);
+                        sR[o * rrow + 32 * nb + col] = acc[mt][nb][r];
+                    }
+        __syncthreads();
+        const int64_t i = i0 + tid;
+        if (i >= m || (upper && i > j)) continue;
+        const int32_t *Rr = sR + tid * rrow;
+        uint32_t lim[NUMAX];
+        int64_t carry = 0;
+#pragma unroll
+        for (int u = 0; u < NUMAX; u++) {
+            lim[u] = 0;
+            if (u < nu) {
+                int64_t a = carry;
+#pragma unroll
+                for (int b = 0; b < 4; b++)
+                    if (4 * u + b < nt) a += (int64_t)Rr[4 * u + b] * ((int64_t)1 << (8 * b));
+                lim[u] = (uint32_t)a;
+                carry = a >> 32;
+            }
+        }
+        // k = round(V / M) from the top three limbs; S = V - k M
+        uint32_t l2 = 0, l1 = 0, l0 = 0;
+#pragma unroll
+        for (int u = 0; u < NUMAX; u++) {
+            if (u == nu - 1) l2 = lim[u];
+            if (u == nu - 2) l1 = lim[u];
+            if (u == nu - 3) l0 = lim[u];
+        }
+        const double vt = (double)(int32_t)l2 * 18446744073709551616.0 + (double)l1 * 4294967296.0 + (double)l0;
+        const int64_t kq = (int64_t)rint(vt / m_top);
+        int64_t br = 0;
+        uint32_t topw = 0;
+#pragma unroll
+        for (int u = 0; u < NUMAX; u++)
+            if (u < nu) {
+                const int64_t t = (int64_t)lim[u] - kq * (int64_t)mlimbs[u] + br;
+                lim[u] = (uint32_t)t;
+                br = t >> 32;
+                topw = lim[u];
+            }
+        const bool negative = (int32_t)topw < 0;
+        if (negative) {
+            uint64_t c = 1;
+#pragma unroll
+            for (int u = 0; u < NUMAX; u++)
+                if (u < nu) {
+                    const uint64_t t = (uint64_t)(~lim[u]) + c;
+                    lim[u] = (uint32_t)t;
+                    c = t >> 32;
+                }
+        }
+        uint32_t *rec = out + (size_t)(packed ? j * (j + 1) / 2 + i : i + j * m) * (2 * n + 2);
+        int top = -1;
+        uint32_t tw = 0;
+#pragma unroll
+        for (int u = 0; u < NUMAX; u++)
+            if (u < nu && lim[u] != 0) {
+                top = u;
+                tw = lim[u];
+            }
+        if (top < 0) {
+            for (int w = 0; w < 2 * n; w++) rec[w] = 0;
+            rec[2 * n] = KIND_ZERO;
+            rec[2 * n + 1] = 0;
+            continue;
+        }
+        const int bits = 32 * top + 32 - __builtin_clz(tw);
+        int shift = bits - P;
+        bool roundb = false, sticky = false;
+        if (shift > 0) {
+            const int rb = shift - 1;
+#pragma unroll
+            for (int u = 0; u < NUMAX; u++) {
+                const int base = 32 * u;
+                if (base + 32 <= rb) sticky |= lim[u] != 0;
+                else if (base < rb) sticky |= (lim[u] & ((1u << (rb - base)) - 1u)) != 0;
+                if (base <= rb && rb < base + 32) roundb = (lim[u] >> (rb - base)) & 1u;
+            }
+        }
+        // mantissa words: bits [shift, shift + P)
+        uint32_t mant[2 * MAX_LIMBS];
+        if (shift >= 0) {
+            const int qs = shift >> 5, rs = shift & 31;
+#pragma unroll
+            for (int sft = 32; sft >= 1; sft >>= 1)
+                if (qs & sft) {
+#pragma unroll
+                    for (int u = 0; u < NUMAX; u++) lim[u] = (u + sft < NUMAX) ? lim[u + sft] : 0u;
+                }
+#pragma unroll
+            for (int w = 0; w < 2 * MAX_LIMBS; w++) {
+                const uint32_t lo = w < NUMAX ? lim[w] : 0u, hi = w + 1 < NUMAX ? lim[w + 1] : 0u;
+                mant[w] = __builtin_amdgcn_alignbit(hi, lo, rs);
+            }
+        } else {
+            const int ls = -shift, qs = ls >> 5, rs = ls & 31;
+#pragma unroll
+            for (int sft = 32; sft >= 1; sft >>= 1)
+                if (qs & sft) {
+#pragma unroll
+                    for (int u = NUMAX - 1; u >= 0; u--) lim[u] = (u - sft >= 0) ? lim[u - sft] : 0u;
+                }
+#pragma unroll
+            for (int w = 0; w < 2 * MAX_LIMBS; w++) {
+                const uint32_t cur = w < NUMAX ? lim[w] : 0u, prev = (w >= 1 && w - 1 < NUMAX) ? lim[w - 1] : 0u;
+                mant[w] = rs ? ((cur << rs) | (prev >> (32 - rs))) : cur;
+            }
+        }
+        if (roundb && (sticky || (mant[0] & 1u))) {
+            uint32_t c = 1;
+#pragma unroll
+            for (int w = 0; w < 2 * MAX_LIMBS; w++)
+                if (w < 2 * n) {
+                    mant[w] += c;
+                    c = c && mant[w] == 0;
+                }
+            if (c) {
+#pragma unroll
+                for (int w = 0; w < 2 * MAX_LIMBS; w++)
+                    if (w < 2 * n) mant[w] = (w == 2 * n - 1) ? 0x80000000u : 0u;
+                shift += 1;
+            }
+        }
+        const int64_t expo = (int64_t)scan_a[i].x + scan_b[j].x + shift + P;
+        if (expo > (int64_t)((1 << 30) - 1)) {
+            for (int w = 0; w < 2 * n; w++) rec[w] = 0;
+            rec[2 * n] = (uint32_t)(negative ? -KIND_INF : KIND_INF);
+            rec[2 * n + 1] = 0;
+            continue;
+        }
+        if (expo < (int64_t)(1 - (1 << 30))) {
+            for (int w = 0; w < 2 * n; w++) rec[w] = 0;
+            rec[2 * n] = KIND_ZERO;
+            rec[2 * n + 1] = 0;
+            continue;
+        }
+#pragma unroll
+        for (int w = 0; w < 2 * MAX_LIMBS; w++)
+            if (w < 2 * n) rec[w] = mant[w];
+        rec[2 * n] = (uint32_t)(negative ? -KIND_REGULAR : KIND_REGULAR);
+        rec[2 * n + 1] = (uint32_t)(int32_t)expo;
+    }
+}
+
 __global__ void zero_values(uint32_t *__restrict__ out, int n, int64_t m, int64_t nn,
                             int upper, int packed) {
     int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -704,7 +919,7 @@ namespace {
 
 enum Slot {
     S_SCAN_A = 0, S_SCAN_B, S_PLANES_A, S_PLANES_B, S_X, S_POW_A, S_POW_B, S_MODS, S_CRT,
-    S_TILES, S_ENC_A, S_ENC_B, S_COUNT
+    S_TILES, S_ENC_A, S_ENC_B, S_CRT2, S_COUNT
 };
 
 struct Operand {
@@ -850,6 +1065,7 @@ std::vector<double> pow2_table(const RnsPlan &plan, int TJ) {
 thread_local RnsStats t_stats;
 bool g_profile = false;
 bool g_encode_mfma = true;  // matrix-core encode where the window allows
+bool g_crt_mfma = true;     // matrix-core CRT where the plan allows
 
 // Events bracketing the four stages (only when profiling).
 struct StageEvents {
@@ -1046,7 +1262,32 @@ int rns_gemm_device(const RnsGemmArgs &g) {
     const double *d_c = d_crt, *d_m = d_crt + (size_t)L * J;
     const int up = g.upper ? 1 : 0, pk = packed ? 1 : 0;
     uint32_t *outw = (uint32_t *)g.out;
-    if (J <= 32)
+    const int nbl = plan->crt_npad / 32;
+    const bool crt_mma = g_crt_mfma && plan->crt_nu <= 40 && nbl <= 5 && n <= 8;
+    if (crt_mma) {
+        const int lpad = plan->crt_lpad, npad = plan->crt_npad, nu = plan->crt_nu;
+        const size_t tb = plan->crt_table.size();
+        uint8_t *d_ct = (uint8_t *)ws.get(S_CRT2, tb + 4 * (size_t)nu);
+        HIP_CHECK(hipMemcpyAsync(d_ct, plan->crt_table.data(), tb, hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(d_ct + tb, plan->m32.data(), 4 * (size_t)nu, hipMemcpyHostToDevice, s));
+        const int trow = 2 * lpad + 16, rrow = npad + 4;
+        const size_t lds = (size_t)npad * trow + std::max<size_t>((size_t)TILE * trow, (size_t)TILE * rrow * 4);
+        const int64_t items = (int64_t)ntiles * TILE;
+        const unsigned grid = (unsigned)std::min<int64_t>(items, 4096);
+        auto launch = [&](auto kern) {
+            if (lds > 65536)
+                HIP_CHECK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                              (int)lds));
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(128), lds, s, (const uint16_t *)X, L, lpad,
+                               (const int2 *)d_tiles, ntiles, tiles_m, tiles_n, (const int8_t *)d_ct,
+                               plan->crt_nt, npad, (const uint32_t *)(d_ct + tb), nu, plan->m_top,
+                               (const int4 *)scan_a, (const int4 *)scan_b, m, nn, up, pk, n, outw);
+        };
+        if (nu <= 20 && nbl <= 3)
+            launch(crt_mfma_kernel<20, 3>);
+        else
+            launch(crt_mfma_kernel<40, 5>);
+    } else if (J <= 32)
         hipLaunchKernelGGL((crt_kernel<32, 256>), dim3(TILE * TILE / 256, ntiles), dim3(256), 0, s,
                            X, L, d_tiles, tiles_m, tiles_n, d_c, d_m, d_qinv, d_q, J, scan_a, scan_b,
                            m, nn, up, pk, n, outw);

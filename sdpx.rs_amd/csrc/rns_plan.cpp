@@ -126,6 +126,26 @@ std::unique_ptr<RnsPlan> build(int count) {
     plan->c.resize((size_t)count * plan->limbs);
     plan->inv.resize(count);
     plan->qinv.resize(count);
+    // matrix-core CRT table (see RnsPlan)
+    int mbits = 0;
+    for (size_t w = 0; w < m.size(); w++)
+        if (m[w]) mbits = 32 * (int)w + 32 - __builtin_clz(m[w]);
+    const int jb = (mbits + 7) / 8;
+    plan->crt_jb = jb;
+    plan->crt_nt = jb + 2;
+    plan->crt_lpad = (count + 15) / 16 * 16;
+    plan->crt_npad = (plan->crt_nt + 31) / 32 * 32;
+    plan->crt_nu = (8 * jb + 16 + 31) / 32;
+    if (4 * plan->crt_nu < plan->crt_nt) plan->crt_nu = (plan->crt_nt + 3) / 4;
+    plan->crt_table.assign((size_t)std::max(plan->crt_npad, 0) * 2 * (size_t)std::max(plan->crt_lpad, 0), 0);
+    plan->m32.assign((size_t)std::max(plan->crt_nu, 1), 0);
+    for (size_t w = 0; w < m.size() && (int)w < plan->crt_nu; w++) plan->m32[w] = m[w];
+    {
+        const int nu = plan->crt_nu;
+        double top = 0;
+        for (int u = nu - 1; u >= nu - 3 && u >= 0; u--) top = top * 4294967296.0 + (double)plan->m32[u];
+        plan->m_top = top;
+    }
     std::vector<uint32_t> mq;
     for (int l = 0; l < count; l++) {
         uint32_t q = plan->q[l];
@@ -134,6 +154,17 @@ std::unique_ptr<RnsPlan> build(int count) {
         plan->qinv[l] = 1.0 / (double)q;
         auto limbs = limbs24(mq, plan->limbs);
         std::copy(limbs.begin(), limbs.end(), plan->c.begin() + (size_t)l * plan->limbs);
+        // balanced base-256 digits of M/q_l (< M: jb bytes, one more for the carry)
+        int carry = 0;
+        for (int t = 0; t <= jb; t++) {
+            const size_t w = (size_t)t / 4;
+            int d = (t < jb && w < mq.size() ? (int)((mq[w] >> (8 * (t % 4))) & 255u) : 0) + carry;
+            carry = d >= 128 ? 1 : 0;
+            if (carry) d -= 256;
+            const size_t row = (size_t)2 * plan->crt_lpad;
+            plan->crt_table[(size_t)t * row + 2 * l] = (int8_t)d;
+            plan->crt_table[(size_t)(t + 1) * row + 2 * l + 1] = (int8_t)d;
+        }
     }
     return plan;
 }

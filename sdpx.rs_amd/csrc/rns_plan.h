@@ -35,6 +35,16 @@ struct RnsPlan {
     int limbs = 0;
     std::vector<double> c;         // count x limbs: M/q_l
     std::vector<double> m;         // limbs: M
+    // Matrix-core CRT constants.  V = sum_l x'_l (M/q_l) for balanced-digit
+    // x'_l = 256 hi + lo is the INT8 product of the digit pairs with table
+    // [t][2l] = c_{l,t}, [t][2l+1] = c_{l,t-1}, c_{l,t} the balanced base-256
+    // digits of M/q_l (t <= crt_jb); rows crt_nt = crt_jb + 2, padded to
+    // crt_npad rows of 2 crt_lpad bytes.  M in crt_nu 32-bit words (room for
+    // |V| <= (count/2) M and a sign), m_top = M / 2^(32 (crt_nu - 3)).
+    int crt_jb = 0, crt_nt = 0, crt_npad = 0, crt_lpad = 0, crt_nu = 0;
+    double m_top = 0;
+    std::vector<int8_t> crt_table;
+    std::vector<uint32_t> m32;
 };
 
 // All moduli in plan order.

@@ -130,10 +130,43 @@ int run(int count, int spread, uint64_t seed) {
     return bad;
 }
 
+// Reciprocals of the mantissas at the edges of the quotient-digit estimate:
+// powers of two, all ones, a single low bit, a high word followed by zeros
+// or ones, and the digit patterns that force the estimate's corrections.
+template <int N>
+int recip_edges() {
+    constexpr int W = 2 * N;
+    const size_t vb = 8 * N + 8;
+    std::vector<uint8_t> a(vb), r1(vb), r2(vb);
+    int bad = 0;
+    const uint32_t pats[][2] = {{0x80000000u, 0}, {0xFFFFFFFFu, 0xFFFFFFFFu}, {0x80000000u, 1},
+                                {0xFFFFFFFFu, 0}, {0x80000000u, 0xFFFFFFFFu}, {0xC0000000u, 0},
+                                {0x80000001u, 0x80000000u}, {0xFFFFFFFEu, 0xFFFFFFFFu}};
+    for (auto &pt : pats)
+        for (int low = 0; low < 3; low++)
+            for (int sgn = 0; sgn < 2; sgn++) {
+                uint32_t *m = (uint32_t *)a.data();
+                for (int i = 0; i < W; i++) m[i] = pt[1];
+                m[W - 1] = pt[0];
+                if (low == 1) m[0] = 1;
+                if (low == 2 && W > 1) m[W - 2] = 0;
+                *(int32_t *)(a.data() + 8 * N) = sgn ? -3 : 3;
+                *(int32_t *)(a.data() + 8 * N + 4) = 17 - 40 * low;
+                mpd::Val<W> va, vr;
+                mpd::load((const uint32_t *)a.data(), va);
+                mpd::recip(va, vr);
+                mpd::store((uint32_t *)r1.data(), vr);
+                mpx::recip(mpx::View{N, a.data()}, r2.data());
+                if (std::memcmp(r1.data(), r2.data(), vb)) bad++, std::printf("recip edge mismatch N=%d\n", N);
+            }
+    return bad;
+}
+
 int main(int argc, char **argv) {
     int count = argc > 1 ? std::atoi(argv[1]) : 20000;
     int bad = run<1>(count, 200, 1) + run<2>(count, 200, 2) + run<4>(count, 300, 3) +
               run<8>(count, 600, 4);
+    bad += recip_edges<1>() + recip_edges<2>() + recip_edges<4>() + recip_edges<8>() + recip_edges<16>();
     std::printf("mismatches %d\n", bad);
     return bad ? 1 : 0;
 }
