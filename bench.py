@@ -366,10 +366,23 @@ def main():
     ap.add_argument("--workload", default="sampled", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
     ap.add_argument("--order", type=int, default=8192, help="matrix order of --workload chol")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run one device-to-device copy of MIB MiB (a known-traffic kernel "
+                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.copy_calibration:
+        import torch
+
+        src = torch.ones(args.copy_calibration << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        progress(f"copy calibration: {args.copy_calibration} MiB read, {args.copy_calibration} MiB written")
+        del src, dst
     r = WORKLOADS[args.workload](args)
     line = {"metric": METRIC, "value": r["value"], "unit": r["unit"], "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["ms_per_step"],

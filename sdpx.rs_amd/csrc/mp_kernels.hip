@@ -133,11 +133,18 @@ __device__ __forceinline__ int64_t group_sum(int64_t v) {
 // range pass.  Any window holding all terms gives the same exact sum, and a
 // hint too wide for the window falls back to the exact pass, so the result
 // and the overflow decision never depend on the hint.
-template <int W, int G, class FX, class FY>
-__device__ inline bool group_dot(int64_t k0, int64_t k1, int gl, FX fx, FY fy, const Val<W> *c,
-                                 bool negate_terms, int64_t *slots, int stride, Val<W> &out,
-                                 int64_t hint_lo = INT64_MAX, int64_t hint_hi = INT64_MIN) {
+// Terms x_k = xb[k * xs], y_k = yb[k * ys] (record strides): every panel
+// kernel's operands are affine in k, so one out-of-line copy per width
+// serves all call sites.
+template <int W, int G>
+__device__ __attribute__((noinline)) bool group_dot(int64_t k0, int64_t k1, int gl,
+                                                    const uint32_t *xb, int64_t xs,
+                                                    const uint32_t *yb, int64_t ys, const Val<W> *c,
+                                                    bool negate_terms, int64_t *slots, int stride,
+                                                    Val<W> &out, int64_t hint_lo, int64_t hint_hi) {
     constexpr int SLOTS = Win<W>::SLOTS;
+    auto fx = [&](int64_t k) { return xb + k * xs * (W + 2); };
+    auto fy = [&](int64_t k) { return yb + k * ys * (W + 2); };
     const int64_t kf = k0 + (((int64_t)gl - k0) % G + G) % G;  // first k of this lane
     int64_t lo = INT64_MAX, hi = INT64_MIN;
     if (hint_lo <= hint_hi && (hint_hi - hint_lo) / 32 + 2 * W + 4 <= SLOTS) {
@@ -256,11 +263,10 @@ __global__ __launch_bounds__(ROW_THREADS) void forward_rows_kernel(
         Val<W> b, r;
         load(rec_at<W>(col, i), b);
         // y_i = RN(b_i - sum_{k in [first_i, i)} L_ik y_k)
-        auto fx = [&](int64_t k) { return rec_at<W>(L, i + k * K); };
-        auto fy = [&](int64_t k) { return rec_at<W>((const uint32_t *)col, k); };
         int64_t hl = INT64_MAX, hh = INT64_MIN;
         if (lr) range_hint<W>(lr[i].x, lr[i].y, ymn, ymx, hl, hh);
-        if (!group_dot<W, G>(first[i], i, gl, fx, fy, &b, true, slots, stride, r, hl, hh)) {
+        if (!group_dot<W, G>(first[i], i, gl, rec_at<W>(L, i), K, col, 1, &b, true, slots, stride, r,
+                             hl, hh)) {
             if (gl == 0) atomicOr(fail, 1);
             return;
         }
@@ -284,12 +290,11 @@ __global__ __launch_bounds__(ROW_THREADS) void trsm_base_kernel(
     const int64_t lo = low[i];
     int32_t xmn = INT32_MAX, xmx = INT32_MIN;  // x_ik so far
     for (int64_t j = lo; j < n; j++) {
-        auto fx = [&](int64_t k) { return rec_at<W>((const uint32_t *)B, i + k * ldb); };
-        auto fy = [&](int64_t k) { return rec_at<W>(L, j + k * ldl); };
         Val<W> t, x, r;
         int64_t hl = INT64_MAX, hh = INT64_MIN;
         if (lr) range_hint<W>(xmn, xmx, lr[j].x, lr[j].y, hl, hh);
-        if (!group_dot<W, G>(lo, j, gl, fx, fy, nullptr, false, slots, stride, t, hl, hh)) {
+        if (!group_dot<W, G>(lo, j, gl, rec_at<W>((const uint32_t *)B, i), ldb, rec_at<W>(L, j), ldl,
+                             nullptr, false, slots, stride, t, hl, hh)) {
             if (gl == 0) atomicOr(fail, 1);
             return;
         }
@@ -315,12 +320,11 @@ __global__ __launch_bounds__(ROW_THREADS) void trsm_base_n_kernel(
     const int stride = blockDim.x;
     int32_t xmn = INT32_MAX, xmx = INT32_MIN;  // x_ik so far (k > j)
     for (int64_t j = n - 1; j >= 0; j--) {
-        auto fx = [&](int64_t k) { return rec_at<W>((const uint32_t *)B, i + k * ldb); };
-        auto fy = [&](int64_t k) { return rec_at<W>(L, k + j * ldl); };
         Val<W> t, x, r;
         int64_t hl = INT64_MAX, hh = INT64_MIN;
         if (lc) range_hint<W>(xmn, xmx, lc[j].x, lc[j].y, hl, hh);
-        if (!group_dot<W, G>(j + 1, n, gl, fx, fy, nullptr, false, slots, stride, t, hl, hh)) {
+        if (!group_dot<W, G>(j + 1, n, gl, rec_at<W>((const uint32_t *)B, i), ldb, rec_at<W>(L, j * ldl),
+                             1, nullptr, false, slots, stride, t, hl, hh)) {
             if (gl == 0) atomicOr(fail, 1);
             return;
         }
@@ -407,16 +411,16 @@ __global__ __launch_bounds__(ldl_threads<W>()) void ldl_dots_kernel(
     // DenseLeaf::pivot of column j from its exact dot (group 0; lane 0 writes)
     auto pivot = [&](int64_t j, int64_t hl, int64_t hh) {
         const int64_t fj = first[j];
-        auto fx = [&](int64_t k) { return rec_at<W>((const uint32_t *)r, tri(j) + k); };
-        auto fy = [&](int64_t k) { return rec_at<W>((const uint32_t *)w, tri(j) + k); };
+        const uint32_t *xr = rec_at<W>((const uint32_t *)r, tri(j));
+        const uint32_t *yr = rec_at<W>((const uint32_t *)w, tri(j));
         Val<W> ajj, t, d;
         load(rec_at<W>(a, j + j * ld), ajj);
-        bool ok = group_dot<W, G>(fj, j, gl, fx, fy, nullptr, false, slots, stride, t, hl, hh);
+        bool ok = group_dot<W, G>(fj, j, gl, xr, 1, yr, 1, nullptr, false, slots, stride, t, hl, hh);
         if (ok) {
             add_signed(ajj, t, true, d);
             // exact retry of a cancelled pivot (uniform over the group)
             if (is_zero(d) && j > fj)
-                ok = group_dot<W, G>(fj, j, gl, fx, fy, &ajj, true, slots, stride, d, hl, hh);
+                ok = group_dot<W, G>(fj, j, gl, xr, 1, yr, 1, &ajj, true, slots, stride, d, hl, hh);
         }
         if (gl != 0) return;
         if (!ok) {
@@ -459,10 +463,10 @@ __global__ __launch_bounds__(ldl_threads<W>()) void ldl_dots_kernel(
                 hint(i, j, rmin[i], rmax[i], wmin[j], wmax[j], hl, hh);
             else
                 hint(i, j, 1, 0, 1, 0, hl, hh);
-            auto fx = [&](int64_t k) { return rec_at<W>((const uint32_t *)r, tri(i) + k); };
-            auto fy = [&](int64_t k) { return rec_at<W>((const uint32_t *)w, tri(j) + k); };
             Val<W> t, aij, diff;
-            if (!group_dot<W, G>(lo, j, gl, fx, fy, nullptr, false, slots, stride, t, hl, hh))
+            if (!group_dot<W, G>(lo, j, gl, rec_at<W>((const uint32_t *)r, tri(i)), 1,
+                                 rec_at<W>((const uint32_t *)w, tri(j)), 1, nullptr, false, slots,
+                                 stride, t, hl, hh))
                 return false;
             load(rec_at<W>(a, i + j * ld), aij);
             add_signed(aij, t, true, diff);

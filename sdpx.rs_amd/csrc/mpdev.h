@@ -18,6 +18,10 @@
 namespace sdpx {
 namespace mpd {
 
+// The larger operations are kept out of line: the panel kernels call them
+// from several sites, and inlined copies would overflow the instruction cache.
+#define SDPX_MP_OUTLINE __host__ __device__ __attribute__((noinline))
+
 constexpr int KIND_ZERO = 2, KIND_REG = 3, KIND_INF = 1, KIND_NAN = 0;
 constexpr int64_t EMAX = (1LL << 30) - 1, EMIN = 1 - (1LL << 30);
 
@@ -144,7 +148,7 @@ __host__ __device__ inline void round_words(bool neg, const uint32_t (&mag)[L], 
 
 // RN(a * b)
 template <int W>
-__host__ __device__ inline void mul(const Val<W> &a, const Val<W> &b, Val<W> &out) {
+SDPX_MP_OUTLINE void mul(const Val<W> &a, const Val<W> &b, Val<W> &out) {
     if (is_zero(a) || is_zero(b)) {
         set_zero(out);
         return;
@@ -169,7 +173,7 @@ __host__ __device__ inline void mul(const Val<W> &a, const Val<W> &b, Val<W> &ou
 
 // RN(a + s*b) with s = +1 (add) or -1 (sub).
 template <int W>
-__host__ __device__ inline void add_signed(const Val<W> &a, const Val<W> &b, bool negate_b,
+SDPX_MP_OUTLINE void add_signed(const Val<W> &a, const Val<W> &b, bool negate_b,
                                            Val<W> &out) {
     const bool na = a.kind < 0, nb = (b.kind < 0) != negate_b;
     if (is_zero(b)) {
@@ -357,7 +361,7 @@ struct DotWindow {
 // 4.3.1, algorithm D: two-word quotient-digit estimate, at most two
 // corrections, multiply-subtract, add back), W + 2 digit steps.
 template <int W>
-__host__ __device__ inline void recip(const Val<W> &d, Val<W> &out) {
+SDPX_MP_OUTLINE void recip(const Val<W> &d, Val<W> &out) {
     if (is_zero(d)) {
         set_zero(out);
         out.kind = d.kind < 0 ? -KIND_INF : KIND_INF;

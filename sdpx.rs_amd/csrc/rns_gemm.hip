@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "common.h"
@@ -556,6 +557,163 @@ __global__ __launch_bounds__(256) void product_kernel(
             w3 = nw[3];
         }
         *(v4i *)(gX + e) = v4i{(int)w0, (int)w1, (int)w2, (int)w3};
+    }
+}
+
+// Second product kernel: BM x 128 output tiles (BM = 256 for one-digit
+// moduli: every wave owns a (BM/2) x 64 block of MI x 2 MFMA tiles, so each
+// LDS fragment feeds MI or 2 MFMAs), double-buffered LDS with one barrier per
+// 64-byte k stage, and the next stage's global loads in flight during the
+// current stage's MFMAs.  Output x planes keep the 128 x 128 tile layout
+// (BM / 128 tiles per workgroup).  tiles2 lists (row block of BM, column tile).
+template <int D, int BM>
+__global__ __launch_bounds__(256) void product2_kernel(
+    const int8_t *__restrict__ A, const int8_t *__restrict__ B, int64_t a_plane, int64_t b_plane,
+    int64_t k_pad, int64_t k_begin, int64_t k_end, const uint32_t *__restrict__ mq,
+    const uint32_t *__restrict__ minv, const double *__restrict__ mqinv, int single,
+    int mod_first, int mod_count, const int2 *__restrict__ tiles2, int ntiles2, int tiles_m,
+    int tiles_n, int accumulate, uint16_t *__restrict__ X) {
+    constexpr int MI = BM / 64;                          // 32-row MFMA blocks per wave
+    constexpr int ROWS = BM + TILE;                      // staged rows per digit
+    constexpr int STAGE = D * ROWS * LDS_ROW;            // bytes per LDS stage
+    constexpr int CH = D * ROWS * (KSTEP / 16) / 256;    // 16-byte chunks per thread
+    constexpr int XROW = BM + 2;                         // u16 per staged x column
+    extern __shared__ __attribute__((aligned(16))) int8_t psm[];
+
+    const int total = mod_count * ntiles2;
+    const int per = (int)(gridDim.x >> 3);
+    const int bid = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if (bid >= total) return;
+    const int l = mod_first + bid / ntiles2;
+    const int2 tile = tiles2[bid % ntiles2];
+    const int tm2 = tile.x, tn = tile.y;
+    const int plane = l < single ? l : single + 2 * (l - single);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+
+    // chunk c: digit d = c / (ROWS * 4), row r = (c / 4) % ROWS, 16-byte part c % 4
+    const int8_t *src[CH];
+    int dst[CH];
+#pragma unroll
+    for (int u = 0; u < CH; u++) {
+        const int c = tid + 256 * u;
+        const int d = c / (ROWS * 4), rr = (c / 4) % ROWS, part = c & 3;
+        const int8_t *base = rr < BM ? A + (size_t)(plane + d) * a_plane + ((size_t)tm2 * BM + rr) * k_pad
+                                     : B + (size_t)(plane + d) * b_plane + ((size_t)tn * TILE + rr - BM) * k_pad;
+        src[u] = base + 16 * part;
+        dst[u] = d * ROWS * LDS_ROW + rr * LDS_ROW + 16 * part;
+    }
+    v4i stg[CH];
+    auto gload = [&](int64_t k) {
+#pragma unroll
+        for (int u = 0; u < CH; u++) stg[u] = *(const v4i *)(src[u] + k);
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < CH; u++) *(v4i *)(psm + buf * STAGE + dst[u]) = stg[u];
+    };
+
+    constexpr int NACC = D == 1 ? 1 : 3;
+    v16i acc[NACC][MI][2];
+#pragma unroll
+    for (int a = 0; a < NACC; a++)
+#pragma unroll
+        for (int i = 0; i < MI; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) acc[a][i][j] = v16i{0};
+
+    const int frag_row = lane & 31, frag_k = (lane >> 5) * 16;
+    gload(k_begin);
+    lstore(0);
+    __syncthreads();
+    int buf = 0;
+    for (int64_t k = k_begin; k < k_end; k += KSTEP) {
+        const bool more = k + KSTEP < k_end;
+        if (more) gload(k + KSTEP);
+        const int8_t *sb = psm + buf * STAGE;
+#pragma unroll
+        for (int kk = 0; kk < KSTEP; kk += 32) {
+            v4i fa[D][MI], fb[D][2];
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+#pragma unroll
+                for (int i = 0; i < MI; i++)
+                    fa[d][i] = *(const v4i *)(sb + d * ROWS * LDS_ROW +
+                                              (wm * (BM / 2) + i * 32 + frag_row) * LDS_ROW + kk + frag_k);
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+                    fb[d][j] = *(const v4i *)(sb + d * ROWS * LDS_ROW +
+                                              (BM + wn * 64 + j * 32 + frag_row) * LDS_ROW + kk + frag_k);
+            }
+#pragma unroll
+            for (int i = 0; i < MI; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++) {
+                    if constexpr (D == 1) {
+                        acc[0][i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[0][i], fb[0][j], acc[0][i][j], 0, 0, 0);
+                    } else {
+                        acc[0][i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[0][i], fb[0][j], acc[0][i][j], 0, 0, 0);
+                        acc[1][i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[0][i], fb[1][j], acc[1][i][j], 0, 0, 0);
+                        acc[1][i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[1][i], fb[0][j], acc[1][i][j], 0, 0, 0);
+                        acc[2][i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[1][i], fb[1][j], acc[2][i][j], 0, 0, 0);
+                    }
+                }
+        }
+        if (more) lstore(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+
+    // residues -> x_l = S (M/q)^-1 mod q (balanced digits), transposed through LDS
+    const int q = (int)mq[l];
+    const double qd = (double)q, qinv = mqinv[l];
+    const double inv = (double)minv[l];
+    const double r16 = (double)((65536 % q)), r8 = (double)(256 % q);
+    uint16_t *sX = (uint16_t *)psm;  // [128 cols][XROW] (row index fastest)
+#pragma unroll
+    for (int i = 0; i < MI; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                int res;
+                if constexpr (D == 1) {
+                    res = mod_q((double)acc[0][i][j][r], qd, qinv);
+                } else {
+                    double hh = (double)mod_q((double)acc[0][i][j][r], qd, qinv);
+                    double mid = (double)mod_q((double)acc[1][i][j][r], qd, qinv);
+                    double ll = (double)mod_q((double)acc[2][i][j][r], qd, qinv);
+                    res = mod_q(fma(hh, r16, fma(mid, r8, ll)), qd, qinv);
+                }
+                const int x = mod_q((double)res * inv, qd, qinv);
+                const int row = wm * (BM / 2) + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                const int col = wn * 64 + j * 32 + (lane & 31);
+                sX[col * XROW + row] = x_encode(x, q);
+            }
+    __syncthreads();
+#pragma unroll
+    for (int sub = 0; sub < BM / TILE; sub++) {
+        const int tm = tm2 * (BM / TILE) + sub;
+        uint16_t *gX = X + ((size_t)(l * tiles_n + tn) * tiles_m + tm) * (TILE * TILE);
+#pragma unroll
+        for (int it = 0; it < 8; it++) {
+            const int e = (it * 256 + tid) * 8;
+            const int col = e >> 7, row = e & 127;
+            const uint32_t *srcx = (const uint32_t *)(sX + col * XROW + sub * TILE + row);
+            uint32_t nw[4] = {srcx[0], srcx[1], srcx[2], srcx[3]};
+            if (accumulate) {
+                const v4i old = *(const v4i *)(gX + e);
+                const uint32_t o[4] = {(uint32_t)old.x, (uint32_t)old.y, (uint32_t)old.z, (uint32_t)old.w};
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    const int lo = x_decode((uint16_t)(nw[t] & 0xFFFF), q) + x_decode((uint16_t)(o[t] & 0xFFFF), q);
+                    const int hi = x_decode((uint16_t)(nw[t] >> 16), q) + x_decode((uint16_t)(o[t] >> 16), q);
+                    nw[t] = (uint32_t)x_encode(lo >= q ? lo - q : lo, q) |
+                            ((uint32_t)x_encode(hi >= q ? hi - q : hi, q) << 16);
+                }
+            }
+            *(v4i *)(gX + e) = v4i{(int)nw[0], (int)nw[1], (int)nw[2], (int)nw[3]};
+        }
     }
 }
 
@@ -1095,6 +1253,37 @@ struct StageEvents {
 
 unsigned padded_grid(int64_t total) { return (unsigned)((total + 7) / 8 * 8); }
 
+// Product kernel selection: 2 (double-buffered, 256 x 128 one-digit tiles)
+// unless SDPX_GPU_PRODUCT_KERNEL=1 selects the first form (for comparison).
+int product_kernel_choice() {
+    static const int choice = [] {
+        const char *e = std::getenv("SDPX_GPU_PRODUCT_KERNEL");
+        return (e && e[0] == '1') ? 1 : 2;
+    }();
+    return choice;
+}
+
+template <int D, int BM>
+void launch_product2(const int8_t *pla, const int8_t *plb, int64_t a_plane, int64_t b_plane,
+                     int64_t k_pad, int64_t k0, int64_t k1, const uint32_t *d_q, const uint32_t *d_inv,
+                     const double *d_qinv, int single, int mod_first, int mod_count, const int2 *tiles,
+                     int ntiles, int tiles_m, int tiles_n, int acc, uint16_t *X, hipStream_t s) {
+    constexpr int STAGE = D * (BM + TILE) * LDS_ROW;
+    constexpr int XT = TILE * (BM + 2) * 2;
+    constexpr size_t lds = 2 * STAGE > XT ? 2 * STAGE : XT;
+    auto kern = product2_kernel<D, BM>;
+    static bool attr = false;
+    if (!attr && lds > 65536) {
+        HIP_CHECK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds));
+        attr = true;
+    }
+    hipLaunchKernelGGL(kern, dim3(padded_grid((int64_t)mod_count * ntiles)), dim3(256), lds, s, pla, plb,
+                       a_plane, b_plane, k_pad, k0, k1, d_q, d_inv, d_qinv, single, mod_first, mod_count,
+                       tiles, ntiles, tiles_m, tiles_n, acc, X);
+    HIP_CHECK(hipGetLastError());
+}
+
 }  // namespace
 
 const RnsStats &rns_last_stats() { return t_stats; }
@@ -1166,7 +1355,11 @@ int rns_gemm_device(const RnsGemmArgs &g) {
 
     // tables: moduli, inverses, 1/q, pow2 (A, B), CRT
     const int64_t k_pad = (k + KSTEP - 1) / KSTEP * KSTEP;
-    const int64_t m_pad = (m + TILE - 1) / TILE * TILE, n_pad = (nn + TILE - 1) / TILE * TILE;
+    // the second product kernel takes 256-row blocks of op(A): pad both
+    // sides to 256 (equal strides keep the Gram form's shared planes valid)
+    const bool p2 = product_kernel_choice() == 2;
+    const int64_t PADM = p2 ? 2 * TILE : TILE;
+    const int64_t m_pad = (m + PADM - 1) / PADM * PADM, n_pad = (nn + PADM - 1) / PADM * PADM;
     const int tiles_m = (int)(m_pad / TILE), tiles_n = (int)(n_pad / TILE);
     std::vector<int2> tl;
     for (int tn = 0; tn < tiles_n; tn++)
@@ -1215,8 +1408,19 @@ int rns_gemm_device(const RnsGemmArgs &g) {
     HIP_CHECK(hipMemcpyAsync(d_crt, plan->c.data(), 8 * (size_t)L * J, hipMemcpyHostToDevice, s));
     HIP_CHECK(hipMemcpyAsync(d_crt + (size_t)L * J, plan->m.data(), 8 * (size_t)J,
                              hipMemcpyHostToDevice, s));
-    int2 *d_tiles = (int2 *)ws.get(S_TILES, sizeof(int2) * ntiles);
+    // 256 x 128 blocks of the second product kernel (one-digit moduli)
+    std::vector<int2> tl2;
+    if (p2)
+        for (int tn = 0; tn < tiles_n; tn++)
+            for (int tm2 = 0; tm2 < tiles_m / 2; tm2++)
+                if (!g.upper || (int64_t)tm2 * 2 * TILE <= (int64_t)tn * TILE + TILE - 1)
+                    tl2.push_back(make_int2(tm2, tn));
+    const int ntiles2 = (int)tl2.size();
+    int2 *d_tiles = (int2 *)ws.get(S_TILES, sizeof(int2) * (ntiles + ntiles2));
+    int2 *d_tiles2 = d_tiles + ntiles;
     HIP_CHECK(hipMemcpyAsync(d_tiles, tl.data(), sizeof(int2) * ntiles, hipMemcpyHostToDevice, s));
+    if (ntiles2)
+        HIP_CHECK(hipMemcpyAsync(d_tiles2, tl2.data(), sizeof(int2) * ntiles2, hipMemcpyHostToDevice, s));
 
     // 2. encode
     sev.mark(1, s);
@@ -1241,6 +1445,17 @@ int rns_gemm_device(const RnsGemmArgs &g) {
     for (int64_t k0 = 0; k0 < k_pad; k0 += KCHUNK) {
         const int64_t k1 = std::min<int64_t>(k_pad, k0 + KCHUNK);
         const int acc = k0 > 0 ? 1 : 0;
+        if (p2) {
+            if (single > 0)
+                launch_product2<1, 2 * TILE>(pla, plb, a_plane, b_plane, k_pad, k0, k1, d_q, d_inv, d_qinv,
+                                             single, 0, single, d_tiles2, ntiles2, tiles_m, tiles_n,
+                                             acc, X, s);
+            if (L > single)
+                launch_product2<2, TILE>(pla, plb, a_plane, b_plane, k_pad, k0, k1, d_q, d_inv, d_qinv,
+                                         single, single, L - single, d_tiles, ntiles, tiles_m, tiles_n,
+                                         acc, X, s);
+            continue;
+        }
         if (single > 0) {
             hipLaunchKernelGGL(product_kernel<1>, dim3(padded_grid(single * ntiles)), dim3(256), 0, s,
                                pla, plb, a_plane, b_plane, k_pad, k0, k1, d_q, d_inv, d_qinv,
