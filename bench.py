@@ -264,6 +264,38 @@ def chol_workload(args):
     sdpx_gpu.set_profiling(False)
     dt = sum(times) / len(times)
     st = sdpx_gpu.mp_ldl_stats()
+    # 16 Gaussian right-hand sides: L D L^T x = b on the device factor, then
+    # the residual b - A x with A x exactly rounded at 1024 bits (records
+    # widened exactly; one RNS product) - the 1024-bit reference residual
+    nrhs = 16
+    bmat = rng.standard_normal((n, nrhs))
+    db = to_device(mp.from_float64(bmat.T.reshape(-1), nl))
+    dx = db.clone()
+    torch.cuda.synchronize()
+    s0 = time.perf_counter()
+    sdpx_gpu.check(L.sdpx_gpu_mp_ldl_solve_dev(nl, n, work.data_ptr(), n, dinv.data_ptr(), nrhs,
+                                               dx.data_ptr(), n))
+    torch.cuda.synchronize()
+    solve_s = time.perf_counter() - s0
+    progress(f"16 right-hand sides solved ({solve_s:.2f} s)")
+
+    def widen(t, count):  # 8-limb records -> 16-limb records, same values
+        r = t.view(count, vb)
+        return torch.cat([torch.zeros(count, 64, dtype=torch.uint8, device="cuda"), r], dim=1).reshape(-1)
+
+    a16 = widen(da, n * n)
+    x16 = widen(dx, n * nrhs)
+    vb16 = mp.dtype(16).itemsize
+    ax16 = torch.empty(n * nrhs * vb16, dtype=torch.uint8, device="cuda")
+    code = sdpx_gpu.rns_gemm_dev("N", "N", n, nrhs, n, a16.data_ptr(), n, x16.data_ptr(), n, 16, False,
+                                 ax16.data_ptr(), n * nrhs)
+    assert code == sdpx_gpu.OK
+    del a16, x16
+    b16 = mp.from_float64(bmat.T.reshape(-1), 16)
+    resid = mp.sub_exact_log2(b16, ax16.cpu().numpy().view(mp.dtype(16)))
+    xv = mp.to_float64(dx.cpu().numpy().view(mp.dtype(nl)))
+    rel = resid - np.log2(max(np.abs(bmat).max(), 1e-300))
+    progress(f"residual max |b - A x| = 2^{resid:.1f} (1024-bit A x)")
     # float64 Cholesky of the same matrix (dynamic pivot rule: cond 2^100)
     a64 = torch.tensor((q * s) @ q.T, dtype=torch.float64, device="cuda")
     f = sdpx_gpu.DenseFactor(n)
@@ -285,7 +317,11 @@ def chol_workload(args):
                                                zip(("scan", "encode", "product", "crt"),
                                                    tot["stage_ms"])},
                    "mp_macs_per_factor": tot["mp_macs"] / args.steps,
-                   "fp64_gpu_s_per_factor": fdt, "mp512_over_fp64": dt / fdt},
+                   "fp64_gpu_s_per_factor": fdt, "mp512_over_fp64": dt / fdt,
+                   "solve_16rhs_s": solve_s,
+                   "residual_log2_max_abs": resid if np.isfinite(resid) else None,
+                   "residual_log2_rel_to_max_b": rel if np.isfinite(rel) else None,
+                   "solution_max_abs": float(np.abs(xv).max())},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": INT8_PEAK_OPS / 1e12,
                      "unit": "TOP/s (INT8)", "frac": achieved / INT8_PEAK_OPS, "traffic": None,
                      "kernel": "product_kernel (INT8 MFMA residue products)",

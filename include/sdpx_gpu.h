@@ -149,6 +149,9 @@ int32_t sdpx_gpu_mp_ldl_dev(uint32_t limbs, uint64_t n, void *a, uint64_t lda, c
  * exponent spread exceeds the GPU window. */
 int32_t sdpx_gpu_mp_ldl_solve(uint32_t limbs, uint64_t n, const void *l, uint64_t ldl,
                               const void *dinv, uint64_t nrhs, void *b, uint64_t ldb);
+/* Device-pointer form of sdpx_gpu_mp_ldl_solve (l, dinv, b in HBM). */
+int32_t sdpx_gpu_mp_ldl_solve_dev(uint32_t limbs, uint64_t n, const void *l, uint64_t ldl,
+                                  const void *dinv, uint64_t nrhs, void *b, uint64_t ldb);
 /* Counters of this thread's last sdpx_gpu_mp_ldl (measurement only). */
 int32_t sdpx_gpu_mp_ldl_stats(int64_t *gpu_products, int64_t *host_products, int64_t *regularized,
                               double *gpu_mp_macs);

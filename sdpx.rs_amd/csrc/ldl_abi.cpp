@@ -112,6 +112,22 @@ int32_t sdpx_gpu_mp_ldl_solve(uint32_t limbs, uint64_t n, const void *l, uint64_
     });
 }
 
+int32_t sdpx_gpu_mp_ldl_solve_dev(uint32_t limbs, uint64_t n, const void *l, uint64_t ldl,
+                                  const void *dinv, uint64_t nrhs, void *b, uint64_t ldb) {
+    return sdpx::abi_boundary([&]() -> int32_t {
+        if (!sdpx::mp_kernels_support((int)limbs))
+            return sdpx::abi_invalid("GPU solves support 128, 256 and 512 bits");
+        if (ldl < n || ldb < n) return sdpx::abi_invalid("leading dimension below n");
+        if (n == 0 || nrhs == 0) return SDPX_GPU_OK;
+        if (!l || !dinv || !b) return sdpx::abi_invalid("null array");
+        sdpx::Device &d = sdpx::device();
+        std::lock_guard<std::recursive_mutex> g(d.mu);
+        int r = sdpx::ldl_solve_device((int)limbs, (const uint8_t *)l, ldl, n, (const uint8_t *)dinv,
+                                       (uint8_t *)b, ldb, nrhs);
+        return r < 0 ? SDPX_GPU_DECLINED : SDPX_GPU_OK;
+    });
+}
+
 int32_t sdpx_gpu_mp_ldl_stats(int64_t *gpu_products, int64_t *host_products, int64_t *regularized,
                               double *gpu_mp_macs) {
     if (gpu_products) *gpu_products = t_ldl_stats.gpu_products;

@@ -846,12 +846,17 @@ __global__ __launch_bounds__(BS) void crt_kernel(
 // digit pairs (lo, hi), so V = sum_l x'_l (M/q_l) = sum_t 256^t R_t with
 //   R_t = sum_l lo_l c_{l,t} + hi_l c_{l,t-1}
 // (c the balanced base-256 digits of M/q_l): one INT8 product of the
-// (outputs x 2L) digit matrix with the (2L x nt) table, exact in INT32.  A
-// workgroup (2 waves) takes one 128-row column of an output tile: stage the
-// x digits through LDS as [output][2 lpad], 2 x nbl 32x32x32 MFMA blocks per
-// wave, R through LDS, then one thread per output carries R into 32-bit
-// limbs, removes k M with k = round(V / M) from the top limbs, and rounds
-// the signed result once, nearest-even, to 64N bits.
+// (nt x 2L) table with the (2L x outputs) digit matrix, exact in INT32.  A
+// workgroup (2 waves) takes one 128-row column of an output tile and stages
+// its x planes in LDS as they lie ([l][output], rows padded for the
+// fragment reads).  In the MFMA result a lane holds, for one output, the
+// four consecutive byte positions of every other 32-bit limb; one
+// cross-half shuffle per limb gives each lane all limbs of one output.  That
+// lane then carries, removes k M with k = round(V / M) from the top limbs,
+// and rounds the signed result once, nearest-even, to 64N bits.
+constexpr int CRT_XROW = TILE + 8;  // u16 per staged x plane row
+constexpr int CRT_MAXW = 16;        // mantissa words handled (up to 512 bits)
+
 template <int NUMAX, int NBMAX>
 __global__ __launch_bounds__(128) void crt_mfma_kernel(
     const uint16_t *__restrict__ X, int L, int lpad, const int2 *__restrict__ tiles, int ntiles,
@@ -860,11 +865,9 @@ __global__ __launch_bounds__(128) void crt_mfma_kernel(
     const int4 *__restrict__ scan_b, int64_t m, int64_t nn, int upper, int packed, int n,
     uint32_t *__restrict__ out) {
     extern __shared__ __attribute__((aligned(16))) int8_t csm[];
-    const int trow = 2 * lpad + 16;  // bytes per table row and per staged x row
-    const int rrow = npad + 4;       // int32 per R row
+    const int trow = 2 * lpad + 16;  // bytes per table row
     int8_t *stab = csm;
-    int8_t *uni = csm + (size_t)npad * trow;  // x staging [128][trow] or R [128][rrow]
-    int32_t *sR = (int32_t *)uni;
+    uint16_t *sx = (uint16_t *)(csm + (size_t)npad * trow);  // [lpad][CRT_XROW]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, h = lane >> 5, col = lane & 31;
     const int nks = lpad / 16, nbl = npad / 32;
     const int c16 = 2 * lpad / 16;
@@ -880,55 +883,67 @@ __global__ __launch_bounds__(128) void crt_mfma_kernel(
         const int jl = (int)(item % TILE);
         const int64_t j = (int64_t)tile.y * TILE + jl, i0 = (int64_t)tile.x * TILE;
         if (j >= nn || (upper && i0 > j)) continue;  // uniform over the workgroup
-        __syncthreads();  // the table is in; the previous item's R is read
+        __syncthreads();  // the table is in; the previous item's planes are read
         const uint16_t *xb = X + ((size_t)tile.y * tiles_m + tile.x) * (TILE * TILE) + (size_t)jl * TILE;
-        for (int e = tid; e < lpad * 64; e += 128) {
-            const int l = e >> 6, o2 = (e & 63) * 2;
-            uint32_t pr = 0;
-            if (l < L) pr = *(const uint32_t *)(xb + (size_t)l * xstride + o2);
-            *(uint16_t *)(uni + o2 * trow + 2 * l) = (uint16_t)(pr & 0xFFFF);
-            *(uint16_t *)(uni + (o2 + 1) * trow + 2 * l) = (uint16_t)(pr >> 16);
+        for (int e = tid; e < lpad * 16; e += 128) {
+            const int l = e >> 4, part = e & 15;
+            v4i v = v4i{0, 0, 0, 0};
+            if (l < L) v = *(const v4i *)(xb + (size_t)l * xstride + 8 * part);
+            *(v4i *)(sx + l * CRT_XROW + 8 * part) = v;
         }
         __syncthreads();
-        v16i acc[2][NBMAX];
+        // partial limb sum_b R_{32 nb + 8 g + 4 h + b} 256^b is limb 8 nb + 2 g + h
+        // of output (mt, o); lane h owns output o = 64 wave + 32 h + col, i.e.
+        // tile mt = h: after each tile's MFMAs the two halves trade partials and
+        // the owning half keeps its own and its partner's
+        int64_t full[NBMAX][8];
 #pragma unroll
-        for (int mt = 0; mt < 2; mt++)
+        for (int mt = 0; mt < 2; mt++) {
+            const int o = 64 * wave + 32 * mt + col;
+            v16i acc[NBMAX];
 #pragma unroll
-            for (int nb = 0; nb < NBMAX; nb++) {
-                acc[mt][nb] = v16i{0};
-                if (nb < nbl) {
-                    const int8_t *pa = uni + (64 * wave + 32 * mt + col) * trow + 16 * h;
-                    const int8_t *pb = stab + (32 * nb + col) * trow + 16 * h;
-                    for (int ks = 0; ks < nks; ks++)
-                        acc[mt][nb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
-                            *(const v4i *)(pa + 32 * ks), *(const v4i *)(pb + 32 * ks), acc[mt][nb], 0, 0, 0);
-                }
+            for (int nb = 0; nb < NBMAX; nb++) acc[nb] = v16i{0};
+            for (int ks = 0; ks < nks; ks++) {
+                const uint16_t *xs = sx + (16 * ks + 8 * h) * CRT_XROW + o;
+                uint32_t d[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    d[u] = (uint32_t)xs[(2 * u) * CRT_XROW] | ((uint32_t)xs[(2 * u + 1) * CRT_XROW] << 16);
+                const v4i fb = v4i{(int)d[0], (int)d[1], (int)d[2], (int)d[3]};
+#pragma unroll
+                for (int nb = 0; nb < NBMAX; nb++)
+                    if (nb < nbl)
+                        acc[nb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
+                            *(const v4i *)(stab + (32 * nb + col) * trow + 32 * ks + 16 * h), fb, acc[nb],
+                            0, 0, 0);
             }
-        __syncthreads();  // x staging fully read: R overwrites it
-#pragma unroll
-        for (int mt = 0; mt < 2; mt++)
 #pragma unroll
             for (int nb = 0; nb < NBMAX; nb++)
-                if (nb < nbl)
 #pragma unroll
-                    for (int r = 0; r < 16; r++) {
-                        const int o = 64 * wave + 32 * mt + 8 * (r >> 2) + 4 * h + (r & 3);
-                        sR[o * rrow + 32 * nb + col] = acc[mt][nb][r];
+                for (int g = 0; g < 4; g++) {
+                    const int64_t pv = (int64_t)acc[nb][4 * g] + (int64_t)acc[nb][4 * g + 1] * 256 +
+                                       (int64_t)acc[nb][4 * g + 2] * 65536 +
+                                       (int64_t)acc[nb][4 * g + 3] * 16777216;
+                    const int64_t got = __shfl_xor(pv, 32, 64);
+                    if (mt == 0 && h == 0) {  // static indices: no register spill
+                        full[nb][2 * g] = pv;
+                        full[nb][2 * g + 1] = got;
                     }
-        __syncthreads();
-        const int64_t i = i0 + tid;
+                    if (mt == 1 && h == 1) {
+                        full[nb][2 * g + 1] = pv;
+                        full[nb][2 * g] = got;
+                    }
+                }
+        }
+        const int64_t i = i0 + 64 * wave + lane;
         if (i >= m || (upper && i > j)) continue;
-        const int32_t *Rr = sR + tid * rrow;
         uint32_t lim[NUMAX];
         int64_t carry = 0;
 #pragma unroll
         for (int u = 0; u < NUMAX; u++) {
             lim[u] = 0;
             if (u < nu) {
-                int64_t a = carry;
-#pragma unroll
-                for (int b = 0; b < 4; b++)
-                    if (4 * u + b < nt) a += (int64_t)Rr[4 * u + b] * ((int64_t)1 << (8 * b));
+                const int64_t a = carry + (u < 8 * NBMAX ? full[u / 8][u % 8] : 0);
                 lim[u] = (uint32_t)a;
                 carry = a >> 32;
             }
@@ -993,7 +1008,7 @@ __global__ __launch_bounds__(128) void crt_mfma_kernel(
             }
         }
         // mantissa words: bits [shift, shift + P)
-        uint32_t mant[2 * MAX_LIMBS];
+        uint32_t mant[CRT_MAXW];
         if (shift >= 0) {
             const int qs = shift >> 5, rs = shift & 31;
 #pragma unroll
@@ -1003,7 +1018,7 @@ __global__ __launch_bounds__(128) void crt_mfma_kernel(
                     for (int u = 0; u < NUMAX; u++) lim[u] = (u + sft < NUMAX) ? lim[u + sft] : 0u;
                 }
 #pragma unroll
-            for (int w = 0; w < 2 * MAX_LIMBS; w++) {
+            for (int w = 0; w < CRT_MAXW; w++) {
                 const uint32_t lo = w < NUMAX ? lim[w] : 0u, hi = w + 1 < NUMAX ? lim[w + 1] : 0u;
                 mant[w] = __builtin_amdgcn_alignbit(hi, lo, rs);
             }
@@ -1016,7 +1031,7 @@ __global__ __launch_bounds__(128) void crt_mfma_kernel(
                     for (int u = NUMAX - 1; u >= 0; u--) lim[u] = (u - sft >= 0) ? lim[u - sft] : 0u;
                 }
 #pragma unroll
-            for (int w = 0; w < 2 * MAX_LIMBS; w++) {
+            for (int w = 0; w < CRT_MAXW; w++) {
                 const uint32_t cur = w < NUMAX ? lim[w] : 0u, prev = (w >= 1 && w - 1 < NUMAX) ? lim[w - 1] : 0u;
                 mant[w] = rs ? ((cur << rs) | (prev >> (32 - rs))) : cur;
             }
@@ -1024,14 +1039,14 @@ __global__ __launch_bounds__(128) void crt_mfma_kernel(
         if (roundb && (sticky || (mant[0] & 1u))) {
             uint32_t c = 1;
 #pragma unroll
-            for (int w = 0; w < 2 * MAX_LIMBS; w++)
+            for (int w = 0; w < CRT_MAXW; w++)
                 if (w < 2 * n) {
                     mant[w] += c;
                     c = c && mant[w] == 0;
                 }
             if (c) {
 #pragma unroll
-                for (int w = 0; w < 2 * MAX_LIMBS; w++)
+                for (int w = 0; w < CRT_MAXW; w++)
                     if (w < 2 * n) mant[w] = (w == 2 * n - 1) ? 0x80000000u : 0u;
                 shift += 1;
             }
@@ -1050,7 +1065,7 @@ __global__ __launch_bounds__(128) void crt_mfma_kernel(
             continue;
         }
 #pragma unroll
-        for (int w = 0; w < 2 * MAX_LIMBS; w++)
+        for (int w = 0; w < CRT_MAXW; w++)
             if (w < 2 * n) rec[w] = mant[w];
         rec[2 * n] = (uint32_t)(negative ? -KIND_REGULAR : KIND_REGULAR);
         rec[2 * n + 1] = (uint32_t)(int32_t)expo;
@@ -1253,12 +1268,15 @@ struct StageEvents {
 
 unsigned padded_grid(int64_t total) { return (unsigned)((total + 7) / 8 * 8); }
 
-// Product kernel selection: 2 (double-buffered, 256 x 128 one-digit tiles)
-// unless SDPX_GPU_PRODUCT_KERNEL=1 selects the first form (for comparison).
+// Product kernel selection (measured on MI355X, 4096^3 at 256 bits): one-digit
+// moduli use product_kernel<1> (128 x 128 tiles, three workgroups per CU),
+// two-digit moduli product2_kernel<2, 128> (double-buffered).
+// SDPX_GPU_PRODUCT_KERNEL=1 selects the first form for both, =2 the
+// double-buffered form for both (256 x 128 one-digit tiles), for comparison.
 int product_kernel_choice() {
     static const int choice = [] {
         const char *e = std::getenv("SDPX_GPU_PRODUCT_KERNEL");
-        return (e && e[0] == '1') ? 1 : 2;
+        return e && e[0] == '1' ? 1 : (e && e[0] == '2' ? 2 : 0);
     }();
     return choice;
 }
@@ -1445,24 +1463,21 @@ int rns_gemm_device(const RnsGemmArgs &g) {
     for (int64_t k0 = 0; k0 < k_pad; k0 += KCHUNK) {
         const int64_t k1 = std::min<int64_t>(k_pad, k0 + KCHUNK);
         const int acc = k0 > 0 ? 1 : 0;
-        if (p2) {
-            if (single > 0)
-                launch_product2<1, 2 * TILE>(pla, plb, a_plane, b_plane, k_pad, k0, k1, d_q, d_inv, d_qinv,
-                                             single, 0, single, d_tiles2, ntiles2, tiles_m, tiles_n,
-                                             acc, X, s);
-            if (L > single)
-                launch_product2<2, TILE>(pla, plb, a_plane, b_plane, k_pad, k0, k1, d_q, d_inv, d_qinv,
-                                         single, single, L - single, d_tiles, ntiles, tiles_m, tiles_n,
-                                         acc, X, s);
-            continue;
-        }
-        if (single > 0) {
+        if (p2 && single > 0)
+            launch_product2<1, 2 * TILE>(pla, plb, a_plane, b_plane, k_pad, k0, k1, d_q, d_inv, d_qinv,
+                                         single, 0, single, d_tiles2, ntiles2, tiles_m, tiles_n, acc,
+                                         X, s);
+        if (product_kernel_choice() != 1 && L > single)
+            launch_product2<2, TILE>(pla, plb, a_plane, b_plane, k_pad, k0, k1, d_q, d_inv, d_qinv,
+                                     single, single, L - single, d_tiles, ntiles, tiles_m, tiles_n, acc,
+                                     X, s);
+        if (!p2 && single > 0) {
             hipLaunchKernelGGL(product_kernel<1>, dim3(padded_grid(single * ntiles)), dim3(256), 0, s,
                                pla, plb, a_plane, b_plane, k_pad, k0, k1, d_q, d_inv, d_qinv,
                                single, 0, single, d_tiles, ntiles, tiles_m, tiles_n, acc, X);
             HIP_CHECK(hipGetLastError());
         }
-        if (L > single) {
+        if (product_kernel_choice() == 1 && L > single) {
             hipLaunchKernelGGL(product_kernel<2>, dim3(padded_grid((L - single) * ntiles)),
                                dim3(256), 0, s, pla, plb, a_plane, b_plane, k_pad, k0, k1, d_q,
                                d_inv, d_qinv, single, single, L - single, d_tiles, ntiles, tiles_m,
@@ -1485,8 +1500,8 @@ int rns_gemm_device(const RnsGemmArgs &g) {
         uint8_t *d_ct = (uint8_t *)ws.get(S_CRT2, tb + 4 * (size_t)nu);
         HIP_CHECK(hipMemcpyAsync(d_ct, plan->crt_table.data(), tb, hipMemcpyHostToDevice, s));
         HIP_CHECK(hipMemcpyAsync(d_ct + tb, plan->m32.data(), 4 * (size_t)nu, hipMemcpyHostToDevice, s));
-        const int trow = 2 * lpad + 16, rrow = npad + 4;
-        const size_t lds = (size_t)npad * trow + std::max<size_t>((size_t)TILE * trow, (size_t)TILE * rrow * 4);
+        const int trow = 2 * lpad + 16;
+        const size_t lds = (size_t)npad * trow + (size_t)lpad * CRT_XROW * 2;
         const int64_t items = (int64_t)ntiles * TILE;
         const unsigned grid = (unsigned)std::min<int64_t>(items, 4096);
         auto launch = [&](auto kern) {

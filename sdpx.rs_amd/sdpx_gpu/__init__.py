@@ -109,8 +109,9 @@ def _declare(L):
     L.sdpx_gpu_mp_ldl_dev.argtypes = [u32, u64, vp, u64, vp, u64, vp, vp, vp, u32,
                                       ctypes.POINTER(i32)]
     L.sdpx_gpu_mp_ldl_dev.restype = i32
-    L.sdpx_gpu_mp_ldl_solve.argtypes = [u32, u64, vp, u64, vp, u64, vp, u64]
-    L.sdpx_gpu_mp_ldl_solve.restype = i32
+    for name in ("sdpx_gpu_mp_ldl_solve", "sdpx_gpu_mp_ldl_solve_dev"):
+        getattr(L, name).argtypes = [u32, u64, vp, u64, vp, u64, vp, u64]
+        getattr(L, name).restype = i32
     L.sdpx_gpu_sampled_arrow_dev.argtypes = [u32, u64, vp, u64, vp, vp, vp, vp, vp, u64, vp,
                                              ctypes.POINTER(i32)]
     L.sdpx_gpu_sampled_arrow_dev.restype = i32

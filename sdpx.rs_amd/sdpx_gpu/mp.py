@@ -147,3 +147,31 @@ def same_bits(x, y):
     eq = (np.all(x["limbs"] == y["limbs"], axis=-1) & (x["kind"] == y["kind"])
           & (x["exponent"] == y["exponent"]))
     return eq | (zx & zy)
+
+
+def _exact(rec):
+    """(integer, binary scale) with value = integer * 2^scale, exactly."""
+    n = limbs_of(np.asarray([rec]) if np.ndim(rec) == 0 else rec)
+    kind = int(rec["kind"])
+    if abs(kind) != REGULAR_KIND:
+        return 0, 0
+    m = 0
+    for i, w in enumerate(rec["limbs"].tolist()):
+        m |= int(w) << (64 * i)
+    if kind < 0:
+        m = -m
+    return m, int(rec["exponent"]) - 64 * n
+
+
+def sub_exact_log2(b, c):
+    """log2 of max_i |b_i - c_i| computed exactly (regular or zero records of
+    any equal width); -inf when every difference is zero."""
+    best = -np.inf
+    for x, y in zip(b, c):
+        mx, ex = _exact(x)
+        my, ey = _exact(y)
+        e = min(ex, ey)
+        d = (mx << (ex - e)) - (my << (ey - e))
+        if d:
+            best = max(best, (abs(d).bit_length() - 1) + e)
+    return float(best)

@@ -88,3 +88,28 @@ def test_solves_residual_vs_1024_bit_oracle(gpu):
         worst = max(worst, float(d / M.Fraction(scale)) if d else 0.0)
     assert worst < 2.0 ** -400, worst
     assert np.abs(r).max() <= 1e-12 * scale
+
+
+def test_device_pointer_solve_matches_host_entry(gpu):
+    # sdpx_gpu_mp_ldl_solve_dev on HBM-resident factor and right-hand sides
+    # gives the host-pointer entry's bits
+    import torch
+    rng = np.random.default_rng(21)
+    nl, n, nrhs = 4, 300, 5
+    s = spread_spd(n, rng, -20)
+    s = (s + s.T) / 2
+    A = mp.from_float64(s.T.reshape(-1), nl)
+    okg, L, dinv = sdpx_gpu.mp_ldl(A, n)
+    assert okg
+    b = mp.from_float64(rng.standard_normal(n * nrhs), nl)
+    x_host = sdpx_gpu.mp_ldl_solve(L, dinv, n, b)
+
+    def dev(a):
+        return torch.from_numpy(np.ascontiguousarray(a).view(np.uint8).copy()).to("cuda")
+
+    dl, dd, db = dev(L), dev(dinv), dev(b)
+    sdpx_gpu.check(sdpx_gpu.lib().sdpx_gpu_mp_ldl_solve_dev(nl, n, dl.data_ptr(), n, dd.data_ptr(), nrhs,
+                                                            db.data_ptr(), n))
+    torch.cuda.synchronize()
+    x_dev = db.cpu().numpy().view(mp.dtype(nl))
+    assert np.all(mp.same_bits(x_dev, x_host))
