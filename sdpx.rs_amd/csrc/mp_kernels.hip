@@ -121,27 +121,95 @@ struct Win {
 // across the group (integer adds of the unresolved slots: the same sum a
 // single lane would hold after all k1 - k0 terms), and every lane then
 // carries and rounds the identical total, so all lanes hold the result.
-template <int G>
-__device__ __forceinline__ int64_t group_sum(int64_t v) {
-#pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+// 16-lane sum of a 64-bit value inside one DPP row: quad permutations
+// [1,0,3,2] and [2,3,0,1], then the half-row and row mirrors; every lane
+// of the row ends with the row's total.
+template <int CTRL>
+__device__ __forceinline__ int64_t dpp_add64(int64_t v) {
+    const int lo = (int)(uint32_t)(uint64_t)v, hi = (int)((uint64_t)v >> 32);
+    const int plo = __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, false);
+    const int phi = __builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, false);
+    return v + (int64_t)(((uint64_t)(uint32_t)phi << 32) | (uint32_t)plo);
+}
+__device__ __forceinline__ int64_t row16_sum(int64_t v) {
+    v = dpp_add64<0xB1>(v);   // quad_perm [1,0,3,2]
+    v = dpp_add64<0x4E>(v);   // quad_perm [2,3,0,1]
+    v = dpp_add64<0x141>(v);  // row_half_mirror
+    v = dpp_add64<0x140>(v);  // row_mirror
     return v;
 }
 
-// hint_lo <= hint_hi, when given, bound the scales of every non-zero term
-// (from running exponent ranges); a window that fits them skips the exact
-// range pass.  Any window holding all terms gives the same exact sum, and a
-// hint too wide for the window falls back to the exact pass, so the result
-// and the overflow decision never depend on the hint.
+// Carry-resolve a window of SLOTS signed 64-bit slots (value
+// sum_k v_k 2^(32 k + base)) in registers and round it once into out: the
+// same result as DotWindow::finish, with the word gather done by log-step
+// shifts so every register index is static.
+template <int W, int SLOTS>
+__device__ inline void finish_regs(const int64_t (&v)[SLOTS], int64_t base, Val<W> &out) {
+    uint32_t w[SLOTS];
+    int64_t carry = 0;
+#pragma unroll
+    for (int k = 0; k < SLOTS; k++) {
+        const int64_t t = v[k] + carry;
+        w[k] = (uint32_t)(uint64_t)t;
+        carry = t >> 32;
+    }
+    const bool neg = carry < 0;
+    if (neg) {
+        uint64_t c = 1;
+#pragma unroll
+        for (int k = 0; k < SLOTS; k++) {
+            const uint64_t t = (uint64_t)(~w[k]) + c;
+            w[k] = (uint32_t)t;
+            c = t >> 32;
+        }
+    }
+    int top = -1;
+#pragma unroll
+    for (int k = 0; k < SLOTS; k++)
+        if (w[k]) top = k;
+    if (top < 0) {
+        set_zero(out);
+        return;
+    }
+    constexpr int GW = W + 2;
+    const int lo = top - GW + 1;
+    bool sticky = false;
+#pragma unroll
+    for (int k = 0; k < SLOTS; k++)
+        if (k < lo && w[k]) sticky = true;
+    if (lo >= 0) {  // w >>= lo words
+#pragma unroll
+        for (int sft = 32; sft >= 1; sft >>= 1)
+            if (lo & sft) {
+#pragma unroll
+                for (int k = 0; k < SLOTS; k++) w[k] = k + sft < SLOTS ? w[k + sft] : 0u;
+            }
+    } else {  // w <<= -lo words (the value sits in the low GW words)
+        const int ls = -lo;
+#pragma unroll
+        for (int sft = 16; sft >= 1; sft >>= 1)
+            if (ls & sft) {
+#pragma unroll
+                for (int k = GW - 1; k >= 0; k--) w[k] = k - sft >= 0 ? w[k - sft] : 0u;
+            }
+    }
+    uint32_t g[GW + 1];
+    g[0] = sticky ? 1u : 0u;  // one sticky bit below the gathered words
+#pragma unroll
+    for (int i = 0; i < GW; i++) g[i + 1] = i < SLOTS ? w[i] : 0u;
+    round_words<W, GW + 1>(neg, g, base + 32 * (int64_t)(lo - 1), out);
+}
+
 // Terms x_k = xb[k * xs], y_k = yb[k * ys] (record strides): every panel
-// kernel's operands are affine in k, so one out-of-line copy per width
-// serves all call sites.
-template <int W, int G>
+// kernel's operands are affine in k, so one out-of-line copy per width and
+// slot stride serves all call sites.  Groups are 16 lanes (one DPP row).
+template <int W, int G, int STRIDE>
 __device__ __attribute__((noinline)) bool group_dot(int64_t k0, int64_t k1, int gl,
                                                     const uint32_t *xb, int64_t xs,
                                                     const uint32_t *yb, int64_t ys, const Val<W> *c,
-                                                    bool negate_terms, int64_t *slots, int stride,
-                                                    Val<W> &out, int64_t hint_lo, int64_t hint_hi) {
+                                                    bool negate_terms, int64_t *slots, Val<W> &out,
+                                                    int64_t hint_lo, int64_t hint_hi) {
+    static_assert(G == 16, "group reductions run inside one 16-lane DPP row");
     constexpr int SLOTS = Win<W>::SLOTS;
     auto fx = [&](int64_t k) { return xb + k * xs * (W + 2); };
     auto fy = [&](int64_t k) { return yb + k * ys * (W + 2); };
@@ -151,21 +219,21 @@ __device__ __attribute__((noinline)) bool group_dot(int64_t k0, int64_t k1, int 
         lo = hint_lo;
         hi = hint_hi;
     } else {
-    for (int64_t k = kf; k < k1; k += G) {
-        const uint32_t *x = fx(k), *y = fy(k);
-        const int32_t kx = (int32_t)x[W], ky = (int32_t)y[W];
-        if (kx == mpd::KIND_ZERO || kx == -mpd::KIND_ZERO || ky == mpd::KIND_ZERO ||
-            ky == -mpd::KIND_ZERO)
-            continue;
-        const int64_t sc = (int64_t)(int32_t)x[W + 1] + (int32_t)y[W + 1] - 64 * (int64_t)W;
-        lo = min(lo, sc);
-        hi = max(hi, sc);
-    }
+        for (int64_t k = kf; k < k1; k += G) {
+            const uint32_t *x = fx(k), *y = fy(k);
+            const int32_t kx = (int32_t)x[W], ky = (int32_t)y[W];
+            if (kx == mpd::KIND_ZERO || kx == -mpd::KIND_ZERO || ky == mpd::KIND_ZERO ||
+                ky == -mpd::KIND_ZERO)
+                continue;
+            const int64_t sc = (int64_t)(int32_t)x[W + 1] + (int32_t)y[W + 1] - 64 * (int64_t)W;
+            lo = min(lo, sc);
+            hi = max(hi, sc);
+        }
 #pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1) {
-        lo = min(lo, (int64_t)__shfl_xor(lo, off, 64));
-        hi = max(hi, (int64_t)__shfl_xor(hi, off, 64));
-    }
+        for (int off = G / 2; off > 0; off >>= 1) {
+            lo = min(lo, (int64_t)__shfl_xor(lo, off, 64));
+            hi = max(hi, (int64_t)__shfl_xor(hi, off, 64));
+        }
     }
     bool any = lo != INT64_MAX;
     const bool cnz = c && !is_zero(*c);
@@ -180,25 +248,30 @@ __device__ __attribute__((noinline)) bool group_dot(int64_t k0, int64_t k1, int 
         return true;
     }
     if ((hi - lo) / 32 + 2 * W + 4 > SLOTS) return false;
-    DotWindow<W, SLOTS> dw{slots, stride, lo};
-    dw.clear();
-    for (int64_t k = kf; k < k1; k += G) {
-        Val<W> x, y;
-        load(fx(k), x);
-        load(fy(k), y);
+    // terms land in slots [0, (hi - lo)/32 + 2W]
+    const int used = (int)min((int64_t)SLOTS, (hi - lo) / 32 + 2 * W + 1);
+    for (int sl = 0; sl < used; sl++) slots[(int64_t)sl * STRIDE] = 0;
+    DotWindow<W, SLOTS> dw{slots, STRIDE, lo};
+    Val<W> xn, yn;
+    int64_t k = kf;
+    if (k < k1) {
+        load(fx(k), xn);
+        load(fy(k), yn);
+    }
+    for (; k < k1; k += G) {
+        const Val<W> x = xn, y = yn;
+        if (k + G < k1) {  // next term in flight during this one
+            load(fx(k + G), xn);
+            load(fy(k + G), yn);
+        }
         if (is_zero(x) || is_zero(y)) continue;
         dw.add_product(x, y, negate_terms);
     }
     if (cnz && gl == 0) dw.add_value(*c, false);
-    if (G > 1) {
-        // products land in slots [0, (hi - lo)/32 + 2W]; the rest stay zero
-        const int used = (int)min((int64_t)SLOTS, (hi - lo) / 32 + 2 * W + 1);
-        for (int sl = 0; sl < used; sl++) {
-            int64_t *p = slots + (int64_t)sl * stride;
-            *p = group_sum<G>(*p);
-        }
-    }
-    dw.finish(out);
+    int64_t v[SLOTS];
+#pragma unroll
+    for (int sl = 0; sl < SLOTS; sl++) v[sl] = sl < used ? row16_sum(slots[(int64_t)sl * STRIDE]) : 0;
+    finish_regs<W, SLOTS>(v, lo, out);
     return true;
 }
 
@@ -256,7 +329,6 @@ __global__ __launch_bounds__(ROW_THREADS) void forward_rows_kernel(
     const int64_t c = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
     if (c >= t) return;  // whole group
     int64_t *slots = slots_sh + threadIdx.x;
-    const int stride = blockDim.x;
     uint32_t *col = Y + c * K * (W + 2);
     int32_t ymn = INT32_MAX, ymx = INT32_MIN;  // y_k so far
     for (int64_t i = 0; i < K; i++) {
@@ -265,8 +337,8 @@ __global__ __launch_bounds__(ROW_THREADS) void forward_rows_kernel(
         // y_i = RN(b_i - sum_{k in [first_i, i)} L_ik y_k)
         int64_t hl = INT64_MAX, hh = INT64_MIN;
         if (lr) range_hint<W>(lr[i].x, lr[i].y, ymn, ymx, hl, hh);
-        if (!group_dot<W, G>(first[i], i, gl, rec_at<W>(L, i), K, col, 1, &b, true, slots, stride, r,
-                             hl, hh)) {
+        if (!group_dot<W, G, ROW_THREADS>(first[i], i, gl, rec_at<W>(L, i), K, col, 1, &b, true, slots, r,
+                                          hl, hh)) {
             if (gl == 0) atomicOr(fail, 1);
             return;
         }
@@ -286,15 +358,14 @@ __global__ __launch_bounds__(ROW_THREADS) void trsm_base_kernel(
     const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
     if (i >= m) return;
     int64_t *slots = slots_sh + threadIdx.x;
-    const int stride = blockDim.x;
     const int64_t lo = low[i];
     int32_t xmn = INT32_MAX, xmx = INT32_MIN;  // x_ik so far
     for (int64_t j = lo; j < n; j++) {
         Val<W> t, x, r;
         int64_t hl = INT64_MAX, hh = INT64_MIN;
         if (lr) range_hint<W>(xmn, xmx, lr[j].x, lr[j].y, hl, hh);
-        if (!group_dot<W, G>(lo, j, gl, rec_at<W>((const uint32_t *)B, i), ldb, rec_at<W>(L, j), ldl,
-                             nullptr, false, slots, stride, t, hl, hh)) {
+        if (!group_dot<W, G, ROW_THREADS>(lo, j, gl, rec_at<W>((const uint32_t *)B, i), ldb, rec_at<W>(L, j),
+                                          ldl, nullptr, false, slots, t, hl, hh)) {
             if (gl == 0) atomicOr(fail, 1);
             return;
         }
@@ -317,14 +388,13 @@ __global__ __launch_bounds__(ROW_THREADS) void trsm_base_n_kernel(
     const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
     if (i >= m) return;
     int64_t *slots = slots_sh + threadIdx.x;
-    const int stride = blockDim.x;
     int32_t xmn = INT32_MAX, xmx = INT32_MIN;  // x_ik so far (k > j)
     for (int64_t j = n - 1; j >= 0; j--) {
         Val<W> t, x, r;
         int64_t hl = INT64_MAX, hh = INT64_MIN;
         if (lc) range_hint<W>(xmn, xmx, lc[j].x, lc[j].y, hl, hh);
-        if (!group_dot<W, G>(j + 1, n, gl, rec_at<W>((const uint32_t *)B, i), ldb, rec_at<W>(L, j * ldl),
-                             1, nullptr, false, slots, stride, t, hl, hh)) {
+        if (!group_dot<W, G, ROW_THREADS>(j + 1, n, gl, rec_at<W>((const uint32_t *)B, i), ldb,
+                                          rec_at<W>(L, j * ldl), 1, nullptr, false, slots, t, hl, hh)) {
             if (gl == 0) atomicOr(fail, 1);
             return;
         }
@@ -371,6 +441,7 @@ __global__ __launch_bounds__(ldl_threads<W>()) void ldl_dots_kernel(
     const LdlBlock *blocks, uint32_t *scratch, const uint32_t *reg_eps, const uint32_t *reg_delta,
     int *status) {
     constexpr int SLOTS = Win<W>::SLOTS;
+    constexpr int TH = ldl_threads<W>();
     extern __shared__ int64_t slots_sh[];
     __shared__ int bad;
     const LdlBlock B = blocks[blockIdx.x];
@@ -384,7 +455,6 @@ __global__ __launch_bounds__(ldl_threads<W>()) void ldl_dots_kernel(
     uint32_t *dstore = w + tri(n) * (W + 2);  // n pivots
     const int nthreads = blockDim.x;
     int64_t *slots = slots_sh + threadIdx.x;
-    const int stride = nthreads;
     const int gl = threadIdx.x & (G - 1), grp = threadIdx.x / G, groups = nthreads / G;
     int32_t *ex = (int32_t *)(slots_sh + (size_t)SLOTS * nthreads);
     int32_t *rmin = ex, *rmax = ex + LDL_STATS_MAX, *wmin = ex + 2 * LDL_STATS_MAX,
@@ -415,12 +485,12 @@ __global__ __launch_bounds__(ldl_threads<W>()) void ldl_dots_kernel(
         const uint32_t *yr = rec_at<W>((const uint32_t *)w, tri(j));
         Val<W> ajj, t, d;
         load(rec_at<W>(a, j + j * ld), ajj);
-        bool ok = group_dot<W, G>(fj, j, gl, xr, 1, yr, 1, nullptr, false, slots, stride, t, hl, hh);
+        bool ok = group_dot<W, G, TH>(fj, j, gl, xr, 1, yr, 1, nullptr, false, slots, t, hl, hh);
         if (ok) {
             add_signed(ajj, t, true, d);
             // exact retry of a cancelled pivot (uniform over the group)
             if (is_zero(d) && j > fj)
-                ok = group_dot<W, G>(fj, j, gl, xr, 1, yr, 1, &ajj, true, slots, stride, d, hl, hh);
+                ok = group_dot<W, G, TH>(fj, j, gl, xr, 1, yr, 1, &ajj, true, slots, d, hl, hh);
         }
         if (gl != 0) return;
         if (!ok) {
@@ -464,9 +534,9 @@ __global__ __launch_bounds__(ldl_threads<W>()) void ldl_dots_kernel(
             else
                 hint(i, j, 1, 0, 1, 0, hl, hh);
             Val<W> t, aij, diff;
-            if (!group_dot<W, G>(lo, j, gl, rec_at<W>((const uint32_t *)r, tri(i)), 1,
-                                 rec_at<W>((const uint32_t *)w, tri(j)), 1, nullptr, false, slots,
-                                 stride, t, hl, hh))
+            if (!group_dot<W, G, TH>(lo, j, gl, rec_at<W>((const uint32_t *)r, tri(i)), 1,
+                                     rec_at<W>((const uint32_t *)w, tri(j)), 1, nullptr, false, slots,
+                                     t, hl, hh))
                 return false;
             load(rec_at<W>(a, i + j * ld), aij);
             add_signed(aij, t, true, diff);

@@ -38,7 +38,13 @@ typedef int v16i __attribute__((ext_vector_type(16)));
 
 constexpr int TILE = 128;        // output tile edge of the product kernel
 constexpr int KSTEP = 64;        // inner-dimension bytes per LDS stage
-constexpr int LDS_ROW = 80;      // padded LDS row (bytes): conflict-free b128 reads
+constexpr int LDS_ROW = KSTEP;   // LDS row (bytes) of one staged operand row
+// 16-byte chunk c of staged row r sits at chunk c ^ ((r >> 2) & 3): the 16
+// lanes of a b128 access (4 rows x 4 chunks on the store side, 16 rows x 1
+// chunk on the MFMA fragment side) then hit 16 distinct 16-byte banks.
+__device__ __forceinline__ int lds_off(int r, int byte) {
+    return r * LDS_ROW + (((byte >> 4) ^ ((r >> 2) & 3)) << 4);
+}
 constexpr int KCHUNK = 32768;    // inner elements per exact INT32 accumulation
 constexpr int XPAD = 130;        // padded LDS column of the transposed x tile
 
@@ -449,10 +455,10 @@ __global__ __launch_bounds__(256) void product_kernel(
     auto store = [&]() {
 #pragma unroll
         for (int d = 0; d < D; d++) {
-            *(v4i *)(sA + d * PLANE_TILE + r0 * LDS_ROW + o0) = ra[d][0];
-            *(v4i *)(sA + d * PLANE_TILE + r1 * LDS_ROW + o1) = ra[d][1];
-            *(v4i *)(sB + d * PLANE_TILE + r0 * LDS_ROW + o0) = rb[d][0];
-            *(v4i *)(sB + d * PLANE_TILE + r1 * LDS_ROW + o1) = rb[d][1];
+            *(v4i *)(sA + d * PLANE_TILE + lds_off(r0, o0)) = ra[d][0];
+            *(v4i *)(sA + d * PLANE_TILE + lds_off(r1, o1)) = ra[d][1];
+            *(v4i *)(sB + d * PLANE_TILE + lds_off(r0, o0)) = rb[d][0];
+            *(v4i *)(sB + d * PLANE_TILE + lds_off(r1, o1)) = rb[d][1];
         }
     };
 
@@ -480,9 +486,9 @@ __global__ __launch_bounds__(256) void product_kernel(
 #pragma unroll
                 for (int i = 0; i < 2; i++) {
                     fa[d][i] = *(const v4i *)(sA + d * PLANE_TILE +
-                                              (wm * 64 + i * 32 + frag_row) * LDS_ROW + kk + frag_k);
+                                              lds_off(wm * 64 + i * 32 + frag_row, kk + frag_k));
                     fb[d][i] = *(const v4i *)(sB + d * PLANE_TILE +
-                                              (wn * 64 + i * 32 + frag_row) * LDS_ROW + kk + frag_k);
+                                              lds_off(wn * 64 + i * 32 + frag_row, kk + frag_k));
                 }
 #pragma unroll
             for (int i = 0; i < 2; i++)
@@ -601,7 +607,7 @@ __global__ __launch_bounds__(256) void product2_kernel(
         const int8_t *base = rr < BM ? A + (size_t)(plane + d) * a_plane + ((size_t)tm2 * BM + rr) * k_pad
                                      : B + (size_t)(plane + d) * b_plane + ((size_t)tn * TILE + rr - BM) * k_pad;
         src[u] = base + 16 * part;
-        dst[u] = d * ROWS * LDS_ROW + rr * LDS_ROW + 16 * part;
+        dst[u] = d * ROWS * LDS_ROW + lds_off(rr, 16 * part);
     }
     v4i stg[CH];
     auto gload = [&](int64_t k) {
@@ -639,11 +645,11 @@ __global__ __launch_bounds__(256) void product2_kernel(
 #pragma unroll
                 for (int i = 0; i < MI; i++)
                     fa[d][i] = *(const v4i *)(sb + d * ROWS * LDS_ROW +
-                                              (wm * (BM / 2) + i * 32 + frag_row) * LDS_ROW + kk + frag_k);
+                                              lds_off(wm * (BM / 2) + i * 32 + frag_row, kk + frag_k));
 #pragma unroll
                 for (int j = 0; j < 2; j++)
                     fb[d][j] = *(const v4i *)(sb + d * ROWS * LDS_ROW +
-                                              (BM + wn * 64 + j * 32 + frag_row) * LDS_ROW + kk + frag_k);
+                                              lds_off(BM + wn * 64 + j * 32 + frag_row, kk + frag_k));
             }
 #pragma unroll
             for (int i = 0; i < MI; i++)
