@@ -133,7 +133,7 @@ def sampled_workload(args):
 
     nl = 4
     sdpx_gpu.check(sdpx_gpu.lib().sdpx_gpu_init(0))
-    prob = workloads.sampled_problem(nl, seed=7)
+    prob = workloads.sampled_problem(nl, seed=7, t=args.border)
     t = prob["t"]
 
     run = sdpx_gpu.SampledDev(prob, nl)   # inputs resident in HBM
@@ -179,7 +179,7 @@ def sampled_workload(args):
         "value": dt, "unit": "s/IPM-iter", "ms_per_step": dt * 1e3,
         "dtype": "mp256 (int8 MFMA residues, int32 accumulation)",
         "config": {"workload": "sampled bootstrap SDP, 40 PSD blocks h~U{10..60}, K=2h-1, "
-                               "2000 border rows, MPFR-256: Schur build + arrow LDL^T",
+                               f"{t} border rows, MPFR-256: Schur build + arrow LDL^T",
                    "precision_bits": 256, "blocks": len(prob["blocks"]), "border": t,
                    "samples": int(sum(b["k"] for b in prob["blocks"])),
                    "phase_s": {k: phases[k] for k in ("build_s", "leaves_s", "border_s")},
@@ -194,7 +194,7 @@ def sampled_workload(args):
                      "kernel": "product_kernel (INT8 MFMA residue products)",
                      "int8_ops_per_step": tot["int8_ops"]},
     }
-    if args.cpu_baseline:
+    if args.cpu_baseline and t <= 2000:  # larger borders: minutes of CPU time
         sys.path[:0] = [os.path.join(ROOT, "tests"), os.path.join(ROOT, "oracle")]
         from helpers import oracle_sampled_arrow
 
@@ -402,6 +402,8 @@ def main():
     ap.add_argument("--workload", default="sampled", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
     ap.add_argument("--order", type=int, default=8192, help="matrix order of --workload chol")
+    ap.add_argument("--border", type=int, default=2000,
+                    help="border rows (free variables) of --workload sampled")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run one device-to-device copy of MIB MiB (a known-traffic kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")

@@ -69,9 +69,39 @@ __host__ __device__ inline int bitlen32(uint32_t x) {
 #endif
 }
 
+// (hi:lo) >> sh, 0 <= sh < 32: the low word of the 64-bit funnel shift
+__host__ __device__ inline uint32_t funnel(uint32_t hi, uint32_t lo, int sh) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_alignbit(hi, lo, (uint32_t)sh);
+#else
+    return (uint32_t)((((uint64_t)hi << 32) | lo) >> sh);
+#endif
+}
+
+// Word shifts of a register array by a run-time count, in log steps so every
+// register index is static: t >>= q words (zero fill), t <<= q words.
+template <int L>
+__host__ __device__ inline void shr_words(uint32_t (&t)[L], int q) {
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1)
+        if (s < 2 * L && (q & s)) {
+#pragma unroll
+            for (int i = 0; i < L; i++) t[i] = i + s < L ? t[i + s] : 0u;
+        }
+}
+template <int L>
+__host__ __device__ inline void shl_words(uint32_t (&t)[L], int q) {
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1)
+        if (s < 2 * L && (q & s)) {
+#pragma unroll
+            for (int i = L - 1; i >= 0; i--) t[i] = i - s >= 0 ? t[i - s] : 0u;
+        }
+}
+
 // Round (-1)^neg * mag * 2^scale once (mag: L words, LSB first) into out.
-// Indexing is static (the word selection loops over all words), so mag may
-// live in registers.
+// Every register index is static (log-step word shifts), so mag may live in
+// registers.
 template <int W, int L>
 __host__ __device__ inline void round_words(bool neg, const uint32_t (&mag)[L], int64_t scale,
                                             Val<W> &out) {
@@ -90,21 +120,9 @@ __host__ __device__ inline void round_words(bool neg, const uint32_t (&mag)[L], 
     const int P = 32 * W;
     const int bits = 32 * top + bitlen32(topw);
     int shift = bits - P;
-    // mantissa word u = bits [shift + 32u, +32)
-#pragma unroll
-    for (int u = 0; u < W; u++) {
-        const int b = shift + 32 * u;
-        uint32_t w = 0;
-#pragma unroll
-        for (int i = 0; i < L; i++) {
-            const int pos = 32 * i - b;  // bit position of mag[i]'s LSB in the word
-            if (pos > -32 && pos < 32) w |= pos >= 0 ? (mag[i] << pos) : (mag[i] >> (-pos));
-        }
-        out.m[u] = w;
-    }
+    bool round = false, sticky = false;
     if (shift > 0) {
         const int rb = shift - 1;
-        bool round = false, sticky = false;
 #pragma unroll
         for (int i = 0; i < L; i++) {
             const int lo = 32 * i;
@@ -116,18 +134,35 @@ __host__ __device__ inline void round_words(bool neg, const uint32_t (&mag)[L], 
                 sticky = true;
             }
         }
-        if (round && (sticky || (out.m[0] & 1))) {
-            uint32_t c = 1;
+    }
+    // mantissa word u = bits [shift + 32u, +32) of mag
+    constexpr int T = L > W ? L + 1 : W + 1;
+    uint32_t t[T];
 #pragma unroll
-            for (int u = 0; u < W; u++) {
-                const uint32_t v = out.m[u] + c;
-                c = (c && v == 0) ? 1 : 0;
-                out.m[u] = v;
-            }
-            if (c) {
-                out.m[W - 1] = 0x80000000u;
-                shift++;
-            }
+    for (int i = 0; i < T; i++) t[i] = i < L ? mag[i] : 0u;
+    if (shift >= 0) {
+        shr_words<T>(t, shift >> 5);
+        const int rs = shift & 31;
+#pragma unroll
+        for (int u = 0; u < W; u++) out.m[u] = funnel(t[u + 1], t[u], rs);
+    } else {
+        const int ls = -shift;
+        shl_words<T>(t, ls >> 5);
+        const int rs = ls & 31;
+#pragma unroll
+        for (int u = 0; u < W; u++) out.m[u] = rs ? funnel(t[u], u ? t[u - 1] : 0u, 32 - rs) : t[u];
+    }
+    if (round && (sticky || (out.m[0] & 1))) {
+        uint32_t c = 1;
+#pragma unroll
+        for (int u = 0; u < W; u++) {
+            const uint32_t v = out.m[u] + c;
+            c = (c && v == 0) ? 1 : 0;
+            out.m[u] = v;
+        }
+        if (c) {
+            out.m[W - 1] = 0x80000000u;
+            shift++;
         }
     }
     const int64_t e = scale + shift + P;
@@ -212,17 +247,13 @@ SDPX_MP_OUTLINE void add_signed(const Val<W> &a, const Val<W> &b, bool negate_b,
         Y[i] = i < W ? y->m[i] : 0;
     }
     const int wsh = (int)(gap >> 5), bsh = (int)(gap & 31);
+    // X = Mx << gap: bit shift into W + 1 words, then a word shift
 #pragma unroll
-    for (int i = 0; i < L; i++) {
-        // X[i] = bits of Mx << gap: word i takes Mx words i - wsh and i - wsh - 1
-        uint32_t v = 0;
-#pragma unroll
-        for (int k = 0; k < W; k++) {
-            if (k == i - wsh) v |= x->m[k] << bsh;
-            if (bsh && k == i - wsh - 1) v |= x->m[k] >> (32 - bsh);
-        }
-        X[i] = v;
+    for (int i = 0; i <= W; i++) {
+        const uint32_t cur = i < W ? x->m[i] : 0u, prev = i ? x->m[i - 1] : 0u;
+        X[i] = bsh ? funnel(cur, prev, 32 - bsh) : cur;
     }
+    shl_words<L>(X, wsh);
     if (nx == ny) {
         uint64_t c = 0;
 #pragma unroll
