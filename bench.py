@@ -133,8 +133,10 @@ def sampled_workload(args):
 
     nl = 4
     sdpx_gpu.check(sdpx_gpu.lib().sdpx_gpu_init(0))
+    progress(f"building the sampled problem ({args.border} border rows)")
     prob = workloads.sampled_problem(nl, seed=7, t=args.border)
     t = prob["t"]
+    progress("problem built")
 
     run = sdpx_gpu.SampledDev(prob, nl)   # inputs resident in HBM
 
@@ -143,12 +145,14 @@ def sampled_workload(args):
 
     for _ in range(args.warmup):
         step()
+    progress("warmup iteration done")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
+    progress(f"{args.steps} timed iterations: {dt:.3f} s each")
     phases = sdpx_gpu.sampled_stats()
     # the host-buffer entry point (records staged over PCIe both ways)
     h0 = time.perf_counter()
