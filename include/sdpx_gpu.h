@@ -152,6 +152,30 @@ int32_t sdpx_gpu_mp_ldl_solve(uint32_t limbs, uint64_t n, const void *l, uint64_
 /* Device-pointer form of sdpx_gpu_mp_ldl_solve (l, dinv, b in HBM). */
 int32_t sdpx_gpu_mp_ldl_solve_dev(uint32_t limbs, uint64_t n, const void *l, uint64_t ldl,
                                   const void *dinv, uint64_t nrhs, void *b, uint64_t ldb);
+/* Exact-dot Cholesky of the stored triangle (uplo 'L': a[i + k lda], i >= k;
+ * 'U': a[k + i lda]) in place.  Replaces XpotrfScalar::xpotrf for MPFR
+ * (crates/solver/src/algebra/dense/blas/mpfr.rs:727-789): each pivot is
+ * d = RN(a_jj - RN(sum_k L_jk^2)), stored, then rejected unless finite and
+ * positive (info = j + 1, the rest untouched), else replaced by RN(sqrt d);
+ * each entry is RN(RN(a_ij - RN(sum_k L_ik L_jk)) / d).  info -1: bad uplo,
+ * -4: lda below n.  128/256/512 bits; returns 2 (nothing written) for
+ * non-finite input or a dot whose exponent spread exceeds the GPU window. */
+int32_t sdpx_gpu_mp_potrf(uint32_t limbs, uint8_t uplo, uint64_t n, void *a, uint64_t lda, int32_t *info);
+/* Solve A X = B with the factor of sdpx_gpu_mp_potrf.  Replaces
+ * XpotrsScalar::xpotrs (mpfr.rs:790-872): per right-hand side the forward
+ * and backward chains v = RN(v - a_ik x_k) (one rounding, the fused
+ * multiply-add) and x_i = RN(v / a_ii), in the reference's order.  info -1:
+ * bad uplo, -5: lda below n, -7: ldb below n.  Returns 2 for non-finite
+ * input. */
+int32_t sdpx_gpu_mp_potrs(uint32_t limbs, uint8_t uplo, uint64_t n, uint64_t nrhs, const void *a,
+                          uint64_t lda, void *b, uint64_t ldb, int32_t *info);
+/* L X = B (or L^T X = B) with the lower triangle of the n x n array a (ld n)
+ * and B n x nrhs (ld n).  Replaces the default xtrsm_lower for MPFR
+ * (crates/solver/src/algebra/dense/blas/traits.rs:85-113): x_i =
+ * RN(RN(b_i - RN(exact dot of the solved entries)) / L_ii).  Returns 2
+ * (nothing written) for non-finite input or a dot beyond the window. */
+int32_t sdpx_gpu_mp_trsm_lower(uint32_t limbs, uint64_t n, const void *a, void *b, uint64_t nrhs,
+                               uint32_t transpose);
 /* Counters of this thread's last sdpx_gpu_mp_ldl (measurement only). */
 int32_t sdpx_gpu_mp_ldl_stats(int64_t *gpu_products, int64_t *host_products, int64_t *regularized,
                               double *gpu_mp_macs);

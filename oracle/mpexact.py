@@ -20,6 +20,7 @@ Everything here is plain Python integers, independent of the C restatement
 (oracle/oracle.c) and of the GPU library, so the three can check each other.
 """
 from fractions import Fraction
+from math import isqrt
 
 NAN_KIND, INF_KIND, ZERO_KIND, REGULAR_KIND = 0, 1, 2, 3
 # MPFR's default exponent range (lib.rs:337-339): +-(2**30 - 1).
@@ -265,6 +266,99 @@ def sub(x, y):
 
 def div(x, y):
     return round_fraction(x.to_fraction() / y.to_fraction(), x.n)
+
+
+def sqrt(x):
+    """RN(sqrt(x)) for x >= 0 (MPFR sqrt, round to nearest): the integer
+    square root of the scaled mantissa with at least 2p + 4 bits, the
+    remainder as sticky, one rounding."""
+    if x.is_zero():
+        return zero(x.n)
+    s, e = x.scaled()
+    if s < 0:
+        raise ValueError("sqrt of a negative value")
+    if e & 1:
+        s <<= 1
+        e -= 1
+    p = 64 * x.n
+    k = max(0, (2 * p + 4 - s.bit_length() + 1) // 2)
+    r = s << (2 * k)
+    q = isqrt(r)
+    sticky = 1 if q * q != r else 0
+    return from_scaled_integer(False, (q << 1) | sticky, e // 2 - k - 1, x.n)
+
+
+def fms(v, a, b):
+    """RN(v - a b) with one rounding (MPFR mul_add of -a, b, v)."""
+    return exact_dot([(v, from_int(1, v.n)), (neg(a), b)], v.n)
+
+
+# ---------------------------------------------------------------------------
+# Exact-dot Cholesky, triangular solve and solve (mpfr.rs:726-872,
+# traits.rs:85-113): every subtracted sum is one exact dot rounded once.
+# ---------------------------------------------------------------------------
+
+def potrf(uplo, n, a, lda):
+    """XpotrfScalar::xpotrf (mpfr.rs:726-789) on a list of Mp, in place.
+    Returns info: 0, or the failing pivot column + 1."""
+    nl = a[0].n if a else 1
+    lower = uplo == "L"
+
+    def at(i, k):
+        return i + k * lda if lower else k + i * lda
+
+    for j in range(n):
+        d = sub(a[j + j * lda], exact_dot([(a[at(j, k)], a[at(j, k)]) for k in range(j)], nl))
+        a[j + j * lda] = d
+        if not d.is_finite() or d.is_zero() or d.negative():
+            return j + 1
+        d = sqrt(d)
+        a[j + j * lda] = d
+        for i in range(j + 1, n):
+            solved = exact_dot([(a[at(i, k)], a[at(j, k)]) for k in range(j)], nl)
+            q = at(i, j)
+            a[q] = div(sub(a[q], solved), d)
+    return 0
+
+
+def trsm_lower(n, a, b, transpose):
+    """XpotrsScalar::xtrsm_lower (traits.rs:85-113): L x = b (or L^T x = b)
+    per column of b (a: n x n column-major lower), in place."""
+    if n == 0:
+        return
+    nl = a[0].n
+    for c0 in range(0, len(b), n):
+        col = b[c0:c0 + n]
+        for step in range(n):
+            i = n - 1 - step if transpose else step
+            if transpose:
+                solved = exact_dot([(a[r + i * n], col[r]) for r in range(i + 1, n)], nl)
+            else:
+                solved = exact_dot([(a[i + k * n], col[k]) for k in range(i)], nl)
+            col[i] = div(sub(col[i], solved), a[i + i * n])
+        b[c0:c0 + n] = col
+
+
+def potrs(uplo, n, nrhs, a, lda, b, ldb):
+    """XpotrsScalar::xpotrs (mpfr.rs:790-872): forward then backward FMA
+    chains per right-hand side, each step one rounding, then a division."""
+    lower = uplo == "L"
+    for r in range(nrhs):
+        col = [b[i + r * ldb] for i in range(n)]
+        for i in range(n):
+            v = col[i]
+            for k in range(i):
+                av = a[i + k * lda] if lower else a[k + i * lda]
+                v = fms(v, av, col[k])
+            col[i] = div(v, a[i + i * lda])
+        for i in reversed(range(n)):
+            v = col[i]
+            for k in range(i + 1, n):
+                av = a[k + i * lda] if lower else a[i + k * lda]
+                v = fms(v, av, col[k])
+            col[i] = div(v, a[i + i * lda])
+        for i in range(n):
+            b[i + r * ldb] = col[i]
 
 
 # ---------------------------------------------------------------------------

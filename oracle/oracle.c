@@ -385,6 +385,136 @@ static void ora_recip(const ora_ctx *c, const uint8_t *d, uint8_t *out) {
     from_scaled_integer(ora_is_neg(c, d), mag, 41, -(int64_t)P - 3 - e - 1, n, out, c->stride, 0);
 }
 
+/* Little-endian 32-bit word helpers for the division and square root. */
+#define BW 200 /* words: radicands and numerators up to 2P + 8 bits, P <= 2048 */
+static void bw_from_mant(uint32_t *w, const uint8_t *v, int n) {
+    memset(w, 0, sizeof(uint32_t) * BW);
+    const uint64_t *m = (const uint64_t *)v;
+    for (int i = 0; i < n; i++) {
+        w[2 * i] = (uint32_t)m[i];
+        w[2 * i + 1] = (uint32_t)(m[i] >> 32);
+    }
+}
+static int bw_bit(const uint32_t *w, int b) { return (w[b >> 5] >> (b & 31)) & 1; }
+static void bw_shl1(uint32_t *w, int words, uint32_t in) {
+    for (int i = 0; i < words; i++) {
+        uint32_t v = w[i];
+        w[i] = (v << 1) | in;
+        in = v >> 31;
+    }
+}
+static int bw_ge(const uint32_t *a, const uint32_t *b, int words) {
+    for (int i = words - 1; i >= 0; i--)
+        if (a[i] != b[i]) return a[i] > b[i];
+    return 1;
+}
+static void bw_sub(uint32_t *a, const uint32_t *b, int words) {
+    int64_t br = 0;
+    for (int i = 0; i < words; i++) {
+        int64_t t = (int64_t)a[i] - b[i] - br;
+        br = t < 0;
+        a[i] = (uint32_t)t;
+    }
+}
+static int bw_nonzero(const uint32_t *a, int words) {
+    for (int i = 0; i < words; i++)
+        if (a[i]) return 1;
+    return 0;
+}
+static void bw_round(const ora_ctx *c, int neg, const uint32_t *q, int qwords, int sticky,
+                     int64_t scale, uint8_t *out) {
+    /* (q << 1 | sticky) * 2^(scale - 1), one rounding */
+    uint64_t mag[BW / 2 + 2] = {0};
+    uint32_t w[BW + 2] = {0};
+    uint32_t carry = (uint32_t)sticky;
+    for (int i = 0; i < qwords; i++) {
+        w[i] = (q[i] << 1) | carry;
+        carry = q[i] >> 31;
+    }
+    w[qwords] = carry;
+    int words = qwords + 1;
+    for (int i = 0; i < words; i += 2) mag[i / 2] = w[i] | ((uint64_t)w[i + 1] << 32);
+    from_scaled_integer(neg, mag, (words + 1) / 2, scale - 1, c->n, out, c->stride, 0);
+}
+
+/* RN(x / d): restoring division of Mx 2^(P+2) by Md, remainder as sticky. */
+static void ora_div(const ora_ctx *c, const uint8_t *x, const uint8_t *d, uint8_t *out) {
+    int n = c->n, P = 64 * n;
+    if (ora_is_zero(c, x)) {
+        memset(out, 0, c->stride);
+        *(int32_t *)(out + 8 * n) = KIND_ZERO;
+        return;
+    }
+    uint32_t num[BW], den[BW], rem[BW] = {0}, q[BW] = {0};
+    bw_from_mant(num, x, n);
+    bw_from_mant(den, d, n);
+    int words = 2 * n + 2;
+    int nbits = P + (P + 2); /* numerator Mx << (P + 2) */
+    for (int b = nbits - 1; b >= 0; b--) {
+        int src = b - (P + 2);
+        bw_shl1(rem, words, src >= 0 ? (uint32_t)bw_bit(num, src) : 0u);
+        bw_shl1(q, words, 0);
+        if (bw_ge(rem, den, words)) {
+            bw_sub(rem, den, words);
+            q[0] |= 1;
+        }
+    }
+    int32_t ex = *(const int32_t *)(x + 8 * n + 4), ed = *(const int32_t *)(d + 8 * n + 4);
+    bw_round(c, ora_is_neg(c, x) != ora_is_neg(c, d), q, words, bw_nonzero(rem, words),
+             (int64_t)ex - ed - P - 2, out);
+}
+
+/* RN(sqrt(a)), a > 0: restoring integer square root (two bits a step) of
+ * the mantissa scaled to an even exponent and 2P + 4 or more bits. */
+static void ora_sqrt(const ora_ctx *c, const uint8_t *a, uint8_t *out) {
+    int n = c->n, P = 64 * n;
+    if (ora_is_zero(c, a)) {
+        memset(out, 0, c->stride);
+        *(int32_t *)(out + 8 * n) = KIND_ZERO;
+        return;
+    }
+    uint32_t m[BW];
+    bw_from_mant(m, a, n);
+    int64_t e = (int64_t)*(const int32_t *)(a + 8 * n + 4) - P; /* value = M 2^e */
+    int extra = 0;                                               /* radicand = M << extra */
+    if (e & 1) {
+        extra = 1;
+        e -= 1;
+    }
+    int k = (2 * P + 4 - (P + extra) + 1) / 2;
+    if (k < 0) k = 0;
+    extra += 2 * k;
+    int rbits = P + extra; /* radicand bits (even count after padding) */
+    if (rbits & 1) rbits++;
+    int words = (rbits / 2 + 40) / 32 + 2;
+    uint32_t rem[BW] = {0}, root[BW] = {0}, trial[BW];
+    for (int b = rbits - 2; b >= 0; b -= 2) {
+        int s1 = b + 1 - extra, s0 = b - extra;
+        uint32_t hi = s1 >= 0 && s1 < P ? (uint32_t)bw_bit(m, s1) : 0u;
+        uint32_t lo = s0 >= 0 && s0 < P ? (uint32_t)bw_bit(m, s0) : 0u;
+        bw_shl1(rem, words, hi);
+        bw_shl1(rem, words, lo);
+        memcpy(trial, root, sizeof(uint32_t) * words);
+        bw_shl1(trial, words, 0);
+        bw_shl1(trial, words, 1); /* (root << 2) | 1 */
+        bw_shl1(root, words, 0);
+        if (bw_ge(rem, trial, words)) {
+            bw_sub(rem, trial, words);
+            root[0] |= 1;
+        }
+    }
+    bw_round(c, 0, root, words, bw_nonzero(rem, words), e / 2 - k, out);
+}
+
+/* RN(v - a b), one rounding (mul_add of -a, b, v) */
+static void ora_fms(const ora_ctx *c, const uint8_t *v, const uint8_t *a, const uint8_t *b, uint8_t *out) {
+    uint8_t na[8 * 32 + 8];
+    memcpy(na, a, c->stride);
+    ora_neg(c, na);
+    const uint8_t *x[2] = {v, na}, *y[2] = {c->one, b};
+    ora_dot_ptrs(c, 2, x, y, out);
+}
+
 /* ------------------------------------------------------------------------
  * LDL^T restatement (arrow.rs:156-490): ldl_dots, solve_unit_lower_t,
  * ldl_rec with envelope cuts.  Products are exact dots rounded once.
@@ -772,3 +902,121 @@ void oracle_mul(int nl, const uint8_t *a, const uint8_t *b, uint8_t *out) {
 }
 
 int oracle_abi(void) { return 1; }
+
+/* ------------------------------------------------------------------------
+ * Exact-dot Cholesky / triangular solve / solve (mpfr.rs:726-872,
+ * traits.rs:85-113).
+ * ---------------------------------------------------------------------- */
+int oracle_potrf(int nl, char uplo, size_t n, uint8_t *a, size_t lda) {
+    ora_ctx cc, *c = &cc;
+    ora_init(c, nl);
+    const int lower = uplo == 'L';
+    size_t st = c->stride;
+    const uint8_t **xs = malloc(sizeof(void *) * (n ? n : 1)), **ys = malloc(sizeof(void *) * (n ? n : 1));
+    uint8_t t[8 * 32 + 8], u[8 * 32 + 8];
+#define ATF(i, k) (a + (lower ? (i) + (k) * lda : (k) + (i) * lda) * st)
+    for (size_t j = 0; j < n; j++) {
+        for (size_t k = 0; k < j; k++) xs[k] = ys[k] = ATF(j, k);
+        ora_dot_ptrs(c, (int)j, xs, ys, t);
+        uint8_t *ajj = a + (j + j * lda) * st;
+        ora_sub(c, ajj, t, u);
+        memcpy(ajj, u, st);
+        if (!ora_is_finite(c, u) || ora_is_zero(c, u) || ora_is_neg(c, u)) {
+            free(xs);
+            free(ys);
+            return (int)j + 1;
+        }
+        ora_sqrt(c, u, ajj);
+        for (size_t i = j + 1; i < n; i++) {
+            for (size_t k = 0; k < j; k++) {
+                xs[k] = ATF(i, k);
+                ys[k] = ATF(j, k);
+            }
+            ora_dot_ptrs(c, (int)j, xs, ys, t);
+            uint8_t *q = ATF(i, j);
+            ora_sub(c, q, t, u);
+            ora_div(c, u, ajj, q);
+        }
+    }
+#undef ATF
+    free(xs);
+    free(ys);
+    return 0;
+}
+
+void oracle_trsm_lower(int nl, size_t n, const uint8_t *a, uint8_t *b, size_t nrhs, int transpose) {
+    ora_ctx cc, *c = &cc;
+    ora_init(c, nl);
+    size_t st = c->stride;
+    const uint8_t **xs = malloc(sizeof(void *) * (n ? n : 1)), **ys = malloc(sizeof(void *) * (n ? n : 1));
+    uint8_t t[8 * 32 + 8], u[8 * 32 + 8];
+    for (size_t r = 0; r < nrhs; r++) {
+        uint8_t *col = b + r * n * st;
+        for (size_t step = 0; step < n; step++) {
+            size_t i = transpose ? n - 1 - step : step;
+            int cnt = 0;
+            if (transpose) {
+                for (size_t k = i + 1; k < n; k++) {
+                    xs[cnt] = a + (k + i * n) * st;
+                    ys[cnt++] = col + k * st;
+                }
+            } else {
+                for (size_t k = 0; k < i; k++) {
+                    xs[cnt] = a + (i + k * n) * st;
+                    ys[cnt++] = col + k * st;
+                }
+            }
+            ora_dot_ptrs(c, cnt, xs, ys, t);
+            ora_sub(c, col + i * st, t, u);
+            ora_div(c, u, a + (i + i * n) * st, col + i * st);
+        }
+    }
+    free(xs);
+    free(ys);
+}
+
+void oracle_potrs(int nl, char uplo, size_t n, size_t nrhs, const uint8_t *a, size_t lda, uint8_t *b,
+                  size_t ldb) {
+    ora_ctx cc, *c = &cc;
+    ora_init(c, nl);
+    size_t st = c->stride;
+    const int lower = uplo == 'L';
+    uint8_t v[8 * 32 + 8], w[8 * 32 + 8];
+    for (size_t r = 0; r < nrhs; r++) {
+        uint8_t *col = b + r * ldb * st;
+        for (size_t i = 0; i < n; i++) {
+            memcpy(v, col + i * st, st);
+            for (size_t k = 0; k < i; k++) {
+                const uint8_t *av = a + (lower ? i + k * lda : k + i * lda) * st;
+                ora_fms(c, v, av, col + k * st, w);
+                memcpy(v, w, st);
+            }
+            ora_div(c, v, a + (i + i * lda) * st, col + i * st);
+        }
+        for (size_t ii = n; ii-- > 0;) {
+            memcpy(v, col + ii * st, st);
+            for (size_t k = ii + 1; k < n; k++) {
+                const uint8_t *av = a + (lower ? k + ii * lda : ii + k * lda) * st;
+                ora_fms(c, v, av, col + k * st, w);
+                memcpy(v, w, st);
+            }
+            ora_div(c, v, a + (ii + ii * lda) * st, col + ii * st);
+        }
+    }
+}
+
+void oracle_div(int nl, const uint8_t *x, const uint8_t *d, uint8_t *out) {
+    ora_ctx c;
+    ora_init(&c, nl);
+    ora_div(&c, x, d, out);
+}
+void oracle_fms(int nl, const uint8_t *v, const uint8_t *a, const uint8_t *b, uint8_t *out) {
+    ora_ctx c;
+    ora_init(&c, nl);
+    ora_fms(&c, v, a, b, out);
+}
+void oracle_sqrt(int nl, const uint8_t *a, uint8_t *out) {
+    ora_ctx c;
+    ora_init(&c, nl);
+    ora_sqrt(&c, a, out);
+}

@@ -40,5 +40,15 @@ void mp_transpose(int limbs, const void *in, int64_t ldi, int64_t m, int64_t n, 
                   int64_t ldo, hipStream_t st);
 void mp_envelope(int limbs, const void *A, int64_t ld, int64_t n, int32_t *first, hipStream_t st);
 void mp_mirror_upper(int limbs, void *S, int64_t n, int64_t ld, hipStream_t st);
+// Exact-dot Cholesky (xpotrf), one launch per column: status[0] = info
+// (failing column + 1), status[2] |= 1 when a dot's exponent spread exceeds
+// the window; both must be zero on entry.
+void mp_potrf(int limbs, void *A, int64_t n, int64_t lda, int lower, int *status, hipStream_t st);
+// xtrsm_lower (a: n x n, ld n; b: n x nrhs, ld n); fail |= window overflow.
+void mp_trsm_lower(int limbs, const void *A, int64_t n, void *B, int64_t nrhs, int transpose, int *fail,
+                   hipStream_t st);
+// xpotrs with a factor from xpotrf (one lane per right-hand side).
+void mp_potrs(int limbs, const void *A, int64_t n, int64_t lda, int lower, void *B, int64_t ldb,
+              int64_t nrhs, hipStream_t st);
 
 }  // namespace sdpx

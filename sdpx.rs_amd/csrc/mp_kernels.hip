@@ -50,7 +50,7 @@ __global__ void ew_sub_packed_kernel(uint32_t *S, int64_t ld, const uint32_t *P,
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t total = n * (n + 1) / 2;
     if (e >= total) return;
-    int64_t j = (int64_t)((sqrt(8.0 * (double)e + 1.0) - 1.0) / 2.0);
+    int64_t j = (int64_t)((::sqrt(8.0 * (double)e + 1.0) - 1.0) / 2.0);
     while (j * (j + 1) / 2 > e) j--;
     while ((j + 1) * (j + 2) / 2 <= e) j++;
     const int64_t i = e - j * (j + 1) / 2;
@@ -625,6 +625,126 @@ __global__ __launch_bounds__(ldl_threads<W>()) void ldl_dots_kernel(
     }
 }
 
+// ---------------------------------------------------------------------------
+// exact-dot Cholesky, triangular solve and solve (mpfr.rs:726-872,
+// traits.rs:85-113)
+// ---------------------------------------------------------------------------
+// xpotrf, one launch per column j (j = -1 forms the first pivot only):
+// status[0] = info, status[2] |= 1 on a dot beyond its window; every launch
+// returns at once when either is set.  One lane group per row i > j forms
+// L_ij = RN(RN(a_ij - RN(sum_{k<j} L_ik L_jk)) / d_j); the group of row j+1
+// then forms the next pivot d = RN(a - RN(sum_{k<=j} L_{j+1,k}^2)), stores
+// it, checks it (finite and positive) and stores RN(sqrt(d)).  Entry (i, k)
+// of the stored triangle is a[i + k lda] (lower) or a[k + i lda] (upper).
+template <int W, int G>
+__global__ __launch_bounds__(ROW_THREADS) void potrf_col_kernel(uint32_t *a, int64_t n, int64_t lda,
+                                                                int lower, int64_t j, int *status) {
+    extern __shared__ int64_t slots_sh[];
+    const int gl = threadIdx.x & (G - 1);
+    const int64_t i = j + 1 + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    if (i >= n || status[0] != 0 || status[2] != 0) return;
+    int64_t *slots = slots_sh + threadIdx.x;
+    const int64_t rs = lower ? lda : 1;  // record stride along a row of the triangle
+    auto row = [&](int64_t r) { return rec_at<W>((const uint32_t *)a, lower ? r : r * lda); };
+    if (j >= 0) {
+        Val<W> t, aij, diff, dj, v;
+        if (!group_dot<W, G, ROW_THREADS>(0, j, gl, row(i), rs, row(j), rs, nullptr, false, slots, t,
+                                          INT64_MAX, INT64_MIN)) {
+            if (gl == 0) atomicOr(status + 2, 1);
+            return;
+        }
+        uint32_t *q = rec_at<W>(a, lower ? i + j * lda : j + i * lda);
+        load(q, aij);
+        load(rec_at<W>((const uint32_t *)a, j + j * lda), dj);
+        add_signed(aij, t, true, diff);
+        div(diff, dj, v);
+        if (gl == (int)(j % G)) store(q, v);  // the lane that reads term j below
+    }
+    if (i != j + 1) return;
+    Val<W> t, aii, d;
+    if (!group_dot<W, G, ROW_THREADS>(0, i, gl, row(i), rs, row(i), rs, nullptr, false, slots, t, INT64_MAX,
+                                      INT64_MIN)) {
+        if (gl == 0) atomicOr(status + 2, 1);
+        return;
+    }
+    if (gl != 0) return;
+    uint32_t *pii = rec_at<W>(a, i + i * lda);
+    load(pii, aii);
+    add_signed(aii, t, true, d);
+    store(pii, d);
+    if (!is_regular(d) || d.kind < 0) {
+        status[0] = (int)(i + 1);
+        return;
+    }
+    Val<W> r;
+    sqrt(d, r);
+    store(pii, r);
+}
+
+// xtrsm_lower: per right-hand side (one lane group each), x_i =
+// RN(RN(b_i - RN(sum_k L_ik x_k)) / L_ii), ascending, or with L^T
+// (transpose: terms L_ki, k > i) descending; a is n x n (ld n), b n x nrhs.
+template <int W, int G>
+__global__ __launch_bounds__(ROW_THREADS) void trsm_lower_kernel(const uint32_t *a, int64_t n, uint32_t *b,
+                                                                 int64_t nrhs, int transpose, int *fail) {
+    extern __shared__ int64_t slots_sh[];
+    const int gl = threadIdx.x & (G - 1);
+    const int64_t c = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    if (c >= nrhs) return;
+    int64_t *slots = slots_sh + threadIdx.x;
+    uint32_t *col = rec_at<W>(b, c * n);
+    for (int64_t step = 0; step < n; step++) {
+        const int64_t i = transpose ? n - 1 - step : step;
+        Val<W> t, bi, diff, aii, x;
+        const bool ok =
+            transpose ? group_dot<W, G, ROW_THREADS>(i + 1, n, gl, rec_at<W>(a, i * n), 1, col, 1, nullptr,
+                                                     false, slots, t, INT64_MAX, INT64_MIN)
+                      : group_dot<W, G, ROW_THREADS>(0, i, gl, rec_at<W>(a, i), n, col, 1, nullptr, false,
+                                                     slots, t, INT64_MAX, INT64_MIN);
+        if (!ok) {
+            if (gl == 0) atomicOr(fail, 1);
+            return;
+        }
+        load(rec_at<W>(col, i), bi);
+        load(rec_at<W>(a, i + i * n), aii);
+        add_signed(bi, t, true, diff);
+        div(diff, aii, x);
+        if (gl == (int)(i % G)) store(rec_at<W>(col, i), x);
+    }
+}
+
+// xpotrs: one lane per right-hand side runs the forward and backward chains
+// v = RN(v - a_ik x_k) (one rounding, the fused multiply-add) and
+// x_i = RN(v / a_ii) in the reference's order.
+template <int W>
+__global__ __launch_bounds__(ROW_THREADS) void potrs_kernel(const uint32_t *a, int64_t n, int64_t lda,
+                                                            int lower, uint32_t *b, int64_t ldb,
+                                                            int64_t nrhs) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nrhs) return;
+    uint32_t *col = rec_at<W>(b, c * ldb);
+    auto entry = [&](int64_t i, int64_t k) { return rec_at<W>(a, lower ? i + k * lda : k + i * lda); };
+    for (int pass = 0; pass < 2; pass++) {
+        for (int64_t step = 0; step < n; step++) {
+            const int64_t i = pass == 0 ? step : n - 1 - step;
+            Val<W> v, w, ak, xk;
+            load(rec_at<W>(col, i), v);
+            const int64_t k0 = pass == 0 ? 0 : i + 1, k1 = pass == 0 ? i : n;
+            for (int64_t k = k0; k < k1; k++) {
+                // forward reads L_ik, backward L_ki (the transposed factor)
+                load(pass == 0 ? entry(i, k) : entry(k, i), ak);
+                load(rec_at<W>(col, k), xk);
+                fms(v, ak, xk, w);
+                v = w;
+            }
+            Val<W> aii, x;
+            load(rec_at<W>(a, i + i * lda), aii);
+            div(v, aii, x);
+            store(rec_at<W>(col, i), x);
+        }
+    }
+}
+
 // S[i + j*lds] = RN(S - P[i + j*ldp]) for an m x n block
 template <int W>
 __global__ void ew_sub_rect_kernel(uint32_t *S, int64_t lds, const uint32_t *P, int64_t ldp,
@@ -812,6 +932,39 @@ void mp_transpose(int limbs, const void *in, int64_t ldi, int64_t m, int64_t n, 
     SDPX_DISPATCH_W(limbs, hipLaunchKernelGGL(transpose_kernel<W>, dim3(blocks_for(m * n, 256)),
                                               dim3(256), 0, st, (const uint32_t *)in, ldi, m, n,
                                               (uint32_t *)out, ldo));
+    HIP_CHECK(hipGetLastError());
+}
+
+void mp_potrf(int limbs, void *A, int64_t n, int64_t lda, int lower, int *status, hipStream_t st) {
+    SDPX_DISPATCH_W(limbs, {
+        auto kern = potrf_col_kernel<W, DOT_G>;
+        const size_t lds = panel_lds<W>((const void *)kern, ROW_THREADS);
+        for (int64_t j = -1; j < n - 1; j++)
+            hipLaunchKernelGGL(kern, dim3(blocks_for((n - j - 1) * DOT_G, ROW_THREADS)), dim3(ROW_THREADS), lds,
+                               st, (uint32_t *)A, n, lda, lower, j, status);
+    });
+    HIP_CHECK(hipGetLastError());
+}
+
+void mp_trsm_lower(int limbs, const void *A, int64_t n, void *B, int64_t nrhs, int transpose, int *fail,
+                   hipStream_t st) {
+    if (n == 0 || nrhs == 0) return;
+    SDPX_DISPATCH_W(limbs, {
+        auto kern = trsm_lower_kernel<W, DOT_G>;
+        const size_t lds = panel_lds<W>((const void *)kern, ROW_THREADS);
+        hipLaunchKernelGGL(kern, dim3(blocks_for(nrhs * DOT_G, ROW_THREADS)), dim3(ROW_THREADS), lds, st,
+                           (const uint32_t *)A, n, (uint32_t *)B, nrhs, transpose, fail);
+    });
+    HIP_CHECK(hipGetLastError());
+}
+
+void mp_potrs(int limbs, const void *A, int64_t n, int64_t lda, int lower, void *B, int64_t ldb,
+              int64_t nrhs, hipStream_t st) {
+    if (n == 0 || nrhs == 0) return;
+    SDPX_DISPATCH_W(limbs, {
+        hipLaunchKernelGGL(potrs_kernel<W>, dim3(blocks_for(nrhs, ROW_THREADS)), dim3(ROW_THREADS), 0, st,
+                           (const uint32_t *)A, n, lda, lower, (uint32_t *)B, ldb, nrhs);
+    });
     HIP_CHECK(hipGetLastError());
 }
 

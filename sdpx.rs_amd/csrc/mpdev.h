@@ -285,6 +285,109 @@ SDPX_MP_OUTLINE void add_signed(const Val<W> &a, const Val<W> &b, bool negate_b,
     round_words<W, L>(cmp > 0 ? nx : ny, D, (int64_t)y->e - P, out);
 }
 
+// RN(c - a*b) with one rounding (the fused multiply-add of the triangular
+// sweeps).  The product is exact in 2W words and c is widened to 2W words;
+// the two are then combined as in add_signed.  An operand more than
+// 64W + 2 bits below the other is replaced by one bit at the bottom of the
+// buffer: both values lie strictly inside the same unit of the buffer's last
+// bit, more than two bits below the W-word rounding point, so they round
+// alike.
+template <int W>
+SDPX_MP_OUTLINE void fms(const Val<W> &c, const Val<W> &a, const Val<W> &b, Val<W> &out) {
+    if (is_zero(a) || is_zero(b)) {
+        if (is_zero(c))
+            set_zero(out);
+        else
+            out = c;
+        return;
+    }
+    constexpr int M = 2 * W;
+    uint32_t p[M];
+#pragma unroll
+    for (int i = 0; i < M; i++) p[i] = 0;
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+        uint64_t carry = 0;
+#pragma unroll
+        for (int j = 0; j < W; j++) {
+            const uint64_t t = (uint64_t)a.m[i] * b.m[j] + p[i + j] + carry;
+            p[i + j] = (uint32_t)t;
+            carry = t >> 32;
+        }
+        p[i + W] = (uint32_t)carry;
+    }
+    const bool np = (a.kind < 0) == (b.kind < 0);  // sign of -a*b
+    const int64_t PM = 32 * M;
+    if (is_zero(c)) {
+        round_words<W, M>(np, p, (int64_t)a.e + b.e - PM, out);
+        return;
+    }
+    // both mantissas normalised to M words: value = m * 2^(e - 32M)
+    int64_t ep = (int64_t)a.e + b.e;
+    if (!(p[M - 1] >> 31)) {
+#pragma unroll
+        for (int i = M - 1; i >= 0; i--) p[i] = funnel(p[i], i ? p[i - 1] : 0u, 31);
+        ep -= 1;
+    }
+    const bool nc = c.kind < 0;
+    const bool swap = ep > (int64_t)c.e;  // x: the operand with the larger exponent
+    uint32_t xm[M], ym[M];
+#pragma unroll
+    for (int i = 0; i < M; i++) {
+        const uint32_t cw = i >= W ? c.m[i - W] : 0u;
+        xm[i] = swap ? p[i] : cw;
+        ym[i] = swap ? cw : p[i];
+    }
+    const int64_t ex = swap ? ep : (int64_t)c.e, ey = swap ? (int64_t)c.e : ep;
+    const bool nx = swap ? np : nc, ny = swap ? nc : np;
+    constexpr int L = 2 * M + 2;
+    int64_t gap = ex - ey;
+    const bool far = gap >= PM + 2;
+    if (far) gap = PM + 2;
+    uint32_t X[L], Y[L];
+#pragma unroll
+    for (int i = 0; i < L; i++) {
+        X[i] = 0;
+        Y[i] = far ? (i == 0 ? 1u : 0u) : (i < M ? ym[i] : 0u);
+    }
+    const int wsh = (int)(gap >> 5), bsh = (int)(gap & 31);
+#pragma unroll
+    for (int i = 0; i <= M; i++) {
+        const uint32_t cur = i < M ? xm[i] : 0u, prev = i ? xm[i - 1] : 0u;
+        X[i] = bsh ? funnel(cur, prev, 32 - bsh) : cur;
+    }
+    shl_words<L>(X, wsh);
+    const int64_t scale = ex - gap - PM;
+    if (nx == ny) {
+        uint64_t cy = 0;
+#pragma unroll
+        for (int i = 0; i < L; i++) {
+            const uint64_t t = (uint64_t)X[i] + Y[i] + cy;
+            X[i] = (uint32_t)t;
+            cy = t >> 32;
+        }
+        round_words<W, L>(nx, X, scale, out);
+        return;
+    }
+    int cmp = 0;
+#pragma unroll
+    for (int i = L - 1; i >= 0; i--)
+        if (cmp == 0 && X[i] != Y[i]) cmp = X[i] > Y[i] ? 1 : -1;
+    if (cmp == 0) {
+        set_zero(out);
+        return;
+    }
+    uint64_t borrow = 0;
+#pragma unroll
+    for (int i = 0; i < L; i++) {
+        const uint64_t big = cmp > 0 ? X[i] : Y[i], small = cmp > 0 ? Y[i] : X[i];
+        const uint64_t d = big - small - borrow;
+        X[i] = (uint32_t)d;
+        borrow = (d >> 63) & 1;
+    }
+    round_words<W, L>(cmp > 0 ? nx : ny, X, scale, out);
+}
+
 // ---------------------------------------------------------------------------
 // Exact dot with a slot window: slots[k * stride] for k < SLOTS, each a
 // signed 64-bit sum of 32-bit partial words at position k (bit 32k above
@@ -458,6 +561,169 @@ SDPX_MP_OUTLINE void recip(const Val<W> &d, Val<W> &out) {
     }
     mag[QW] = c;
     round_words<W, QW + 1>(d.kind < 0, mag, -(int64_t)P - 2 - d.e, out);
+}
+
+// RN(x / d): Q = floor(Mx 2^(P+2) / Md) by the same base-2^32 schoolbook
+// division (W + 2 quotient digits), the remainder's sticky bit, one rounding
+// of (Q << 1 | sticky) 2^(ex - ed - P - 3).
+template <int W>
+SDPX_MP_OUTLINE void div(const Val<W> &x, const Val<W> &d, Val<W> &out) {
+    if (is_zero(x)) {
+        set_zero(out);
+        return;
+    }
+    if (is_zero(d)) {
+        set_zero(out);
+        out.kind = (x.kind < 0) != (d.kind < 0) ? -KIND_INF : KIND_INF;
+        return;
+    }
+    const int P = 32 * W;
+    constexpr int QW = W + 2;
+    // numerator Mx << (P + 2): digits W .. 2W hold Mx << 2
+    uint32_t u[2 * W + 2];
+#pragma unroll
+    for (int i = 0; i < 2 * W + 2; i++) u[i] = 0;
+#pragma unroll
+    for (int i = 0; i <= W; i++) {
+        const uint32_t cur = i < W ? x.m[i] : 0u, prev = i ? x.m[i - 1] : 0u;
+        u[W + i] = funnel(cur, prev, 30);
+    }
+    const uint32_t *v = d.m;
+    const uint64_t vt = v[W - 1], vs = W >= 2 ? v[W - 2] : 0;
+    uint32_t q[QW + 1];
+    q[QW] = 0;
+#pragma unroll
+    for (int j = QW - 1; j >= 0; j--) {
+        const uint64_t num = ((uint64_t)u[j + W] << 32) | u[j + W - 1];
+        uint64_t qh = num / vt, rh = num % vt;
+        const uint64_t ulow = u[j + W - 2 >= 0 ? j + W - 2 : 0];
+        while (qh >> 32 || qh * vs > ((rh << 32) | (j + W - 2 >= 0 ? ulow : 0))) {
+            qh--;
+            rh += vt;
+            if (rh >> 32) break;
+        }
+        uint64_t mc = 0;
+        int64_t br = 0;
+#pragma unroll
+        for (int i = 0; i < W; i++) {
+            const uint64_t pr = qh * v[i] + mc;
+            mc = pr >> 32;
+            const int64_t t = (int64_t)u[j + i] - (int64_t)(uint32_t)pr + br;
+            u[j + i] = (uint32_t)t;
+            br = t >> 32;
+        }
+        const int64_t t = (int64_t)u[j + W] - (int64_t)mc + br;
+        u[j + W] = (uint32_t)t;
+        if (t < 0) {
+            qh--;
+            uint64_t c = 0;
+#pragma unroll
+            for (int i = 0; i < W; i++) {
+                const uint64_t s2 = (uint64_t)u[j + i] + v[i] + c;
+                u[j + i] = (uint32_t)s2;
+                c = s2 >> 32;
+            }
+            u[j + W] += (uint32_t)c;
+        }
+        q[j] = (uint32_t)qh;
+    }
+    bool sticky = false;
+#pragma unroll
+    for (int i = 0; i < W; i++) sticky |= u[i] != 0;
+    uint32_t mag[QW + 1];
+    uint32_t c = sticky ? 1u : 0u;
+#pragma unroll
+    for (int i = 0; i < QW; i++) {
+        mag[i] = (q[i] << 1) | c;
+        c = q[i] >> 31;
+    }
+    mag[QW] = c;
+    round_words<W, QW + 1>((x.kind < 0) != (d.kind < 0), mag,
+                           (int64_t)x.e - d.e - P - 3, out);
+}
+
+// RN(sqrt(a)) for a > 0: restoring integer square root, two radicand bits a
+// step, of the mantissa padded to an even exponent and 2P + 4 or more bits
+// (P + 3 root bits), the remainder's sticky bit, one rounding.
+template <int W>
+SDPX_MP_OUTLINE void sqrt(const Val<W> &a, Val<W> &out) {
+    if (is_zero(a)) {
+        set_zero(out);
+        return;
+    }
+    const int P = 32 * W;
+    // value = M 2^e; radicand R = M 2^ex with ex = (e odd) + 2k, bits >= 2P + 4
+    int64_t e = (int64_t)a.e - P;
+    int ex = 0;
+    if (e & 1) {
+        ex = 1;
+        e -= 1;
+    }
+    const int k = (2 * P + 4 - (P + ex) + 1) / 2;
+    ex += 2 * k;
+    int rbits = P + ex;
+    rbits += rbits & 1;
+    constexpr int RW = W + 3;  // root and remainder words (P + 3 root bits)
+    uint32_t rem[RW + 1], root[RW + 1];
+#pragma unroll
+    for (int i = 0; i <= RW; i++) rem[i] = root[i] = 0;
+    for (int b = rbits - 2; b >= 0; b -= 2) {
+        const int s1 = b + 1 - ex, s0 = b - ex;
+        const uint32_t hi = s1 >= 0 && s1 < P ? (a.m[s1 >> 5] >> (s1 & 31)) & 1u : 0u;
+        const uint32_t lo = s0 >= 0 && s0 < P ? (a.m[s0 >> 5] >> (s0 & 31)) & 1u : 0u;
+        // rem = (rem << 2) | hi lo; trial = (root << 2) | 1
+        uint32_t in = (hi << 1) | lo;
+#pragma unroll
+        for (int i = 0; i <= RW; i++) {
+            const uint32_t vv = rem[i];
+            rem[i] = (vv << 2) | in;
+            in = vv >> 30;
+        }
+        uint32_t trial[RW + 1];
+        uint32_t tin = 1;
+#pragma unroll
+        for (int i = 0; i <= RW; i++) {
+            trial[i] = (root[i] << 2) | tin;
+            tin = root[i] >> 30;
+        }
+        bool ge = true;
+        bool decided = false;
+#pragma unroll
+        for (int i = RW; i >= 0; i--)
+            if (!decided && rem[i] != trial[i]) {
+                ge = rem[i] > trial[i];
+                decided = true;
+            }
+        uint32_t rin = 0;
+#pragma unroll
+        for (int i = 0; i <= RW; i++) {
+            const uint32_t vv = root[i];
+            root[i] = (vv << 1) | rin;
+            rin = vv >> 31;
+        }
+        if (ge) {
+            int64_t br = 0;
+#pragma unroll
+            for (int i = 0; i <= RW; i++) {
+                const int64_t t = (int64_t)rem[i] - trial[i] - br;
+                rem[i] = (uint32_t)t;
+                br = t < 0;
+            }
+            root[0] |= 1u;
+        }
+    }
+    bool sticky = false;
+#pragma unroll
+    for (int i = 0; i <= RW; i++) sticky |= rem[i] != 0;
+    uint32_t mag[RW + 2];
+    uint32_t c = sticky ? 1u : 0u;
+#pragma unroll
+    for (int i = 0; i <= RW; i++) {
+        mag[i] = (root[i] << 1) | c;
+        c = root[i] >> 31;
+    }
+    mag[RW + 1] = c;
+    round_words<W, RW + 2>(false, mag, e / 2 - k - 1, out);
 }
 
 }  // namespace mpd

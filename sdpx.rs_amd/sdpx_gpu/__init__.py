@@ -115,6 +115,12 @@ def _declare(L):
     L.sdpx_gpu_sampled_arrow_dev.argtypes = [u32, u64, vp, u64, vp, vp, vp, vp, vp, u64, vp,
                                              ctypes.POINTER(i32)]
     L.sdpx_gpu_sampled_arrow_dev.restype = i32
+    L.sdpx_gpu_mp_potrf.argtypes = [u32, u8, u64, vp, u64, ctypes.POINTER(i32)]
+    L.sdpx_gpu_mp_potrf.restype = i32
+    L.sdpx_gpu_mp_potrs.argtypes = [u32, u8, u64, u64, vp, u64, vp, u64, ctypes.POINTER(i32)]
+    L.sdpx_gpu_mp_potrs.restype = i32
+    L.sdpx_gpu_mp_trsm_lower.argtypes = [u32, u64, vp, vp, u64, u32]
+    L.sdpx_gpu_mp_trsm_lower.restype = i32
     gemm_args = [u32, u8, u8, u64, u64, u64, vp, u64, vp, u64, u32, vp, u64]
     for name in ("sdpx_gpu_rns_gemm", "sdpx_gpu_rns_gemm_dev"):
         f = getattr(L, name)
@@ -482,6 +488,35 @@ def xgemm_blocks_upper_exact(m, blocks, out):
     code = lib().sdpx_gpu_rns_gemm_blocks_upper(nl, m, len(blocks), arr, _ptr(out))
     check(code, allow_decline=True)
     return code == OK
+
+
+def xpotrf(uplo, n, a, lda):
+    """XpotrfScalar::xpotrf for MPFR (blas/mpfr.rs:727-789) on the GPU: a is
+    a record array (column-major, leading dimension lda), factored in place.
+    Returns info (0, the failing pivot column + 1, or -1/-4 for a bad uplo
+    or lda); raises on a decline."""
+    nl = mp.limbs_of(a)
+    info = ctypes.c_int32(0)
+    check(lib().sdpx_gpu_mp_potrf(nl, ord(uplo), n, _ptr(a), lda, ctypes.byref(info)))
+    return info.value
+
+
+def xpotrs(uplo, n, nrhs, a, lda, b, ldb):
+    """XpotrsScalar::xpotrs for MPFR (blas/mpfr.rs:790-872): b (n x nrhs,
+    leading dimension ldb) is overwritten by the solution.  Returns info."""
+    nl = mp.limbs_of(a)
+    info = ctypes.c_int32(0)
+    check(lib().sdpx_gpu_mp_potrs(nl, ord(uplo), n, nrhs, _ptr(a), lda, _ptr(b), ldb,
+                                  ctypes.byref(info)))
+    return info.value
+
+
+def xtrsm_lower(n, a, b, transpose):
+    """xtrsm_lower for MPFR (blas/traits.rs:85-113): L X = B (or L^T X = B)
+    with a n x n and b n x nrhs, both with leading dimension n; b in place."""
+    nl = mp.limbs_of(a)
+    nrhs = b.size // max(n, 1) if n else 0
+    check(lib().sdpx_gpu_mp_trsm_lower(nl, n, _ptr(a), _ptr(b), nrhs, 1 if transpose else 0))
 
 
 def mp_ldl_solve(l, dinv, n, b):

@@ -27,6 +27,17 @@ def oracle():
         L.oracle_gemm_exact.restype = i32
         L.oracle_gemm_entries.argtypes = [i32, c, c, sz, vp, sz, vp, sz, sz, vp, vp, vp, i32]
         L.oracle_gemm_entries.restype = None
+        L.oracle_potrf.argtypes = [i32, c, sz, vp, sz]
+        L.oracle_potrf.restype = i32
+        L.oracle_trsm_lower.argtypes = [i32, sz, vp, vp, sz, i32]
+        L.oracle_trsm_lower.restype = None
+        L.oracle_potrs.argtypes = [i32, c, sz, sz, vp, sz, vp, sz]
+        L.oracle_potrs.restype = None
+        for name in ("oracle_div",):
+            getattr(L, name).argtypes = [i32, vp, vp, vp]
+            getattr(L, name).restype = None
+        L.oracle_sqrt.argtypes = [i32, vp, vp]
+        L.oracle_sqrt.restype = None
         L.oracle_ldl.argtypes = [i32, vp, sz, sz, vp, sz, vp, vp, vp, i32,
                                  ctypes.POINTER(ctypes.c_longlong), i32]
         L.oracle_ldl.restype = i32
@@ -140,3 +151,25 @@ def oracle_sampled_arrow(problem, nl, border_signs=(-1,), threads=8):
     ok = L.oracle_sampled_arrow(nl, len(blocks), leaves, t, _p(border), _p(signs), signs.size,
                                 _p(bdinv), lp, dp, yp, threads)
     return bool(ok), border, bdinv, outs
+
+
+def oracle_potrf(a, n, lda, uplo="L"):
+    """C oracle xpotrf on a copy of the record array; returns (info, a)."""
+    nl = mp.limbs_of(a)
+    out = a.copy()
+    info = oracle().oracle_potrf(nl, uplo.encode(), n, _p(out), lda)
+    return info, out
+
+
+def oracle_trsm_lower(a, n, b, transpose):
+    nl = mp.limbs_of(a)
+    out = b.copy()
+    oracle().oracle_trsm_lower(nl, n, _p(a), _p(out), b.size // max(n, 1), 1 if transpose else 0)
+    return out
+
+
+def oracle_potrs(a, n, lda, b, nrhs, ldb, uplo="L"):
+    nl = mp.limbs_of(a)
+    out = b.copy()
+    oracle().oracle_potrs(nl, uplo.encode(), n, nrhs, _p(a), lda, _p(out), ldb)
+    return out

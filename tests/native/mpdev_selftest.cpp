@@ -1,6 +1,8 @@
 // Host instantiation of the GPU panel arithmetic (mpdev.h) against the host
 // library's arithmetic (mpx.h), bit for bit, on random values.
 // Build: hipcc -x hip --offload-host-only -O2 -std=c++17 (see test_mpdev.py).
+#include <dlfcn.h>
+
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -162,11 +164,131 @@ int recip_edges() {
     return bad;
 }
 
+// Division and square root against the C oracle (liboracle.so, path in
+// argv[2]): both are correctly rounded, so the bits must agree.
+typedef void (*ora_div_fn)(int, const uint8_t *, const uint8_t *, uint8_t *);
+typedef void (*ora_sqrt_fn)(int, const uint8_t *, uint8_t *);
+template <int N>
+int div_sqrt_vs_oracle(ora_div_fn odiv, ora_sqrt_fn osqrt, int count, uint64_t seed) {
+    constexpr int W = 2 * N;
+    const size_t vb = 8 * N + 8;
+    std::mt19937_64 rng(seed);
+    std::vector<uint8_t> a(vb), b(vb), r1(vb), r2(vb);
+    int bad = 0;
+    for (int t = 0; t < count; t++) {
+        random_value<N>(rng, a.data(), 300);
+        random_value<N>(rng, b.data(), 300);
+        if (t < 4) {  // power of two, all ones, one low bit
+            uint32_t *m = (uint32_t *)a.data();
+            for (int i = 0; i < W; i++) m[i] = t == 1 ? 0xFFFFFFFFu : 0u;
+            m[W - 1] |= 0x80000000u;
+            if (t == 2) m[0] |= 1u;
+            *(int32_t *)(a.data() + 8 * N) = 3;
+            *(int32_t *)(a.data() + 8 * N + 4) = t == 3 ? 7 : 0;
+        }
+        mpd::Val<W> va, vb2, vr;
+        mpd::load((const uint32_t *)a.data(), va);
+        mpd::load((const uint32_t *)b.data(), vb2);
+        if (!mpd::is_zero(vb2)) {
+            mpd::div(va, vb2, vr);
+            mpd::store((uint32_t *)r1.data(), vr);
+            odiv(N, a.data(), b.data(), r2.data());
+            if (std::memcmp(r1.data(), r2.data(), vb)) bad++, std::printf("div mismatch N=%d %d\n", N, t);
+        }
+        if (!mpd::is_zero(va)) {
+            va.kind = mpd::KIND_REG;  // positive radicand
+            *(int32_t *)(a.data() + 8 * N) = 3;
+            mpd::sqrt(va, vr);
+            mpd::store((uint32_t *)r1.data(), vr);
+            osqrt(N, a.data(), r2.data());
+            if (std::memcmp(r1.data(), r2.data(), vb)) bad++, std::printf("sqrt mismatch N=%d %d\n", N, t);
+        }
+    }
+    return bad;
+}
+
+typedef void (*ora_fms_fn)(int, const uint8_t *, const uint8_t *, const uint8_t *, uint8_t *);
+
+// RN(v - a b) against the oracle: random operands, v = RN(a b) (deep
+// cancellation, also exact zero), v far above and far below the product,
+// and zero operands.
+template <int N>
+int fms_vs_oracle(ora_fms_fn ofms, int count, uint64_t seed) {
+    constexpr int W = 2 * N;
+    const size_t vb = 8 * N + 8;
+    std::mt19937_64 rng(seed);
+    std::vector<uint8_t> v(vb), a(vb), b(vb), r1(vb), r2(vb);
+    int bad = 0;
+    for (int t = 0; t < count; t++) {
+        random_value<N>(rng, v.data(), 300);
+        random_value<N>(rng, a.data(), 300);
+        random_value<N>(rng, b.data(), 300);
+        mpd::Val<W> va, vb2, vv, vr;
+        mpd::load((const uint32_t *)a.data(), va);
+        mpd::load((const uint32_t *)b.data(), vb2);
+        mpd::load((const uint32_t *)v.data(), vv);
+        const int mode = t % 6;
+        if (mode == 1 || mode == 2) {  // v = RN(a b), or it nudged by one unit
+            if (!mpd::is_zero(va) && !mpd::is_zero(vb2)) {
+                mpd::mul(va, vb2, vv);
+                if (mode == 2) vv.m[0] ^= 1u;
+            }
+        } else if (mode == 3 && mpd::is_regular(vv) && mpd::is_regular(va) && mpd::is_regular(vb2)) {
+            // v far below or far above the product, up to the buffer's width
+            const int64_t pe = (int64_t)va.e + vb2.e;
+            const int64_t off = (int64_t)(rng() % (4 * 64 * N + 40)) - (2 * 64 * N + 20);
+            const int64_t e = pe + off;
+            if (e > -100000 && e < 100000) vv.e = (int32_t)e;
+        }
+        if (mode == 4 && mpd::is_regular(vv)) {
+            // a b exactly halfway between two W-word values (a = 3 * 2^k,
+            // b's mantissa ending in binary 10) and v far below: only the
+            // sign of v decides the rounding
+            for (int i = 0; i < W; i++) va.m[i] = 0;
+            va.m[W - 1] = 0xC0000000u;
+            va.kind = mpd::KIND_REG;
+            vb2.kind = mpd::KIND_REG;
+            for (int i = 0; i < W; i++) vb2.m[i] = (uint32_t)rng();
+            vb2.m[W - 1] |= 0x80000000u;
+            vb2.m[0] = (vb2.m[0] & ~3u) | 2u;
+            va.e = 5;
+            vb2.e = -3;
+            vv.e = 2 - 32 * W - (int32_t)(rng() % (96 * W));
+            mpd::store((uint32_t *)a.data(), va);
+            mpd::store((uint32_t *)b.data(), vb2);
+        }
+        mpd::store((uint32_t *)v.data(), vv);
+        mpd::fms(vv, va, vb2, vr);
+        mpd::store((uint32_t *)r1.data(), vr);
+        ofms(N, v.data(), a.data(), b.data(), r2.data());
+        if (std::memcmp(r1.data(), r2.data(), vb)) bad++, std::printf("fms mismatch N=%d %d mode %d\n", N, t, mode);
+    }
+    return bad;
+}
+
 int main(int argc, char **argv) {
     int count = argc > 1 ? std::atoi(argv[1]) : 20000;
     int bad = run<1>(count, 200, 1) + run<2>(count, 200, 2) + run<4>(count, 300, 3) +
               run<8>(count, 600, 4);
     bad += recip_edges<1>() + recip_edges<2>() + recip_edges<4>() + recip_edges<8>() + recip_edges<16>();
+    if (argc > 2) {
+        void *h = dlopen(argv[2], RTLD_NOW);
+        if (!h) {
+            std::printf("cannot load %s\n", argv[2]);
+            return 2;
+        }
+        auto odiv = (ora_div_fn)dlsym(h, "oracle_div");
+        auto osqrt = (ora_sqrt_fn)dlsym(h, "oracle_sqrt");
+        auto ofms = (ora_fms_fn)dlsym(h, "oracle_fms");
+        if (!odiv || !osqrt || !ofms) {
+            std::printf("oracle symbols missing\n");
+            return 2;
+        }
+        bad += fms_vs_oracle<1>(ofms, count / 4, 21) + fms_vs_oracle<2>(ofms, count / 4, 22) +
+               fms_vs_oracle<4>(ofms, count / 4, 23) + fms_vs_oracle<8>(ofms, count / 4, 24);
+        bad += div_sqrt_vs_oracle<1>(odiv, osqrt, count / 4, 11) + div_sqrt_vs_oracle<2>(odiv, osqrt, count / 4, 12) +
+               div_sqrt_vs_oracle<4>(odiv, osqrt, count / 4, 13) + div_sqrt_vs_oracle<8>(odiv, osqrt, count / 4, 14);
+    }
     std::printf("mismatches %d\n", bad);
     return bad ? 1 : 0;
 }

@@ -36,3 +36,22 @@ def test_abi_version_and_error_channel():
                                sdpx_gpu._ptr(a), 2, 0, sdpx_gpu._ptr(a), 4)
     assert code == sdpx_gpu.INVALID
     assert "transpose" in sdpx_gpu.last_error()
+
+
+def test_cholesky_argument_codes_before_device_work():
+    # xpotrf / xpotrs argument codes (blas/mpfr.rs:728-736, 801-813) come
+    # back before any device call
+    import ctypes
+    from sdpx_gpu import mp
+    L = sdpx_gpu.lib()
+    a = mp.zeros(4, 2)
+    info = ctypes.c_int32(99)
+    assert L.sdpx_gpu_mp_potrf(2, ord("X"), 2, sdpx_gpu._ptr(a), 2, ctypes.byref(info)) == 0
+    assert info.value == -1
+    assert L.sdpx_gpu_mp_potrf(2, ord("L"), 2, sdpx_gpu._ptr(a), 1, ctypes.byref(info)) == 0
+    assert info.value == -4
+    assert L.sdpx_gpu_mp_potrs(2, ord("U"), 2, 1, sdpx_gpu._ptr(a), 2, sdpx_gpu._ptr(a), 1,
+                               ctypes.byref(info)) == 0
+    assert info.value == -7
+    assert L.sdpx_gpu_mp_potrf(2, ord("L"), 0, None, 1, ctypes.byref(info)) == 0
+    assert info.value == 0

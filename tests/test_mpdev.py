@@ -1,8 +1,10 @@
 """The GPU panel arithmetic (sdpx.rs_amd/csrc/mpdev.h), compiled for the
 host, agrees bit for bit with the host library's arithmetic (mpx.h) on
 random values: products, sums, differences, reciprocals and exact dots
-(with and without a subtrahend) at 64..512 bits.  Both are pinned further
-against the oracles through the LDL^T tests."""
+(with and without a subtrahend) at 64..512 bits; its division and square
+root agree bit for bit with the C oracle's (both correctly rounded).  Both
+are pinned further against the oracles through the LDL^T and Cholesky
+tests."""
 import os
 import subprocess
 
@@ -17,8 +19,10 @@ def test_mpdev_matches_host_arithmetic(tmp_path):
     hipcc = "/opt/rocm/bin/hipcc"
     if not os.path.exists(hipcc):
         pytest.skip("hipcc not present")
-    subprocess.run([hipcc, "-x", "hip", "--offload-host-only", "-O2", "-std=c++17", src, "-o", exe],
-                   check=True, capture_output=True)
-    r = subprocess.run([exe, "6000"], capture_output=True, text=True, timeout=600)
+    subprocess.run([hipcc, "-x", "hip", "--offload-host-only", "-O2", "-std=c++17", src, "-o", exe,
+                    "-ldl"], check=True, capture_output=True)
+    from helpers import oracle
+    so = oracle()._name
+    r = subprocess.run([exe, "6000", so], capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stdout[-2000:]
     assert "mismatches 0" in r.stdout

@@ -158,3 +158,107 @@ def test_decline_rules():
     assert oracle_gemm("N", "N", 3, 3, 4, wide, 3, b, 4) is None
     ok = a.copy(); ok["kind"][:] = 3; ok["exponent"][:] = 0; ok["exponent"][0] = 4096
     assert oracle_gemm("N", "N", 3, 3, 4, ok, 3, b, 4) is not None
+
+
+# ---------------------------------------------------------------------------
+# Exact-dot Cholesky / triangular solves (mpfr.rs:726-872, traits.rs:85-113)
+# ---------------------------------------------------------------------------
+
+def _mp_of(m, e, n, neg=False):
+    return M.Mp(n, [(m >> (64 * i)) & ((1 << 64) - 1) for i in range(n)], -3 if neg else 3, e)
+
+
+@pytest.mark.parametrize("n", [1, 2, 4, 8])
+def test_sqrt_and_division_correctly_rounded(n):
+    # closed form: mpmath's correctly rounded sqrt / division at 64n bits
+    from mpmath import libmp
+    rng = random.Random(n)
+    p = 64 * n
+    specials = [1 << (p - 1), (1 << p) - 1, (1 << (p - 1)) + 1, 3 << (p - 2)]
+    for t in range(200):
+        m = specials[t] if t < len(specials) else rng.getrandbits(p) | (1 << (p - 1))
+        e = rng.randint(-400, 400)
+        x = _mp_of(m, e, n)
+        want = libmp.mpf_sqrt(libmp.from_man_exp(m, e - p), p, "n")
+        assert M.sqrt(x).to_fraction() == Fraction(*libmp.to_rational(want))
+        m2 = rng.getrandbits(p) | (1 << (p - 1))
+        e2 = rng.randint(-300, 300)
+        y = _mp_of(m2, e2, n, neg=bool(t & 1))
+        want = libmp.mpf_div(libmp.from_man_exp(m, e - p), libmp.from_man_exp(-m2 if t & 1 else m2, e2 - p), p, "n")
+        assert M.div(x, y).to_fraction() == Fraction(*libmp.to_rational(want))
+
+
+@pytest.mark.parametrize("n", [2, 4, 8])
+def test_c_oracle_sqrt_div_match_python(n):
+    from helpers import oracle
+    rng = np.random.default_rng(n)
+    a = mp.random_full(60, n, 300, rng, zero_fraction=0.0)
+    a["kind"] = np.abs(a["kind"])  # positive radicands
+    b = mp.random_full(60, n, 300, rng, zero_fraction=0.0)
+    L = oracle()
+    out = mp.zeros(1, n)
+    for i in range(60):
+        L.oracle_sqrt(n, a[i:i + 1].ctypes.data, out.ctypes.data)
+        assert to_mp(out)[0].key() == M.sqrt(to_mp(a[i:i + 1])[0]).key()
+        L.oracle_div(n, a[i:i + 1].ctypes.data, b[i:i + 1].ctypes.data, out.ctypes.data)
+        assert to_mp(out)[0].key() == M.div(to_mp(a[i:i + 1])[0], to_mp(b[i:i + 1])[0]).key()
+
+
+def test_potrf_potrs_reference_known_answer():
+    # tests/it/mpfr_dense.rs:29-38: xpotrf('L', 2, [4, 2, 987, 3]) keeps the
+    # upper entry, and xpotrs solves A x = (6, 5) to x = (1, 1)
+    from helpers import oracle_potrf, oracle_potrs
+    for nl in (2, 4):
+        a = mp.from_float64(np.array([4.0, 2.0, 987.0, 3.0]), nl)
+        info, ch = oracle_potrf(a, 2, 2, "L")
+        assert info == 0
+        assert mp.to_float64(ch)[2] == 987.0
+        py = list(to_mp(a))
+        assert M.potrf("L", 2, py, 2) == 0
+        assert [v.key() for v in py] == [v.key() for v in to_mp(ch)]
+        rhs = oracle_potrs(ch, 2, 2, mp.from_float64(np.array([6.0, 5.0]), nl), 1, 2, "L")
+        x = [v.to_fraction() for v in to_mp(rhs)]
+        eps = Fraction(1, 2 ** (64 * nl - 1))
+        assert all(abs(v - 1) <= 10000 * eps for v in x)
+
+
+def _spd(n, rng, nl, spread=20):
+    g = rng.standard_normal((n, n)) * 2.0 ** rng.integers(-spread, spread, (n, n))
+    s = g @ g.T + n * np.eye(n)
+    return mp.from_float64(s.T.reshape(-1), nl), s
+
+
+@pytest.mark.parametrize("nl,n,uplo", [(2, 9, "L"), (2, 7, "U"), (4, 6, "L")])
+def test_c_oracle_potrf_trsm_potrs_match_python(nl, n, uplo):
+    from helpers import oracle_potrf, oracle_potrs, oracle_trsm_lower
+    rng = np.random.default_rng(n)
+    a, _ = _spd(n, rng, nl)
+    info, ch = oracle_potrf(a, n, n, uplo)
+    py = list(to_mp(a))
+    assert info == 0 and M.potrf(uplo, n, py, n) == 0
+    assert [v.key() for v in py] == [v.key() for v in to_mp(ch)]
+    nrhs = 3
+    b = mp.from_float64(rng.standard_normal(n * nrhs), nl)
+    x = oracle_potrs(ch, n, n, b, nrhs, n, uplo)
+    pyb = list(to_mp(b))
+    M.potrs(uplo, n, nrhs, py, n, pyb, n)
+    assert [v.key() for v in pyb] == [v.key() for v in to_mp(x)]
+    if uplo == "L":
+        for tr in (False, True):
+            t = oracle_trsm_lower(ch, n, b, tr)
+            pyt = list(to_mp(b))
+            M.trsm_lower(n, py, pyt, tr)
+            assert [v.key() for v in pyt] == [v.key() for v in to_mp(t)]
+
+
+def test_potrf_reports_failing_pivot():
+    # an indefinite matrix: info = failing column + 1, the diagonal keeps the
+    # (non-positive) pivot before the square root (mpfr.rs:745-750)
+    from helpers import oracle_potrf
+    s = np.array([[4.0, 2.0, 0.0], [2.0, 1.0, 1.0], [0.0, 1.0, 5.0]])
+    a = mp.from_float64(s.T.reshape(-1), 2)
+    info, ch = oracle_potrf(a, 3, 3, "L")
+    assert info == 2
+    assert mp.to_float64(ch)[4] == 0.0
+    py = list(to_mp(a))
+    assert M.potrf("L", 3, py, 3) == 2
