@@ -86,7 +86,9 @@ def test_failing_pivot_matches_oracle(gpu, uplo):
 def test_trsm_lower_matches_oracle(gpu, nl, n, nrhs):
     print("trsm_lower", nl, n, nrhs, flush=True)
     rng = np.random.default_rng(n + nl)
-    lo = np.tril(rng.standard_normal((n, n)) * 2.0 ** rng.integers(-20, 20, (n, n)))
+    # bounded growth: off-diagonal entries 2^U{-12..-3} N(0,1), diagonal in
+    # [1, 2); the solved entries then span a few hundred bits at most
+    lo = np.tril(rng.standard_normal((n, n)) * 2.0 ** rng.integers(-12, -3, (n, n)), -1)
     lo[np.diag_indices(n)] = 1.0 + rng.random(n)
     a = mp.from_float64(lo.T.reshape(-1), nl)
     b = mp.from_float64(rng.standard_normal(n * nrhs), nl)
@@ -110,3 +112,23 @@ def test_argument_codes_and_declines(gpu):
     with pytest.raises(sdpx_gpu.DeviceError):
         sdpx_gpu.xpotrf("L", n, bad, n)
     assert np.all(mp.same_bits(bad, before))
+
+
+def test_trsm_lower_growth_matches_or_declines(gpu):
+    # entries up to 2^20 make the solution grow by ~2^20 a row: the exact
+    # dots then span more exponents than the GPU window holds, and the call
+    # declines (code 2) with b untouched, or it matches the oracle bitwise
+    nl, n = 4, 120
+    rng = np.random.default_rng(3)
+    lo = np.tril(rng.standard_normal((n, n)) * 2.0 ** rng.integers(-20, 20, (n, n)))
+    lo[np.diag_indices(n)] = 1.0 + rng.random(n)
+    a = mp.from_float64(lo.T.reshape(-1), nl)
+    b = mp.from_float64(rng.standard_normal(n), nl)
+    x = b.copy()
+    try:
+        sdpx_gpu.xtrsm_lower(n, a, x, False)
+    except sdpx_gpu.DeviceError as e:
+        assert "error 2" in str(e)
+        assert np.all(mp.same_bits(x, b))
+        return
+    assert np.all(mp.same_bits(x, oracle_trsm_lower(a, n, b, False)))
