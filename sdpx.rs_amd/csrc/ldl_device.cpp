@@ -83,10 +83,10 @@ int dots_device(DCtx &c, uint8_t *l, size_t ld, size_t n, const std::vector<size
     LdlBlockDesc bd{(int64_t)0, (int64_t)ld, (int64_t)n, 0, 0, (int64_t)sign_off, 0};
     LdlBlockDesc *d_bd = (LdlBlockDesc *)c.arena->alloc(sizeof(LdlBlockDesc));
     HIP_CHECK(hipMemcpyAsync(d_bd, &bd, sizeof bd, hipMemcpyHostToDevice, c.st));
-    const size_t need = (n * (n + 1) + n) * c.vb;
+    const size_t need = ldl_scratch_records(n) * c.vb;
     if (need > c.ldl_scratch_bytes) throw Error(SDPX_GPU_DEVICE_ERROR, "ldl scratch too small");
     HIP_CHECK(hipMemsetAsync(c.d_status, 0, sizeof(int), c.st));
-    mp_ldl_dots(c.n, l, dinv, d_first, c.d_signs, c.sign_mode, d_bd, 1, c.ldl_scratch, c.d_reg_eps,
+    mp_ldl_dots(c.n, l, dinv, d_first, c.d_signs, c.sign_mode, d_bd, 1, (int64_t)n, c.ldl_scratch, c.d_reg_eps,
                 c.d_reg_delta, c.d_status, c.st);
     int status[2] = {0, 0};
     HIP_CHECK(hipMemcpyAsync(status, c.d_status, sizeof(int) * 2, hipMemcpyDeviceToHost, c.st));
@@ -279,7 +279,7 @@ int ldl_factor_device(int limbs, uint8_t *d_l, size_t ld, size_t n, const int8_t
                       LdlStats &stats, bool recursive) {
     const size_t vb = value_bytes(limbs);
     const size_t blk = recursive ? std::min<size_t>(n, 128) : n;
-    DeviceSetup ds(limbs, ldl_arena_bytes(n, vb), (blk * (blk + 1) + blk) * vb + 4096, signs, nsigns,
+    DeviceSetup ds(limbs, ldl_arena_bytes(n, vb), ldl_scratch_records(blk) * vb + 4096, signs, nsigns,
                    reg_eps, reg_delta, &stats);
     DCtx &c = ds.ctx;
     std::vector<size_t> first = envelope_device(c, d_l, ld, n);
@@ -332,8 +332,8 @@ int sampled_arrow_device(int limbs, const std::vector<SampledLeafDev> &leaves, s
     }
     size_t arena = ldl_arena_bytes(t, vb) + (hmax * kmax + kmax * (kmax + 1) / 2 + kmax * t) * vb +
                    t * (t + 1) / 2 * vb + (64u << 20);
-    size_t per_leaf = (kmax * (kmax + 1) + kmax);
-    size_t scratch = std::max<size_t>(leaves.size() * per_leaf * vb, (128 * 129 + 128) * vb) + 4096;
+    size_t per_leaf = ldl_scratch_records(kmax);
+    size_t scratch = std::max<size_t>(leaves.size() * per_leaf * vb, ldl_scratch_records(128) * vb) + 4096;
     LdlStats ls;
     DeviceSetup ds(limbs, arena, scratch, border_signs, nborder_signs, nullptr, nullptr, &ls);
     DCtx &c = ds.ctx;
@@ -376,7 +376,7 @@ int sampled_arrow_device(int limbs, const std::vector<SampledLeafDev> &leaves, s
         const int8_t one = 1;
         HIP_CHECK(hipMemcpyAsync(plus, &one, 1, hipMemcpyHostToDevice, s));
         HIP_CHECK(hipMemsetAsync(c.d_status, 0, sizeof(int) * 2, s));
-        mp_ldl_dots(limbs, d_l_all, d_dinv_all, d_first, plus, 1, d_bd, (int)bds.size(),
+        mp_ldl_dots(limbs, d_l_all, d_dinv_all, d_first, plus, 1, d_bd, (int)bds.size(), (int64_t)kmax,
                     c.ldl_scratch, nullptr, nullptr, c.d_status, s);
         int status[2];
         HIP_CHECK(hipMemcpyAsync(status, c.d_status, sizeof status, hipMemcpyDeviceToHost, s));

@@ -30,10 +30,15 @@ void mp_forward_rows(int limbs, const void *L, const int32_t *first, int64_t K, 
                      const void *lranges, int *fail, hipStream_t st);
 void mp_trsm_base(int limbs, const void *L, int64_t ldl, int64_t n, void *B, int64_t ldb,
                   int64_t m, const int32_t *low, const void *lranges, int *fail, hipStream_t st);
-// status[0] = 0 ok, 1 failed pivot, 2 window overflow; status[1] += regularized
+// status[0] = 0 ok, 1 failed pivot, 2 window overflow (zero on entry);
+// status[1] += regularized.  nmax: the largest block order.  Each block's
+// scratch holds ldl_scratch_records(n) records.
 void mp_ldl_dots(int limbs, void *A, void *dinv, const int32_t *first, const int8_t *signs,
-                 int sign_mode, const void *blocks, int nblocks, void *scratch, const void *reg_eps,
-                 const void *reg_delta, int *status, hipStream_t st);
+                 int sign_mode, const void *blocks, int nblocks, int64_t nmax, void *scratch,
+                 const void *reg_eps, const void *reg_delta, int *status, hipStream_t st);
+// r and w triangles, the pivots, and per-row exponent ranges (16 bytes a
+// row, within one record of any width)
+inline size_t ldl_scratch_records(size_t n) { return n * (n + 1) + 2 * n; }
 void mp_trsm_base_n(int limbs, const void *L, int64_t ldl, int64_t n, void *B, int64_t ldb,
                     int64_t m, const void *lranges, int *fail, hipStream_t st);
 void mp_transpose(int limbs, const void *in, int64_t ldi, int64_t m, int64_t n, void *out,

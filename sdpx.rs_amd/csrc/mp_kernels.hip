@@ -625,6 +625,173 @@ __global__ __launch_bounds__(ldl_threads<W>()) void ldl_dots_kernel(
     }
 }
 
+// ldl_dots over many workgroups (arrow.rs:156-248): launch j forms column j
+// of every block with n > j + 1, one lane group per row i > j, and the group
+// of row j + 1 then forms pivot j + 1 (its look-ahead); launch -1 forms
+// pivot 0.  The kernel boundary between launches is the only
+// synchronisation, so the rows of a column spread over as many CUs as there
+// are rows.  State between launches lives in the block's scratch: the r and w
+// rows (row-major triangles), the pivots d, and per-row exponent ranges
+// (r min, r max, w min, w max) that bound each dot's window.  status[0]
+// (1: failed pivot, 2: window overflow) stops every later launch.
+template <int W>
+struct LdlCol {
+    uint32_t *a, *dv, *r, *w, *dstore;
+    int4 *rng;
+    const int32_t *first;
+    int64_t n, ld;
+    static __device__ int64_t tri(int64_t i) { return i * (i + 1) / 2; }
+    __device__ LdlCol(uint32_t *A, uint32_t *dinv, const int32_t *first_all, uint32_t *scratch,
+                      const LdlBlock &B) {
+        a = A + B.a_off * (W + 2);
+        dv = dinv + B.dinv_off * (W + 2);
+        first = first_all + B.first_off;
+        n = B.n;
+        ld = B.ld;
+        r = scratch + B.scratch_off * (W + 2);
+        w = r + tri(n) * (W + 2);
+        dstore = w + tri(n) * (W + 2);
+        rng = (int4 *)(dstore + n * (W + 2));
+    }
+};
+
+// DenseLeaf::pivot (arrow.rs:618-634) of column j from the exact dot of
+// its r and w rows (one lane group; lane 0 writes d and 1/d).  Returns the
+// status code (0 ok, 1 failed pivot, 2 window overflow) in every lane.
+template <int W, int G>
+__device__ int ldl_col_pivot(const LdlCol<W> &c, int64_t j, int gl, int64_t hl, int64_t hh,
+                             int64_t *slots, const int8_t *signs, int sign_mode, int64_t sign_off,
+                             const uint32_t *reg_eps, const uint32_t *reg_delta, int *status) {
+    const int64_t fj = c.first[j];
+    const uint32_t *xr = rec_at<W>((const uint32_t *)c.r, LdlCol<W>::tri(j));
+    const uint32_t *yr = rec_at<W>((const uint32_t *)c.w, LdlCol<W>::tri(j));
+    Val<W> ajj, t, d;
+    load(rec_at<W>(c.a, j + j * c.ld), ajj);
+    bool ok = group_dot<W, G, ROW_THREADS>(fj, j, gl, xr, 1, yr, 1, nullptr, false, slots, t, hl, hh);
+    if (ok) {
+        add_signed(ajj, t, true, d);
+        // a pivot cancelled to zero is the exact difference rounded once
+        if (is_zero(d) && j > fj)
+            ok = group_dot<W, G, ROW_THREADS>(fj, j, gl, xr, 1, yr, 1, &ajj, true, slots, d, hl, hh);
+    }
+    if (!ok) return 2;
+    const int sg = sign_mode == 0 ? 1 : (sign_mode == 1 ? signs[0] : signs[sign_off + j]);
+    if (!is_regular(d) && !is_zero(d)) return 1;
+    if (reg_eps) {
+        Val<W> ds = d, e, diff;
+        if (sg < 0 && is_regular(ds)) ds.kind = -ds.kind;
+        load(reg_eps, e);
+        add_signed(ds, e, true, diff);
+        if (!is_zero(diff) && diff.kind < 0) {
+            load(reg_delta, d);
+            if (sg < 0 && is_regular(d)) d.kind = -d.kind;
+            if (gl == 0) atomicAdd(status + 1, 1);  // regularized count
+        }
+    }
+    if (is_zero(d)) return 1;
+    if (gl == 0) {
+        store(rec_at<W>(c.dstore, j), d);
+        Val<W> di;
+        recip(d, di);
+        store(rec_at<W>(c.dv, j), di);
+    }
+    return 0;
+}
+
+template <int W, int G>
+__global__ __launch_bounds__(ROW_THREADS) void ldl_col_kernel(
+    uint32_t *A, uint32_t *dinv, const int32_t *first_all, const int8_t *signs, int sign_mode,
+    const LdlBlock *blocks, uint32_t *scratch, const uint32_t *reg_eps, const uint32_t *reg_delta,
+    int *status, int64_t j) {
+    extern __shared__ int64_t slots_sh[];
+    if (status[0] != 0) return;
+    const LdlBlock B = blocks[blockIdx.y];
+    const int gl = threadIdx.x & (G - 1);
+    const int64_t i = j + 1 + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    if (i >= B.n || (j < 0 && i > 0)) return;  // whole groups
+    int64_t *slots = slots_sh + threadIdx.x;
+    const LdlCol<W> c(A, dinv, first_all, scratch, B);
+    auto fail = [&](int code) {
+        if (gl == 0) atomicMax(status, code);
+    };
+    if (j < 0) {  // pivot 0: an empty dot
+        if (i == 0) {
+            if (gl == 0) c.rng[0] = make_int4(INT32_MAX, INT32_MIN, INT32_MAX, INT32_MIN);
+            const int code = ldl_col_pivot<W, G>(c, 0, gl, INT64_MAX, INT64_MIN, slots, signs, sign_mode,
+                                                 B.sign_off, reg_eps, reg_delta, status);
+            if (code) fail(code);
+        }
+        return;
+    }
+    // row i's ranges so far (column 0 starts them)
+    int4 ri = j == 0 ? make_int4(INT32_MAX, INT32_MIN, INT32_MAX, INT32_MIN) : c.rng[i];
+    Val<W> dj, dij, v, wv;
+    load(rec_at<W>((const uint32_t *)c.dstore, j), dj);
+    load(rec_at<W>((const uint32_t *)c.dv, j), dij);
+    if (c.first[i] > j) {
+        set_zero(v);
+    } else {
+        const int64_t lo = max(c.first[i], c.first[j]);
+        const int4 rj = c.rng[j];
+        int64_t hl, hh;
+        range_hint<W>(ri.x, ri.y, rj.z, rj.w, hl, hh);
+        Val<W> t, aij, diff;
+        if (!group_dot<W, G, ROW_THREADS>(lo, j, gl, rec_at<W>((const uint32_t *)c.r, LdlCol<W>::tri(i)), 1,
+                                          rec_at<W>((const uint32_t *)c.w, LdlCol<W>::tri(j)), 1, nullptr,
+                                          false, slots, t, hl, hh)) {
+            fail(2);
+            return;
+        }
+        load(rec_at<W>(c.a, i + j * c.ld), aij);
+        add_signed(aij, t, true, diff);
+        mul(diff, dij, v);
+    }
+    mul(v, dj, wv);
+    if (gl == (int)(j % G)) {  // the lane that reads term j back
+        store(rec_at<W>(c.r, LdlCol<W>::tri(i) + j), v);
+        store(rec_at<W>(c.w, LdlCol<W>::tri(i) + j), wv);
+    }
+    if (!is_zero(v)) {
+        ri.x = min(ri.x, v.e);
+        ri.y = max(ri.y, v.e);
+    }
+    if (!is_zero(wv)) {
+        ri.z = min(ri.z, wv.e);
+        ri.w = max(ri.w, wv.e);
+    }
+    if (gl == 0) c.rng[i] = ri;
+    if (i != j + 1) return;
+    int64_t hl, hh;
+    range_hint<W>(ri.x, ri.y, ri.z, ri.w, hl, hh);
+    const int code = ldl_col_pivot<W, G>(c, i, gl, hl, hh, slots, signs, sign_mode, B.sign_off, reg_eps,
+                                         reg_delta, status);
+    if (code) fail(code);
+}
+
+// the unit lower factor of every block (after the last column launch)
+template <int W>
+__global__ void ldl_col_writeout_kernel(uint32_t *A, const LdlBlock *blocks, uint32_t *scratch,
+                                        const int *status) {
+    if (status[0] != 0) return;
+    const LdlBlock B = blocks[blockIdx.y];
+    const int64_t n = B.n, e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * n) return;
+    const int64_t i = e % n, j = e / n;
+    if (i < j) return;
+    uint32_t *a = A + B.a_off * (W + 2);
+    const uint32_t *r = scratch + B.scratch_off * (W + 2);
+    Val<W> v;
+    if (i == j) {
+        for (int u = 0; u < W; u++) v.m[u] = 0;
+        v.m[W - 1] = 0x80000000u;
+        v.kind = mpd::KIND_REG;
+        v.e = 1;
+    } else {
+        load(rec_at<W>(r, i * (i + 1) / 2 + j), v);
+    }
+    store(rec_at<W>(a, i + j * B.ld), v);
+}
+
 // ---------------------------------------------------------------------------
 // exact-dot Cholesky, triangular solve and solve (mpfr.rs:726-872,
 // traits.rs:85-113)
@@ -878,21 +1045,49 @@ void mp_trsm_base(int limbs, const void *L, int64_t ldl, int64_t n, void *B, int
     HIP_CHECK(hipGetLastError());
 }
 
+// SDPX_GPU_LDL_ONE_WORKGROUP=1: every block's columns on one workgroup
+// (the single-launch form), for comparison.
+static bool ldl_one_workgroup() {
+    static const bool v = [] {
+        const char *e = std::getenv("SDPX_GPU_LDL_ONE_WORKGROUP");
+        return e && e[0] == '1';
+    }();
+    return v;
+}
+
 void mp_ldl_dots(int limbs, void *A, void *dinv, const int32_t *first, const int8_t *signs,
-                 int sign_mode, const void *blocks, int nblocks, void *scratch, const void *reg_eps,
-                 const void *reg_delta, int *status, hipStream_t st) {
-    if (nblocks == 0) return;
+                 int sign_mode, const void *blocks, int nblocks, int64_t nmax, void *scratch,
+                 const void *reg_eps, const void *reg_delta, int *status, hipStream_t st) {
+    if (nblocks == 0 || nmax == 0) return;
+    if (ldl_one_workgroup() && nmax <= LDL_STATS_MAX) {
+        SDPX_DISPATCH_W(limbs, {
+            auto kern = ldl_dots_kernel<W, DOT_G>;
+            constexpr int threads = ldl_threads<W>();
+            const size_t lds = slot_bytes<W>(threads) + 4 * LDL_STATS_MAX * sizeof(int32_t);
+            if (lds > 65536)
+                HIP_CHECK(hipFuncSetAttribute((const void *)kern,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            hipLaunchKernelGGL(kern, dim3(nblocks), dim3(threads), lds, st, (uint32_t *)A,
+                               (uint32_t *)dinv, first, signs, sign_mode, (const LdlBlock *)blocks,
+                               (uint32_t *)scratch, (const uint32_t *)reg_eps,
+                               (const uint32_t *)reg_delta, status);
+        });
+        HIP_CHECK(hipGetLastError());
+        return;
+    }
     SDPX_DISPATCH_W(limbs, {
-        auto kern = ldl_dots_kernel<W, DOT_G>;
-        constexpr int threads = ldl_threads<W>();
-        const size_t lds = slot_bytes<W>(threads) + 4 * LDL_STATS_MAX * sizeof(int32_t);
-        if (lds > 65536)
-            HIP_CHECK(hipFuncSetAttribute((const void *)kern,
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(kern, dim3(nblocks), dim3(threads), lds, st, (uint32_t *)A,
-                           (uint32_t *)dinv, first, signs, sign_mode, (const LdlBlock *)blocks,
-                           (uint32_t *)scratch, (const uint32_t *)reg_eps,
-                           (const uint32_t *)reg_delta, status);
+        auto kern = ldl_col_kernel<W, DOT_G>;
+        const size_t lds = panel_lds<W>((const void *)kern, ROW_THREADS);
+        for (int64_t j = -1; j < nmax - 1; j++) {
+            const int64_t rows = j < 0 ? 1 : nmax - j - 1;
+            hipLaunchKernelGGL(kern, dim3(blocks_for(rows * DOT_G, ROW_THREADS), nblocks), dim3(ROW_THREADS),
+                               lds, st, (uint32_t *)A, (uint32_t *)dinv, first, signs, sign_mode,
+                               (const LdlBlock *)blocks, (uint32_t *)scratch, (const uint32_t *)reg_eps,
+                               (const uint32_t *)reg_delta, status, j);
+        }
+        hipLaunchKernelGGL(ldl_col_writeout_kernel<W>, dim3(blocks_for(nmax * nmax, 256), nblocks), dim3(256),
+                           0, st, (uint32_t *)A, (const LdlBlock *)blocks, (uint32_t *)scratch,
+                           (const int *)status);
     });
     HIP_CHECK(hipGetLastError());
 }

@@ -1287,6 +1287,34 @@ int product_kernel_choice() {
     return choice;
 }
 
+// Tile order of the product kernels.  Each XCD runs a contiguous range of
+// the list with some 64 workgroups in flight, and they read the same k
+// slices of their operand panels at about the same time; listing tiles in
+// bands of `group` column tiles (rows outer, columns inner inside a band)
+// makes that window a compact block of about group x (64 / group) tiles, so
+// each panel slice is fetched into the XCD's L2 once per block instead of
+// once per tile.  SDPX_GPU_TILE_GROUP overrides the band width (0: one band).
+int tile_group() {
+    static const int g = [] {
+        const char *e = std::getenv("SDPX_GPU_TILE_GROUP");
+        return e ? std::atoi(e) : 8;
+    }();
+    return g;
+}
+
+// (row block, column tile) pairs in band order; keep(r, c) filters (upper
+// products keep the blocks that meet the upper triangle).
+template <typename Keep>
+std::vector<int2> banded_tiles(int rows, int cols, Keep keep) {
+    const int g = tile_group() > 0 ? tile_group() : cols;
+    std::vector<int2> out;
+    for (int c0 = 0; c0 < cols; c0 += g)
+        for (int r = 0; r < rows; r++)
+            for (int c = c0; c < std::min(cols, c0 + g); c++)
+                if (keep(r, c)) out.push_back(make_int2(r, c));
+    return out;
+}
+
 template <int D, int BM>
 void launch_product2(const int8_t *pla, const int8_t *plb, int64_t a_plane, int64_t b_plane,
                      int64_t k_pad, int64_t k0, int64_t k1, const uint32_t *d_q, const uint32_t *d_inv,
@@ -1385,10 +1413,8 @@ int rns_gemm_device(const RnsGemmArgs &g) {
     const int64_t PADM = p2 ? 2 * TILE : TILE;
     const int64_t m_pad = (m + PADM - 1) / PADM * PADM, n_pad = (nn + PADM - 1) / PADM * PADM;
     const int tiles_m = (int)(m_pad / TILE), tiles_n = (int)(n_pad / TILE);
-    std::vector<int2> tl;
-    for (int tn = 0; tn < tiles_n; tn++)
-        for (int tm = 0; tm < tiles_m; tm++)
-            if (!g.upper || tm <= tn) tl.push_back(make_int2(tm, tn));
+    const std::vector<int2> tl =
+        banded_tiles(tiles_m, tiles_n, [&](int tm, int tn) { return !g.upper || tm <= tn; });
     const int ntiles = (int)tl.size();
 
     size_t mods_bytes = (size_t)L * (4 + 4 + 8);
@@ -1435,10 +1461,9 @@ int rns_gemm_device(const RnsGemmArgs &g) {
     // 256 x 128 blocks of the second product kernel (one-digit moduli)
     std::vector<int2> tl2;
     if (p2)
-        for (int tn = 0; tn < tiles_n; tn++)
-            for (int tm2 = 0; tm2 < tiles_m / 2; tm2++)
-                if (!g.upper || (int64_t)tm2 * 2 * TILE <= (int64_t)tn * TILE + TILE - 1)
-                    tl2.push_back(make_int2(tm2, tn));
+        tl2 = banded_tiles(tiles_m / 2, tiles_n, [&](int tm2, int tn) {
+            return !g.upper || (int64_t)tm2 * 2 * TILE <= (int64_t)tn * TILE + TILE - 1;
+        });
     const int ntiles2 = (int)tl2.size();
     int2 *d_tiles = (int2 *)ws.get(S_TILES, sizeof(int2) * (ntiles + ntiles2));
     int2 *d_tiles2 = d_tiles + ntiles;
