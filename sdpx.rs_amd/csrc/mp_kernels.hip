@@ -121,37 +121,105 @@ struct Win {
 // across the group (integer adds of the unresolved slots: the same sum a
 // single lane would hold after all k1 - k0 terms), and every lane then
 // carries and rounds the identical total, so all lanes hold the result.
-// 16-lane sum of a 64-bit value inside one DPP row: quad permutations
-// [1,0,3,2] and [2,3,0,1], then the half-row and row mirrors; every lane
-// of the row ends with the row's total.
-template <int CTRL>
-__device__ __forceinline__ int64_t dpp_add64(int64_t v) {
-    const int lo = (int)(uint32_t)(uint64_t)v, hi = (int)((uint64_t)v >> 32);
-    const int plo = __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, false);
-    const int phi = __builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, false);
-    return v + (int64_t)(((uint64_t)(uint32_t)phi << 32) | (uint32_t)plo);
-}
-__device__ __forceinline__ int64_t row16_sum(int64_t v) {
-    v = dpp_add64<0xB1>(v);   // quad_perm [1,0,3,2]
-    v = dpp_add64<0x4E>(v);   // quad_perm [2,3,0,1]
-    v = dpp_add64<0x141>(v);  // row_half_mirror
-    v = dpp_add64<0x140>(v);  // row_mirror
-    return v;
+// LDS slot windows: thread t's slot k is the int64 at slots_sh[k * STRIDE + t],
+// so slot k of the 16 lanes of a group is one contiguous 128-byte run.  The
+// routines below take LDS-typed pointers so every access is a ds_ operation.
+typedef __attribute__((address_space(3))) int64_t lds_i64;
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+typedef long long i64x2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) i64x2 lds_i64x2;
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) u32x4 lds_u32x4;
+
+// s * (p << off) into this lane's window (ls: its slot 0; off >= 0 bits
+// above the window base): L + 1 slots, all read, then all written.
+template <int L, int STRIDE>
+__device__ __forceinline__ void lds_add_words(lds_i64 *ls, int64_t off, const uint32_t (&p)[L], bool neg) {
+    const int wo = (int)(off >> 5), bo = (int)(off & 31);
+    lds_i64 *s = ls + (int64_t)wo * STRIDE;
+    int64_t cur[L + 1];
+#pragma unroll
+    for (int i = 0; i <= L; i++) cur[i] = s[i * STRIDE];
+#pragma unroll
+    for (int i = 0; i <= L; i++) {
+        const uint32_t hi = i < L ? p[i] : 0u, lo = i ? p[i - 1] : 0u;
+        const uint32_t v = bo ? funnel(hi, lo, 32 - bo) : hi;  // (p << bo) word i
+        s[i * STRIDE] = neg ? cur[i] - (int64_t)v : cur[i] + (int64_t)v;
+    }
 }
 
-// Carry-resolve a window of SLOTS signed 64-bit slots (value
-// sum_k v_k 2^(32 k + base)) in registers and round it once into out: the
-// same result as DotWindow::finish, with the word gather done by log-step
-// shifts so every register index is static.
-template <int W, int SLOTS>
-__device__ inline void finish_regs(const int64_t (&v)[SLOTS], int64_t base, Val<W> &out) {
+// s * (a * b) into this lane's window whose slot 0 sits at scale base.
+template <int W, int STRIDE>
+__device__ __forceinline__ void lds_add_product(lds_i64 *ls, int64_t base, const Val<W> &a, const Val<W> &b,
+                                                bool negate) {
+    uint32_t p[2 * W];
+#pragma unroll
+    for (int i = 0; i < 2 * W; i++) p[i] = 0;
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+        uint64_t carry = 0;
+#pragma unroll
+        for (int j = 0; j < W; j++) {
+            const uint64_t t = (uint64_t)a.m[i] * b.m[j] + p[i + j] + carry;
+            p[i + j] = (uint32_t)t;
+            carry = t >> 32;
+        }
+        p[i + W] = (uint32_t)carry;
+    }
+    const bool neg = ((a.kind < 0) != (b.kind < 0)) != negate;
+    lds_add_words<2 * W, STRIDE>(ls, (int64_t)a.e + b.e - 64 * (int64_t)W - base, p, neg);
+}
+
+// Sum the 16 windows of a lane group (slots [0, used); higher slots count as
+// zero), carry-resolve the total and round it once into out; every lane of
+// the group ends with the result.  Lane s % 16 sums slot s over the group in
+// 16-byte reads and writes the total into its own slot s / 16; every lane
+// then reads all totals back, resolves the carries and writes the 32-bit
+// words to the group's first two slot rows, from which the words around the
+// top are gathered.  The LDS operations of a wave complete in issue order, so
+// each read above precedes the writes that reuse its location.
+template <int W, int SLOTS, int STRIDE>
+__device__ __forceinline__ void group_finish(lds_i64 *ls, int gl, int used, int64_t base, Val<W> &out) {
+    constexpr int ROWS = (SLOTS + 15) / 16;
+    static_assert(SLOTS <= 64, "the 32-bit words fit two slot rows");
+    lds_i64 *g0 = ls - gl;  // slot 0 of the group's first lane
+    int64_t tot[ROWS];
+#pragma unroll
+    for (int r = 0; r < ROWS; r++) {
+        const int sl = gl + 16 * r;
+        int64_t acc = 0;
+        if (sl < used) {
+            const lds_i64x2 *row = (const lds_i64x2 *)(g0 + sl * STRIDE);
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const i64x2 x = row[q];
+                acc += x.x + x.y;
+            }
+        }
+        tot[r] = acc;
+    }
+#pragma unroll
+    for (int r = 0; r < ROWS; r++) ls[r * STRIDE] = tot[r];
     uint32_t w[SLOTS];
     int64_t carry = 0;
 #pragma unroll
-    for (int k = 0; k < SLOTS; k++) {
-        const int64_t t = v[k] + carry;
-        w[k] = (uint32_t)(uint64_t)t;
-        carry = t >> 32;
+    for (int r = 0; r < ROWS; r++) {
+        const lds_i64x2 *row = (const lds_i64x2 *)(g0 + r * STRIDE);
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int k = 16 * r + 2 * q;
+            if (k < SLOTS) {
+                const i64x2 x = row[q];
+                int64_t t = x.x + carry;
+                w[k] = (uint32_t)(uint64_t)t;
+                carry = t >> 32;
+                if (k + 1 < SLOTS) {
+                    t = x.y + carry;
+                    w[k + 1] = (uint32_t)(uint64_t)t;
+                    carry = t >> 32;
+                }
+            }
+        }
     }
     const bool neg = carry < 0;
     if (neg) {
@@ -163,40 +231,33 @@ __device__ inline void finish_regs(const int64_t (&v)[SLOTS], int64_t base, Val<
             c = t >> 32;
         }
     }
-    int top = -1;
+    int top = -1, low = SLOTS;
 #pragma unroll
-    for (int k = 0; k < SLOTS; k++)
+    for (int k = 0; k < SLOTS; k++) {
         if (w[k]) top = k;
+        if (w[SLOTS - 1 - k]) low = SLOTS - 1 - k;
+    }
     if (top < 0) {
         set_zero(out);
         return;
     }
+    // words 4q .. 4q+3 at u32 offset 4q of the rows (row length 2 * 16 words)
+#pragma unroll
+    for (int q = 0; q < (SLOTS + 3) / 4; q++) {
+        const int k = 4 * q;
+        lds_u32x4 *dst = (lds_u32x4 *)((lds_u32 *)(g0 + (k / 32) * STRIDE) + (k % 32));
+        *dst = u32x4{w[k], k + 1 < SLOTS ? w[k + 1] : 0u, k + 2 < SLOTS ? w[k + 2] : 0u,
+                     k + 3 < SLOTS ? w[k + 3] : 0u};
+    }
     constexpr int GW = W + 2;
     const int lo = top - GW + 1;
-    bool sticky = false;
-#pragma unroll
-    for (int k = 0; k < SLOTS; k++)
-        if (k < lo && w[k]) sticky = true;
-    if (lo >= 0) {  // w >>= lo words
-#pragma unroll
-        for (int sft = 32; sft >= 1; sft >>= 1)
-            if (lo & sft) {
-#pragma unroll
-                for (int k = 0; k < SLOTS; k++) w[k] = k + sft < SLOTS ? w[k + sft] : 0u;
-            }
-    } else {  // w <<= -lo words (the value sits in the low GW words)
-        const int ls = -lo;
-#pragma unroll
-        for (int sft = 16; sft >= 1; sft >>= 1)
-            if (ls & sft) {
-#pragma unroll
-                for (int k = GW - 1; k >= 0; k--) w[k] = k - sft >= 0 ? w[k - sft] : 0u;
-            }
-    }
     uint32_t g[GW + 1];
-    g[0] = sticky ? 1u : 0u;  // one sticky bit below the gathered words
+    g[0] = low < lo ? 1u : 0u;  // one sticky bit below the gathered words
 #pragma unroll
-    for (int i = 0; i < GW; i++) g[i + 1] = i < SLOTS ? w[i] : 0u;
+    for (int i = 0; i < GW; i++) {
+        const int k = lo + i;
+        g[i + 1] = k >= 0 ? *((lds_u32 *)(g0 + (k >> 5) * STRIDE) + (k & 31)) : 0u;
+    }
     round_words<W, GW + 1>(neg, g, base + 32 * (int64_t)(lo - 1), out);
 }
 
@@ -250,8 +311,8 @@ __device__ __attribute__((noinline)) bool group_dot(int64_t k0, int64_t k1, int 
     if ((hi - lo) / 32 + 2 * W + 4 > SLOTS) return false;
     // terms land in slots [0, (hi - lo)/32 + 2W]
     const int used = (int)min((int64_t)SLOTS, (hi - lo) / 32 + 2 * W + 1);
-    for (int sl = 0; sl < used; sl++) slots[(int64_t)sl * STRIDE] = 0;
-    DotWindow<W, SLOTS> dw{slots, STRIDE, lo};
+    lds_i64 *ls = (lds_i64 *)slots;
+    for (int sl = 0; sl < used; sl++) ls[sl * STRIDE] = 0;
     Val<W> xn, yn;
     int64_t k = kf;
     if (k < k1) {
@@ -265,13 +326,10 @@ __device__ __attribute__((noinline)) bool group_dot(int64_t k0, int64_t k1, int 
             load(fy(k + G), yn);
         }
         if (is_zero(x) || is_zero(y)) continue;
-        dw.add_product(x, y, negate_terms);
+        lds_add_product<W, STRIDE>(ls, lo, x, y, negate_terms);
     }
-    if (cnz && gl == 0) dw.add_value(*c, false);
-    int64_t v[SLOTS];
-#pragma unroll
-    for (int sl = 0; sl < SLOTS; sl++) v[sl] = sl < used ? row16_sum(slots[(int64_t)sl * STRIDE]) : 0;
-    finish_regs<W, SLOTS>(v, lo, out);
+    if (cnz && gl == 0) lds_add_words<W, STRIDE>(ls, (int64_t)c->e - 32 * W - lo, c->m, c->kind < 0);
+    group_finish<W, SLOTS, STRIDE>(ls, gl, used, lo, out);
     return true;
 }
 
