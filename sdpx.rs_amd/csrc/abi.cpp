@@ -140,7 +140,9 @@ int32_t sdpx_gpu_rns_gemm_dev(uint32_t limbs, uint8_t ta, uint8_t tb, uint64_t m
         sdpx::RnsGemmArgs args{(int)limbs, trans_norm(ta), trans_norm(tb), (int64_t)m, (int64_t)n,
                                (int64_t)k, a, (int64_t)lda, b, (int64_t)ldb, upper != 0, out_len,
                                out};
-        return sdpx::rns_gemm_device(args);
+        const int32_t r = sdpx::rns_gemm_device(args);
+        HIP_CHECK(hipStreamSynchronize(dev.stream));  // the result is complete on return
+        return r;
     });
 }
 

@@ -330,8 +330,9 @@ int sampled_arrow_device(int limbs, const std::vector<SampledLeafDev> &leaves, s
         yoff.push_back(ksum * t);
         ksum += l.k;
     }
-    size_t arena = ldl_arena_bytes(t, vb) + (hmax * kmax + kmax * (kmax + 1) / 2 + kmax * t) * vb +
-                   t * (t + 1) / 2 * vb + (64u << 20);
+    // leaf scratch, Z of every leaf, one packed contribution, the border factor
+    size_t arena = ldl_arena_bytes(t, vb) + (hmax * kmax + kmax * (kmax + 1) / 2 + ksum * t) * vb +
+                   t * (t + 1) / 2 * vb + 16 * ksum + (64u << 20);
     size_t per_leaf = ldl_scratch_records(kmax);
     size_t scratch = std::max<size_t>(leaves.size() * per_leaf * vb, ldl_scratch_records(128) * vb) + 4096;
     LdlStats ls;
@@ -388,25 +389,39 @@ int sampled_arrow_device(int limbs, const std::vector<SampledLeafDev> &leaves, s
             for (auto &e : ev) (void)hipEventDestroy(e);
             return 0;
         }
-        // 3. Y, Z, contribution and border update, in leaf order
-        for (size_t i = 0; i < leaves.size() && !c.declined; i++) {
-            const auto &lf = leaves[i];
-            size_t m2 = c.arena->top;
-            int32_t *d_f = upload_ints(c, fl[i]);
-            uint8_t *Y = c.at(d_y_all, yoff[i]);
-            HIP_CHECK(hipMemcpyAsync(Y, lf.b, lf.k * t * vb, hipMemcpyDeviceToDevice, s));
-            void *ranges = c.arena->alloc(8 * lf.k);
-            mp_exp_ranges(limbs, c.at(d_l_all, loff[i]), (int64_t)lf.k, (int64_t)lf.k, 0, ranges, s);
-            mp_forward_rows(limbs, c.at(d_l_all, loff[i]), d_f, (int64_t)lf.k, Y, (int64_t)t, ranges,
+        // 3. Y = L^-1 B of every leaf in one launch and Z = RN(D^-1 Y); then
+        // the contributions Y^T Z, each exact and rounded once, subtracted
+        // from the border in leaf order
+        if (!c.declined) {
+            size_t mark3 = c.arena->top;
+            int32_t *d_fc = upload_ints(c, first_cat);
+            int2 *ranges = (int2 *)c.arena->alloc(sizeof(int2) * std::max<size_t>(ksum, 1));
+            std::vector<FwdLeaf> fd;
+            for (size_t i = 0; i < leaves.size(); i++) {
+                const size_t K = leaves[i].k;
+                HIP_CHECK(hipMemcpyAsync(c.at(d_y_all, yoff[i]), leaves[i].b, K * t * vb, hipMemcpyDeviceToDevice,
+                                         s));
+                mp_exp_ranges(limbs, c.at(d_l_all, loff[i]), (int64_t)K, (int64_t)K, 0, ranges + doff[i], s);
+                fd.push_back(FwdLeaf{(int64_t)loff[i], (int64_t)doff[i], (int64_t)K, (int64_t)yoff[i],
+                                     (int64_t)doff[i]});
+            }
+            FwdLeaf *d_fd = (FwdLeaf *)c.arena->alloc(sizeof(FwdLeaf) * fd.size());
+            HIP_CHECK(hipMemcpyAsync(d_fd, fd.data(), sizeof(FwdLeaf) * fd.size(), hipMemcpyHostToDevice, s));
+            mp_forward_rows(limbs, d_l_all, d_fc, d_y_all, ranges, d_fd, (int)fd.size(), (int64_t)t,
                             c.d_status + 2, s);
-            uint8_t *Z = (uint8_t *)c.arena->alloc(lf.k * t * vb);
-            mp_ew_mul_diag(limbs, Y, lf.k, c.at(d_dinv_all, doff[i]), Z, lf.k, lf.k, t, true, s);
+            uint8_t *Z_all = (uint8_t *)c.arena->alloc(std::max<size_t>(ksum, 1) * t * vb);
+            for (size_t i = 0; i < leaves.size(); i++)
+                mp_ew_mul_diag(limbs, c.at(d_y_all, yoff[i]), leaves[i].k, c.at(d_dinv_all, doff[i]),
+                               c.at(Z_all, yoff[i]), leaves[i].k, leaves[i].k, t, true, s);
             uint8_t *P = (uint8_t *)c.arena->alloc(t * (t + 1) / 2 * vb);
-            product(c, 'T', 'N', t, t, lf.k, Y, lf.k, Z, lf.k, P, true);
-            if (c.declined) break;
-            mp_ew_sub_packed(limbs, d_border, t, P, t, 0, s);
-            HIP_CHECK(hipStreamSynchronize(s));
-            c.arena->top = m2;
+            for (size_t i = 0; i < leaves.size() && !c.declined; i++) {
+                const size_t K = leaves[i].k;
+                product(c, 'T', 'N', t, t, K, c.at(d_y_all, yoff[i]), K, c.at(Z_all, yoff[i]), K, P, true);
+                if (c.declined) break;
+                mp_ew_sub_packed(limbs, d_border, t, P, t, 0, s);
+            }
+            HIP_CHECK(hipStreamSynchronize(s));  // fd's host copy and the arena
+            c.arena->top = mark3;
         }
     }
     HIP_CHECK(hipEventRecord(ev[2], s));

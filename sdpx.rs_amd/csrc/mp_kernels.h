@@ -26,8 +26,16 @@ void mp_sampled_entries(int limbs, const void *G, const void *w, int64_t K, void
 // on them.
 void mp_exp_ranges(int limbs, const void *L, int64_t ldl, int64_t n, int mode, void *out,
                    hipStream_t st);
-void mp_forward_rows(int limbs, const void *L, const int32_t *first, int64_t K, void *Y, int64_t t,
-                     const void *lranges, int *fail, hipStream_t st);
+// Y = L^-1 B of many leaves in one launch (forward_suffix_rows per leaf):
+// leaf i's L (K x K, ld K) at record l_off of L_all, its envelope at
+// first_off of first_all, Y (K x t, ld K, B on entry) at record y_off of
+// Y_all, its row ranges (int2) at range_off of ranges_all (may be null).
+struct FwdLeaf {
+    int64_t l_off, first_off, k, y_off, range_off;
+};
+void mp_forward_rows(int limbs, const void *L_all, const int32_t *first_all, void *Y_all,
+                     const void *ranges_all, const FwdLeaf *leaves, int nleaves, int64_t t, int *fail,
+                     hipStream_t st);
 void mp_trsm_base(int limbs, const void *L, int64_t ldl, int64_t n, void *B, int64_t ldb,
                   int64_t m, const int32_t *low, const void *lranges, int *fail, hipStream_t st);
 // status[0] = 0 ok, 1 failed pivot, 2 window overflow (zero on entry);

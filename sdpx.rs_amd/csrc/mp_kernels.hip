@@ -379,15 +379,20 @@ __global__ void exp_ranges_kernel(const uint32_t *L, int64_t ldl, int64_t n, int
 }
 
 template <int W, int G>
-__global__ __launch_bounds__(ROW_THREADS) void forward_rows_kernel(
-    const uint32_t *L, const int32_t *first, int64_t K, uint32_t *Y, int64_t t, const int2 *lr,
-    int *fail) {
+__global__ __launch_bounds__(ROW_THREADS) void forward_rows_kernel(const uint32_t *L_all, const int32_t *first_all,
+                                                                   uint32_t *Y_all, const int2 *lr_all,
+                                                                   const FwdLeaf *leaves, int64_t t, int *fail) {
     extern __shared__ int64_t slots_sh[];
+    const FwdLeaf lf = leaves[blockIdx.y];
     const int gl = threadIdx.x & (G - 1);
     const int64_t c = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
     if (c >= t) return;  // whole group
+    const int64_t K = lf.k;
+    const uint32_t *L = rec_at<W>(L_all, lf.l_off);
+    const int32_t *first = first_all + lf.first_off;
+    const int2 *lr = lr_all ? lr_all + lf.range_off : nullptr;
     int64_t *slots = slots_sh + threadIdx.x;
-    uint32_t *col = Y + c * K * (W + 2);
+    uint32_t *col = rec_at<W>(Y_all, lf.y_off + c * K);
     int32_t ymn = INT32_MAX, ymx = INT32_MIN;  // y_k so far
     for (int64_t i = 0; i < K; i++) {
         Val<W> b, r;
@@ -1077,15 +1082,16 @@ void mp_exp_ranges(int limbs, const void *L, int64_t ldl, int64_t n, int mode, v
     HIP_CHECK(hipGetLastError());
 }
 
-void mp_forward_rows(int limbs, const void *L, const int32_t *first, int64_t K, void *Y, int64_t t,
-                     const void *lranges, int *fail, hipStream_t st) {
-    if (K == 0 || t == 0) return;
+void mp_forward_rows(int limbs, const void *L_all, const int32_t *first_all, void *Y_all,
+                     const void *ranges_all, const FwdLeaf *leaves, int nleaves, int64_t t, int *fail,
+                     hipStream_t st) {
+    if (nleaves == 0 || t == 0) return;
     SDPX_DISPATCH_W(limbs, {
         auto kern = forward_rows_kernel<W, DOT_G>;
         const size_t lds = panel_lds<W>((const void *)kern, ROW_THREADS);
-        hipLaunchKernelGGL(kern, dim3(blocks_for(t * DOT_G, ROW_THREADS)),
-                           dim3(ROW_THREADS), lds, st, (const uint32_t *)L, first, K,
-                           (uint32_t *)Y, t, (const int2 *)lranges, fail);
+        hipLaunchKernelGGL(kern, dim3(blocks_for(t * DOT_G, ROW_THREADS), nleaves), dim3(ROW_THREADS), lds, st,
+                           (const uint32_t *)L_all, first_all, (uint32_t *)Y_all, (const int2 *)ranges_all,
+                           leaves, t, fail);
     });
     HIP_CHECK(hipGetLastError());
 }

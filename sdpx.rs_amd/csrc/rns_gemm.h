@@ -38,7 +38,8 @@ struct RnsTotals {
 };
 RnsTotals &rns_totals();
 
-// Runs on the library stream and returns after the results are complete.
+// Runs on the library stream; the results are complete once the work queued
+// so far on that stream is (the call waits only for its scan windows).
 // Returns an SDPX_GPU_* code; throws sdpx::Error on device failures.
 int rns_gemm_device(const RnsGemmArgs &args);
 const RnsStats &rns_last_stats();

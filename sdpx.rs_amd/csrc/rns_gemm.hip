@@ -22,6 +22,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <climits>
 #include <cmath>
 #include <cstdlib>
@@ -1343,6 +1345,94 @@ thread_local RnsTotals t_totals;
 RnsTotals &rns_totals() { return t_totals; }
 void rns_set_profiling(bool on) { g_profile = on; }
 
+// Device copies of the tables that depend only on the modulus plan (and the
+// encode window) or on the tile grid.  They are made once per device and
+// kept for the life of the process, so a product uploads nothing but reads
+// back its two scan windows.
+struct PlanTables {
+    uint32_t *q = nullptr, *inv = nullptr;
+    double *qinv = nullptr, *crt = nullptr;
+    uint8_t *crt2 = nullptr;
+    EncodeTables enc[ENC_MAX_NK + 1];
+    uint8_t *d_enc[ENC_MAX_NK + 1] = {};
+    std::map<int, double *> pow2;
+};
+std::mutex g_tables_mu;
+std::map<std::pair<int, const RnsPlan *>, PlanTables> g_plan_tables;
+std::map<std::vector<int>, int2 *> g_tile_tables;
+
+template <typename T>
+T *upload_once(const void *src, size_t bytes) {
+    void *d = nullptr;
+    HIP_CHECK(hipMalloc(&d, std::max<size_t>(bytes, 16)));
+    if (bytes) HIP_CHECK(hipMemcpy(d, src, bytes, hipMemcpyHostToDevice));
+    return (T *)d;
+}
+
+int current_device() {
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    return dev;
+}
+
+// The plan's moduli, CRT and matrix-core CRT tables (made on first use).
+PlanTables &plan_tables(const RnsPlan &plan) {
+    std::lock_guard<std::mutex> g(g_tables_mu);
+    PlanTables &t = g_plan_tables[{current_device(), &plan}];
+    if (!t.q) {
+        const size_t L = plan.count, J = plan.limbs;
+        t.q = upload_once<uint32_t>(plan.q.data(), 4 * L);
+        t.inv = upload_once<uint32_t>(plan.inv.data(), 4 * L);
+        t.qinv = upload_once<double>(plan.qinv.data(), 8 * L);
+        std::vector<double> crt(plan.c.begin(), plan.c.begin() + L * J);
+        crt.insert(crt.end(), plan.m.begin(), plan.m.begin() + J);
+        t.crt = upload_once<double>(crt.data(), 8 * crt.size());
+        std::vector<uint8_t> c2(plan.crt_table.begin(), plan.crt_table.end());
+        const uint8_t *m32 = (const uint8_t *)plan.m32.data();
+        c2.insert(c2.end(), m32, m32 + 4 * plan.m32.size());
+        t.crt2 = upload_once<uint8_t>(c2.data(), c2.size());
+    }
+    return t;
+}
+
+// Matrix-core encode tables for nk k-steps of value bytes.
+const EncodeTables &encode_tables_dev(const RnsPlan &plan, int nk, uint8_t *&dev) {
+    PlanTables &t = plan_tables(plan);
+    std::lock_guard<std::mutex> g(g_tables_mu);
+    if (!t.d_enc[nk]) {
+        t.enc[nk] = encode_tables(plan, nk);
+        t.d_enc[nk] = upload_once<uint8_t>(t.enc[nk].bytes.data(), t.enc[nk].bytes.size());
+    }
+    dev = t.d_enc[nk];
+    return t.enc[nk];
+}
+
+// 24-bit chunk powers for the VALU encode form.
+double *pow2_table_dev(const RnsPlan &plan, int TJ) {
+    PlanTables &t = plan_tables(plan);
+    std::lock_guard<std::mutex> g(g_tables_mu);
+    double *&d = t.pow2[TJ];
+    if (!d) {
+        const std::vector<double> h = pow2_table(plan, TJ);
+        d = upload_once<double>(h.data(), 8 * h.size());
+    }
+    return d;
+}
+
+// The product kernels' tile lists (tl then tl2) for one grid.
+int2 *tile_tables_dev(const std::vector<int2> &tl, const std::vector<int2> &tl2, const std::vector<int> &key) {
+    std::lock_guard<std::mutex> g(g_tables_mu);
+    std::vector<int> k = key;
+    k.push_back(current_device());
+    int2 *&d = g_tile_tables[k];
+    if (!d) {
+        std::vector<int2> all(tl);
+        all.insert(all.end(), tl2.begin(), tl2.end());
+        d = upload_once<int2>(all.data(), sizeof(int2) * all.size());
+    }
+    return d;
+}
+
 int rns_gemm_device(const RnsGemmArgs &g) {
     Device &dev = device();
     hipStream_t s = dev.stream;
@@ -1417,47 +1507,22 @@ int rns_gemm_device(const RnsGemmArgs &g) {
         banded_tiles(tiles_m, tiles_n, [&](int tm, int tn) { return !g.upper || tm <= tn; });
     const int ntiles = (int)tl.size();
 
-    size_t mods_bytes = (size_t)L * (4 + 4 + 8);
-    char *mods = (char *)ws.get(S_MODS, mods_bytes);
-    uint32_t *d_q = (uint32_t *)mods;
-    uint32_t *d_inv = (uint32_t *)(mods + 4 * L);
-    double *d_qinv = (double *)(mods + 8 * L);
-    HIP_CHECK(hipMemcpyAsync(d_q, plan->q.data(), 4 * L, hipMemcpyHostToDevice, s));
-    HIP_CHECK(hipMemcpyAsync(d_inv, plan->inv.data(), 4 * L, hipMemcpyHostToDevice, s));
-    HIP_CHECK(hipMemcpyAsync(d_qinv, plan->qinv.data(), 8 * L, hipMemcpyHostToDevice, s));
+    PlanTables &ptab = plan_tables(*plan);
+    uint32_t *d_q = ptab.q;
+    uint32_t *d_inv = ptab.inv;
+    double *d_qinv = ptab.qinv;
     // encode tables: the matrix-core form when the window fits 3 k-steps,
     // else 24-bit chunk powers for the VALU form
     const int nka = (int)(((wa.width + 7) / 8 + 31) / 32), nkb = (int)(((wb.width + 7) / 8 + 31) / 32);
     const bool mfma_a = g_encode_mfma && nka <= ENC_MAX_NK, mfma_b = g_encode_mfma && nkb <= ENC_MAX_NK;
-    EncodeTables eta, etb;
+    static const EncodeTables no_tables;
     uint8_t *d_eta = nullptr, *d_etb = nullptr;
-    double *d_pa = nullptr, *d_pb = nullptr;
-    std::vector<double> pa, pb;
-    if (mfma_a) {
-        eta = encode_tables(*plan, nka);
-        d_eta = (uint8_t *)ws.get(S_ENC_A, eta.bytes.size());
-        HIP_CHECK(hipMemcpyAsync(d_eta, eta.bytes.data(), eta.bytes.size(), hipMemcpyHostToDevice, s));
-    } else {
-        pa = pow2_table(*plan, TJa);
-        d_pa = (double *)ws.get(S_POW_A, 8 * pa.size());
-        HIP_CHECK(hipMemcpyAsync(d_pa, pa.data(), 8 * pa.size(), hipMemcpyHostToDevice, s));
-    }
-    if (!same) {
-        if (mfma_b) {
-            etb = encode_tables(*plan, nkb);
-            d_etb = (uint8_t *)ws.get(S_ENC_B, etb.bytes.size());
-            HIP_CHECK(hipMemcpyAsync(d_etb, etb.bytes.data(), etb.bytes.size(), hipMemcpyHostToDevice, s));
-        } else {
-            pb = pow2_table(*plan, TJb);
-            d_pb = (double *)ws.get(S_POW_B, 8 * pb.size());
-            HIP_CHECK(hipMemcpyAsync(d_pb, pb.data(), 8 * pb.size(), hipMemcpyHostToDevice, s));
-        }
-    }
+    const EncodeTables &eta = mfma_a ? encode_tables_dev(*plan, nka, d_eta) : no_tables;
+    const EncodeTables &etb = !same && mfma_b ? encode_tables_dev(*plan, nkb, d_etb) : no_tables;
+    double *d_pa = mfma_a ? nullptr : pow2_table_dev(*plan, TJa);
+    double *d_pb = same || mfma_b ? nullptr : pow2_table_dev(*plan, TJb);
     const int J = plan->limbs;
-    double *d_crt = (double *)ws.get(S_CRT, 8 * ((size_t)L * J + J));
-    HIP_CHECK(hipMemcpyAsync(d_crt, plan->c.data(), 8 * (size_t)L * J, hipMemcpyHostToDevice, s));
-    HIP_CHECK(hipMemcpyAsync(d_crt + (size_t)L * J, plan->m.data(), 8 * (size_t)J,
-                             hipMemcpyHostToDevice, s));
+    double *d_crt = ptab.crt;
     // 256 x 128 blocks of the second product kernel (one-digit moduli)
     std::vector<int2> tl2;
     if (p2)
@@ -1465,11 +1530,8 @@ int rns_gemm_device(const RnsGemmArgs &g) {
             return !g.upper || (int64_t)tm2 * 2 * TILE <= (int64_t)tn * TILE + TILE - 1;
         });
     const int ntiles2 = (int)tl2.size();
-    int2 *d_tiles = (int2 *)ws.get(S_TILES, sizeof(int2) * (ntiles + ntiles2));
+    int2 *d_tiles = tile_tables_dev(tl, tl2, {tiles_m, tiles_n, g.upper ? 1 : 0, p2 ? 1 : 0, tile_group()});
     int2 *d_tiles2 = d_tiles + ntiles;
-    HIP_CHECK(hipMemcpyAsync(d_tiles, tl.data(), sizeof(int2) * ntiles, hipMemcpyHostToDevice, s));
-    if (ntiles2)
-        HIP_CHECK(hipMemcpyAsync(d_tiles2, tl2.data(), sizeof(int2) * ntiles2, hipMemcpyHostToDevice, s));
 
     // 2. encode
     sev.mark(1, s);
@@ -1528,9 +1590,7 @@ int rns_gemm_device(const RnsGemmArgs &g) {
     if (crt_mma) {
         const int lpad = plan->crt_lpad, npad = plan->crt_npad, nu = plan->crt_nu;
         const size_t tb = plan->crt_table.size();
-        uint8_t *d_ct = (uint8_t *)ws.get(S_CRT2, tb + 4 * (size_t)nu);
-        HIP_CHECK(hipMemcpyAsync(d_ct, plan->crt_table.data(), tb, hipMemcpyHostToDevice, s));
-        HIP_CHECK(hipMemcpyAsync(d_ct + tb, plan->m32.data(), 4 * (size_t)nu, hipMemcpyHostToDevice, s));
+        const uint8_t *d_ct = ptab.crt2;
         const int trow = 2 * lpad + 16;
         const size_t lds = (size_t)npad * trow + (size_t)lpad * CRT_XROW * 2;
         const int64_t items = (int64_t)ntiles * TILE;
@@ -1562,8 +1622,7 @@ int rns_gemm_device(const RnsGemmArgs &g) {
                            m, nn, up, pk, n, outw);
     HIP_CHECK(hipGetLastError());
     sev.mark(4, s);
-    HIP_CHECK(hipStreamSynchronize(s));
-    sev.collect(t_stats);
+    sev.collect(t_stats);  // waits for the stage events when profiling; otherwise nothing waits here
     if (g_profile) {
         // algorithmic work: one exact dot of k terms per output; one INT8
         // GEMM per single-digit modulus and four per two-digit modulus
