@@ -170,28 +170,29 @@ __device__ __forceinline__ void lds_add_product(lds_i64 *ls, int64_t base, const
     lds_add_words<2 * W, STRIDE>(ls, (int64_t)a.e + b.e - 64 * (int64_t)W - base, p, neg);
 }
 
-// Sum the 16 windows of a lane group (slots [0, used); higher slots count as
+// Sum the G windows of a lane group (slots [0, used); higher slots count as
 // zero), carry-resolve the total and round it once into out; every lane of
-// the group ends with the result.  Lane s % 16 sums slot s over the group in
-// 16-byte reads and writes the total into its own slot s / 16; every lane
+// the group ends with the result.  Lane s % G sums slot s over the group in
+// 16-byte reads and writes the total into its own slot s / G; every lane
 // then reads all totals back, resolves the carries and writes the 32-bit
-// words to the group's first two slot rows, from which the words around the
-// top are gathered.  The LDS operations of a wave complete in issue order, so
-// each read above precedes the writes that reuse its location.
-template <int W, int SLOTS, int STRIDE>
+// words to the group's first slot rows (2G words a row), from which the
+// words around the top are gathered.  The LDS operations of a wave complete
+// in issue order, so each read above precedes the writes that reuse its
+// location.
+template <int W, int G, int SLOTS, int STRIDE>
 __device__ __forceinline__ void group_finish(lds_i64 *ls, int gl, int used, int64_t base, Val<W> &out) {
-    constexpr int ROWS = (SLOTS + 15) / 16;
-    static_assert(SLOTS <= 64, "the 32-bit words fit two slot rows");
-    lds_i64 *g0 = ls - gl;  // slot 0 of the group's first lane
+    constexpr int ROWS = (SLOTS + G - 1) / G;
+    constexpr int WROW = 2 * G;  // 32-bit words per slot row
+    lds_i64 *g0 = ls - gl;       // slot 0 of the group's first lane
     int64_t tot[ROWS];
 #pragma unroll
     for (int r = 0; r < ROWS; r++) {
-        const int sl = gl + 16 * r;
+        const int sl = gl + G * r;
         int64_t acc = 0;
         if (sl < used) {
             const lds_i64x2 *row = (const lds_i64x2 *)(g0 + sl * STRIDE);
 #pragma unroll
-            for (int q = 0; q < 8; q++) {
+            for (int q = 0; q < G / 2; q++) {
                 const i64x2 x = row[q];
                 acc += x.x + x.y;
             }
@@ -203,22 +204,15 @@ __device__ __forceinline__ void group_finish(lds_i64 *ls, int gl, int used, int6
     uint32_t w[SLOTS];
     int64_t carry = 0;
 #pragma unroll
-    for (int r = 0; r < ROWS; r++) {
-        const lds_i64x2 *row = (const lds_i64x2 *)(g0 + r * STRIDE);
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const int k = 16 * r + 2 * q;
-            if (k < SLOTS) {
-                const i64x2 x = row[q];
-                int64_t t = x.x + carry;
-                w[k] = (uint32_t)(uint64_t)t;
-                carry = t >> 32;
-                if (k + 1 < SLOTS) {
-                    t = x.y + carry;
-                    w[k + 1] = (uint32_t)(uint64_t)t;
-                    carry = t >> 32;
-                }
-            }
+    for (int k = 0; k < SLOTS; k += 2) {
+        const i64x2 x = *(const lds_i64x2 *)(g0 + (k / G) * STRIDE + (k % G));
+        int64_t t = x.x + carry;
+        w[k] = (uint32_t)(uint64_t)t;
+        carry = t >> 32;
+        if (k + 1 < SLOTS) {
+            t = x.y + carry;
+            w[k + 1] = (uint32_t)(uint64_t)t;
+            carry = t >> 32;
         }
     }
     const bool neg = carry < 0;
@@ -241,11 +235,10 @@ __device__ __forceinline__ void group_finish(lds_i64 *ls, int gl, int used, int6
         set_zero(out);
         return;
     }
-    // words 4q .. 4q+3 at u32 offset 4q of the rows (row length 2 * 16 words)
 #pragma unroll
     for (int q = 0; q < (SLOTS + 3) / 4; q++) {
         const int k = 4 * q;
-        lds_u32x4 *dst = (lds_u32x4 *)((lds_u32 *)(g0 + (k / 32) * STRIDE) + (k % 32));
+        lds_u32x4 *dst = (lds_u32x4 *)((lds_u32 *)(g0 + (k / WROW) * STRIDE) + (k % WROW));
         *dst = u32x4{w[k], k + 1 < SLOTS ? w[k + 1] : 0u, k + 2 < SLOTS ? w[k + 2] : 0u,
                      k + 3 < SLOTS ? w[k + 3] : 0u};
     }
@@ -256,21 +249,21 @@ __device__ __forceinline__ void group_finish(lds_i64 *ls, int gl, int used, int6
 #pragma unroll
     for (int i = 0; i < GW; i++) {
         const int k = lo + i;
-        g[i + 1] = k >= 0 ? *((lds_u32 *)(g0 + (k >> 5) * STRIDE) + (k & 31)) : 0u;
+        g[i + 1] = k >= 0 ? *((lds_u32 *)(g0 + (k / WROW) * STRIDE) + (k % WROW)) : 0u;
     }
     round_words<W, GW + 1>(neg, g, base + 32 * (int64_t)(lo - 1), out);
 }
 
 // Terms x_k = xb[k * xs], y_k = yb[k * ys] (record strides): every panel
 // kernel's operands are affine in k, so one out-of-line copy per width and
-// slot stride serves all call sites.  Groups are 16 lanes (one DPP row).
+// slot stride serves all call sites.  Groups are 16, 32 or 64 lanes.
 template <int W, int G, int STRIDE>
 __device__ __attribute__((noinline)) bool group_dot(int64_t k0, int64_t k1, int gl,
                                                     const uint32_t *xb, int64_t xs,
                                                     const uint32_t *yb, int64_t ys, const Val<W> *c,
                                                     bool negate_terms, int64_t *slots, Val<W> &out,
                                                     int64_t hint_lo, int64_t hint_hi) {
-    static_assert(G == 16, "group reductions run inside one 16-lane DPP row");
+    static_assert(G == 16 || G == 32 || G == 64, "a group is an aligned part of one wave");
     constexpr int SLOTS = Win<W>::SLOTS;
     auto fx = [&](int64_t k) { return xb + k * xs * (W + 2); };
     auto fy = [&](int64_t k) { return yb + k * ys * (W + 2); };
@@ -329,7 +322,7 @@ __device__ __attribute__((noinline)) bool group_dot(int64_t k0, int64_t k1, int 
         lds_add_product<W, STRIDE>(ls, lo, x, y, negate_terms);
     }
     if (cnz && gl == 0) lds_add_words<W, STRIDE>(ls, (int64_t)c->e - 32 * W - lo, c->m, c->kind < 0);
-    group_finish<W, SLOTS, STRIDE>(ls, gl, used, lo, out);
+    group_finish<W, G, SLOTS, STRIDE>(ls, gl, used, lo, out);
     return true;
 }
 
@@ -1043,6 +1036,27 @@ inline unsigned blocks_for(int64_t n, int t) { return (unsigned)((n + t - 1) / t
         default: throw Error(SDPX_GPU_INVALID, "GPU panels support 128, 256 and 512 bits"); \
     }
 
+// Lanes per exact dot in the latency-bound panel kernels (the rows of one
+// factor column, the rows of a triangular solve): 64 (one wave a dot) unless
+// SDPX_GPU_DOT_LANES selects 16 or 32.  The forward solve over many leaf
+// columns keeps 16-lane groups (four dots a wave).
+namespace {
+int dot_lanes() {
+    static const int g = [] {
+        const char *e = std::getenv("SDPX_GPU_DOT_LANES");
+        const int v = e ? std::atoi(e) : 64;
+        return v == 16 || v == 32 ? v : 64;
+    }();
+    return g;
+}
+}  // namespace
+#define SDPX_DISPATCH_G(...)                                \
+    switch (dot_lanes()) {                                  \
+        case 16: { constexpr int G = 16; __VA_ARGS__; break; } \
+        case 32: { constexpr int G = 32; __VA_ARGS__; break; } \
+        default: { constexpr int G = 64; __VA_ARGS__; break; } \
+    }
+
 bool mp_kernels_support(int limbs) { return limbs == 2 || limbs == 4 || limbs == 8; }
 
 void mp_ew_sub_packed(int limbs, void *S, int64_t ld, const void *P, int64_t n, int lower,
@@ -1100,11 +1114,13 @@ void mp_trsm_base(int limbs, const void *L, int64_t ldl, int64_t n, void *B, int
                   int64_t m, const int32_t *low, const void *lranges, int *fail, hipStream_t st) {
     if (m == 0 || n == 0) return;
     SDPX_DISPATCH_W(limbs, {
-        auto kern = trsm_base_kernel<W, DOT_G>;
-        const size_t lds = panel_lds<W>((const void *)kern, ROW_THREADS);
-        hipLaunchKernelGGL(kern, dim3(blocks_for(m * DOT_G, ROW_THREADS)),
-                           dim3(ROW_THREADS), lds, st, (const uint32_t *)L, ldl, n,
-                           (uint32_t *)B, ldb, m, low, (const int2 *)lranges, fail);
+        SDPX_DISPATCH_G({
+            auto kern = trsm_base_kernel<W, G>;
+            const size_t lds = panel_lds<W>((const void *)kern, ROW_THREADS);
+            hipLaunchKernelGGL(kern, dim3(blocks_for(m * G, ROW_THREADS)), dim3(ROW_THREADS), lds, st,
+                               (const uint32_t *)L, ldl, n, (uint32_t *)B, ldb, m, low, (const int2 *)lranges,
+                               fail);
+        });
     });
     HIP_CHECK(hipGetLastError());
 }
@@ -1140,15 +1156,17 @@ void mp_ldl_dots(int limbs, void *A, void *dinv, const int32_t *first, const int
         return;
     }
     SDPX_DISPATCH_W(limbs, {
-        auto kern = ldl_col_kernel<W, DOT_G>;
-        const size_t lds = panel_lds<W>((const void *)kern, ROW_THREADS);
-        for (int64_t j = -1; j < nmax - 1; j++) {
-            const int64_t rows = j < 0 ? 1 : nmax - j - 1;
-            hipLaunchKernelGGL(kern, dim3(blocks_for(rows * DOT_G, ROW_THREADS), nblocks), dim3(ROW_THREADS),
-                               lds, st, (uint32_t *)A, (uint32_t *)dinv, first, signs, sign_mode,
-                               (const LdlBlock *)blocks, (uint32_t *)scratch, (const uint32_t *)reg_eps,
-                               (const uint32_t *)reg_delta, status, j);
-        }
+        SDPX_DISPATCH_G({
+            auto kern = ldl_col_kernel<W, G>;
+            const size_t lds = panel_lds<W>((const void *)kern, ROW_THREADS);
+            for (int64_t j = -1; j < nmax - 1; j++) {
+                const int64_t rows = j < 0 ? 1 : nmax - j - 1;
+                hipLaunchKernelGGL(kern, dim3(blocks_for(rows * G, ROW_THREADS), nblocks), dim3(ROW_THREADS), lds,
+                                   st, (uint32_t *)A, (uint32_t *)dinv, first, signs, sign_mode,
+                                   (const LdlBlock *)blocks, (uint32_t *)scratch, (const uint32_t *)reg_eps,
+                                   (const uint32_t *)reg_delta, status, j);
+            }
+        });
         hipLaunchKernelGGL(ldl_col_writeout_kernel<W>, dim3(blocks_for(nmax * nmax, 256), nblocks), dim3(256),
                            0, st, (uint32_t *)A, (const LdlBlock *)blocks, (uint32_t *)scratch,
                            (const int *)status);
@@ -1176,11 +1194,12 @@ void mp_trsm_base_n(int limbs, const void *L, int64_t ldl, int64_t n, void *B, i
                     int64_t m, const void *lranges, int *fail, hipStream_t st) {
     if (m == 0 || n == 0) return;
     SDPX_DISPATCH_W(limbs, {
-        auto kern = trsm_base_n_kernel<W, DOT_G>;
-        const size_t lds = panel_lds<W>((const void *)kern, ROW_THREADS);
-        hipLaunchKernelGGL(kern, dim3(blocks_for(m * DOT_G, ROW_THREADS)),
-                           dim3(ROW_THREADS), lds, st, (const uint32_t *)L, ldl, n,
-                           (uint32_t *)B, ldb, m, (const int2 *)lranges, fail);
+        SDPX_DISPATCH_G({
+            auto kern = trsm_base_n_kernel<W, G>;
+            const size_t lds = panel_lds<W>((const void *)kern, ROW_THREADS);
+            hipLaunchKernelGGL(kern, dim3(blocks_for(m * G, ROW_THREADS)), dim3(ROW_THREADS), lds, st,
+                               (const uint32_t *)L, ldl, n, (uint32_t *)B, ldb, m, (const int2 *)lranges, fail);
+        });
     });
     HIP_CHECK(hipGetLastError());
 }
@@ -1196,11 +1215,13 @@ void mp_transpose(int limbs, const void *in, int64_t ldi, int64_t m, int64_t n, 
 
 void mp_potrf(int limbs, void *A, int64_t n, int64_t lda, int lower, int *status, hipStream_t st) {
     SDPX_DISPATCH_W(limbs, {
-        auto kern = potrf_col_kernel<W, DOT_G>;
-        const size_t lds = panel_lds<W>((const void *)kern, ROW_THREADS);
-        for (int64_t j = -1; j < n - 1; j++)
-            hipLaunchKernelGGL(kern, dim3(blocks_for((n - j - 1) * DOT_G, ROW_THREADS)), dim3(ROW_THREADS), lds,
-                               st, (uint32_t *)A, n, lda, lower, j, status);
+        SDPX_DISPATCH_G({
+            auto kern = potrf_col_kernel<W, G>;
+            const size_t lds = panel_lds<W>((const void *)kern, ROW_THREADS);
+            for (int64_t j = -1; j < n - 1; j++)
+                hipLaunchKernelGGL(kern, dim3(blocks_for((n - j - 1) * G, ROW_THREADS)), dim3(ROW_THREADS), lds,
+                                   st, (uint32_t *)A, n, lda, lower, j, status);
+        });
     });
     HIP_CHECK(hipGetLastError());
 }
@@ -1209,10 +1230,12 @@ void mp_trsm_lower(int limbs, const void *A, int64_t n, void *B, int64_t nrhs, i
                    hipStream_t st) {
     if (n == 0 || nrhs == 0) return;
     SDPX_DISPATCH_W(limbs, {
-        auto kern = trsm_lower_kernel<W, DOT_G>;
-        const size_t lds = panel_lds<W>((const void *)kern, ROW_THREADS);
-        hipLaunchKernelGGL(kern, dim3(blocks_for(nrhs * DOT_G, ROW_THREADS)), dim3(ROW_THREADS), lds, st,
-                           (const uint32_t *)A, n, (uint32_t *)B, nrhs, transpose, fail);
+        SDPX_DISPATCH_G({
+            auto kern = trsm_lower_kernel<W, G>;
+            const size_t lds = panel_lds<W>((const void *)kern, ROW_THREADS);
+            hipLaunchKernelGGL(kern, dim3(blocks_for(nrhs * G, ROW_THREADS)), dim3(ROW_THREADS), lds, st,
+                               (const uint32_t *)A, n, (uint32_t *)B, nrhs, transpose, fail);
+        });
     });
     HIP_CHECK(hipGetLastError());
 }
