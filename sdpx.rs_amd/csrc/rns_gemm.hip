@@ -903,59 +903,59 @@ __global__ __launch_bounds__(128) void crt_mfma_kernel(
         // partial limb sum_b R_{32 nb + 8 g + 4 h + b} 256^b is limb 8 nb + 2 g + h
         // of output (mt, o); lane h owns output o = 64 wave + 32 h + col, i.e.
         // tile mt = h: after each tile's MFMAs the two halves trade partials and
-        // the owning half keeps its own and its partner's
-        int64_t full[NBMAX][8];
+        // the owning half carries them into its limbs at once
+        uint32_t lim[NUMAX];
 #pragma unroll
+        for (int u = 0; u < NUMAX; u++) lim[u] = 0;
+        int64_t carry = 0;
+#pragma unroll 1
         for (int mt = 0; mt < 2; mt++) {
             const int o = 64 * wave + 32 * mt + col;
-            v16i acc[NBMAX];
+            const bool mine = h == mt;
+            // one 32-row block of table rows at a time: its partial limbs are
+            // carried before the next block's products start
 #pragma unroll
-            for (int nb = 0; nb < NBMAX; nb++) acc[nb] = v16i{0};
-            for (int ks = 0; ks < nks; ks++) {
-                const uint16_t *xs = sx + (16 * ks + 8 * h) * CRT_XROW + o;
-                uint32_t d[4];
+            for (int nb = 0; nb < NBMAX; nb++) {
+                v16i acc = v16i{0};
+                if (nb < nbl)
+                    for (int ks = 0; ks < nks; ks++) {
+                        const uint16_t *xs = sx + (16 * ks + 8 * h) * CRT_XROW + o;
+                        uint32_t d[4];
 #pragma unroll
-                for (int u = 0; u < 4; u++)
-                    d[u] = (uint32_t)xs[(2 * u) * CRT_XROW] | ((uint32_t)xs[(2 * u + 1) * CRT_XROW] << 16);
-                const v4i fb = v4i{(int)d[0], (int)d[1], (int)d[2], (int)d[3]};
-#pragma unroll
-                for (int nb = 0; nb < NBMAX; nb++)
-                    if (nb < nbl)
-                        acc[nb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
-                            *(const v4i *)(stab + (32 * nb + col) * trow + 32 * ks + 16 * h), fb, acc[nb],
-                            0, 0, 0);
-            }
-#pragma unroll
-            for (int nb = 0; nb < NBMAX; nb++)
+                        for (int u = 0; u < 4; u++)
+                            d[u] = (uint32_t)xs[(2 * u) * CRT_XROW] | ((uint32_t)xs[(2 * u + 1) * CRT_XROW] << 16);
+                        const v4i fb = v4i{(int)d[0], (int)d[1], (int)d[2], (int)d[3]};
+                        acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(
+                            *(const v4i *)(stab + (32 * nb + col) * trow + 32 * ks + 16 * h), fb, acc, 0, 0, 0);
+                    }
 #pragma unroll
                 for (int g = 0; g < 4; g++) {
-                    const int64_t pv = (int64_t)acc[nb][4 * g] + (int64_t)acc[nb][4 * g + 1] * 256 +
-                                       (int64_t)acc[nb][4 * g + 2] * 65536 +
-                                       (int64_t)acc[nb][4 * g + 3] * 16777216;
+                    const int64_t pv = (int64_t)acc[4 * g] + (int64_t)acc[4 * g + 1] * 256 +
+                                       (int64_t)acc[4 * g + 2] * 65536 + (int64_t)acc[4 * g + 3] * 16777216;
                     const int64_t got = __shfl_xor(pv, 32, 64);
-                    if (mt == 0 && h == 0) {  // static indices: no register spill
-                        full[nb][2 * g] = pv;
-                        full[nb][2 * g + 1] = got;
+                    const int64_t p0 = h == 0 ? pv : got, p1 = h == 0 ? got : pv;  // limbs 2g, 2g + 1
+                    const int u = 8 * nb + 2 * g;
+                    if (u < NUMAX) {
+                        const int64_t a = carry + p0;
+                        lim[u] = mine ? (uint32_t)a : lim[u];
+                        carry = mine ? a >> 32 : carry;
                     }
-                    if (mt == 1 && h == 1) {
-                        full[nb][2 * g + 1] = pv;
-                        full[nb][2 * g] = got;
+                    if (u + 1 < NUMAX) {
+                        const int64_t a = carry + p1;
+                        lim[u + 1] = mine ? (uint32_t)a : lim[u + 1];
+                        carry = mine ? a >> 32 : carry;
                     }
                 }
-        }
-        const int64_t i = i0 + 64 * wave + lane;
-        if (i >= m || (upper && i > j)) continue;
-        uint32_t lim[NUMAX];
-        int64_t carry = 0;
-#pragma unroll
-        for (int u = 0; u < NUMAX; u++) {
-            lim[u] = 0;
-            if (u < nu) {
-                const int64_t a = carry + (u < 8 * NBMAX ? full[u / 8][u % 8] : 0);
-                lim[u] = (uint32_t)a;
-                carry = a >> 32;
             }
         }
+#pragma unroll
+        for (int u = 8 * NBMAX; u < NUMAX; u++) {  // the carry above the partial limbs
+            const int64_t a = carry;
+            lim[u] = (uint32_t)a;
+            carry = a >> 32;
+        }
+        const int64_t i = i0 + 64 * wave + lane;
+        const bool active = i < m && !(upper && i > j);
         // k = round(V / M) from the top three limbs; S = V - k M
         uint32_t l2 = 0, l1 = 0, l0 = 0;
 #pragma unroll
@@ -987,61 +987,53 @@ __global__ __launch_bounds__(128) void crt_mfma_kernel(
                     c = t >> 32;
                 }
         }
-        uint32_t *rec = out + (size_t)(packed ? j * (j + 1) / 2 + i : i + j * m) * (2 * n + 2);
-        int top = -1;
+        int top = -1, low = NUMAX;
         uint32_t tw = 0;
 #pragma unroll
-        for (int u = 0; u < NUMAX; u++)
+        for (int u = 0; u < NUMAX; u++) {
             if (u < nu && lim[u] != 0) {
                 top = u;
                 tw = lim[u];
             }
+            const int d = NUMAX - 1 - u;
+            if (d < nu && lim[d] != 0) low = d;
+        }
+        // the magnitude's words, [word][thread], in the x-plane area (every
+        // wave has finished its fragment reads at the barrier)
+        __syncthreads();
+        uint32_t *sl = (uint32_t *)sx;
+#pragma unroll
+        for (int u = 0; u < NUMAX; u++) sl[u * 128 + tid] = u < nu ? lim[u] : 0u;
+        if (!active) continue;
+        uint32_t *rec = out + (size_t)(packed ? j * (j + 1) / 2 + i : i + j * m) * (2 * n + 2);
         if (top < 0) {
             for (int w = 0; w < 2 * n; w++) rec[w] = 0;
             rec[2 * n] = KIND_ZERO;
             rec[2 * n + 1] = 0;
             continue;
         }
+        auto word = [&](int u) -> uint32_t { return u >= 0 && u < NUMAX ? sl[u * 128 + tid] : 0u; };
         const int bits = 32 * top + 32 - __builtin_clz(tw);
         int shift = bits - P;
         bool roundb = false, sticky = false;
         if (shift > 0) {
-            const int rb = shift - 1;
-#pragma unroll
-            for (int u = 0; u < NUMAX; u++) {
-                const int base = 32 * u;
-                if (base + 32 <= rb) sticky |= lim[u] != 0;
-                else if (base < rb) sticky |= (lim[u] & ((1u << (rb - base)) - 1u)) != 0;
-                if (base <= rb && rb < base + 32) roundb = (lim[u] >> (rb - base)) & 1u;
-            }
+            const int rb = shift - 1, qr = rb >> 5, br2 = rb & 31;
+            const uint32_t wr = word(qr);
+            roundb = (wr >> br2) & 1u;
+            sticky = low < qr || (br2 && (wr & ((1u << br2) - 1u)) != 0);
         }
-        // mantissa words: bits [shift, shift + P)
+        // mantissa words: bits [shift, shift + P) of the magnitude
         uint32_t mant[CRT_MAXW];
-        if (shift >= 0) {
-            const int qs = shift >> 5, rs = shift & 31;
-#pragma unroll
-            for (int sft = 32; sft >= 1; sft >>= 1)
-                if (qs & sft) {
-#pragma unroll
-                    for (int u = 0; u < NUMAX; u++) lim[u] = (u + sft < NUMAX) ? lim[u + sft] : 0u;
-                }
+        {
+            const int q = shift >> 5, rs = shift & 31;  // arithmetic: floor for negative shifts
 #pragma unroll
             for (int w = 0; w < CRT_MAXW; w++) {
-                const uint32_t lo = w < NUMAX ? lim[w] : 0u, hi = w + 1 < NUMAX ? lim[w + 1] : 0u;
-                mant[w] = __builtin_amdgcn_alignbit(hi, lo, rs);
-            }
-        } else {
-            const int ls = -shift, qs = ls >> 5, rs = ls & 31;
-#pragma unroll
-            for (int sft = 32; sft >= 1; sft >>= 1)
-                if (qs & sft) {
-#pragma unroll
-                    for (int u = NUMAX - 1; u >= 0; u--) lim[u] = (u - sft >= 0) ? lim[u - sft] : 0u;
+                if (w < 2 * n) {
+                    const uint32_t lo = word(q + w), hi = word(q + w + 1);
+                    mant[w] = __builtin_amdgcn_alignbit(hi, lo, rs);
+                } else {
+                    mant[w] = 0;
                 }
-#pragma unroll
-            for (int w = 0; w < CRT_MAXW; w++) {
-                const uint32_t cur = w < NUMAX ? lim[w] : 0u, prev = (w >= 1 && w - 1 < NUMAX) ? lim[w - 1] : 0u;
-                mant[w] = rs ? ((cur << rs) | (prev >> (32 - rs))) : cur;
             }
         }
         if (roundb && (sticky || (mant[0] & 1u))) {
@@ -1592,7 +1584,9 @@ int rns_gemm_device(const RnsGemmArgs &g) {
         const size_t tb = plan->crt_table.size();
         const uint8_t *d_ct = ptab.crt2;
         const int trow = 2 * lpad + 16;
-        const size_t lds = (size_t)npad * trow + (size_t)lpad * CRT_XROW * 2;
+        const int numax = nu <= 20 && nbl <= 3 ? 20 : (nu <= 32 && nbl <= 4 ? 32 : 40);
+        // x planes, then (reused) the magnitudes' words, [word][thread]
+        const size_t lds = (size_t)npad * trow + std::max<size_t>((size_t)lpad * CRT_XROW * 2, (size_t)numax * 128 * 4);
         const int64_t items = (int64_t)ntiles * TILE;
         const unsigned grid = (unsigned)std::min<int64_t>(items, 4096);
         auto launch = [&](auto kern) {
@@ -1604,8 +1598,10 @@ int rns_gemm_device(const RnsGemmArgs &g) {
                                plan->crt_nt, npad, (const uint32_t *)(d_ct + tb), nu, plan->m_top,
                                (const int4 *)scan_a, (const int4 *)scan_b, m, nn, up, pk, n, outw);
         };
-        if (nu <= 20 && nbl <= 3)
+        if (numax == 20)
             launch(crt_mfma_kernel<20, 3>);
+        else if (numax == 32)
+            launch(crt_mfma_kernel<32, 4>);
         else
             launch(crt_mfma_kernel<40, 5>);
     } else if (J <= 32)
