@@ -255,10 +255,10 @@ __device__ __forceinline__ void group_finish(lds_i64 *ls, int gl, int used, int6
 }
 
 // Terms x_k = xb[k * xs], y_k = yb[k * ys] (record strides): every panel
-// kernel's operands are affine in k, so one out-of-line copy per width and
-// slot stride serves all call sites.  Groups are 16, 32 or 64 lanes.
+// kernel's operands are affine in k.  Inlined at each call site, so the
+// operands and the result stay in registers.  Groups are 16, 32 or 64 lanes.
 template <int W, int G, int STRIDE>
-__device__ __attribute__((noinline)) bool group_dot(int64_t k0, int64_t k1, int gl,
+__device__ __forceinline__ bool group_dot(int64_t k0, int64_t k1, int gl,
                                                     const uint32_t *xb, int64_t xs,
                                                     const uint32_t *yb, int64_t ys, const Val<W> *c,
                                                     bool negate_terms, int64_t *slots, Val<W> &out,

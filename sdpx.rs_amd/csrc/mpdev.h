@@ -18,9 +18,10 @@
 namespace sdpx {
 namespace mpd {
 
-// The larger operations are kept out of line: the panel kernels call them
-// from several sites, and inlined copies would overflow the instruction cache.
-#define SDPX_MP_OUTLINE __host__ __device__ __attribute__((noinline))
+// The larger operations are inlined into every kernel: an out-of-line call
+// passes its Val operands through the stack (private scratch memory), and
+// those round trips cost microseconds per call on the GPU.
+#define SDPX_MP_OUTLINE __host__ __device__ inline __attribute__((always_inline))
 
 constexpr int KIND_ZERO = 2, KIND_REG = 3, KIND_INF = 1, KIND_NAN = 0;
 constexpr int64_t EMAX = (1LL << 30) - 1, EMIN = 1 - (1LL << 30);
@@ -489,6 +490,25 @@ struct DotWindow {
     }
 };
 
+// floor(num / vt) and its remainder for num < 2^64 and vt in [2^31, 2^32)
+// (a schoolbook quotient-digit estimate).  The binary64 estimate
+// num * RN(1/vt) is within 3.3 2^-53 relative of num / vt, so below 2^33 it
+// is off by less than one: two below it is a lower bound, and at most four
+// steps of r -= vt make the pair exact.  Replaces a 64-bit integer division
+// (a long software sequence on the GPU) on the critical path of every
+// reciprocal.
+__host__ __device__ inline uint64_t qdigit(uint64_t num, uint64_t vt, double rcp, uint64_t &rem) {
+    uint64_t q = (uint64_t)((double)num * rcp);
+    q = q > 2 ? q - 2 : 0;
+    uint64_t r = num - q * vt;
+    while (r >= vt) {
+        r -= vt;
+        q++;
+    }
+    rem = r;
+    return q;
+}
+
 // RN(1 / d): Q = floor(2^(2P+1) / D) for the P-bit mantissa D (top bit set)
 // and the remainder's sticky bit, then one rounding of (Q << 1 | sticky).
 // Q comes from schoolbook division in base 2^32 (Knuth, TAOCP vol. 2,
@@ -510,12 +530,13 @@ SDPX_MP_OUTLINE void recip(const Val<W> &d, Val<W> &out) {
     u[2 * W] = 2;
     const uint32_t *v = d.m;
     const uint64_t vt = v[W - 1], vs = W >= 2 ? v[W - 2] : 0;
+    const double rcp = 1.0 / (double)vt;
     uint32_t q[QW + 1];
     q[QW] = 0;
 #pragma unroll
     for (int j = QW - 1; j >= 0; j--) {
         const uint64_t num = ((uint64_t)u[j + W] << 32) | u[j + W - 1];
-        uint64_t qh = num / vt, rh = num % vt;
+        uint64_t rh, qh = qdigit(num, vt, rcp, rh);
         const uint64_t ulow = (W >= 2 && j + W - 2 >= 0) ? u[j + W - 2 >= 0 ? j + W - 2 : 0] : 0;
         while (qh >> 32 || qh * vs > ((rh << 32) | ulow)) {
             qh--;
@@ -590,12 +611,13 @@ SDPX_MP_OUTLINE void div(const Val<W> &x, const Val<W> &d, Val<W> &out) {
     }
     const uint32_t *v = d.m;
     const uint64_t vt = v[W - 1], vs = W >= 2 ? v[W - 2] : 0;
+    const double rcp = 1.0 / (double)vt;
     uint32_t q[QW + 1];
     q[QW] = 0;
 #pragma unroll
     for (int j = QW - 1; j >= 0; j--) {
         const uint64_t num = ((uint64_t)u[j + W] << 32) | u[j + W - 1];
-        uint64_t qh = num / vt, rh = num % vt;
+        uint64_t rh, qh = qdigit(num, vt, rcp, rh);
         const uint64_t ulow = u[j + W - 2 >= 0 ? j + W - 2 : 0];
         while (qh >> 32 || qh * vs > ((rh << 32) | (j + W - 2 >= 0 ? ulow : 0))) {
             qh--;
