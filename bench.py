@@ -24,6 +24,17 @@ FP64_PEAK = 78.6e12         # MI355X dense FP64 MFMA
 HBM_PEAK = 8.0e12           # bytes/s
 
 
+def product_traffic(workload):
+    """HBM bytes per product-kernel launch of this workload from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counters scaled by a copy
+    kernel of known size), or None."""
+    try:
+        d = json.load(open(os.path.join(ROOT, "profiles", "product_traffic.json")))[workload]
+        return d["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def progress(msg):
     """A progress line on stderr (the JSON result stays alone on stdout)."""
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
@@ -85,8 +96,10 @@ def rns_gemm_workload(args):
                                 "crt": stages[3]},
                    "mp_macs_per_s": m * n * k / dt},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": INT8_PEAK_OPS / 1e12,
-                     "unit": "TOP/s (INT8)", "frac": achieved / INT8_PEAK_OPS, "traffic": None,
-                     "kernel": "product_kernel (all moduli)", "int8_ops_per_step": int8_ops},
+                     "unit": "TOP/s (INT8)", "frac": achieved / INT8_PEAK_OPS,
+                     "traffic": product_traffic("rns_gemm"),
+                     "kernel": "product_kernel (all moduli)", "int8_ops_per_step": int8_ops,
+                     "share_of_step": product_s / dt},
     }
     if args.cpu_baseline:
         result["cpu_baseline"] = cpu_baseline_gemm(a, b, m, n, k, nl)
@@ -194,9 +207,13 @@ def sampled_workload(args):
                    "mp256_over_fp64": dt / fdt,
                    "host_entry_s_per_iter": host_dt},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": INT8_PEAK_OPS / 1e12,
-                     "unit": "TOP/s (INT8)", "frac": achieved / INT8_PEAK_OPS, "traffic": None,
+                     "unit": "TOP/s (INT8)", "frac": achieved / INT8_PEAK_OPS,
+                     "traffic": product_traffic("sampled") if t == 2000 else None,
                      "kernel": "product_kernel (INT8 MFMA residue products)",
-                     "int8_ops_per_step": tot["int8_ops"]},
+                     "int8_ops_per_step": tot["int8_ops"], "share_of_step": product_s / dt,
+                     "note": "the largest share of the step is the latency-bound exact-dot panel "
+                             "kernels (ldl_col_kernel, trsm_base_kernel), which have no MFMA or "
+                             "HBM roofline"},
     }
     if args.cpu_baseline and t <= 2000:  # larger borders: minutes of CPU time
         sys.path[:0] = [os.path.join(ROOT, "tests"), os.path.join(ROOT, "oracle")]
