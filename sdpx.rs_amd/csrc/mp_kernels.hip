@@ -251,7 +251,65 @@ __device__ __forceinline__ void group_finish(lds_i64 *ls, int gl, int used, int6
         const int k = lo + i;
         g[i + 1] = k >= 0 ? *((lds_u32 *)(g0 + (k / WROW) * STRIDE) + (k % WROW)) : 0u;
     }
-    round_words<W, GW + 1>(neg, g, base + 32 * (int64_t)(lo - 1), out);
+    round_norm<W, GW + 1>(neg, g, base + 32 * (int64_t)(lo - 1), out);  // g[GW] = w[top] != 0
+}
+
+// group_finish for a whole-wave group (G = 64, SLOTS <= 64) with the
+// carries resolved across lanes: lane s sums slot s over the wave, then each
+// lane below the top slot keeps its low word and passes the rest to the lane
+// above, until no lane has a carry left (a ripple through every slot at
+// most; usually two or three steps).  The top slot keeps the sign.  A
+// negative total is negated across lanes (the +1 stops at the lowest
+// non-zero word), the top and lowest non-zero words come from one ballot,
+// and the words under the top are read with uniform lane indices, so the
+// rounding runs once on scalar values.  Same words, sign and rounding as
+// group_finish.
+template <int W, int SLOTS, int STRIDE>
+__device__ __forceinline__ void wave_finish(lds_i64 *ls, int gl, int used, int64_t base, Val<W> &out) {
+    static_assert(SLOTS <= 64, "one slot a lane");
+    lds_i64 *g0 = ls - gl;  // slot 0 of lane 0
+    int64_t T = 0;
+    if (gl < used) {
+        const lds_i64x2 *row = (const lds_i64x2 *)(g0 + gl * STRIDE);
+#pragma unroll
+        for (int q = 0; q < 32; q++) {
+            const i64x2 x = row[q];
+            T += x.x + x.y;
+        }
+    }
+    const bool below_top = gl < SLOTS - 1;
+    for (int it = 0; it < SLOTS; it++) {
+        const int64_t c = below_top ? (T >> 32) : 0;
+        if (__ballot(c != 0) == 0) break;
+        if (below_top) T &= 0xFFFFFFFFll;
+        const int64_t cin = __shfl_up(c, 1, 64);
+        if (gl > 0) T += cin;
+    }
+    const int64_t ttop = __shfl(T, SLOTS - 1, 64);
+    const bool neg = ttop < 0;
+    uint32_t w = gl < SLOTS ? (uint32_t)(uint64_t)T : 0u;
+    if (neg) {
+        const uint64_t nzm = __ballot(w != 0);  // non-zero: the total is not 0
+        const int f = __builtin_ctzll(nzm);
+        w = gl < f ? 0u : (gl == f ? (uint32_t)(0u - w) : ~w);
+        if (gl >= SLOTS) w = 0u;
+    }
+    const uint64_t nz = __ballot(w != 0);
+    if (nz == 0) {
+        set_zero(out);
+        return;
+    }
+    constexpr int GW = W + 2;
+    const int top = 63 - __builtin_clzll(nz), low = __builtin_ctzll(nz);
+    const int lo = top - GW + 1;
+    uint32_t g[GW + 1];
+    g[0] = low < lo ? 1u : 0u;  // one sticky bit below the gathered words
+#pragma unroll
+    for (int i = 0; i < GW; i++) {
+        const int k = lo + i;
+        g[i + 1] = k >= 0 ? (uint32_t)__builtin_amdgcn_readlane((int)w, k) : 0u;
+    }
+    round_norm<W, GW + 1>(neg, g, base + 32 * (int64_t)(lo - 1), out);  // g[GW] = w[top] != 0
 }
 
 // Terms x_k = xb[k * xs], y_k = yb[k * ys] (record strides): every panel
@@ -322,7 +380,10 @@ __device__ __forceinline__ bool group_dot(int64_t k0, int64_t k1, int gl,
         lds_add_product<W, STRIDE>(ls, lo, x, y, negate_terms);
     }
     if (cnz && gl == 0) lds_add_words<W, STRIDE>(ls, (int64_t)c->e - 32 * W - lo, c->m, c->kind < 0);
-    group_finish<W, G, SLOTS, STRIDE>(ls, gl, used, lo, out);
+    if constexpr (G == 64 && SLOTS <= 64)
+        wave_finish<W, SLOTS, STRIDE>(ls, gl, used, lo, out);
+    else
+        group_finish<W, G, SLOTS, STRIDE>(ls, gl, used, lo, out);
     return true;
 }
 
@@ -712,8 +773,8 @@ struct LdlCol {
 };
 
 // DenseLeaf::pivot (arrow.rs:618-634) of column j from the exact dot of
-// its r and w rows (one lane group; lane 0 writes d and 1/d).  Returns the
-// status code (0 ok, 1 failed pivot, 2 window overflow) in every lane.
+// its r and w rows (one lane group; lane 0 writes d).  Returns the status
+// code (0 ok, 1 failed pivot, 2 window overflow) in every lane.
 template <int W, int G>
 __device__ int ldl_col_pivot(const LdlCol<W> &c, int64_t j, int gl, int64_t hl, int64_t hh,
                              int64_t *slots, const int8_t *signs, int sign_mode, int64_t sign_off,
@@ -745,33 +806,37 @@ __device__ int ldl_col_pivot(const LdlCol<W> &c, int64_t j, int gl, int64_t hl, 
         }
     }
     if (is_zero(d)) return 1;
-    if (gl == 0) {
-        store(rec_at<W>(c.dstore, j), d);
-        Val<W> di;
-        recip(d, di);
-        store(rec_at<W>(c.dv, j), di);
-    }
+    if (gl == 0) store(rec_at<W>(c.dstore, j), d);  // RN(1/d) is formed by the next launch
     return 0;
 }
 
+// Launch j: the first wave of a workgroup holds the row lane groups, the
+// second forms RN(1 / d_j) meanwhile (the reciprocal the rows' entries need
+// only after their dots), hands it over through LDS and, in workgroup 0 of
+// the block, stores it.  The status is read once per workgroup, so both
+// waves take the same path to the barrier.
 template <int W, int G>
-__global__ __launch_bounds__(ROW_THREADS) void ldl_col_kernel(
+__global__ __launch_bounds__(2 * ROW_THREADS) void ldl_col_kernel(
     uint32_t *A, uint32_t *dinv, const int32_t *first_all, const int8_t *signs, int sign_mode,
     const LdlBlock *blocks, uint32_t *scratch, const uint32_t *reg_eps, const uint32_t *reg_delta,
     int *status, int64_t j) {
     extern __shared__ int64_t slots_sh[];
-    if (status[0] != 0) return;
+    __shared__ uint32_t dinv_sh[W + 2];
+    __shared__ int stop;
+    if (threadIdx.x == 0) stop = status[0];
+    __syncthreads();
+    if (stop != 0) return;  // the whole workgroup
     const LdlBlock B = blocks[blockIdx.y];
-    const int gl = threadIdx.x & (G - 1);
-    const int64_t i = j + 1 + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
-    if (i >= B.n || (j < 0 && i > 0)) return;  // whole groups
-    int64_t *slots = slots_sh + threadIdx.x;
     const LdlCol<W> c(A, dinv, first_all, scratch, B);
+    const bool recip_wave = threadIdx.x >= ROW_THREADS;
+    const int gl = threadIdx.x & (G - 1);
+    const int64_t i = j + 1 + ((int64_t)blockIdx.x * ROW_THREADS + (threadIdx.x & (ROW_THREADS - 1))) / G;
+    int64_t *slots = slots_sh + (threadIdx.x & (ROW_THREADS - 1));
     auto fail = [&](int code) {
         if (gl == 0) atomicMax(status, code);
     };
     if (j < 0) {  // pivot 0: an empty dot
-        if (i == 0) {
+        if (!recip_wave && i == 0) {
             if (gl == 0) c.rng[0] = make_int4(INT32_MAX, INT32_MIN, INT32_MAX, INT32_MIN);
             const int code = ldl_col_pivot<W, G>(c, 0, gl, INT64_MAX, INT64_MIN, slots, signs, sign_mode,
                                                  B.sign_off, reg_eps, reg_delta, status);
@@ -779,27 +844,52 @@ __global__ __launch_bounds__(ROW_THREADS) void ldl_col_kernel(
         }
         return;
     }
+    if (j >= B.n) return;  // the whole workgroup: this block has no column j
+    if (recip_wave) {
+        if (threadIdx.x == ROW_THREADS) {
+            Val<W> dj, di;
+            load(rec_at<W>((const uint32_t *)c.dstore, j), dj);
+            recip(dj, di);
+            store(dinv_sh, di);
+            if (blockIdx.x == 0) store(rec_at<W>(c.dv, j), di);
+        }
+        __syncthreads();
+        return;
+    }
+    const bool active = i < B.n;
     // row i's ranges so far (column 0 starts them)
-    int4 ri = j == 0 ? make_int4(INT32_MAX, INT32_MIN, INT32_MAX, INT32_MIN) : c.rng[i];
-    Val<W> dj, dij, v, wv;
-    load(rec_at<W>((const uint32_t *)c.dstore, j), dj);
-    load(rec_at<W>((const uint32_t *)c.dv, j), dij);
-    if (c.first[i] > j) {
+    int4 ri = make_int4(INT32_MAX, INT32_MIN, INT32_MAX, INT32_MIN);
+    Val<W> dj, dij, v, wv, diff;
+    bool dot_ok = true, zero_row = true;
+    if (active) {
+        if (j > 0) ri = c.rng[i];
+        load(rec_at<W>((const uint32_t *)c.dstore, j), dj);
+        zero_row = c.first[i] > j;
+        if (!zero_row) {
+            const int64_t lo = max(c.first[i], c.first[j]);
+            const int4 rj = c.rng[j];
+            int64_t hl, hh;
+            range_hint<W>(ri.x, ri.y, rj.z, rj.w, hl, hh);
+            Val<W> t, aij;
+            dot_ok = group_dot<W, G, ROW_THREADS>(lo, j, gl, rec_at<W>((const uint32_t *)c.r, LdlCol<W>::tri(i)),
+                                                  1, rec_at<W>((const uint32_t *)c.w, LdlCol<W>::tri(j)), 1,
+                                                  nullptr, false, slots, t, hl, hh);
+            if (dot_ok) {
+                load(rec_at<W>(c.a, i + j * c.ld), aij);
+                add_signed(aij, t, true, diff);
+            }
+        }
+    }
+    __syncthreads();  // RN(1 / d_j) in dinv_sh
+    if (!active) return;
+    if (!dot_ok) {
+        fail(2);
+        return;
+    }
+    if (zero_row) {
         set_zero(v);
     } else {
-        const int64_t lo = max(c.first[i], c.first[j]);
-        const int4 rj = c.rng[j];
-        int64_t hl, hh;
-        range_hint<W>(ri.x, ri.y, rj.z, rj.w, hl, hh);
-        Val<W> t, aij, diff;
-        if (!group_dot<W, G, ROW_THREADS>(lo, j, gl, rec_at<W>((const uint32_t *)c.r, LdlCol<W>::tri(i)), 1,
-                                          rec_at<W>((const uint32_t *)c.w, LdlCol<W>::tri(j)), 1, nullptr,
-                                          false, slots, t, hl, hh)) {
-            fail(2);
-            return;
-        }
-        load(rec_at<W>(c.a, i + j * c.ld), aij);
-        add_signed(aij, t, true, diff);
+        load(dinv_sh, dij);
         mul(diff, dij, v);
     }
     mul(v, dj, wv);
@@ -826,11 +916,18 @@ __global__ __launch_bounds__(ROW_THREADS) void ldl_col_kernel(
 
 // the unit lower factor of every block (after the last column launch)
 template <int W>
-__global__ void ldl_col_writeout_kernel(uint32_t *A, const LdlBlock *blocks, uint32_t *scratch,
+__global__ void ldl_col_writeout_kernel(uint32_t *A, uint32_t *dinv, const LdlBlock *blocks, uint32_t *scratch,
                                         const int *status) {
     if (status[0] != 0) return;
     const LdlBlock B = blocks[blockIdx.y];
     const int64_t n = B.n, e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e == 0 && n > 0) {  // RN(1 / d) of the last pivot (no column launch follows it)
+        const uint32_t *dstore = scratch + B.scratch_off * (W + 2) + n * (n + 1) * (W + 2);
+        Val<W> d, di;
+        load(rec_at<W>(dstore, n - 1), d);
+        recip(d, di);
+        store(rec_at<W>(dinv + B.dinv_off * (W + 2), n - 1), di);
+    }
     if (e >= n * n) return;
     const int64_t i = e % n, j = e / n;
     if (i < j) return;
@@ -1161,14 +1258,14 @@ void mp_ldl_dots(int limbs, void *A, void *dinv, const int32_t *first, const int
             const size_t lds = panel_lds<W>((const void *)kern, ROW_THREADS);
             for (int64_t j = -1; j < nmax - 1; j++) {
                 const int64_t rows = j < 0 ? 1 : nmax - j - 1;
-                hipLaunchKernelGGL(kern, dim3(blocks_for(rows * G, ROW_THREADS), nblocks), dim3(ROW_THREADS), lds,
+                hipLaunchKernelGGL(kern, dim3(blocks_for(rows * G, ROW_THREADS), nblocks), dim3(2 * ROW_THREADS), lds,
                                    st, (uint32_t *)A, (uint32_t *)dinv, first, signs, sign_mode,
                                    (const LdlBlock *)blocks, (uint32_t *)scratch, (const uint32_t *)reg_eps,
                                    (const uint32_t *)reg_delta, status, j);
             }
         });
         hipLaunchKernelGGL(ldl_col_writeout_kernel<W>, dim3(blocks_for(nmax * nmax, 256), nblocks), dim3(256),
-                           0, st, (uint32_t *)A, (const LdlBlock *)blocks, (uint32_t *)scratch,
+                           0, st, (uint32_t *)A, (uint32_t *)dinv, (const LdlBlock *)blocks, (uint32_t *)scratch,
                            (const int *)status);
     });
     HIP_CHECK(hipGetLastError());

@@ -182,6 +182,60 @@ __host__ __device__ inline void round_words(bool neg, const uint32_t (&mag)[L], 
     out.e = (int32_t)e;
 }
 
+// round_words for a buffer whose top word is non-zero (mag[L - 1] != 0,
+// L > W): the rounding point then lies in word B = L - 1 - W at the bit
+// below the top word's length, so the mantissa is one funnel shift of the
+// top W + 1 words and no run-time word shift is needed.  Same result as
+// round_words on the same buffer.
+template <int W, int L>
+__host__ __device__ inline void round_norm(bool neg, const uint32_t (&mag)[L], int64_t scale,
+                                           Val<W> &out) {
+    static_assert(L > W, "the buffer must extend below the mantissa");
+    constexpr int B = L - 1 - W;
+    const int bl = bitlen32(mag[L - 1]);  // 1 .. 32
+    const int P = 32 * W;
+    int shift = 32 * B + bl;
+    if (bl == 32) {
+#pragma unroll
+        for (int u = 0; u < W; u++) out.m[u] = mag[B + 1 + u];
+    } else {
+#pragma unroll
+        for (int u = 0; u < W; u++) out.m[u] = funnel(mag[B + 1 + u], mag[B + u], bl);
+    }
+    const uint32_t rw = mag[B];
+    const bool round = (rw >> (bl - 1)) & 1u;
+    bool sticky = (rw & ((1u << (bl - 1)) - 1u)) != 0;
+#pragma unroll
+    for (int i = 0; i < B; i++) sticky |= mag[i] != 0;
+    if (round && (sticky || (out.m[0] & 1))) {
+        uint32_t c = 1;
+#pragma unroll
+        for (int u = 0; u < W; u++) {
+            const uint32_t v = out.m[u] + c;
+            c = (c && v == 0) ? 1 : 0;
+            out.m[u] = v;
+        }
+        if (c) {
+            out.m[W - 1] = 0x80000000u;
+            shift++;
+        }
+    }
+    const int64_t e = scale + shift + P;
+    if (e > EMAX) {
+#pragma unroll
+        for (int u = 0; u < W; u++) out.m[u] = 0;
+        out.kind = neg ? -KIND_INF : KIND_INF;
+        out.e = 0;
+        return;
+    }
+    if (e < EMIN) {
+        set_zero(out);
+        return;
+    }
+    out.kind = neg ? -KIND_REG : KIND_REG;
+    out.e = (int32_t)e;
+}
+
 // RN(a * b)
 template <int W>
 SDPX_MP_OUTLINE void mul(const Val<W> &a, const Val<W> &b, Val<W> &out) {
@@ -203,24 +257,20 @@ SDPX_MP_OUTLINE void mul(const Val<W> &a, const Val<W> &b, Val<W> &out) {
         }
         p[i + W] = (uint32_t)carry;
     }
-    round_words<W, 2 * W>((a.kind < 0) != (b.kind < 0), p,
-                          (int64_t)a.e + b.e - 2 * 32 * (int64_t)W, out);
+    // normalized mantissas put the product's top bit in its top word
+    if (p[2 * W - 1])
+        round_norm<W, 2 * W>((a.kind < 0) != (b.kind < 0), p, (int64_t)a.e + b.e - 2 * 32 * (int64_t)W, out);
+    else
+        round_words<W, 2 * W>((a.kind < 0) != (b.kind < 0), p,
+                              (int64_t)a.e + b.e - 2 * 32 * (int64_t)W, out);
 }
 
-// RN(a + s*b) with s = +1 (add) or -1 (sub).
+// RN(a + s*b) for non-zero a, b: the exact sum over 2W + 2 words, then one
+// rounding (any mantissas; the general path of add_signed).
 template <int W>
-SDPX_MP_OUTLINE void add_signed(const Val<W> &a, const Val<W> &b, bool negate_b,
-                                           Val<W> &out) {
+__host__ __device__ inline void add_signed_exact(const Val<W> &a, const Val<W> &b, bool negate_b,
+                                                 Val<W> &out) {
     const bool na = a.kind < 0, nb = (b.kind < 0) != negate_b;
-    if (is_zero(b)) {
-        out = a;
-        return;
-    }
-    if (is_zero(a)) {
-        out = b;
-        if (negate_b && is_regular(out)) out.kind = -out.kind;
-        return;
-    }
     const bool swap = b.e > a.e;
     Val<W> xv, yv;
 #pragma unroll
@@ -284,6 +334,114 @@ SDPX_MP_OUTLINE void add_signed(const Val<W> &a, const Val<W> &b, bool negate_b,
         borrow = (d >> 63) & 1;
     }
     round_words<W, L>(cmp > 0 ? nx : ny, D, (int64_t)y->e - P, out);
+}
+
+// RN(a + s*b) with s = +1 (add) or -1 (sub).  With both mantissas
+// normalized (top bit set) the sum is formed at x's scale over W + 3 words,
+// x = the operand of larger exponent shifted up 64 bits and y shifted down
+// by the exponent gap, the bits y loses folded into one sticky bit at the
+// bottom: a lost bit needs a gap above 64, which leaves the result's top bit
+// at least P + 62 bits above the bottom, so the sticky bit only breaks ties
+// and the rounding is the exact sum's.  A difference that loses at most one
+// bit at the top is rounded from the top one or two words (round_norm), a
+// deeper cancellation (exponent gap 0 or 1, exact) by round_words.
+template <int W>
+SDPX_MP_OUTLINE void add_signed(const Val<W> &a, const Val<W> &b, bool negate_b,
+                                Val<W> &out) {
+    if (is_zero(b)) {
+        out = a;
+        return;
+    }
+    if (is_zero(a)) {
+        out = b;
+        if (negate_b && is_regular(out)) out.kind = -out.kind;
+        return;
+    }
+    if (!(a.m[W - 1] >> 31) || !(b.m[W - 1] >> 31)) {
+        add_signed_exact(a, b, negate_b, out);
+        return;
+    }
+    const bool na = a.kind < 0, nb = (b.kind < 0) != negate_b;
+    const bool swap = b.e > a.e;
+    const Val<W> &x = swap ? b : a, &y = swap ? a : b;
+    const bool nx = swap ? nb : na, ny = swap ? na : nb;
+    constexpr int P = 32 * W;
+    const int64_t gap = (int64_t)x.e - y.e;
+    if (gap >= P + 2) {
+        out = x;
+        out.kind = nx ? -KIND_REG : KIND_REG;
+        return;
+    }
+    // X = Mx << 64, Y = (My << 64) >> gap over W + 3 words (top word spare)
+    constexpr int L = W + 3;
+    uint32_t X[L], Y[L];
+#pragma unroll
+    for (int i = 0; i < L; i++) {
+        X[i] = i >= 2 && i < W + 2 ? x.m[i - 2] : 0u;
+        Y[i] = i >= 2 && i < W + 2 ? y.m[i - 2] : 0u;
+    }
+    const int wq = (int)(gap >> 5), bq = (int)(gap & 31);
+    bool sticky = false;
+#pragma unroll
+    for (int i = 0; i < L; i++) {
+        if (i < wq) sticky |= Y[i] != 0;
+        if (i == wq && bq) sticky |= (Y[i] & ((1u << bq) - 1u)) != 0;
+    }
+    shr_words<L>(Y, wq);
+    if (bq) {
+#pragma unroll
+        for (int i = 0; i < L; i++) Y[i] = funnel(i + 1 < L ? Y[i + 1] : 0u, Y[i], bq);
+    }
+    bool neg = nx;
+    if (nx == ny) {
+        uint64_t c = 0;
+#pragma unroll
+        for (int i = 0; i < L; i++) {
+            const uint64_t t = (uint64_t)X[i] + Y[i] + c;
+            X[i] = (uint32_t)t;
+            c = t >> 32;
+        }
+    } else {
+        if (gap == 0) {  // exact: the larger mantissa first
+            int cmp = 0;
+#pragma unroll
+            for (int i = L - 1; i >= 0; i--)
+                if (cmp == 0 && X[i] != Y[i]) cmp = X[i] > Y[i] ? 1 : -1;
+            if (cmp == 0) {
+                set_zero(out);
+                return;
+            }
+            if (cmp < 0) {
+#pragma unroll
+                for (int i = 0; i < L; i++) {
+                    const uint32_t t = X[i];
+                    X[i] = Y[i];
+                    Y[i] = t;
+                }
+                neg = ny;
+            }
+        }
+        // X - Y - sticky (X > Y + 1 whenever a bit was lost)
+        int64_t br = sticky ? 1 : 0;
+#pragma unroll
+        for (int i = 0; i < L; i++) {
+            const int64_t t = (int64_t)X[i] - Y[i] - br;
+            X[i] = (uint32_t)t;
+            br = t < 0 ? 1 : 0;
+        }
+    }
+    if (sticky) X[0] |= 1u;
+    const int64_t scale = (int64_t)x.e - P - 64;
+    if (X[L - 1]) {
+        round_norm<W, L>(neg, X, scale, out);
+    } else if (X[L - 2]) {
+        uint32_t X2[L - 1];
+#pragma unroll
+        for (int i = 0; i < L - 1; i++) X2[i] = X[i];
+        round_norm<W, L - 1>(neg, X2, scale, out);
+    } else {
+        round_words<W, L>(neg, X, scale, out);
+    }
 }
 
 // RN(c - a*b) with one rounding (the fused multiply-add of the triangular
@@ -581,7 +739,14 @@ SDPX_MP_OUTLINE void recip(const Val<W> &d, Val<W> &out) {
         c = q[i] >> 31;
     }
     mag[QW] = c;
-    round_words<W, QW + 1>(d.kind < 0, mag, -(int64_t)P - 2 - d.e, out);
+    // Q lies in (2^(P+1), 2^(P+2)], so (Q << 1 | sticky) tops out in word W
+    uint32_t mw[W + 1];
+#pragma unroll
+    for (int i = 0; i <= W; i++) mw[i] = mag[i];
+    if (mw[W] && !mag[W + 1] && !mag[W + 2])
+        round_norm<W, W + 1>(d.kind < 0, mw, -(int64_t)P - 2 - d.e, out);
+    else
+        round_words<W, QW + 1>(d.kind < 0, mag, -(int64_t)P - 2 - d.e, out);
 }
 
 // RN(x / d): Q = floor(Mx 2^(P+2) / Md) by the same base-2^32 schoolbook
@@ -660,8 +825,15 @@ SDPX_MP_OUTLINE void div(const Val<W> &x, const Val<W> &d, Val<W> &out) {
         c = q[i] >> 31;
     }
     mag[QW] = c;
-    round_words<W, QW + 1>((x.kind < 0) != (d.kind < 0), mag,
-                           (int64_t)x.e - d.e - P - 3, out);
+    // Q lies in (2^(P+1), 2^(P+3)), so (Q << 1 | sticky) tops out in word W
+    uint32_t mw[W + 1];
+#pragma unroll
+    for (int i = 0; i <= W; i++) mw[i] = mag[i];
+    if (mw[W] && !mag[W + 1] && !mag[W + 2])
+        round_norm<W, W + 1>((x.kind < 0) != (d.kind < 0), mw, (int64_t)x.e - d.e - P - 3, out);
+    else
+        round_words<W, QW + 1>((x.kind < 0) != (d.kind < 0), mag,
+                               (int64_t)x.e - d.e - P - 3, out);
 }
 
 // RN(sqrt(a)) for a > 0: restoring integer square root, two radicand bits a

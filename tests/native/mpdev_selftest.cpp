@@ -164,6 +164,70 @@ int recip_edges() {
     return bad;
 }
 
+// Sums, differences and products at the edges of the rounding paths:
+// exponent gaps around each word boundary, 64 and P + 2, mantissas that
+// cancel to any depth, exact ties (a power of two plus half an ulp),
+// all-ones mantissas whose rounding carries out, against mpx.
+template <int N>
+int add_mul_edges(uint64_t seed) {
+    constexpr int W = 2 * N;
+    constexpr int P = 64 * N;
+    const size_t vb = 8 * N + 8;
+    std::mt19937_64 rng(seed);
+    std::vector<uint8_t> a(vb), b(vb), r1(vb), r2(vb);
+    int bad = 0;
+    std::vector<int> gaps = {0, 1, 2, 3, 31, 32, 33, 63, 64, 65, 66, 95, 96, 97, P - 2, P - 1, P, P + 1, P + 2, P + 3};
+    auto setm = [&](uint8_t *rec, int pat, const uint8_t *like) {
+        uint32_t *m = (uint32_t *)rec;
+        for (int i = 0; i < W; i++) {
+            switch (pat) {
+                case 0: m[i] = 0; break;                                    // power of two
+                case 1: m[i] = 0xFFFFFFFFu; break;                          // all ones
+                case 2: m[i] = i == 0 ? 1u : 0u; break;                      // power of two + 1 ulp
+                case 3: m[i] = (uint32_t)rng(); break;                       // random
+                case 4: m[i] = ((const uint32_t *)like)[i]; break;           // same as a
+                case 5: m[i] = ((const uint32_t *)like)[i] ^ (i < W / 2 ? (uint32_t)rng() : 0u); break;  // low half differs
+                case 6: m[i] = ((const uint32_t *)like)[i] - (i == 0 ? 1u : 0u); break;  // one ulp below a (a's low word non-zero)
+                default: m[i] = i < W - 1 ? 0u : 0x40000000u; break;          // top bit clear until fixed below
+            }
+        }
+        m[W - 1] |= 0x80000000u;
+    };
+    for (int gap : gaps)
+        for (int pa = 0; pa < 4; pa++)
+            for (int pb = 0; pb < 7; pb++)
+                for (int sg = 0; sg < 4; sg++) {
+                    setm(a.data(), pa, nullptr);
+                    if (pa == 3) ((uint32_t *)a.data())[0] |= 1u;
+                    setm(b.data(), pb, a.data());
+                    const int ea = (int)(rng() % 200) - 100;
+                    *(int32_t *)(a.data() + 8 * N) = (sg & 1) ? -3 : 3;
+                    *(int32_t *)(a.data() + 8 * N + 4) = ea;
+                    *(int32_t *)(b.data() + 8 * N) = (sg & 2) ? -3 : 3;
+                    *(int32_t *)(b.data() + 8 * N + 4) = (pb >= 4 && gap > 3) ? ea : ea - gap;
+                    for (int swap = 0; swap < 2; swap++) {
+                        const uint8_t *pa_ = swap ? b.data() : a.data(), *pb_ = swap ? a.data() : b.data();
+                        mpd::Val<W> va, vb2, vr;
+                        mpd::load((const uint32_t *)pa_, va);
+                        mpd::load((const uint32_t *)pb_, vb2);
+                        mpx::View xa{N, pa_}, xb{N, pb_};
+                        mpd::add_signed(va, vb2, false, vr);
+                        mpd::store((uint32_t *)r1.data(), vr);
+                        mpx::add(xa, xb, r2.data());
+                        if (std::memcmp(r1.data(), r2.data(), vb)) bad++, std::printf("add edge mismatch N=%d gap=%d\n", N, gap);
+                        mpd::add_signed(va, vb2, true, vr);
+                        mpd::store((uint32_t *)r1.data(), vr);
+                        mpx::sub(xa, xb, r2.data());
+                        if (std::memcmp(r1.data(), r2.data(), vb)) bad++, std::printf("sub edge mismatch N=%d gap=%d\n", N, gap);
+                        mpd::mul(va, vb2, vr);
+                        mpd::store((uint32_t *)r1.data(), vr);
+                        mpx::mul(xa, xb, r2.data());
+                        if (std::memcmp(r1.data(), r2.data(), vb)) bad++, std::printf("mul edge mismatch N=%d\n", N);
+                    }
+                }
+    return bad;
+}
+
 // Division and square root against the C oracle (liboracle.so, path in
 // argv[2]): both are correctly rounded, so the bits must agree.
 typedef void (*ora_div_fn)(int, const uint8_t *, const uint8_t *, uint8_t *);
@@ -271,6 +335,7 @@ int main(int argc, char **argv) {
     int bad = run<1>(count, 200, 1) + run<2>(count, 200, 2) + run<4>(count, 300, 3) +
               run<8>(count, 600, 4);
     bad += recip_edges<1>() + recip_edges<2>() + recip_edges<4>() + recip_edges<8>() + recip_edges<16>();
+    bad += add_mul_edges<1>(31) + add_mul_edges<2>(32) + add_mul_edges<4>(33) + add_mul_edges<8>(34);
     if (argc > 2) {
         void *h = dlopen(argv[2], RTLD_NOW);
         if (!h) {
