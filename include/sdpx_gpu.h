@@ -64,6 +64,14 @@ int32_t sdpx_gpu_last_plan(uint32_t *moduli, uint32_t *planes, double *bits);
  * returns this thread's most recent stage times in ms: scan, encode,
  * product (all INT8 MFMA launches), crt. */
 int32_t sdpx_gpu_set_profiling(int32_t on);
+
+/* Named tuning switches of the GPU panel kernels, for measurement and tests;
+ * no reference counterpart.  "trsm_rl" (0/1): the right-looking pass of the
+ * triangular-solve base (default 1); "trsm_rl_room" (bits, default 128): its
+ * exact windows' room below their first term, a negative value sends every
+ * row to the left-looking kernel.  Results never depend on either.  Returns
+ * 1 for an unknown name. */
+int32_t sdpx_gpu_set_tuning(const char *name, int64_t value);
 int32_t sdpx_gpu_last_stage_times(double *ms, uint32_t count);
 /* Totals over this thread's profiled exact products since the last reset:
  * stage ms (4, as above), algorithmic INT8 operations (2 x outputs x k x

@@ -6,6 +6,7 @@
 #include <string>
 
 #include "common.h"
+#include "mp_kernels.h"
 #include "mpx.h"
 #include "rns_gemm.h"
 
@@ -103,6 +104,11 @@ int32_t sdpx_gpu_last_plan(uint32_t *moduli, uint32_t *planes, double *bits) {
     if (moduli) *moduli = (uint32_t)s.moduli;
     if (planes) *planes = (uint32_t)s.planes;
     if (bits) *bits = s.bits;
+    return SDPX_GPU_OK;
+}
+
+int32_t sdpx_gpu_set_tuning(const char *name, int64_t value) {
+    if (!name || !sdpx::mp_set_tuning(name, value)) return invalid("unknown tuning switch");
     return SDPX_GPU_OK;
 }
 

@@ -106,8 +106,9 @@ void solve_t_device(DCtx &c, size_t m, size_t n, uint8_t *L, size_t ldl, uint8_t
         size_t mark = c.arena->top;
         int32_t *d_low = upload_ints(c, low);
         void *ranges = c.arena->alloc(8 * n);
+        void *work = c.arena->alloc(4 * (m + 1));
         mp_exp_ranges(c.n, L, ldl, n, 0, ranges, c.st);
-        mp_trsm_base(c.n, L, ldl, n, B, ldb, m, d_low, ranges, c.d_status + 2, c.st);
+        mp_trsm_base(c.n, L, ldl, n, B, ldb, m, d_low, ranges, c.d_status + 2, work, c.st);
         HIP_CHECK(hipStreamSynchronize(c.st));
         c.arena->top = mark;
         c.stats->base_rows += m;

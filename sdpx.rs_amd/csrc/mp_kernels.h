@@ -12,6 +12,8 @@ struct LdlBlockDesc {  // mirrors the kernel's LdlBlock
 };
 
 bool mp_kernels_support(int limbs);
+// Named tuning switches of the panel kernels (false: unknown name).
+bool mp_set_tuning(const char *name, int64_t value);
 void mp_ew_sub_packed(int limbs, void *S, int64_t ld, const void *P, int64_t n, int lower,
                       hipStream_t st);
 void mp_ew_sub_rect(int limbs, void *S, int64_t lds, const void *P, int64_t ldp, int64_t m,
@@ -36,8 +38,12 @@ struct FwdLeaf {
 void mp_forward_rows(int limbs, const void *L_all, const int32_t *first_all, void *Y_all,
                      const void *ranges_all, const FwdLeaf *leaves, int nleaves, int64_t t, int *fail,
                      hipStream_t st);
+// X L^T = B rows (n <= 128 takes a right-looking pass first and solves only
+// the rows it hands back with the left-looking kernel).  work: device
+// memory of at least 4 (m + 1) bytes.
 void mp_trsm_base(int limbs, const void *L, int64_t ldl, int64_t n, void *B, int64_t ldb,
-                  int64_t m, const int32_t *low, const void *lranges, int *fail, hipStream_t st);
+                  int64_t m, const int32_t *low, const void *lranges, int *fail, void *work,
+                  hipStream_t st);
 // status[0] = 0 ok, 1 failed pivot, 2 window overflow (zero on entry);
 // status[1] += regularized.  nmax: the largest block order.  Each block's
 // scratch holds ldl_scratch_records(n) records.

@@ -20,7 +20,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
+#include <cstring>
 
 #include "common.h"
 #include "mp_kernels.h"
@@ -264,6 +266,9 @@ __device__ __forceinline__ void group_finish(lds_i64 *ls, int gl, int used, int6
 // and the words under the top are read with uniform lane indices, so the
 // rounding runs once on scalar values.  Same words, sign and rounding as
 // group_finish.
+template <int W, int SLOTS>
+__device__ __forceinline__ void wave_round(int64_t T, int gl, int64_t base, Val<W> &out);
+
 template <int W, int SLOTS, int STRIDE>
 __device__ __forceinline__ void wave_finish(lds_i64 *ls, int gl, int used, int64_t base, Val<W> &out) {
     static_assert(SLOTS <= 64, "one slot a lane");
@@ -277,6 +282,14 @@ __device__ __forceinline__ void wave_finish(lds_i64 *ls, int gl, int used, int64
             T += x.x + x.y;
         }
     }
+    wave_round<W, SLOTS>(T, gl, base, out);
+}
+
+// The rounding half of wave_finish: lane s holds slot s's total (lanes at
+// or above SLOTS hold zero); every lane ends with RN of the window's value
+// at scale base.
+template <int W, int SLOTS>
+__device__ __forceinline__ void wave_round(int64_t T, int gl, int64_t base, Val<W> &out) {
     const bool below_top = gl < SLOTS - 1;
     for (int it = 0; it < SLOTS; it++) {
         const int64_t c = below_top ? (T >> 32) : 0;
@@ -464,15 +477,120 @@ __global__ __launch_bounds__(ROW_THREADS) void forward_rows_kernel(const uint32_
     }
 }
 
+// X L^T = B rows (solve_unit_lower_t base, arrow.rs:425-447) right-looking:
+// one wave a row, lane l owning columns l and l + 64 (n <= 128), each with
+// its own exact slot window in LDS.  Step j rounds column j's window once
+// (all lanes together, wave_round: the exact dot of x_k L_jk over k < j),
+// forms x_j = RN(b_j - t) in every lane, and each lane adds x_j L_cj into
+// the windows of its columns c > j.  So a step costs one rounding and a
+// product per lane instead of a whole exact dot.  A window's base is set at
+// its first term from the exponent range of L's row (rows of L over k < c)
+// with `room` bits (128 by default) for x to shrink; a term outside the window puts the
+// row on the fallback list (fb_rows) with B's row untouched, and the
+// left-looking kernel then solves it.  Results are those of the left-looking
+// kernel: the same exact dots, each rounded once.
+template <int W>
+__global__ __launch_bounds__(ROW_THREADS) void trsm_rl_kernel(
+    const uint32_t *L, int64_t ldl, int64_t n, uint32_t *B, int64_t ldb, int64_t m,
+    const int32_t *low, const int2 *lr, int64_t room, int *fb_count, int32_t *fb_rows) {
+    constexpr int SLOTS = Win<W>::SLOTS;
+    static_assert(SLOTS <= 64, "one slot a lane");
+    extern __shared__ int64_t slots_sh[];  // [2][SLOTS][64]
+    const int l = threadIdx.x;
+    const int64_t i = blockIdx.x;
+    if (i >= m) return;
+    lds_i64 *win = (lds_i64 *)slots_sh;
+    const int64_t lo = low[i];
+    Val<W> bq[2], xq[2];
+    int64_t base[2] = {INT64_MIN, INT64_MIN};
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+        const int64_t c = l + 64 * q;
+        set_zero(xq[q]);
+        if (c >= lo && c < n) load(rec_at<W>((const uint32_t *)B, i + c * ldb), bq[q]);
+        else set_zero(bq[q]);
+#pragma unroll
+        for (int sl = 0; sl < SLOTS; sl++) win[(q * SLOTS + sl) * 64 + l] = 0;
+    }
+    // L's column j entries of this lane's rows, one step ahead
+    Val<W> lv[2], ln[2];
+    auto load_l = [&](int64_t j, Val<W> (&dst)[2]) {
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const int64_t c = l + 64 * q;
+            if (c > j && c < n) load(rec_at<W>(L, c + j * ldl), dst[q]);
+            else set_zero(dst[q]);
+        }
+    };
+    if (lo < n) load_l(lo, ln);
+    bool ovf = false;
+    for (int64_t j = lo; j < n; j++) {
+        lv[0] = ln[0];
+        lv[1] = ln[1];
+        if (j + 1 < n) load_l(j + 1, ln);
+        const int o = (int)(j & 63), q = (int)(j >> 6);
+        const int64_t bj = __shfl(q ? base[1] : base[0], o, 64);
+        Val<W> t;
+        if (bj == INT64_MIN) {
+            set_zero(t);
+        } else {
+            const int64_t T = l < SLOTS ? (int64_t)win[(q * SLOTS + l) * 64 + o] : 0;
+            wave_round<W, SLOTS>(T, l, bj, t);
+        }
+        // b_j from its owner lane, then x_j = RN(b_j - t) in every lane
+        Val<W> b, x;
+        const Val<W> &own = q ? bq[1] : bq[0];
+#pragma unroll
+        for (int u = 0; u < W; u++) b.m[u] = (uint32_t)__shfl((int)own.m[u], o, 64);
+        b.kind = __shfl(own.kind, o, 64);
+        b.e = __shfl(own.e, o, 64);
+        add_signed(b, t, true, x);
+        if (l == o) {
+            if (q) xq[1] = x;
+            else xq[0] = x;
+        }
+        if (is_zero(x)) continue;
+#pragma unroll
+        for (int q2 = 0; q2 < 2; q2++) {
+            if (is_zero(lv[q2])) continue;  // includes columns c <= j and c >= n
+            const int64_t sc = (int64_t)x.e + lv[q2].e - 64 * W;
+            if (base[q2] == INT64_MIN) {
+                const int64_t c = l + 64 * q2;
+                base[q2] = lr ? (int64_t)x.e - room + lr[c].x - 64 * W : sc - 2 * room;
+            }
+            const int64_t off = sc - base[q2];
+            if (off < 0 || off / 32 + 2 * W + 4 > SLOTS) {
+                ovf = true;
+                continue;
+            }
+            lds_add_product<W, 64>(win + (q2 * SLOTS) * 64 + l, base[q2], x, lv[q2], false);
+        }
+        if (__ballot(ovf) != 0) break;
+    }
+    if (__ballot(ovf) != 0) {
+        if (l == 0) fb_rows[atomicAdd(fb_count, 1)] = (int32_t)i;
+        return;
+    }
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+        const int64_t c = l + 64 * q;
+        if (c >= lo && c < n) store(rec_at<W>(B, i + c * ldb), xq[q]);
+    }
+}
+
 // X L^T = B rows (solve_unit_lower_t base): B m x n (ldb), L n x n (ldl,
 // unit lower, diagonal unread), low[m]; one lane group per row.
 template <int W, int G>
 __global__ __launch_bounds__(ROW_THREADS) void trsm_base_kernel(
     const uint32_t *L, int64_t ldl, int64_t n, uint32_t *B, int64_t ldb, int64_t m,
-    const int32_t *low, const int2 *lr, int *fail) {
+    const int32_t *low, const int2 *lr, int *fail, const int32_t *rows, const int *nrows) {
     extern __shared__ int64_t slots_sh[];
     const int gl = threadIdx.x & (G - 1);
-    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    if (rows) {  // only the listed rows
+        if (i >= *nrows) return;
+        i = rows[i];
+    }
     if (i >= m) return;
     int64_t *slots = slots_sh + threadIdx.x;
     const int64_t lo = low[i];
@@ -1207,16 +1325,58 @@ void mp_forward_rows(int limbs, const void *L_all, const int32_t *first_all, voi
     HIP_CHECK(hipGetLastError());
 }
 
+// Tuning switches (sdpx_gpu_set_tuning): "trsm_rl" 0/1 turns the
+// right-looking pass of the triangular-solve base off/on (default on, or
+// SDPX_GPU_TRSM_RL=0 in the environment), "trsm_rl_room" sets its window's
+// room below the first term in bits (default 128; a negative room sends
+// every row to the left-looking kernel).
+static std::atomic<int> g_trsm_rl{-1};
+static std::atomic<int64_t> g_trsm_room{128};
+static bool trsm_right_looking() {
+    const int v = g_trsm_rl.load();
+    if (v >= 0) return v != 0;
+    static const bool env = [] {
+        const char *e = std::getenv("SDPX_GPU_TRSM_RL");
+        return !(e && e[0] == '0');
+    }();
+    return env;
+}
+bool mp_set_tuning(const char *name, int64_t value) {
+    if (!std::strcmp(name, "trsm_rl")) {
+        g_trsm_rl = value ? 1 : 0;
+        return true;
+    }
+    if (!std::strcmp(name, "trsm_rl_room")) {
+        g_trsm_room = value;
+        return true;
+    }
+    return false;
+}
+
 void mp_trsm_base(int limbs, const void *L, int64_t ldl, int64_t n, void *B, int64_t ldb,
-                  int64_t m, const int32_t *low, const void *lranges, int *fail, hipStream_t st) {
+                  int64_t m, const int32_t *low, const void *lranges, int *fail, void *work,
+                  hipStream_t st) {
     if (m == 0 || n == 0) return;
+    const bool rl = trsm_right_looking() && n <= 128;
+    int *fb_count = (int *)work;
+    int32_t *fb_rows = (int32_t *)work + 1;
     SDPX_DISPATCH_W(limbs, {
+        if (rl) {
+            HIP_CHECK(hipMemsetAsync(fb_count, 0, sizeof(int), st));
+            auto kr = trsm_rl_kernel<W>;
+            const size_t lds = 2 * slot_bytes<W>(ROW_THREADS);
+            if (lds > 65536)
+                HIP_CHECK(hipFuncSetAttribute((const void *)kr, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            hipLaunchKernelGGL(kr, dim3((unsigned)m), dim3(ROW_THREADS), lds, st, (const uint32_t *)L, ldl, n,
+                               (uint32_t *)B, ldb, m, low, (const int2 *)lranges, (int64_t)g_trsm_room.load(),
+                               fb_count, fb_rows);
+        }
         SDPX_DISPATCH_G({
             auto kern = trsm_base_kernel<W, G>;
             const size_t lds = panel_lds<W>((const void *)kern, ROW_THREADS);
             hipLaunchKernelGGL(kern, dim3(blocks_for(m * G, ROW_THREADS)), dim3(ROW_THREADS), lds, st,
                                (const uint32_t *)L, ldl, n, (uint32_t *)B, ldb, m, low, (const int2 *)lranges,
-                               fail);
+                               fail, rl ? fb_rows : nullptr, rl ? fb_count : nullptr);
         });
     });
     HIP_CHECK(hipGetLastError());

@@ -65,6 +65,8 @@ def _declare(L):
     L.sdpx_gpu_last_plan.restype = i32
     L.sdpx_gpu_set_profiling.argtypes = [i32]
     L.sdpx_gpu_set_profiling.restype = i32
+    L.sdpx_gpu_set_tuning.argtypes = [ctypes.c_char_p, ctypes.c_int64]
+    L.sdpx_gpu_set_tuning.restype = i32
     L.sdpx_gpu_last_stage_times.argtypes = [dp, u32]
     L.sdpx_gpu_last_stage_times.restype = i32
     L.sdpx_gpu_dense_create.argtypes = [u64, ctypes.POINTER(vp)]
@@ -155,6 +157,11 @@ def last_plan():
 
 def set_profiling(on):
     lib().sdpx_gpu_set_profiling(1 if on else 0)
+
+
+def set_tuning(name, value):
+    """A named tuning switch of the panel kernels (sdpx_gpu_set_tuning)."""
+    check(lib().sdpx_gpu_set_tuning(name.encode(), int(value)))
 
 
 def last_stage_times():

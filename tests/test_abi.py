@@ -55,3 +55,10 @@ def test_cholesky_argument_codes_before_device_work():
     assert info.value == -7
     assert L.sdpx_gpu_mp_potrf(2, ord("L"), 0, None, 1, ctypes.byref(info)) == 0
     assert info.value == 0
+
+
+def test_tuning_switch_names():
+    L = sdpx_gpu.lib()
+    assert L.sdpx_gpu_set_tuning(b"no_such_switch", 1) == sdpx_gpu.INVALID
+    assert "tuning" in sdpx_gpu.last_error()
+    assert L.sdpx_gpu_set_tuning(b"trsm_rl_room", 128) == sdpx_gpu.OK

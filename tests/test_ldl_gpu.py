@@ -48,6 +48,34 @@ def test_random_gpu_products(gpu, nl, n, kind):
     assert np.all(mp.same_bits(dg, dinv))
 
 
+_spread_case = {}
+
+
+@pytest.mark.parametrize("room", [-100000, 0, 128])
+def test_trsm_right_looking_rows_and_fallback(gpu, room):
+    """The triangular-solve base's right-looking pass: with a negative room
+    every row goes to the left-looking kernel, with no room below the first
+    term some rows do, and by default (128 bits) few or none; the factor is
+    the oracle's bit for bit in each case."""
+    nl, n = 4, 520
+    if not _spread_case:
+        rng = np.random.default_rng(77)
+        s = spread_spd(n, rng, -12)
+        A = mp.from_float64(((s + s.T) / 2).T.reshape(-1), nl)
+        _spread_case["A"] = A
+        _spread_case["ref"] = oracle_ldl(A, n, threads=16)
+    A = _spread_case["A"]
+    ok, L, dinv, _ = _spread_case["ref"]
+    print("trsm room", room, flush=True)
+    sdpx_gpu.set_tuning("trsm_rl_room", room)
+    try:
+        okg, Lg, dg = sdpx_gpu.mp_ldl(A, n)
+    finally:
+        sdpx_gpu.set_tuning("trsm_rl_room", 128)
+    assert ok and okg
+    assert np.all(mp.same_bits(Lg, L)) and np.all(mp.same_bits(dg, dinv))
+
+
 def widen(rec, nl_to):
     """Exact widening of records to 64*nl_to bits (zero low limbs)."""
     nl = mp.limbs_of(rec)
