@@ -266,6 +266,16 @@ __device__ __forceinline__ void group_finish(lds_i64 *ls, int gl, int used, int6
 // and the words under the top are read with uniform lane indices, so the
 // rounding runs once on scalar values.  Same words, sign and rounding as
 // group_finish.
+// Lane i receives lane i - 1's value (lane 0 receives 0): a whole-wave DPP
+// shift.
+__device__ __forceinline__ int lane_up(int v) { return __builtin_amdgcn_mov_dpp(v, 0x138, 0xf, 0xf, true); }
+// A lane's 64-bit value in every lane (lane: a wave-uniform index).
+__device__ __forceinline__ int64_t readlane64(int64_t v, int lane) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(uint64_t)v, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), lane);
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
 template <int W, int SLOTS>
 __device__ __forceinline__ void wave_round(int64_t T, int gl, int64_t base, Val<W> &out);
 
@@ -292,13 +302,12 @@ template <int W, int SLOTS>
 __device__ __forceinline__ void wave_round(int64_t T, int gl, int64_t base, Val<W> &out) {
     const bool below_top = gl < SLOTS - 1;
     for (int it = 0; it < SLOTS; it++) {
-        const int64_t c = below_top ? (T >> 32) : 0;
+        const int c = below_top ? (int)(T >> 32) : 0;  // T >> 32 fits 32 bits
         if (__ballot(c != 0) == 0) break;
         if (below_top) T &= 0xFFFFFFFFll;
-        const int64_t cin = __shfl_up(c, 1, 64);
-        if (gl > 0) T += cin;
+        T += lane_up(c);
     }
-    const int64_t ttop = __shfl(T, SLOTS - 1, 64);
+    const int64_t ttop = readlane64(T, SLOTS - 1);
     const bool neg = ttop < 0;
     uint32_t w = gl < SLOTS ? (uint32_t)(uint64_t)T : 0u;
     if (neg) {
@@ -478,104 +487,94 @@ __global__ __launch_bounds__(ROW_THREADS) void forward_rows_kernel(const uint32_
 }
 
 // X L^T = B rows (solve_unit_lower_t base, arrow.rs:425-447) right-looking:
-// one wave a row, lane l owning columns l and l + 64 (n <= 128), each with
-// its own exact slot window in LDS.  Step j rounds column j's window once
-// (all lanes together, wave_round: the exact dot of x_k L_jk over k < j),
-// forms x_j = RN(b_j - t) in every lane, and each lane adds x_j L_cj into
-// the windows of its columns c > j.  So a step costs one rounding and a
-// product per lane instead of a whole exact dot.  A window's base is set at
-// its first term from the exponent range of L's row (rows of L over k < c)
-// with `room` bits (128 by default) for x to shrink; a term outside the window puts the
-// row on the fallback list (fb_rows) with B's row untouched, and the
-// left-looking kernel then solves it.  Results are those of the left-looking
-// kernel: the same exact dots, each rounded once.
+// one workgroup of two waves a row, thread c owning column c (n <= 128) with
+// its own exact slot window in LDS.  Step j rounds column j's window once in
+// each wave (wave_round: the exact dot of x_k L_jk over k < j, the same
+// result in both), forms x_j = RN(b_j - t), and each thread adds x_j L_cj
+// into the window of its column c > j; one barrier a step.  So a step costs
+// one rounding and one product per thread instead of a whole exact dot.
+// b_j and the base of column j's window reach both waves through LDS (the
+// owner of column j + 1 posts its own after step j's products).  A window's
+// base is set at its first term from the exponent range of L's row (rows of
+// L over k < c) with `room` bits (128 by default) for x to shrink; a term
+// outside the window puts the row on the fallback list (fb_rows) with B's
+// row untouched, and the left-looking kernel then solves it.  Results are
+// those of the left-looking kernel: the same exact dots, each rounded once.
+constexpr int RL_THREADS = 128;
+
 template <int W>
-__global__ __launch_bounds__(ROW_THREADS) void trsm_rl_kernel(
+__global__ __launch_bounds__(RL_THREADS) void trsm_rl_kernel(
     const uint32_t *L, int64_t ldl, int64_t n, uint32_t *B, int64_t ldb, int64_t m,
     const int32_t *low, const int2 *lr, int64_t room, int *fb_count, int32_t *fb_rows) {
     constexpr int SLOTS = Win<W>::SLOTS;
     static_assert(SLOTS <= 64, "one slot a lane");
-    extern __shared__ int64_t slots_sh[];  // [2][SLOTS][64]
-    const int l = threadIdx.x;
+    extern __shared__ int64_t slots_sh[];  // [SLOTS][RL_THREADS] windows, then two posted (b, base) pairs
+    const int c = threadIdx.x, wl = c & 63;
     const int64_t i = blockIdx.x;
-    if (i >= m) return;
+    if (i >= m) return;  // the whole workgroup
     lds_i64 *win = (lds_i64 *)slots_sh;
+    uint32_t *post = (uint32_t *)(slots_sh + SLOTS * RL_THREADS);  // 2 x (W + 2 words, pad, int64 base)
+    constexpr int PW = W + 4;  // posted words: record, then the base as two words
     const int64_t lo = low[i];
-    Val<W> bq[2], xq[2];
-    int64_t base[2] = {INT64_MIN, INT64_MIN};
+    Val<W> b, xo;
+    set_zero(xo);
+    int64_t base = INT64_MIN;
+    if (c >= lo && c < n) load(rec_at<W>((const uint32_t *)B, i + c * ldb), b);
+    else set_zero(b);
 #pragma unroll
-    for (int q = 0; q < 2; q++) {
-        const int64_t c = l + 64 * q;
-        set_zero(xq[q]);
-        if (c >= lo && c < n) load(rec_at<W>((const uint32_t *)B, i + c * ldb), bq[q]);
-        else set_zero(bq[q]);
-#pragma unroll
-        for (int sl = 0; sl < SLOTS; sl++) win[(q * SLOTS + sl) * 64 + l] = 0;
-    }
-    // L's column j entries of this lane's rows, one step ahead
-    Val<W> lv[2], ln[2];
-    auto load_l = [&](int64_t j, Val<W> (&dst)[2]) {
-#pragma unroll
-        for (int q = 0; q < 2; q++) {
-            const int64_t c = l + 64 * q;
-            if (c > j && c < n) load(rec_at<W>(L, c + j * ldl), dst[q]);
-            else set_zero(dst[q]);
-        }
+    for (int sl = 0; sl < SLOTS; sl++) win[sl * RL_THREADS + c] = 0;
+    auto post_own = [&](int64_t j) {  // column j's b and window base, for step j
+        uint32_t *pw = post + (j & 1) * PW;
+        store(pw, b);
+        pw[W + 2] = (uint32_t)(uint64_t)base;
+        pw[W + 3] = (uint32_t)((uint64_t)base >> 32);
+    };
+    if (c == lo) post_own(lo);
+    // L's entry (c, j) of this thread's column, one step ahead
+    Val<W> lv, ln;
+    auto load_l = [&](int64_t j, Val<W> &dst) {
+        if (c > j && c < n) load(rec_at<W>(L, c + j * ldl), dst);
+        else set_zero(dst);
     };
     if (lo < n) load_l(lo, ln);
+    __syncthreads();
     bool ovf = false;
     for (int64_t j = lo; j < n; j++) {
-        lv[0] = ln[0];
-        lv[1] = ln[1];
+        lv = ln;
         if (j + 1 < n) load_l(j + 1, ln);
-        const int o = (int)(j & 63), q = (int)(j >> 6);
-        const int64_t bj = __shfl(q ? base[1] : base[0], o, 64);
-        Val<W> t;
-        if (bj == INT64_MIN) {
+        const uint32_t *pw = post + (j & 1) * PW;
+        Val<W> bj, t, x;
+#pragma unroll
+        for (int u = 0; u < W; u++) bj.m[u] = __builtin_amdgcn_readfirstlane(pw[u]);
+        bj.kind = __builtin_amdgcn_readfirstlane((int)pw[W]);
+        bj.e = __builtin_amdgcn_readfirstlane((int)pw[W + 1]);
+        const int64_t basej = (int64_t)(((uint64_t)__builtin_amdgcn_readfirstlane(pw[W + 3]) << 32) |
+                                        (uint32_t)__builtin_amdgcn_readfirstlane(pw[W + 2]));
+        if (basej == INT64_MIN) {
             set_zero(t);
         } else {
-            const int64_t T = l < SLOTS ? (int64_t)win[(q * SLOTS + l) * 64 + o] : 0;
-            wave_round<W, SLOTS>(T, l, bj, t);
+            const int64_t T = wl < SLOTS ? (int64_t)win[wl * RL_THREADS + j] : 0;
+            wave_round<W, SLOTS>(T, wl, basej, t);
         }
-        // b_j from its owner lane, then x_j = RN(b_j - t) in every lane
-        Val<W> b, x;
-        const Val<W> &own = q ? bq[1] : bq[0];
-#pragma unroll
-        for (int u = 0; u < W; u++) b.m[u] = (uint32_t)__shfl((int)own.m[u], o, 64);
-        b.kind = __shfl(own.kind, o, 64);
-        b.e = __shfl(own.e, o, 64);
-        add_signed(b, t, true, x);
-        if (l == o) {
-            if (q) xq[1] = x;
-            else xq[0] = x;
-        }
-        if (is_zero(x)) continue;
-#pragma unroll
-        for (int q2 = 0; q2 < 2; q2++) {
-            if (is_zero(lv[q2])) continue;  // includes columns c <= j and c >= n
-            const int64_t sc = (int64_t)x.e + lv[q2].e - 64 * W;
-            if (base[q2] == INT64_MIN) {
-                const int64_t c = l + 64 * q2;
-                base[q2] = lr ? (int64_t)x.e - room + lr[c].x - 64 * W : sc - 2 * room;
-            }
-            const int64_t off = sc - base[q2];
-            if (off < 0 || off / 32 + 2 * W + 4 > SLOTS) {
+        add_signed(bj, t, true, x);  // x_j = RN(b_j - t), the same in every thread
+        if (c == j) xo = x;
+        if (!is_zero(x) && !is_zero(lv)) {  // lv is zero for c <= j and c >= n
+            const int64_t sc = (int64_t)x.e + lv.e - 64 * W;
+            if (base == INT64_MIN) base = lr ? (int64_t)x.e - room + lr[c].x - 64 * W : sc - 2 * room;
+            const int64_t off = sc - base;
+            if (off < 0 || off / 32 + 2 * W + 4 > SLOTS)
                 ovf = true;
-                continue;
-            }
-            lds_add_product<W, 64>(win + (q2 * SLOTS) * 64 + l, base[q2], x, lv[q2], false);
+            else
+                lds_add_product<W, RL_THREADS>(win + c, base, x, lv, false);
         }
-        if (__ballot(ovf) != 0) break;
+        if (c == j + 1) post_own(j + 1);
+        if (__syncthreads_or(ovf)) break;
     }
-    if (__ballot(ovf) != 0) {
-        if (l == 0) fb_rows[atomicAdd(fb_count, 1)] = (int32_t)i;
+    if (__syncthreads_or(ovf)) {
+        if (c == 0) fb_rows[atomicAdd(fb_count, 1)] = (int32_t)i;
         return;
     }
-#pragma unroll
-    for (int q = 0; q < 2; q++) {
-        const int64_t c = l + 64 * q;
-        if (c >= lo && c < n) store(rec_at<W>(B, i + c * ldb), xq[q]);
-    }
+    if (c >= lo && c < n) store(rec_at<W>(B, i + c * ldb), xo);
 }
 
 // X L^T = B rows (solve_unit_lower_t base): B m x n (ldb), L n x n (ldl,
@@ -1364,10 +1363,10 @@ void mp_trsm_base(int limbs, const void *L, int64_t ldl, int64_t n, void *B, int
         if (rl) {
             HIP_CHECK(hipMemsetAsync(fb_count, 0, sizeof(int), st));
             auto kr = trsm_rl_kernel<W>;
-            const size_t lds = 2 * slot_bytes<W>(ROW_THREADS);
+            const size_t lds = slot_bytes<W>(RL_THREADS) + 2 * (W + 4) * sizeof(uint32_t);
             if (lds > 65536)
                 HIP_CHECK(hipFuncSetAttribute((const void *)kr, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            hipLaunchKernelGGL(kr, dim3((unsigned)m), dim3(ROW_THREADS), lds, st, (const uint32_t *)L, ldl, n,
+            hipLaunchKernelGGL(kr, dim3((unsigned)m), dim3(RL_THREADS), lds, st, (const uint32_t *)L, ldl, n,
                                (uint32_t *)B, ldb, m, low, (const int2 *)lranges, (int64_t)g_trsm_room.load(),
                                fb_count, fb_rows);
         }
