@@ -334,6 +334,34 @@ __device__ __forceinline__ void wave_round(int64_t T, int gl, int64_t base, Val<
     round_norm<W, GW + 1>(neg, g, base + 32 * (int64_t)(lo - 1), out);  // g[GW] = w[top] != 0
 }
 
+// The lane windows of a group dot before its finish: slots [0, used) of
+// each lane cleared, then the lane's terms (k % G == gl) added at window
+// base lo.  Every term must lie in the window.
+template <int W, int G, int STRIDE>
+__device__ __forceinline__ void group_accumulate(int64_t k0, int64_t k1, int gl, const uint32_t *xb, int64_t xs,
+                                                 const uint32_t *yb, int64_t ys, bool negate_terms, lds_i64 *ls,
+                                                 int used, int64_t lo) {
+    auto fx = [&](int64_t k) { return xb + k * xs * (W + 2); };
+    auto fy = [&](int64_t k) { return yb + k * ys * (W + 2); };
+    const int64_t kf = k0 + (((int64_t)gl - k0) % G + G) % G;  // first k of this lane
+    for (int sl = 0; sl < used; sl++) ls[sl * STRIDE] = 0;
+    Val<W> xn, yn;
+    int64_t k = kf;
+    if (k < k1) {
+        load(fx(k), xn);
+        load(fy(k), yn);
+    }
+    for (; k < k1; k += G) {
+        const Val<W> x = xn, y = yn;
+        if (k + G < k1) {  // next term in flight during this one
+            load(fx(k + G), xn);
+            load(fy(k + G), yn);
+        }
+        if (is_zero(x) || is_zero(y)) continue;
+        lds_add_product<W, STRIDE>(ls, lo, x, y, negate_terms);
+    }
+}
+
 // Terms x_k = xb[k * xs], y_k = yb[k * ys] (record strides): every panel
 // kernel's operands are affine in k.  Inlined at each call site, so the
 // operands and the result stay in registers.  Groups are 16, 32 or 64 lanes.
@@ -385,22 +413,7 @@ __device__ __forceinline__ bool group_dot(int64_t k0, int64_t k1, int gl,
     // terms land in slots [0, (hi - lo)/32 + 2W]
     const int used = (int)min((int64_t)SLOTS, (hi - lo) / 32 + 2 * W + 1);
     lds_i64 *ls = (lds_i64 *)slots;
-    for (int sl = 0; sl < used; sl++) ls[sl * STRIDE] = 0;
-    Val<W> xn, yn;
-    int64_t k = kf;
-    if (k < k1) {
-        load(fx(k), xn);
-        load(fy(k), yn);
-    }
-    for (; k < k1; k += G) {
-        const Val<W> x = xn, y = yn;
-        if (k + G < k1) {  // next term in flight during this one
-            load(fx(k + G), xn);
-            load(fy(k + G), yn);
-        }
-        if (is_zero(x) || is_zero(y)) continue;
-        lds_add_product<W, STRIDE>(ls, lo, x, y, negate_terms);
-    }
+    group_accumulate<W, G, STRIDE>(k0, k1, gl, xb, xs, yb, ys, negate_terms, ls, used, lo);
     if (cnz && gl == 0) lds_add_words<W, STRIDE>(ls, (int64_t)c->e - 32 * W - lo, c->m, c->kind < 0);
     if constexpr (G == 64 && SLOTS <= 64)
         wave_finish<W, SLOTS, STRIDE>(ls, gl, used, lo, out);
@@ -441,17 +454,24 @@ __device__ __forceinline__ void range_add(int32_t &mn, int32_t &mx, int32_t kind
 // exponent range (min, max) of the non-zero entries of each row of the
 // strictly lower triangle of L (mode 0: row j over columns k < j) or of each
 // column (mode 1: column j over rows k > j); empty gives (INT32_MAX, INT32_MIN)
+// one wave a row (column): the lanes stride the entries, then a wave min/max
 template <int W>
-__global__ void exp_ranges_kernel(const uint32_t *L, int64_t ldl, int64_t n, int mode, int2 *out) {
-    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
+__global__ __launch_bounds__(64) void exp_ranges_kernel(const uint32_t *L, int64_t ldl, int64_t n, int mode,
+                                                        int2 *out) {
+    const int64_t j = blockIdx.x;
+    const int lane = threadIdx.x;
     int32_t mn = INT32_MAX, mx = INT32_MIN;
     const int64_t k0 = mode == 0 ? 0 : j + 1, k1 = mode == 0 ? j : n;
-    for (int64_t k = k0; k < k1; k++) {
+    for (int64_t k = k0 + lane; k < k1; k += 64) {
         const uint32_t *rec = rec_at<W>(L, mode == 0 ? j + k * ldl : k + j * ldl);
         range_add(mn, mx, (int32_t)rec[W], (int32_t)rec[W + 1]);
     }
-    out[j] = make_int2(mn, mx);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        mn = min(mn, __shfl_xor(mn, off, 64));
+        mx = max(mx, __shfl_xor(mx, off, 64));
+    }
+    if (lane == 0) out[j] = make_int2(mn, mx);
 }
 
 template <int W, int G>
@@ -890,18 +910,24 @@ struct LdlCol {
 };
 
 // DenseLeaf::pivot (arrow.rs:618-634) of column j from the exact dot of
-// its r and w rows (one lane group; lane 0 writes d).  Returns the status
-// code (0 ok, 1 failed pivot, 2 window overflow) in every lane.
+// its r and w rows (one lane group; lane 0 writes d), or from that dot's
+// rounding when the caller has it.  Returns the status code (0 ok, 1 failed
+// pivot, 2 window overflow) in every lane.
 template <int W, int G>
 __device__ int ldl_col_pivot(const LdlCol<W> &c, int64_t j, int gl, int64_t hl, int64_t hh,
                              int64_t *slots, const int8_t *signs, int sign_mode, int64_t sign_off,
-                             const uint32_t *reg_eps, const uint32_t *reg_delta, int *status) {
+                             const uint32_t *reg_eps, const uint32_t *reg_delta, int *status,
+                             const Val<W> *dot = nullptr) {
     const int64_t fj = c.first[j];
     const uint32_t *xr = rec_at<W>((const uint32_t *)c.r, LdlCol<W>::tri(j));
     const uint32_t *yr = rec_at<W>((const uint32_t *)c.w, LdlCol<W>::tri(j));
     Val<W> ajj, t, d;
     load(rec_at<W>(c.a, j + j * c.ld), ajj);
-    bool ok = group_dot<W, G, ROW_THREADS>(fj, j, gl, xr, 1, yr, 1, nullptr, false, slots, t, hl, hh);
+    bool ok = true;
+    if (dot)  // RN of the exact dot, formed by the caller
+        t = *dot;
+    else
+        ok = group_dot<W, G, ROW_THREADS>(fj, j, gl, xr, 1, yr, 1, nullptr, false, slots, t, hl, hh);
     if (ok) {
         add_signed(ajj, t, true, d);
         // a pivot cancelled to zero is the exact difference rounded once
@@ -927,33 +953,41 @@ __device__ int ldl_col_pivot(const LdlCol<W> &c, int64_t j, int gl, int64_t hl, 
     return 0;
 }
 
-// Launch j: the first wave of a workgroup holds the row lane groups, the
+// Launch j: the first wave of a workgroup holds the row lane groups; the
 // second forms RN(1 / d_j) meanwhile (the reciprocal the rows' entries need
 // only after their dots), hands it over through LDS and, in workgroup 0 of
-// the block, stores it.  The status is read once per workgroup, so both
-// waves take the same path to the barrier.
+// the block, stores it; in workgroup 0 (the one holding row j + 1, with
+// whole-wave groups) the third accumulates pivot j + 1's exact dot over
+// k < j into its lane windows, so after the entry (j + 1, j) only its one
+// term is added before the rounding.  The window is the row's exponent
+// range so far with 128 bits of room each side; a last term outside it, or
+// no range, sends the pivot through the whole dot.  The status is read once
+// per workgroup, so all waves take the same path to the barrier.
 template <int W, int G>
-__global__ __launch_bounds__(2 * ROW_THREADS) void ldl_col_kernel(
+__global__ __launch_bounds__(3 * ROW_THREADS) void ldl_col_kernel(
     uint32_t *A, uint32_t *dinv, const int32_t *first_all, const int8_t *signs, int sign_mode,
     const LdlBlock *blocks, uint32_t *scratch, const uint32_t *reg_eps, const uint32_t *reg_delta,
     int *status, int64_t j) {
-    extern __shared__ int64_t slots_sh[];
+    constexpr int SLOTS = Win<W>::SLOTS;
+    extern __shared__ int64_t slots_sh[];  // [SLOTS][64] row groups, then [SLOTS][64] pivot windows
     __shared__ uint32_t dinv_sh[W + 2];
     __shared__ int stop;
+    __shared__ int64_t pre_lo;
     if (threadIdx.x == 0) stop = status[0];
     __syncthreads();
     if (stop != 0) return;  // the whole workgroup
     const LdlBlock B = blocks[blockIdx.y];
     const LdlCol<W> c(A, dinv, first_all, scratch, B);
-    const bool recip_wave = threadIdx.x >= ROW_THREADS;
-    const int gl = threadIdx.x & (G - 1);
-    const int64_t i = j + 1 + ((int64_t)blockIdx.x * ROW_THREADS + (threadIdx.x & (ROW_THREADS - 1))) / G;
-    int64_t *slots = slots_sh + (threadIdx.x & (ROW_THREADS - 1));
+    const int wave = threadIdx.x / ROW_THREADS, lane = threadIdx.x & (ROW_THREADS - 1);
+    const int gl = lane & (G - 1);
+    const int64_t i = j + 1 + ((int64_t)blockIdx.x * ROW_THREADS + lane) / G;
+    int64_t *slots = slots_sh + lane;
+    lds_i64 *pre_win = (lds_i64 *)(slots_sh + SLOTS * ROW_THREADS) + lane;
     auto fail = [&](int code) {
         if (gl == 0) atomicMax(status, code);
     };
     if (j < 0) {  // pivot 0: an empty dot
-        if (!recip_wave && i == 0) {
+        if (wave == 0 && i == 0) {
             if (gl == 0) c.rng[0] = make_int4(INT32_MAX, INT32_MIN, INT32_MAX, INT32_MIN);
             const int code = ldl_col_pivot<W, G>(c, 0, gl, INT64_MAX, INT64_MIN, slots, signs, sign_mode,
                                                  B.sign_off, reg_eps, reg_delta, status);
@@ -962,8 +996,8 @@ __global__ __launch_bounds__(2 * ROW_THREADS) void ldl_col_kernel(
         return;
     }
     if (j >= B.n) return;  // the whole workgroup: this block has no column j
-    if (recip_wave) {
-        if (threadIdx.x == ROW_THREADS) {
+    if (wave == 1) {
+        if (lane == 0) {
             Val<W> dj, di;
             load(rec_at<W>((const uint32_t *)c.dstore, j), dj);
             recip(dj, di);
@@ -973,31 +1007,47 @@ __global__ __launch_bounds__(2 * ROW_THREADS) void ldl_col_kernel(
         __syncthreads();
         return;
     }
+    if (wave == 2) {
+        int64_t lo_w = INT64_MIN;
+        const int64_t p = j + 1;
+        if (G == 64 && blockIdx.x == 0 && p < B.n && j > 0 && c.first[p] < j) {
+            const int4 rp = c.rng[p];  // row p's ranges over k < j
+            int64_t hl, hh;
+            range_hint<W>(rp.x, rp.y, rp.z, rp.w, hl, hh);
+            if (hl <= hh && (hh - hl + 256) / 32 + 2 * W + 4 <= SLOTS) {
+                lo_w = hl - 128;
+                group_accumulate<W, ROW_THREADS, ROW_THREADS>(
+                    c.first[p], j, lane, rec_at<W>((const uint32_t *)c.r, LdlCol<W>::tri(p)), 1,
+                    rec_at<W>((const uint32_t *)c.w, LdlCol<W>::tri(p)), 1, false, pre_win, SLOTS, lo_w);
+            }
+        }
+        if (lane == 0) pre_lo = lo_w;
+        __syncthreads();
+        return;
+    }
     const bool active = i < B.n;
     // row i's ranges so far (column 0 starts them)
     int4 ri = make_int4(INT32_MAX, INT32_MIN, INT32_MAX, INT32_MIN);
-    Val<W> dj, dij, v, wv, diff;
+    Val<W> dj, dij, v, wv, diff, aij;
     bool dot_ok = true, zero_row = true;
     if (active) {
         if (j > 0) ri = c.rng[i];
         load(rec_at<W>((const uint32_t *)c.dstore, j), dj);
         zero_row = c.first[i] > j;
         if (!zero_row) {
+            load(rec_at<W>(c.a, i + j * c.ld), aij);
             const int64_t lo = max(c.first[i], c.first[j]);
             const int4 rj = c.rng[j];
             int64_t hl, hh;
             range_hint<W>(ri.x, ri.y, rj.z, rj.w, hl, hh);
-            Val<W> t, aij;
+            Val<W> t;
             dot_ok = group_dot<W, G, ROW_THREADS>(lo, j, gl, rec_at<W>((const uint32_t *)c.r, LdlCol<W>::tri(i)),
                                                   1, rec_at<W>((const uint32_t *)c.w, LdlCol<W>::tri(j)), 1,
                                                   nullptr, false, slots, t, hl, hh);
-            if (dot_ok) {
-                load(rec_at<W>(c.a, i + j * c.ld), aij);
-                add_signed(aij, t, true, diff);
-            }
+            if (dot_ok) add_signed(aij, t, true, diff);
         }
     }
-    __syncthreads();  // RN(1 / d_j) in dinv_sh
+    __syncthreads();  // RN(1 / d_j) in dinv_sh, pivot j + 1's windows
     if (!active) return;
     if (!dot_ok) {
         fail(2);
@@ -1026,6 +1076,25 @@ __global__ __launch_bounds__(2 * ROW_THREADS) void ldl_col_kernel(
     if (i != j + 1) return;
     int64_t hl, hh;
     range_hint<W>(ri.x, ri.y, ri.z, ri.w, hl, hh);
+    const int64_t plo = pre_lo;
+    if constexpr (G == 64) {
+        if (plo != INT64_MIN) {  // pivot j + 1: the pre-accumulated windows plus the term v wv
+            bool fits = true;
+            if (!is_zero(v) && !is_zero(wv)) {
+                const int64_t sc = (int64_t)v.e + wv.e - 64 * W;
+                fits = sc >= plo && (sc - plo) / 32 + 2 * W + 4 <= SLOTS;
+                if (fits && lane == 0) lds_add_product<W, ROW_THREADS>(pre_win, plo, v, wv, false);
+            }
+            if (fits) {
+                Val<W> t;
+                wave_finish<W, SLOTS, ROW_THREADS>(pre_win, lane, SLOTS, plo, t);
+                const int code = ldl_col_pivot<W, G>(c, i, gl, hl, hh, slots, signs, sign_mode, B.sign_off,
+                                                     reg_eps, reg_delta, status, &t);
+                if (code) fail(code);
+                return;
+            }
+        }
+    }
     const int code = ldl_col_pivot<W, G>(c, i, gl, hl, hh, slots, signs, sign_mode, B.sign_off, reg_eps,
                                          reg_delta, status);
     if (code) fail(code);
@@ -1305,8 +1374,8 @@ void mp_sampled_entries(int limbs, const void *G, const void *w, int64_t K, void
 void mp_exp_ranges(int limbs, const void *L, int64_t ldl, int64_t n, int mode, void *out,
                    hipStream_t st) {
     if (n == 0) return;
-    SDPX_DISPATCH_W(limbs, hipLaunchKernelGGL(exp_ranges_kernel<W>, dim3(blocks_for(n, 64)), dim3(64),
-                                              0, st, (const uint32_t *)L, ldl, n, mode, (int2 *)out));
+    SDPX_DISPATCH_W(limbs, hipLaunchKernelGGL(exp_ranges_kernel<W>, dim3((unsigned)n), dim3(64), 0, st,
+                                              (const uint32_t *)L, ldl, n, mode, (int2 *)out));
     HIP_CHECK(hipGetLastError());
 }
 
@@ -1414,10 +1483,10 @@ void mp_ldl_dots(int limbs, void *A, void *dinv, const int32_t *first, const int
     SDPX_DISPATCH_W(limbs, {
         SDPX_DISPATCH_G({
             auto kern = ldl_col_kernel<W, G>;
-            const size_t lds = panel_lds<W>((const void *)kern, ROW_THREADS);
+            const size_t lds = panel_lds<W>((const void *)kern, 2 * ROW_THREADS);  // row groups, pivot windows
             for (int64_t j = -1; j < nmax - 1; j++) {
                 const int64_t rows = j < 0 ? 1 : nmax - j - 1;
-                hipLaunchKernelGGL(kern, dim3(blocks_for(rows * G, ROW_THREADS), nblocks), dim3(2 * ROW_THREADS), lds,
+                hipLaunchKernelGGL(kern, dim3(blocks_for(rows * G, ROW_THREADS), nblocks), dim3(3 * ROW_THREADS), lds,
                                    st, (uint32_t *)A, (uint32_t *)dinv, first, signs, sign_mode,
                                    (const LdlBlock *)blocks, (uint32_t *)scratch, (const uint32_t *)reg_eps,
                                    (const uint32_t *)reg_delta, status, j);
