@@ -186,6 +186,36 @@ def test_full_size_fixed_point_spot_check(gpu):
     assert ok and out2.tobytes() == out.tobytes()
 
 
+@pytest.mark.parametrize("m,n,k,upper", [(2100, 2300, 1100, False), (2050, 2050, 1030, True)])
+def test_large_product_blocks_spot_check(gpu, m, n, k, upper):
+    """Products large enough for the 256 x 256 one-digit blocks, with ragged
+    edges (m, n not multiples of 256) and the packed upper Gram form: the
+    corners and 700 sampled entries bit for bit against exact dots."""
+    rng = np.random.default_rng(m + n + k)
+    nl = 4
+    print("large product", m, n, k, upper, flush=True)
+    if upper:
+        a = mp.random_full(k * n, nl, 30, rng)
+        ok, out = gpu.rns_gemm("T", "N", n, n, k, a, k, a, k, upper_only=True)
+        assert ok
+        cols = np.concatenate([[0, n - 1, n - 1, 255, 256], rng.integers(0, n, 700)])
+        rows = np.concatenate([[0, 0, n - 1, 255, 255], rng.integers(0, n, 700)])
+        rows = np.minimum(rows, cols)  # upper entries
+        ref = oracle_entries("T", "N", k, a, k, a, k, rows, cols)
+        got = out[cols * (cols + 1) // 2 + rows]
+    else:
+        a = mp.random_full(m * k, nl, 30, rng)
+        b = mp.random_full(k * n, nl, 30, rng)
+        ok, out = gpu.rns_gemm("N", "N", m, n, k, a, m, b, k)
+        assert ok
+        rows = np.concatenate([[0, m - 1, 0, m - 1, 255, 256], rng.integers(0, m, 700)])
+        cols = np.concatenate([[0, 0, n - 1, n - 1, 256, 255], rng.integers(0, n, 700)])
+        ref = oracle_entries("N", "N", k, a, m, b, k, rows, cols)
+        got = out[rows + cols * m]
+    bad = ~mp.same_bits(got, ref)
+    assert not bad.any(), f"{bad.sum()} sampled entries differ"
+
+
 def test_grouped_blocks_upper(gpu):
     # xgemm_blocks_upper_exact: uneven blocks on sorted global columns, the
     # exact sum over all blocks rounded once (rns_blas.rs:2671-2922)
