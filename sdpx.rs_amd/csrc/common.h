@@ -44,15 +44,28 @@ struct Workspace {
     void release_all();
 };
 
+// Pinned host staging for small uploads on the library stream: a ring whose
+// wrap first waits for the stream, so staged bytes are never overwritten
+// before their copy has run.
+struct PinnedRing {
+    uint8_t *base = nullptr;
+    size_t size = 0, top = 0;
+};
+
 // The library's device state: current device, its stream and workspace.
 struct Device {
     int id = -1;
     hipStream_t stream = nullptr;
     Workspace ws;
+    PinnedRing pin;
     std::recursive_mutex mu;
 };
 Device &device();  // initialised on first use (device 0 unless selected)
 void device_select(int id);
+// Host-to-device copy of `bytes` from src (any host memory, free to reuse on
+// return) into device memory dst, queued on the library stream without a
+// wait.
+void upload_async(void *dst, const void *src, size_t bytes);
 
 // Value layout helpers: a value is 2N+2 u32 words (limbs, kind, exponent).
 inline size_t value_words(int n) { return 2 * (size_t)n + 2; }

@@ -38,6 +38,30 @@ void Workspace::release_all() {
 
 static Device g_device;
 
+void upload_async(void *dst, const void *src, size_t bytes) {
+    if (bytes == 0) return;
+    Device &d = device();
+    constexpr size_t RING = 4u << 20;
+    if (bytes > RING / 4) {  // large: a plain copy, waited for
+        HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, d.stream));
+        HIP_CHECK(hipStreamSynchronize(d.stream));
+        return;
+    }
+    if (!d.pin.base) {
+        HIP_CHECK(hipHostMalloc((void **)&d.pin.base, RING, hipHostMallocDefault));
+        d.pin.size = RING;
+        d.pin.top = 0;
+    }
+    size_t off = (d.pin.top + 255) & ~size_t(255);
+    if (off + bytes > d.pin.size) {  // wrap: every earlier staged copy has run
+        HIP_CHECK(hipStreamSynchronize(d.stream));
+        off = 0;
+    }
+    std::memcpy(d.pin.base + off, src, bytes);
+    HIP_CHECK(hipMemcpyAsync(dst, d.pin.base + off, bytes, hipMemcpyHostToDevice, d.stream));
+    d.pin.top = off + bytes;
+}
+
 static void init_device(int id) {
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
@@ -67,6 +91,9 @@ void device_select(int id) {
     if (g_device.id == id) return;
     if (g_device.id >= 0) {
         g_device.ws.release_all();
+        if (g_device.stream) (void)hipStreamSynchronize(g_device.stream);
+        if (g_device.pin.base) (void)hipHostFree(g_device.pin.base);
+        g_device.pin = PinnedRing{};
         if (g_device.stream) (void)hipStreamDestroy(g_device.stream);
         g_device.stream = nullptr;
     }

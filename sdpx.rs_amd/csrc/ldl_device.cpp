@@ -50,13 +50,12 @@ struct DCtx {
     uint8_t *at(uint8_t *p, size_t idx) const { return p + idx * vb; }
 };
 
+// (uploads go through pinned staging, so no wait: arena memory released
+// at a reset is only reused by work queued later on the same stream)
 int32_t *upload_ints(DCtx &c, const std::vector<size_t> &v) {
     std::vector<int32_t> h(v.begin(), v.end());
     int32_t *d = (int32_t *)c.arena->alloc(sizeof(int32_t) * std::max<size_t>(h.size(), 1));
-    if (!h.empty())
-        HIP_CHECK(hipMemcpyAsync(d, h.data(), sizeof(int32_t) * h.size(), hipMemcpyHostToDevice, c.st));
-    // the host vector dies before the copy completes: wait
-    HIP_CHECK(hipStreamSynchronize(c.st));
+    upload_async(d, h.data(), sizeof(int32_t) * h.size());
     return d;
 }
 
@@ -75,28 +74,23 @@ void product(DCtx &c, char ta, char tb, size_t m, size_t n, size_t k, const uint
     c.stats->gpu_mp_macs += (double)k * (packed ? n * (n + 1) / 2.0 : (double)m * n);
 }
 
-// ldl_dots of one block on the GPU; returns 1 ok, 0 failed pivot.
-int dots_device(DCtx &c, uint8_t *l, size_t ld, size_t n, const std::vector<size_t> &first,
-                size_t sign_off, uint8_t *dinv) {
+// ldl_dots of one block on the GPU, queued without a wait: a failed pivot
+// (status 1) or a dot beyond its window (status 2) stays in d_status[0]
+// (the kernels keep the largest code and every later launch returns at
+// once), and the caller reads the outcome once when the whole factorization
+// is queued.
+void dots_device(DCtx &c, uint8_t *l, size_t ld, size_t n, const std::vector<size_t> &first,
+                 size_t sign_off, uint8_t *dinv) {
     size_t mark = c.arena->top;
     int32_t *d_first = upload_ints(c, first);
     LdlBlockDesc bd{(int64_t)0, (int64_t)ld, (int64_t)n, 0, 0, (int64_t)sign_off, 0};
     LdlBlockDesc *d_bd = (LdlBlockDesc *)c.arena->alloc(sizeof(LdlBlockDesc));
-    HIP_CHECK(hipMemcpyAsync(d_bd, &bd, sizeof bd, hipMemcpyHostToDevice, c.st));
+    upload_async(d_bd, &bd, sizeof bd);
     const size_t need = ldl_scratch_records(n) * c.vb;
     if (need > c.ldl_scratch_bytes) throw Error(SDPX_GPU_DEVICE_ERROR, "ldl scratch too small");
-    HIP_CHECK(hipMemsetAsync(c.d_status, 0, sizeof(int), c.st));
     mp_ldl_dots(c.n, l, dinv, d_first, c.d_signs, c.sign_mode, d_bd, 1, (int64_t)n, c.ldl_scratch, c.d_reg_eps,
                 c.d_reg_delta, c.d_status, c.st);
-    int status[2] = {0, 0};
-    HIP_CHECK(hipMemcpyAsync(status, c.d_status, sizeof(int) * 2, hipMemcpyDeviceToHost, c.st));
-    HIP_CHECK(hipStreamSynchronize(c.st));
     c.arena->top = mark;
-    if (status[0] == 2) {
-        c.declined = true;  // window overflow: host path
-        return 1;
-    }
-    return status[0] == 0 ? 1 : 0;
 }
 
 void solve_t_device(DCtx &c, size_t m, size_t n, uint8_t *L, size_t ldl, uint8_t *B, size_t ldb,
@@ -109,7 +103,6 @@ void solve_t_device(DCtx &c, size_t m, size_t n, uint8_t *L, size_t ldl, uint8_t
         void *work = c.arena->alloc(4 * (m + 1));
         mp_exp_ranges(c.n, L, ldl, n, 0, ranges, c.st);
         mp_trsm_base(c.n, L, ldl, n, B, ldb, m, d_low, ranges, c.d_status + 2, work, c.st);
-        HIP_CHECK(hipStreamSynchronize(c.st));
         c.arena->top = mark;
         c.stats->base_rows += m;
         return;
@@ -127,7 +120,6 @@ void solve_t_device(DCtx &c, size_t m, size_t n, uint8_t *L, size_t ldl, uint8_t
     if (c.declined) return;
     uint8_t *B2 = c.at(B, n1 * ldb);
     mp_ew_sub_rect(c.n, B2, ldb, p, m, m, n2, c.st);
-    HIP_CHECK(hipStreamSynchronize(c.st));
     c.arena->top = mark;
     solve_t_device(c, m, n2, c.at(L, n1 + n1 * ldl), ldl, B2, ldb, low2);
 }
@@ -153,15 +145,18 @@ void solve_n_device(DCtx &c, size_t m, size_t n, uint8_t *L, size_t ldl, uint8_t
     product(c, 'N', 'N', m, n1, n2, B2, ldb, c.at(L, n1), ldl, p, false);
     if (c.declined) return;
     mp_ew_sub_rect(c.n, B, ldb, p, m, m, n1, c.st);
-    HIP_CHECK(hipStreamSynchronize(c.st));
     c.arena->top = mark;
     solve_n_device(c, m, n1, L, ldl, B, ldb);
 }
 
-int rec_device(DCtx &c, uint8_t *l, size_t ld, size_t n, const std::vector<size_t> &first,
-               size_t sign_off, uint8_t *dinv) {
-    if (c.declined) return 1;
-    if (n <= 128) return dots_device(c, l, ld, n, first, sign_off, dinv);
+// ldl_rec on the GPU, queued; the outcome is in d_status (see dots_device)
+void rec_device(DCtx &c, uint8_t *l, size_t ld, size_t n, const std::vector<size_t> &first,
+                size_t sign_off, uint8_t *dinv) {
+    if (c.declined) return;
+    if (n <= 128) {
+        dots_device(c, l, ld, n, first, sign_off, dinv);
+        return;
+    }
     size_t low = n, cut = 0;
     bool have = false;
     for (size_t cc = n - 1; cc >= 1; cc--) {
@@ -175,15 +170,14 @@ int rec_device(DCtx &c, uint8_t *l, size_t ld, size_t n, const std::vector<size_
     if (have) {
         std::vector<size_t> f1(first.begin(), first.begin() + cut), tail(n - cut);
         for (size_t i = cut; i < n; i++) tail[i - cut] = first[i] - cut;
-        int ok1 = rec_device(c, l, ld, cut, f1, sign_off, dinv);
-        int ok2 = rec_device(c, c.at(l, cut + cut * ld), ld, n - cut, tail, sign_off + cut,
-                             c.at(dinv, cut));
-        return ok1 && ok2;
+        rec_device(c, l, ld, cut, f1, sign_off, dinv);
+        rec_device(c, c.at(l, cut + cut * ld), ld, n - cut, tail, sign_off + cut, c.at(dinv, cut));
+        return;
     }
     const size_t n1 = n / 2, n2 = n - n / 2;
     std::vector<size_t> f1(first.begin(), first.begin() + n1);
-    if (!rec_device(c, l, ld, n1, f1, sign_off, dinv)) return 0;
-    if (c.declined) return 1;
+    rec_device(c, l, ld, n1, f1, sign_off, dinv);
+    if (c.declined) return;
     size_t mark = c.arena->top;
     uint8_t *l11 = (uint8_t *)c.arena->alloc(n1 * n1 * c.vb);
     HIP_CHECK(hipMemcpy2DAsync(l11, n1 * c.vb, l, ld * c.vb, n1 * c.vb, n1, hipMemcpyDeviceToDevice, c.st));
@@ -191,21 +185,46 @@ int rec_device(DCtx &c, uint8_t *l, size_t ld, size_t n, const std::vector<size_
     for (size_t i = 0; i < n2; i++) lowv[i] = std::min(first[n1 + i], n1);
     uint8_t *a21 = c.at(l, n1);
     solve_t_device(c, n2, n1, l11, n1, a21, ld, lowv);
-    if (c.declined) return 1;
+    if (c.declined) return;
     c.arena->top = mark;  // l11 no longer read (stream order)
     uint8_t *w = (uint8_t *)c.arena->alloc(n2 * n1 * c.vb);
     HIP_CHECK(hipMemcpy2DAsync(w, n2 * c.vb, a21, ld * c.vb, n2 * c.vb, n1, hipMemcpyDeviceToDevice, c.st));
     mp_ew_mul_diag(c.n, a21, ld, dinv, a21, ld, n2, n1, false, c.st);
     uint8_t *packed = (uint8_t *)c.arena->alloc(n2 * (n2 + 1) / 2 * c.vb);
     product(c, 'N', 'T', n2, n2, n1, w, n2, a21, ld, packed, true);
-    if (c.declined) return 1;
+    if (c.declined) return;
     uint8_t *a22 = c.at(l, n1 + n1 * ld);
     mp_ew_sub_packed(c.n, a22, ld, packed, n2, 1, c.st);
-    HIP_CHECK(hipStreamSynchronize(c.st));
     c.arena->top = mark;
     std::vector<size_t> tail(n2);
     for (size_t i = 0; i < n2; i++) tail[i] = first[n1 + i] > n1 ? first[n1 + i] - n1 : 0;
-    return rec_device(c, a22, ld, n2, tail, sign_off + n1, c.at(dinv, n1));
+    rec_device(c, a22, ld, n2, tail, sign_off + n1, c.at(dinv, n1));
+}
+
+// The envelopes of several square blocks (pointer, ld, order) with one
+// read-back.
+std::vector<std::vector<size_t>> envelopes_device(DCtx &c, const std::vector<const uint8_t *> &ls,
+                                                  const std::vector<size_t> &lds,
+                                                  const std::vector<size_t> &ns) {
+    size_t mark = c.arena->top, total = 0;
+    for (size_t n : ns) total += n;
+    int32_t *d = (int32_t *)c.arena->alloc(sizeof(int32_t) * std::max<size_t>(total, 1));
+    size_t off = 0;
+    for (size_t b = 0; b < ls.size(); b++) {
+        mp_envelope(c.n, ls[b], lds[b], ns[b], d + off, c.st);
+        off += ns[b];
+    }
+    std::vector<int32_t> h(total);
+    if (total) HIP_CHECK(hipMemcpyAsync(h.data(), d, sizeof(int32_t) * total, hipMemcpyDeviceToHost, c.st));
+    HIP_CHECK(hipStreamSynchronize(c.st));
+    c.arena->top = mark;
+    std::vector<std::vector<size_t>> out;
+    off = 0;
+    for (size_t n : ns) {
+        out.emplace_back(h.begin() + off, h.begin() + off + n);
+        off += n;
+    }
+    return out;
 }
 
 std::vector<size_t> envelope_device(DCtx &c, const uint8_t *l, size_t ld, size_t n) {
@@ -254,11 +273,10 @@ struct DeviceSetup {
         }
         HIP_CHECK(hipStreamSynchronize(ctx.st));
     }
-    int regularized() {
-        int s[4];
+    // {worst ldl code, regularized pivots, window overflow of the row kernels, -}
+    void status(int (&s)[4]) {
         HIP_CHECK(hipMemcpyAsync(s, ctx.d_status, sizeof s, hipMemcpyDeviceToHost, ctx.st));
         HIP_CHECK(hipStreamSynchronize(ctx.st));
-        return s[1];
     }
     bool window_overflow() {
         int s[4];
@@ -284,14 +302,15 @@ int ldl_factor_device(int limbs, uint8_t *d_l, size_t ld, size_t n, const int8_t
                    reg_eps, reg_delta, &stats);
     DCtx &c = ds.ctx;
     std::vector<size_t> first = envelope_device(c, d_l, ld, n);
-    int ok;
     if (recursive)
-        ok = rec_device(c, d_l, ld, n, first, 0, d_dinv);
+        rec_device(c, d_l, ld, n, first, 0, d_dinv);
     else
-        ok = dots_device(c, d_l, ld, n, first, 0, d_dinv);
-    if (c.declined || ds.window_overflow()) return -1;  // caller reruns on the host
-    stats.regularized += ds.regularized();
-    return ok;
+        dots_device(c, d_l, ld, n, first, 0, d_dinv);
+    int s[4];
+    ds.status(s);
+    if (c.declined || s[2] != 0 || s[0] == 2) return -1;  // caller reruns on the host
+    stats.regularized += s[1];
+    return s[0] == 0 ? 1 : 0;
 }
 
 int ldl_solve_device(int limbs, const uint8_t *d_l, size_t ldl, size_t n, const uint8_t *d_dinv,
@@ -353,30 +372,33 @@ int sampled_arrow_device(int limbs, const std::vector<SampledLeafDev> &leaves, s
         product(c, 'T', 'N', lf.k, lf.k, lf.h, V, lf.h, V, lf.h, G, true);
         if (c.declined) break;
         mp_sampled_entries(limbs, G, lf.w, (int64_t)lf.k, c.at(d_l_all, loff[i]), s);
-        HIP_CHECK(hipStreamSynchronize(s));
         c.arena->top = mark;
     }
     HIP_CHECK(hipEventRecord(ev[1], s));
     if (!c.declined) {
         // 2. every leaf factor in one launch, one workgroup per leaf
         std::vector<size_t> first_cat;
-        std::vector<std::vector<size_t>> fl;
         std::vector<LdlBlockDesc> bds;
+        std::vector<const uint8_t *> lp;
+        std::vector<size_t> lk;
+        for (size_t i = 0; i < leaves.size(); i++) {
+            lp.push_back(c.at(d_l_all, loff[i]));
+            lk.push_back(leaves[i].k);
+        }
+        const std::vector<std::vector<size_t>> fl = envelopes_device(c, lp, lk, lk);
         for (size_t i = 0; i < leaves.size(); i++) {
             const size_t K = leaves[i].k;
-            fl.push_back(envelope_device(c, c.at(d_l_all, loff[i]), K, K));
             bds.push_back(LdlBlockDesc{(int64_t)loff[i], (int64_t)K, (int64_t)K, (int64_t)doff[i],
                                        (int64_t)first_cat.size(), 0, (int64_t)(i * per_leaf)});
-            first_cat.insert(first_cat.end(), fl.back().begin(), fl.back().end());
+            first_cat.insert(first_cat.end(), fl[i].begin(), fl[i].end());
         }
         size_t mark = c.arena->top;
         int32_t *d_first = upload_ints(c, first_cat);
         LdlBlockDesc *d_bd = (LdlBlockDesc *)c.arena->alloc(sizeof(LdlBlockDesc) * bds.size());
-        HIP_CHECK(hipMemcpyAsync(d_bd, bds.data(), sizeof(LdlBlockDesc) * bds.size(),
-                                 hipMemcpyHostToDevice, s));
+        upload_async(d_bd, bds.data(), sizeof(LdlBlockDesc) * bds.size());
         int8_t *plus = (int8_t *)c.arena->alloc(1);
         const int8_t one = 1;
-        HIP_CHECK(hipMemcpyAsync(plus, &one, 1, hipMemcpyHostToDevice, s));
+        upload_async(plus, &one, 1);
         HIP_CHECK(hipMemsetAsync(c.d_status, 0, sizeof(int) * 2, s));
         mp_ldl_dots(limbs, d_l_all, d_dinv_all, d_first, plus, 1, d_bd, (int)bds.size(), (int64_t)kmax,
                     c.ldl_scratch, nullptr, nullptr, c.d_status, s);
@@ -407,7 +429,7 @@ int sampled_arrow_device(int limbs, const std::vector<SampledLeafDev> &leaves, s
                                      (int64_t)doff[i]});
             }
             FwdLeaf *d_fd = (FwdLeaf *)c.arena->alloc(sizeof(FwdLeaf) * fd.size());
-            HIP_CHECK(hipMemcpyAsync(d_fd, fd.data(), sizeof(FwdLeaf) * fd.size(), hipMemcpyHostToDevice, s));
+            upload_async(d_fd, fd.data(), sizeof(FwdLeaf) * fd.size());
             mp_forward_rows(limbs, d_l_all, d_fc, d_y_all, ranges, d_fd, (int)fd.size(), (int64_t)t,
                             c.d_status + 2, s);
             uint8_t *Z_all = (uint8_t *)c.arena->alloc(std::max<size_t>(ksum, 1) * t * vb);
@@ -421,17 +443,17 @@ int sampled_arrow_device(int limbs, const std::vector<SampledLeafDev> &leaves, s
                 if (c.declined) break;
                 mp_ew_sub_packed(limbs, d_border, t, P, t, 0, s);
             }
-            HIP_CHECK(hipStreamSynchronize(s));  // fd's host copy and the arena
             c.arena->top = mark3;
         }
     }
     HIP_CHECK(hipEventRecord(ev[2], s));
-    int ok = 1;
     if (!c.declined && !ds.window_overflow()) {
         mp_mirror_upper(limbs, d_border, t, t, s);
         std::vector<size_t> first = envelope_device(c, d_border, t, t);
-        ok = t >= 256 ? rec_device(c, d_border, t, t, first, 0, d_border_dinv)
-                      : dots_device(c, d_border, t, t, first, 0, d_border_dinv);
+        if (t >= 256)
+            rec_device(c, d_border, t, t, first, 0, d_border_dinv);
+        else
+            dots_device(c, d_border, t, t, first, 0, d_border_dinv);
     }
     HIP_CHECK(hipEventRecord(ev[3], s));
     HIP_CHECK(hipEventSynchronize(ev[3]));
@@ -444,8 +466,10 @@ int sampled_arrow_device(int limbs, const std::vector<SampledLeafDev> &leaves, s
     st.leaves_s = b / 1e3;
     st.border_s = d / 1e3;
     st.gpu_products += ls.gpu_products;
-    if (c.declined || ds.window_overflow()) return -1;
-    return ok;
+    int sv[4];
+    ds.status(sv);
+    if (c.declined || sv[2] != 0 || sv[0] == 2) return -1;
+    return sv[0] == 0 ? 1 : 0;
 }
 
 }  // namespace sdpx

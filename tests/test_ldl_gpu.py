@@ -48,6 +48,24 @@ def test_random_gpu_products(gpu, nl, n, kind):
     assert np.all(mp.same_bits(dg, dinv))
 
 
+@pytest.mark.parametrize("n", [300, 100])
+def test_failed_pivot_reported(gpu, n):
+    """A zero row and column make one pivot exactly zero: the device path
+    (recursive at 300, one ldl_dots block at 100) reports the failed pivot
+    as the oracle does, after queueing the whole factorization."""
+    nl = 4
+    rng = np.random.default_rng(n)
+    g = rng.standard_normal((n, n))
+    s = g @ g.T / n + np.eye(n)
+    z = n // 2 + 3
+    s[z, :] = 0.0
+    s[:, z] = 0.0
+    A = mp.from_float64(s.T.reshape(-1), nl)
+    ok, _, _, _ = oracle_ldl(A, n, threads=16)
+    okg, _, _ = sdpx_gpu.mp_ldl(A, n)
+    assert not ok and not okg
+
+
 _spread_case = {}
 
 
