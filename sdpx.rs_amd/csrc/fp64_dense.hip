@@ -307,9 +307,12 @@ void potrf_f64(double *A, int64_t n, int64_t lda, int *d_info, double *ws, size_
 }
 
 // Forward/back substitution with the factor for nrhs columns:
-// lower: solve L Y = B; upper (trans): solve L^T X = Y.  One workgroup per
-// 64-row diagonal block step, RHS columns spread over threads, then a GEMM
-// update of the remaining rows.
+// lower: solve L Y = B; upper (trans): solve L^T X = Y, one 64-row diagonal
+// block a launch (then a GEMM update of the remaining rows).  One wave a
+// right-hand side, lane i holding row i: step i divides row i by the
+// diagonal and every later (earlier, for the transpose) row subtracts its
+// multiple, so each row sees the same sequence of updates as a row-by-row
+// substitution; four columns a 256-thread workgroup.
 __global__ __launch_bounds__(256) void trsv_block(const double *L, int64_t ldl, double *B,
                                                   int64_t ldb, int64_t j0, int nb, int64_t nrhs,
                                                   int trans) {
@@ -320,42 +323,29 @@ __global__ __launch_bounds__(256) void trsv_block(const double *L, int64_t ldl, 
         s[i][j] = i >= j ? L[(j0 + i) + (j0 + j) * ldl] : 0.0;
     }
     __syncthreads();
-    int64_t c = (int64_t)blockIdx.x * 256 + tid;
-    if (c >= nrhs) return;
-    double x[NB];
-#pragma unroll
-    for (int i = 0; i < NB; i++) x[i] = i < nb ? B[(j0 + i) + c * ldb] : 0.0;
+    const int lane = tid & 63;
+    const int64_t c = (int64_t)blockIdx.x * 4 + (tid >> 6);
+    if (c >= nrhs) return;  // whole waves
+    double x = lane < nb ? B[(j0 + lane) + c * ldb] : 0.0;
     if (!trans) {
-#pragma unroll
-        for (int i = 0; i < NB; i++) {
-            if (i < nb) {
-                double v = x[i];
-#pragma unroll
-                for (int l = 0; l < NB; l++)
-                    if (l < i) v -= s[i][l] * x[l];
-                x[i] = v / s[i][i];
-            }
+        for (int i = 0; i < nb; i++) {
+            const double xi = __shfl(x, i, 64) / s[i][i];
+            if (lane == i) x = xi;
+            if (lane > i && lane < nb) x -= s[lane][i] * xi;
         }
     } else {
-#pragma unroll
-        for (int ii = NB - 1; ii >= 0; ii--) {
-            if (ii < nb) {
-                double v = x[ii];
-#pragma unroll
-                for (int l = 0; l < NB; l++)
-                    if (l > ii && l < nb) v -= s[l][ii] * x[l];
-                x[ii] = v / s[ii][ii];
-            }
+        for (int ii = nb - 1; ii >= 0; ii--) {
+            const double xi = __shfl(x, ii, 64) / s[ii][ii];
+            if (lane == ii) x = xi;
+            if (lane < ii) x -= s[ii][lane] * xi;
         }
     }
-#pragma unroll
-    for (int i = 0; i < NB; i++)
-        if (i < nb) B[(j0 + i) + c * ldb] = x[i];
+    if (lane < nb) B[(j0 + lane) + c * ldb] = x;
 }
 
 void trsm_lower_f64(const double *L, int64_t n, int64_t ldl, double *B, int64_t nrhs, int64_t ldb,
                     hipStream_t s) {
-    const unsigned cgrid = (unsigned)((nrhs + 255) / 256);
+    const unsigned cgrid = (unsigned)((nrhs + 3) / 4);
     for (int64_t j0 = 0; j0 < n; j0 += NB) {
         const int nb = (int)std::min<int64_t>(NB, n - j0);
         hipLaunchKernelGGL(trsv_block, dim3(cgrid), dim3(256), 0, s, L, ldl, B, ldb, j0, nb, nrhs, 0);
@@ -380,7 +370,7 @@ void trsm_lower_f64(const double *L, int64_t n, int64_t ldl, double *B, int64_t 
 
 void trsm_lower_t_f64(const double *L, int64_t n, int64_t ldl, double *B, int64_t nrhs,
                       int64_t ldb, hipStream_t s) {
-    const unsigned cgrid = (unsigned)((nrhs + 255) / 256);
+    const unsigned cgrid = (unsigned)((nrhs + 3) / 4);
     int64_t last = ((n - 1) / NB) * NB;
     for (int64_t j0 = last; j0 >= 0; j0 -= NB) {
         const int nb = (int)std::min<int64_t>(NB, n - j0);
