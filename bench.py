@@ -411,8 +411,95 @@ def rns_syrk_workload(args):
                          "kernel": "product_kernel", "int8_ops_per_step": tot["int8_ops"]}}
 
 
+def fp64_workload(args):
+    """Float64 forms: one IPM iteration's Schur build (M = svec(Rinv A_i
+    Rinv^T) . svec(Rinv A_j Rinv^T) on FP64 MFMA) + dense Cholesky of the
+    dense-SDP configurations, n = 200, m = 1000 (timed) and n = 50, m = 100,
+    and the adjoint with 64 cotangents through the same factor, all on
+    device-resident data.  Rinv = I + G / (4 sqrt n) stands for the NT
+    scaling factor of an iterate."""
+    import ctypes
+
+    import torch
+
+    import sdpx_gpu
+    from sdpx_gpu import ipm
+
+    L = sdpx_gpu.lib()
+    sdpx_gpu.check(L.sdpx_gpu_init(0))
+    vp = ctypes.c_void_p
+
+    def setup(n, m, seed):
+        prob = ipm.dense_problem(n, m, seed)
+        rng = np.random.default_rng(seed + 1)
+        rinv = np.eye(n) + rng.standard_normal((n, n)) / (4.0 * np.sqrt(n))
+        d_r = torch.tensor(np.asfortranarray(rinv).T.copy().reshape(-1), dtype=torch.float64, device="cuda")
+        d_a = torch.tensor(prob["a_svec"].T.copy().reshape(-1), dtype=torch.float64, device="cuda")
+        d_s = torch.zeros(m * m, dtype=torch.float64, device="cuda")
+        f = sdpx_gpu.DenseFactor(m)
+        return prob, rinv, d_r, d_a, d_s, f
+
+    def iteration(n, m, d_r, d_a, d_s, f):
+        sdpx_gpu.check(L.sdpx_gpu_schur_psd_f64_dev(n, m, vp(d_r.data_ptr()), vp(d_a.data_ptr()),
+                                                    vp(d_s.data_ptr()), m))
+        assert f.refactor_dev(d_s.data_ptr(), m)
+
+    def timed(fn, steps):
+        for _ in range(max(1, args.warmup)):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    out = {}
+    for n, m, seed in ((200, 1000, 4), (50, 100, 0)):
+        prob, rinv, d_r, d_a, d_s, f = setup(n, m, seed)
+        dt = timed(lambda: iteration(n, m, d_r, d_a, d_s, f), max(5, args.steps))
+        out[(n, m)] = (dt, prob, rinv, d_r, d_a, f)
+        progress(f"fp64 n={n} m={m}: {dt * 1e3:.3f} ms per Schur build + factor")
+    dt, prob, rinv, d_r, d_a, f = out[(200, 1000)]
+    n, m, s_len, r = 200, 1000, 200 * 201 // 2, 64
+    ybar = torch.tensor(np.random.default_rng(1).standard_normal((r, m)).reshape(-1), dtype=torch.float64,
+                        device="cuda")
+    d_bg = torch.zeros(m * r, dtype=torch.float64, device="cuda")
+    d_cg = torch.zeros(s_len * r, dtype=torch.float64, device="cuda")
+
+    def adjoint():
+        sdpx_gpu.check(L.sdpx_gpu_adjoint_psd_f64_dev(f._h, n, vp(d_r.data_ptr()), vp(d_a.data_ptr()), r,
+                                                      vp(ybar.data_ptr()), vp(d_bg.data_ptr()),
+                                                      vp(d_cg.data_ptr())))
+    adt = timed(adjoint, max(5, args.steps))
+    progress(f"adjoint, 64 cotangents: {adt * 1e3:.3f} ms")
+    flops = m * 4.0 * n ** 3 + m * (m + 1) * s_len + m ** 3 / 3.0
+    result = {
+        "value": dt, "unit": "s/IPM-iter", "ms_per_step": dt * 1e3, "dtype": "f64",
+        "config": {"workload": "Float64 dense SDP, one PSD block n=200, m=1000: Schur build (FP64 MFMA "
+                               "congruences + SYRK) + Cholesky per IPM iteration",
+                   "n": n, "m": m, "cfg_n50_m100_s_per_iter": out[(50, 100)][0],
+                   "adjoint_64_cotangents_s": adt, "flops_per_iter": flops},
+        "roofline": {"bound": "mfma", "achieved": flops / dt / 1e12, "peak": FP64_PEAK / 1e12,
+                     "unit": "TFLOP/s (FP64)", "frac": flops / dt / FP64_PEAK, "traffic": None,
+                     "kernel": "schur_psd_f64 + dense Cholesky (whole iteration)"},
+    }
+    if args.cpu_baseline:
+        A = prob["A"]
+        cores = os.cpu_count() or 1
+        c0 = time.perf_counter()
+        Ms = np.stack([ipm.svec(rinv @ A[i] @ rinv.T) for i in range(m)], axis=1)
+        M = Ms.T @ Ms
+        np.linalg.cholesky(M)
+        cdt = time.perf_counter() - c0
+        result["cpu_baseline"] = {"value": cdt, "unit": "s/IPM-iter", "cores": cores, "kind": "port",
+                                  "sample": "the same iteration (n=200, m=1000) in numpy/LAPACK, the "
+                                            "reference formula of kkt/condensed/psd.rs:1299-1456"}
+    return result
+
+
 WORKLOADS = {"rns_gemm": rns_gemm_workload, "sampled": sampled_workload,
-             "chol": chol_workload, "rns_syrk": rns_syrk_workload}
+             "chol": chol_workload, "rns_syrk": rns_syrk_workload, "fp64": fp64_workload}
 
 
 def main():
