@@ -224,10 +224,14 @@ __global__ __launch_bounds__(256) void potrf_diag(double *A, int64_t lda, int64_
         const double djj = s[j][j];
         for (int i = j + 1 + tid; i < nb; i += 256) s[i][j] /= djj;
         __syncthreads();
-        // trailing rank-1 update of the lower triangle
-        for (int e = tid; e < (nb - j - 1) * (nb - j - 1); e += 256) {
-            int i = j + 1 + e % (nb - j - 1), c = j + 1 + e / (nb - j - 1);
-            if (i >= c) s[i][c] -= s[i][j] * s[c][j];
+        // trailing rank-1 update of the lower triangle: thread (row i = tid %
+        // 64, columns c = tid / 64 + 4 t)
+        {
+            const int i = tid & 63;
+            if (i > j && i < nb) {
+                const double lij = s[i][j];
+                for (int c = j + 1 + (tid >> 6); c <= i; c += 4) s[i][c] -= lij * s[c][j];
+            }
         }
         __syncthreads();
     }
@@ -242,7 +246,10 @@ __global__ __launch_bounds__(256) void potrf_diag(double *A, int64_t lda, int64_
 }
 
 // X L^T = B for the rows below a diagonal block: row panel rows r0..r1,
-// columns j0..j0+nb; one thread per row, forward substitution.
+// columns j0..j0+nb.  One wave a row, lane j holding x_j: step j divides x_j
+// by the diagonal and every later lane subtracts its multiple, so each x_j
+// sees the updates of a row-by-row substitution in the same order; four
+// rows a 256-thread workgroup.
 __global__ __launch_bounds__(256) void trsm_panel(double *A, int64_t lda, int64_t j0, int nb,
                                                   int64_t r0, int64_t r1, const int *info) {
     __shared__ double L[NB][NB + 1];
@@ -253,24 +260,16 @@ __global__ __launch_bounds__(256) void trsm_panel(double *A, int64_t lda, int64_
         L[i][j] = i >= j ? A[(j0 + i) + (j0 + j) * lda] : 0.0;
     }
     __syncthreads();
-    int64_t row = r0 + (int64_t)blockIdx.x * 256 + tid;
-    if (row >= r1) return;
-    double x[NB];
-#pragma unroll
-    for (int j = 0; j < NB; j++) x[j] = j < nb ? A[row + (j0 + j) * lda] : 0.0;
-#pragma unroll
-    for (int j = 0; j < NB; j++) {
-        if (j < nb) {
-            double v = x[j];
-#pragma unroll
-            for (int l = 0; l < NB; l++)
-                if (l < j) v -= x[l] * L[j][l];
-            x[j] = v / L[j][j];
-        }
+    const int lane = tid & 63;
+    const int64_t row = r0 + (int64_t)blockIdx.x * 4 + (tid >> 6);
+    if (row >= r1) return;  // whole waves
+    double x = lane < nb ? A[row + (j0 + lane) * lda] : 0.0;
+    for (int j = 0; j < nb; j++) {
+        const double xj = __shfl(x, j, 64) / L[j][j];
+        if (lane == j) x = xj;
+        if (lane > j && lane < nb) x -= xj * L[lane][j];
     }
-#pragma unroll
-    for (int j = 0; j < NB; j++)
-        if (j < nb) A[row + (j0 + j) * lda] = x[j];
+    if (lane < nb) A[row + (j0 + lane) * lda] = x;
 }
 
 void potrf_f64(double *A, int64_t n, int64_t lda, int *d_info, double *ws, size_t ws_bytes,
@@ -283,7 +282,7 @@ void potrf_f64(double *A, int64_t n, int64_t lda, int *d_info, double *ws, size_
         HIP_CHECK(hipGetLastError());
         const int64_t r0 = j0 + nb;
         if (r0 >= n) break;
-        hipLaunchKernelGGL(trsm_panel, dim3((unsigned)((n - r0 + 255) / 256)), dim3(256), 0, s, A,
+        hipLaunchKernelGGL(trsm_panel, dim3((unsigned)((n - r0 + 3) / 4)), dim3(256), 0, s, A,
                            lda, j0, nb, r0, n, d_info);
         HIP_CHECK(hipGetLastError());
         // A22 -= P P^T (lower triangle)

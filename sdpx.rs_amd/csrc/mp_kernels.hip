@@ -1393,6 +1393,20 @@ void mp_forward_rows(int limbs, const void *L_all, const int32_t *first_all, voi
     HIP_CHECK(hipGetLastError());
 }
 
+// Workgroups of `lds` bytes that the device holds at once (CUs x per-CU LDS).
+static int64_t rows_resident(size_t lds) {
+    static int64_t cus = 0, lds_cu = 0;
+    if (!cus) {
+        int dev = 0, v = 0;
+        HIP_CHECK(hipGetDevice(&dev));
+        HIP_CHECK(hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev));
+        cus = v;
+        HIP_CHECK(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
+        lds_cu = v > 0 ? v : 65536;
+    }
+    return cus * std::max<int64_t>(1, lds_cu / (int64_t)lds);
+}
+
 // Tuning switches (sdpx_gpu_set_tuning): "trsm_rl" 0/1 turns the
 // right-looking pass of the triangular-solve base off/on (default on, or
 // SDPX_GPU_TRSM_RL=0 in the environment), "trsm_rl_room" sets its window's
@@ -1425,14 +1439,16 @@ void mp_trsm_base(int limbs, const void *L, int64_t ldl, int64_t n, void *B, int
                   int64_t m, const int32_t *low, const void *lranges, int *fail, void *work,
                   hipStream_t st) {
     if (m == 0 || n == 0) return;
-    const bool rl = trsm_right_looking() && n <= 128;
     int *fb_count = (int *)work;
     int32_t *fb_rows = (int32_t *)work + 1;
     SDPX_DISPATCH_W(limbs, {
+        // the right-looking form holds a window per column (twice the LDS of
+        // a left-looking row): it runs when all m rows fit on the chip at once
+        const size_t lds = slot_bytes<W>(RL_THREADS) + 2 * (W + 4) * sizeof(uint32_t);
+        const bool rl = trsm_right_looking() && n <= 128 && m <= rows_resident(lds);
         if (rl) {
             HIP_CHECK(hipMemsetAsync(fb_count, 0, sizeof(int), st));
             auto kr = trsm_rl_kernel<W>;
-            const size_t lds = slot_bytes<W>(RL_THREADS) + 2 * (W + 4) * sizeof(uint32_t);
             if (lds > 65536)
                 HIP_CHECK(hipFuncSetAttribute((const void *)kr, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
             hipLaunchKernelGGL(kr, dim3((unsigned)m), dim3(RL_THREADS), lds, st, (const uint32_t *)L, ldl, n,
