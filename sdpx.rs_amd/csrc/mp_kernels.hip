@@ -27,6 +27,7 @@
 #include "common.h"
 #include "mp_kernels.h"
 #include "mpdev.h"
+#include "wavemp.h"
 
 namespace sdpx {
 
@@ -279,9 +280,9 @@ __device__ __forceinline__ int64_t readlane64(int64_t v, int lane) {
 template <int W, int SLOTS>
 __device__ __forceinline__ void wave_round(int64_t T, int gl, int64_t base, Val<W> &out);
 
-template <int W, int SLOTS, int STRIDE>
-__device__ __forceinline__ void wave_finish(lds_i64 *ls, int gl, int used, int64_t base, Val<W> &out) {
-    static_assert(SLOTS <= 64, "one slot a lane");
+// Lane s's total of slot s over the wave's 64 windows (0 for s >= used).
+template <int STRIDE>
+__device__ __forceinline__ int64_t window_total(lds_i64 *ls, int gl, int used) {
     lds_i64 *g0 = ls - gl;  // slot 0 of lane 0
     int64_t T = 0;
     if (gl < used) {
@@ -292,7 +293,13 @@ __device__ __forceinline__ void wave_finish(lds_i64 *ls, int gl, int used, int64
             T += x.x + x.y;
         }
     }
-    wave_round<W, SLOTS>(T, gl, base, out);
+    return T;
+}
+
+template <int W, int SLOTS, int STRIDE>
+__device__ __forceinline__ void wave_finish(lds_i64 *ls, int gl, int used, int64_t base, Val<W> &out) {
+    static_assert(SLOTS <= 64, "one slot a lane");
+    wave_round<W, SLOTS>(window_total<STRIDE>(ls, gl, used), gl, base, out);
 }
 
 // The rounding half of wave_finish: lane s holds slot s's total (lanes at
@@ -419,6 +426,50 @@ __device__ __forceinline__ bool group_dot(int64_t k0, int64_t k1, int gl,
         wave_finish<W, SLOTS, STRIDE>(ls, gl, used, lo, out);
     else
         group_finish<W, G, SLOTS, STRIDE>(ls, gl, used, lo, out);
+    return true;
+}
+
+// group_dot for a whole wave (G = 64) without a subtrahend, its result
+// spread over the wave (wavemp.h): the same window, terms and rounding.
+template <int W, int STRIDE>
+__device__ __forceinline__ bool wave_dot(int64_t k0, int64_t k1, int gl, const uint32_t *xb, int64_t xs,
+                                         const uint32_t *yb, int64_t ys, int64_t *slots, wmp::WV<W> &out,
+                                         int64_t hint_lo, int64_t hint_hi) {
+    constexpr int SLOTS = Win<W>::SLOTS;
+    static_assert(SLOTS <= 64, "one slot a lane");
+    auto fx = [&](int64_t k) { return xb + k * xs * (W + 2); };
+    auto fy = [&](int64_t k) { return yb + k * ys * (W + 2); };
+    int64_t lo = INT64_MAX, hi = INT64_MIN;
+    if (hint_lo <= hint_hi && (hint_hi - hint_lo) / 32 + 2 * W + 4 <= SLOTS) {
+        lo = hint_lo;
+        hi = hint_hi;
+    } else {
+        const int64_t kf = k0 + (((int64_t)gl - k0) % 64 + 64) % 64;
+        for (int64_t k = kf; k < k1; k += 64) {
+            const uint32_t *x = fx(k), *y = fy(k);
+            const int32_t kx = (int32_t)x[W], ky = (int32_t)y[W];
+            if (kx == mpd::KIND_ZERO || kx == -mpd::KIND_ZERO || ky == mpd::KIND_ZERO ||
+                ky == -mpd::KIND_ZERO)
+                continue;
+            const int64_t sc = (int64_t)(int32_t)x[W + 1] + (int32_t)y[W + 1] - 64 * (int64_t)W;
+            lo = min(lo, sc);
+            hi = max(hi, sc);
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            lo = min(lo, (int64_t)__shfl_xor(lo, off, 64));
+            hi = max(hi, (int64_t)__shfl_xor(hi, off, 64));
+        }
+    }
+    if (lo == INT64_MAX) {
+        out = wmp::zero<W>();
+        return true;
+    }
+    if ((hi - lo) / 32 + 2 * W + 4 > SLOTS) return false;
+    const int used = (int)min((int64_t)SLOTS, (hi - lo) / 32 + 2 * W + 1);
+    lds_i64 *ls = (lds_i64 *)slots;
+    group_accumulate<W, 64, STRIDE>(k0, k1, gl, xb, xs, yb, ys, false, ls, used, lo);
+    out = wmp::round_window<W, SLOTS>(window_total<STRIDE>(ls, gl, used), lo);
     return true;
 }
 
@@ -563,20 +614,13 @@ __global__ __launch_bounds__(RL_THREADS) void trsm_rl_kernel(
         lv = ln;
         if (j + 1 < n) load_l(j + 1, ln);
         const uint32_t *pw = post + (j & 1) * PW;
-        Val<W> bj, t, x;
-#pragma unroll
-        for (int u = 0; u < W; u++) bj.m[u] = __builtin_amdgcn_readfirstlane(pw[u]);
-        bj.kind = __builtin_amdgcn_readfirstlane((int)pw[W]);
-        bj.e = __builtin_amdgcn_readfirstlane((int)pw[W + 1]);
         const int64_t basej = (int64_t)(((uint64_t)__builtin_amdgcn_readfirstlane(pw[W + 3]) << 32) |
                                         (uint32_t)__builtin_amdgcn_readfirstlane(pw[W + 2]));
-        if (basej == INT64_MIN) {
-            set_zero(t);
-        } else {
-            const int64_t T = wl < SLOTS ? (int64_t)win[wl * RL_THREADS + j] : 0;
-            wave_round<W, SLOTS>(T, wl, basej, t);
-        }
-        add_signed(bj, t, true, x);  // x_j = RN(b_j - t), the same in every thread
+        // x_j = RN(b_j - t), the same in every thread: formed spread over each wave
+        const wmp::WV<W> t = basej == INT64_MIN
+                                 ? wmp::zero<W>()
+                                 : wmp::round_window<W, SLOTS>(wl < SLOTS ? (int64_t)win[wl * RL_THREADS + j] : 0, basej);
+        const Val<W> x = wmp::to_val(wmp::add(wmp::load<W>(pw), t, true));
         if (c == j) xo = x;
         if (!is_zero(x) && !is_zero(lv)) {  // lv is zero for c <= j and c >= n
             const int64_t sc = (int64_t)x.e + lv.e - 64 * W;
@@ -953,6 +997,39 @@ __device__ int ldl_col_pivot(const LdlCol<W> &c, int64_t j, int gl, int64_t hl, 
     return 0;
 }
 
+// ldl_col_pivot for a whole wave from the RN of the exact dot t spread over
+// the wave, with a_jj (and the regularization's eps and delta) loaded
+// beforehand.  Returns 3 when the pivot cancels to zero (the caller then
+// takes ldl_col_pivot's whole-dot path, which rounds the exact difference
+// once).
+template <int W>
+__device__ __forceinline__ int ldl_col_pivot_wave(const LdlCol<W> &c, int64_t j, const wmp::WV<W> &t,
+                                                  const wmp::WV<W> &ajj, bool reg, const wmp::WV<W> &eps,
+                                                  const wmp::WV<W> &delta, int sg, int *status) {
+    wmp::WV<W> d = wmp::add(ajj, t, true);
+    if (wmp::is_zero(d) && j > c.first[j]) return 3;
+    if (!wmp::is_regular(d) && !wmp::is_zero(d)) return 1;
+    if (reg) {
+        wmp::WV<W> ds = d;
+        if (sg < 0 && wmp::is_regular(ds)) ds.kind = -ds.kind;
+        // RN(ds - eps) < 0 exactly when ds < eps
+        const bool below = wmp::is_regular(eps) || wmp::is_zero(eps)
+                               ? wmp::compare(ds, eps) < 0
+                               : [&] {
+                                     const wmp::WV<W> diff = wmp::add(ds, eps, true);
+                                     return !wmp::is_zero(diff) && diff.kind < 0;
+                                 }();
+        if (below) {
+            d = delta;
+            if (sg < 0 && wmp::is_regular(d)) d.kind = -d.kind;
+            if (wmp::lane() == 0) atomicAdd(status + 1, 1);  // regularized count
+        }
+    }
+    if (wmp::is_zero(d)) return 1;
+    wmp::store(rec_at<W>(c.dstore, j), d);  // RN(1/d) is formed by the next launch
+    return 0;
+}
+
 // Launch j: the first wave of a workgroup holds the row lane groups; the
 // second forms RN(1 / d_j) meanwhile (the reciprocal the rows' entries need
 // only after their dots), hands it over through LDS and, in workgroup 0 of
@@ -1097,6 +1174,187 @@ __global__ __launch_bounds__(3 * ROW_THREADS) void ldl_col_kernel(
     }
     const int code = ldl_col_pivot<W, G>(c, i, gl, hl, hh, slots, signs, sign_mode, B.sign_off, reg_eps,
                                          reg_delta, status);
+    if (code) fail(code);
+}
+
+// ldl_col_kernel with one row a wave and the column's chain spread over the
+// wave (wavemp.h): the same three waves, launches, windows and roundings.
+// Every operand a wave needs is loaded before the workgroup reads the
+// status, so those loads overlap the launch's first barrier: the pivot
+// record d_j (waves 0 and 1), the row's entry, ranges and dot terms
+// k = lane, lane + 64 (wave 0; further terms are loaded in the dot), and
+// the next pivot's a_jj, eps and delta (wave 0 of workgroup 0).  The dot
+// runs over k in [0, j): the r and w entries left of a row's envelope are
+// stored zeros, which the exact dot skips.
+template <int W>
+__global__ __launch_bounds__(3 * ROW_THREADS) void ldl_col_wave_kernel(
+    uint32_t *A, uint32_t *dinv, const int32_t *first_all, const int8_t *signs, int sign_mode,
+    const LdlBlock *blocks, uint32_t *scratch, const uint32_t *reg_eps, const uint32_t *reg_delta,
+    int *status, int64_t j) {
+    constexpr int SLOTS = Win<W>::SLOTS;
+    extern __shared__ int64_t slots_sh[];  // [SLOTS][64] row windows, then [SLOTS][64] pivot windows
+    __shared__ uint32_t dinv_sh[W + 2];
+    __shared__ int stop;
+    __shared__ int64_t pre_lo;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (threadIdx.x == 0) stop = status[0];
+    const LdlBlock B = blocks[blockIdx.y];
+    const LdlCol<W> c(A, dinv, first_all, scratch, B);
+    const int64_t i = j + 1 + blockIdx.x;
+    int64_t *slots = slots_sh + lane;
+    lds_i64 *pre_win = (lds_i64 *)(slots_sh + SLOTS * ROW_THREADS) + lane;
+    // loads ahead of the status barrier (every address valid for 0 <= j < n)
+    const bool live = j >= 0 && j < B.n;
+    const bool row = live && wave == 0 && i < B.n;
+    const bool next = row && i == j + 1;  // this wave forms pivot j + 1
+    wmp::WV<W> dj = wmp::zero<W>(), aij = wmp::zero<W>(), ajj = wmp::zero<W>(), eps = wmp::zero<W>(),
+               delta = wmp::zero<W>();
+    Val<W> x0, y0, x1, y1;
+    int4 ri = make_int4(INT32_MAX, INT32_MIN, INT32_MAX, INT32_MIN), rj = ri;
+    int32_t fi = 0;
+    int sg = 1;
+    if (live && wave <= 1) dj = wmp::load<W>(rec_at<W>((const uint32_t *)c.dstore, j));
+    if (row) {
+        if (j > 0) ri = c.rng[i];
+        rj = c.rng[j];
+        fi = c.first[i];
+        aij = wmp::load<W>(rec_at<W>((const uint32_t *)c.a, i + j * c.ld));
+        const uint32_t *xr = rec_at<W>((const uint32_t *)c.r, LdlCol<W>::tri(i));
+        const uint32_t *yr = rec_at<W>((const uint32_t *)c.w, LdlCol<W>::tri(j));
+        if (lane < j) {
+            load(rec_at<W>(xr, lane), x0);
+            load(rec_at<W>(yr, lane), y0);
+        } else {
+            set_zero(x0);
+            set_zero(y0);
+        }
+        if (lane + 64 < j) {
+            load(rec_at<W>(xr, lane + 64), x1);
+            load(rec_at<W>(yr, lane + 64), y1);
+        } else {
+            set_zero(x1);
+            set_zero(y1);
+        }
+        if (next) {
+            ajj = wmp::load<W>(rec_at<W>((const uint32_t *)c.a, i + i * c.ld));
+            if (reg_eps) {
+                eps = wmp::load<W>(reg_eps);
+                delta = wmp::load<W>(reg_delta);
+            }
+            sg = sign_mode == 0 ? 1 : (sign_mode == 1 ? signs[0] : signs[B.sign_off + i]);
+        }
+    }
+    __syncthreads();
+    if (stop != 0) return;  // the whole workgroup
+    auto fail = [&](int code) {
+        if (lane == 0) atomicMax(status, code);
+    };
+    if (j < 0) {  // pivot 0: an empty dot
+        if (wave == 0 && i == 0) {
+            if (lane == 0) c.rng[0] = make_int4(INT32_MAX, INT32_MIN, INT32_MAX, INT32_MIN);
+            const int code = ldl_col_pivot<W, 64>(c, 0, lane, INT64_MAX, INT64_MIN, slots, signs, sign_mode,
+                                                  B.sign_off, reg_eps, reg_delta, status);
+            if (code) fail(code);
+        }
+        return;
+    }
+    if (j >= B.n) return;  // the whole workgroup: this block has no column j
+    if (wave == 1) {  // RN(1 / d_j) for this workgroup's rows; workgroup 0 stores it
+        const wmp::WV<W> di = wmp::recip(dj);
+        wmp::store(dinv_sh, di);
+        if (blockIdx.x == 0) wmp::store(rec_at<W>(c.dv, j), di);
+        __syncthreads();
+        return;
+    }
+    if (wave == 2) {  // pivot j + 1's dot over k < j into the pivot windows
+        int64_t lo_w = INT64_MIN;
+        const int64_t p = j + 1;
+        if (blockIdx.x == 0 && p < B.n && j > 0 && c.first[p] < j) {
+            const int4 rp = c.rng[p];  // row p's ranges over k < j
+            int64_t hl, hh;
+            range_hint<W>(rp.x, rp.y, rp.z, rp.w, hl, hh);
+            if (hl <= hh && (hh - hl + 256) / 32 + 2 * W + 4 <= SLOTS) {
+                lo_w = hl - 128;
+                group_accumulate<W, ROW_THREADS, ROW_THREADS>(
+                    c.first[p], j, lane, rec_at<W>((const uint32_t *)c.r, LdlCol<W>::tri(p)), 1,
+                    rec_at<W>((const uint32_t *)c.w, LdlCol<W>::tri(p)), 1, false, pre_win, SLOTS, lo_w);
+            }
+        }
+        if (lane == 0) pre_lo = lo_w;
+        __syncthreads();
+        return;
+    }
+    // wave 0: row i's entry (i, j)
+    wmp::WV<W> diff = wmp::zero<W>();
+    bool dot_ok = true;
+    const bool zero_row = !row || fi > j;
+    if (!zero_row) {
+        int64_t hl, hh;
+        range_hint<W>(ri.x, ri.y, rj.z, rj.w, hl, hh);
+        wmp::WV<W> t;
+        if (hl <= hh && (hh - hl) / 32 + 2 * W + 4 <= SLOTS) {  // the window from the ranges; terms preloaded
+            const int used = (int)min((int64_t)SLOTS, (hh - hl) / 32 + 2 * W + 1);
+            lds_i64 *ls = (lds_i64 *)slots;
+            for (int sl = 0; sl < used; sl++) ls[sl * ROW_THREADS] = 0;
+            if (!is_zero(x0) && !is_zero(y0)) lds_add_product<W, ROW_THREADS>(ls, hl, x0, y0, false);
+            if (!is_zero(x1) && !is_zero(y1)) lds_add_product<W, ROW_THREADS>(ls, hl, x1, y1, false);
+            for (int64_t k = lane + 128; k < j; k += 64) {
+                Val<W> x, y;
+                load(rec_at<W>((const uint32_t *)c.r, LdlCol<W>::tri(i) + k), x);
+                load(rec_at<W>((const uint32_t *)c.w, LdlCol<W>::tri(j) + k), y);
+                if (!is_zero(x) && !is_zero(y)) lds_add_product<W, ROW_THREADS>(ls, hl, x, y, false);
+            }
+            t = wmp::round_window<W, SLOTS>(window_total<ROW_THREADS>(ls, lane, used), hl);
+        } else {
+            dot_ok = wave_dot<W, ROW_THREADS>(0, j, lane, rec_at<W>((const uint32_t *)c.r, LdlCol<W>::tri(i)), 1,
+                                              rec_at<W>((const uint32_t *)c.w, LdlCol<W>::tri(j)), 1, slots, t,
+                                              hl, hh);
+        }
+        if (dot_ok) diff = wmp::add(aij, t, true);
+    }
+    __syncthreads();  // RN(1 / d_j) in dinv_sh, pivot j + 1's windows
+    if (!row) return;
+    if (!dot_ok) {
+        fail(2);
+        return;
+    }
+    const wmp::WV<W> v = zero_row ? wmp::zero<W>() : wmp::mul(diff, wmp::load<W>(dinv_sh));
+    const wmp::WV<W> wv = wmp::mul(v, dj);
+    wmp::store(rec_at<W>(c.r, LdlCol<W>::tri(i) + j), v);
+    wmp::store(rec_at<W>(c.w, LdlCol<W>::tri(i) + j), wv);
+    if (!wmp::is_zero(v)) {
+        ri.x = min(ri.x, v.e);
+        ri.y = max(ri.y, v.e);
+    }
+    if (!wmp::is_zero(wv)) {
+        ri.z = min(ri.z, wv.e);
+        ri.w = max(ri.w, wv.e);
+    }
+    if (lane == 0) c.rng[i] = ri;
+    if (!next) return;
+    const int64_t plo = pre_lo;
+    if (plo != INT64_MIN) {  // pivot j + 1: the pre-accumulated windows plus the term v wv
+        int64_t T = window_total<ROW_THREADS>(pre_win, lane, SLOTS);
+        bool fits = true;
+        if (!wmp::is_zero(v) && !wmp::is_zero(wv)) {
+            const int64_t sc = (int64_t)v.e + wv.e - 64 * W;
+            fits = sc >= plo && (sc - plo) / 32 + 2 * W + 4 <= SLOTS;
+            if (fits) T = wmp::add_product<W, SLOTS>(T, v, wv, sc - plo, false);
+        }
+        if (fits) {
+            const int code = ldl_col_pivot_wave<W>(c, i, wmp::round_window<W, SLOTS>(T, plo), ajj, reg_eps != nullptr,
+                                                   eps, delta, sg, status);
+            if (code != 3) {
+                if (code) fail(code);
+                return;
+            }
+        }
+    }
+    int64_t hl, hh;
+    range_hint<W>(ri.x, ri.y, ri.z, ri.w, hl, hh);
+    __threadfence_block();  // the whole dot reads the entries this wave's lanes just stored
+    const int code = ldl_col_pivot<W, 64>(c, i, lane, hl, hh, slots, signs, sign_mode, B.sign_off, reg_eps,
+                                          reg_delta, status);
     if (code) fail(code);
 }
 
@@ -1498,7 +1756,7 @@ void mp_ldl_dots(int limbs, void *A, void *dinv, const int32_t *first, const int
     }
     SDPX_DISPATCH_W(limbs, {
         SDPX_DISPATCH_G({
-            auto kern = ldl_col_kernel<W, G>;
+            auto kern = G == 64 ? ldl_col_wave_kernel<W> : ldl_col_kernel<W, G>;
             const size_t lds = panel_lds<W>((const void *)kern, 2 * ROW_THREADS);  // row groups, pivot windows
             for (int64_t j = -1; j < nmax - 1; j++) {
                 const int64_t rows = j < 0 ? 1 : nmax - j - 1;

@@ -900,51 +900,63 @@ __global__ __launch_bounds__(128) void crt_mfma_kernel(
             *(v4i *)(sx + l * CRT_XROW + 8 * part) = v;
         }
         __syncthreads();
+        // B fragments of both output tiles (mt = 0, 1) for every k-slice,
+        // built once from the staged planes; every table block reuses them
+        constexpr int NKS = 2 * NBMAX;
+        v4i fb[2][NKS];
+#pragma unroll
+        for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+            for (int ks = 0; ks < NKS; ks++) {
+                fb[mt][ks] = v4i{0, 0, 0, 0};
+                if (ks < nks) {
+                    const uint16_t *xs = sx + (16 * ks + 8 * h) * CRT_XROW + 64 * wave + 32 * mt + col;
+                    uint32_t d[4];
+#pragma unroll
+                    for (int u = 0; u < 4; u++)
+                        d[u] = (uint32_t)xs[(2 * u) * CRT_XROW] | ((uint32_t)xs[(2 * u + 1) * CRT_XROW] << 16);
+                    fb[mt][ks] = v4i{(int)d[0], (int)d[1], (int)d[2], (int)d[3]};
+                }
+            }
         // partial limb sum_b R_{32 nb + 8 g + 4 h + b} 256^b is limb 8 nb + 2 g + h
         // of output (mt, o); lane h owns output o = 64 wave + 32 h + col, i.e.
-        // tile mt = h: after each tile's MFMAs the two halves trade partials and
-        // the owning half carries them into its limbs at once
+        // tile mt = h.  Each table block multiplies both tiles; the two halves
+        // then trade the partials of the tile the other half owns, and every
+        // lane carries its own output's limbs, one 32-row block at a time.
         uint32_t lim[NUMAX];
 #pragma unroll
         for (int u = 0; u < NUMAX; u++) lim[u] = 0;
         int64_t carry = 0;
-#pragma unroll 1
-        for (int mt = 0; mt < 2; mt++) {
-            const int o = 64 * wave + 32 * mt + col;
-            const bool mine = h == mt;
-            // one 32-row block of table rows at a time: its partial limbs are
-            // carried before the next block's products start
 #pragma unroll
-            for (int nb = 0; nb < NBMAX; nb++) {
-                v16i acc = v16i{0};
-                if (nb < nbl)
-                    for (int ks = 0; ks < nks; ks++) {
-                        const uint16_t *xs = sx + (16 * ks + 8 * h) * CRT_XROW + o;
-                        uint32_t d[4];
+        for (int nb = 0; nb < NBMAX; nb++) {
+            v16i acc0 = v16i{0}, acc1 = v16i{0};  // blocks past nbl add zeros and carry on
 #pragma unroll
-                        for (int u = 0; u < 4; u++)
-                            d[u] = (uint32_t)xs[(2 * u) * CRT_XROW] | ((uint32_t)xs[(2 * u + 1) * CRT_XROW] << 16);
-                        const v4i fb = v4i{(int)d[0], (int)d[1], (int)d[2], (int)d[3]};
-                        acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(
-                            *(const v4i *)(stab + (32 * nb + col) * trow + 32 * ks + 16 * h), fb, acc, 0, 0, 0);
-                    }
+            for (int ks = 0; ks < NKS; ks++) {
+                if (nb < nbl && ks < nks) {
+                    const v4i fa = *(const v4i *)(stab + (32 * nb + col) * trow + 32 * ks + 16 * h);
+                    acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa, fb[0][ks], acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa, fb[1][ks], acc1, 0, 0, 0);
+                }
+            }
 #pragma unroll
-                for (int g = 0; g < 4; g++) {
-                    const int64_t pv = (int64_t)acc[4 * g] + (int64_t)acc[4 * g + 1] * 256 +
-                                       (int64_t)acc[4 * g + 2] * 65536 + (int64_t)acc[4 * g + 3] * 16777216;
-                    const int64_t got = __shfl_xor(pv, 32, 64);
-                    const int64_t p0 = h == 0 ? pv : got, p1 = h == 0 ? got : pv;  // limbs 2g, 2g + 1
-                    const int u = 8 * nb + 2 * g;
-                    if (u < NUMAX) {
-                        const int64_t a = carry + p0;
-                        lim[u] = mine ? (uint32_t)a : lim[u];
-                        carry = mine ? a >> 32 : carry;
-                    }
-                    if (u + 1 < NUMAX) {
-                        const int64_t a = carry + p1;
-                        lim[u + 1] = mine ? (uint32_t)a : lim[u + 1];
-                        carry = mine ? a >> 32 : carry;
-                    }
+            for (int g = 0; g < 4; g++) {
+                const int64_t pv0 = (int64_t)acc0[4 * g] + (int64_t)acc0[4 * g + 1] * 256 +
+                                    (int64_t)acc0[4 * g + 2] * 65536 + (int64_t)acc0[4 * g + 3] * 16777216;
+                const int64_t pv1 = (int64_t)acc1[4 * g] + (int64_t)acc1[4 * g + 1] * 256 +
+                                    (int64_t)acc1[4 * g + 2] * 65536 + (int64_t)acc1[4 * g + 3] * 16777216;
+                const int64_t got = __shfl_xor(h ? pv0 : pv1, 32, 64);  // the other half's partial of my tile
+                const int64_t own = h ? pv1 : pv0;
+                const int64_t p0 = h ? got : own, p1 = h ? own : got;  // limbs 2g, 2g + 1
+                const int u = 8 * nb + 2 * g;
+                if (u < NUMAX) {
+                    const int64_t a = carry + p0;
+                    lim[u] = (uint32_t)a;
+                    carry = a >> 32;
+                }
+                if (u + 1 < NUMAX) {
+                    const int64_t a = carry + p1;
+                    lim[u + 1] = (uint32_t)a;
+                    carry = a >> 32;
                 }
             }
         }
@@ -1578,7 +1590,9 @@ int rns_gemm_device(const RnsGemmArgs &g) {
     const int up = g.upper ? 1 : 0, pk = packed ? 1 : 0;
     uint32_t *outw = (uint32_t *)g.out;
     const int nbl = plan->crt_npad / 32;
-    const bool crt_mma = g_crt_mfma && plan->crt_nu <= 40 && nbl <= 5 && n <= 8;
+    // the MFMA form holds every k-slice's fragments: at most 2 per 32-row table block
+    const bool crt_mma = g_crt_mfma && plan->crt_nu <= 40 && nbl <= 5 && n <= 8 &&
+                         plan->crt_lpad / 16 <= 2 * (plan->crt_nu <= 20 && nbl <= 3 ? 3 : (plan->crt_nu <= 32 && nbl <= 4 ? 4 : 5));
     if (crt_mma) {
         const int lpad = plan->crt_lpad, npad = plan->crt_npad, nu = plan->crt_nu;
         const size_t tb = plan->crt_table.size();
