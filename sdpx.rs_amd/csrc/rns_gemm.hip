@@ -864,18 +864,20 @@ __global__ __launch_bounds__(BS) void crt_kernel(
 // and rounds the signed result once, nearest-even, to 64N bits.
 constexpr int CRT_XROW = TILE + 8;  // u16 per staged x plane row
 constexpr int CRT_MAXW = 16;        // mantissa words handled (up to 512 bits)
+constexpr size_t CRT_NU_MAX = 40;   // words of M the matrix-core CRT reads (zero-padded)
 
-template <int NUMAX, int NBMAX>
-__global__ __launch_bounds__(128) void crt_mfma_kernel(
+template <int NUMAX, int NBMAX, int NL>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void crt_mfma_kernel(
     const uint16_t *__restrict__ X, int L, int lpad, const int2 *__restrict__ tiles, int ntiles,
     int tiles_m, int tiles_n, const int8_t *__restrict__ table, int nt, int npad,
     const uint32_t *__restrict__ mlimbs, int nu, double m_top, const int4 *__restrict__ scan_a,
-    const int4 *__restrict__ scan_b, int64_t m, int64_t nn, int upper, int packed, int n,
+    const int4 *__restrict__ scan_b, int64_t m, int64_t nn, int upper, int packed, int n_arg,
     uint32_t *__restrict__ out) {
+    const int n = NL ? NL : n_arg;  // output limbs (a constant in the common widths)
     extern __shared__ __attribute__((aligned(16))) int8_t csm[];
     const int trow = 2 * lpad + 16;  // bytes per table row
     int8_t *stab = csm;
-    uint16_t *sx = (uint16_t *)(csm + (size_t)npad * trow);  // [lpad][CRT_XROW]
+    uint32_t *sl = (uint32_t *)(csm + (size_t)npad * trow);  // [NUMAX][128]: each thread its column
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, h = lane >> 5, col = lane & 31;
     const int nks = lpad / 16, nbl = npad / 32;
     const int c16 = 2 * lpad / 16;
@@ -883,6 +885,7 @@ __global__ __launch_bounds__(128) void crt_mfma_kernel(
         const int t = e / c16, c = e - t * c16;
         *(v4i *)(stab + t * trow + 16 * c) = *(const v4i *)(table + (size_t)t * 2 * lpad + 16 * c);
     }
+    __syncthreads();  // the table is in (the only barrier: everything else is per wave or per thread)
     const size_t xstride = (size_t)tiles_n * tiles_m * (TILE * TILE);
     const int64_t items = (int64_t)ntiles * TILE;
     const int P = 64 * n;
@@ -891,17 +894,11 @@ __global__ __launch_bounds__(128) void crt_mfma_kernel(
         const int jl = (int)(item % TILE);
         const int64_t j = (int64_t)tile.y * TILE + jl, i0 = (int64_t)tile.x * TILE;
         if (j >= nn || (upper && i0 > j)) continue;  // uniform over the workgroup
-        __syncthreads();  // the table is in; the previous item's planes are read
         const uint16_t *xb = X + ((size_t)tile.y * tiles_m + tile.x) * (TILE * TILE) + (size_t)jl * TILE;
-        for (int e = tid; e < lpad * 16; e += 128) {
-            const int l = e >> 4, part = e & 15;
-            v4i v = v4i{0, 0, 0, 0};
-            if (l < L) v = *(const v4i *)(xb + (size_t)l * xstride + 8 * part);
-            *(v4i *)(sx + l * CRT_XROW + 8 * part) = v;
-        }
-        __syncthreads();
         // B fragments of both output tiles (mt = 0, 1) for every k-slice,
-        // built once from the staged planes; every table block reuses them
+        // read once from the planes; every table block reuses them.  Planes
+        // L .. lpad - 1 are allocated but never written: their table
+        // coefficients are zero, so whatever they hold adds nothing.
         constexpr int NKS = 2 * NBMAX;
         v4i fb[2][NKS];
 #pragma unroll
@@ -910,11 +907,12 @@ __global__ __launch_bounds__(128) void crt_mfma_kernel(
             for (int ks = 0; ks < NKS; ks++) {
                 fb[mt][ks] = v4i{0, 0, 0, 0};
                 if (ks < nks) {
-                    const uint16_t *xs = sx + (16 * ks + 8 * h) * CRT_XROW + 64 * wave + 32 * mt + col;
+                    const int l0 = 16 * ks + 8 * h;
+                    const uint16_t *xs = xb + (size_t)l0 * xstride + 64 * wave + 32 * mt + col;
                     uint32_t d[4];
 #pragma unroll
                     for (int u = 0; u < 4; u++)
-                        d[u] = (uint32_t)xs[(2 * u) * CRT_XROW] | ((uint32_t)xs[(2 * u + 1) * CRT_XROW] << 16);
+                        d[u] = (uint32_t)xs[(size_t)(2 * u) * xstride] | ((uint32_t)xs[(size_t)(2 * u + 1) * xstride] << 16);
                     fb[mt][ks] = v4i{(int)d[0], (int)d[1], (int)d[2], (int)d[3]};
                 }
             }
@@ -968,54 +966,48 @@ __global__ __launch_bounds__(128) void crt_mfma_kernel(
         }
         const int64_t i = i0 + 64 * wave + lane;
         const bool active = i < m && !(upper && i > j);
-        // k = round(V / M) from the top three limbs; S = V - k M
-        uint32_t l2 = 0, l1 = 0, l0 = 0;
+        // V's words in this thread's LDS column: the three top limbs below
+        // word nu are read back at the run-time index
+#pragma unroll
+        for (int u = 0; u < NUMAX; u++) sl[u * 128 + tid] = lim[u];
+        const uint32_t l2 = sl[(nu - 1) * 128 + tid], l1 = nu >= 2 ? sl[(nu - 2) * 128 + tid] : 0u,
+                       l0 = nu >= 3 ? sl[(nu - 3) * 128 + tid] : 0u;
+        // k = round(V / M); S = V - k M over all NUMAX words (M's words
+        // zero-padded to NUMAX; S is small, so the words above nu carry its
+        // sign)
+        const double vt = (double)(int32_t)l2 * 18446744073709551616.0 + (double)l1 * 4294967296.0 + (double)l0;
+        const int32_t kq = (int32_t)rint(vt / m_top);
+        int64_t br = 0;
 #pragma unroll
         for (int u = 0; u < NUMAX; u++) {
-            if (u == nu - 1) l2 = lim[u];
-            if (u == nu - 2) l1 = lim[u];
-            if (u == nu - 3) l0 = lim[u];
+            const int64_t t = (int64_t)lim[u] - (int64_t)kq * (int64_t)mlimbs[u] + br;
+            lim[u] = (uint32_t)t;
+            br = t >> 32;
         }
-        const double vt = (double)(int32_t)l2 * 18446744073709551616.0 + (double)l1 * 4294967296.0 + (double)l0;
-        const int64_t kq = (int64_t)rint(vt / m_top);
-        int64_t br = 0;
-        uint32_t topw = 0;
-#pragma unroll
-        for (int u = 0; u < NUMAX; u++)
-            if (u < nu) {
-                const int64_t t = (int64_t)lim[u] - kq * (int64_t)mlimbs[u] + br;
-                lim[u] = (uint32_t)t;
-                br = t >> 32;
-                topw = lim[u];
-            }
-        const bool negative = (int32_t)topw < 0;
+        const bool negative = (int32_t)lim[NUMAX - 1] < 0;
         if (negative) {
             uint64_t c = 1;
 #pragma unroll
-            for (int u = 0; u < NUMAX; u++)
-                if (u < nu) {
-                    const uint64_t t = (uint64_t)(~lim[u]) + c;
-                    lim[u] = (uint32_t)t;
-                    c = t >> 32;
-                }
+            for (int u = 0; u < NUMAX; u++) {
+                const uint64_t t = (uint64_t)(~lim[u]) + c;
+                lim[u] = (uint32_t)t;
+                c = t >> 32;
+            }
         }
         int top = -1, low = NUMAX;
         uint32_t tw = 0;
 #pragma unroll
         for (int u = 0; u < NUMAX; u++) {
-            if (u < nu && lim[u] != 0) {
+            if (lim[u] != 0) {
                 top = u;
                 tw = lim[u];
             }
             const int d = NUMAX - 1 - u;
-            if (d < nu && lim[d] != 0) low = d;
+            if (lim[d] != 0) low = d;
         }
-        // the magnitude's words, [word][thread], in the x-plane area (every
-        // wave has finished its fragment reads at the barrier)
-        __syncthreads();
-        uint32_t *sl = (uint32_t *)sx;
+        // the magnitude's words over V's (each thread its own column)
 #pragma unroll
-        for (int u = 0; u < NUMAX; u++) sl[u * 128 + tid] = u < nu ? lim[u] : 0u;
+        for (int u = 0; u < NUMAX; u++) sl[u * 128 + tid] = lim[u];
         if (!active) continue;
         uint32_t *rec = out + (size_t)(packed ? j * (j + 1) / 2 + i : i + j * m) * (2 * n + 2);
         if (top < 0) {
@@ -1394,6 +1386,7 @@ PlanTables &plan_tables(const RnsPlan &plan) {
         std::vector<uint8_t> c2(plan.crt_table.begin(), plan.crt_table.end());
         const uint8_t *m32 = (const uint8_t *)plan.m32.data();
         c2.insert(c2.end(), m32, m32 + 4 * plan.m32.size());
+        if (plan.m32.size() < CRT_NU_MAX) c2.resize(c2.size() + 4 * (CRT_NU_MAX - plan.m32.size()), 0);  // M zero-padded
         t.crt2 = upload_once<uint8_t>(c2.data(), c2.size());
     }
     return t;
@@ -1556,7 +1549,8 @@ int rns_gemm_device(const RnsGemmArgs &g) {
 
     // 3. products, in exact INT32 chunks of the inner dimension
     sev.mark(2, s);
-    uint16_t *X = (uint16_t *)ws.get(S_X, (size_t)L * tiles_m * tiles_n * TILE * TILE * 2);
+    // planes up to the matrix-core CRT's padded count (read there, times zero coefficients)
+    uint16_t *X = (uint16_t *)ws.get(S_X, (size_t)std::max(L, plan->crt_lpad) * tiles_m * tiles_n * TILE * TILE * 2);
     for (int64_t k0 = 0; k0 < k_pad; k0 += KCHUNK) {
         const int64_t k1 = std::min<int64_t>(k_pad, k0 + KCHUNK);
         const int acc = k0 > 0 ? 1 : 0;
@@ -1599,8 +1593,8 @@ int rns_gemm_device(const RnsGemmArgs &g) {
         const uint8_t *d_ct = ptab.crt2;
         const int trow = 2 * lpad + 16;
         const int numax = nu <= 20 && nbl <= 3 ? 20 : (nu <= 32 && nbl <= 4 ? 32 : 40);
-        // x planes, then (reused) the magnitudes' words, [word][thread]
-        const size_t lds = (size_t)npad * trow + std::max<size_t>((size_t)lpad * CRT_XROW * 2, (size_t)numax * 128 * 4);
+        // the table, then each thread's column of words
+        const size_t lds = (size_t)npad * trow + (size_t)numax * 128 * 4;
         const int64_t items = (int64_t)ntiles * TILE;
         const unsigned grid = (unsigned)std::min<int64_t>(items, 4096);
         auto launch = [&](auto kern) {
@@ -1612,12 +1606,21 @@ int rns_gemm_device(const RnsGemmArgs &g) {
                                plan->crt_nt, npad, (const uint32_t *)(d_ct + tb), nu, plan->m_top,
                                (const int4 *)scan_a, (const int4 *)scan_b, m, nn, up, pk, n, outw);
         };
+        auto pick = [&](auto k2, auto k4, auto k8, auto kn) {
+            if (n == 2) launch(k2);
+            else if (n == 4) launch(k4);
+            else if (n == 8) launch(k8);
+            else launch(kn);
+        };
         if (numax == 20)
-            launch(crt_mfma_kernel<20, 3>);
+            pick(crt_mfma_kernel<20, 3, 2>, crt_mfma_kernel<20, 3, 4>, crt_mfma_kernel<20, 3, 8>,
+                 crt_mfma_kernel<20, 3, 0>);
         else if (numax == 32)
-            launch(crt_mfma_kernel<32, 4>);
+            pick(crt_mfma_kernel<32, 4, 2>, crt_mfma_kernel<32, 4, 4>, crt_mfma_kernel<32, 4, 8>,
+                 crt_mfma_kernel<32, 4, 0>);
         else
-            launch(crt_mfma_kernel<40, 5>);
+            pick(crt_mfma_kernel<40, 5, 2>, crt_mfma_kernel<40, 5, 4>, crt_mfma_kernel<40, 5, 8>,
+                 crt_mfma_kernel<40, 5, 0>);
     } else if (J <= 32)
         hipLaunchKernelGGL((crt_kernel<32, 256>), dim3(TILE * TILE / 256, ntiles), dim3(256), 0, s,
                            X, L, d_tiles, tiles_m, tiles_n, d_c, d_m, d_qinv, d_q, J, scan_a, scan_b,
