@@ -212,7 +212,7 @@ def sampled_workload(args):
                      "kernel": "product_kernel (INT8 MFMA residue products)",
                      "int8_ops_per_step": tot["int8_ops"], "share_of_step": product_s / dt,
                      "note": "the largest share of the step is the latency-bound exact-dot panel "
-                             "kernels (ldl_col_kernel, trsm_rl_kernel), which have no MFMA or "
+                             "kernels (ldl_col_wave_kernel, trsm_rl_kernel), which have no MFMA or "
                              "HBM roofline"},
     }
     if args.cpu_baseline and t <= 2000:  # larger borders: minutes of CPU time
