@@ -558,20 +558,31 @@ __global__ __launch_bounds__(ROW_THREADS) void forward_rows_kernel(const uint32_
 }
 
 // X L^T = B rows (solve_unit_lower_t base, arrow.rs:425-447) right-looking:
-// one workgroup of two waves a row, thread c owning column c (n <= 128) with
-// its own exact slot window in LDS.  Step j rounds column j's window once in
-// each wave (wave_round: the exact dot of x_k L_jk over k < j, the same
-// result in both), forms x_j = RN(b_j - t), and each thread adds x_j L_cj
-// into the window of its column c > j; one barrier a step.  So a step costs
-// one rounding and one product per thread instead of a whole exact dot.
-// b_j and the base of column j's window reach both waves through LDS (the
-// owner of column j + 1 posts its own after step j's products).  A window's
-// base is set at its first term from the exponent range of L's row (rows of
-// L over k < c) with `room` bits (128 by default) for x to shrink; a term
-// outside the window puts the row on the fallback list (fb_rows) with B's
-// row untouched, and the left-looking kernel then solves it.  Results are
-// those of the left-looking kernel: the same exact dots, each rounded once.
-constexpr int RL_THREADS = 128;
+// one workgroup a row, each column c < n (n <= 128) with its own exact slot
+// window in LDS holding the terms x_k L_ck, k < c.  Wave 0 runs the chain:
+// step j takes column j's window, adds the one term x_{j-1} L_{j,j-1} in
+// registers, rounds once (the exact dot of x_k L_jk over k < j) and forms
+// x_j = RN(b_j - t) on the wave (wavemp.h).  Waves 1 and 2 (thread c - 64
+// owning column c) add the other terms one step behind: in the interval of
+// chain step j they add x_{j-1} L_{c,j-1} for c > j.  One barrier an
+// interval, so a step costs the chain's rounding, sum and one product while
+// the per-column products run beside it.  A window's base is set at its
+// first term (by its column's thread, or by the chain when that term is the
+// last one) from the exponent range of L's row (rows of L over k < c) with
+// `room` bits (128 by default) for x to shrink; a term outside the window
+// puts the row on the fallback list (fb_rows) with B's row untouched, and
+// the left-looking kernel then solves it.  x_j is kept in column j's window
+// once the chain has read it (word w in the low or high half of slot w / 2),
+// so LDS holds only the windows and their bases.  Results are those of the
+// left-looking kernel: the same exact dots, each rounded once.
+constexpr int RL_COLS = 128;               // columns (window slots per slot row)
+constexpr int RL_THREADS = RL_COLS + 64;   // the chain wave, then one thread a column
+
+template <int W>
+__device__ __forceinline__ int64_t rl_base(int32_t xe, int32_t le, const int2 *lr, int64_t c, int64_t room) {
+    const int64_t sc = (int64_t)xe + le - 64 * W;
+    return lr ? (int64_t)xe - room + lr[c].x - 64 * W : sc - 2 * room;
+}
 
 template <int W>
 __global__ __launch_bounds__(RL_THREADS) void trsm_rl_kernel(
@@ -579,66 +590,93 @@ __global__ __launch_bounds__(RL_THREADS) void trsm_rl_kernel(
     const int32_t *low, const int2 *lr, int64_t room, int *fb_count, int32_t *fb_rows) {
     constexpr int SLOTS = Win<W>::SLOTS;
     static_assert(SLOTS <= 64, "one slot a lane");
-    extern __shared__ int64_t slots_sh[];  // [SLOTS][RL_THREADS] windows, then two posted (b, base) pairs
-    const int c = threadIdx.x, wl = c & 63;
+    // [SLOTS][RL_COLS] windows, then the window bases [RL_COLS]
+    extern __shared__ int64_t slots_sh[];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int64_t i = blockIdx.x;
     if (i >= m) return;  // the whole workgroup
     lds_i64 *win = (lds_i64 *)slots_sh;
-    uint32_t *post = (uint32_t *)(slots_sh + SLOTS * RL_THREADS);  // 2 x (W + 2 words, pad, int64 base)
-    constexpr int PW = W + 4;  // posted words: record, then the base as two words
+    int64_t *bases = slots_sh + SLOTS * RL_COLS;
+    // word w of x_c, in column c's window after the chain has consumed it
+    auto xword = [&](int64_t c, int w) -> lds_u32 * { return (lds_u32 *)(win + (w >> 1) * RL_COLS + c) + (w & 1); };
     const int64_t lo = low[i];
-    Val<W> b, xo;
-    set_zero(xo);
-    int64_t base = INT64_MIN;
-    if (c >= lo && c < n) load(rec_at<W>((const uint32_t *)B, i + c * ldb), b);
-    else set_zero(b);
-#pragma unroll
-    for (int sl = 0; sl < SLOTS; sl++) win[sl * RL_THREADS + c] = 0;
-    auto post_own = [&](int64_t j) {  // column j's b and window base, for step j
-        uint32_t *pw = post + (j & 1) * PW;
-        store(pw, b);
-        pw[W + 2] = (uint32_t)(uint64_t)base;
-        pw[W + 3] = (uint32_t)((uint64_t)base >> 32);
-    };
-    if (c == lo) post_own(lo);
-    // L's entry (c, j) of this thread's column, one step ahead
-    Val<W> lv, ln;
-    auto load_l = [&](int64_t j, Val<W> &dst) {
-        if (c > j && c < n) load(rec_at<W>(L, c + j * ldl), dst);
-        else set_zero(dst);
-    };
-    if (lo < n) load_l(lo, ln);
+    for (int e = tid; e < SLOTS * RL_COLS; e += RL_THREADS) win[e] = 0;
+    for (int e = tid; e < RL_COLS; e += RL_THREADS) bases[e] = INT64_MIN;
     __syncthreads();
     bool ovf = false;
-    for (int64_t j = lo; j < n; j++) {
-        lv = ln;
-        if (j + 1 < n) load_l(j + 1, ln);
-        const uint32_t *pw = post + (j & 1) * PW;
-        const int64_t basej = (int64_t)(((uint64_t)__builtin_amdgcn_readfirstlane(pw[W + 3]) << 32) |
-                                        (uint32_t)__builtin_amdgcn_readfirstlane(pw[W + 2]));
-        // x_j = RN(b_j - t), the same in every thread: formed spread over each wave
-        const wmp::WV<W> t = basej == INT64_MIN
-                                 ? wmp::zero<W>()
-                                 : wmp::round_window<W, SLOTS>(wl < SLOTS ? (int64_t)win[wl * RL_THREADS + j] : 0, basej);
-        const Val<W> x = wmp::to_val(wmp::add(wmp::load<W>(pw), t, true));
-        if (c == j) xo = x;
-        if (!is_zero(x) && !is_zero(lv)) {  // lv is zero for c <= j and c >= n
-            const int64_t sc = (int64_t)x.e + lv.e - 64 * W;
-            if (base == INT64_MIN) base = lr ? (int64_t)x.e - room + lr[c].x - 64 * W : sc - 2 * room;
-            const int64_t off = sc - base;
-            if (off < 0 || off / 32 + 2 * W + 4 > SLOTS)
-                ovf = true;
-            else
-                lds_add_product<W, RL_THREADS>(win + c, base, x, lv, false);
+    if (wave == 0) {  // the chain
+        wmp::WV<W> xprev = wmp::zero<W>();
+        // b_j and L_{j,j-1} one step ahead
+        uint32_t bn = lo < n ? wmp::load_word<W>(rec_at<W>((const uint32_t *)B, i + lo * ldb)) : 0u, ln = 0u;
+        for (int64_t j = lo; j < n; j++) {
+            const uint32_t bw = bn, lw = ln;
+            if (j + 1 < n) {
+                bn = wmp::load_word<W>(rec_at<W>((const uint32_t *)B, i + (j + 1) * ldb));
+                ln = wmp::load_word<W>(rec_at<W>(L, (j + 1) + j * ldl));
+            }
+            const wmp::WV<W> bj = wmp::from_word<W>(bw);
+            const wmp::WV<W> lj = j > lo ? wmp::from_word<W>(lw) : wmp::zero<W>();
+            int64_t T = lane < SLOTS ? (int64_t)win[lane * RL_COLS + j] : 0;
+            int64_t base = bases[j];
+            if (!wmp::is_zero(xprev) && !wmp::is_zero(lj)) {  // the last term x_{j-1} L_{j,j-1}
+                if (base == INT64_MIN) base = rl_base<W>(xprev.e, lj.e, lr, j, room);
+                const int64_t off = (int64_t)xprev.e + lj.e - 64 * W - base;
+                if (off < 0 || off / 32 + 2 * W + 4 > SLOTS)
+                    ovf = true;
+                else
+                    T = wmp::add_product<W, SLOTS>(T, xprev, lj, off, false);
+            }
+            const wmp::WV<W> t = base == INT64_MIN ? wmp::zero<W>() : wmp::round_window<W, SLOTS>(T, base);
+            xprev = wmp::add(bj, t, true);  // x_j = RN(b_j - t)
+            if (lane < W + 2)  // after the window's reads above (LDS operations of a wave complete in order)
+                *xword(j, lane) = lane < W ? xprev.w : (lane == W ? (uint32_t)xprev.kind : (uint32_t)xprev.e);
+            if (__syncthreads_or(ovf)) break;
         }
-        if (c == j + 1) post_own(j + 1);
-        if (__syncthreads_or(ovf)) break;
+    } else {  // thread c adds x_k L_ck for c > k + 1, one step behind the chain
+        const int c = tid - 64;
+        int64_t base = INT64_MIN;
+        Val<W> lv, ln;
+        auto load_l = [&](int64_t k, Val<W> &dst) {
+            if (c > k + 1 && c < n) load(rec_at<W>(L, c + k * ldl), dst);
+            else set_zero(dst);
+        };
+        if (lo < n) load_l(lo, ln);
+        for (int64_t j = lo; j < n; j++) {
+            const int64_t k = j - 1;  // the chain's previous step
+            if (k >= lo) {
+                lv = ln;
+                if (k + 1 < n) load_l(k + 1, ln);
+                if (!is_zero(lv)) {  // zero for c <= k + 1 and c >= n
+                    Val<W> x;
+#pragma unroll
+                    for (int w = 0; w < W; w++) x.m[w] = *xword(k, w);
+                    x.kind = (int32_t)*xword(k, W);
+                    x.e = (int32_t)*xword(k, W + 1);
+                    if (!is_zero(x)) {
+                        if (base == INT64_MIN) {
+                            base = rl_base<W>(x.e, lv.e, lr, c, room);
+                            bases[c] = base;
+                        }
+                        const int64_t off = (int64_t)x.e + lv.e - 64 * W - base;
+                        if (off < 0 || off / 32 + 2 * W + 4 > SLOTS)
+                            ovf = true;
+                        else
+                            lds_add_product<W, RL_COLS>(win + c, base, x, lv, false);
+                    }
+                }
+            }
+            if (__syncthreads_or(ovf)) break;
+        }
     }
     if (__syncthreads_or(ovf)) {
-        if (c == 0) fb_rows[atomicAdd(fb_count, 1)] = (int32_t)i;
+        if (tid == 0) fb_rows[atomicAdd(fb_count, 1)] = (int32_t)i;
         return;
     }
-    if (c >= lo && c < n) store(rec_at<W>(B, i + c * ldb), xo);
+    for (int64_t e = lo * (W + 2) + tid; e < n * (W + 2); e += RL_THREADS) {  // B's row: x_c for c in [lo, n)
+        const int64_t c = e / (W + 2);
+        const int w = (int)(e - c * (W + 2));
+        B[(size_t)(i + c * ldb) * (W + 2) + w] = *xword(c, w);
+    }
 }
 
 // X L^T = B rows (solve_unit_lower_t base): B m x n (ldb), L n x n (ldl,
@@ -1702,7 +1740,7 @@ void mp_trsm_base(int limbs, const void *L, int64_t ldl, int64_t n, void *B, int
     SDPX_DISPATCH_W(limbs, {
         // the right-looking form holds a window per column (twice the LDS of
         // a left-looking row): it runs when all m rows fit on the chip at once
-        const size_t lds = slot_bytes<W>(RL_THREADS) + 2 * (W + 4) * sizeof(uint32_t);
+        const size_t lds = slot_bytes<W>(RL_COLS) + RL_COLS * sizeof(int64_t);
         const bool rl = trsm_right_looking() && n <= 128 && m <= rows_resident(lds);
         if (rl) {
             HIP_CHECK(hipMemsetAsync(fb_count, 0, sizeof(int), st));

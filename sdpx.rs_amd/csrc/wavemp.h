@@ -69,6 +69,17 @@ __device__ __forceinline__ WV<W> load(const uint32_t *rec) {
     const uint32_t v = k < W + 2 ? rec[k] : 0u;
     return WV<W>{k < W ? v : 0u, (int32_t)rl(v, W), (int32_t)rl(v, W + 1)};
 }
+// The same in two halves, so a load can be issued a step ahead: the lane's
+// word of the record (load_word), then the value (from_word, which waits).
+template <int W>
+__device__ __forceinline__ uint32_t load_word(const uint32_t *rec) {
+    const int k = lane();
+    return k < W + 2 ? rec[k] : 0u;
+}
+template <int W>
+__device__ __forceinline__ WV<W> from_word(uint32_t v) {
+    return WV<W>{lane() < W ? v : 0u, (int32_t)rl(v, W), (int32_t)rl(v, W + 1)};
+}
 template <int W>
 __device__ __forceinline__ void store(uint32_t *rec, const WV<W> &x) {
     const int k = lane();
