@@ -742,6 +742,119 @@ __global__ __launch_bounds__(ROW_THREADS) void trsm_base_n_kernel(
 }
 
 // out (n x m, ldo) = in^T (in: m x n, ldi), values
+// trsm_base_kernel / trsm_base_n_kernel for one whole wave a row and
+// n <= 128: the row's solved x_k stay in registers (lane k % 64 holds x_k),
+// so each step's dot reads only L's entries (fetched a step ahead) and
+// rounds on the wave (wavemp.h); x_j = RN(b_j - t) is formed on the wave
+// too.  Forward (solve_unit_lower_t): j = lo .. n - 1 over terms
+// x_k L(j, k), k in [lo, j), window hint from the x range and lr[j] (L's row
+// range); backward: j = n - 1 .. 0 over x_k L(k, j), k in (j, n), hint from
+// lr[j] (L's column range).  Same windows, terms and roundings as the group
+// kernels; a dot beyond its window sets *fail.
+template <int W, bool BACK>
+__device__ __forceinline__ void trsm_row_wave(const uint32_t *L, int64_t ldl, int64_t n, uint32_t *B, int64_t ldb,
+                                              int64_t i, int64_t lo, const int2 *lr, int *fail, int64_t *slots_sh) {
+    constexpr int SLOTS = Win<W>::SLOTS;
+    static_assert(SLOTS <= 64, "one slot a lane");
+    const int lane = threadIdx.x & 63;
+    lds_i64 *ls = (lds_i64 *)(slots_sh + lane);
+    Val<W> x0, x1;  // x_k for k = lane, lane + 64
+    set_zero(x0);
+    set_zero(x1);
+    int32_t xmn = INT32_MAX, xmx = INT32_MIN;  // x_k solved so far
+    // the step's terms: k in [lo, j) forward, (j, n) backward
+    auto in_dot = [&](int64_t j, int64_t k) { return BACK ? (k > j && k < n) : (k >= lo && k < j); };
+    auto load_l = [&](int64_t j, int64_t k, Val<W> &dst) {
+        if (in_dot(j, k)) load(rec_at<W>(L, BACK ? k + j * ldl : j + k * ldl), dst);
+        else set_zero(dst);
+    };
+    const int64_t j0 = BACK ? n - 1 : lo, jend = BACK ? -1 : n, dj = BACK ? -1 : 1;
+    Val<W> l0n, l1n;
+    uint32_t bn = 0;
+    if (j0 != jend) {
+        load_l(j0, lane, l0n);
+        load_l(j0, lane + 64, l1n);
+        bn = wmp::load_word<W>(rec_at<W>((const uint32_t *)B, i + j0 * ldb));
+    }
+    for (int64_t j = j0; j != jend; j += dj) {
+        const Val<W> l0 = l0n, l1 = l1n;
+        const uint32_t bw = bn;
+        if (j + dj != jend) {
+            load_l(j + dj, lane, l0n);
+            load_l(j + dj, lane + 64, l1n);
+            bn = wmp::load_word<W>(rec_at<W>((const uint32_t *)B, i + (j + dj) * ldb));
+        }
+        const bool t0 = !is_zero(x0) && !is_zero(l0), t1 = !is_zero(x1) && !is_zero(l1);
+        int64_t hl = INT64_MAX, hh = INT64_MIN;
+        if (lr) range_hint<W>(xmn, xmx, lr[j].x, lr[j].y, hl, hh);
+        if (!(hl <= hh && (hh - hl) / 32 + 2 * W + 4 <= SLOTS)) {  // the terms' own range
+            hl = INT64_MAX;
+            hh = INT64_MIN;
+            if (t0) {
+                const int64_t sc = (int64_t)x0.e + l0.e - 64 * W;
+                hl = min(hl, sc);
+                hh = max(hh, sc);
+            }
+            if (t1) {
+                const int64_t sc = (int64_t)x1.e + l1.e - 64 * W;
+                hl = min(hl, sc);
+                hh = max(hh, sc);
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                hl = min(hl, (int64_t)__shfl_xor(hl, off, 64));
+                hh = max(hh, (int64_t)__shfl_xor(hh, off, 64));
+            }
+        }
+        wmp::WV<W> t = wmp::zero<W>();
+        if (hl != INT64_MAX) {
+            if ((hh - hl) / 32 + 2 * W + 4 > SLOTS) {
+                if (lane == 0) atomicOr(fail, 1);
+                return;
+            }
+            const int used = (int)min((int64_t)SLOTS, (hh - hl) / 32 + 2 * W + 1);
+            for (int sl = 0; sl < used; sl++) ls[sl * 64] = 0;
+            if (t0) lds_add_product<W, 64>(ls, hl, x0, l0, false);
+            if (t1) lds_add_product<W, 64>(ls, hl, x1, l1, false);
+            t = wmp::round_window<W, SLOTS>(window_total<64>(ls, lane, used), hl);
+        }
+        const wmp::WV<W> r = wmp::add(wmp::from_word<W>(bw), t, true);
+        if (!wmp::is_zero(r)) {
+            xmn = min(xmn, r.e);
+            xmx = max(xmx, r.e);
+        }
+        wmp::store(rec_at<W>(B, i + j * ldb), r);
+        const Val<W> rv = wmp::to_val(r);
+        if (lane == (int)(j & 63)) {
+            if (j < 64) x0 = rv;
+            else x1 = rv;
+        }
+    }
+}
+
+template <int W>
+__global__ __launch_bounds__(64) void trsm_base_wave_kernel(const uint32_t *L, int64_t ldl, int64_t n, uint32_t *B,
+                                                            int64_t ldb, int64_t m, const int32_t *low, const int2 *lr,
+                                                            int *fail, const int32_t *rows, const int *nrows) {
+    extern __shared__ int64_t slots_sh[];
+    int64_t i = blockIdx.x;
+    if (rows) {  // only the listed rows
+        if (i >= *nrows) return;
+        i = rows[i];
+    }
+    if (i >= m) return;
+    trsm_row_wave<W, false>(L, ldl, n, B, ldb, i, low[i], lr, fail, slots_sh);
+}
+
+template <int W>
+__global__ __launch_bounds__(64) void trsm_base_n_wave_kernel(const uint32_t *L, int64_t ldl, int64_t n, uint32_t *B,
+                                                              int64_t ldb, int64_t m, const int2 *lc, int *fail) {
+    extern __shared__ int64_t slots_sh[];
+    const int64_t i = blockIdx.x;
+    if (i >= m) return;
+    trsm_row_wave<W, true>(L, ldl, n, B, ldb, i, 0, lc, fail, slots_sh);
+}
+
 template <int W>
 __global__ void transpose_kernel(const uint32_t *in, int64_t ldi, int64_t m, int64_t n,
                                  uint32_t *out, int64_t ldo) {
@@ -1752,7 +1865,7 @@ void mp_trsm_base(int limbs, const void *L, int64_t ldl, int64_t n, void *B, int
                                fb_count, fb_rows);
         }
         SDPX_DISPATCH_G({
-            auto kern = trsm_base_kernel<W, G>;
+            auto kern = G == 64 && n <= 128 ? trsm_base_wave_kernel<W> : trsm_base_kernel<W, G>;
             const size_t lds = panel_lds<W>((const void *)kern, ROW_THREADS);
             hipLaunchKernelGGL(kern, dim3(blocks_for(m * G, ROW_THREADS)), dim3(ROW_THREADS), lds, st,
                                (const uint32_t *)L, ldl, n, (uint32_t *)B, ldb, m, low, (const int2 *)lranges,
@@ -1832,7 +1945,7 @@ void mp_trsm_base_n(int limbs, const void *L, int64_t ldl, int64_t n, void *B, i
     if (m == 0 || n == 0) return;
     SDPX_DISPATCH_W(limbs, {
         SDPX_DISPATCH_G({
-            auto kern = trsm_base_n_kernel<W, G>;
+            auto kern = G == 64 && n <= 128 ? trsm_base_n_wave_kernel<W> : trsm_base_n_kernel<W, G>;
             const size_t lds = panel_lds<W>((const void *)kern, ROW_THREADS);
             hipLaunchKernelGGL(kern, dim3(blocks_for(m * G, ROW_THREADS)), dim3(ROW_THREADS), lds, st,
                                (const uint32_t *)L, ldl, n, (uint32_t *)B, ldb, m, (const int2 *)lranges, fail);
