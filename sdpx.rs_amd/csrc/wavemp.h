@@ -230,30 +230,48 @@ __device__ __forceinline__ WV<W> mul(const WV<W> &a, const WV<W> &b) {
 }
 
 // RN(1 / d) for a normalized d: Q = floor(2^(2P+1) / D) by base-2^32
-// schoolbook division (Knuth, TAOCP vol. 2, 4.3.1, algorithm D) with the
-// remainder window on lanes 0..W, shifted up one lane per quotient digit;
-// each digit's multiply-subtract is one product per lane and one borrow
-// resolution.  Then (Q << 1 | sticky) rounded once: recip's result.
+// schoolbook division with the remainder window on lanes 0..W, shifted up
+// one lane per quotient digit; each digit's multiply-subtract is one
+// product per lane and one borrow resolution.  Then (Q << 1 | sticky)
+// rounded once: recip's result.
+//
+// Quotient digits: q = floor(R / V) (R < 2^32 V) is estimated in binary64 as
+// R3 / V2 from R's top three words and V's top two.  Truncating both moves
+// the quotient by at most (q + 2) / V2 < 2^-30, and forming R3, V2, 1 / V2
+// and the product rounds six times at most (relative 2^-53 each), so the
+// estimate lies within 2^-18 of R / V: when its fraction is at least 2^-16
+// from an integer, its floor is q.  Otherwise the digit comes from Knuth's
+// estimate (algorithm D, TAOCP vol. 2, 4.3.1: two-word estimate, the
+// v_{n-2} test, add back when the remainder goes negative).
 template <int W>
 __device__ __forceinline__ WV<W> recip(const WV<W> &d) {
     if (is_zero(d)) return WV<W>{0u, d.kind < 0 ? -KIND_INF : KIND_INF, 0};
     constexpr int P = 32 * W, QW = W + 2;
     const int k = lane();
     const uint32_t v = d.w;
-    const uint64_t vt = rl(v, W - 1), vs = W >= 2 ? rl(v, W - 2) : 0;
+    const uint64_t vt = rl(v, W - 1), vs = rl(v, W - 2);
     const double rcp = 1.0 / (double)vt;
+    const double rv2 = 1.0 / fma((double)vt, 4294967296.0, (double)vs);  // 1 / V2
     uint32_t R = k == W - 1 ? 2u : 0u;  // numerator words u[W+1 .. 2W+1]: u[2W] = 2
     uint32_t Q = 0;
 #pragma unroll 1
     for (int j = QW - 1; j >= 0; j--) {
         if (j < QW - 1) R = up1(R);  // next window: R * 2^32 (the numerator's lower words are 0)
-        const uint64_t num = ((uint64_t)rl(R, W) << 32) | rl(R, W - 1);
-        uint64_t rh, qh = mpd::qdigit(num, vt, rcp, rh);
-        const uint64_t ulow = W >= 2 ? rl(R, W - 2) : 0;
-        while (qh >> 32 || qh * vs > ((rh << 32) | ulow)) {
-            qh--;
-            rh += vt;
-            if (rh >> 32) break;
+        const uint32_t r2 = rl(R, W), r1 = rl(R, W - 1), r0 = rl(R, W - 2);
+        const double est = fma(fma((double)r2, 4294967296.0, (double)r1), 4294967296.0, (double)r0) * rv2;
+        const double fl = floor(est), fr = est - fl;
+        uint64_t qh;
+        if (fr >= 0x1p-16 && fr <= 1.0 - 0x1p-16) {
+            qh = (uint32_t)fl;  // exact (see above); below 2^32 since R < 2^32 V
+        } else {
+            const uint64_t num = ((uint64_t)r2 << 32) | r1;
+            uint64_t rh;
+            qh = mpd::qdigit(num, vt, rcp, rh);
+            while (qh >> 32 || qh * vs > ((rh << 32) | r0)) {
+                qh--;
+                rh += vt;
+                if (rh >> 32) break;
+            }
         }
         // R -= qh * V over lanes 0..W
         const uint64_t p = (uint64_t)(uint32_t)qh * v;
