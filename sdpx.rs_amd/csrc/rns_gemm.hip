@@ -518,6 +518,10 @@ __global__ __launch_bounds__(256) void product_kernel(
     const double qd = (double)q, qinv = mqinv[l];
     const double inv = (double)minv[l];
     const double r16 = (double)((65536 % q)), r8 = (double)(256 % q);
+    // x = S (M/q)^-1 mod q in one reduction: the digit weights 2^16, 2^8 and
+    // (M/q)^-1 folded into w16, w8, w1 < q; with |acc| <= 2^30 and q < 2^16
+    // the weighted sum stays below 2^47, exact in binary64
+    const double w1 = inv, w8 = (double)mod_q(r8 * inv, qd, qinv), w16 = (double)mod_q(r16 * inv, qd, qinv);
     __syncthreads();
     uint16_t *sX = (uint16_t *)smem;  // [128 cols][XPAD] (row index fastest)
 #pragma unroll
@@ -526,16 +530,11 @@ __global__ __launch_bounds__(256) void product_kernel(
         for (int j = 0; j < 2; j++)
 #pragma unroll
             for (int r = 0; r < 16; r++) {
-                int res;
-                if constexpr (D == 1) {
-                    res = mod_q((double)acc[0][i][j][r], qd, qinv);
-                } else {
-                    double hh = (double)mod_q((double)acc[0][i][j][r], qd, qinv);
-                    double mid = (double)mod_q((double)acc[1][i][j][r], qd, qinv);
-                    double ll = (double)mod_q((double)acc[2][i][j][r], qd, qinv);
-                    res = mod_q(fma(hh, r16, fma(mid, r8, ll)), qd, qinv);
-                }
-                int x = mod_q((double)res * inv, qd, qinv);
+                const int x = D == 1 ? mod_q((double)acc[0][i][j][r] * w1, qd, qinv)
+                                     : mod_q(fma((double)acc[0][i][j][r], w16,
+                                                 fma((double)acc[D == 1 ? 0 : 1][i][j][r], w8,
+                                                     (double)acc[D == 1 ? 0 : 2][i][j][r] * w1)),
+                                             qd, qinv);
                 int row = wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
                 int col = wn * 64 + j * 32 + (lane & 31);
                 sX[col * XPAD + row] = x_encode(x, q);
@@ -677,6 +676,10 @@ __global__ __launch_bounds__(256) void product2_kernel(
     const double qd = (double)q, qinv = mqinv[l];
     const double inv = (double)minv[l];
     const double r16 = (double)((65536 % q)), r8 = (double)(256 % q);
+    // x = S (M/q)^-1 mod q in one reduction: the digit weights 2^16, 2^8 and
+    // (M/q)^-1 folded into w16, w8, w1 < q; with |acc| <= 2^30 and q < 2^16
+    // the weighted sum stays below 2^47, exact in binary64
+    const double w1 = inv, w8 = (double)mod_q(r8 * inv, qd, qinv), w16 = (double)mod_q(r16 * inv, qd, qinv);
     uint16_t *sX = (uint16_t *)psm;  // [128 cols][XROW] (row index fastest)
 #pragma unroll
     for (int i = 0; i < MI; i++)
@@ -684,16 +687,11 @@ __global__ __launch_bounds__(256) void product2_kernel(
         for (int j = 0; j < 2; j++)
 #pragma unroll
             for (int r = 0; r < 16; r++) {
-                int res;
-                if constexpr (D == 1) {
-                    res = mod_q((double)acc[0][i][j][r], qd, qinv);
-                } else {
-                    double hh = (double)mod_q((double)acc[0][i][j][r], qd, qinv);
-                    double mid = (double)mod_q((double)acc[1][i][j][r], qd, qinv);
-                    double ll = (double)mod_q((double)acc[2][i][j][r], qd, qinv);
-                    res = mod_q(fma(hh, r16, fma(mid, r8, ll)), qd, qinv);
-                }
-                const int x = mod_q((double)res * inv, qd, qinv);
+                const int x = D == 1 ? mod_q((double)acc[0][i][j][r] * w1, qd, qinv)
+                                     : mod_q(fma((double)acc[0][i][j][r], w16,
+                                                 fma((double)acc[D == 1 ? 0 : 1][i][j][r], w8,
+                                                     (double)acc[D == 1 ? 0 : 2][i][j][r] * w1)),
+                                             qd, qinv);
                 const int row = wm * (BM / 2) + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
                 const int col = wn * 64 + j * 32 + (lane & 31);
                 sX[col * XROW + row] = x_encode(x, q);
@@ -862,7 +860,6 @@ __global__ __launch_bounds__(BS) void crt_kernel(
 // cross-half shuffle per limb gives each lane all limbs of one output.  That
 // lane then carries, removes k M with k = round(V / M) from the top limbs,
 // and rounds the signed result once, nearest-even, to 64N bits.
-constexpr int CRT_XROW = TILE + 8;  // u16 per staged x plane row
 constexpr int CRT_MAXW = 16;        // mantissa words handled (up to 512 bits)
 constexpr size_t CRT_NU_MAX = 40;   // words of M the matrix-core CRT reads (zero-padded)
 
