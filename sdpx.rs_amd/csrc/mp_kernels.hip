@@ -1347,6 +1347,7 @@ __global__ __launch_bounds__(3 * ROW_THREADS) void ldl_col_wave_kernel(
     __shared__ uint32_t dinv_sh[W + 2];
     __shared__ int stop;
     __shared__ int64_t pre_lo;
+    __shared__ int64_t pre_tot[64];  // pivot j + 1's window totals, slot s at [s]
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (threadIdx.x == 0) stop = status[0];
     const LdlBlock B = blocks[blockIdx.y];
@@ -1429,6 +1430,7 @@ __global__ __launch_bounds__(3 * ROW_THREADS) void ldl_col_wave_kernel(
                 group_accumulate<W, ROW_THREADS, ROW_THREADS>(
                     c.first[p], j, lane, rec_at<W>((const uint32_t *)c.r, LdlCol<W>::tri(p)), 1,
                     rec_at<W>((const uint32_t *)c.w, LdlCol<W>::tri(p)), 1, false, pre_win, SLOTS, lo_w);
+                pre_tot[lane] = window_total<ROW_THREADS>(pre_win, lane, SLOTS);  // summed off the chain
             }
         }
         if (lane == 0) pre_lo = lo_w;
@@ -1485,7 +1487,7 @@ __global__ __launch_bounds__(3 * ROW_THREADS) void ldl_col_wave_kernel(
     if (!next) return;
     const int64_t plo = pre_lo;
     if (plo != INT64_MIN) {  // pivot j + 1: the pre-accumulated windows plus the term v wv
-        int64_t T = window_total<ROW_THREADS>(pre_win, lane, SLOTS);
+        int64_t T = pre_tot[lane];
         bool fits = true;
         if (!wmp::is_zero(v) && !wmp::is_zero(wv)) {
             const int64_t sc = (int64_t)v.e + wv.e - 64 * W;

@@ -88,22 +88,35 @@ __device__ __forceinline__ void scan_merge(ScanAcc &a, const ScanAcc &b) {
     a.flags |= b.flags;
 }
 
-// Vectors contiguous in memory (sv == 1): 64 vectors x 4 position lanes.
+// Vectors contiguous in memory (sv == 1): a block takes 64 vectors x 4
+// position lanes over one chunk of positions (blockIdx.y: chunks of pch);
+// the chunks merge into out by atomic min / max / or (out set to the empty
+// accumulator first by scan_init), so the grid spans the positions of a few
+// long vectors as well as many short ones.
+__global__ void scan_init(int64_t V, int4 *__restrict__ out) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < V) out[v] = make_int4(INT_MAX, INT_MAX, INT_MIN, 0);
+}
 __global__ __launch_bounds__(256) void scan_vec_fast(const uint32_t *__restrict__ data, int n,
                                                      int64_t V, int64_t K, int64_t sv, int64_t sp,
-                                                     int4 *__restrict__ out) {
+                                                     int64_t pch, int4 *__restrict__ out) {
     __shared__ ScanAcc red[4][64];
     int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     int64_t v = (int64_t)blockIdx.x * 64 + tx;
+    const int64_t p0 = (int64_t)blockIdx.y * pch, p1 = min(K, p0 + pch);
     ScanAcc s{INT_MAX, INT_MAX, INT_MIN, 0};
     size_t W = 2 * (size_t)n + 2;
     if (v < V)
-        for (int64_t p = ty; p < K; p += 4) scan_value(data + (v * sv + p * sp) * W, n, s);
+        for (int64_t p = p0 + ty; p < p1; p += 4) scan_value(data + (v * sv + p * sp) * W, n, s);
     red[ty][tx] = s;
     __syncthreads();
     if (ty == 0 && v < V) {
         for (int t = 1; t < 4; t++) scan_merge(s, red[t][tx]);
-        out[v] = make_int4(s.lo, s.emin, s.emax, s.flags);
+        int *o = (int *)(out + v);
+        if (s.lo != INT_MAX) atomicMin(o, s.lo);
+        if (s.emin != INT_MAX) atomicMin(o + 1, s.emin);
+        if (s.emax != INT_MIN) atomicMax(o + 2, s.emax);
+        if (s.flags) atomicOr(o + 3, s.flags);
     }
 }
 
@@ -1104,8 +1117,12 @@ struct Operand {
 void launch_scan(const Operand &op, int n, int4 *out, hipStream_t s) {
     if (op.V == 0) return;
     if (op.sv == 1 && op.sp != 1) {
-        hipLaunchKernelGGL(scan_vec_fast, dim3((unsigned)((op.V + 63) / 64)), dim3(256), 0, s,
-                           op.data, n, op.V, op.K, op.sv, op.sp, out);
+        // position chunks of at least 32 (8 records a thread), at most 64 chunks
+        const int64_t chunks = std::max<int64_t>(1, std::min<int64_t>(64, op.K / 32));
+        const int64_t pch = (op.K + chunks - 1) / chunks;
+        hipLaunchKernelGGL(scan_init, dim3((unsigned)((op.V + 255) / 256)), dim3(256), 0, s, op.V, out);
+        hipLaunchKernelGGL(scan_vec_fast, dim3((unsigned)((op.V + 63) / 64), (unsigned)chunks), dim3(256), 0, s,
+                           op.data, n, op.V, op.K, op.sv, op.sp, pch, out);
     } else {
         hipLaunchKernelGGL(scan_pos_fast, dim3((unsigned)((op.V + 3) / 4)), dim3(256), 0, s,
                            op.data, n, op.V, op.K, op.sv, op.sp, out);
