@@ -50,9 +50,11 @@ void mp_trsm_base(int limbs, const void *L, int64_t ldl, int64_t n, void *B, int
 void mp_ldl_dots(int limbs, void *A, void *dinv, const int32_t *first, const int8_t *signs,
                  int sign_mode, const void *blocks, int nblocks, int64_t nmax, void *scratch,
                  const void *reg_eps, const void *reg_delta, int *status, hipStream_t st);
-// r and w triangles, the pivots, and per-row exponent ranges (16 bytes a
-// row, within one record of any width)
-inline size_t ldl_scratch_records(size_t n) { return n * (n + 1) + 2 * n; }
+// r and w triangles, the pivots, per-row exponent ranges (16 bytes a row,
+// within one record of any width), and two per-row exact-dot windows
+// (slot totals and base, 8 (2W + 21) bytes a row each: within 10 records
+// of any width)
+inline size_t ldl_scratch_records(size_t n) { return n * (n + 1) + 2 * n + 20 * n; }
 void mp_trsm_base_n(int limbs, const void *L, int64_t ldl, int64_t n, void *B, int64_t ldb,
                     int64_t m, const void *lranges, int *fail, hipStream_t st);
 void mp_transpose(int limbs, const void *in, int64_t ldi, int64_t m, int64_t n, void *out,

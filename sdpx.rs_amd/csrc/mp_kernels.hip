@@ -1087,6 +1087,7 @@ template <int W>
 struct LdlCol {
     uint32_t *a, *dv, *r, *w, *dstore;
     int4 *rng;
+    int64_t *prow;  // [2][n][SLOTS + 1]: row windows by column parity (slot totals, then the base)
     const int32_t *first;
     int64_t n, ld;
     static __device__ int64_t tri(int64_t i) { return i * (i + 1) / 2; }
@@ -1101,6 +1102,11 @@ struct LdlCol {
         w = r + tri(n) * (W + 2);
         dstore = w + tri(n) * (W + 2);
         rng = (int4 *)(dstore + n * (W + 2));
+        prow = (int64_t *)(dstore + 2 * n * (W + 2));
+    }
+    // row i's window for column j (written by launch j - 1, read by launch j)
+    __device__ int64_t *row_window(int64_t j, int64_t i) const {
+        return prow + ((j & 1) * n + i) * (Win<W>::SLOTS + 1);
     }
 };
 
@@ -1362,6 +1368,8 @@ __global__ __launch_bounds__(3 * ROW_THREADS) void ldl_col_wave_kernel(
     wmp::WV<W> dj = wmp::zero<W>(), aij = wmp::zero<W>(), ajj = wmp::zero<W>(), eps = wmp::zero<W>(),
                delta = wmp::zero<W>();
     Val<W> x0, y0, x1, y1;
+    int64_t pre_t = 0, pre_b = INT64_MIN;
+    uint32_t last_r = 0, last_w = 0;
     int4 ri = make_int4(INT32_MAX, INT32_MIN, INT32_MAX, INT32_MIN), rj = ri;
     int32_t fi = 0;
     int sg = 1;
@@ -1386,6 +1394,13 @@ __global__ __launch_bounds__(3 * ROW_THREADS) void ldl_col_wave_kernel(
         } else {
             set_zero(x1);
             set_zero(y1);
+        }
+        if (j > 0) {  // the window launch j - 1 summed over k < j - 1, and the term k = j - 1
+            const int64_t *pw = c.row_window(j, i);
+            pre_t = lane < SLOTS ? pw[lane] : 0;
+            pre_b = pw[SLOTS];
+            last_r = wmp::load_word<W>(rec_at<W>(xr, j - 1));
+            last_w = wmp::load_word<W>(rec_at<W>(yr, j - 1));
         }
         if (next) {
             ajj = wmp::load<W>(rec_at<W>((const uint32_t *)c.a, i + i * c.ld));
@@ -1434,6 +1449,25 @@ __global__ __launch_bounds__(3 * ROW_THREADS) void ldl_col_wave_kernel(
             }
         }
         if (lane == 0) pre_lo = lo_w;
+        if (blockIdx.x > 0 && i < B.n && p < B.n) {
+            // row i's dot for column p over k < j, for the next launch (which
+            // adds the term k = j); the base leaves 128 bits of room each side
+            int64_t lo_r = INT64_MIN;
+            if (j > 0 && c.first[i] < j) {
+                const int4 rp = c.rng[p], rr = c.rng[i];  // w of row p, r of row i, over k < j (or wider)
+                int64_t hl, hh;
+                range_hint<W>(rr.x, rr.y, rp.z, rp.w, hl, hh);
+                if (hl <= hh && (hh - hl + 256) / 32 + 2 * W + 4 <= SLOTS) {
+                    lo_r = hl - 128;
+                    group_accumulate<W, ROW_THREADS, ROW_THREADS>(
+                        max(c.first[i], c.first[p]), j, lane, rec_at<W>((const uint32_t *)c.r, LdlCol<W>::tri(i)), 1,
+                        rec_at<W>((const uint32_t *)c.w, LdlCol<W>::tri(p)), 1, false, pre_win, SLOTS, lo_r);
+                    const int64_t T = window_total<ROW_THREADS>(pre_win, lane, SLOTS);
+                    if (lane < SLOTS) c.row_window(p, i)[lane] = T;
+                }
+            }
+            if (lane == 0) c.row_window(p, i)[SLOTS] = lo_r;
+        }
         __syncthreads();
         return;
     }
@@ -1445,7 +1479,22 @@ __global__ __launch_bounds__(3 * ROW_THREADS) void ldl_col_wave_kernel(
         int64_t hl, hh;
         range_hint<W>(ri.x, ri.y, rj.z, rj.w, hl, hh);
         wmp::WV<W> t;
-        if (hl <= hh && (hh - hl) / 32 + 2 * W + 4 <= SLOTS) {  // the window from the ranges; terms preloaded
+        bool done = false;
+        if (pre_b != INT64_MIN) {  // the window summed by launch j - 1 plus the term k = j - 1
+            const wmp::WV<W> rl = wmp::from_word<W>(last_r), wl = wmp::from_word<W>(last_w);
+            bool fits = true;
+            if (!wmp::is_zero(rl) && !wmp::is_zero(wl)) {
+                const int64_t sc = (int64_t)rl.e + wl.e - 64 * W;
+                fits = sc >= pre_b && (sc - pre_b) / 32 + 2 * W + 4 <= SLOTS;
+                if (fits) pre_t = wmp::add_product<W, SLOTS>(pre_t, rl, wl, sc - pre_b, false);
+            }
+            if (fits) {
+                t = wmp::round_window<W, SLOTS>(pre_t, pre_b);
+                done = true;
+            }
+        }
+        if (done) {
+        } else if (hl <= hh && (hh - hl) / 32 + 2 * W + 4 <= SLOTS) {  // the window from the ranges; terms preloaded
             const int used = (int)min((int64_t)SLOTS, (hh - hl) / 32 + 2 * W + 1);
             lds_i64 *ls = (lds_i64 *)slots;
             for (int sl = 0; sl < used; sl++) ls[sl * ROW_THREADS] = 0;
