@@ -235,10 +235,12 @@ __global__ __launch_bounds__(256) void encode_kernel(
 // C_l = 128 sum_b T_bl.  Per 32 values x 32 moduli that is two 32x32x32
 // MFMAs per 32 bytes; E_l in [2^30, 2^30 + q_l) with E_l == C_l makes
 // 256 GH + GL + E_l a positive 31-bit integer, reduced by a Barrett step
-// (floor(2^32 / q) estimate, one correction).  A workgroup encodes a tile of
-// 4 vectors x 64 positions (wave w: vector v0 + w, two 32-position M-tiles),
+// (floor(2^32 / q) estimate, one correction).  A workgroup encodes tiles of
+// 4 vectors x 64 positions (wave w: vector v0 + w, two 32-position M-tiles)
 // one block of 32 moduli at a time, staging the digit bytes in LDS so the
-// plane writes are 16-byte runs along k.
+// plane writes are 16-byte runs along k.  With few tiles the blocks of a
+// tile run side by side instead (blockIdx.y: the block, gridDim.y of them,
+// each workgroup holding only its block's table rows).
 constexpr int ENC_TV = 4, ENC_TP = 64;
 constexpr int ENC_ROW = ENC_TV * ENC_TP + 16;  // LDS bytes per staged plane
 
@@ -250,22 +252,24 @@ __global__ __launch_bounds__(256) void encode_mfma_kernel(
     int64_t tiles_v, int64_t tiles_p) {
     extern __shared__ __attribute__((aligned(16))) int8_t esm[];
     const int bpad = 32 * nk, trow = bpad + 16;  // table row: padded for b128 reads
-    const int nblocks = (L + 31) / 32, lpad = 32 * nblocks;
-    int8_t *sth = esm, *stl = esm + lpad * trow;
-    int8_t *sout = esm + 2 * lpad * trow;  // [64 staged planes][ENC_ROW]
+    const int nblocks = (L + 31) / 32;
+    const int split = gridDim.y > 1;                        // one block per workgroup
+    const int srows = split ? 32 : 32 * nblocks, row0 = split ? 32 * (int)blockIdx.y : 0;
+    int8_t *sth = esm, *stl = esm + srows * trow;
+    int8_t *sout = esm + 2 * srows * trow;  // [64 staged planes][ENC_ROW]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int h = lane >> 5, col = lane & 31;
     const int W32 = 2 * n + 2;
-    // tables -> LDS (zero rows for padded moduli)
-    for (int e = tid; e < lpad * (bpad / 4); e += 256) {
-        const int l = e / (bpad / 4), w4 = e - l * (bpad / 4);
+    // table rows -> LDS (zero rows for padded moduli)
+    for (int e = tid; e < srows * (bpad / 4); e += 256) {
+        const int lr = e / (bpad / 4), w4 = e - lr * (bpad / 4), l = row0 + lr;
         uint32_t vh = 0, vl = 0;
         if (l < L) {
             vh = ((const uint32_t *)th)[(size_t)l * (bpad / 4) + w4];
             vl = ((const uint32_t *)tl)[(size_t)l * (bpad / 4) + w4];
         }
-        *(uint32_t *)(sth + l * trow + 4 * w4) = vh;
-        *(uint32_t *)(stl + l * trow + 4 * w4) = vl;
+        *(uint32_t *)(sth + lr * trow + 4 * w4) = vh;
+        *(uint32_t *)(stl + lr * trow + 4 * w4) = vl;
     }
     __syncthreads();
     const int64_t ntiles = tiles_v * tiles_p;
@@ -313,7 +317,7 @@ __global__ __launch_bounds__(256) void encode_mfma_kernel(
                 }
             }
         }
-        for (int nb = 0; nb < nblocks; nb++) {
+        for (int nb = blockIdx.y; nb < nblocks; nb += gridDim.y) {
             const int c = 32 * nb + col;  // this lane's modulus
             const int pfirst = 32 * nb <= single ? 32 * nb : single + 2 * (32 * nb - single);
             const int clast = min(L, 32 * nb + 32);
@@ -324,7 +328,7 @@ __global__ __launch_bounds__(256) void encode_mfma_kernel(
                 mqc = mbar[c];
                 ec = econst[c];
             }
-            const int8_t *bh = sth + c * trow + 16 * h, *bl = stl + c * trow + 16 * h;
+            const int8_t *bh = sth + (c - row0) * trow + 16 * h, *bl = stl + (c - row0) * trow + 16 * h;
             __syncthreads();  // staged planes of the previous block are out
 #pragma unroll
             for (int t = 0; t < 2; t++) {
@@ -1224,15 +1228,17 @@ constexpr int ENC_MAX_NK = 3;
 void launch_encode_mfma(const Operand &op, int n, const int4 *scan, int nk, const uint8_t *d_tab,
                         const EncodeTables &t, int L, int single, int8_t *planes,
                         int64_t plane_stride, int64_t k_pad, int64_t v_pad, hipStream_t s) {
-    const int lpad = (L + 31) / 32 * 32, trow = 32 * nk + 16;
-    const size_t lds = 2 * (size_t)lpad * trow + 64 * (size_t)ENC_ROW;
+    const int nblocks = (L + 31) / 32, trow = 32 * nk + 16;
+    const int64_t tiles_v = v_pad / ENC_TV, tiles_p = k_pad / ENC_TP;
+    const int64_t tiles = tiles_v * tiles_p;
+    // few tiles: the modulus blocks side by side (the grid would not fill the chip)
+    const int gy = tiles < 1024 ? nblocks : 1;
+    const size_t lds = 2 * (size_t)(gy > 1 ? 32 : 32 * nblocks) * trow + 64 * (size_t)ENC_ROW;
     if (lds > 65536)
         HIP_CHECK(hipFuncSetAttribute((const void *)encode_mfma_kernel,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    const int64_t tiles_v = v_pad / ENC_TV, tiles_p = k_pad / ENC_TP;
-    const int64_t tiles = tiles_v * tiles_p;
     const unsigned grid = (unsigned)std::min<int64_t>(tiles, 2048);
-    hipLaunchKernelGGL(encode_mfma_kernel, dim3(grid), dim3(256), lds, s, op.data, n, op.V, op.K,
+    hipLaunchKernelGGL(encode_mfma_kernel, dim3(grid, gy), dim3(256), lds, s, op.data, n, op.V, op.K,
                        op.sv, op.sp, scan, nk, (const int8_t *)(d_tab + t.th),
                        (const int8_t *)(d_tab + t.tl), (const uint32_t *)(d_tab + t.e),
                        (const uint32_t *)(d_tab + t.q), (const uint32_t *)(d_tab + t.mbar), L,
