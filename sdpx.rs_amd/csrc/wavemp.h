@@ -296,24 +296,41 @@ __device__ __forceinline__ WV<W> recip(const WV<W> &d) {
 }
 
 // An exact dot's window: lane s holds slot s's signed 64-bit total
-// (weight 2^(32 s) at scale base; lanes >= SLOTS hold 0).  Carries ripple
-// until no lane below the top slot has one left (the top slot keeps the
-// sign), a negative total is negated across lanes, then one rounding: the
-// value wave_round forms.
+// (weight 2^(32 s) at scale base; lanes >= SLOTS hold 0, the top slot keeps
+// the sign).  The carries resolve in fixed steps: each slot below the top
+// keeps its low word and sends the rest (|T| < 2^62, so under 2^30) up one
+// lane; the words then carry at most +1 or -1 into the next lane, and those
+// resolve by lookahead over the lanes below the top (the +1 carries, then
+// the -1 borrows), whatever leaves the top word going into the top slot's
+// signed total.  A negative total is negated across lanes, then one
+// rounding: the value wave_round forms.
 template <int W, int SLOTS>
 __device__ __forceinline__ WV<W> round_window(int64_t T, int64_t base) {
+    static_assert(SLOTS <= 64, "one slot a lane");
     const int k = lane();
     const bool below_top = k < SLOTS - 1;
-    for (int it = 0; it < SLOTS; it++) {
-        const int c = below_top ? (int)(T >> 32) : 0;  // T >> 32 fits 32 bits
-        if (ballot(c != 0) == 0) break;
-        if (below_top) T &= 0xFFFFFFFFll;
-        T += (int32_t)up1((uint32_t)c);
-    }
-    const int64_t ttop = (int64_t)(((uint64_t)rl((uint32_t)((uint64_t)T >> 32), SLOTS - 1) << 32) |
-                                   rl((uint32_t)(uint64_t)T, SLOTS - 1));
+    // multi-bit carries, one lane up
+    const int32_t c1 = below_top ? (int32_t)(T >> 32) : 0;
+    int64_t U = (below_top ? (int64_t)(uint32_t)(uint64_t)T : T) + (int32_t)up1((uint32_t)c1);
+    // below the top: U in (-2^31, 2^32 + 2^31), a word and a carry of -1, 0 or +1
+    const int32_t c2 = below_top ? (int32_t)(U >> 32) : 0;
+    const uint32_t ap = up1(c2 == 1 ? 1u : 0u), an = up1(c2 == -1 ? 1u : 0u);
+    uint32_t x = (uint32_t)(uint64_t)U + (below_top ? ap : 0u);
+    const uint64_t G = ballot(below_top && ap != 0u && x == 0u), P = ballot(below_top && x == 0xFFFFFFFFu);
+    const uint64_t C = ((G | P) + G) ^ P;  // +1 carries into each lane
+    const uint32_t cp = (uint32_t)((C >> k) & 1u);
+    x += below_top ? cp : 0u;
+    const uint32_t x0 = x;
+    x -= below_top ? an : 0u;
+    const uint64_t G2 = ballot(below_top && an != 0u && x0 == 0u), P2 = ballot(below_top && x == 0u);
+    const uint64_t B = ((G2 | P2) + G2) ^ P2;  // -1 borrows into each lane
+    const uint32_t bn = (uint32_t)((B >> k) & 1u);
+    x -= below_top ? bn : 0u;
+    if (k == SLOTS - 1) U += (int64_t)ap + cp - an - bn;  // the top slot takes what leaves the words below
+    const int64_t ttop = (int64_t)(((uint64_t)rl((uint32_t)((uint64_t)U >> 32), SLOTS - 1) << 32) |
+                                   rl((uint32_t)(uint64_t)U, SLOTS - 1));
     const bool neg = ttop < 0;
-    uint32_t w = k < SLOTS ? (uint32_t)(uint64_t)T : 0u;
+    uint32_t w = below_top ? x : (k == SLOTS - 1 ? (uint32_t)(uint64_t)U : 0u);
     if (neg) {
         const uint64_t nzm = ballot(w != 0u);
         const int f = __builtin_ctzll(nzm);

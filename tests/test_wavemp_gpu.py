@@ -172,11 +172,24 @@ def test_wave_window_rounding(gpu, nl):
     T[n // 10: n // 5, :] = 0                 # a single word
     T[np.arange(n // 10, n // 5), top[n // 10: n // 5]] = rng.integers(-2**31, 2**31, n // 10)
     bo = np.stack([rng.integers(-3000, 3000, n), rng.integers(0, 32 * (SL - 2 - 2 * W), n)], axis=1)
+    # carry and borrow chains through many slots: runs of all-ones or zero
+    # words with a +1 / -1 entering at the bottom, and a small top slot
+    m32 = np.int64(0xFFFFFFFF)
+    for c in range(n // 3, n // 3 + 400):
+        run = int(rng.integers(2, SL - 2))
+        T[c, :] = 0
+        if c % 2:
+            T[c, :run] = m32
+            T[c, 0] = m32 + rng.integers(1, 3)          # +1 / +2 ripples up the ones
+        else:
+            T[c, 0] = -rng.integers(1, 3)               # borrows through the zeros
+            T[c, run] = rng.integers(1, 2**31)
+        T[c, SL - 1] = rng.integers(-2, 3)              # the top slot keeps the sign
     z = mp.zeros(n, nl)
     S, O = _run(lib, nl, 5, z, z, T, bo)
     _check(S, O, "window", (mp.REGULAR_KIND, mp.ZERO_KIND))
     a = mp.random_full(n, nl, 60, rng)
     b = mp.random_full(n, nl, 60, rng)
-    T[n // 5: n // 3, :] = 0                  # the product alone
+    T[n // 5: n // 3, :] = 0                  # the product alone (the chains above stay)
     S, O = _run(lib, nl, 6, a, b, T, bo)
     _check(S, O, "window + a * b")
