@@ -257,26 +257,11 @@ __device__ __forceinline__ void group_finish(lds_i64 *ls, int gl, int used, int6
     round_norm<W, GW + 1>(neg, g, base + 32 * (int64_t)(lo - 1), out);  // g[GW] = w[top] != 0
 }
 
-// group_finish for a whole-wave group (G = 64, SLOTS <= 64) with the
-// carries resolved across lanes: lane s sums slot s over the wave, then each
-// lane below the top slot keeps its low word and passes the rest to the lane
-// above, until no lane has a carry left (a ripple through every slot at
-// most; usually two or three steps).  The top slot keeps the sign.  A
-// negative total is negated across lanes (the +1 stops at the lowest
-// non-zero word), the top and lowest non-zero words come from one ballot,
-// and the words under the top are read with uniform lane indices, so the
-// rounding runs once on scalar values.  Same words, sign and rounding as
-// group_finish.
-// Lane i receives lane i - 1's value (lane 0 receives 0): a whole-wave DPP
-// shift.
-__device__ __forceinline__ int lane_up(int v) { return __builtin_amdgcn_mov_dpp(v, 0x138, 0xf, 0xf, true); }
-// A lane's 64-bit value in every lane (lane: a wave-uniform index).
-__device__ __forceinline__ int64_t readlane64(int64_t v, int lane) {
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(uint64_t)v, lane);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), lane);
-    return (int64_t)(((uint64_t)hi << 32) | lo);
-}
-
+// group_finish for a whole-wave group (G = 64, SLOTS <= 64): lane s sums
+// slot s over the wave, the window is carried and rounded on the wave
+// (wmp::round_window: fixed carry steps, the top slot keeping the sign,
+// round and sticky bits from ballots) and read into registers.  Same
+// result as group_finish.
 template <int W, int SLOTS>
 __device__ __forceinline__ void wave_round(int64_t T, int gl, int64_t base, Val<W> &out);
 
@@ -304,41 +289,11 @@ __device__ __forceinline__ void wave_finish(lds_i64 *ls, int gl, int used, int64
 
 // The rounding half of wave_finish: lane s holds slot s's total (lanes at
 // or above SLOTS hold zero); every lane ends with RN of the window's value
-// at scale base.
+// at scale base (the window rounded on the wave, then read into registers).
 template <int W, int SLOTS>
 __device__ __forceinline__ void wave_round(int64_t T, int gl, int64_t base, Val<W> &out) {
-    const bool below_top = gl < SLOTS - 1;
-    for (int it = 0; it < SLOTS; it++) {
-        const int c = below_top ? (int)(T >> 32) : 0;  // T >> 32 fits 32 bits
-        if (__ballot(c != 0) == 0) break;
-        if (below_top) T &= 0xFFFFFFFFll;
-        T += lane_up(c);
-    }
-    const int64_t ttop = readlane64(T, SLOTS - 1);
-    const bool neg = ttop < 0;
-    uint32_t w = gl < SLOTS ? (uint32_t)(uint64_t)T : 0u;
-    if (neg) {
-        const uint64_t nzm = __ballot(w != 0);  // non-zero: the total is not 0
-        const int f = __builtin_ctzll(nzm);
-        w = gl < f ? 0u : (gl == f ? (uint32_t)(0u - w) : ~w);
-        if (gl >= SLOTS) w = 0u;
-    }
-    const uint64_t nz = __ballot(w != 0);
-    if (nz == 0) {
-        set_zero(out);
-        return;
-    }
-    constexpr int GW = W + 2;
-    const int top = 63 - __builtin_clzll(nz), low = __builtin_ctzll(nz);
-    const int lo = top - GW + 1;
-    uint32_t g[GW + 1];
-    g[0] = low < lo ? 1u : 0u;  // one sticky bit below the gathered words
-#pragma unroll
-    for (int i = 0; i < GW; i++) {
-        const int k = lo + i;
-        g[i + 1] = k >= 0 ? (uint32_t)__builtin_amdgcn_readlane((int)w, k) : 0u;
-    }
-    round_norm<W, GW + 1>(neg, g, base + 32 * (int64_t)(lo - 1), out);  // g[GW] = w[top] != 0
+    (void)gl;
+    out = wmp::to_val(wmp::round_window<W, SLOTS>(T, base));
 }
 
 // The lane windows of a group dot before its finish: slots [0, used) of
