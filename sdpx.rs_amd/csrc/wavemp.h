@@ -252,11 +252,13 @@ __device__ __forceinline__ WV<W> recip(const WV<W> &d) {
     const uint64_t vt = rl(v, W - 1), vs = rl(v, W - 2);
     const double rcp = 1.0 / (double)vt;
     const double rv2 = 1.0 / fma((double)vt, 4294967296.0, (double)vs);  // 1 / V2
-    uint32_t R = k == W - 1 ? 2u : 0u;  // numerator words u[W+1 .. 2W+1]: u[2W] = 2
+    // Q < 2^(P+2) has W + 1 digits (its digit W + 1 is 0): the first window
+    // is numerator words u[W .. 2W] with u[2W] = 2
+    uint32_t R = k == W ? 2u : 0u;
     uint32_t Q = 0;
 #pragma unroll 1
-    for (int j = QW - 1; j >= 0; j--) {
-        if (j < QW - 1) R = up1(R);  // next window: R * 2^32 (the numerator's lower words are 0)
+    for (int j = QW - 2; j >= 0; j--) {
+        if (j < QW - 2) R = up1(R);  // next window: R * 2^32 (the numerator's lower words are 0)
         const uint32_t r2 = rl(R, W), r1 = rl(R, W - 1), r0 = rl(R, W - 2);
         const double est = fma(fma((double)r2, 4294967296.0, (double)r1), 4294967296.0, (double)r0) * rv2;
         const double fl = floor(est), fr = est - fl;
