@@ -555,10 +555,29 @@ __global__ __launch_bounds__(RL_THREADS) void trsm_rl_kernel(
     // word w of x_c, in column c's window after the chain has consumed it
     auto xword = [&](int64_t c, int w) -> lds_u32 * { return (lds_u32 *)(win + (w >> 1) * RL_COLS + c) + (w & 1); };
     const int64_t lo = low[i];
+    // a term outside its window: ovf_step[j & 1] stops every wave after the
+    // barrier of step j (two slots, so a wave already in step j + 1 never
+    // writes the slot another wave is about to read); ovf_any for the end
+    __shared__ int ovf_step[2], ovf_any;
     for (int e = tid; e < SLOTS * RL_COLS; e += RL_THREADS) win[e] = 0;
     for (int e = tid; e < RL_COLS; e += RL_THREADS) bases[e] = INT64_MIN;
+    if (tid == 0) {
+        ovf_step[0] = ovf_step[1] = 0;
+        ovf_any = 0;
+    }
     __syncthreads();
     bool ovf = false;
+    // the waves exchange only LDS data inside the loop: a barrier with LDS fences
+    auto step_end = [&](int64_t j) -> bool {
+        if (ovf) {
+            ovf_step[j & 1] = 1;
+            ovf_any = 1;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+        return ovf_step[j & 1] != 0;
+    };
     if (wave == 0) {  // the chain
         wmp::WV<W> xprev = wmp::zero<W>();
         // b_j and L_{j,j-1} one step ahead
@@ -585,7 +604,7 @@ __global__ __launch_bounds__(RL_THREADS) void trsm_rl_kernel(
             xprev = wmp::add(bj, t, true);  // x_j = RN(b_j - t)
             if (lane < W + 2)  // after the window's reads above (LDS operations of a wave complete in order)
                 *xword(j, lane) = lane < W ? xprev.w : (lane == W ? (uint32_t)xprev.kind : (uint32_t)xprev.e);
-            if (__syncthreads_or(ovf)) break;
+            if (step_end(j)) break;
         }
     } else {  // thread c adds x_k L_ck for c > k + 1, one step behind the chain
         const int c = tid - 64;
@@ -620,10 +639,11 @@ __global__ __launch_bounds__(RL_THREADS) void trsm_rl_kernel(
                     }
                 }
             }
-            if (__syncthreads_or(ovf)) break;
+            if (step_end(j)) break;
         }
     }
-    if (__syncthreads_or(ovf)) {
+    __syncthreads();
+    if (ovf_any) {
         if (tid == 0) fb_rows[atomicAdd(fb_count, 1)] = (int32_t)i;
         return;
     }
