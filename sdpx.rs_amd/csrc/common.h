@@ -58,6 +58,7 @@ struct Device {
     hipStream_t stream = nullptr;
     Workspace ws;
     PinnedRing pin;
+    int64_t *readback = nullptr;  // pinned, 16 words that kernels write for the host to read after a sync
     std::recursive_mutex mu;
 };
 Device &device();  // initialised on first use (device 0 unless selected)
@@ -66,6 +67,8 @@ void device_select(int id);
 // return) into device memory dst, queued on the library stream without a
 // wait.
 void upload_async(void *dst, const void *src, size_t bytes);
+// The device's pinned read-back words (16 int64), allocated on first use.
+int64_t *readback_words();
 
 // Value layout helpers: a value is 2N+2 u32 words (limbs, kind, exponent).
 inline size_t value_words(int n) { return 2 * (size_t)n + 2; }

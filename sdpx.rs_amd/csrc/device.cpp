@@ -62,6 +62,12 @@ void upload_async(void *dst, const void *src, size_t bytes) {
     d.pin.top = off + bytes;
 }
 
+int64_t *readback_words() {
+    Device &d = device();
+    if (!d.readback) HIP_CHECK(hipHostMalloc((void **)&d.readback, 16 * sizeof(int64_t), hipHostMallocDefault));
+    return d.readback;
+}
+
 static void init_device(int id) {
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
@@ -94,6 +100,8 @@ void device_select(int id) {
         if (g_device.stream) (void)hipStreamSynchronize(g_device.stream);
         if (g_device.pin.base) (void)hipHostFree(g_device.pin.base);
         g_device.pin = PinnedRing{};
+        if (g_device.readback) (void)hipHostFree(g_device.readback);
+        g_device.readback = nullptr;
         if (g_device.stream) (void)hipStreamDestroy(g_device.stream);
         g_device.stream = nullptr;
     }

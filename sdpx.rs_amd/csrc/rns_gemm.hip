@@ -1140,19 +1140,62 @@ struct Window {
     bool nonfinite = false, any = false;
 };
 
-Window reduce_scan(const std::vector<int4> &s) {
-    Window w;
-    for (const int4 &v : s) {
-        if (v.w & 1) w.nonfinite = true;
-        if (v.w & 2) {
-            w.any = true;
-            w.width = std::max<int64_t>(w.width, (int64_t)v.z - v.x);
-            w.emin = std::min<int64_t>(w.emin, v.y);
-            w.emax = std::max<int64_t>(w.emax, v.z);
+// The operand's window from its per-vector scan (the largest top-to-lowest
+// bit spread of a vector, the exponent range, the flags), written to
+// out[0..5) = width, emin, emax, non-finite, any: one block, so the host
+// reads five words instead of the whole scan.
+__global__ __launch_bounds__(256) void scan_summary(const int4 *__restrict__ scan, int64_t V, int64_t *out) {
+    __shared__ int64_t red[4][4];
+    int64_t width = 0, emin = INT64_MAX, emax = INT64_MIN;
+    int flags = 0;
+    for (int64_t v = threadIdx.x; v < V; v += 256) {
+        const int4 x = scan[v];
+        flags |= x.w;
+        if (x.w & 2) {
+            width = max(width, (int64_t)x.z - x.x);
+            emin = min(emin, (int64_t)x.y);
+            emax = max(emax, (int64_t)x.z);
         }
     }
-    return w;
+    for (int off = 32; off > 0; off >>= 1) {
+        width = max(width, (int64_t)__shfl_xor(width, off, 64));
+        emin = min(emin, (int64_t)__shfl_xor(emin, off, 64));
+        emax = max(emax, (int64_t)__shfl_xor(emax, off, 64));
+        flags |= __shfl_xor(flags, off, 64);
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[wave][0] = width;
+        red[wave][1] = emin;
+        red[wave][2] = emax;
+        red[wave][3] = flags;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; w++) {
+            width = max(width, red[w][0]);
+            emin = min(emin, red[w][1]);
+            emax = max(emax, red[w][2]);
+            flags |= (int)red[w][3];
+        }
+        out[0] = width;
+        out[1] = emin;
+        out[2] = emax;
+        out[3] = flags & 1;
+        out[4] = (flags >> 1) & 1;
+    }
 }
+
+Window window_of(const int64_t *w) {
+    Window r;
+    r.width = w[0];
+    r.emin = w[1];
+    r.emax = w[2];
+    r.nonfinite = w[3] != 0;
+    r.any = w[4] != 0;
+    return r;
+}
+
 
 template <int TJMAX>
 void launch_encode_t(const Operand &op, int n, const int4 *scan, int TJ, const double *pow2,
@@ -1485,14 +1528,12 @@ int rns_gemm_device(const RnsGemmArgs &g) {
     int4 *scan_b = same ? scan_a : (int4 *)ws.get(S_SCAN_B, sizeof(int4) * nn);
     launch_scan(oa, n, scan_a, s);
     if (!same) launch_scan(ob, n, scan_b, s);
-    std::vector<int4> ha(m), hb;
-    HIP_CHECK(hipMemcpyAsync(ha.data(), scan_a, sizeof(int4) * m, hipMemcpyDeviceToHost, s));
-    if (!same) {
-        hb.resize(nn);
-        HIP_CHECK(hipMemcpyAsync(hb.data(), scan_b, sizeof(int4) * nn, hipMemcpyDeviceToHost, s));
-    }
+    int64_t *rb = readback_words();
+    hipLaunchKernelGGL(scan_summary, dim3(1), dim3(256), 0, s, (const int4 *)scan_a, m, rb);
+    if (!same) hipLaunchKernelGGL(scan_summary, dim3(1), dim3(256), 0, s, (const int4 *)scan_b, nn, rb + 8);
+    HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipStreamSynchronize(s));
-    Window wa = reduce_scan(ha), wb = same ? wa : reduce_scan(hb);
+    Window wa = window_of(rb), wb = same ? wa : window_of(rb + 8);
     if (wa.nonfinite || wb.nonfinite) return SDPX_GPU_DECLINED;
     if ((wa.any && wa.emax - wa.emin > MAX_SPREAD) || (wb.any && wb.emax - wb.emin > MAX_SPREAD))
         return SDPX_GPU_DECLINED;
