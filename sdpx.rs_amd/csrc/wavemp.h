@@ -305,7 +305,8 @@ __device__ __forceinline__ WV<W> recip(const WV<W> &d) {
 // resolve by lookahead over the lanes below the top (the +1 carries, then
 // the -1 borrows), whatever leaves the top word going into the top slot's
 // signed total.  A negative total is negated across lanes, then one
-// rounding: the value wave_round forms.
+// rounding: RN of the window's exact value, as the serial carry, negation
+// and round_words give it.
 template <int W, int SLOTS>
 __device__ __forceinline__ WV<W> round_window(int64_t T, int64_t base) {
     static_assert(SLOTS <= 64, "one slot a lane");
