@@ -64,7 +64,8 @@ void upload_async(void *dst, const void *src, size_t bytes) {
 
 int64_t *readback_words() {
     Device &d = device();
-    if (!d.readback) HIP_CHECK(hipHostMalloc((void **)&d.readback, 16 * sizeof(int64_t), hipHostMallocDefault));
+    // fine-grained (coherent): kernel writes reach the host without a cache flush
+    if (!d.readback) HIP_CHECK(hipHostMalloc((void **)&d.readback, 16 * sizeof(int64_t), hipHostMallocCoherent));
     return d.readback;
 }
 
