@@ -21,7 +21,11 @@
  *
  * Host-pointer entry points stage through device memory; the `_dev` variants
  * take device pointers and run on the library's stream (returned by
- * sdpx_gpu_stream) and return once the results are complete.
+ * sdpx_gpu_stream) and return once the results are complete.  That stream
+ * is created non-blocking, so it does not wait for the legacy default
+ * stream: a caller that writes a `_dev` input on another stream must make
+ * that write complete (a stream synchronize, or an event the library
+ * stream waits on) before the call.
  */
 #ifndef SDPX_GPU_H
 #define SDPX_GPU_H
@@ -31,7 +35,7 @@
 extern "C" {
 #endif
 
-#define SDPX_GPU_ABI_VERSION 1u
+#define SDPX_GPU_ABI_VERSION 2u
 #define SDPX_GPU_OK 0
 #define SDPX_GPU_INVALID 1
 #define SDPX_GPU_DECLINED 2
@@ -131,7 +135,11 @@ int32_t sdpx_gpu_rns_gemm_blocks_upper(uint32_t limbs, uint64_t m, uint64_t nblo
  * (n values) receives D^-1.  signs: 0 entries (all +1), 1 (shared) or n
  * (per pivot, +1/-1); reg_eps / reg_delta (one value each, or both NULL)
  * select the pivot regularization of DenseLeaf::pivot (618-634).  *ok = 0
- * on a failed pivot (the reference's Err; contents unspecified).  Every
+ * on a failed pivot (the reference's Err; contents unspecified).  As
+ * factor_loaded (arrow.rs:819-846), a recursive factor that fails on a pivot
+ * (one rounding per level can cancel a pivot to an exact zero that exact
+ * dots keep nonzero) is redone from the input by ldl_dots, and only that
+ * factor's regularization count is kept.  Every
  * exact product of the recursion runs as an RNS product on the GPU.  flags:
  * SDPX_GPU_LDL_DOTS forces ldl_dots at any order (the reference's
  * factor_dots fallback); SDPX_GPU_LDL_HOST_PRODUCTS keeps every product on
@@ -143,7 +151,9 @@ int32_t sdpx_gpu_mp_ldl(uint32_t limbs, uint64_t n, void *a, uint64_t lda, const
                         uint32_t flags, int32_t *ok);
 /* Device-pointer form of sdpx_gpu_mp_ldl (128/256/512 bits; signs and the
  * regularization values stay host pointers).  Returns 2 (declined) when a
- * dot's exponent spread or a product exceeds the GPU limits. */
+ * dot's exponent spread or a product exceeds the GPU limits; a declined
+ * call leaves a and dinv unspecified (the caller refactors from its own
+ * copy of the input). */
 int32_t sdpx_gpu_mp_ldl_dev(uint32_t limbs, uint64_t n, void *a, uint64_t lda, const int8_t *signs,
                             uint64_t nsigns, const void *reg_eps, const void *reg_delta, void *dinv,
                             uint32_t flags, int32_t *ok);
@@ -184,9 +194,13 @@ int32_t sdpx_gpu_mp_potrs(uint32_t limbs, uint8_t uplo, uint64_t n, uint64_t nrh
  * (nothing written) for non-finite input or a dot beyond the window. */
 int32_t sdpx_gpu_mp_trsm_lower(uint32_t limbs, uint64_t n, const void *a, void *b, uint64_t nrhs,
                                uint32_t transpose);
-/* Counters of this thread's last sdpx_gpu_mp_ldl (measurement only). */
+/* Counters of this thread's last sdpx_gpu_mp_ldl / _dev (measurement and
+ * tests; each pointer optional): exact products on the GPU and on host
+ * dots, regularized pivots, GPU multiple-precision MACs, declined (1 when
+ * the device panels declined and the host panels produced the result), and
+ * retried (1 when the recursive factor failed and exact dots redid it). */
 int32_t sdpx_gpu_mp_ldl_stats(int64_t *gpu_products, int64_t *host_products, int64_t *regularized,
-                              double *gpu_mp_macs);
+                              double *gpu_mp_macs, int64_t *declined, int64_t *retried);
 
 /* ---- Sampled Schur build + arrow factor (multiple precision) ------------
  *
@@ -205,7 +219,11 @@ int32_t sdpx_gpu_mp_ldl_stats(int64_t *gpu_products, int64_t *host_products, int
  * ld t) holds the border block C in its upper triangle on entry and the
  * border factor in its lower triangle on return; border_dinv receives D_b^-1.
  * factors (optional, one per leaf) receive L (k x k), D^-1 (k), Y (k x t).
- * Every exact product runs as an RNS product on the GPU. */
+ * Every exact product runs as an RNS product on the GPU.  Leaves of 256 or
+ * more samples are declined (2, nothing written): the reference factors them
+ * recursively and solves their Y by recursive panels (factor_loaded,
+ * arrow.rs:819; Leaf::refactor 1596-1632), which this pipeline does not.
+ * The border factor retries by exact dots as sdpx_gpu_mp_ldl does. */
 typedef struct {
     uint64_t side, samples;      /* h (basis rows), k (samples) */
     const void *rinv;            /* h x h */
@@ -230,9 +248,13 @@ int32_t sdpx_gpu_sampled_arrow_dev(uint32_t limbs, uint64_t nleaves,
                                    void *dinv_all, void *y_all, void *border,
                                    const int8_t *border_signs, uint64_t nborder_signs,
                                    void *border_dinv, int32_t *ok);
-/* Phase times (s: build, leaves, border) and product counts of this
- * thread's last sdpx_gpu_sampled_arrow (measurement only). */
-int32_t sdpx_gpu_sampled_stats(double *seconds3, int64_t *gpu_products, int64_t *host_products);
+/* Phase times (s: build, leaves, border), product counts, declined (1 when
+ * the device pipeline declined and host panels produced the result),
+ * retried (border factors redone by exact dots) and the index of a leaf
+ * whose factor failed (-1: none) of this thread's last
+ * sdpx_gpu_sampled_arrow / _dev (measurement and tests; pointers optional). */
+int32_t sdpx_gpu_sampled_stats(double *seconds3, int64_t *gpu_products, int64_t *host_products,
+                               int64_t *declined, int64_t *retried, int64_t *failed_leaf);
 
 /* Float64 form of the same Schur build + arrow factor on device pointers
  * (the binary64 dense elimination H = L L^T, Y = L^-1 B, S = C + Y^T Y of

@@ -535,6 +535,32 @@ def ldl_rec(l, ld, n, first, sign, reg, stats, dinv, off=0, doff=0):
                    stats, dinv, off + n1 + n1 * ld, doff + n1)
 
 
+REC_FACTOR_MIN = 256
+
+
+def factor_loaded(l, ld, n, sign, reg, stats, dinv):
+    """DenseLeaf::factor_loaded for wide types (arrow.rs:808-850): from order
+    REC_FACTOR_MIN (arrow.rs:62) ldl_rec; when it fails, the input restored
+    and factored by ldl_dots, the failed attempt's regularization count
+    dropped (arrow.rs:820-846).  Below that order ldl_dots.  The envelope is
+    taken once, from the input.  stats["retried"] records the retry."""
+    first = envelope(l, ld, n)
+    stats.setdefault("retried", False)
+    if n >= REC_FACTOR_MIN:
+        saved = list(l)
+        count = {"regularized": 0}
+        if ldl_rec(l, ld, n, first, sign, reg, count, dinv):
+            stats["regularized"] += count["regularized"]
+            return True
+        l[:] = saved
+        stats["retried"] = True
+    d = {}
+    ok = ldl_dots(l, ld, n, first, sign, reg, stats, d)
+    for k, v in d.items():
+        dinv[k] = v
+    return ok
+
+
 def envelope(l, ld, n):
     first = []
     for i in range(n):

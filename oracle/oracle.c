@@ -745,12 +745,18 @@ static int ora_ldl_rec(const ora_ctx *c, uint8_t *l, size_t ld, size_t n, const 
 }
 
 /* Entry point: factor the lower triangle of l (n x n, ld) in place.
- * recursive != 0: ldl_rec, else ldl_dots.  reg_eps/reg_delta NULL: no
+ * mode 0: ldl_dots; 1: ldl_rec; 2: DenseLeaf::factor_loaded for wide types
+ * (arrow.rs:808-850): from order REC_FACTOR_MIN = 256 (arrow.rs:62) ldl_rec,
+ * and when it fails (one rounding per level can cancel a pivot to an exact
+ * zero that exact dots keep nonzero, arrow.rs:820-846) the input is restored
+ * and factored by ldl_dots; the failed attempt's regularization count is
+ * dropped, as the reference's local count is; below order 256 ldl_dots.
+ * The envelope is taken once, from the input.  reg_eps/reg_delta NULL: no
  * regularization.  Returns 1 on success; *regularized counts replaced
- * pivots. */
-int oracle_ldl(int nl, uint8_t *l, size_t ld, size_t n, const int8_t *signs, size_t nsigns,
-               const uint8_t *reg_eps, const uint8_t *reg_delta, uint8_t *dinv, int recursive,
-               long long *regularized, int threads) {
+ * pivots; *retried (optional) is set when the exact-dots retry ran. */
+int oracle_ldl_mode(int nl, uint8_t *l, size_t ld, size_t n, const int8_t *signs, size_t nsigns,
+                    const uint8_t *reg_eps, const uint8_t *reg_delta, uint8_t *dinv, int mode,
+                    long long *regularized, int *retried, int threads) {
     ora_ctx c;
     ora_init(&c, nl);
     size_t *first = malloc(sizeof(size_t) * (n ? n : 1));
@@ -760,16 +766,36 @@ int oracle_ldl(int nl, uint8_t *l, size_t ld, size_t n, const int8_t *signs, siz
         first[i] = k;
     }
     ora_ldl_opts o = {reg_eps != NULL, reg_eps, reg_delta, signs, nsigns, 0, 0, threads};
-    int ok = recursive ? ora_ldl_rec(&c, l, ld, n, first, &o, dinv)
-                       : ora_ldl_dots(&c, l, ld, n, first, &o, dinv);
+    int ok;
+    if (retried) *retried = 0;
+    if (mode == 1) {
+        ok = ora_ldl_rec(&c, l, ld, n, first, &o, dinv);
+    } else if (mode == 2 && n >= 256) {
+        uint8_t *save = malloc(c.stride * n * n);
+        for (size_t j = 0; j < n; j++) memcpy(save + j * n * c.stride, l + j * ld * c.stride, n * c.stride);
+        ok = ora_ldl_rec(&c, l, ld, n, first, &o, dinv);
+        if (!ok) {
+            for (size_t j = 0; j < n; j++) memcpy(l + j * ld * c.stride, save + j * n * c.stride, n * c.stride);
+            o.regularized = 0;
+            if (retried) *retried = 1;
+            ok = ora_ldl_dots(&c, l, ld, n, first, &o, dinv);
+        }
+        free(save);
+    } else {
+        ok = ora_ldl_dots(&c, l, ld, n, first, &o, dinv);
+    }
     if (regularized) *regularized = o.regularized;
     free(first);
     return ok;
 }
 
+/* mode as above with recursive != 0 meaning factor_loaded (2). */
 int oracle_ldl(int nl, uint8_t *l, size_t ld, size_t n, const int8_t *signs, size_t nsigns,
                const uint8_t *reg_eps, const uint8_t *reg_delta, uint8_t *dinv, int recursive,
-               long long *regularized, int threads);
+               long long *regularized, int threads) {
+    return oracle_ldl_mode(nl, l, ld, n, signs, nsigns, reg_eps, reg_delta, dinv,
+                           recursive ? 2 : 0, regularized, NULL, threads);
+}
 
 /* ------------------------------------------------------------------------
  * Sampled Schur build + arrow factor restatement (sampled/mod.rs:1836-2143,
@@ -792,6 +818,11 @@ int oracle_sampled_arrow(int nl, size_t nleaves, const ora_leaf *leaves, size_t 
     ((uint64_t *)half)[nl - 1] = 1ull << 63;
     *(int32_t *)(half + 8 * nl) = KIND_REG;
     *(int32_t *)(half + 8 * nl + 4) = 0;
+    /* Leaves of order 256 and more factor recursively and solve Y by
+     * recursive panels in the reference (factor_loaded, arrow.rs:819;
+     * forward_panel_rows, arrow.rs:1596-1632); not restated here. */
+    for (size_t b = 0; b < nleaves; b++)
+        if (leaves[b].samples >= 256) return -1;
     for (size_t b = 0; b < nleaves; b++) {
         const ora_leaf *lf = &leaves[b];
         size_t h = lf->side, K = lf->samples;
@@ -881,7 +912,8 @@ int oracle_sampled_arrow(int nl, size_t nleaves, const ora_leaf *leaves, size_t 
     for (size_t j = 0; j < t; j++)
         for (size_t i = j + 1; i < t; i++) memcpy(AT(border, i + j * t), AT(border, j + i * t), st);
     long long reg = 0;
-    return oracle_ldl(nl, border, t, t, bsigns, nbsigns, NULL, NULL, border_dinv, t >= 256, &reg, threads);
+    /* factor_border (arrow.rs:3378-3404) through factor_loaded */
+    return oracle_ldl_mode(nl, border, t, t, bsigns, nbsigns, NULL, NULL, border_dinv, 2, &reg, NULL, threads);
 }
 
 /* Scalar checks for the tests. */

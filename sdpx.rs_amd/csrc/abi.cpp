@@ -42,6 +42,7 @@ int32_t invalid(const char *what) {
 namespace sdpx {
 int32_t abi_boundary(const std::function<int32_t()> &f) { return boundary(f); }
 int32_t abi_invalid(const char *what) { return invalid(what); }
+void set_last_error(const char *what) { t_error = what; }
 }  // namespace sdpx
 
 namespace {

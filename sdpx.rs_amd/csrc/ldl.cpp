@@ -351,7 +351,20 @@ bool ldl_factor(int limbs, uint8_t *l, size_t ld, size_t n, const int8_t *signs,
     Ctx cx{limbs, value_bytes(limbs), &opt, &stats};
     std::vector<size_t> first = envelope(limbs, l, ld, n);
     Sign sign = [&](size_t k) { return nsigns == 0 ? 1 : (int)signs[nsigns == 1 ? 0 : k]; };
-    if (recursive) return ldl_rec(cx, l, ld, n, first.data(), sign, dinv);
+    if (recursive) {
+        // DenseLeaf::factor_loaded (arrow.rs:819-846): a pivot that one
+        // rounding per level cancels to an exact zero fails the recursive
+        // factor; the input is restored and factored by exact dots, and the
+        // failed attempt's regularization count is dropped.
+        const size_t vb = cx.vb;
+        std::vector<uint8_t> saved(n * n * vb);
+        for (size_t j = 0; j < n; j++) std::memcpy(&saved[j * n * vb], l + j * ld * vb, n * vb);
+        const int64_t reg0 = stats.regularized;
+        if (ldl_rec(cx, l, ld, n, first.data(), sign, dinv)) return true;
+        for (size_t j = 0; j < n; j++) std::memcpy(l + j * ld * vb, &saved[j * n * vb], n * vb);
+        stats.regularized = reg0;
+        stats.retried++;
+    }
     return ldl_dots(cx, l, ld, n, first.data(), sign, dinv);
 }
 

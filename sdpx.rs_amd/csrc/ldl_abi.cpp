@@ -12,6 +12,7 @@
 namespace sdpx {
 int32_t abi_boundary(const std::function<int32_t()> &f);
 int32_t abi_invalid(const char *what);
+void set_last_error(const char *what);
 }  // namespace sdpx
 
 namespace {
@@ -129,11 +130,13 @@ int32_t sdpx_gpu_mp_ldl_solve_dev(uint32_t limbs, uint64_t n, const void *l, uin
 }
 
 int32_t sdpx_gpu_mp_ldl_stats(int64_t *gpu_products, int64_t *host_products, int64_t *regularized,
-                              double *gpu_mp_macs) {
+                              double *gpu_mp_macs, int64_t *declined, int64_t *retried) {
     if (gpu_products) *gpu_products = t_ldl_stats.gpu_products;
     if (host_products) *host_products = t_ldl_stats.host_products;
     if (regularized) *regularized = t_ldl_stats.regularized;
     if (gpu_mp_macs) *gpu_mp_macs = t_ldl_stats.gpu_mp_macs;
+    if (declined) *declined = t_ldl_stats.declined;
+    if (retried) *retried = t_ldl_stats.retried;
     return SDPX_GPU_OK;
 }
 
@@ -143,6 +146,19 @@ int32_t sdpx_gpu_mp_ldl_stats(int64_t *gpu_products, int64_t *host_products, int
 
 namespace {
 thread_local sdpx::SampledStats t_sampled_stats;
+
+// Leaves of order (samples) 256 and more factor recursively and solve Y by
+// recursive panels in the reference (factor_loaded, arrow.rs:819;
+// Leaf::refactor, arrow.rs:1596-1632); this pipeline factors every leaf by
+// exact dots, so it declines them (nothing written; the caller falls back).
+int32_t leaf_order_limit(uint64_t nleaves, const sdpx_gpu_sampled_leaf *leaves) {
+    for (uint64_t i = 0; i < nleaves; i++)
+        if (leaves[i].samples >= 256) {
+            sdpx::set_last_error("sampled leaf of order 256 or more (recursive leaf factor) declined");
+            return SDPX_GPU_DECLINED;
+        }
+    return 0;
+}
 
 // Stage a host-resident sampled problem through HBM and run the device
 // pipeline; -1 when the GPU path declines (then the host path runs).
@@ -230,6 +246,7 @@ int32_t sdpx_gpu_sampled_arrow_dev(uint32_t limbs, uint64_t nleaves,
                 return sdpx::abi_invalid("null leaf array");
         if (nleaves && (!l_all || !dinv_all || (t && !y_all)))
             return sdpx::abi_invalid("null leaf factor output");
+        if (int32_t r = leaf_order_limit(nleaves, leaves)) return r;
         sdpx::Device &d = sdpx::device();
         std::lock_guard<std::recursive_mutex> g(d.mu);
         std::vector<sdpx::SampledLeafDev> dev(nleaves);
@@ -263,6 +280,7 @@ int32_t sdpx_gpu_sampled_arrow(uint32_t limbs, uint64_t nleaves, const sdpx_gpu_
         if (t && (!border || !border_dinv)) return sdpx::abi_invalid("null border");
         if (nborder_signs != 0 && nborder_signs != 1 && nborder_signs != t)
             return sdpx::abi_invalid("border signs: 0, 1 or t entries");
+        if (int32_t r = leaf_order_limit(nleaves, leaves)) return r;
         std::vector<sdpx::SampledLeafIn> in(nleaves);
         for (uint64_t i = 0; i < nleaves; i++) {
             const auto &l = leaves[i];
@@ -285,6 +303,7 @@ int32_t sdpx_gpu_sampled_arrow(uint32_t limbs, uint64_t nleaves, const sdpx_gpu_
                 if (ok) *ok = r;
                 return SDPX_GPU_OK;
             }
+            t_sampled_stats.declined++;  // not handled on the GPU: host panels below
         }
         bool good = sdpx::sampled_arrow_factor((int)limbs, in, t, (uint8_t *)border, border_signs,
                                                nborder_signs, opt, out, (uint8_t *)border_dinv,
@@ -304,7 +323,8 @@ int32_t sdpx_gpu_sampled_arrow(uint32_t limbs, uint64_t nleaves, const sdpx_gpu_
     });
 }
 
-int32_t sdpx_gpu_sampled_stats(double *seconds3, int64_t *gpu_products, int64_t *host_products) {
+int32_t sdpx_gpu_sampled_stats(double *seconds3, int64_t *gpu_products, int64_t *host_products,
+                               int64_t *declined, int64_t *retried, int64_t *failed_leaf) {
     if (seconds3) {
         seconds3[0] = t_sampled_stats.build_s;
         seconds3[1] = t_sampled_stats.leaves_s;
@@ -312,6 +332,9 @@ int32_t sdpx_gpu_sampled_stats(double *seconds3, int64_t *gpu_products, int64_t 
     }
     if (gpu_products) *gpu_products = t_sampled_stats.gpu_products;
     if (host_products) *host_products = t_sampled_stats.host_products;
+    if (declined) *declined = t_sampled_stats.declined;
+    if (retried) *retried = t_sampled_stats.retried;
+    if (failed_leaf) *failed_leaf = t_sampled_stats.failed_leaf;
     return SDPX_GPU_OK;
 }
 

@@ -20,6 +20,9 @@
 namespace sdpx {
 namespace {
 
+// Workspace slots of the device factor (the ABI's staging uses 72-79).
+constexpr int WS_LDL_ARENA = 70, WS_LDL_SCRATCH = 71, WS_LDL_SAVED = 80;
+
 // Bump allocator over one workspace slot; stream order makes reuse after a
 // reset safe (all users are on the library stream).
 struct Arena {
@@ -246,7 +249,7 @@ struct DeviceSetup {
                 size_t nsigns, const void *reg_eps_host, const void *reg_delta_host, LdlStats *st)
         : dev(device()) {
         const size_t vb = value_bytes(limbs);
-        arena.base = (uint8_t *)dev.ws.get(70, arena_bytes);
+        arena.base = (uint8_t *)dev.ws.get(WS_LDL_ARENA, arena_bytes);
         arena.size = arena_bytes;
         arena.top = 0;
         ctx.n = limbs;
@@ -254,7 +257,7 @@ struct DeviceSetup {
         ctx.st = dev.stream;
         ctx.arena = &arena;
         ctx.stats = st;
-        ctx.ldl_scratch = (uint8_t *)dev.ws.get(71, scratch_bytes);
+        ctx.ldl_scratch = (uint8_t *)dev.ws.get(WS_LDL_SCRATCH, scratch_bytes);
         ctx.ldl_scratch_bytes = scratch_bytes;
         // constants: status, signs, regularization values
         ctx.d_status = (int *)arena.alloc(sizeof(int) * 4);
@@ -291,6 +294,44 @@ size_t ldl_arena_bytes(size_t n, size_t vb) {
     return (size_t)(2.5 * (double)n * n * vb) + (64u << 20);
 }
 
+// DenseLeaf::factor_loaded (arrow.rs:808-850) on the device, queued and then
+// waited for: from order 256 ldl_rec, and when that fails on a pivot (one
+// rounding per level can cancel a pivot to an exact zero that exact dots
+// keep nonzero, arrow.rs:820-846) the input, saved before the factor, is
+// restored and factored by ldl_dots, the failed attempt's regularization
+// count dropped; below order 256 ldl_dots.  The envelope comes from the
+// input.  Returns 1 ok, 0 failed pivot, -1 declined (window or product).
+int factor_loaded_device(DeviceSetup &ds, uint8_t *l, size_t ld, size_t n, uint8_t *dinv,
+                         LdlStats &stats, bool recursive) {
+    DCtx &c = ds.ctx;
+    std::vector<size_t> first = envelope_device(c, l, ld, n);
+    int s[4];
+    if (recursive) {
+        uint8_t *saved = (uint8_t *)ds.dev.ws.get(WS_LDL_SAVED, n * n * c.vb);
+        HIP_CHECK(hipMemcpy2DAsync(saved, n * c.vb, l, ld * c.vb, n * c.vb, n, hipMemcpyDeviceToDevice, c.st));
+        rec_device(c, l, ld, n, first, 0, dinv);
+        ds.status(s);
+        if (c.declined || s[2] != 0 || s[0] == 2) return -1;
+        if (s[0] == 0) {
+            stats.regularized += s[1];
+            return 1;
+        }
+        HIP_CHECK(hipMemcpy2DAsync(l, ld * c.vb, saved, n * c.vb, n * c.vb, n, hipMemcpyDeviceToDevice, c.st));
+        HIP_CHECK(hipMemsetAsync(c.d_status, 0, sizeof(int) * 2, c.st));
+        stats.retried++;
+    }
+    const size_t need = ldl_scratch_records(n) * c.vb + 4096;
+    if (need > c.ldl_scratch_bytes) {
+        c.ldl_scratch = (uint8_t *)ds.dev.ws.get(WS_LDL_SCRATCH, need);
+        c.ldl_scratch_bytes = need;
+    }
+    dots_device(c, l, ld, n, first, 0, dinv);
+    ds.status(s);
+    if (c.declined || s[2] != 0 || s[0] == 2) return -1;
+    stats.regularized += s[1];
+    return s[0] == 0 ? 1 : 0;
+}
+
 }  // namespace
 
 int ldl_factor_device(int limbs, uint8_t *d_l, size_t ld, size_t n, const int8_t *signs,
@@ -300,17 +341,7 @@ int ldl_factor_device(int limbs, uint8_t *d_l, size_t ld, size_t n, const int8_t
     const size_t blk = recursive ? std::min<size_t>(n, 128) : n;
     DeviceSetup ds(limbs, ldl_arena_bytes(n, vb), ldl_scratch_records(blk) * vb + 4096, signs, nsigns,
                    reg_eps, reg_delta, &stats);
-    DCtx &c = ds.ctx;
-    std::vector<size_t> first = envelope_device(c, d_l, ld, n);
-    if (recursive)
-        rec_device(c, d_l, ld, n, first, 0, d_dinv);
-    else
-        dots_device(c, d_l, ld, n, first, 0, d_dinv);
-    int s[4];
-    ds.status(s);
-    if (c.declined || s[2] != 0 || s[0] == 2) return -1;  // caller reruns on the host
-    stats.regularized += s[1];
-    return s[0] == 0 ? 1 : 0;
+    return factor_loaded_device(ds, d_l, ld, n, d_dinv, stats, recursive);
 }
 
 int ldl_solve_device(int limbs, const uint8_t *d_l, size_t ldl, size_t n, const uint8_t *d_dinv,
@@ -402,13 +433,12 @@ int sampled_arrow_device(int limbs, const std::vector<SampledLeafDev> &leaves, s
         HIP_CHECK(hipMemsetAsync(c.d_status, 0, sizeof(int) * 2, s));
         mp_ldl_dots(limbs, d_l_all, d_dinv_all, d_first, plus, 1, d_bd, (int)bds.size(), (int64_t)kmax,
                     c.ldl_scratch, nullptr, nullptr, c.d_status, s);
-        int status[2];
-        HIP_CHECK(hipMemcpyAsync(status, c.d_status, sizeof status, hipMemcpyDeviceToHost, s));
-        HIP_CHECK(hipStreamSynchronize(s));
+        int status[4];
+        ds.status(status);
         c.arena->top = mark;
         if (status[0] == 2) c.declined = true;
         if (status[0] == 1) {
-            st.failed_leaf = 0;
+            st.failed_leaf = status[3] - 1;
             for (auto &e : ev) (void)hipEventDestroy(e);
             return 0;
         }
@@ -447,13 +477,12 @@ int sampled_arrow_device(int limbs, const std::vector<SampledLeafDev> &leaves, s
         }
     }
     HIP_CHECK(hipEventRecord(ev[2], s));
+    int border_result = -1;
     if (!c.declined && !ds.window_overflow()) {
+        // factor_border (arrow.rs:3378-3404): the Schur complement mirrored
+        // from the upper triangle, factored as factor_loaded does
         mp_mirror_upper(limbs, d_border, t, t, s);
-        std::vector<size_t> first = envelope_device(c, d_border, t, t);
-        if (t >= 256)
-            rec_device(c, d_border, t, t, first, 0, d_border_dinv);
-        else
-            dots_device(c, d_border, t, t, first, 0, d_border_dinv);
+        border_result = factor_loaded_device(ds, d_border, t, t, d_border_dinv, ls, t >= 256);
     }
     HIP_CHECK(hipEventRecord(ev[3], s));
     HIP_CHECK(hipEventSynchronize(ev[3]));
@@ -466,10 +495,8 @@ int sampled_arrow_device(int limbs, const std::vector<SampledLeafDev> &leaves, s
     st.leaves_s = b / 1e3;
     st.border_s = d / 1e3;
     st.gpu_products += ls.gpu_products;
-    int sv[4];
-    ds.status(sv);
-    if (c.declined || sv[2] != 0 || sv[0] == 2) return -1;
-    return sv[0] == 0 ? 1 : 0;
+    st.retried += ls.retried;
+    return border_result;
 }
 
 }  // namespace sdpx

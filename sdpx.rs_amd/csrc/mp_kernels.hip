@@ -1029,7 +1029,10 @@ __global__ __launch_bounds__(ldl_threads<W>()) void ldl_dots_kernel(
         __syncthreads();
     }
     if (bad) {
-        if (threadIdx.x == 0) atomicMax(status, bad);
+        if (threadIdx.x == 0) {
+            atomicMax(status, bad);
+            if (bad == 1) atomicMax(status + 3, (int)blockIdx.x + 1);  // a failing block, 1-based
+        }
         return;
     }
     // write the unit lower factor
@@ -1193,7 +1196,10 @@ __global__ __launch_bounds__(3 * ROW_THREADS) void ldl_col_kernel(
     int64_t *slots = slots_sh + lane;
     lds_i64 *pre_win = (lds_i64 *)(slots_sh + SLOTS * ROW_THREADS) + lane;
     auto fail = [&](int code) {
-        if (gl == 0) atomicMax(status, code);
+        if (gl == 0) {
+            atomicMax(status, code);
+            if (code == 1) atomicMax(status + 3, (int)blockIdx.y + 1);  // a failing block, 1-based
+        }
     };
     if (j < 0) {  // pivot 0: an empty dot
         if (wave == 0 && i == 0) {
@@ -1389,7 +1395,10 @@ __global__ __launch_bounds__(3 * ROW_THREADS) void ldl_col_wave_kernel(
     __syncthreads();
     if (stop != 0) return;  // the whole workgroup
     auto fail = [&](int code) {
-        if (lane == 0) atomicMax(status, code);
+        if (lane == 0) {
+            atomicMax(status, code);
+            if (code == 1) atomicMax(status + 3, (int)blockIdx.y + 1);  // a failing block, 1-based
+        }
     };
     if (j < 0) {  // pivot 0: an empty dot
         if (wave == 0 && i == 0) {

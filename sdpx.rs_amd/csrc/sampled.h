@@ -21,7 +21,10 @@ struct SampledLeafOut {
 };
 
 struct SampledStats {
+    // declined: the device pipeline declined and the host panels ran;
+    // retried: border factors redone by exact dots after a failed recursive one
     int64_t gpu_products = 0, host_products = 0, regularized = 0, failed_leaf = -1;
+    int64_t declined = 0, retried = 0;
     double build_s = 0, leaves_s = 0, border_s = 0;
 };
 

@@ -90,13 +90,12 @@ def _declare(L):
                                   ctypes.POINTER(i32)]
     L.sdpx_gpu_mp_ldl.restype = i32
     i64p = ctypes.POINTER(ctypes.c_int64)
-    L.sdpx_gpu_mp_ldl_stats.argtypes = [i64p, i64p, i64p, dp]
+    L.sdpx_gpu_mp_ldl_stats.argtypes = [i64p, i64p, i64p, dp, i64p, i64p]
     L.sdpx_gpu_mp_ldl_stats.restype = i32
     L.sdpx_gpu_sampled_arrow.argtypes = [u32, u64, vp, u64, vp, vp, u64, vp, vp, u32,
                                          ctypes.POINTER(i32)]
     L.sdpx_gpu_sampled_arrow.restype = i32
-    L.sdpx_gpu_sampled_stats.argtypes = [dp, ctypes.POINTER(ctypes.c_int64),
-                                         ctypes.POINTER(ctypes.c_int64)]
+    L.sdpx_gpu_sampled_stats.argtypes = [dp] + [ctypes.POINTER(ctypes.c_int64)] * 5
     L.sdpx_gpu_sampled_stats.restype = i32
     L.sdpx_gpu_sampled_arrow_f64_dev.argtypes = [u64, vp, u64, vp, vp, vp, ctypes.c_double,
                                                  ctypes.c_double, ctypes.POINTER(i32), dp]
@@ -318,11 +317,15 @@ def mp_ldl(a, n, signs=None, reg=None, flags=0):
 
 
 def mp_ldl_stats():
-    a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    """Counters of this thread's last mp_ldl: ``declined`` is 1 when the
+    device panels declined and the host panels produced the factor,
+    ``retried`` 1 when the recursive factor failed and exact dots redid it."""
+    a, b, c, e, f = (ctypes.c_int64() for _ in range(5))
     d = ctypes.c_double()
-    lib().sdpx_gpu_mp_ldl_stats(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(d))
+    lib().sdpx_gpu_mp_ldl_stats(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(d),
+                                ctypes.byref(e), ctypes.byref(f))
     return {"gpu_products": a.value, "host_products": b.value, "regularized": c.value,
-            "gpu_mp_macs": d.value}
+            "gpu_mp_macs": d.value, "declined": e.value, "retried": f.value}
 
 
 class SampledLeaf(ctypes.Structure):
@@ -361,6 +364,21 @@ def sampled_arrow(problem, nl, border_signs=(-1,), flags=0, want_factors=False):
     return bool(ok.value), border, bdinv, outs
 
 
+_lib_stream = None
+
+
+def library_stream():
+    """The library's HIP stream as a torch stream object: work queued under
+    ``with torch.cuda.stream(library_stream())`` is ordered with the
+    ``_dev`` entry points, which run on that stream."""
+    global _lib_stream
+    if _lib_stream is None:
+        import torch
+
+        _lib_stream = torch.cuda.ExternalStream(lib().sdpx_gpu_stream(), device="cuda")
+    return _lib_stream
+
+
 class SampledDev:
     """Device-resident multiple-precision form of a sampled problem (torch
     uint8 tensors on cuda holding the interchange records) for
@@ -394,8 +412,13 @@ class SampledDev:
 
     def run(self):
         """One Schur build + arrow factor; returns ok (False: a pivot failed).
-        Raises on a decline (the device limits were exceeded)."""
-        self.border.copy_(self.border0)
+        Raises on a decline (the device limits were exceeded).  The fresh
+        border is copied on the library's own stream, so the factor is
+        ordered after it."""
+        import torch
+
+        with torch.cuda.stream(library_stream()):
+            self.border.copy_(self.border0)
         ok = ctypes.c_int32()
         check(lib().sdpx_gpu_sampled_arrow_dev(self.nl, len(self.leaves), self.leaves, self.t,
                                                self.l_all.data_ptr(), self.dinv_all.data_ptr(),
@@ -411,11 +434,15 @@ class SampledDev:
 
 
 def sampled_stats():
+    """Phase times and counters of this thread's last sampled arrow call;
+    ``declined`` 1 when the device pipeline declined and host panels ran."""
     secs = (ctypes.c_double * 3)()
-    g, h = ctypes.c_int64(), ctypes.c_int64()
-    lib().sdpx_gpu_sampled_stats(secs, ctypes.byref(g), ctypes.byref(h))
+    g, h, dc, rt, fl = (ctypes.c_int64() for _ in range(5))
+    lib().sdpx_gpu_sampled_stats(secs, ctypes.byref(g), ctypes.byref(h), ctypes.byref(dc),
+                                 ctypes.byref(rt), ctypes.byref(fl))
     return {"build_s": secs[0], "leaves_s": secs[1], "border_s": secs[2],
-            "gpu_products": g.value, "host_products": h.value}
+            "gpu_products": g.value, "host_products": h.value, "declined": dc.value,
+            "retried": rt.value, "failed_leaf": fl.value}
 
 
 class SampledLeafF64(ctypes.Structure):

@@ -41,6 +41,9 @@ def oracle():
         L.oracle_ldl.argtypes = [i32, vp, sz, sz, vp, sz, vp, vp, vp, i32,
                                  ctypes.POINTER(ctypes.c_longlong), i32]
         L.oracle_ldl.restype = i32
+        L.oracle_ldl_mode.argtypes = [i32, vp, sz, sz, vp, sz, vp, vp, vp, i32,
+                                      ctypes.POINTER(ctypes.c_longlong), vp, i32]
+        L.oracle_ldl_mode.restype = i32
         for name in ("oracle_recip",):
             getattr(L, name).argtypes = [i32, vp, vp]
         for name in ("oracle_sub", "oracle_mul"):
@@ -101,9 +104,14 @@ def lcg_operands(m, n, k, spread, seed, nl, ta, tb):
     return from_mp(a, nl), ar, from_mp(b, nl), br
 
 
-def oracle_ldl(a, n, signs=None, reg=None, recursive=True, threads=8):
+def oracle_ldl(a, n, signs=None, reg=None, recursive=True, threads=8, mode=None):
     """C oracle LDL^T of the lower triangle of a (n x n records, ld n).
-    Returns (ok, l, dinv, regularized)."""
+    mode "loaded" (default; recursive=False gives "dots"): factor_loaded,
+    ldl_rec from order 256 with the exact-dots retry, else ldl_dots; "rec":
+    ldl_rec alone; "dots": ldl_dots.  Returns (ok, l, dinv, regularized)."""
+    if mode is None:
+        mode = "loaded" if recursive else "dots"
+    code = {"dots": 0, "rec": 1, "loaded": 2}[mode]
     nl = mp.limbs_of(a)
     l = a.copy()
     dinv = mp.zeros(n, nl)
@@ -112,9 +120,9 @@ def oracle_ldl(a, n, signs=None, reg=None, recursive=True, threads=8):
     if reg is not None:
         eps, delta = reg
     cnt = ctypes.c_longlong()
-    ok = oracle().oracle_ldl(nl, _p(l), n, n, _p(sg) if sg is not None else None,
-                             0 if sg is None else sg.size, _p(eps), _p(delta), _p(dinv),
-                             1 if recursive else 0, ctypes.byref(cnt), threads)
+    ok = oracle().oracle_ldl_mode(nl, _p(l), n, n, _p(sg) if sg is not None else None,
+                                  0 if sg is None else sg.size, _p(eps), _p(delta), _p(dinv),
+                                  code, ctypes.byref(cnt), None, threads)
     return bool(ok), l, dinv, cnt.value
 
 
@@ -126,7 +134,8 @@ class OraLeaf(ctypes.Structure):
 
 def oracle_sampled_arrow(problem, nl, border_signs=(-1,), threads=8):
     """C oracle of the sampled Schur build + arrow factor.  Returns
-    (ok, border, border_dinv, [(l, dinv, y) per leaf])."""
+    (ok, border, border_dinv, [(l, dinv, y) per leaf]); ok is None when a
+    leaf has 256 or more samples (the recursive leaf path, not restated)."""
     blocks = problem["blocks"]
     t = problem["t"]
     L = oracle()
@@ -150,6 +159,8 @@ def oracle_sampled_arrow(problem, nl, border_signs=(-1,), threads=8):
     signs = np.ascontiguousarray(border_signs, np.int8)
     ok = L.oracle_sampled_arrow(nl, len(blocks), leaves, t, _p(border), _p(signs), signs.size,
                                 _p(bdinv), lp, dp, yp, threads)
+    if ok < 0:  # a leaf of order 256 or more: not restated
+        return None, border, bdinv, outs
     return bool(ok), border, bdinv, outs
 
 

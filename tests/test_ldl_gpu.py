@@ -17,7 +17,7 @@ def test_reference_input_gpu_products(gpu):
     ok, L, dinv, count = oracle_ldl(A, n, signs, reg)
     okg, Lg, dg = sdpx_gpu.mp_ldl(A, n, signs, reg)
     st = sdpx_gpu.mp_ldl_stats()
-    assert st["gpu_products"] >= 1 and st["regularized"] == count
+    assert st["gpu_products"] >= 1 and st["regularized"] == count and st["declined"] == 0
     assert okg and np.all(mp.same_bits(Lg, L)) and np.all(mp.same_bits(dg, dinv))
 
 
@@ -41,7 +41,8 @@ def test_random_gpu_products(gpu, nl, n, kind):
     A = mp.from_float64(s.T.reshape(-1), nl)
     ok, L, dinv, _ = oracle_ldl(A, n, threads=16)
     okg, Lg, dg = sdpx_gpu.mp_ldl(A, n)
-    assert sdpx_gpu.mp_ldl_stats()["gpu_products"] >= 2
+    st = sdpx_gpu.mp_ldl_stats()
+    assert st["gpu_products"] >= 2 and st["declined"] == 0
     assert ok and okg
     bad = ~mp.same_bits(Lg, L)
     assert not bad.any(), f"{bad.sum()} factor entries differ"
@@ -63,7 +64,7 @@ def test_failed_pivot_reported(gpu, n):
     A = mp.from_float64(s.T.reshape(-1), nl)
     ok, _, _, _ = oracle_ldl(A, n, threads=16)
     okg, _, _ = sdpx_gpu.mp_ldl(A, n)
-    assert not ok and not okg
+    assert not ok and not okg and sdpx_gpu.mp_ldl_stats()["declined"] == 0
 
 
 _spread_case = {}
@@ -88,9 +89,10 @@ def test_trsm_right_looking_rows_and_fallback(gpu, room):
     sdpx_gpu.set_tuning("trsm_rl_room", room)
     try:
         okg, Lg, dg = sdpx_gpu.mp_ldl(A, n)
+        declined = sdpx_gpu.mp_ldl_stats()["declined"]
     finally:
         sdpx_gpu.set_tuning("trsm_rl_room", 128)
-    assert ok and okg
+    assert ok and okg and declined == 0
     assert np.all(mp.same_bits(Lg, L)) and np.all(mp.same_bits(dg, dinv))
 
 

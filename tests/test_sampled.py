@@ -45,3 +45,19 @@ def test_host_path_matches_oracle():
     idx = lower[0] + lower[1] * t
     assert np.all(mp.same_bits(bp[idx], border[idx]))
     assert np.all(mp.same_bits(dp, bdinv))
+
+
+def test_leaves_of_order_256_declined():
+    """Leaves of 256 or more samples (recursive leaf factor in the
+    reference) are declined with nothing written; the oracle does not
+    restate them either."""
+    nl = 2
+    prob = workloads.sampled_problem(nl, nblocks=2, hmin=129, hmax=129, t=4, seed=1, lift=False)
+    assert prob["blocks"][0]["k"] == 257
+    border = prob["border"].copy()
+    import pytest
+    with pytest.raises(sdpx_gpu.DeviceError, match="declined"):
+        sdpx_gpu.sampled_arrow(prob, nl, flags=sdpx_gpu.LDL_HOST_PRODUCTS)
+    ok, _, _, _ = oracle_sampled_arrow(prob, nl)
+    assert ok is None
+    assert np.all(mp.same_bits(prob["border"], border))

@@ -17,7 +17,7 @@ def compare(prob, nl, threads=16):
     st = sdpx_gpu.sampled_stats()
     print("sampled", nl, len(prob["blocks"]), prob["t"], st, flush=True)
     assert ok and okg
-    assert st["gpu_products"] > 0
+    assert st["gpu_products"] > 0 and st["declined"] == 0, "device pipeline declined"
     for (l, d, y), (lg, dgg, yg) in zip(outs, fg):
         assert np.all(mp.same_bits(lg, l))
         assert np.all(mp.same_bits(dgg, d))
@@ -49,6 +49,7 @@ def test_device_resident_entry_matches_host_entry(gpu):
     lower = np.tril_indices(t)
     idx = lower[0] + lower[1] * t
     dt = mp.dtype(4)
+    assert sdpx_gpu.sampled_stats()["declined"] == 0
     for rep in range(2):
         print("sampled dev rep", rep, flush=True)
         assert run.run() and okh
@@ -65,3 +66,16 @@ def test_device_resident_entry_matches_host_entry(gpu):
             assert np.all(mp.same_bits(dinv_all[dlo:dlo + k], d))
             assert np.all(mp.same_bits(y_all[ylo:ylo + k * t], y))
             lo, dlo, ylo = lo + k * k, dlo + k, ylo + k * t
+
+
+def test_failed_leaf_index_reported(gpu):
+    """A zero weight makes a leaf's Schur block singular: the device
+    pipeline reports that leaf's index, as the host panels do."""
+    nl = 4
+    prob = workloads.sampled_problem(nl, nblocks=5, hmin=8, hmax=14, t=64, seed=19)
+    prob["blocks"][2]["w"][3] = mp.zeros(1, nl)[0]
+    ok, _, _, _ = sdpx_gpu.sampled_arrow(prob, nl)
+    st = sdpx_gpu.sampled_stats()
+    assert not ok and st["declined"] == 0 and st["failed_leaf"] == 2
+    okh, _, _, _ = sdpx_gpu.sampled_arrow(prob, nl, flags=sdpx_gpu.LDL_HOST_PRODUCTS)
+    assert not okh and sdpx_gpu.sampled_stats()["failed_leaf"] == 2
