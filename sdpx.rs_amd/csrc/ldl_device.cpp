@@ -311,13 +311,18 @@ int factor_loaded_device(DeviceSetup &ds, uint8_t *l, size_t ld, size_t n, uint8
         HIP_CHECK(hipMemcpy2DAsync(saved, n * c.vb, l, ld * c.vb, n * c.vb, n, hipMemcpyDeviceToDevice, c.st));
         rec_device(c, l, ld, n, first, 0, dinv);
         ds.status(s);
-        if (c.declined || s[2] != 0 || s[0] == 2) return -1;
-        if (s[0] == 0) {
+        // A base block runs only while no window overflow was flagged
+        // before it, so a failed pivot (status 1) is the recursive factor's
+        // own; what ran after it (products and row kernels on the unfinished
+        // factor) is discarded, declines and overflows included.
+        if (s[0] != 1) {
+            if (c.declined || s[2] != 0 || s[0] == 2) return -1;
             stats.regularized += s[1];
             return 1;
         }
         HIP_CHECK(hipMemcpy2DAsync(l, ld * c.vb, saved, n * c.vb, n * c.vb, n, hipMemcpyDeviceToDevice, c.st));
-        HIP_CHECK(hipMemsetAsync(c.d_status, 0, sizeof(int) * 2, c.st));
+        HIP_CHECK(hipMemsetAsync(c.d_status, 0, sizeof(int) * 4, c.st));
+        c.declined = false;
         stats.retried++;
     }
     const size_t need = ldl_scratch_records(n) * c.vb + 4096;
@@ -327,9 +332,10 @@ int factor_loaded_device(DeviceSetup &ds, uint8_t *l, size_t ld, size_t n, uint8
     }
     dots_device(c, l, ld, n, first, 0, dinv);
     ds.status(s);
+    if (s[0] == 1) return 0;  // only the dots kernels ran: the failure is the factor's
     if (c.declined || s[2] != 0 || s[0] == 2) return -1;
     stats.regularized += s[1];
-    return s[0] == 0 ? 1 : 0;
+    return 1;
 }
 
 }  // namespace

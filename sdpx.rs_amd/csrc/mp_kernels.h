@@ -46,8 +46,10 @@ void mp_trsm_base(int limbs, const void *L, int64_t ldl, int64_t n, void *B, int
                   hipStream_t st);
 // status[0] = 0 ok, 1 failed pivot, 2 window overflow (zero on entry);
 // status[1] += regularized; status[3] = 1 + the index of a block with a
-// failed pivot (status is int[4]).  nmax: the largest block order.  Each
-// block's scratch holds ldl_scratch_records(n) records.
+// failed pivot (status is int[4]).  A nonzero status[0] or status[2] (a
+// row kernel's window overflow) on entry stops the launch.  nmax: the
+// largest block order.  Each block's scratch holds ldl_scratch_records(n)
+// records.
 void mp_ldl_dots(int limbs, void *A, void *dinv, const int32_t *first, const int8_t *signs,
                  int sign_mode, const void *blocks, int nblocks, int64_t nmax, void *scratch,
                  const void *reg_eps, const void *reg_delta, int *status, hipStream_t st);

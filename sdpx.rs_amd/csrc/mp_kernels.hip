@@ -889,8 +889,14 @@ __global__ __launch_bounds__(ldl_threads<W>()) void ldl_dots_kernel(
             rmin[i] = wmin[i] = INT32_MAX;
             rmax[i] = wmax[i] = INT32_MIN;
         }
-    if (threadIdx.x == 0) bad = 0;
+    __shared__ int stop;
+    if (threadIdx.x == 0) {
+        bad = 0;
+        // a failed pivot or an earlier row kernel's window overflow stops the block
+        stop = status[0] | status[2];
+    }
     __syncthreads();
+    if (stop != 0) return;  // the whole workgroup
 
     // window hint of the dot of r row i with w row j from the ranges so far
     auto hint = [&](int64_t i, int64_t j, int32_t rmn, int32_t rmx, int32_t wmn, int32_t wmx,
@@ -1185,7 +1191,7 @@ __global__ __launch_bounds__(3 * ROW_THREADS) void ldl_col_kernel(
     __shared__ uint32_t dinv_sh[W + 2];
     __shared__ int stop;
     __shared__ int64_t pre_lo;
-    if (threadIdx.x == 0) stop = status[0];
+    if (threadIdx.x == 0) stop = status[0] | status[2];  // a failed pivot, or an earlier row kernel's window overflow
     __syncthreads();
     if (stop != 0) return;  // the whole workgroup
     const LdlBlock B = blocks[blockIdx.y];
@@ -1336,7 +1342,7 @@ __global__ __launch_bounds__(3 * ROW_THREADS) void ldl_col_wave_kernel(
     __shared__ int64_t pre_lo;
     __shared__ int64_t pre_tot[64];  // pivot j + 1's window totals, slot s at [s]
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (threadIdx.x == 0) stop = status[0];
+    if (threadIdx.x == 0) stop = status[0] | status[2];  // a failed pivot, or an earlier row kernel's window overflow
     const LdlBlock B = blocks[blockIdx.y];
     const LdlCol<W> c(A, dinv, first_all, scratch, B);
     const int64_t i = j + 1 + blockIdx.x;
