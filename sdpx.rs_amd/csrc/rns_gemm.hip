@@ -989,7 +989,11 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void c
         // k = round(V / M); S = V - k M over all NUMAX words (M's words
         // zero-padded to NUMAX; S is small, so the words above nu carry its
         // sign)
-        const double vt = (double)(int32_t)l2 * 18446744073709551616.0 + (double)l1 * 4294967296.0 + (double)l0;
+        // (the same top min(nu, 3) words as m_top, so plans of one or two
+        // words scale alike)
+        double vt = (double)(int32_t)l2;
+        if (nu >= 2) vt = vt * 4294967296.0 + (double)l1;
+        if (nu >= 3) vt = vt * 4294967296.0 + (double)l0;
         const int32_t kq = (int32_t)rint(vt / m_top);
         int64_t br = 0;
 #pragma unroll

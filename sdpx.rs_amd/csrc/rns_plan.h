@@ -40,7 +40,8 @@ struct RnsPlan {
     // [t][2l] = c_{l,t}, [t][2l+1] = c_{l,t-1}, c_{l,t} the balanced base-256
     // digits of M/q_l (t <= crt_jb); rows crt_nt = crt_jb + 2, padded to
     // crt_npad rows of 2 crt_lpad bytes.  M in crt_nu 32-bit words (room for
-    // |V| <= (count/2) M and a sign), m_top = M / 2^(32 (crt_nu - 3)).
+    // |V| <= (count/2) M and a sign), m_top = M / 2^(32 max(crt_nu - 3, 0))
+    // (M's top min(crt_nu, 3) words).
     int crt_jb = 0, crt_nt = 0, crt_npad = 0, crt_lpad = 0, crt_nu = 0;
     double m_top = 0;
     std::vector<int8_t> crt_table;

@@ -247,3 +247,35 @@ def test_grouped_blocks_upper(gpu):
                         pairs.append((M.mul(dr, yi), yj))
             ref = M.exact_dot(pairs, nl)
             assert mp.same_bits(out[i + j * m: i + j * m + 1], from_mp([ref], nl))[0], (i, j)
+
+
+@pytest.mark.parametrize("bits", [1, 2, 3, 8, 13, 24, 31, 40, 53])
+def test_narrow_operands_every_small_plan(gpu, bits):
+    """Operands of a few significant bits (small integers scaled by powers
+    of two) give integer products of a few dozen bits, so the plans have
+    one, two or three moduli words: the CRT's quotient estimate must scale
+    M and V alike for every word count (a one-word plan once lifted 1 to
+    1 - (2^31 - 1) M).  Includes the 2 x 2 x 2 product of the analytic
+    sampled block (basis [1, 1; 1, -1])."""
+    rng = np.random.default_rng(bits)
+    for nl in (2, 4, 8):
+        for m, n, k in ((2, 2, 2), (5, 3, 7), (64, 65, 33), (130, 129, 300)):
+            for ta, tb in (("N", "N"), ("T", "N")):
+                ar, ac = (m, k) if ta == "N" else (k, m)
+                br, bc = (k, n) if tb == "N" else (n, k)
+                lim = 2 ** (bits - 1)
+                av = rng.integers(-lim, lim + 1, ar * ac) * np.ldexp(1.0, rng.integers(-4, 5, ar * ac))
+                bv = rng.integers(-lim, lim + 1, br * bc) * np.ldexp(1.0, rng.integers(-4, 5, br * bc))
+                a, b = mp.from_float64(av, nl), mp.from_float64(bv, nl)
+                ok, out = gpu.rns_gemm(ta, tb, m, n, k, a, ar, b, br)
+                ref = oracle_gemm(ta, tb, m, n, k, a, ar, b, br)
+                assert ok
+                assert np.all(mp.same_bits(out, ref)), (bits, nl, m, n, k, ta, tb)
+                if m == n:
+                    ok, up = gpu.rns_gemm(ta, tb, m, n, k, a, ar, b, br, upper_only=True)
+                    refp = oracle_gemm(ta, tb, m, n, k, a, ar, b, br, upper=True)
+                    assert ok and np.all(mp.same_bits(up, refp))
+    q = mp.from_float64([1.0, 1.0, 1.0, -1.0], 2)
+    eye = mp.from_float64([1.0, 0.0, 0.0, 1.0], 2)
+    ok, v = gpu.rns_gemm("N", "N", 2, 2, 2, eye, 2, q, 2)
+    assert ok and np.all(mp.same_bits(v, q))
