@@ -108,6 +108,28 @@ int32_t sdpx_gpu_rns_gemm_dev(uint32_t limbs, uint8_t transa, uint8_t transb, ui
                               uint64_t n, uint64_t k, const void *a, uint64_t lda, const void *b,
                               uint64_t ldb, uint32_t upper_only, void *out, uint64_t out_len);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): sdpx_gpu_rns_gemm_dev with every temporary (operand
+ * scans, INT8 residue planes, the CRT's residue buffer, plan tables) in one
+ * fixed array of SDPX_GPU_NOALLOC_SCRATCH_BYTES per device, compiled into
+ * the library and held until the process ends, so the call allocates
+ * nothing (no hipMalloc, hipFree or pinned host memory).  Inputs and outputs
+ * are the caller's device buffers.  Returns 6 (capacity; output untouched,
+ * with a message) when the product's temporaries exceed the array: about
+ * G (m + n) k + 2 L m n bytes (dimensions padded to 128 or 256, k to 64)
+ * for L moduli and G INT8 planes, roughly L = 65, G = 100 for full 256-bit
+ * operands and L = 120, G = 150 at 512 bits, so 256 x 256 x 256 fits at
+ * either width.  Every call on a device
+ * shares that device's array and leaves its contents behind; the library's
+ * device lock serialises calls within a process, so only calls that
+ * overlap on one device from outside that lock would race. */
+#define SDPX_GPU_NOALLOC_SCRATCH_BYTES (64u << 20)
+int32_t sdpx_gpu_rns_gemm_dev_noalloc(uint32_t limbs, uint8_t transa, uint8_t transb, uint64_t m,
+                                      uint64_t n, uint64_t k, const void *a, uint64_t lda,
+                                      const void *b, uint64_t ldb, uint32_t upper_only, void *out,
+                                      uint64_t out_len);
+
 /* Grouped exact blocks: replaces rns_blas::gemm_blocks_upper
  * (rns_blas.rs:2671-2922) behind XgemmScalar::xgemm_blocks_upper_exact
  * (traits.rs:250-258): c[i + j*m] for i <= j receives the upper triangle of

@@ -153,6 +153,25 @@ int32_t sdpx_gpu_rns_gemm_dev(uint32_t limbs, uint8_t ta, uint8_t tb, uint64_t m
     });
 }
 
+int32_t sdpx_gpu_rns_gemm_dev_noalloc(uint32_t limbs, uint8_t ta, uint8_t tb, uint64_t m, uint64_t n,
+                                      uint64_t k, const void *a, uint64_t lda, const void *b,
+                                      uint64_t ldb, uint32_t upper, void *out, uint64_t out_len) {
+    return boundary([&]() -> int32_t {
+        int32_t c = check_gemm(limbs, ta, tb, m, n, k, a, lda, b, ldb, upper, out, out_len);
+        if (c) return c;
+        sdpx::Device &dev = sdpx::device();
+        std::lock_guard<std::recursive_mutex> g(dev.mu);
+        sdpx::RnsGemmArgs args{(int)limbs, trans_norm(ta), trans_norm(tb), (int64_t)m, (int64_t)n,
+                               (int64_t)k, a, (int64_t)lda, b, (int64_t)ldb, upper != 0, out_len,
+                               out};
+        args.scratch = sdpx::rns_noalloc_scratch();
+        args.scratch_bytes = sdpx::NOALLOC_SCRATCH_BYTES;
+        const int32_t r = sdpx::rns_gemm_device(args);
+        HIP_CHECK(hipStreamSynchronize(dev.stream));  // the result is complete on return
+        return r;
+    });
+}
+
 int32_t sdpx_gpu_rns_gemm(uint32_t limbs, uint8_t ta, uint8_t tb, uint64_t m, uint64_t n,
                           uint64_t k, const void *a, uint64_t lda, const void *b, uint64_t ldb,
                           uint32_t upper, void *out, uint64_t out_len) {

@@ -2,6 +2,7 @@
 #include <cstring>
 
 #include "common.h"
+#include "rns_gemm.h"
 
 namespace sdpx {
 
@@ -82,6 +83,8 @@ static void init_device(int id) {
         g_device.stream = s;
     }
     g_device.id = id;
+    // set-up of the allocation-free exact product: this device's scratch address
+    rns_noalloc_setup();
 }
 
 Device &device() {

@@ -15,7 +15,18 @@ struct RnsGemmArgs {
     bool upper;         // only i <= j
     uint64_t out_len;   // packed upper when upper, m == n and out_len == n(n+1)/2
     void *out;          // device values, ldc = m unless packed
+    // allocation-free form: every temporary in this device memory (null:
+    // the library's workspace); more than scratch_bytes is a capacity error
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
 };
+
+// Size of the per-device array of the allocation-free exact product.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+// Looks up the current device's array address (once per device; called at
+// device set-up), and returns it.
+void rns_noalloc_setup();
+void *rns_noalloc_scratch();
 
 struct RnsStats {
     int moduli = 0, planes = 0;

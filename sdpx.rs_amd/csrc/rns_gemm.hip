@@ -1497,10 +1497,81 @@ int2 *tile_tables_dev(const std::vector<int2> &tl, const std::vector<int2> &tl2,
     return d;
 }
 
+// The fixed device memory of the allocation-free exact product
+// (sdpx_gpu_rns_gemm_dev_noalloc): one array per device, held until the
+// process ends.  Host code never passes the array's name to a kernel or a
+// copy: it uses the device address looked up once per device at set-up.
+__device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, void *> g_scratch_ptr;  // device -> address of that device's scratch
+}  // namespace
+
+void rns_noalloc_setup() {
+    const int dev = current_device();
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (g_scratch_ptr.count(dev)) return;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || ptr == nullptr)
+        throw Error(SDPX_GPU_DEVICE_ERROR, std::string("scratch address lookup failed: ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = ptr;
+}
+
+void *rns_noalloc_scratch() {
+    const int dev = current_device();
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    auto it = g_scratch_ptr.find(dev);
+    if (it == g_scratch_ptr.end())
+        throw Error(SDPX_GPU_DEVICE_ERROR, "allocation-free scratch not set up on this device (sdpx_gpu_init)");
+    return it->second;
+}
+
+namespace {
+// Device buffers of one exact product: workspace slots, or (allocation-free
+// form) consecutive 256-byte-aligned pieces of the caller's fixed scratch,
+// whose overflow is a capacity error raised before any output is written.
+struct Buffers {
+    Workspace *ws;
+    uint8_t *base;
+    size_t cap, top = 0;
+    bool fixed() const { return base != nullptr; }
+    void *get(int slot, size_t bytes) {
+        if (!base) return ws->get(slot, bytes);
+        bytes = (bytes + 255) & ~size_t(255);
+        if (top + bytes > cap)
+            throw Error(SDPX_GPU_CAPACITY, "exact product temporaries need " + std::to_string((top + bytes) >> 20) +
+                                               "+ MiB, above the " + std::to_string(cap >> 20) +
+                                               " MiB allocation-free scratch");
+        void *p = base + top;
+        top += bytes;
+        return p;
+    }
+};
+}  // namespace
+
 int rns_gemm_device(const RnsGemmArgs &g) {
     Device &dev = device();
     hipStream_t s = dev.stream;
-    Workspace &ws = dev.ws;
+    Buffers ws{&dev.ws, (uint8_t *)g.scratch, g.scratch_bytes};
+    const bool fixed = ws.fixed();
+    // allocation-free form: host tables copied into the scratch stay alive
+    // here until the stream has run the copies
+    std::vector<std::vector<uint8_t>> keep;
+    auto table_copy = [&](const void *src, size_t bytes) -> void * {
+        void *d = ws.get(-1, std::max<size_t>(bytes, 16));
+        keep.emplace_back((const uint8_t *)src, (const uint8_t *)src + bytes);
+        if (bytes) HIP_CHECK(hipMemcpyAsync(d, keep.back().data(), bytes, hipMemcpyHostToDevice, s));
+        return d;
+    };
+    struct Drain {  // the copies above read host memory owned by this call
+        bool on;
+        hipStream_t s;
+        ~Drain() {
+            if (on) (void)hipStreamSynchronize(s);
+        }
+    } drain{fixed, s};
     const int n = g.limbs;
     const bool tan = (g.ta == 'N' || g.ta == 'n');
     const bool tbn = (g.tb == 'N' || g.tb == 'n');
@@ -1526,16 +1597,23 @@ int rns_gemm_device(const RnsGemmArgs &g) {
     }
 
     // 1. scan
-    StageEvents sev(g_profile);
+    StageEvents sev(g_profile && !fixed);
     sev.mark(0, s);
     int4 *scan_a = (int4 *)ws.get(S_SCAN_A, sizeof(int4) * m);
     int4 *scan_b = same ? scan_a : (int4 *)ws.get(S_SCAN_B, sizeof(int4) * nn);
     launch_scan(oa, n, scan_a, s);
     if (!same) launch_scan(ob, n, scan_b, s);
-    int64_t *rb = readback_words();
+    // the five-word summaries: pinned read-back words, or (allocation-free
+    // form) 16 words of the scratch copied back
+    int64_t rb_host[16] = {};
+    int64_t *rb = fixed ? (int64_t *)ws.get(-1, sizeof rb_host) : readback_words();
     hipLaunchKernelGGL(scan_summary, dim3(1), dim3(256), 0, s, (const int4 *)scan_a, m, rb);
     if (!same) hipLaunchKernelGGL(scan_summary, dim3(1), dim3(256), 0, s, (const int4 *)scan_b, nn, rb + 8);
     HIP_CHECK(hipGetLastError());
+    if (fixed) {
+        HIP_CHECK(hipMemcpyAsync(rb_host, rb, sizeof rb_host, hipMemcpyDeviceToHost, s));
+        rb = rb_host;
+    }
     HIP_CHECK(hipStreamSynchronize(s));
     Window wa = window_of(rb), wb = same ? wa : window_of(rb + 8);
     if (wa.nonfinite || wb.nonfinite) return SDPX_GPU_DECLINED;
@@ -1569,22 +1647,58 @@ int rns_gemm_device(const RnsGemmArgs &g) {
         banded_tiles(tiles_m, tiles_n, [&](int tm, int tn) { return !g.upper || tm <= tn; });
     const int ntiles = (int)tl.size();
 
-    PlanTables &ptab = plan_tables(*plan);
-    uint32_t *d_q = ptab.q;
-    uint32_t *d_inv = ptab.inv;
-    double *d_qinv = ptab.qinv;
+    const int J = plan->limbs;
     // encode tables: the matrix-core form when the window fits 3 k-steps,
     // else 24-bit chunk powers for the VALU form
     const int nka = (int)(((wa.width + 7) / 8 + 31) / 32), nkb = (int)(((wb.width + 7) / 8 + 31) / 32);
     const bool mfma_a = g_encode_mfma && nka <= ENC_MAX_NK, mfma_b = g_encode_mfma && nkb <= ENC_MAX_NK;
     static const EncodeTables no_tables;
     uint8_t *d_eta = nullptr, *d_etb = nullptr;
-    const EncodeTables &eta = mfma_a ? encode_tables_dev(*plan, nka, d_eta) : no_tables;
-    const EncodeTables &etb = !same && mfma_b ? encode_tables_dev(*plan, nkb, d_etb) : no_tables;
-    double *d_pa = mfma_a ? nullptr : pow2_table_dev(*plan, TJa);
-    double *d_pb = same || mfma_b ? nullptr : pow2_table_dev(*plan, TJb);
-    const int J = plan->limbs;
-    double *d_crt = ptab.crt;
+    uint32_t *d_q, *d_inv;
+    double *d_qinv, *d_crt, *d_pa = nullptr, *d_pb = nullptr;
+    const uint8_t *d_ct;
+    EncodeTables eta_fixed, etb_fixed;  // allocation-free form: made for this call
+    const EncodeTables *eta = &no_tables, *etb = &no_tables;
+    if (!fixed) {
+        PlanTables &ptab = plan_tables(*plan);
+        d_q = ptab.q;
+        d_inv = ptab.inv;
+        d_qinv = ptab.qinv;
+        d_crt = ptab.crt;
+        d_ct = ptab.crt2;
+        if (mfma_a) eta = &encode_tables_dev(*plan, nka, d_eta);
+        if (!same && mfma_b) etb = &encode_tables_dev(*plan, nkb, d_etb);
+        if (!mfma_a) d_pa = pow2_table_dev(*plan, TJa);
+        if (!same && !mfma_b) d_pb = pow2_table_dev(*plan, TJb);
+    } else {
+        d_q = (uint32_t *)table_copy(plan->q.data(), 4 * (size_t)L);
+        d_inv = (uint32_t *)table_copy(plan->inv.data(), 4 * (size_t)L);
+        d_qinv = (double *)table_copy(plan->qinv.data(), 8 * (size_t)L);
+        std::vector<double> crt(plan->c.begin(), plan->c.begin() + (size_t)L * J);
+        crt.insert(crt.end(), plan->m.begin(), plan->m.begin() + J);
+        d_crt = (double *)table_copy(crt.data(), 8 * crt.size());
+        std::vector<uint8_t> c2(plan->crt_table.begin(), plan->crt_table.end());
+        const uint8_t *m32 = (const uint8_t *)plan->m32.data();
+        c2.insert(c2.end(), m32, m32 + 4 * plan->m32.size());
+        if (plan->m32.size() < CRT_NU_MAX) c2.resize(c2.size() + 4 * (CRT_NU_MAX - plan->m32.size()), 0);
+        d_ct = (const uint8_t *)table_copy(c2.data(), c2.size());
+        if (mfma_a) {
+            eta_fixed = encode_tables(*plan, nka);
+            d_eta = (uint8_t *)table_copy(eta_fixed.bytes.data(), eta_fixed.bytes.size());
+            eta = &eta_fixed;
+        } else {
+            const std::vector<double> h = pow2_table(*plan, TJa);
+            d_pa = (double *)table_copy(h.data(), 8 * h.size());
+        }
+        if (!same && mfma_b) {
+            etb_fixed = encode_tables(*plan, nkb);
+            d_etb = (uint8_t *)table_copy(etb_fixed.bytes.data(), etb_fixed.bytes.size());
+            etb = &etb_fixed;
+        } else if (!same) {
+            const std::vector<double> h = pow2_table(*plan, TJb);
+            d_pb = (double *)table_copy(h.data(), 8 * h.size());
+        }
+    }
     // 256 x 128 blocks of the second product kernel (one-digit moduli)
     std::vector<int2> tl2;
     if (p2)
@@ -1592,21 +1706,33 @@ int rns_gemm_device(const RnsGemmArgs &g) {
             return !g.upper || (int64_t)tm2 * 2 * TILE <= (int64_t)tn * TILE + TILE - 1;
         });
     const int ntiles2 = (int)tl2.size();
-    int2 *d_tiles = tile_tables_dev(tl, tl2, {tiles_m, tiles_n, g.upper ? 1 : 0, p2 ? 1 : 0, tile_group()});
+    int2 *d_tiles;
+    if (!fixed) {
+        d_tiles = tile_tables_dev(tl, tl2, {tiles_m, tiles_n, g.upper ? 1 : 0, p2 ? 1 : 0, tile_group()});
+    } else {
+        std::vector<int2> all(tl);
+        all.insert(all.end(), tl2.begin(), tl2.end());
+        d_tiles = (int2 *)table_copy(all.data(), sizeof(int2) * all.size());
+    }
     int2 *d_tiles2 = d_tiles + ntiles;
 
-    // 2. encode
-    sev.mark(1, s);
+    // every buffer is in place before the first encode launch (in the
+    // allocation-free form a capacity error is raised here, output untouched)
     const int64_t a_plane = m_pad * k_pad, b_plane = n_pad * k_pad;
     int8_t *pla = (int8_t *)ws.get(S_PLANES_A, (size_t)G * a_plane);
     int8_t *plb = same ? pla : (int8_t *)ws.get(S_PLANES_B, (size_t)G * b_plane);
+    // planes up to the matrix-core CRT's padded count (read there, times zero coefficients)
+    uint16_t *X = (uint16_t *)ws.get(S_X, (size_t)std::max(L, plan->crt_lpad) * tiles_m * tiles_n * TILE * TILE * 2);
+
+    // 2. encode
+    sev.mark(1, s);
     if (mfma_a)
-        launch_encode_mfma(oa, n, scan_a, nka, d_eta, eta, L, single, pla, a_plane, k_pad, m_pad, s);
+        launch_encode_mfma(oa, n, scan_a, nka, d_eta, *eta, L, single, pla, a_plane, k_pad, m_pad, s);
     else
         launch_encode(oa, n, scan_a, TJa, d_pa, d_q, d_qinv, L, single, pla, a_plane, k_pad, m_pad, s);
     if (!same) {
         if (mfma_b)
-            launch_encode_mfma(ob, n, scan_b, nkb, d_etb, etb, L, single, plb, b_plane, k_pad, n_pad, s);
+            launch_encode_mfma(ob, n, scan_b, nkb, d_etb, *etb, L, single, plb, b_plane, k_pad, n_pad, s);
         else
             launch_encode(ob, n, scan_b, TJb, d_pb, d_q, d_qinv, L, single, plb, b_plane, k_pad,
                           n_pad, s);
@@ -1614,8 +1740,6 @@ int rns_gemm_device(const RnsGemmArgs &g) {
 
     // 3. products, in exact INT32 chunks of the inner dimension
     sev.mark(2, s);
-    // planes up to the matrix-core CRT's padded count (read there, times zero coefficients)
-    uint16_t *X = (uint16_t *)ws.get(S_X, (size_t)std::max(L, plan->crt_lpad) * tiles_m * tiles_n * TILE * TILE * 2);
     for (int64_t k0 = 0; k0 < k_pad; k0 += KCHUNK) {
         const int64_t k1 = std::min<int64_t>(k_pad, k0 + KCHUNK);
         const int acc = k0 > 0 ? 1 : 0;
@@ -1655,7 +1779,6 @@ int rns_gemm_device(const RnsGemmArgs &g) {
     if (crt_mma) {
         const int lpad = plan->crt_lpad, npad = plan->crt_npad, nu = plan->crt_nu;
         const size_t tb = plan->crt_table.size();
-        const uint8_t *d_ct = ptab.crt2;
         const int trow = 2 * lpad + 16;
         const int numax = nu <= 20 && nbl <= 3 ? 20 : (nu <= 32 && nbl <= 4 ? 32 : 40);
         // the table, then each thread's column of words
