@@ -276,15 +276,17 @@ struct DeviceSetup {
         }
         HIP_CHECK(hipStreamSynchronize(ctx.st));
     }
-    // {worst ldl code, regularized pivots, window overflow of the row kernels, -}
+    // {worst ldl code, regularized pivots, window overflow of the row kernels,
+    // 1 + a failing block}; code 4 (a bounded wait of the one-launch panel
+    // kernel ran out) is a device error
     void status(int (&s)[4]) {
         HIP_CHECK(hipMemcpyAsync(s, ctx.d_status, sizeof s, hipMemcpyDeviceToHost, ctx.st));
         HIP_CHECK(hipStreamSynchronize(ctx.st));
+        if (s[0] == 4) throw Error(SDPX_GPU_DEVICE_ERROR, "a panel kernel's bounded wait ran out");
     }
     bool window_overflow() {
         int s[4];
-        HIP_CHECK(hipMemcpyAsync(s, ctx.d_status, sizeof s, hipMemcpyDeviceToHost, ctx.st));
-        HIP_CHECK(hipStreamSynchronize(ctx.st));
+        status(s);
         return s[2] != 0;
     }
 };

@@ -44,7 +44,8 @@ void mp_forward_rows(int limbs, const void *L_all, const int32_t *first_all, voi
 void mp_trsm_base(int limbs, const void *L, int64_t ldl, int64_t n, void *B, int64_t ldb,
                   int64_t m, const int32_t *low, const void *lranges, int *fail, void *work,
                   hipStream_t st);
-// status[0] = 0 ok, 1 failed pivot, 2 window overflow (zero on entry);
+// status[0] = 0 ok, 1 failed pivot, 2 window overflow, 4 a bounded wait of
+// the one-launch form ran out (a device error) (zero on entry);
 // status[1] += regularized; status[3] = 1 + the index of a block with a
 // failed pivot (status is int[4]).  A nonzero status[0] or status[2] (a
 // row kernel's window overflow) on entry stops the launch.  nmax: the
@@ -54,10 +55,10 @@ void mp_ldl_dots(int limbs, void *A, void *dinv, const int32_t *first, const int
                  int sign_mode, const void *blocks, int nblocks, int64_t nmax, void *scratch,
                  const void *reg_eps, const void *reg_delta, int *status, hipStream_t st);
 // r and w triangles, the pivots, per-row exponent ranges (16 bytes a row,
-// within one record of any width), and two per-row exact-dot windows
-// (slot totals and base, 8 (2W + 21) bytes a row each: within 10 records
-// of any width)
-inline size_t ldl_scratch_records(size_t n) { return n * (n + 1) + 2 * n + 20 * n; }
+// within one record of any width), two per-row exact-dot windows (slot
+// totals and base, 8 (2W + 21) bytes a row each: within 10 records of any
+// width), and two int flags a row (the one-launch form's order)
+inline size_t ldl_scratch_records(size_t n) { return n * (n + 1) + 2 * n + 20 * n + n; }
 void mp_trsm_base_n(int limbs, const void *L, int64_t ldl, int64_t n, void *B, int64_t ldb,
                     int64_t m, const void *lranges, int *fail, hipStream_t st);
 void mp_transpose(int limbs, const void *in, int64_t ldi, int64_t m, int64_t n, void *out,

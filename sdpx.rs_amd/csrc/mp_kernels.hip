@@ -1072,6 +1072,7 @@ struct LdlCol {
     uint32_t *a, *dv, *r, *w, *dstore;
     int4 *rng;
     int64_t *prow;  // [2][n][SLOTS + 1]: row windows by column parity (slot totals, then the base)
+    int *flags;     // [2][n]: pivot_ready, progress (the one-launch form, ldl_rows_kernel)
     const int32_t *first;
     int64_t n, ld;
     static __device__ int64_t tri(int64_t i) { return i * (i + 1) / 2; }
@@ -1087,6 +1088,7 @@ struct LdlCol {
         dstore = w + tri(n) * (W + 2);
         rng = (int4 *)(dstore + n * (W + 2));
         prow = (int64_t *)(dstore + 2 * n * (W + 2));
+        flags = (int *)(dstore + 22 * n * (W + 2));
     }
     // row i's window for column j (written by launch j - 1, read by launch j)
     __device__ int64_t *row_window(int64_t j, int64_t i) const {
@@ -1550,6 +1552,267 @@ __global__ __launch_bounds__(3 * ROW_THREADS) void ldl_col_wave_kernel(
     if (code) fail(code);
 }
 
+// ---------------------------------------------------------------------------
+// ldl_dots in one launch: workgroup (x, y) owns row i = x of block y and forms
+// its entries (i, j), j < i, column by column, then (row i) pivot i.  The
+// column order that ldl_col_wave_kernel gets from kernel boundaries comes
+// from flags in the block's scratch: row j sets pivot_ready[j] once d_j, its
+// r and w rows and its ranges are stored (agent-scope release), and every
+// entry (i, j) waits for it (acquire); row i sets progress[i] = j + 1 after
+// storing entry (i, j), which the look-ahead of later rows waits for (their
+// column j' + 1 dot over k < j' reads row j' + 1).  A row waits only on rows
+// of lower index, so in-order dispatch drains the grid even when not every
+// workgroup is resident.  Every wait is bounded (status code 4 after
+// LDL_WAIT_LIMIT ticks of the 100 MHz clock) and ends on any status code, and
+// a workgroup's three waves leave together after a barrier, so every wave
+// reaches the end.  Per step: wave 1 forms RN(1 / d_j) while wave 0 finishes
+// the entry's dot (the window wave 2 summed in the previous step plus the
+// term k = j - 1) and wave 2 sums the next column's window (or, in row i's
+// last step, pivot i's window over its own row); after the barrier wave 0
+// forms the entry and its W value, and in the last step pivot i.  Same
+// arithmetic, bits and status codes as the column launches.
+constexpr uint64_t LDL_WAIT_LIMIT = 200000000ull;  // 2 s of the 100 MHz clock
+
+__device__ __forceinline__ bool ldl_wait(const int *flag, int target, const int *status) {
+    const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+    while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        if ((__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) |
+             __hip_atomic_load(status + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 0)
+            return false;
+        if (__builtin_amdgcn_s_memrealtime() - t0 > LDL_WAIT_LIMIT) {
+            if (wmp::lane() == 0) atomicMax((int *)status, 4);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    return true;
+}
+
+// flag = value after this wave's earlier stores (agent-scope release)
+__device__ __forceinline__ void ldl_publish(int *flag, int value) {
+    if (wmp::lane() == 0) __hip_atomic_store(flag, value, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int W>
+__global__ void ldl_flags_clear_kernel(uint32_t *A, uint32_t *dinv, const int32_t *first_all, uint32_t *scratch,
+                                       const LdlBlock *blocks) {
+    const LdlBlock B = blocks[blockIdx.y];
+    const LdlCol<W> c(A, dinv, first_all, scratch, B);
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < 2 * B.n; e += (int64_t)gridDim.x * blockDim.x)
+        c.flags[e] = 0;
+}
+
+template <int W>
+__global__ __launch_bounds__(3 * ROW_THREADS) void ldl_rows_kernel(
+    uint32_t *A, uint32_t *dinv, const int32_t *first_all, const int8_t *signs, int sign_mode,
+    const LdlBlock *blocks, uint32_t *scratch, const uint32_t *reg_eps, const uint32_t *reg_delta,
+    int *status) {
+    constexpr int SLOTS = Win<W>::SLOTS;
+    extern __shared__ int64_t slots_sh[];  // [SLOTS][64] row windows, then [SLOTS][64] look-ahead windows
+    __shared__ uint32_t dinv_sh[W + 2];
+    __shared__ int64_t nxt_tot[2][64];  // row i's window for column j (slot s at [s]), by column parity
+    __shared__ int64_t nxt_base[2];
+    __shared__ int64_t pre_tot[64];     // pivot i's window over k < i - 1
+    __shared__ int64_t pre_lo;
+    __shared__ int ok_sh[3];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const LdlBlock B = blocks[blockIdx.y];
+    const int64_t i = blockIdx.x;
+    if (i >= B.n) return;  // the whole workgroup
+    const LdlCol<W> c(A, dinv, first_all, scratch, B);
+    int *const pivot_ready = c.flags, *const progress = c.flags + c.n;
+    int64_t *slots = slots_sh + lane;
+    lds_i64 *pre_win = (lds_i64 *)(slots_sh + SLOTS * ROW_THREADS) + lane;
+    auto fail = [&](int code) {
+        if (lane == 0) {
+            atomicMax(status, code);
+            if (code == 1) atomicMax(status + 3, (int)blockIdx.y + 1);  // a failing block, 1-based
+        }
+    };
+    if (i == 0) {  // pivot 0: an empty dot
+        if (wave == 0) {
+            if (lane == 0) c.rng[0] = make_int4(INT32_MAX, INT32_MIN, INT32_MAX, INT32_MIN);
+            const int code = ldl_col_pivot<W, 64>(c, 0, lane, INT64_MAX, INT64_MIN, slots, signs, sign_mode,
+                                                  B.sign_off, reg_eps, reg_delta, status);
+            if (code)
+                fail(code);
+            else
+                ldl_publish(pivot_ready, 1);
+        }
+        return;  // the whole workgroup (no barrier follows)
+    }
+    const int64_t fi = c.first[i];
+    if (threadIdx.x < 2) nxt_base[threadIdx.x] = INT64_MIN;
+    __syncthreads();
+    int4 ri = make_int4(INT32_MAX, INT32_MIN, INT32_MAX, INT32_MIN);  // wave 0: row i's ranges so far
+    wmp::WV<W> vprev = wmp::zero<W>();                                  // wave 0: r_{i, j - 1}
+    for (int64_t j = 0; j < i; j++) {
+        const int par = (int)(j & 1);
+        const bool last = j == i - 1;  // this step also forms pivot i
+        const bool zero_row = fi > j;  // no entry (i, j): left of the envelope
+        bool okw = true;
+        wmp::WV<W> dj = wmp::zero<W>(), diff = wmp::zero<W>();
+        bool dot_ok = true;
+        if (wave == 1) {  // RN(1 / d_j); row j + 1 also stores it
+            okw = ldl_wait(pivot_ready + j, 1, status);
+            if (okw) {
+                const wmp::WV<W> di = wmp::recip(wmp::load<W>(rec_at<W>((const uint32_t *)c.dstore, j)));
+                wmp::store(dinv_sh, di);
+                if (last) wmp::store(rec_at<W>(c.dv, j), di);
+            }
+        } else if (wave == 2) {
+            if (last) {  // pivot i's dot over k < j (its own r and w rows)
+                int64_t lo_w = INT64_MIN;
+                if (j > 0 && fi < j) {
+                    const int4 rp = c.rng[i];  // row i's ranges over k < j
+                    int64_t hl, hh;
+                    range_hint<W>(rp.x, rp.y, rp.z, rp.w, hl, hh);
+                    if (hl <= hh && (hh - hl + 256) / 32 + 2 * W + 4 <= SLOTS) {
+                        lo_w = hl - 128;
+                        group_accumulate<W, ROW_THREADS, ROW_THREADS>(
+                            fi, j, lane, rec_at<W>((const uint32_t *)c.r, LdlCol<W>::tri(i)), 1,
+                            rec_at<W>((const uint32_t *)c.w, LdlCol<W>::tri(i)), 1, false, pre_win, SLOTS, lo_w);
+                        pre_tot[lane] = window_total<ROW_THREADS>(pre_win, lane, SLOTS);
+                    }
+                }
+                if (lane == 0) pre_lo = lo_w;
+            } else {  // row i's dot for column p = j + 1 over k < j (row p through column j - 1)
+                const int64_t p = j + 1;
+                int64_t lo_r = INT64_MIN;
+                if (j > 0 && fi < j) {
+                    okw = ldl_wait(progress + p, (int)j, status);
+                    if (okw) {
+                        const int4 rp = c.rng[p], rr = c.rng[i];  // w of row p, r of row i, over k < j (or wider)
+                        int64_t hl, hh;
+                        range_hint<W>(rr.x, rr.y, rp.z, rp.w, hl, hh);
+                        if (hl <= hh && (hh - hl + 256) / 32 + 2 * W + 4 <= SLOTS) {
+                            lo_r = hl - 128;
+                            group_accumulate<W, ROW_THREADS, ROW_THREADS>(
+                                max(fi, (int64_t)c.first[p]), j, lane,
+                                rec_at<W>((const uint32_t *)c.r, LdlCol<W>::tri(i)), 1,
+                                rec_at<W>((const uint32_t *)c.w, LdlCol<W>::tri(p)), 1, false, pre_win, SLOTS, lo_r);
+                            const int64_t T = window_total<ROW_THREADS>(pre_win, lane, SLOTS);
+                            nxt_tot[par ^ 1][lane] = lane < SLOTS ? T : 0;
+                        }
+                    }
+                }
+                if (lane == 0) nxt_base[par ^ 1] = lo_r;
+            }
+        } else {  // wave 0: entry (i, j) up to its product with RN(1 / d_j)
+            okw = ldl_wait(pivot_ready + j, 1, status);
+            if (okw) {
+                dj = wmp::load<W>(rec_at<W>((const uint32_t *)c.dstore, j));
+                if (!zero_row) {
+                    const int4 rj = c.rng[j];
+                    const wmp::WV<W> aij = wmp::load<W>(rec_at<W>((const uint32_t *)c.a, i + j * c.ld));
+                    int64_t hl, hh;
+                    range_hint<W>(ri.x, ri.y, rj.z, rj.w, hl, hh);
+                    wmp::WV<W> t;
+                    bool done = false;
+                    const int64_t pre_b = nxt_base[par];
+                    if (pre_b != INT64_MIN) {  // the window wave 2 summed over k < j - 1, and the term k = j - 1
+                        int64_t pre_t = lane < SLOTS ? nxt_tot[par][lane] : 0;
+                        const wmp::WV<W> wl =
+                            wmp::load<W>(rec_at<W>((const uint32_t *)c.w, LdlCol<W>::tri(j) + j - 1));
+                        bool fits = true;
+                        if (!wmp::is_zero(vprev) && !wmp::is_zero(wl)) {
+                            const int64_t sc = (int64_t)vprev.e + wl.e - 64 * W;
+                            fits = sc >= pre_b && (sc - pre_b) / 32 + 2 * W + 4 <= SLOTS;
+                            if (fits) pre_t = wmp::add_product<W, SLOTS>(pre_t, vprev, wl, sc - pre_b, false);
+                        }
+                        if (fits) {
+                            t = wmp::round_window<W, SLOTS>(pre_t, pre_b);
+                            done = true;
+                        }
+                    }
+                    if (done) {
+                    } else if (hl <= hh && (hh - hl) / 32 + 2 * W + 4 <= SLOTS) {  // the window from the ranges
+                        const int used = (int)min((int64_t)SLOTS, (hh - hl) / 32 + 2 * W + 1);
+                        lds_i64 *ls = (lds_i64 *)slots;
+                        for (int sl = 0; sl < used; sl++) ls[sl * ROW_THREADS] = 0;
+                        for (int64_t k = lane; k < j; k += 64) {
+                            Val<W> x, y;
+                            load(rec_at<W>((const uint32_t *)c.r, LdlCol<W>::tri(i) + k), x);
+                            load(rec_at<W>((const uint32_t *)c.w, LdlCol<W>::tri(j) + k), y);
+                            if (!is_zero(x) && !is_zero(y)) lds_add_product<W, ROW_THREADS>(ls, hl, x, y, false);
+                        }
+                        t = wmp::round_window<W, SLOTS>(window_total<ROW_THREADS>(ls, lane, used), hl);
+                    } else if (j == 0) {
+                        t = wmp::zero<W>();
+                    } else {
+                        dot_ok = wave_dot<W, ROW_THREADS>(0, j, lane,
+                                                          rec_at<W>((const uint32_t *)c.r, LdlCol<W>::tri(i)), 1,
+                                                          rec_at<W>((const uint32_t *)c.w, LdlCol<W>::tri(j)), 1,
+                                                          slots, t, hl, hh);
+                    }
+                    if (dot_ok) diff = wmp::add(aij, t, true);
+                }
+            }
+        }
+        if (lane == 0) ok_sh[wave] = okw ? 1 : 0;
+        __syncthreads();  // RN(1 / d_j) in dinv_sh, the windows of wave 2
+        const bool go = ok_sh[0] && ok_sh[1] && ok_sh[2];
+        if (go && wave == 0) {
+            if (!dot_ok) {
+                fail(2);
+            } else {
+                const wmp::WV<W> v = zero_row ? wmp::zero<W>() : wmp::mul(diff, wmp::load<W>(dinv_sh));
+                const wmp::WV<W> wv = wmp::mul(v, dj);
+                wmp::store(rec_at<W>(c.r, LdlCol<W>::tri(i) + j), v);
+                wmp::store(rec_at<W>(c.w, LdlCol<W>::tri(i) + j), wv);
+                if (!wmp::is_zero(v)) {
+                    ri.x = min(ri.x, v.e);
+                    ri.y = max(ri.y, v.e);
+                }
+                if (!wmp::is_zero(wv)) {
+                    ri.z = min(ri.z, wv.e);
+                    ri.w = max(ri.w, wv.e);
+                }
+                if (lane == 0) c.rng[i] = ri;
+                vprev = v;
+                if (!last) {
+                    ldl_publish(progress + i, (int)(j + 1));
+                } else {  // pivot i: the window of wave 2 plus the term v wv
+                    int code = 3;
+                    const int64_t plo = pre_lo;
+                    if (plo != INT64_MIN) {
+                        const wmp::WV<W> aii = wmp::load<W>(rec_at<W>((const uint32_t *)c.a, i + i * c.ld));
+                        wmp::WV<W> eps = wmp::zero<W>(), delta = wmp::zero<W>();
+                        if (reg_eps) {
+                            eps = wmp::load<W>(reg_eps);
+                            delta = wmp::load<W>(reg_delta);
+                        }
+                        const int sg = sign_mode == 0 ? 1 : (sign_mode == 1 ? signs[0] : signs[B.sign_off + i]);
+                        int64_t T = pre_tot[lane];
+                        bool fits = true;
+                        if (!wmp::is_zero(v) && !wmp::is_zero(wv)) {
+                            const int64_t sc = (int64_t)v.e + wv.e - 64 * W;
+                            fits = sc >= plo && (sc - plo) / 32 + 2 * W + 4 <= SLOTS;
+                            if (fits) T = wmp::add_product<W, SLOTS>(T, v, wv, sc - plo, false);
+                        }
+                        if (fits)
+                            code = ldl_col_pivot_wave<W>(c, i, wmp::round_window<W, SLOTS>(T, plo), aii,
+                                                         reg_eps != nullptr, eps, delta, sg, status);
+                    }
+                    if (code == 3) {  // the whole dot (also when the pivot cancels: rounded once exactly)
+                        int64_t hl, hh;
+                        range_hint<W>(ri.x, ri.y, ri.z, ri.w, hl, hh);
+                        __threadfence_block();  // the whole dot reads the entries this wave's lanes just stored
+                        code = ldl_col_pivot<W, 64>(c, i, lane, hl, hh, slots, signs, sign_mode, B.sign_off,
+                                                    reg_eps, reg_delta, status);
+                    }
+                    if (code)
+                        fail(code);
+                    else
+                        ldl_publish(pivot_ready + i, 1);
+                }
+            }
+        }
+        __syncthreads();  // the step's stores before the next step's look-ahead reads them
+        if (!go) break;   // the whole workgroup
+    }
+}
+
 // the unit lower factor of every block (after the last column launch)
 template <int W>
 __global__ void ldl_col_writeout_kernel(uint32_t *A, uint32_t *dinv, const LdlBlock *blocks, uint32_t *scratch,
@@ -1873,7 +2136,19 @@ static bool trsm_right_looking() {
     }();
     return env;
 }
+// "ldl_one_launch" (default 1): ldl_dots blocks in one launch of
+// ldl_rows_kernel (rows ordered by flags) instead of one launch per column.
+static std::atomic<int> g_ldl_one_launch{1};
+static bool ldl_one_launch() { return g_ldl_one_launch.load() != 0; }
+// rows of one launch of ldl_rows_kernel at most (larger blocks take the
+// column launches)
+constexpr int64_t LDL_ROWS_MAX = 4096;
+
 bool mp_set_tuning(const char *name, int64_t value) {
+    if (!std::strcmp(name, "ldl_one_launch")) {
+        g_ldl_one_launch = value ? 1 : 0;
+        return true;
+    }
     if (!std::strcmp(name, "trsm_rl")) {
         g_trsm_rl = value ? 1 : 0;
         return true;
@@ -1942,6 +2217,25 @@ void mp_ldl_dots(int limbs, void *A, void *dinv, const int32_t *first, const int
                                (uint32_t *)dinv, first, signs, sign_mode, (const LdlBlock *)blocks,
                                (uint32_t *)scratch, (const uint32_t *)reg_eps,
                                (const uint32_t *)reg_delta, status);
+        });
+        HIP_CHECK(hipGetLastError());
+        return;
+    }
+    // one block (the recursive factor's base blocks): one launch; many
+    // blocks (the leaves) stay on column launches, which measured faster
+    // there (rows of later blocks wait for residency behind earlier ones)
+    if (ldl_one_launch() && nblocks == 1 && nmax <= LDL_ROWS_MAX) {
+        SDPX_DISPATCH_W(limbs, {
+            hipLaunchKernelGGL(ldl_flags_clear_kernel<W>, dim3(blocks_for(2 * nmax, 256), nblocks), dim3(256), 0, st,
+                               (uint32_t *)A, (uint32_t *)dinv, first, (uint32_t *)scratch, (const LdlBlock *)blocks);
+            auto kern = ldl_rows_kernel<W>;
+            const size_t lds = panel_lds<W>((const void *)kern, 2 * ROW_THREADS);  // row windows, look-ahead windows
+            hipLaunchKernelGGL(kern, dim3((unsigned)nmax, nblocks), dim3(3 * ROW_THREADS), lds, st, (uint32_t *)A,
+                               (uint32_t *)dinv, first, signs, sign_mode, (const LdlBlock *)blocks,
+                               (uint32_t *)scratch, (const uint32_t *)reg_eps, (const uint32_t *)reg_delta, status);
+            hipLaunchKernelGGL(ldl_col_writeout_kernel<W>, dim3(blocks_for(nmax * nmax, 256), nblocks), dim3(256),
+                               0, st, (uint32_t *)A, (uint32_t *)dinv, (const LdlBlock *)blocks, (uint32_t *)scratch,
+                               (const int *)status);
         });
         HIP_CHECK(hipGetLastError());
         return;

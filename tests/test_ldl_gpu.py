@@ -161,3 +161,25 @@ def test_device_pointer_solve_matches_host_entry(gpu):
     torch.cuda.synchronize()
     x_dev = db.cpu().numpy().view(mp.dtype(nl))
     assert np.all(mp.same_bits(x_dev, x_host))
+
+
+@pytest.mark.parametrize("one_launch", [0, 1])
+def test_column_launches_and_one_launch_agree(gpu, one_launch):
+    """ldl_dots blocks as one launch of row workgroups ordered by flags, or
+    as one launch per column: both give the oracle's bits (recursive order
+    600 at 256 bits: base blocks of 75 and 150 columns, plus a dots-only
+    order 200 at 512 bits)."""
+    rng = np.random.default_rng(91)
+    print("ldl one_launch", one_launch, flush=True)
+    sdpx_gpu.set_tuning("ldl_one_launch", one_launch)
+    try:
+        for nl, n, flags in ((4, 600, 0), (8, 200, sdpx_gpu.LDL_DOTS)):
+            s = spread_spd(n, rng, -12)
+            A = mp.from_float64(((s + s.T) / 2).T.reshape(-1), nl)
+            ok, L, dinv, _ = oracle_ldl(A, n, recursive=not flags, threads=16)
+            okg, Lg, dg = sdpx_gpu.mp_ldl(A, n, flags=flags)
+            assert sdpx_gpu.mp_ldl_stats()["declined"] == 0
+            assert ok and okg
+            assert np.all(mp.same_bits(Lg, L)) and np.all(mp.same_bits(dg, dinv))
+    finally:
+        sdpx_gpu.set_tuning("ldl_one_launch", 1)
