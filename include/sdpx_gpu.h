@@ -270,6 +270,27 @@ int32_t sdpx_gpu_sampled_arrow_dev(uint32_t limbs, uint64_t nleaves,
                                    void *dinv_all, void *y_all, void *border,
                                    const int8_t *border_signs, uint64_t nborder_signs,
                                    void *border_dinv, int32_t *ok);
+/* ArrowLDLSolver::solve_many (crates/solver/src/solver/kkt/ldl/arrow.rs:
+ * 3793-3980) on the factors of sdpx_gpu_sampled_arrow_dev (device pointers
+ * except samples, a host array): samples[i] is leaf i's order K_i; l_all, dinv_all,
+ * y_all, border (its lower triangle the unit factor) and border_dinv as
+ * that call left them.  x (n x nrhs, ldx >= n, n = sum K_i + t) holds the
+ * right-hand sides on entry (each column: leaf 0's K_0 rows, ..., leaf
+ * last's, then the t border rows) and the solutions on return.  Bitwise
+ * the reference's sequence for leaves whose every row couples: per leaf
+ * w = L^-1 b by single-rounding FMA updates in DenseLeaf::forward_many's
+ * order (1155-1164), v = RN(w D^-1) (first_many 1682-1712); each border
+ * entry minus RN(Y_l^T v_l), one exact dot per leaf, leaf by leaf (the
+ * coupling of 3857-3898); the border factor's forward sweep, D_b^-1 and
+ * backward sweep in blocks of 32 (DenseLeaf::solve_many, backward 1165-1183,
+ * FORWARD_BLOCK); per leaf RN(RN(w - RN(Y_l y)) D^-1) and the backward
+ * sweep in blocks of 16 (second_many 1713-1745, LEAF_BLOCK).  Orders up
+ * to 8192; returns 2 (declined, x unspecified) when a product declines. */
+int32_t sdpx_gpu_sampled_arrow_solve_dev(uint32_t limbs, uint64_t nleaves, const uint64_t *samples, uint64_t t,
+                                         const void *l_all, const void *dinv_all, const void *y_all,
+                                         const void *border, const void *border_dinv, uint64_t nrhs, void *x,
+                                         uint64_t ldx);
+
 /* Phase times (s: build, leaves, border), product counts, declined (1 when
  * the device pipeline declined and host panels produced the result),
  * retried (border factors redone by exact dots) and the index of a leaf

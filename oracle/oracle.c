@@ -916,6 +916,126 @@ int oracle_sampled_arrow(int nl, size_t nleaves, const ora_leaf *leaves, size_t 
     return oracle_ldl_mode(nl, border, t, t, bsigns, nbsigns, NULL, NULL, border_dinv, 2, &reg, NULL, threads);
 }
 
+/* ------------------------------------------------------------------------
+ * Arrow solves for the sampled arrow factor: ArrowLDLSolver::solve_many
+ * (arrow.rs:3793-3980) with the leaves' first_many / second_many
+ * (1682-1745) and DenseLeaf's FMA sweeps (forward_many 1155-1164,
+ * backward 1165-1183, solve_many 1184-1195), for leaves whose every row
+ * couples (coupling_start 0).  x (n x nrhs, ld ldx, n = sum K + t; leaf
+ * rows in leaf order, then the t border rows) holds b on entry, x on
+ * return.  Per column:
+ *   w_l = L_l^-1 b_l          forward: w_i = RN(w_i - L_ik w_k), k ascending
+ *   v_l = RN(w_l D_l^-1)
+ *   y_j = c_j, then per leaf in order y_j = RN(y_j - RN(sum_i Y_l[i,j] v_l[i]))
+ *   y   = L_b^-T RN(D_b^-1 L_b^-1 y)   (border sweeps, backward blocks of 32)
+ *   x_l = L_l^-T RN(RN(w_l - RN(Y_l y)) D_l^-1)   (backward blocks of 16)
+ * The backward sweep of an order-n factor with block `back` takes, for row
+ * i, the blocks [n - back, n), [n - 2 back, n - back), ... down to row
+ * i + 1, ascending within a block (DenseLeaf::backward, arrow.rs:1165-1183).
+ * ---------------------------------------------------------------------- */
+static void ora_forward_fma(const ora_ctx *c, const uint8_t *L, size_t ld, size_t n, uint8_t *x) {
+    size_t st = c->stride;
+    uint8_t t[8 * 32 + 8];
+    for (size_t i = 0; i < n; i++)
+        for (size_t k = 0; k < i; k++) {
+            ora_fms(c, AT(x, i), AT(L, i + k * ld), AT(x, k), t);
+            memcpy(AT(x, i), t, st);
+        }
+}
+
+static void ora_backward_fma(const ora_ctx *c, const uint8_t *L, size_t ld, size_t n, size_t back, uint8_t *x) {
+    size_t st = c->stride;
+    uint8_t t[8 * 32 + 8];
+    for (size_t ii = n; ii-- > 0;) {
+        size_t r1 = n;
+        while (r1 > ii + 1) {
+            size_t r0 = r1 > back ? r1 - back : 0;
+            for (size_t k = (r0 > ii + 1 ? r0 : ii + 1); k < r1; k++) {
+                ora_fms(c, AT(x, ii), AT(L, k + ii * ld), AT(x, k), t);
+                memcpy(AT(x, ii), t, st);
+            }
+            r1 = r0;
+        }
+    }
+}
+
+int oracle_sampled_arrow_solve(int nl, size_t nleaves, const uint64_t *ks, size_t t, const uint8_t *l_all,
+                               const uint8_t *dinv_all, const uint8_t *y_all, const uint8_t *border,
+                               const uint8_t *border_dinv, size_t nrhs, uint8_t *x, size_t ldx, int threads) {
+    ora_ctx cc;
+    ora_init(&cc, nl);
+    const ora_ctx *c = &cc;
+    size_t st = c->stride, ksum = 0, kmax = 1;
+    for (size_t b = 0; b < nleaves; b++) {
+        ksum += ks[b];
+        if (ks[b] > kmax) kmax = ks[b];
+    }
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 0 ? threads : 1)
+    for (long long cl = 0; cl < (long long)nrhs; cl++) {
+        uint8_t *col = x + (size_t)cl * ldx * st;
+        uint8_t *w = malloc(st * (ksum ? ksum : 1)), *v = malloc(st * (ksum ? ksum : 1));
+        uint8_t *yb = col + ksum * st;
+        const uint8_t **xs = malloc(sizeof(uint8_t *) * (kmax + t + 1)),
+                      **ys = malloc(sizeof(uint8_t *) * (kmax + t + 1));
+        uint8_t t1[8 * 32 + 8], t2[8 * 32 + 8];
+        size_t lo = 0, ko = 0;
+        /* first: w = L^-1 b, v = RN(w D^-1) per leaf */
+        for (size_t b = 0; b < nleaves; b++) {
+            size_t K = ks[b];
+            memcpy(w + ko * st, col + ko * st, K * st);
+            ora_forward_fma(c, l_all + lo * st, K, K, w + ko * st);
+            for (size_t i = 0; i < K; i++) ora_mul(c, AT(w, ko + i), AT(dinv_all, ko + i), AT(v, ko + i));
+            lo += K * K;
+            ko += K;
+        }
+        /* couple: y_j -= RN(Y_l^T v_l) leaf by leaf */
+        for (size_t j = 0; j < t; j++) {
+            size_t kk = 0;
+            for (size_t b = 0; b < nleaves; b++) {
+                size_t K = ks[b];
+                for (size_t i = 0; i < K; i++) {
+                    xs[i] = AT(y_all, kk * t + i + j * K);
+                    ys[i] = AT(v, kk + i);
+                }
+                ora_dot_ptrs(c, (int)K, xs, ys, t1);
+                ora_sub(c, AT(yb, j), t1, t2);
+                memcpy(AT(yb, j), t2, st);
+                kk += K;
+            }
+        }
+        /* trunk: L_b D_b L_b^T y = y */
+        ora_forward_fma(c, border, t, t, yb);
+        for (size_t j = 0; j < t; j++) {
+            ora_mul(c, AT(yb, j), AT(border_dinv, j), t1);
+            memcpy(AT(yb, j), t1, st);
+        }
+        ora_backward_fma(c, border, t, t, 32, yb);
+        /* second: x_l = L^-T RN(RN(w - RN(Y y)) D^-1) */
+        lo = 0;
+        ko = 0;
+        for (size_t b = 0; b < nleaves; b++) {
+            size_t K = ks[b];
+            for (size_t i = 0; i < K; i++) {
+                for (size_t j = 0; j < t; j++) {
+                    xs[j] = AT(y_all, ko * t + i + j * K);
+                    ys[j] = AT(yb, j);
+                }
+                ora_dot_ptrs(c, (int)t, xs, ys, t1);
+                ora_sub(c, AT(w, ko + i), t1, t2);
+                ora_mul(c, t2, AT(dinv_all, ko + i), AT(col, ko + i));
+            }
+            ora_backward_fma(c, l_all + lo * st, K, K, 16, col + ko * st);
+            lo += K * K;
+            ko += K;
+        }
+        free(w);
+        free(v);
+        free(xs);
+        free(ys);
+    }
+    return 1;
+}
+
 /* Scalar checks for the tests. */
 void oracle_recip(int nl, const uint8_t *d, uint8_t *out) {
     ora_ctx c;

@@ -323,6 +323,35 @@ int32_t sdpx_gpu_sampled_arrow(uint32_t limbs, uint64_t nleaves, const sdpx_gpu_
     });
 }
 
+int32_t sdpx_gpu_sampled_arrow_solve_dev(uint32_t limbs, uint64_t nleaves, const uint64_t *samples, uint64_t t,
+                                         const void *l_all, const void *dinv_all, const void *y_all,
+                                         const void *border, const void *border_dinv, uint64_t nrhs, void *x,
+                                         uint64_t ldx) {
+    return sdpx::abi_boundary([&]() -> int32_t {
+        if (!sdpx::mp_kernels_support((int)limbs))
+            return sdpx::abi_invalid("arrow solves support 128, 256 and 512 bits");
+        if (nleaves && !samples) return sdpx::abi_invalid("null leaf orders");
+        uint64_t n = t;
+        std::vector<size_t> ks(nleaves);
+        for (uint64_t i = 0; i < nleaves; i++) {
+            ks[i] = samples[i];
+            n += samples[i];
+        }
+        if (ldx < n) return sdpx::abi_invalid("ldx below the system order");
+        if (nrhs && n && !x) return sdpx::abi_invalid("null right-hand sides");
+        if (nleaves && (!l_all || !dinv_all || (t && !y_all))) return sdpx::abi_invalid("null leaf factor");
+        if (t && (!border || !border_dinv)) return sdpx::abi_invalid("null border factor");
+        if (nrhs == 0 || n == 0) return SDPX_GPU_OK;
+        sdpx::Device &d = sdpx::device();
+        std::lock_guard<std::recursive_mutex> g(d.mu);
+        const int r = sdpx::sampled_arrow_solve_device((int)limbs, ks, t, (const uint8_t *)l_all,
+                                                       (const uint8_t *)dinv_all, (const uint8_t *)y_all,
+                                                       (const uint8_t *)border, (const uint8_t *)border_dinv, nrhs,
+                                                       (uint8_t *)x, ldx);
+        return r < 0 ? SDPX_GPU_DECLINED : SDPX_GPU_OK;
+    });
+}
+
 int32_t sdpx_gpu_sampled_stats(double *seconds3, int64_t *gpu_products, int64_t *host_products,
                                int64_t *declined, int64_t *retried, int64_t *failed_leaf) {
     if (seconds3) {

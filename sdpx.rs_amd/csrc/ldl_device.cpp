@@ -374,6 +374,71 @@ int ldl_solve_device(int limbs, const uint8_t *d_l, size_t ldl, size_t n, const 
     return 1;
 }
 
+int sampled_arrow_solve_device(int limbs, const std::vector<size_t> &ks, size_t t, const uint8_t *d_l_all,
+                               const uint8_t *d_dinv_all, const uint8_t *d_y_all, const uint8_t *d_border,
+                               const uint8_t *d_border_dinv, size_t nrhs, uint8_t *d_x, size_t ldx) {
+    const size_t vb = value_bytes(limbs);
+    size_t ksum = 0, kmax = 0;
+    std::vector<size_t> loff, koff;
+    for (size_t K : ks) {
+        loff.push_back(loff.empty() ? 0 : loff.back() + ks[loff.size() - 1] * ks[loff.size() - 1]);
+        koff.push_back(ksum);
+        ksum += K;
+        kmax = std::max(kmax, K);
+    }
+    if (!mp_sweep_supports((int64_t)kmax) || !mp_sweep_supports((int64_t)t)) return -1;
+    if (nrhs == 0) return 1;
+    LdlStats ls;
+    const size_t pmax = std::max(t, kmax);
+    DeviceSetup ds(limbs, (2 * ksum + pmax) * nrhs * vb + 16 * (ks.size() + 1) * sizeof(SweepDescHost) + (64u << 20),
+                   4096, nullptr, 0, nullptr, nullptr, &ls);
+    DCtx &c = ds.ctx;
+    hipStream_t s = c.st;
+    uint8_t *Wb = (uint8_t *)c.arena->alloc(std::max<size_t>(ksum, 1) * nrhs * vb);
+    uint8_t *Vb = (uint8_t *)c.arena->alloc(std::max<size_t>(ksum, 1) * nrhs * vb);
+    uint8_t *P = (uint8_t *)c.arena->alloc(std::max<size_t>(pmax, 1) * nrhs * vb);
+    std::vector<SweepDescHost> ld;
+    for (size_t l = 0; l < ks.size(); l++)
+        ld.push_back(SweepDescHost{(int64_t)loff[l], (int64_t)ks[l], (int64_t)koff[l], (int64_t)ldx, (int64_t)koff[l]});
+    SweepDescHost bd{0, (int64_t)t, (int64_t)ksum, (int64_t)ldx, 0};
+    SweepDescHost *d_ld = (SweepDescHost *)c.arena->alloc(sizeof(SweepDescHost) * std::max<size_t>(ld.size(), 1));
+    SweepDescHost *d_bd = (SweepDescHost *)c.arena->alloc(sizeof(SweepDescHost));
+    if (!ld.empty()) upload_async(d_ld, ld.data(), sizeof(SweepDescHost) * ld.size());
+    upload_async(d_bd, &bd, sizeof bd);
+    // first_many: w = L^-1 b (kept), v = RN(w D^-1), every leaf and column at once
+    mp_arrow_forward(limbs, d_l_all, d_dinv_all, d_ld, (int)ld.size(), (int64_t)kmax, (int64_t)nrhs, d_x, Wb, Vb,
+                     (int64_t)ksum, s);
+    // couple: border rows -= RN(Y_l^T v_l), leaf by leaf
+    uint8_t *xb = c.at(d_x, ksum);
+    for (size_t l = 0; l < ks.size() && t; l++) {
+        product(c, 'T', 'N', t, nrhs, ks[l], c.at((uint8_t *)d_y_all, koff[l] * t), ks[l], c.at(Vb, koff[l]), ksum,
+                P, false);
+        if (c.declined) return -1;
+        mp_ew_sub_rect(limbs, xb, (int64_t)ldx, P, (int64_t)t, (int64_t)t, (int64_t)nrhs, s);
+    }
+    // trunk: the border factor's sweeps (backward blocks of 32, FORWARD_BLOCK)
+    if (t) {
+        mp_arrow_forward(limbs, d_border, d_border_dinv, d_bd, 1, (int64_t)t, (int64_t)nrhs, d_x, nullptr, nullptr, 0,
+                         s);
+        mp_arrow_backward(limbs, d_border, d_bd, 1, (int64_t)t, (int64_t)nrhs, d_x, 32, s);
+    }
+    // second_many: x_l = L^-T RN(RN(w - RN(Y_l y)) D^-1) (backward blocks of 16, LEAF_BLOCK)
+    for (size_t l = 0; l < ks.size(); l++) {
+        const uint8_t *S = P;
+        if (t) {
+            product(c, 'N', 'N', ks[l], nrhs, t, c.at((uint8_t *)d_y_all, koff[l] * t), ks[l], xb, ldx, P, false);
+            if (c.declined) return -1;
+        } else {
+            mp_zero_values(limbs, P, (int64_t)(ks[l] * nrhs), s);  // no border: Y y is empty
+        }
+        mp_arrow_second(limbs, c.at(Wb, koff[l]), (int64_t)ksum, S, (int64_t)ks[l], c.at((uint8_t *)d_dinv_all, koff[l]),
+                        c.at(d_x, koff[l]), (int64_t)ldx, (int64_t)ks[l], (int64_t)nrhs, s);
+    }
+    mp_arrow_backward(limbs, d_l_all, d_ld, (int)ld.size(), (int64_t)kmax, (int64_t)nrhs, d_x, 16, s);
+    HIP_CHECK(hipStreamSynchronize(s));
+    return 1;
+}
+
 int sampled_arrow_device(int limbs, const std::vector<SampledLeafDev> &leaves, size_t t,
                          uint8_t *d_l_all, uint8_t *d_dinv_all, uint8_t *d_y_all, uint8_t *d_border,
                          const int8_t *border_signs, size_t nborder_signs, uint8_t *d_border_dinv,

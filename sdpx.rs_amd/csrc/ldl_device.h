@@ -32,4 +32,12 @@ int sampled_arrow_device(int limbs, const std::vector<SampledLeafDev> &leaves, s
                          const int8_t *border_signs, size_t nborder_signs, uint8_t *d_border_dinv,
                          SampledStats &st);
 
+// ArrowLDLSolver::solve_many (arrow.rs:3793-3980) on the factors of
+// sampled_arrow_device: x (n x nrhs, ldx, n = sum k + t; leaf rows in leaf
+// order, then the border rows) holds b on entry, x on return.  1 ok, -1
+// not handled on the GPU (a product declined, or an order above 8192).
+int sampled_arrow_solve_device(int limbs, const std::vector<size_t> &ks, size_t t, const uint8_t *d_l_all,
+                               const uint8_t *d_dinv_all, const uint8_t *d_y_all, const uint8_t *d_border,
+                               const uint8_t *d_border_dinv, size_t nrhs, uint8_t *d_x, size_t ldx);
+
 }  // namespace sdpx

@@ -65,6 +65,25 @@ void mp_transpose(int limbs, const void *in, int64_t ldi, int64_t m, int64_t n, 
                   int64_t ldo, hipStream_t st);
 void mp_envelope(int limbs, const void *A, int64_t ld, int64_t n, int32_t *first, hipStream_t st);
 void mp_mirror_upper(int limbs, void *S, int64_t n, int64_t ld, hipStream_t st);
+// FMA sweeps of the arrow solves (DenseLeaf forward_many / backward order):
+// descs (device) lists ndesc factors, each an order-n unit lower L (ld n) at
+// record l_off of L_all and right-hand sides x (column c at record x_off +
+// c ldx of X), one workgroup per (factor, column).  Forward: the sweep,
+// then w stored to Wout and RN(w D^-1) (D at record d_off of dinv_all) to
+// Vout (column c at x_off + c ldw), or in place (Wout null) when Vout is
+// null.  Backward: blocks of `back` rows from the bottom.  Orders up to 8192.
+struct SweepDescHost {
+    int64_t l_off, n, x_off, ldx, d_off;
+};
+bool mp_sweep_supports(int64_t n);
+void mp_arrow_forward(int limbs, const void *L_all, const void *dinv_all, const void *descs, int ndesc,
+                      int64_t nmax, int64_t cols, void *X, void *Wout, void *Vout, int64_t ldw, hipStream_t st);
+void mp_arrow_backward(int limbs, const void *L_all, const void *descs, int ndesc, int64_t nmax, int64_t cols,
+                       void *X, int back, hipStream_t st);
+void mp_zero_values(int limbs, void *P, int64_t count, hipStream_t st);
+// X = RN(RN(Wm - S) D) rows i (D per row) of an m x n block
+void mp_arrow_second(int limbs, const void *Wm, int64_t ldw, const void *S, int64_t lds, const void *D, void *X,
+                     int64_t ldx, int64_t m, int64_t n, hipStream_t st);
 // Exact-dot Cholesky (xpotrf), one launch per column: status[0] = info
 // (failing column + 1), status[2] |= 1 when a dot's exponent spread exceeds
 // the window; both must be zero on entry.

@@ -1964,6 +1964,162 @@ __global__ __launch_bounds__(ROW_THREADS) void potrs_kernel(const uint32_t *a, i
     }
 }
 
+// ---------------------------------------------------------------------------
+// FMA sweeps of the arrow solves (DenseLeaf::forward_many / backward,
+// arrow.rs:1155-1183): one workgroup per (factor, right-hand side), row
+// i + r T of the column in thread i's register r.  Every entry sees the
+// reference's sequence of single-rounding updates x_i = RN(x_i - l x_k):
+// the forward sweep applies k = 0, 1, ... (right-looking: step k broadcasts
+// the final x_k through LDS and every later row applies it, which keeps
+// each row's ascending order); the backward sweep takes blocks of `back`
+// rows from the bottom, and a row applies a block's columns in ascending
+// order: the block's own rows finish one by one from its last (each over
+// the block's later rows), then every earlier row applies the whole block
+// (DenseLeaf::backward_blocked, arrow.rs:1104-1140).
+struct SweepDesc {  // one factor and its right-hand sides
+    int64_t l_off, n, x_off, ldx, d_off;
+};
+
+template <int W, int R>
+__global__ __launch_bounds__(1024) void arrow_forward_kernel(const uint32_t *L_all, const uint32_t *dinv_all,
+                                                            const SweepDesc *descs, uint32_t *X, uint32_t *Wout,
+                                                            uint32_t *Vout, int64_t ldw) {
+    __shared__ uint32_t xk_sh[2][W + 2];
+    const SweepDesc d = descs[blockIdx.y];
+    const int64_t c = blockIdx.x, n = d.n, T = blockDim.x;
+    const uint32_t *L = L_all + d.l_off * (W + 2);
+    uint32_t *x = X + (d.x_off + c * d.ldx) * (W + 2);
+    Val<W> v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int64_t i = threadIdx.x + r * T;
+        if (i < n) load(rec_at<W>((const uint32_t *)x, i), v[r]);
+        else set_zero(v[r]);
+    }
+    for (int64_t k = 0; k < n; k++) {
+        const int par = (int)(k & 1);
+        if (threadIdx.x == k % T) {  // x_k is final: every update k' < k is in
+#pragma unroll
+            for (int r = 0; r < R; r++)
+                if (r == k / T) store(xk_sh[par], v[r]);
+        }
+        __syncthreads();
+        Val<W> xk;
+        load((const uint32_t *)xk_sh[par], xk);
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int64_t i = threadIdx.x + r * T;
+            if (i > k && i < n) {
+                Val<W> l, t;
+                load(rec_at<W>(L, i + k * n), l);
+                fms(v[r], l, xk, t);
+                v[r] = t;
+            }
+        }
+    }
+    // w (the forward result) and, when dinv is given, v = RN(w D^-1)
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int64_t i = threadIdx.x + r * T;
+        if (i >= n) continue;
+        if (Wout) store(rec_at<W>(Wout, d.x_off + i + c * ldw), v[r]);
+        if (dinv_all) {
+            Val<W> di, sv;
+            load(rec_at<W>(dinv_all, d.d_off + i), di);
+            mul(v[r], di, sv);
+            if (Vout) store(rec_at<W>(Vout, d.x_off + i + c * ldw), sv);
+            else store(rec_at<W>(x, i), sv);
+        } else if (!Wout) {
+            store(rec_at<W>(x, i), v[r]);
+        }
+    }
+}
+
+template <int W, int R>
+__global__ __launch_bounds__(1024) void arrow_backward_kernel(const uint32_t *L_all, const SweepDesc *descs,
+                                                             uint32_t *X, int back) {
+    extern __shared__ uint32_t blk_sh[];  // [back][W + 2]: the current block's final x
+    const SweepDesc d = descs[blockIdx.y];
+    const int64_t c = blockIdx.x, n = d.n, T = blockDim.x;
+    const uint32_t *L = L_all + d.l_off * (W + 2);
+    uint32_t *x = X + (d.x_off + c * d.ldx) * (W + 2);
+    Val<W> v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int64_t i = threadIdx.x + r * T;
+        if (i < n) load(rec_at<W>((const uint32_t *)x, i), v[r]);
+        else set_zero(v[r]);
+    }
+    for (int64_t r1 = n; r1 > 0;) {
+        const int64_t r0 = r1 > back ? r1 - back : 0;
+        // the block's rows, last first: row i applies k = i + 1 .. r1 - 1
+        for (int64_t i = r1 - 1; i >= r0; i--) {
+            if (threadIdx.x == i % T) {
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    if (r != i / T) continue;
+                    for (int64_t k = i + 1; k < r1; k++) {
+                        Val<W> l, xk, t;
+                        load(rec_at<W>(L, k + i * n), l);
+                        load((const uint32_t *)(blk_sh + (k - r0) * (W + 2)), xk);
+                        fms(v[r], l, xk, t);
+                        v[r] = t;
+                    }
+                    store(blk_sh + (i - r0) * (W + 2), v[r]);
+                }
+            }
+            __syncthreads();
+        }
+        // every earlier row applies the block, ascending
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int64_t i = threadIdx.x + r * T;
+            if (i >= r0) continue;
+            for (int64_t k = r0; k < r1; k++) {
+                Val<W> l, xk, t;
+                load(rec_at<W>(L, k + i * n), l);
+                load((const uint32_t *)(blk_sh + (k - r0) * (W + 2)), xk);
+                fms(v[r], l, xk, t);
+                v[r] = t;
+            }
+        }
+        __syncthreads();  // the block's LDS rows are read before the next block writes them
+        r1 = r0;
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int64_t i = threadIdx.x + r * T;
+        if (i < n) store(rec_at<W>(x, i), v[r]);
+    }
+}
+
+// count zero values
+template <int W>
+__global__ void zero_values_kernel(uint32_t *P, int64_t count) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= count) return;
+    Val<W> z;
+    set_zero(z);
+    store(rec_at<W>(P, e), z);
+}
+
+// X[i + j ldx] = RN(RN(Wm[i + j ldw] - S[i + j lds]) D[i]) (second_many's
+// update, arrow.rs:1717-1729) for an m x n block
+template <int W>
+__global__ void arrow_second_kernel(const uint32_t *Wm, int64_t ldw, const uint32_t *S, int64_t lds,
+                                    const uint32_t *D, uint32_t *X, int64_t ldx, int64_t m, int64_t n) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m * n) return;
+    const int64_t i = e % m, j = e / m;
+    Val<W> w, sv, t, di, r;
+    load(rec_at<W>(Wm, i + j * ldw), w);
+    load(rec_at<W>(S, i + j * lds), sv);
+    add_signed(w, sv, true, t);
+    load(rec_at<W>(D, i), di);
+    mul(t, di, r);
+    store(rec_at<W>(X, i + j * ldx), r);
+}
+
 // S[i + j*lds] = RN(S - P[i + j*ldp]) for an m x n block
 template <int W>
 __global__ void ew_sub_rect_kernel(uint32_t *S, int64_t lds, const uint32_t *P, int64_t ldp,
@@ -2265,6 +2421,69 @@ void mp_ew_sub_rect(int limbs, void *S, int64_t lds, const void *P, int64_t ldp,
     SDPX_DISPATCH_W(limbs, hipLaunchKernelGGL(ew_sub_rect_kernel<W>, dim3(blocks_for(m * n, 256)),
                                               dim3(256), 0, st, (uint32_t *)S, lds,
                                               (const uint32_t *)P, ldp, m, n));
+    HIP_CHECK(hipGetLastError());
+}
+
+namespace {
+// rows per thread of the sweep kernels for an order-n factor, and threads
+int sweep_rows(int64_t n, int &threads) {
+    threads = (int)std::min<int64_t>(1024, std::max<int64_t>(64, (n + 63) / 64 * 64));
+    const int64_t r = (n + threads - 1) / threads;
+    return r <= 1 ? 1 : (r <= 2 ? 2 : (r <= 4 ? 4 : (r <= 8 ? 8 : 0)));
+}
+}  // namespace
+
+#define SDPX_DISPATCH_R(R_, ...)                                    \
+    switch (R_) {                                                  \
+        case 1: { constexpr int R = 1; __VA_ARGS__; break; }       \
+        case 2: { constexpr int R = 2; __VA_ARGS__; break; }       \
+        case 4: { constexpr int R = 4; __VA_ARGS__; break; }       \
+        default: { constexpr int R = 8; __VA_ARGS__; break; }      \
+    }
+
+bool mp_sweep_supports(int64_t n) {
+    int threads;
+    return sweep_rows(n, threads) != 0;
+}
+
+void mp_arrow_forward(int limbs, const void *L_all, const void *dinv_all, const void *descs, int ndesc,
+                      int64_t nmax, int64_t cols, void *X, void *Wout, void *Vout, int64_t ldw, hipStream_t st) {
+    if (ndesc == 0 || nmax == 0 || cols == 0) return;
+    int threads;
+    const int R = sweep_rows(nmax, threads);
+    if (R == 0) throw Error(SDPX_GPU_INVALID, "sweep order above 8192");
+    SDPX_DISPATCH_W(limbs, SDPX_DISPATCH_R(R, hipLaunchKernelGGL((arrow_forward_kernel<W, R>),
+        dim3((unsigned)cols, ndesc), dim3(threads), 0, st, (const uint32_t *)L_all, (const uint32_t *)dinv_all,
+        (const SweepDesc *)descs, (uint32_t *)X, (uint32_t *)Wout, (uint32_t *)Vout, ldw)));
+    HIP_CHECK(hipGetLastError());
+}
+
+void mp_arrow_backward(int limbs, const void *L_all, const void *descs, int ndesc, int64_t nmax, int64_t cols,
+                       void *X, int back, hipStream_t st) {
+    if (ndesc == 0 || nmax == 0 || cols == 0) return;
+    int threads;
+    const int R = sweep_rows(nmax, threads);
+    if (R == 0) throw Error(SDPX_GPU_INVALID, "sweep order above 8192");
+    const size_t lds = (size_t)back * (value_words(limbs)) * 4;
+    SDPX_DISPATCH_W(limbs, SDPX_DISPATCH_R(R, hipLaunchKernelGGL((arrow_backward_kernel<W, R>),
+        dim3((unsigned)cols, ndesc), dim3(threads), lds, st, (const uint32_t *)L_all, (const SweepDesc *)descs,
+        (uint32_t *)X, back)));
+    HIP_CHECK(hipGetLastError());
+}
+
+void mp_zero_values(int limbs, void *P, int64_t count, hipStream_t st) {
+    if (count == 0) return;
+    SDPX_DISPATCH_W(limbs, hipLaunchKernelGGL(zero_values_kernel<W>, dim3(blocks_for(count, 256)), dim3(256), 0, st,
+                                              (uint32_t *)P, count));
+    HIP_CHECK(hipGetLastError());
+}
+
+void mp_arrow_second(int limbs, const void *Wm, int64_t ldw, const void *S, int64_t lds, const void *D, void *X,
+                     int64_t ldx, int64_t m, int64_t n, hipStream_t st) {
+    if (m * n == 0) return;
+    SDPX_DISPATCH_W(limbs, hipLaunchKernelGGL(arrow_second_kernel<W>, dim3(blocks_for(m * n, 256)), dim3(256), 0,
+                                              st, (const uint32_t *)Wm, ldw, (const uint32_t *)S, lds,
+                                              (const uint32_t *)D, (uint32_t *)X, ldx, m, n));
     HIP_CHECK(hipGetLastError());
 }
 

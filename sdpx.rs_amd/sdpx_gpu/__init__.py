@@ -96,6 +96,8 @@ def _declare(L):
                                          ctypes.POINTER(i32)]
     L.sdpx_gpu_sampled_arrow.restype = i32
     L.sdpx_gpu_sampled_stats.argtypes = [dp] + [ctypes.POINTER(ctypes.c_int64)] * 5
+    L.sdpx_gpu_sampled_arrow_solve_dev.argtypes = [u32, u64, vp, u64, vp, vp, vp, vp, vp, u64, vp, u64]
+    L.sdpx_gpu_sampled_arrow_solve_dev.restype = i32
     L.sdpx_gpu_sampled_stats.restype = i32
     L.sdpx_gpu_sampled_arrow_f64_dev.argtypes = [u64, vp, u64, vp, vp, vp, ctypes.c_double,
                                                  ctypes.c_double, ctypes.POINTER(i32), dp]
@@ -401,6 +403,7 @@ class SampledDev:
             self.leaves[i] = SampledLeaf(b["h"], b["k"], *[a.data_ptr() for a in arrs])
         vb = mp.dtype(nl).itemsize
         ks = [b["k"] for b in blocks]
+        self.ks = ks
         self.border0 = dev(problem["border"])
         self.border = torch.empty_like(self.border0)
         self.bdinv = torch.zeros(self.t * vb, dtype=torch.uint8, device="cuda")
@@ -426,6 +429,26 @@ class SampledDev:
                                                _ptr(self.signs), self.signs.size,
                                                self.bdinv.data_ptr(), ctypes.byref(ok)))
         return bool(ok.value)
+
+    def solve(self, rhs, nrhs):
+        """ArrowLDLSolver::solve_many on the factors of the last run():
+        rhs (n x nrhs records, column-major, n = sum K + t: leaf rows in
+        leaf order, then the border rows) -> x, same layout."""
+        import torch
+
+        nl = self.nl
+        n = sum(self.ks) + self.t
+        dt = mp.dtype(nl)
+        rhs = np.ascontiguousarray(rhs, dtype=dt).reshape(-1)
+        assert rhs.size == n * nrhs
+        dx = torch.from_numpy(rhs.view(np.uint8).copy()).to("cuda")
+        torch.cuda.synchronize()
+        ks = (ctypes.c_uint64 * max(1, len(self.ks)))(*self.ks)
+        check(lib().sdpx_gpu_sampled_arrow_solve_dev(nl, len(self.ks), ks, self.t, self.l_all.data_ptr(),
+                                                     self.dinv_all.data_ptr(), self.y_all.data_ptr(),
+                                                     self.border.data_ptr(), self.bdinv.data_ptr(), nrhs,
+                                                     dx.data_ptr(), n))
+        return dx.cpu().numpy().view(dt).copy()
 
     def result(self):
         """(border factor, border dinv) as host record arrays."""

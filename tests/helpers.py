@@ -184,3 +184,24 @@ def oracle_potrs(a, n, lda, b, nrhs, ldb, uplo="L"):
     out = b.copy()
     oracle().oracle_potrs(nl, uplo.encode(), n, nrhs, _p(a), lda, _p(out), ldb)
     return out
+
+
+def oracle_sampled_arrow_solve(outs, t, border, bdinv, rhs, nrhs, threads=8):
+    """C oracle of ArrowLDLSolver::solve_many on the factors of
+    oracle_sampled_arrow (outs: [(l, dinv, y)] per leaf; border with its
+    unit factor in the lower triangle): rhs (n x nrhs records) -> x."""
+    L = oracle()
+    vp, sz = ctypes.c_void_p, ctypes.c_size_t
+    L.oracle_sampled_arrow_solve.argtypes = [ctypes.c_int, sz, vp, sz, vp, vp, vp, vp, vp, sz, vp, sz,
+                                             ctypes.c_int]
+    L.oracle_sampled_arrow_solve.restype = ctypes.c_int
+    nl = mp.limbs_of(border)
+    ks = np.array([d.size for _, d, _ in outs], dtype=np.uint64)
+    l_all = np.concatenate([l for l, _, _ in outs]) if outs else mp.zeros(1, nl)
+    d_all = np.concatenate([d for _, d, _ in outs]) if outs else mp.zeros(1, nl)
+    y_all = np.concatenate([y for _, _, y in outs]) if outs else mp.zeros(1, nl)
+    n = int(ks.sum()) + t
+    x = np.ascontiguousarray(rhs).reshape(-1).copy()
+    L.oracle_sampled_arrow_solve(nl, len(outs), _p(ks), t, _p(l_all), _p(d_all), _p(y_all), _p(border),
+                                 _p(bdinv), nrhs, _p(x), n, threads)
+    return x

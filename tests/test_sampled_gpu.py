@@ -79,3 +79,27 @@ def test_failed_leaf_index_reported(gpu):
     assert not ok and st["declined"] == 0 and st["failed_leaf"] == 2
     okh, _, _, _ = sdpx_gpu.sampled_arrow(prob, nl, flags=sdpx_gpu.LDL_HOST_PRODUCTS)
     assert not okh and sdpx_gpu.sampled_stats()["failed_leaf"] == 2
+
+
+@pytest.mark.parametrize("full", [False, True])
+def test_arrow_solve_matches_oracle(gpu, full):
+    """ArrowLDLSolver::solve_many on the device factors: bitwise the
+    oracle's FMA sweeps and exact coupling dots, on a reduced instance and
+    on the full 40-block, t = 2000 instance (4 right-hand sides)."""
+    from helpers import oracle_sampled_arrow_solve
+
+    nl = 4
+    prob = (workloads.sampled_problem(nl, seed=7) if full else
+            workloads.sampled_problem(nl, nblocks=6, hmin=10, hmax=40, t=300, seed=17))
+    t, nrhs = prob["t"], 4
+    n = sum(b["k"] for b in prob["blocks"]) + t
+    print("arrow solve", full, n, flush=True)
+    ok, border, bdinv, outs = oracle_sampled_arrow(prob, nl, threads=16)
+    run = sdpx_gpu.SampledDev(prob, nl)
+    assert ok and run.run()
+    rng = np.random.default_rng(3)
+    rhs = mp.from_float64(rng.standard_normal(n * nrhs), nl)
+    x = run.solve(rhs, nrhs)
+    xo = oracle_sampled_arrow_solve(outs, t, border, bdinv, rhs, nrhs, threads=16)
+    bad = ~mp.same_bits(x, xo)
+    assert not bad.any(), f"{bad.sum()} of {x.size} solution entries differ"
