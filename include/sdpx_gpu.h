@@ -379,6 +379,30 @@ int32_t sdpx_gpu_adjoint_psd_f64_dev(sdpx_gpu_dense *h, uint64_t n, const double
                                      const double *a_svec, uint64_t nrhs, const double *ybar,
                                      double *bgrad, double *cgrad);
 
+/* Full adjoint at the solution's central-path point (X, y, S: A(X) = b,
+ * A*(y) + S = C, X S = mu I; the final iterate of an IPM that targets mu),
+ * with the same factor of M: for nrhs functionals
+ *   L_k = ybar_k . y + <Xbar_k, X> + obj_k <C, X>
+ * (each of ybar m x nrhs, xbar s x nrhs, obj nrhs optional: null is zero)
+ * it returns d L_k / db (bgrad m x nrhs), d L_k / d svec(C) (cgrad s x
+ * nrhs) and, when agrad is not null, d L_k / d svec(A_i) for every i
+ * (agrad s x m x nrhs, column i of block k).  With W = Rinv^T Rinv the NT
+ * scaling (W S W = X), the point moves as dX = -W dS W, dS = dC - dA*(y) -
+ * A*(dy), M dy = db + A(W (dC - dA*(y)) W) - dA(X); so with Z = W (Xbar +
+ * obj C) W, v = M^-1 (ybar + A(Z)) and U = W A*(v) W:
+ *   bgrad = v,  cgrad = U - Z + obj svec(X),  agrad_i = y_i (Z - U) - v_i svec(X).
+ * The transposed KKT solve is the same factor (M is symmetric; KKT
+ * quasi-definite, paper/sdpx.tex:389-396).  y (m) and x_svec, c_svec (s)
+ * are the point's y, svec(X) and svec(C).  No reference counterpart. */
+int32_t sdpx_gpu_adjoint_psd_full_f64(sdpx_gpu_dense *h, uint64_t n, const double *rinv, const double *a_svec,
+                                      const double *y, const double *x_svec, const double *c_svec, uint64_t nrhs,
+                                      const double *ybar, const double *xbar, const double *obj, double *bgrad,
+                                      double *cgrad, double *agrad);
+int32_t sdpx_gpu_adjoint_psd_full_f64_dev(sdpx_gpu_dense *h, uint64_t n, const double *rinv, const double *a_svec,
+                                          const double *y, const double *x_svec, const double *c_svec,
+                                          uint64_t nrhs, const double *ybar, const double *xbar,
+                                          const double *obj, double *bgrad, double *cgrad, double *agrad);
+
 #ifdef __cplusplus
 }
 #endif

@@ -297,4 +297,58 @@ int32_t sdpx_gpu_adjoint_psd_f64_dev(sdpx_gpu_dense *h, uint64_t n, const double
     });
 }
 
+
+int32_t sdpx_gpu_adjoint_psd_full_f64_dev(sdpx_gpu_dense *h, uint64_t n, const double *rinv, const double *a_svec,
+                                          const double *y, const double *x_svec, const double *c_svec,
+                                          uint64_t nrhs, const double *ybar, const double *xbar,
+                                          const double *obj, double *bgrad, double *cgrad, double *agrad) {
+    return abi_boundary([&]() -> int32_t {
+        if (!h || !h->factored) return abi_invalid("no successful factorization");
+        if (!rinv || !a_svec || !y || !x_svec || !bgrad || !cgrad) return abi_invalid("null operand");
+        if (obj && !c_svec) return abi_invalid("an objective weight needs C");
+        if (h->n == 0 || nrhs == 0) return SDPX_GPU_OK;
+        sdpx::Device &d = sdpx::device();
+        std::lock_guard<std::recursive_mutex> g(d.mu);
+        sdpx::adjoint_psd_full_f64(h->L, h->n, (int64_t)n, rinv, a_svec, y, x_svec, c_svec, (int64_t)nrhs, ybar,
+                                   xbar, obj, bgrad, cgrad, agrad, d.stream);
+        HIP_CHECK(hipStreamSynchronize(d.stream));
+        return SDPX_GPU_OK;
+    });
+}
+
+int32_t sdpx_gpu_adjoint_psd_full_f64(sdpx_gpu_dense *h, uint64_t n, const double *rinv, const double *a_svec,
+                                      const double *y, const double *x_svec, const double *c_svec, uint64_t nrhs,
+                                      const double *ybar, const double *xbar, const double *obj, double *bgrad,
+                                      double *cgrad, double *agrad) {
+    return abi_boundary([&]() -> int32_t {
+        if (!h || !h->factored) return abi_invalid("no successful factorization");
+        if (!rinv || !a_svec || !y || !x_svec || !bgrad || !cgrad) return abi_invalid("null operand");
+        if (obj && !c_svec) return abi_invalid("an objective weight needs C");
+        const uint64_t m = (uint64_t)h->n, s = n * (n + 1) / 2;
+        if (m == 0 || nrhs == 0) return SDPX_GPU_OK;
+        sdpx::Device &d = sdpx::device();
+        std::lock_guard<std::recursive_mutex> g(d.mu);
+        hipStream_t st = d.stream;
+        auto put = [&](int slot, const double *src, size_t count) -> double * {
+            if (!src) return nullptr;
+            double *p = (double *)d.ws.get(slot, 8 * std::max<size_t>(count, 1));
+            HIP_CHECK(hipMemcpyAsync(p, src, 8 * count, hipMemcpyHostToDevice, st));
+            return p;
+        };
+        // staging slots of the full adjoint (100-110: no other entry point uses them)
+        double *dr = put(100, rinv, n * n), *da = put(101, a_svec, s * m), *dy = put(102, y, m);
+        double *dx = put(103, x_svec, s), *dc = put(104, c_svec, s), *dyb = put(105, ybar, m * nrhs);
+        double *dxb = put(106, xbar, s * nrhs), *dob = put(107, obj, nrhs);
+        double *db = (double *)d.ws.get(108, 8 * m * nrhs), *dcg = (double *)d.ws.get(109, 8 * s * nrhs);
+        double *dag = agrad ? (double *)d.ws.get(110, 8 * s * m * nrhs) : nullptr;
+        sdpx::adjoint_psd_full_f64(h->L, (int64_t)m, (int64_t)n, dr, da, dy, dx, dc, (int64_t)nrhs, dyb, dxb, dob,
+                                   db, dcg, dag, st);
+        HIP_CHECK(hipMemcpyAsync(bgrad, db, 8 * m * nrhs, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(cgrad, dcg, 8 * s * nrhs, hipMemcpyDeviceToHost, st));
+        if (agrad) HIP_CHECK(hipMemcpyAsync(agrad, dag, 8 * s * m * nrhs, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        return SDPX_GPU_OK;
+    });
+}
+
 }  // extern "C"

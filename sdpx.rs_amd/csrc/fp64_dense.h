@@ -51,6 +51,15 @@ void adjoint_psd_f64(const double *Lm, int64_t m, int64_t n, const double *rinv,
                      const double *a_svec, int64_t nrhs, const double *ybar, double *bgrad,
                      double *cgrad, hipStream_t st);
 
+// Full adjoint at a central-path point (see fp64_dense.hip): gradients of
+// L = ybar . y + <Xbar, X> + obj <C, X> with respect to b (m x nrhs), C
+// (s x nrhs, svec) and, when agrad is given, every A_i (s x m x nrhs).
+// ybar, xbar, obj (nrhs weights) and agrad may be null.
+void adjoint_psd_full_f64(const double *Lm, int64_t m, int64_t n, const double *rinv, const double *a_svec,
+                          const double *y, const double *x_svec, const double *c_svec, int64_t nrhs,
+                          const double *ybar, const double *xbar, const double *obj, double *bgrad,
+                          double *cgrad, double *agrad, hipStream_t st);
+
 __global__ void svec_unpack(const double *svec, int64_t s, int64_t n, int64_t m, double *out);
 __global__ void svec_pack(const double *U, int64_t s, int64_t n, int64_t m, double *M);
 

@@ -576,3 +576,117 @@ void adjoint_psd_f64(const double *Lm, int64_t m, int64_t n, const double *rinv,
 }
 
 }  // namespace sdpx
+
+namespace sdpx {
+namespace {
+// W X_k W for nrhs symmetric matrices given as svec columns (in: s x nrhs),
+// W = Rinv^T Rinv, through two congruences of the stacked matrices
+void w_congruence(const double *rinv, int64_t n, const double *in, int64_t nrhs, double *out, double *stack,
+                  double *x, hipStream_t st) {
+    const int64_t s = n * (n + 1) / 2, total = s * nrhs;
+    hipLaunchKernelGGL(svec_unpack, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, in, s, n, nrhs, stack);
+    HIP_CHECK(hipGetLastError());
+    for (int pass = 0; pass < 2; pass++) {
+        const bool t = pass == 1;
+        Fp64Gemm g1;  // X_k = Z_k F^T (F = Rinv, then Rinv^T)
+        g1.m = n * nrhs; g1.n = n; g1.k = n;
+        g1.A = stack; g1.lda = n * nrhs; g1.B = rinv; g1.ldb = n; g1.tb = !t;
+        g1.C = x; g1.ldc = n * nrhs;
+        gemm_f64(g1, st);
+        Fp64Gemm g2;  // Z_k = F X_k
+        g2.m = n; g2.n = n; g2.k = n;
+        g2.A = rinv; g2.lda = n; g2.ta = t; g2.B = x; g2.ldb = n * nrhs; g2.sb = n;
+        g2.C = stack; g2.ldc = n * nrhs; g2.sc = n; g2.batch = nrhs;
+        gemm_f64(g2, st);
+    }
+    hipLaunchKernelGGL(svec_pack, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, stack, s, n, nrhs, out);
+    HIP_CHECK(hipGetLastError());
+}
+
+// out[:, k] = a[:, k] + w_k b  (s x nrhs; a may be null: zero)
+__global__ void axpy_cols(const double *a, const double *wts, const double *b, int64_t s, int64_t nrhs, double *out) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= s * nrhs) return;
+    const int64_t i = e % s, k = e / s;
+    out[e] = (a ? a[e] : 0.0) + (wts ? wts[k] * b[i] : 0.0);
+}
+
+// cgrad[:, k] = U[:, k] - Z[:, k] + w_k x
+__global__ void cgrad_cols(const double *U, const double *Z, const double *wts, const double *x, int64_t s,
+                           int64_t nrhs, double *out) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= s * nrhs) return;
+    const int64_t i = e % s, k = e / s;
+    out[e] = U[e] - Z[e] + (wts ? wts[k] * x[i] : 0.0);
+}
+
+// agrad[i + s (j + m k)] = y_j (Z - U)[i, k] - v[j, k] x_i
+__global__ void agrad_kernel(const double *U, const double *Z, const double *y, const double *v, const double *x,
+                             int64_t s, int64_t m, int64_t nrhs, double *agrad) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= s * m * nrhs) return;
+    const int64_t i = e % s, j = (e / s) % m, k = e / (s * m);
+    agrad[e] = y[j] * (Z[i + s * k] - U[i + s * k]) - v[j + m * k] * x[i];
+}
+}  // namespace
+
+// Full adjoint at a central-path point (no reference counterpart).  The
+// point (X, y, S) with A(X) = b, A*(y) + S = C, X S = mu I moves with the
+// data as dX = -W dS W (W = Rinv^T Rinv the NT scaling: W S W = X), dS = dC
+// - dA*(y) - A*(dy), and M dy = db + A(W (dC - dA*(y)) W) - dA(X), M_ij =
+// <A_i, W A_j W>.  For the functional L = ybar . y + <Xbar, X> + obj <C, X>,
+// with Z = W (Xbar + obj C) W, g = ybar + A(Z), v = M^-1 g (the same
+// factor: M is symmetric) and U = W A*(v) W:
+//   dL/db = v,  dL/dC = U - Z + obj X,  dL/dA_i = y_i (Z - U) - v_i X.
+// All operands device pointers; matrices as svec columns (s = n(n+1)/2).
+void adjoint_psd_full_f64(const double *Lm, int64_t m, int64_t n, const double *rinv, const double *a_svec,
+                          const double *y, const double *x_svec, const double *c_svec, int64_t nrhs,
+                          const double *ybar, const double *xbar, const double *obj, double *bgrad,
+                          double *cgrad, double *agrad, hipStream_t st) {
+    Workspace &ws = device().ws;
+    const int64_t s = n * (n + 1) / 2, total = s * nrhs;
+    double *z = (double *)ws.get(60, 8 * (size_t)s * nrhs);
+    double *u = (double *)ws.get(68, 8 * (size_t)s * nrhs);
+    double *stack = (double *)ws.get(61, 8 * (size_t)n * n * nrhs);
+    double *x = (double *)ws.get(62, 8 * (size_t)n * n * nrhs);
+    // Z = W (Xbar + obj C) W
+    const bool have_z = xbar || obj;
+    if (have_z) {
+        hipLaunchKernelGGL(axpy_cols, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, xbar, obj, c_svec, s,
+                           nrhs, u);
+        HIP_CHECK(hipGetLastError());
+        w_congruence(rinv, n, u, nrhs, z, stack, x, st);
+    } else {
+        HIP_CHECK(hipMemsetAsync(z, 0, 8 * (size_t)total, st));
+    }
+    // g = ybar + A(Z) into bgrad, then v = M^-1 g in place
+    if (ybar)
+        HIP_CHECK(hipMemcpyAsync(bgrad, ybar, 8 * (size_t)m * nrhs, hipMemcpyDeviceToDevice, st));
+    if (have_z || !ybar) {
+        Fp64Gemm g0;
+        g0.m = m; g0.n = nrhs; g0.k = s;
+        g0.A = a_svec; g0.lda = s; g0.ta = true; g0.B = z; g0.ldb = s; g0.C = bgrad; g0.ldc = m;
+        g0.beta = ybar ? 1.0 : 0.0;
+        gemm_f64(g0, st);
+    }
+    potrs_f64(Lm, m, m, bgrad, nrhs, m, nullptr, 0, st);
+    // U = W A*(v) W
+    Fp64Gemm g1;
+    g1.m = s; g1.n = nrhs; g1.k = m;
+    g1.A = a_svec; g1.lda = s; g1.B = bgrad; g1.ldb = m; g1.C = u; g1.ldc = s;
+    gemm_f64(g1, st);
+    double *wu = cgrad;  // W A*(v) W, then U - Z + obj X in place
+    w_congruence(rinv, n, u, nrhs, wu, stack, x, st);
+    HIP_CHECK(hipMemcpyAsync(u, wu, 8 * (size_t)total, hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(cgrad_cols, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, u, z, obj, x_svec, s,
+                       nrhs, cgrad);
+    HIP_CHECK(hipGetLastError());
+    if (agrad) {
+        const int64_t ta = s * m * nrhs;
+        hipLaunchKernelGGL(agrad_kernel, dim3((unsigned)((ta + 255) / 256)), dim3(256), 0, st, u, z, y, bgrad, x_svec,
+                           s, m, nrhs, agrad);
+        HIP_CHECK(hipGetLastError());
+    }
+}
+
+}  // namespace sdpx

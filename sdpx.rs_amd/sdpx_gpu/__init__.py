@@ -107,6 +107,9 @@ def _declare(L):
     for name in ("sdpx_gpu_adjoint_psd_f64", "sdpx_gpu_adjoint_psd_f64_dev"):
         getattr(L, name).argtypes = [vp, u64, vp, vp, u64, vp, vp, vp]
         getattr(L, name).restype = i32
+    for name in ("sdpx_gpu_adjoint_psd_full_f64", "sdpx_gpu_adjoint_psd_full_f64_dev"):
+        getattr(L, name).argtypes = [vp, u64, vp, vp, vp, vp, vp, u64, vp, vp, vp, vp, vp, vp]
+        getattr(L, name).restype = i32
     L.sdpx_gpu_rns_gemm_blocks_upper.argtypes = [u32, u64, u64, vp, vp]
     L.sdpx_gpu_rns_gemm_blocks_upper.restype = i32
     L.sdpx_gpu_mp_ldl_dev.argtypes = [u32, u64, vp, u64, vp, u64, vp, vp, vp, u32,
@@ -524,6 +527,30 @@ def adjoint_psd_f64(factor, rinv, a_svec, ybar):
                                          a_svec.ctypes.data_as(vp), r, ybar.ctypes.data_as(vp),
                                          bgrad.ctypes.data_as(vp), cgrad.ctypes.data_as(vp)))
     return bgrad, cgrad
+
+
+def adjoint_psd_full_f64(factor, rinv, a_svec, y, x_svec, c_svec=None, ybar=None, xbar=None, obj=None,
+                         want_a=False):
+    """Full adjoint at the central-path point (sdpx_gpu_adjoint_psd_full_f64):
+    for functionals L_k = ybar_k . y + <Xbar_k, X> + obj_k <C, X> returns
+    (bgrad m x r, cgrad s x r, agrad s x m x r or None)."""
+    f64 = lambda a: None if a is None else np.asfortranarray(a, dtype=np.float64)
+    rinv, a_svec, y, x_svec, c_svec = f64(rinv), f64(a_svec), f64(y), f64(x_svec), f64(c_svec)
+    n = rinv.shape[0]
+    s, m = a_svec.shape
+    cols = [a.shape[1] for a in (ybar, xbar) if a is not None and np.ndim(a) == 2]
+    r = cols[0] if cols else len(obj)
+    ybar, xbar = f64(ybar), f64(xbar)
+    obj = None if obj is None else np.ascontiguousarray(obj, dtype=np.float64)
+    bgrad = np.zeros((m, r), order="F")
+    cgrad = np.zeros((s, r), order="F")
+    agrad = np.zeros((s, m, r), order="F") if want_a else None
+    vp = ctypes.c_void_p
+    ptr = lambda a: None if a is None else a.ctypes.data_as(vp)
+    check(lib().sdpx_gpu_adjoint_psd_full_f64(factor._h, n, ptr(rinv), ptr(a_svec), ptr(y), ptr(x_svec),
+                                              ptr(c_svec), r, ptr(ybar), ptr(xbar), ptr(obj), ptr(bgrad),
+                                              ptr(cgrad), ptr(agrad)))
+    return bgrad, cgrad, agrad
 
 
 class ExactBlock(ctypes.Structure):

@@ -77,11 +77,12 @@ class Stats(dict):
     pass
 
 
-def solve(prob, backend="gpu", tol=1e-8, max_iter=80, verbose=False, target_mu=None):
+def solve(prob, backend="gpu", tol=1e-8, max_iter=80, verbose=False, target_mu=None, centre_tol=1e-9):
     """Returns (X, y, S, info).  With target_mu, returns instead the point of
     the central path X S = target_mu I (Newton centring steps once mu is
-    near it); the adjoint of the Schur solve is the exact derivative of that
-    point, so finite differences of it pin the adjoint at any step size."""
+    near it, until |X S - mu I| / (mu sqrt n) < centre_tol); the adjoint of
+    the Schur solve is the exact derivative of that point, so finite
+    differences of it pin the adjoint at any step size."""
     n, m = prob["n"], prob["m"]
     A, a_svec, b, C = prob["A"], prob["a_svec"], prob["b"], prob["C"]
 
@@ -109,7 +110,7 @@ def solve(prob, backend="gpu", tol=1e-8, max_iter=80, verbose=False, target_mu=N
         centring = target_mu is not None and mu < 10 * target_mu
         if target_mu is not None:
             cres = np.linalg.norm(X @ S - target_mu * np.eye(n)) / (target_mu * np.sqrt(n))
-            if centring and cres < 1e-9 and pinf < 1e-10 and dinf < 1e-12:
+            if centring and cres < centre_tol and pinf < 1e-10 and dinf < 1e-12:
                 info["status"] = "centred"
                 break
         elif gap < tol and pinf < tol and dinf < tol:
