@@ -360,6 +360,30 @@ int32_t sdpx_gpu_schur_psd_f64(uint64_t n, uint64_t m, const double *rinv, const
 int32_t sdpx_gpu_schur_psd_f64_dev(uint64_t n, uint64_t m, const double *rinv,
                                    const double *a_svec, double *s, uint64_t lds);
 
+/* Condensed Schur assembly of a reduced KKT with several cones (replaces
+ * CondensedKKT::assemble, kkt/condensed/mod.rs:1962-2147, for Float64
+ * PSD and orthant cones; device pointers): the upper triangle of S (m x m,
+ * lds) becomes
+ *   P (CSC: colptr m + 1, rowval, nzval; entries i <= j; optional)
+ *   + each PSD block's S_ab = <A_a, W A_b W> (W = Rinv^T Rinv, the block's
+ *     NT scaling: kkt.rs:1119-1127, psd.rs:1299-1456) at its global
+ *     columns, blocks published in cone order (2071-2086)
+ *   + the orthant rows' Gram B^T diag(w) B (rows x m, ldb; the eliminated
+ *     rows of 2090-2101).
+ * Summation order differs from the reference's BLAS; parity is by
+ * tolerance (1e-12 relative).  struct_size must be sizeof the struct. */
+typedef struct {
+    uint64_t struct_size;       /* sizeof(sdpx_gpu_psd_block_f64) */
+    uint64_t n, ncols;          /* block order, constraint columns touching it */
+    const double *rinv;         /* n x n */
+    const double *a_svec;       /* s x ncols, s = n(n+1)/2 */
+    const uint64_t *columns;    /* ncols global column indices < m */
+} sdpx_gpu_psd_block_f64;
+int32_t sdpx_gpu_condensed_assemble_f64_dev(uint64_t m, const int64_t *p_colptr, const int64_t *p_rowval,
+                                            const double *p_nzval, uint64_t nblocks,
+                                            const sdpx_gpu_psd_block_f64 *blocks, uint64_t rows, const double *b,
+                                            uint64_t ldb, const double *w, double *s, uint64_t lds);
+
 /* ---- Adjoint (differentiation) ---------------------------------------
  *
  * No counterpart in the reference (it has no differentiation); built on its

@@ -351,4 +351,33 @@ int32_t sdpx_gpu_adjoint_psd_full_f64(sdpx_gpu_dense *h, uint64_t n, const doubl
     });
 }
 
+
+int32_t sdpx_gpu_condensed_assemble_f64_dev(uint64_t m, const int64_t *p_colptr, const int64_t *p_rowval,
+                                            const double *p_nzval, uint64_t nblocks,
+                                            const sdpx_gpu_psd_block_f64 *blocks, uint64_t rows, const double *b,
+                                            uint64_t ldb, const double *w, double *s, uint64_t lds) {
+    return abi_boundary([&]() -> int32_t {
+        if (lds < m) return abi_invalid("lds below m");
+        if (m && !s) return abi_invalid("null output");
+        if ((p_colptr == nullptr) != (p_nzval == nullptr) || (p_colptr && !p_rowval))
+            return abi_invalid("P needs colptr, rowval and nzval");
+        if (rows && (!b || !w || ldb < rows)) return abi_invalid("orthant rows need b (ldb >= rows) and w");
+        if (nblocks && !blocks) return abi_invalid("null blocks");
+        std::vector<sdpx::PsdBlockF64> bl(nblocks);
+        for (uint64_t i = 0; i < nblocks; i++) {
+            const auto &x = blocks[i];
+            if (x.struct_size != sizeof(sdpx_gpu_psd_block_f64)) return abi_invalid("psd block struct_size");
+            if (x.ncols && (!x.rinv || !x.a_svec || !x.columns)) return abi_invalid("null block array");
+            bl[i] = sdpx::PsdBlockF64{(int64_t)x.n, (int64_t)x.ncols, x.rinv, x.a_svec, x.columns};
+        }
+        if (m == 0) return SDPX_GPU_OK;
+        sdpx::Device &d = sdpx::device();
+        std::lock_guard<std::recursive_mutex> g(d.mu);
+        sdpx::condensed_assemble_f64((int64_t)m, p_colptr, p_rowval, p_nzval, bl, (int64_t)rows, b, (int64_t)ldb, w,
+                                     s, (int64_t)lds, d.stream);
+        HIP_CHECK(hipStreamSynchronize(d.stream));
+        return SDPX_GPU_OK;
+    });
+}
+
 }  // extern "C"

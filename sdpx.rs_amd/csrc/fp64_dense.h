@@ -1,5 +1,6 @@
 // Float64 dense kernels (see fp64_dense.hip).
 #pragma once
+#include <vector>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -59,6 +60,18 @@ void adjoint_psd_full_f64(const double *Lm, int64_t m, int64_t n, const double *
                           const double *y, const double *x_svec, const double *c_svec, int64_t nrhs,
                           const double *ybar, const double *xbar, const double *obj, double *bgrad,
                           double *cgrad, double *agrad, hipStream_t st);
+
+// One PSD block of a condensed assembly: ncols constraint columns given as
+// svec columns (s x ncols) over an n x n block, its NT factor Rinv, and the
+// columns' global indices (device).
+struct PsdBlockF64 {
+    int64_t n, ncols;
+    const double *rinv, *a_svec;
+    const uint64_t *columns;
+};
+void condensed_assemble_f64(int64_t m, const int64_t *p_colptr, const int64_t *p_rowval, const double *p_nzval,
+                            const std::vector<PsdBlockF64> &blocks, int64_t rows, const double *B, int64_t ldb,
+                            const double *w, double *S, int64_t lds, hipStream_t st);
 
 __global__ void svec_unpack(const double *svec, int64_t s, int64_t n, int64_t m, double *out);
 __global__ void svec_pack(const double *U, int64_t s, int64_t n, int64_t m, double *M);

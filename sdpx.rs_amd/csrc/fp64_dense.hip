@@ -690,3 +690,83 @@ void adjoint_psd_full_f64(const double *Lm, int64_t m, int64_t n, const double *
 }
 
 }  // namespace sdpx
+
+namespace sdpx {
+namespace {
+// S[i + j lds] += P (CSC, entries i <= j), one thread a column
+__global__ void add_csc_upper(const int64_t *colptr, const int64_t *rowval, const double *nzval, int64_t m,
+                              double *S, int64_t lds) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= m) return;
+    for (int64_t p = colptr[j]; p < colptr[j + 1]; p++) {
+        const int64_t i = rowval[p];
+        if (i <= j) S[i + j * lds] += nzval[p];
+    }
+}
+
+// S[c_a, c_b] += T[a, b] for a <= b (T upper, ld k): a block's columns
+// scattered into the reduced Schur's upper triangle
+__global__ void scatter_block_upper(const double *T, int64_t k, const uint64_t *cols, double *S, int64_t lds) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= k * k) return;
+    const int64_t a = e % k, b = e / k;
+    if (a > b) return;
+    const uint64_t ca = cols[a], cb = cols[b];
+    const uint64_t i = ca < cb ? ca : cb, j = ca < cb ? cb : ca;
+    S[i + j * lds] += T[a + b * k];
+}
+
+// G[r + c ld] = w_r B[r + c ld]
+__global__ void scale_rows(const double *B, int64_t ld, const double *w, int64_t r, int64_t m, double *G) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= r * m) return;
+    const int64_t i = e % r, c = e / r;
+    G[i + c * ld] = w[i] * B[i + c * ld];
+}
+
+__global__ void zero_upper(double *S, int64_t m, int64_t lds) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m * m) return;
+    const int64_t i = e % m, j = e / m;
+    if (i <= j) S[i + j * lds] = 0.0;
+}
+}  // namespace
+
+// The condensed Schur assembly of a reduced KKT (device pointers): S's upper
+// triangle = P, then each PSD block's contribution S_ab = <A_a, W A_b W>
+// (W = Rinv^T Rinv) scattered to its global columns in cone order, then the
+// orthant rows' Gram B^T diag(w) B.  Workspace slots 34-36.
+void condensed_assemble_f64(int64_t m, const int64_t *p_colptr, const int64_t *p_rowval, const double *p_nzval,
+                            const std::vector<PsdBlockF64> &blocks, int64_t rows, const double *B, int64_t ldb,
+                            const double *w, double *S, int64_t lds, hipStream_t st) {
+    Workspace &ws = device().ws;
+    hipLaunchKernelGGL(zero_upper, dim3((unsigned)((m * m + 255) / 256)), dim3(256), 0, st, S, m, lds);
+    HIP_CHECK(hipGetLastError());
+    if (p_colptr) {
+        hipLaunchKernelGGL(add_csc_upper, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, p_colptr, p_rowval,
+                           p_nzval, m, S, lds);
+        HIP_CHECK(hipGetLastError());
+    }
+    for (const auto &b : blocks) {
+        if (b.ncols == 0) continue;
+        double *T = (double *)ws.get(34, 8 * (size_t)b.ncols * b.ncols);
+        schur_psd_f64(b.n, b.ncols, b.rinv, b.a_svec, T, b.ncols, st);
+        const int64_t kk = b.ncols * b.ncols;
+        hipLaunchKernelGGL(scatter_block_upper, dim3((unsigned)((kk + 255) / 256)), dim3(256), 0, st, T, b.ncols,
+                           b.columns, S, lds);
+        HIP_CHECK(hipGetLastError());
+    }
+    if (rows > 0 && B) {
+        double *G = (double *)ws.get(35, 8 * (size_t)ldb * m);
+        hipLaunchKernelGGL(scale_rows, dim3((unsigned)((rows * m + 255) / 256)), dim3(256), 0, st, B, ldb, w, rows, m,
+                           G);
+        HIP_CHECK(hipGetLastError());
+        Fp64Gemm g;
+        g.m = m; g.n = m; g.k = rows;
+        g.A = B; g.lda = ldb; g.ta = true; g.B = G; g.ldb = ldb; g.C = S; g.ldc = lds;
+        g.beta = 1.0; g.tri = 1;
+        gemm_f64(g, st);
+    }
+}
+
+}  // namespace sdpx

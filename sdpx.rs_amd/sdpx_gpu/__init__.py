@@ -110,6 +110,8 @@ def _declare(L):
     for name in ("sdpx_gpu_adjoint_psd_full_f64", "sdpx_gpu_adjoint_psd_full_f64_dev"):
         getattr(L, name).argtypes = [vp, u64, vp, vp, vp, vp, vp, u64, vp, vp, vp, vp, vp, vp]
         getattr(L, name).restype = i32
+    L.sdpx_gpu_condensed_assemble_f64_dev.argtypes = [u64, vp, vp, vp, u64, vp, u64, vp, u64, vp, vp, u64]
+    L.sdpx_gpu_condensed_assemble_f64_dev.restype = i32
     L.sdpx_gpu_rns_gemm_blocks_upper.argtypes = [u32, u64, u64, vp, vp]
     L.sdpx_gpu_rns_gemm_blocks_upper.restype = i32
     L.sdpx_gpu_mp_ldl_dev.argtypes = [u32, u64, vp, u64, vp, u64, vp, vp, vp, u32,
@@ -551,6 +553,44 @@ def adjoint_psd_full_f64(factor, rinv, a_svec, y, x_svec, c_svec=None, ybar=None
                                               ptr(c_svec), r, ptr(ybar), ptr(xbar), ptr(obj), ptr(bgrad),
                                               ptr(cgrad), ptr(agrad)))
     return bgrad, cgrad, agrad
+
+
+class PsdBlockF64(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint64), ("n", ctypes.c_uint64), ("ncols", ctypes.c_uint64),
+                ("rinv", ctypes.c_void_p), ("a_svec", ctypes.c_void_p), ("columns", ctypes.c_void_p)]
+
+
+def condensed_assemble_f64(m, p_csc, blocks, b_rows=None, w=None):
+    """CondensedKKT::assemble on the GPU (sdpx_gpu_condensed_assemble_f64_dev):
+    p_csc = (colptr, rowval, nzval) or None; blocks = [(rinv n x n,
+    a_svec s x k, columns (k,))]; b_rows (r x m) and w (r,) the orthant
+    rows.  Returns the m x m upper triangle (lower part zero)."""
+    import torch
+
+    dev = lambda a, dt=torch.float64: torch.tensor(np.asarray(a), dtype=dt, device="cuda")
+    keep = []
+    bl = (PsdBlockF64 * max(1, len(blocks)))()
+    for i, (rinv, a_svec, cols) in enumerate(blocks):
+        n = rinv.shape[0]
+        r = dev(np.asfortranarray(rinv).T.reshape(-1))
+        a = dev(np.asfortranarray(a_svec).T.reshape(-1))
+        c = dev(np.asarray(cols, dtype=np.int64), torch.int64)
+        keep += [r, a, c]
+        bl[i] = PsdBlockF64(ctypes.sizeof(PsdBlockF64), n, len(cols), r.data_ptr(), a.data_ptr(), c.data_ptr())
+    S = torch.zeros(m * m, dtype=torch.float64, device="cuda")
+    pc = pr = pv = None
+    if p_csc is not None:
+        pc, pr, pv = dev(p_csc[0], torch.int64), dev(p_csc[1], torch.int64), dev(p_csc[2])
+    bb = ww = None
+    rows = 0
+    if b_rows is not None:
+        rows = b_rows.shape[0]
+        bb, ww = dev(np.asfortranarray(b_rows).T.reshape(-1)), dev(w)
+    ptr = lambda t: None if t is None else t.data_ptr()
+    torch.cuda.synchronize()
+    check(lib().sdpx_gpu_condensed_assemble_f64_dev(m, ptr(pc), ptr(pr), ptr(pv), len(blocks), bl, rows, ptr(bb),
+                                                    max(rows, 1), ptr(ww), S.data_ptr(), m))
+    return S.cpu().numpy().reshape(m, m).T.copy()
 
 
 class ExactBlock(ctypes.Structure):

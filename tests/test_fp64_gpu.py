@@ -146,3 +146,50 @@ def test_ipm_dense_sdp_n50_m100_cpu_and_gpu(gpu):
     assert np.linalg.norm(yg - yc) <= 1e-5 * (1 + np.linalg.norm(yc))
     # strict complementarity at the solution: <X, S> small against |C|
     assert np.sum(Xg * Sg) <= 1e-6 * (1 + np.linalg.norm(prob["C"]))
+
+
+def test_condensed_assembly_several_cones(gpu):
+    """P + three PSD blocks (uneven orders, overlapping column subsets) in
+    cone order + orthant rows' Gram, against the same sums in extended
+    precision, within 1e-12 relative (CondensedKKT::assemble,
+    kkt/condensed/mod.rs:1962-2147)."""
+    from sdpx_gpu import ipm
+
+    rng = np.random.default_rng(12)
+    m = 60
+    blocks, ref = [], np.zeros((m, m), dtype=np.longdouble)
+    # P: a sparse symmetric pattern, stored upper CSC
+    Pd = np.zeros((m, m))
+    for _ in range(80):
+        i, j = sorted(rng.integers(0, m, 2))
+        Pd[i, j] = Pd[j, i] = rng.standard_normal()
+    colptr, rowval, nzval = [0], [], []
+    for j in range(m):
+        for i in range(j + 1):
+            if Pd[i, j] != 0:
+                rowval.append(i)
+                nzval.append(Pd[i, j])
+        colptr.append(len(rowval))
+    ref += np.triu(Pd).astype(np.longdouble)
+    for n, k in ((8, 25), (15, 30), (5, 60)):
+        cols = np.sort(rng.choice(m, k, replace=False))
+        rinv = np.eye(n) + rng.standard_normal((n, n)) / (4 * np.sqrt(n))
+        A = [(lambda g: g + g.T)(rng.standard_normal((n, n))) for _ in range(k)]
+        a_svec = np.stack([ipm.svec(a) for a in A], axis=1)
+        blocks.append((rinv, a_svec, cols))
+        W = (rinv.T @ rinv).astype(np.longdouble)
+        for a in range(k):
+            WAW = W @ A[a].astype(np.longdouble) @ W
+            for b2 in range(k):
+                i, j = min(cols[a], cols[b2]), max(cols[a], cols[b2])
+                if a <= b2:
+                    ref[i, j] += np.sum(A[b2].astype(np.longdouble) * WAW)
+    r = 20
+    B = rng.standard_normal((r, m))
+    w = np.exp(rng.standard_normal(r))
+    Bl = B.astype(np.longdouble)
+    ref += np.triu(Bl.T @ (w[:, None].astype(np.longdouble) * Bl))
+    S = gpu.condensed_assemble_f64(m, (colptr, rowval, nzval), blocks, B, w)
+    scale = float(np.abs(ref).max())
+    err = np.abs(np.triu(S).astype(np.longdouble) - ref).max() / scale
+    assert float(err) <= 1e-12, float(err)
