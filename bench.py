@@ -35,6 +35,17 @@ def product_traffic(workload):
         return None
 
 
+def cpu_model():
+    """The host CPU's model name (from /proc/cpuinfo) or None."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def progress(msg):
     """A progress line on stderr (the JSON result stays alone on stdout)."""
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
@@ -128,7 +139,7 @@ def cpu_baseline_gemm(a, b, m, n, k, nl):
     t0 = time.perf_counter()
     oracle_entries("N", "N", k, a, m, b, k, rows, cols, threads=cores)
     dt = time.perf_counter() - t0
-    return {"value": dt * (m * n) / count, "unit": "s/GEMM", "cores": cores, "kind": "port",
+    return {"value": dt * (m * n) / count, "unit": "s/GEMM", "cores": cores, "kind": "port", "cpu": cpu_model(),
             "sample": f"{count} random outputs (exact {k}-term dots) of the same product, "
                       f"{dt:.2f} s, scaled by {m * n}/{count}"}
 
@@ -175,8 +186,10 @@ def sampled_workload(args):
     # one profiled pass for the kernel roofline (HIP events on the library stream)
     sdpx_gpu.set_profiling(True)
     sdpx_gpu.rns_totals(reset=True)
+    sdpx_gpu.panel_phase_times(reset=True)
     step()
     tot = sdpx_gpu.rns_totals(reset=True)
+    pms, plaunch = sdpx_gpu.panel_phase_times(reset=True)
     sdpx_gpu.set_profiling(False)
     # float64 on the same GPU, same problem (dynamic pivot rule: the 256-bit
     # leaves are singular in binary64)
@@ -192,6 +205,23 @@ def sampled_workload(args):
     fdt = (time.perf_counter() - f0) / fsteps
     product_s = tot["stage_ms"][2] / 1e3
     achieved = tot["int8_ops"] / product_s if product_s > 0 else 0.0
+    kmax = max(b["k"] for b in prob["blocks"])
+    chain = t + kmax  # sequential pivot columns: the border's, after the leaves' (one launch per leaf column)
+    phase_table = {
+        "ldl_dots": {"ms": pms["ldl_dots"], "bound": "latency: a chain of dependent 256-bit operations per "
+                     "pivot column", "columns": chain, "us_per_column": 1e3 * pms["ldl_dots"] / chain},
+        "trsm_base": {"ms": pms["trsm_base"], "bound": "latency: one dependent step per solved column",
+                      "launch_groups": plaunch["trsm_base"]},
+        "forward_rows": {"ms": pms["forward_rows"], "bound": "latency/VALU: K dependent exact-dot rows per "
+                         "column, all leaves and border columns in one launch", "steps": kmax,
+                         "us_per_step": 1e3 * pms["forward_rows"] / kmax},
+        "exact_products": {"ms": pms["exact_products"], "bound": "mfma (INT8) for the product stage, hbm for "
+                           "scan / encode / CRT", "stage_ms": dict(zip(("scan", "encode", "product", "crt"),
+                                                                       tot["stage_ms"])),
+                           "product_int8_tops": achieved / 1e12, "product_frac_of_int8_peak":
+                           achieved / INT8_PEAK_OPS},
+        "other_ms": dt * 1e3 - sum(pms.values()),
+    }
     result = {
         "value": dt, "unit": "s/IPM-iter", "ms_per_step": dt * 1e3,
         "dtype": "mp256 (int8 MFMA residues, int32 accumulation)",
@@ -205,6 +235,7 @@ def sampled_workload(args):
                    "rns_calls": tot["calls"], "mp_macs": tot["mp_macs"],
                    "fp64_gpu_s_per_iter": fdt, "fp64_phase_ms": ftimes,
                    "mp256_over_fp64": dt / fdt,
+                   "phase_table_ms": phase_table,
                    "host_entry_s_per_iter": host_dt},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": INT8_PEAK_OPS / 1e12,
                      "unit": "TOP/s (INT8)", "frac": achieved / INT8_PEAK_OPS,
@@ -215,6 +246,8 @@ def sampled_workload(args):
                              "kernels (ldl_col_wave_kernel, trsm_rl_kernel), which have no MFMA or "
                              "HBM roofline"},
     }
+    if args.target and t != 8192:
+        result["config"]["target_8192"] = sampled_target_point(args)
     if args.cpu_baseline and t <= 2000:  # larger borders: minutes of CPU time
         sys.path[:0] = [os.path.join(ROOT, "tests"), os.path.join(ROOT, "oracle")]
         from helpers import oracle_sampled_arrow
@@ -224,9 +257,47 @@ def sampled_workload(args):
         ok, *_ = oracle_sampled_arrow(prob, nl, threads=cores)
         cdt = time.perf_counter() - c0
         result["cpu_baseline"] = {"value": cdt, "unit": "s/IPM-iter", "cores": cores, "kind": "port",
+                                  "cpu": cpu_model(),
                                   "sample": "the whole workload (one iteration) through the C "
                                             "restatement with exact dots, OpenMP on all cores"}
     return result
+
+
+def sampled_target_point(args):
+    """The same workload at an 8192-row Schur border (the order the target
+    is stated at): seconds per iteration at 256 bits and of the Float64
+    form on the same GPU, and their ratio (2 timed iterations each)."""
+    import torch
+
+    import sdpx_gpu
+    from sdpx_gpu import workloads
+
+    nl = 4
+    progress("target point: building the 8192-row problem")
+    prob = workloads.sampled_problem(nl, seed=7, t=8192)
+    run = sdpx_gpu.SampledDev(prob, nl)
+    assert run.run()  # warmup
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    steps = 2
+    for _ in range(steps):
+        assert run.run()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    phases = sdpx_gpu.sampled_stats()
+    del run
+    dev = sdpx_gpu.SampledF64(prob)
+    dev.run(dynamic=(1e-13, 2e-7))
+    torch.cuda.synchronize()
+    f0 = time.perf_counter()
+    for _ in range(3):
+        dev.run(dynamic=(1e-13, 2e-7))
+    torch.cuda.synchronize()
+    fdt = (time.perf_counter() - f0) / 3
+    progress(f"target point: {dt:.3f} s MP256, {fdt * 1e3:.2f} ms Float64 per iteration")
+    return {"border": 8192, "mp256_s_per_iter": dt, "fp64_gpu_s_per_iter": fdt, "mp256_over_fp64": dt / fdt,
+            "phase_s": {k: phases[k] for k in ("build_s", "leaves_s", "border_s")},
+            "samples": int(sum(b["k"] for b in prob["blocks"]))}
 
 
 def chol_workload(args):
@@ -359,7 +430,7 @@ def chol_workload(args):
         oracle_ldl(a_sub, sub, threads=cores)
         cdt = time.perf_counter() - c0
         result["cpu_baseline"] = {"value": cdt * (n / sub) ** 3, "unit": "s/factor", "cores": cores,
-                                  "kind": "port",
+                                  "kind": "port", "cpu": cpu_model(),
                                   "sample": f"C restatement (exact dots) on the leading {sub}x{sub} "
                                             f"block ({cdt:.1f} s), scaled by (n/{sub})^3"}
     return result
@@ -493,6 +564,7 @@ def fp64_workload(args):
         np.linalg.cholesky(M)
         cdt = time.perf_counter() - c0
         result["cpu_baseline"] = {"value": cdt, "unit": "s/IPM-iter", "cores": cores, "kind": "port",
+                                  "cpu": cpu_model(),
                                   "sample": "the same iteration (n=200, m=1000) in numpy/LAPACK, the "
                                             "reference formula of kkt/condensed/psd.rs:1299-1456"}
     return result
@@ -509,6 +581,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="sampled", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
+    ap.add_argument("--no-target", dest="target", action="store_false",
+                    help="skip the 8192-row measurement of --workload sampled")
     ap.add_argument("--order", type=int, default=8192, help="matrix order of --workload chol")
     ap.add_argument("--border", type=int, default=2000,
                     help="border rows (free variables) of --workload sampled")

@@ -270,6 +270,14 @@ int32_t sdpx_gpu_sampled_arrow_dev(uint32_t limbs, uint64_t nleaves,
                                    void *dinv_all, void *y_all, void *border,
                                    const int8_t *border_signs, uint64_t nborder_signs,
                                    void *border_dinv, int32_t *ok);
+/* Measurement only (no reference counterpart): with profiling on
+ * (sdpx_gpu_set_profiling), the device LDL^T / sampled pipelines of this
+ * thread time their launches by kernel category; returns the sums in ms
+ * (and the timed launch groups) since the last reset: 0 ldl_dots panels
+ * (the pivot chains), 1 triangular-solve bases, 2 forward rows (Y = L^-1 B),
+ * 3 exact products (all RNS stages).  reset != 0 clears them. */
+int32_t sdpx_gpu_panel_phase_times(double *ms4, int64_t *launches4, int32_t reset);
+
 /* ArrowLDLSolver::solve_many (crates/solver/src/solver/kkt/ldl/arrow.rs:
  * 3793-3980) on the factors of sdpx_gpu_sampled_arrow_dev (device pointers
  * except samples, a host array): samples[i] is leaf i's order K_i; l_all, dinv_all,

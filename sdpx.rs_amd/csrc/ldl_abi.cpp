@@ -352,6 +352,13 @@ int32_t sdpx_gpu_sampled_arrow_solve_dev(uint32_t limbs, uint64_t nleaves, const
     });
 }
 
+int32_t sdpx_gpu_panel_phase_times(double *ms4, int64_t *launches4, int32_t reset) {
+    return sdpx::abi_boundary([&]() -> int32_t {
+        sdpx::panel_phase_times(ms4, launches4, reset != 0);
+        return SDPX_GPU_OK;
+    });
+}
+
 int32_t sdpx_gpu_sampled_stats(double *seconds3, int64_t *gpu_products, int64_t *host_products,
                                int64_t *declined, int64_t *retried, int64_t *failed_leaf) {
     if (seconds3) {

@@ -20,6 +20,35 @@
 namespace sdpx {
 namespace {
 
+// Kernel-category times of the device pipelines (measurement only): HIP
+// event pairs around the launches of each category while profiling is on,
+// summed when collected.  0 ldl_dots panels, 1 triangular-solve bases,
+// 2 forward rows (Y), 3 exact products (every RNS stage).
+struct PhaseLog {
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
+    double ms[4] = {0, 0, 0, 0};
+    int64_t launches[4] = {0, 0, 0, 0};
+};
+thread_local PhaseLog t_phase;
+
+struct PhaseMark {
+    int cat;
+    hipStream_t st;
+    hipEvent_t a = nullptr, b = nullptr;
+    PhaseMark(int c, hipStream_t s) : cat(c), st(s) {
+        if (!rns_profiling()) return;
+        HIP_CHECK(hipEventCreate(&a));
+        HIP_CHECK(hipEventCreate(&b));
+        HIP_CHECK(hipEventRecord(a, st));
+    }
+    ~PhaseMark() {
+        if (!a) return;
+        (void)hipEventRecord(b, st);
+        t_phase.marks.push_back({cat, {a, b}});
+        t_phase.launches[cat]++;
+    }
+};
+
 // Workspace slots of the device factor (the ABI's staging uses 72-79).
 constexpr int WS_LDL_ARENA = 70, WS_LDL_SCRATCH = 71, WS_LDL_SAVED = 80;
 
@@ -65,6 +94,7 @@ int32_t *upload_ints(DCtx &c, const std::vector<size_t> &v) {
 void product(DCtx &c, char ta, char tb, size_t m, size_t n, size_t k, const uint8_t *a, size_t lda,
              const uint8_t *b, size_t ldb, uint8_t *out, bool packed) {
     if (m == 0 || n == 0) return;
+    PhaseMark pm(3, c.st);
     RnsGemmArgs g{c.n, ta, tb, (int64_t)m, (int64_t)n, (int64_t)k, a, (int64_t)lda, b,
                   (int64_t)ldb, packed, packed ? n * (n + 1) / 2 : m * n, out};
     int r = rns_gemm_device(g);
@@ -91,6 +121,7 @@ void dots_device(DCtx &c, uint8_t *l, size_t ld, size_t n, const std::vector<siz
     upload_async(d_bd, &bd, sizeof bd);
     const size_t need = ldl_scratch_records(n) * c.vb;
     if (need > c.ldl_scratch_bytes) throw Error(SDPX_GPU_DEVICE_ERROR, "ldl scratch too small");
+    PhaseMark pm(0, c.st);
     mp_ldl_dots(c.n, l, dinv, d_first, c.d_signs, c.sign_mode, d_bd, 1, (int64_t)n, c.ldl_scratch, c.d_reg_eps,
                 c.d_reg_delta, c.d_status, c.st);
     c.arena->top = mark;
@@ -104,6 +135,7 @@ void solve_t_device(DCtx &c, size_t m, size_t n, uint8_t *L, size_t ldl, uint8_t
         int32_t *d_low = upload_ints(c, low);
         void *ranges = c.arena->alloc(8 * n);
         void *work = c.arena->alloc(4 * (m + 1));
+        PhaseMark pm(1, c.st);
         mp_exp_ranges(c.n, L, ldl, n, 0, ranges, c.st);
         mp_trsm_base(c.n, L, ldl, n, B, ldb, m, d_low, ranges, c.d_status + 2, work, c.st);
         c.arena->top = mark;
@@ -135,6 +167,7 @@ void solve_n_device(DCtx &c, size_t m, size_t n, uint8_t *L, size_t ldl, uint8_t
         // the ranges buffer is released at the caller's next arena reset,
         // after this launch in stream order
         void *ranges = c.arena->alloc(8 * n);
+        PhaseMark pm(1, c.st);
         mp_exp_ranges(c.n, L, ldl, n, 1, ranges, c.st);
         mp_trsm_base_n(c.n, L, ldl, n, B, ldb, m, ranges, c.d_status + 2, c.st);
         return;
@@ -342,6 +375,23 @@ int factor_loaded_device(DeviceSetup &ds, uint8_t *l, size_t ld, size_t n, uint8
 
 }  // namespace
 
+void panel_phase_times(double *ms, int64_t *launches, bool reset) {
+    for (auto &m : t_phase.marks) {
+        HIP_CHECK(hipEventSynchronize(m.second.second));
+        float x = 0;
+        HIP_CHECK(hipEventElapsedTime(&x, m.second.first, m.second.second));
+        t_phase.ms[m.first] += x;
+        (void)hipEventDestroy(m.second.first);
+        (void)hipEventDestroy(m.second.second);
+    }
+    t_phase.marks.clear();
+    for (int i = 0; i < 4; i++) {
+        if (ms) ms[i] = t_phase.ms[i];
+        if (launches) launches[i] = t_phase.launches[i];
+    }
+    if (reset) t_phase = PhaseLog{};
+}
+
 int ldl_factor_device(int limbs, uint8_t *d_l, size_t ld, size_t n, const int8_t *signs,
                       size_t nsigns, const void *reg_eps, const void *reg_delta, uint8_t *d_dinv,
                       LdlStats &stats, bool recursive) {
@@ -504,8 +554,11 @@ int sampled_arrow_device(int limbs, const std::vector<SampledLeafDev> &leaves, s
         const int8_t one = 1;
         upload_async(plus, &one, 1);
         HIP_CHECK(hipMemsetAsync(c.d_status, 0, sizeof(int) * 2, s));
-        mp_ldl_dots(limbs, d_l_all, d_dinv_all, d_first, plus, 1, d_bd, (int)bds.size(), (int64_t)kmax,
-                    c.ldl_scratch, nullptr, nullptr, c.d_status, s);
+        {
+            PhaseMark pm(0, s);
+            mp_ldl_dots(limbs, d_l_all, d_dinv_all, d_first, plus, 1, d_bd, (int)bds.size(), (int64_t)kmax,
+                        c.ldl_scratch, nullptr, nullptr, c.d_status, s);
+        }
         int status[4];
         ds.status(status);
         c.arena->top = mark;
@@ -533,8 +586,11 @@ int sampled_arrow_device(int limbs, const std::vector<SampledLeafDev> &leaves, s
             }
             FwdLeaf *d_fd = (FwdLeaf *)c.arena->alloc(sizeof(FwdLeaf) * fd.size());
             upload_async(d_fd, fd.data(), sizeof(FwdLeaf) * fd.size());
-            mp_forward_rows(limbs, d_l_all, d_fc, d_y_all, ranges, d_fd, (int)fd.size(), (int64_t)t,
-                            c.d_status + 2, s);
+            {
+                PhaseMark pm(2, s);
+                mp_forward_rows(limbs, d_l_all, d_fc, d_y_all, ranges, d_fd, (int)fd.size(), (int64_t)t,
+                                c.d_status + 2, s);
+            }
             uint8_t *Z_all = (uint8_t *)c.arena->alloc(std::max<size_t>(ksum, 1) * t * vb);
             for (size_t i = 0; i < leaves.size(); i++)
                 mp_ew_mul_diag(limbs, c.at(d_y_all, yoff[i]), leaves[i].k, c.at(d_dinv_all, doff[i]),

@@ -32,6 +32,11 @@ int sampled_arrow_device(int limbs, const std::vector<SampledLeafDev> &leaves, s
                          const int8_t *border_signs, size_t nborder_signs, uint8_t *d_border_dinv,
                          SampledStats &st);
 
+// Kernel-category times (ms) and event-pair counts of this thread's device
+// pipelines while profiling is on: 0 ldl_dots panels, 1 triangular-solve
+// bases, 2 forward rows, 3 exact products; reset clears them.
+void panel_phase_times(double *ms, int64_t *launches, bool reset);
+
 // ArrowLDLSolver::solve_many (arrow.rs:3793-3980) on the factors of
 // sampled_arrow_device: x (n x nrhs, ldx, n = sum k + t; leaf rows in leaf
 // order, then the border rows) holds b on entry, x on return.  1 ok, -1

@@ -39,6 +39,7 @@ struct RnsStats {
 
 // Record HIP events around the pipeline stages of later calls.
 void rns_set_profiling(bool on);
+bool rns_profiling();
 
 // Totals over this thread's profiled calls since the last reset: stage ms
 // (scan, encode, product, crt), algorithmic INT8 ops of the products, calls.

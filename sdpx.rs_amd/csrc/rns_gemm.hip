@@ -1407,6 +1407,7 @@ const RnsStats &rns_last_stats() { return t_stats; }
 thread_local RnsTotals t_totals;
 RnsTotals &rns_totals() { return t_totals; }
 void rns_set_profiling(bool on) { g_profile = on; }
+bool rns_profiling() { return g_profile; }
 
 // Device copies of the tables that depend only on the modulus plan (and the
 // encode window) or on the tile grid.  They are made once per device and

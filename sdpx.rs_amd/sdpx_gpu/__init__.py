@@ -96,6 +96,8 @@ def _declare(L):
                                          ctypes.POINTER(i32)]
     L.sdpx_gpu_sampled_arrow.restype = i32
     L.sdpx_gpu_sampled_stats.argtypes = [dp] + [ctypes.POINTER(ctypes.c_int64)] * 5
+    L.sdpx_gpu_panel_phase_times.argtypes = [vp, vp, i32]
+    L.sdpx_gpu_panel_phase_times.restype = i32
     L.sdpx_gpu_sampled_arrow_solve_dev.argtypes = [u32, u64, vp, u64, vp, vp, vp, vp, vp, u64, vp, u64]
     L.sdpx_gpu_sampled_arrow_solve_dev.restype = i32
     L.sdpx_gpu_sampled_stats.restype = i32
@@ -459,6 +461,16 @@ class SampledDev:
         """(border factor, border dinv) as host record arrays."""
         dt = mp.dtype(self.nl)
         return (self.border.cpu().numpy().view(dt).copy(), self.bdinv.cpu().numpy().view(dt).copy())
+
+
+def panel_phase_times(reset=False):
+    """Kernel-category times (ms) of this thread's device pipelines while
+    profiling is on: ldl_dots, trsm bases, forward rows, exact products."""
+    ms = (ctypes.c_double * 4)()
+    n = (ctypes.c_int64 * 4)()
+    check(lib().sdpx_gpu_panel_phase_times(ms, n, 1 if reset else 0))
+    names = ("ldl_dots", "trsm_base", "forward_rows", "exact_products")
+    return {k: ms[i] for i, k in enumerate(names)}, {k: n[i] for i, k in enumerate(names)}
 
 
 def sampled_stats():

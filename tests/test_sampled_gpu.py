@@ -103,3 +103,11 @@ def test_arrow_solve_matches_oracle(gpu, full):
     xo = oracle_sampled_arrow_solve(outs, t, border, bdinv, rhs, nrhs, threads=16)
     bad = ~mp.same_bits(x, xo)
     assert not bad.any(), f"{bad.sum()} of {x.size} solution entries differ"
+
+
+def test_border_4096_instance(gpu):
+    """A 4096-row border (the larger Schur order of the 8192-row target,
+    halved to keep the oracle's exact dots to about a minute) with 8
+    uneven leaves at 256 bits: bitwise the oracle, no decline."""
+    print("sampled t=4096", flush=True)
+    compare(workloads.sampled_problem(4, nblocks=8, hmin=10, hmax=60, t=4096, seed=29), 4, threads=16)
