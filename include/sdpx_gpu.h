@@ -146,6 +146,48 @@ typedef struct {
 int32_t sdpx_gpu_rns_gemm_blocks_upper(uint32_t limbs, uint64_t m, uint64_t nblocks,
                                        const sdpx_gpu_exact_block *blocks, void *out);
 
+/* ---- Exact multilinear forms -------------------------------------------
+ *
+ * The residue-BLAS quadratic forms behind XgemmScalar (blas/traits.rs:
+ * 259-296, blas/mpfr.rs:410-430): every output entry is the exact value of
+ * the form rounded once to nearest-even at 64N bits (128, 256 or 512 bits
+ * here).  Column-major operands; 2 = declined (the caller takes its own
+ * path; outputs may be partially written, as the reference allows).  The
+ * _dev forms take device pointers and run on the library stream. */
+
+/* diag_congruence (rns_blas.rs:2997-3195), xdiag_congruence_upper_exact:
+ * c[i + j*m] = (a^T diag(d) a)_ij for i <= j, a k x m (ld k), d k values;
+ * the strict lower triangle is left untouched. */
+int32_t sdpx_gpu_diag_congruence_upper_exact(uint32_t limbs, uint64_t m, uint64_t k, const void *a,
+                                             const void *d, void *c);
+/* Same on device pointers; upper_only = 0 also writes the lower triangle. */
+int32_t sdpx_gpu_diag_congruence_upper_exact_dev(uint32_t limbs, uint64_t m, uint64_t k, const void *a,
+                                                 const void *d, void *c, uint32_t upper_only);
+/* congruence (rns_blas.rs:3563-3790), xcongruence_exact: c (ldc = m) =
+ * op(a) x op(a)^T for x k x k (ld ldx), op(a) m x k; upper triangle only
+ * when upper_only (then the strict lower triangle is left untouched). */
+int32_t sdpx_gpu_congruence_exact(uint32_t limbs, uint8_t transa, uint64_t m, uint64_t k, const void *a,
+                                  uint64_t lda, const void *x, uint64_t ldx, void *c, uint32_t upper_only);
+int32_t sdpx_gpu_congruence_exact_dev(uint32_t limbs, uint8_t transa, uint64_t m, uint64_t k, const void *a,
+                                      uint64_t lda, const void *x, uint64_t ldx, void *c, uint32_t upper_only);
+/* svec_quadratic (rns_blas.rs:4032-4173), xsvec_quadratic_exact:
+ * out[k] = sum_{i <= j} c_ij x_t q_ik q_jk for k < kmax, x an svec (entry
+ * (i, j), i <= j, at t = j(j+1)/2 + i), c_ii = 1 and c_ij = *sqrt2 (one
+ * value) otherwise; q h x kmax (ld h), read as |q| when abs_q. */
+int32_t sdpx_gpu_svec_quadratic_exact(uint32_t limbs, uint64_t h, uint64_t kmax, const void *q, uint32_t abs_q,
+                                      const void *x, const void *sqrt2, void *out);
+int32_t sdpx_gpu_svec_quadratic_exact_dev(uint32_t limbs, uint64_t h, uint64_t kmax, const void *q,
+                                          uint32_t abs_q, const void *x, const void *sqrt2, void *out);
+/* symmetric_bilinear (rns_blas.rs:4175-4276), xsymmetric_bilinear_exact:
+ * out[p] = q_a^T X q_b for (a, b) = (pairs[2p], pairs[2p+1]) (host array,
+ * indices < columns, else 1), X symmetric h x h given as its column-packed
+ * upper triangle (no svec factors), q h x columns (ld h).  Declined for
+ * h = 0, as the reference's form. */
+int32_t sdpx_gpu_symmetric_bilinear_exact(uint32_t limbs, uint64_t h, uint64_t columns, const void *q,
+                                          const void *x, uint64_t npairs, const uint64_t *pairs, void *out);
+int32_t sdpx_gpu_symmetric_bilinear_exact_dev(uint32_t limbs, uint64_t h, uint64_t columns, const void *q,
+                                              const void *x, uint64_t npairs, const uint64_t *pairs, void *out);
+
 /* ---- Multiple-precision dense LDL^T -----------------------------------
  *
  * sdpx_gpu_mp_ldl replaces DenseLeaf::factor_leaf / factor_loaded of the

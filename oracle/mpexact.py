@@ -596,3 +596,153 @@ def reference_leaf_matrix(n, nl):
             a[i + j * n] = round_fraction(Fraction(v), nl)
     reg = (round_fraction(Fraction(1e-20), nl), from_int(1, nl))
     return a, signs, reg
+
+
+# ---------------------------------------------------------------------------
+# Exact multilinear forms (crates/solver/src/algebra/dense/blas/rns_blas.rs:
+# diag_congruence 2997-3195, congruence 3563-3790, svec_quadratic 4032-4173,
+# symmetric_bilinear 4175-4276; the XgemmScalar hooks at blas/traits.rs:
+# 259-296).  Every output is the exact value of the form rounded once.  Each
+# operand is carried as exact integers over its own common scale, the way
+# the reference's residue encoding sees it, so the sums below are exact.
+# ---------------------------------------------------------------------------
+
+def _integers(vals):
+    """(ints, scale) with vals[i] == ints[i] * 2**scale, or None where the
+    reference's exponent-range gate declines (a non-finite entry or a spread
+    above MAX_SPREAD, rns_blas.rs:620-632).  An all-zero list gives scale
+    None."""
+    r = _exponent_range(vals)
+    if r == "nonfinite":
+        return None
+    lo, hi = r
+    if lo is None:
+        return [0] * len(vals), None
+    if hi - lo > MAX_SPREAD:
+        return None
+    ints = []
+    for v in vals:
+        if v.is_zero():
+            ints.append(0)
+        else:
+            s, e = v.scaled()
+            ints.append(s << (e - lo))
+    return ints, lo
+
+
+def _rounded(total, scale, n):
+    if total == 0 or scale is None:
+        return zero(n)
+    return from_scaled_integer(total < 0, abs(total), scale, n)
+
+
+def diag_congruence_exact(m, k, a, d, upper_only=True):
+    """rns_blas.rs:2997-3195: (a^T diag(d) a)_ij, a k x m column-major
+    (ld k), d k values; entries i > j are None when upper_only."""
+    nl = (a[0] if a else d[0]).n if (a or d) else 1
+    ia, id_ = _integers(a[: k * m]), _integers(d[:k])
+    if ia is None or id_ is None:
+        return None
+    (ia, sa), (idv, sd) = ia, id_
+    scale = None if sa is None or sd is None else 2 * sa + sd
+    out = [None] * (m * m)
+    for j in range(m):
+        for i in range(j + 1 if upper_only else m):
+            total = sum(ia[t + i * k] * idv[t] * ia[t + j * k] for t in range(k))
+            out[i + j * m] = _rounded(total, scale, nl)
+    return out
+
+
+def congruence_exact(ta, m, k, a, lda, x, ldx, upper_only=False):
+    """rns_blas.rs:3563-3790: op(a) x op(a)^T (ldc = m), x k x k (ld ldx),
+    op(a) m x k; entries i > j are None when upper_only."""
+    ta = ta.upper()
+    rows, cols = (m, k) if ta == "N" else (k, m)
+    av = [a[i + j * lda] for j in range(cols) for i in range(rows)]
+    xv = [x[i + j * ldx] for j in range(k) for i in range(k)]
+    nl = (av + xv)[0].n if (av or xv) else 1
+    ia, ix = _integers(av), _integers(xv)
+    if ia is None or ix is None:
+        return None
+    (ia, sa), (ixv, sx) = ia, ix
+    scale = None if sa is None or sx is None else 2 * sa + sx
+
+    def opa(i, p):  # op(a)[i, p] over the view (rows x cols, ld rows)
+        return ia[i + p * rows] if ta == "N" else ia[p + i * rows]
+
+    # t[p][j] = (x op(a)^T)[p, j], exact
+    t = [[sum(ixv[p + q * k] * opa(j, q) for q in range(k)) for j in range(m)] for p in range(k)]
+    out = [None] * (m * m)
+    for j in range(m):
+        for i in range(j + 1 if upper_only else m):
+            out[i + j * m] = _rounded(sum(opa(i, p) * t[p][j] for p in range(k)), scale, nl)
+    return out
+
+
+def svec_quadratic_exact(h, kmax, q, abs_q, x, s):
+    """rns_blas.rs:4032-4173: v[k] = sum_{i <= j} c_ij x_t q_ik q_jk with
+    t = j(j+1)/2 + i, c_ii = 1, c_ij = s; q h x kmax (ld h), |q| when abs_q.
+    None where the reference declines: the operand gates above, and a weight
+    s with sigma = exponent - P above 0 or below -(P + 64)
+    (rns_blas.rs:4083-4085)."""
+    nl = s.n
+    tri = h * (h + 1) // 2
+    qv = [abs_mp(v) if abs_q else v for v in q[: h * kmax]]
+    iq, ix = _integers(qv), _integers(x[:tri])
+    if iq is None or ix is None:
+        return None
+    (iq, sq), (ixv, sx) = iq, ix
+    if sq is None or sx is None:
+        return [zero(nl)] * kmax
+    if s.is_zero():
+        s_int, sigma = 0, 0
+    else:
+        if not s.is_regular():
+            return None
+        s_int, sigma = s.scaled()
+        if sigma > 0 or -sigma > s.p + 64:
+            return None
+    out = []
+    for c in range(kmax):
+        total = 0
+        for j in range(h):
+            for i in range(j + 1):
+                w = (1 << -sigma) if i == j else s_int
+                total += w * ixv[j * (j + 1) // 2 + i] * iq[i + c * h] * iq[j + c * h]
+        out.append(_rounded(total, sigma + sx + 2 * sq, nl))
+    return out
+
+
+def symmetric_bilinear_exact(h, columns, q, x, pairs):
+    """rns_blas.rs:4175-4276: q_a^T X q_b for (a, b) in pairs, X symmetric
+    given as its column-packed upper triangle (no svec factors), q h x
+    columns (ld h).  None for h = 0 or a pair index >= columns, as there."""
+    if h == 0 or any(a >= columns or b >= columns for a, b in pairs):
+        return None
+    nl = q[0].n
+    tri = h * (h + 1) // 2
+    iq, ix = _integers(q[: h * columns]), _integers(x[:tri])
+    if iq is None or ix is None:
+        return None
+    (iq, sq), (ixv, sx) = iq, ix
+    if sq is None or sx is None:
+        return [zero(nl)] * len(pairs)
+
+    def xe(i, j):
+        i, j = min(i, j), max(i, j)
+        return ixv[j * (j + 1) // 2 + i]
+
+    out = []
+    for a, b in pairs:
+        total = 0
+        for i in range(h):
+            tb = sum(xe(i, j) * iq[j + b * h] for j in range(h))
+            total += iq[i + a * h] * tb
+        out.append(_rounded(total, sx + 2 * sq, nl))
+    return out
+
+
+def abs_mp(v):
+    if v.kind < 0:
+        return Mp(v.n, v.limbs, -v.kind, v.exponent)
+    return v

@@ -81,6 +81,19 @@ void mp_arrow_forward(int limbs, const void *L_all, const void *dinv_all, const 
 void mp_arrow_backward(int limbs, const void *L_all, const void *descs, int ndesc, int64_t nmax, int64_t cols,
                        void *X, int back, hipStream_t st);
 void mp_zero_values(int limbs, void *P, int64_t count, hipStream_t st);
+// hi = RN(d x), lo = d x - hi (exact) for an m x n block of X; d per row
+// (mode 0), per column (1) or a single value (2)
+void mp_two_product(int limbs, const void *X, int64_t ldx, const void *D, int mode, void *Hi, void *Lo, int64_t ldo,
+                    int64_t m, int64_t n, hipStream_t st);
+// svec_quadratic's weights from an svec x and the constant s (device):
+// Hi + Lo = diag(x_ii) + s triu(x) exactly (h x h each, zero below)
+void mp_quadratic_weights(int limbs, const void *P, int64_t h, const void *S, void *Hi, void *Lo, hipStream_t st);
+// the full symmetric h x h matrix of a column-packed upper triangle
+void mp_unpack_symmetric(int limbs, const void *P, int64_t h, void *out, hipStream_t st);
+void mp_abs_values(int limbs, void *X, int64_t count, hipStream_t st);
+// out[p] = G[ia[p] + ib[p] ldg] (ia, ib device)
+void mp_gather(int limbs, const void *G, int64_t ldg, const int64_t *ia, const int64_t *ib, int64_t count, void *out,
+               hipStream_t st);
 // X = RN(RN(Wm - S) D) rows i (D per row) of an m x n block
 void mp_arrow_second(int limbs, const void *Wm, int64_t ldw, const void *S, int64_t lds, const void *D, void *X,
                      int64_t ldx, int64_t m, int64_t n, hipStream_t st);

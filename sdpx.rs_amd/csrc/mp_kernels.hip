@@ -2093,6 +2093,90 @@ __global__ __launch_bounds__(1024) void arrow_backward_kernel(const uint32_t *L_
     }
 }
 
+// ---------------------------------------------------------------------------
+// Helpers of the exact multilinear forms (exact_forms.cpp)
+// ---------------------------------------------------------------------------
+// The exact product d x as hi + lo: hi = RN(d x), lo = x d - hi = -RN(hi -
+// d x) (exactly representable: the error of a rounded product), for an
+// m x n block of X (ldx) with d per row (mode 0), per column (1) or one
+// value (2); hi and lo m x n with ld ldo.
+template <int W>
+__global__ void two_product_kernel(const uint32_t *X, int64_t ldx, const uint32_t *D, int mode, uint32_t *Hi,
+                                   uint32_t *Lo, int64_t ldo, int64_t m, int64_t n) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m * n) return;
+    const int64_t i = e % m, j = e / m;
+    Val<W> x, d, hi, r;
+    load(rec_at<W>(X, i + j * ldx), x);
+    load(rec_at<W>(D, mode == 0 ? i : (mode == 1 ? j : 0)), d);
+    mul(d, x, hi);
+    fms(hi, d, x, r);  // RN(hi - d x), exact
+    if (r.kind == mpd::KIND_REG || r.kind == -mpd::KIND_REG) r.kind = -r.kind;
+    store(rec_at<W>(Hi, i + j * ldo), hi);
+    store(rec_at<W>(Lo, i + j * ldo), r);
+}
+
+// out (h x h, ld h) = the symmetric matrix of a column-packed upper triangle
+// (entry (i, j), i <= j, at j (j + 1) / 2 + i), times `scale` off the
+// diagonal when scale is given (an exact power of two or one value)
+template <int W>
+__global__ void unpack_symmetric_kernel(const uint32_t *P, int64_t h, uint32_t *out) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= h * h) return;
+    const int64_t i = e % h, j = e / h;
+    const int64_t a = i < j ? i : j, b = i < j ? j : i;
+    Val<W> v;
+    load(rec_at<W>(P, b * (b + 1) / 2 + a), v);
+    store(rec_at<W>(out, e), v);
+}
+
+// The upper-triangular weight matrix of svec_quadratic as two exact parts:
+// from the svec x (entry (i, j), i <= j, at j (j + 1) / 2 + i), M_ii = x_ii
+// (lo 0) and, for i < j, s x_ij = hi + lo exactly; zero below the diagonal
+template <int W>
+__global__ void quadratic_weights_kernel(const uint32_t *P, int64_t h, const uint32_t *S, uint32_t *Hi, uint32_t *Lo) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= h * h) return;
+    const int64_t i = e % h, j = e / h;
+    Val<W> hi, lo;
+    set_zero(hi);
+    set_zero(lo);
+    if (i == j) {
+        load(rec_at<W>(P, j * (j + 1) / 2 + i), hi);
+    } else if (i < j) {
+        Val<W> x, s, r;
+        load(rec_at<W>(P, j * (j + 1) / 2 + i), x);
+        load(S, s);
+        mul(s, x, hi);
+        fms(hi, s, x, r);
+        if (r.kind == mpd::KIND_REG || r.kind == -mpd::KIND_REG) r.kind = -r.kind;
+        lo = r;
+    }
+    store(rec_at<W>(Hi, e), hi);
+    store(rec_at<W>(Lo, e), lo);
+}
+
+// |x| in place (count values)
+template <int W>
+__global__ void abs_values_kernel(uint32_t *X, int64_t count) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= count) return;
+    uint32_t *k = rec_at<W>(X, e) + W;
+    const int32_t kind = (int32_t)*k;
+    if (kind < 0) *k = (uint32_t)(-kind);
+}
+
+// out[p] = G[ia[p] + ib[p] ldg]
+template <int W>
+__global__ void gather_kernel(const uint32_t *G, int64_t ldg, const int64_t *ia, const int64_t *ib, int64_t count,
+                              uint32_t *out) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= count) return;
+    Val<W> v;
+    load(rec_at<W>(G, ia[p] + ib[p] * ldg), v);
+    store(rec_at<W>(out, p), v);
+}
+
 // count zero values
 template <int W>
 __global__ void zero_values_kernel(uint32_t *P, int64_t count) {
@@ -2468,6 +2552,45 @@ void mp_arrow_backward(int limbs, const void *L_all, const void *descs, int ndes
     SDPX_DISPATCH_W(limbs, SDPX_DISPATCH_R(R, hipLaunchKernelGGL((arrow_backward_kernel<W, R>),
         dim3((unsigned)cols, ndesc), dim3(threads), lds, st, (const uint32_t *)L_all, (const SweepDesc *)descs,
         (uint32_t *)X, back)));
+    HIP_CHECK(hipGetLastError());
+}
+
+void mp_two_product(int limbs, const void *X, int64_t ldx, const void *D, int mode, void *Hi, void *Lo, int64_t ldo,
+                    int64_t m, int64_t n, hipStream_t st) {
+    if (m * n == 0) return;
+    SDPX_DISPATCH_W(limbs, hipLaunchKernelGGL(two_product_kernel<W>, dim3(blocks_for(m * n, 256)), dim3(256), 0, st,
+                                              (const uint32_t *)X, ldx, (const uint32_t *)D, mode, (uint32_t *)Hi,
+                                              (uint32_t *)Lo, ldo, m, n));
+    HIP_CHECK(hipGetLastError());
+}
+
+void mp_quadratic_weights(int limbs, const void *P, int64_t h, const void *S, void *Hi, void *Lo, hipStream_t st) {
+    if (h == 0) return;
+    SDPX_DISPATCH_W(limbs, hipLaunchKernelGGL(quadratic_weights_kernel<W>, dim3(blocks_for(h * h, 256)), dim3(256), 0,
+                                              st, (const uint32_t *)P, h, (const uint32_t *)S, (uint32_t *)Hi,
+                                              (uint32_t *)Lo));
+    HIP_CHECK(hipGetLastError());
+}
+
+void mp_unpack_symmetric(int limbs, const void *P, int64_t h, void *out, hipStream_t st) {
+    if (h == 0) return;
+    SDPX_DISPATCH_W(limbs, hipLaunchKernelGGL(unpack_symmetric_kernel<W>, dim3(blocks_for(h * h, 256)), dim3(256), 0,
+                                              st, (const uint32_t *)P, h, (uint32_t *)out));
+    HIP_CHECK(hipGetLastError());
+}
+
+void mp_abs_values(int limbs, void *X, int64_t count, hipStream_t st) {
+    if (count == 0) return;
+    SDPX_DISPATCH_W(limbs, hipLaunchKernelGGL(abs_values_kernel<W>, dim3(blocks_for(count, 256)), dim3(256), 0, st,
+                                              (uint32_t *)X, count));
+    HIP_CHECK(hipGetLastError());
+}
+
+void mp_gather(int limbs, const void *G, int64_t ldg, const int64_t *ia, const int64_t *ib, int64_t count, void *out,
+               hipStream_t st) {
+    if (count == 0) return;
+    SDPX_DISPATCH_W(limbs, hipLaunchKernelGGL(gather_kernel<W>, dim3(blocks_for(count, 256)), dim3(256), 0, st,
+                                              (const uint32_t *)G, ldg, ia, ib, count, (uint32_t *)out));
     HIP_CHECK(hipGetLastError());
 }
 

@@ -19,6 +19,14 @@ struct RnsGemmArgs {
     // the library's workspace); more than scratch_bytes is a capacity error
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
+    // expansion output (split_capacity > 0): instead of the rounded product,
+    // chunk c (c < *split_used <= split_capacity) of every entry's exact value
+    // at out + c m n records (full layout, ldc = m; upper entries only when
+    // upper), each chunk P bits of the exact integer taken from the top, so
+    // the chunks sum to the exact product; declined when more chunks would be
+    // needed or the plan takes the VALU CRT
+    int split_capacity = 0;
+    int *split_used = nullptr;
 };
 
 // Size of the per-device array of the allocation-free exact product.

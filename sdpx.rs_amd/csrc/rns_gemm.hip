@@ -886,7 +886,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void c
     int tiles_m, int tiles_n, const int8_t *__restrict__ table, int nt, int npad,
     const uint32_t *__restrict__ mlimbs, int nu, double m_top, const int4 *__restrict__ scan_a,
     const int4 *__restrict__ scan_b, int64_t m, int64_t nn, int upper, int packed, int n_arg,
-    uint32_t *__restrict__ out) {
+    uint32_t *__restrict__ out, int chunks, int64_t chunk_stride) {
     const int n = NL ? NL : n_arg;  // output limbs (a constant in the common widths)
     extern __shared__ __attribute__((aligned(16))) int8_t csm[];
     const int trow = 2 * lpad + 16;  // bytes per table row
@@ -1029,13 +1029,58 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void c
         if (!active) continue;
         uint32_t *rec = out + (size_t)(packed ? j * (j + 1) / 2 + i : i + j * m) * (2 * n + 2);
         if (top < 0) {
-            for (int w = 0; w < 2 * n; w++) rec[w] = 0;
-            rec[2 * n] = KIND_ZERO;
-            rec[2 * n + 1] = 0;
+            for (int cidx = 0; cidx < (chunks > 0 ? chunks : 1); cidx++) {
+                uint32_t *crec = chunks > 0 ? out + ((size_t)cidx * chunk_stride + (size_t)(i + j * m)) * (2 * n + 2)
+                                            : rec;
+                for (int w = 0; w < 2 * n; w++) crec[w] = 0;
+                crec[2 * n] = KIND_ZERO;
+                crec[2 * n + 1] = 0;
+            }
             continue;
         }
         auto word = [&](int u) -> uint32_t { return u >= 0 && u < NUMAX ? sl[u * 128 + tid] : 0u; };
         const int bits = 32 * top + 32 - __builtin_clz(tw);
+        if (chunks > 0) {
+            // expansion form: chunk c holds bits [bits - (c + 1) P, bits - c P)
+            // of the magnitude, exactly (no rounding), each normalized on its
+            // own; the chunks sum to the exact value
+            for (int cidx = 0; cidx < chunks; cidx++) {
+                uint32_t *crec = out + ((size_t)cidx * chunk_stride + (size_t)(i + j * m)) * (2 * n + 2);
+                const int sc = bits - (cidx + 1) * P;
+                const int q = sc >> 5, rs = sc & 31;
+                uint32_t mc[CRT_MAXW];
+                int hw = -1;
+#pragma unroll
+                for (int w = 0; w < CRT_MAXW; w++) {
+                    mc[w] = w < 2 * n ? __builtin_amdgcn_alignbit(word(q + w + 1), word(q + w), rs) : 0u;
+                    if (w < 2 * n && mc[w] != 0) hw = w;
+                }
+                if (hw < 0) {
+                    for (int w = 0; w < 2 * n; w++) crec[w] = 0;
+                    crec[2 * n] = KIND_ZERO;
+                    crec[2 * n + 1] = 0;
+                    continue;
+                }
+                const int lz = 32 * (2 * n - 1 - hw) + __builtin_clz(mc[hw]);
+                const int ws = lz >> 5, bs = lz & 31;
+                const int64_t expo = (int64_t)scan_a[i].x + scan_b[j].x + sc - lz + P;
+                if (expo > (int64_t)((1 << 30) - 1) || expo < (int64_t)(1 - (1 << 30))) {
+                    const bool inf = expo > 0;
+                    for (int w = 0; w < 2 * n; w++) crec[w] = 0;
+                    crec[2 * n] = (uint32_t)(inf ? (negative ? -KIND_INF : KIND_INF) : KIND_ZERO);
+                    crec[2 * n + 1] = 0;
+                    continue;
+                }
+                for (int w = 2 * n - 1; w >= 0; w--) {
+                    const int src = w - ws;
+                    const uint32_t hi = src >= 0 ? mc[src] : 0u, lo = src >= 1 ? mc[src - 1] : 0u;
+                    crec[w] = bs ? (hi << bs) | (lo >> (32 - bs)) : hi;
+                }
+                crec[2 * n] = (uint32_t)(negative ? -KIND_REGULAR : KIND_REGULAR);
+                crec[2 * n + 1] = (uint32_t)(int32_t)expo;
+            }
+            continue;
+        }
         int shift = bits - P;
         bool roundb = false, sticky = false;
         if (shift > 0) {
@@ -1552,6 +1597,14 @@ struct Buffers {
 };
 }  // namespace
 
+// Whether a plan's CRT runs on the matrix cores (crt_mfma_kernel): the MFMA
+// form holds every k-slice's fragments, at most 2 per 32-row table block.
+bool crt_matrix_core(const RnsPlan &plan, int n) {
+    const int nbl = plan.crt_npad / 32;
+    return g_crt_mfma && plan.crt_nu <= 40 && nbl <= 5 && n <= 8 &&
+           plan.crt_lpad / 16 <= 2 * (plan.crt_nu <= 20 && nbl <= 3 ? 3 : (plan.crt_nu <= 32 && nbl <= 4 ? 4 : 5));
+}
+
 int rns_gemm_device(const RnsGemmArgs &g) {
     Device &dev = device();
     hipStream_t s = dev.stream;
@@ -1593,6 +1646,7 @@ int rns_gemm_device(const RnsGemmArgs &g) {
         HIP_CHECK(hipStreamSynchronize(s));
     };
     if (k == 0) {
+        if (g.split_capacity > 0 && g.split_used) *g.split_used = 1;
         zero_fill();
         return SDPX_GPU_OK;
     }
@@ -1621,6 +1675,7 @@ int rns_gemm_device(const RnsGemmArgs &g) {
     if ((wa.any && wa.emax - wa.emin > MAX_SPREAD) || (wb.any && wb.emax - wb.emin > MAX_SPREAD))
         return SDPX_GPU_DECLINED;
     if (!wa.any || !wb.any) {
+        if (g.split_capacity > 0 && g.split_used) *g.split_used = 1;
         zero_fill();
         return SDPX_GPU_OK;
     }
@@ -1630,6 +1685,13 @@ int rns_gemm_device(const RnsGemmArgs &g) {
     if (!plan) return SDPX_GPU_DECLINED;
     if (plan->limbs > 128) return SDPX_GPU_DECLINED;
     const int L = plan->count, G = plan->planes, single = plan->single;
+    // expansion output: enough P-bit chunks for the largest exact value
+    int split = 0;
+    if (g.split_capacity > 0) {
+        split = (int)std::ceil((plan->bits + 2.0) / (64.0 * n)) + 1;
+        if (split > g.split_capacity || !crt_matrix_core(*plan, n)) return SDPX_GPU_DECLINED;
+        if (g.split_used) *g.split_used = split;
+    }
     t_stats.moduli = L;
     t_stats.planes = G;
     t_stats.bits = plan->bits;
@@ -1774,9 +1836,7 @@ int rns_gemm_device(const RnsGemmArgs &g) {
     const int up = g.upper ? 1 : 0, pk = packed ? 1 : 0;
     uint32_t *outw = (uint32_t *)g.out;
     const int nbl = plan->crt_npad / 32;
-    // the MFMA form holds every k-slice's fragments: at most 2 per 32-row table block
-    const bool crt_mma = g_crt_mfma && plan->crt_nu <= 40 && nbl <= 5 && n <= 8 &&
-                         plan->crt_lpad / 16 <= 2 * (plan->crt_nu <= 20 && nbl <= 3 ? 3 : (plan->crt_nu <= 32 && nbl <= 4 ? 4 : 5));
+    const bool crt_mma = crt_matrix_core(*plan, n);
     if (crt_mma) {
         const int lpad = plan->crt_lpad, npad = plan->crt_npad, nu = plan->crt_nu;
         const size_t tb = plan->crt_table.size();
@@ -1793,7 +1853,8 @@ int rns_gemm_device(const RnsGemmArgs &g) {
             hipLaunchKernelGGL(kern, dim3(grid), dim3(128), lds, s, (const uint16_t *)X, L, lpad,
                                (const int2 *)d_tiles, ntiles, tiles_m, tiles_n, (const int8_t *)d_ct,
                                plan->crt_nt, npad, (const uint32_t *)(d_ct + tb), nu, plan->m_top,
-                               (const int4 *)scan_a, (const int4 *)scan_b, m, nn, up, pk, n, outw);
+                               (const int4 *)scan_a, (const int4 *)scan_b, m, nn, up, pk, n, outw, split,
+                               (int64_t)(m * nn));
         };
         auto pick = [&](auto k2, auto k4, auto k8, auto kn) {
             if (n == 2) launch(k2);

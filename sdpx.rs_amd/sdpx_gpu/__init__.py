@@ -136,6 +136,19 @@ def _declare(L):
         f = getattr(L, name)
         f.argtypes = gemm_args
         f.restype = i32
+    L.sdpx_gpu_diag_congruence_upper_exact.argtypes = [u32, u64, u64, vp, vp, vp]
+    L.sdpx_gpu_diag_congruence_upper_exact_dev.argtypes = [u32, u64, u64, vp, vp, vp, u32]
+    for name in ("sdpx_gpu_congruence_exact", "sdpx_gpu_congruence_exact_dev"):
+        getattr(L, name).argtypes = [u32, u8, u64, u64, vp, u64, vp, u64, vp, u32]
+    for name in ("sdpx_gpu_svec_quadratic_exact", "sdpx_gpu_svec_quadratic_exact_dev"):
+        getattr(L, name).argtypes = [u32, u64, u64, vp, u32, vp, vp, vp]
+    for name in ("sdpx_gpu_symmetric_bilinear_exact", "sdpx_gpu_symmetric_bilinear_exact_dev"):
+        getattr(L, name).argtypes = [u32, u64, u64, vp, vp, u64, vp, vp]
+    for name in ("sdpx_gpu_diag_congruence_upper_exact", "sdpx_gpu_diag_congruence_upper_exact_dev",
+                 "sdpx_gpu_congruence_exact", "sdpx_gpu_congruence_exact_dev", "sdpx_gpu_svec_quadratic_exact",
+                 "sdpx_gpu_svec_quadratic_exact_dev", "sdpx_gpu_symmetric_bilinear_exact",
+                 "sdpx_gpu_symmetric_bilinear_exact_dev"):
+        getattr(L, name).restype = i32
 
 
 def last_error():
@@ -622,6 +635,47 @@ def xgemm_blocks_upper_exact(m, blocks, out):
         arr[i] = ExactBlock(d.size, cols.size, _ptr(y), _ptr(d), _ptr(cols))
     nl = mp.limbs_of(out)
     code = lib().sdpx_gpu_rns_gemm_blocks_upper(nl, m, len(blocks), arr, _ptr(out))
+    check(code, allow_decline=True)
+    return code == OK
+
+
+def diag_congruence_upper_exact(m, k, a, d, c):
+    """XgemmScalar::diag_congruence_upper_exact (traits.rs:259-267,
+    rns_blas.rs:2997-3195): the upper triangle of a^T diag(d) a (a k x m,
+    ld k) into c[i + j m], each entry rounded once.  False on a decline."""
+    code = lib().sdpx_gpu_diag_congruence_upper_exact(mp.limbs_of(c), m, k, _ptr(a), _ptr(d), _ptr(c))
+    check(code, allow_decline=True)
+    return code == OK
+
+
+def xcongruence_exact(transa, m, k, a, lda, x, ldx, c, upper_only):
+    """XgemmScalar::xcongruence_exact (traits.rs:290-296, rns_blas.rs:
+    3563-3790): op(a) x op(a)^T into c (ldc = m), upper triangle only when
+    upper_only.  False on a decline."""
+    code = lib().sdpx_gpu_congruence_exact(mp.limbs_of(c), ord(transa), m, k, _ptr(a), lda, _ptr(x), ldx,
+                                           _ptr(c), 1 if upper_only else 0)
+    check(code, allow_decline=True)
+    return code == OK
+
+
+def xsvec_quadratic_exact(h, kmax, q, abs_q, x, sqrt2, out):
+    """XgemmScalar::xsvec_quadratic_exact (traits.rs:272-282, rns_blas.rs:
+    4032-4173): out[k] = sum_{i<=j} c_ij x_t q_ik q_jk (svec x, c_ij = sqrt2
+    off the diagonal; |q| when abs_q).  sqrt2 is a one-value record array.
+    False on a decline."""
+    code = lib().sdpx_gpu_svec_quadratic_exact(mp.limbs_of(out), h, kmax, _ptr(q), 1 if abs_q else 0, _ptr(x),
+                                               _ptr(sqrt2), _ptr(out))
+    check(code, allow_decline=True)
+    return code == OK
+
+
+def xsymmetric_bilinear_exact(h, columns, q, x, pairs, out):
+    """XgemmScalar::xsymmetric_bilinear_exact (traits.rs:283-289, rns_blas.rs:
+    4175-4276): out[p] = q_a^T X q_b for pairs[p] = (a, b), X symmetric as
+    its packed upper triangle.  False on a decline."""
+    pr = np.ascontiguousarray(np.asarray(pairs, dtype=np.uint64).reshape(-1))
+    code = lib().sdpx_gpu_symmetric_bilinear_exact(mp.limbs_of(out), h, columns, _ptr(q), _ptr(x),
+                                                   len(pairs), _ptr(pr), _ptr(out))
     check(code, allow_decline=True)
     return code == OK
 
