@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define SDPX_GPU_ABI_VERSION 2u
+#define SDPX_GPU_ABI_VERSION 3u
 #define SDPX_GPU_OK 0
 #define SDPX_GPU_INVALID 1
 #define SDPX_GPU_DECLINED 2
@@ -138,6 +138,7 @@ int32_t sdpx_gpu_rns_gemm_dev_noalloc(uint32_t limbs, uint8_t transa, uint8_t tr
  * blocks rounded once.  y is column-major rows x ncols.  Returns 2 on a
  * decline (outputs may be partially written, as the reference allows). */
 typedef struct {
+    uint64_t struct_size;       /* sizeof(sdpx_gpu_exact_block) */
     uint64_t rows, ncols;
     const void *y;              /* rows x ncols values */
     const void *d;              /* rows values */
@@ -145,6 +146,33 @@ typedef struct {
 } sdpx_gpu_exact_block;
 int32_t sdpx_gpu_rns_gemm_blocks_upper(uint32_t limbs, uint64_t m, uint64_t nblocks,
                                        const sdpx_gpu_exact_block *blocks, void *out);
+/* Same with each block's y and d and the output on the device (columns stay
+ * host arrays); 128, 256 and 512 bits.  The blocks are scattered into HBM
+ * and summed by one exact product launch. */
+int32_t sdpx_gpu_rns_gemm_blocks_upper_dev(uint32_t limbs, uint64_t m, uint64_t nblocks,
+                                           const sdpx_gpu_exact_block *blocks, void *out);
+
+/* General products with scaling, replacing the MPFR XgemmScalar::xgemm and
+ * XsyrkScalar::xsyrk (blas/mpfr.rs:245-348 gemm, 540-626 syrk):
+ *   xgemm: c (ldc) = axpby(alpha, RN(op(a) op(b)), beta, c)
+ *   xsyrk: the uplo triangle of c = axpby(alpha, RN(op(a) op(a)^T), beta, c)
+ *          (op(a) n x k; c column-packed upper when uplo = 'U', ldc = n and
+ *          c_len = n(n+1)/2, else ldc >= n and the other triangle untouched)
+ * with the reference's axpby (mpfr.rs:61-80): p = 0, x, -x or RN(alpha x)
+ * for alpha 0, 1, -1 or other; then p, RN(p + c), RN(p - c) or
+ * RN(p + RN(beta c)) for beta 0, 1, -1 or other.  alpha = 0 computes no
+ * product.  alpha and beta are one host value each, also in the _dev forms.
+ * Returns 2 (declined, c untouched) where the exact product declines. */
+int32_t sdpx_gpu_xgemm(uint32_t limbs, uint8_t transa, uint8_t transb, uint64_t m, uint64_t n, uint64_t k,
+                       const void *alpha, const void *a, uint64_t lda, const void *b, uint64_t ldb, const void *beta,
+                       void *c, uint64_t ldc);
+int32_t sdpx_gpu_xgemm_dev(uint32_t limbs, uint8_t transa, uint8_t transb, uint64_t m, uint64_t n, uint64_t k,
+                           const void *alpha, const void *a, uint64_t lda, const void *b, uint64_t ldb,
+                           const void *beta, void *c, uint64_t ldc);
+int32_t sdpx_gpu_xsyrk(uint32_t limbs, uint8_t uplo, uint8_t trans, uint64_t n, uint64_t k, const void *alpha,
+                       const void *a, uint64_t lda, const void *beta, void *c, uint64_t ldc, uint64_t c_len);
+int32_t sdpx_gpu_xsyrk_dev(uint32_t limbs, uint8_t uplo, uint8_t trans, uint64_t n, uint64_t k, const void *alpha,
+                           const void *a, uint64_t lda, const void *beta, void *c, uint64_t ldc, uint64_t c_len);
 
 /* ---- Exact multilinear forms -------------------------------------------
  *
@@ -289,6 +317,7 @@ int32_t sdpx_gpu_mp_ldl_stats(int64_t *gpu_products, int64_t *host_products, int
  * arrow.rs:819; Leaf::refactor 1596-1632), which this pipeline does not.
  * The border factor retries by exact dots as sdpx_gpu_mp_ldl does. */
 typedef struct {
+    uint64_t struct_size;        /* sizeof(sdpx_gpu_sampled_leaf) */
     uint64_t side, samples;      /* h (basis rows), k (samples) */
     const void *rinv;            /* h x h */
     const void *basis;           /* h x k */
@@ -296,6 +325,7 @@ typedef struct {
     const void *coupling;        /* k x t */
 } sdpx_gpu_sampled_leaf;
 typedef struct {
+    uint64_t struct_size;        /* sizeof(sdpx_gpu_sampled_factor) */
     void *l, *dinv, *y;          /* outputs, each optional */
 } sdpx_gpu_sampled_factor;
 int32_t sdpx_gpu_sampled_arrow(uint32_t limbs, uint64_t nleaves, const sdpx_gpu_sampled_leaf *leaves,
@@ -358,6 +388,7 @@ int32_t sdpx_gpu_sampled_stats(double *seconds3, int64_t *gpu_products, int64_t 
  * info: 0, or the failing pivot column + 1 of the last failing factor.
  * times_ms (optional, 3): build, leaves, border. */
 typedef struct {
+    uint64_t struct_size;        /* sizeof(sdpx_gpu_sampled_leaf_f64) */
     uint64_t side, samples;
     const double *rinv, *basis, *weights, *coupling;  /* device */
 } sdpx_gpu_sampled_leaf_f64;

@@ -746,3 +746,31 @@ def abs_mp(v):
     if v.kind < 0:
         return Mp(v.n, v.limbs, -v.kind, v.exponent)
     return v
+
+
+def scalar_code(v):
+    """The reference's scalar tests in axpby and gemm (F == 0, F == 1,
+    F == -1, blas/mpfr.rs:61-80): 0, 1, 2, else 3."""
+    if v.is_zero():
+        return 0
+    if v.is_regular():
+        f = v.to_fraction()
+        if f == 1:
+            return 1
+        if f == -1:
+            return 2
+    return 3
+
+
+def axpby(alpha, x, beta, y):
+    """blas/mpfr.rs:61-80: p = 0, x, -x or RN(alpha x); then p, RN(p + y),
+    RN(p - y) or RN(p + RN(beta y))."""
+    a, b = scalar_code(alpha), scalar_code(beta)
+    p = zero(x.n) if a == 0 else x if a == 1 else neg(x) if a == 2 else mul(alpha, x)
+    if b == 0:
+        return p
+    if b == 1:
+        return add(p, y)
+    if b == 2:
+        return sub(p, y)
+    return add(p, mul(beta, y))

@@ -40,6 +40,9 @@ int32_t invalid(const char *what) {
 }  // namespace
 
 namespace sdpx {
+int32_t check_exact_blocks(uint64_t m, uint64_t nblocks, const sdpx_gpu_exact_block *blocks, uint64_t &K);
+int32_t blocks_upper_device(int limbs, uint64_t m, uint64_t nblocks, const sdpx_gpu_exact_block *blocks, void *out,
+                            bool dev);
 int32_t abi_boundary(const std::function<int32_t()> &f) { return boundary(f); }
 int32_t abi_invalid(const char *what) { return invalid(what); }
 void set_last_error(const char *what) { t_error = what; }
@@ -211,20 +214,14 @@ int32_t sdpx_gpu_rns_gemm_blocks_upper(uint32_t limbs, uint64_t m, uint64_t nblo
                                        const sdpx_gpu_exact_block *blocks, void *out) {
     return boundary([&]() -> int32_t {
         if (limbs < 1 || limbs > (uint32_t)sdpx::MAX_LIMBS) return invalid("limbs must be 1..16");
-        if (nblocks && !blocks) return invalid("null blocks");
         const size_t vb = sdpx::value_bytes((int)limbs);
         uint64_t K = 0;
-        for (uint64_t b = 0; b < nblocks; b++) {
-            const auto &bl = blocks[b];
-            if (bl.rows && (!bl.y || !bl.d || (bl.ncols && !bl.columns))) return invalid("null block array");
-            for (uint64_t c = 0; c < bl.ncols; c++) {
-                if (bl.columns[c] >= m) return invalid("block column out of range");
-                if (c && bl.columns[c] <= bl.columns[c - 1]) return invalid("block columns must ascend");
-            }
-            K += bl.rows;
-        }
+        if (int32_t r = sdpx::check_exact_blocks(m, nblocks, blocks, K)) return r;
         if (m == 0) return SDPX_GPU_OK;
-        // scatter every block into dense K x m operands Z (= RN(diag(d) y)) and Y;
+        if (!out) return invalid("null output");
+        if (sdpx::mp_kernels_support((int)limbs))
+            return sdpx::blocks_upper_device((int)limbs, m, nblocks, blocks, out, false);
+        // (other widths) scatter every block into dense K x m operands Z (= RN(diag(d) y)) and Y;
         // columns a block does not touch are exact zeros, which leave every
         // exact sum unchanged (rns_blas.rs:2671-2922 sums the blocks exactly
         // and rounds once)

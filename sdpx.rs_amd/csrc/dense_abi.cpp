@@ -220,6 +220,9 @@ int32_t sdpx_gpu_sampled_arrow_f64_dev(uint64_t nleaves, const sdpx_gpu_sampled_
     return abi_boundary([&]() -> int32_t {
         if (nleaves && !leaves) return abi_invalid("null leaves");
         if (t && (!c || !s || !border_factor)) return abi_invalid("null border");
+        for (uint64_t i = 0; i < nleaves; i++)
+            if (leaves[i].struct_size != sizeof(sdpx_gpu_sampled_leaf_f64))
+                return abi_invalid("leaf struct_size is not sizeof(sdpx_gpu_sampled_leaf_f64)");
         sdpx::Device &d = sdpx::device();
         std::lock_guard<std::recursive_mutex> g(d.mu);
         std::vector<sdpx::Fp64Leaf> in(nleaves);

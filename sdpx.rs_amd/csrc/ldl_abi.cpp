@@ -153,6 +153,9 @@ thread_local sdpx::SampledStats t_sampled_stats;
 // exact dots, so it declines them (nothing written; the caller falls back).
 int32_t leaf_order_limit(uint64_t nleaves, const sdpx_gpu_sampled_leaf *leaves) {
     for (uint64_t i = 0; i < nleaves; i++)
+        if (leaves[i].struct_size != sizeof(sdpx_gpu_sampled_leaf))
+            return sdpx::abi_invalid("leaf struct_size is not sizeof(sdpx_gpu_sampled_leaf)");
+    for (uint64_t i = 0; i < nleaves; i++)
         if (leaves[i].samples >= 256) {
             sdpx::set_last_error("sampled leaf of order 256 or more (recursive leaf factor) declined");
             return SDPX_GPU_DECLINED;
@@ -240,13 +243,13 @@ int32_t sdpx_gpu_sampled_arrow_dev(uint32_t limbs, uint64_t nleaves,
         if (nborder_signs != 0 && nborder_signs != 1 && nborder_signs != t)
             return sdpx::abi_invalid("border signs: 0, 1 or t entries");
         if (nborder_signs && !border_signs) return sdpx::abi_invalid("null border signs");
+        if (int32_t r = leaf_order_limit(nleaves, leaves)) return r;
         for (uint64_t i = 0; i < nleaves; i++)
             if (!leaves[i].rinv || !leaves[i].basis || !leaves[i].weights ||
                 (t && !leaves[i].coupling))
                 return sdpx::abi_invalid("null leaf array");
         if (nleaves && (!l_all || !dinv_all || (t && !y_all)))
             return sdpx::abi_invalid("null leaf factor output");
-        if (int32_t r = leaf_order_limit(nleaves, leaves)) return r;
         sdpx::Device &d = sdpx::device();
         std::lock_guard<std::recursive_mutex> g(d.mu);
         std::vector<sdpx::SampledLeafDev> dev(nleaves);
@@ -281,6 +284,10 @@ int32_t sdpx_gpu_sampled_arrow(uint32_t limbs, uint64_t nleaves, const sdpx_gpu_
         if (nborder_signs != 0 && nborder_signs != 1 && nborder_signs != t)
             return sdpx::abi_invalid("border signs: 0, 1 or t entries");
         if (int32_t r = leaf_order_limit(nleaves, leaves)) return r;
+        if (factors)
+            for (uint64_t i = 0; i < nleaves; i++)
+                if (factors[i].struct_size != sizeof(sdpx_gpu_sampled_factor))
+                    return sdpx::abi_invalid("factor struct_size is not sizeof(sdpx_gpu_sampled_factor)");
         std::vector<sdpx::SampledLeafIn> in(nleaves);
         for (uint64_t i = 0; i < nleaves; i++) {
             const auto &l = leaves[i];

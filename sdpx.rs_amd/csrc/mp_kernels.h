@@ -91,6 +91,23 @@ void mp_quadratic_weights(int limbs, const void *P, int64_t h, const void *S, vo
 // the full symmetric h x h matrix of a column-packed upper triangle
 void mp_unpack_symmetric(int limbs, const void *P, int64_t h, void *out, hipStream_t st);
 void mp_abs_values(int limbs, void *X, int64_t count, hipStream_t st);
+// C = axpby(alpha, P, beta, C) as the reference's MPFR axpby; codes 0 zero,
+// 1 one, 2 minus one, 3 other (alpha, beta: one device value each, read only
+// for code 3); tri 0 all, 1 upper, 2 lower; packed: C is column-packed upper;
+// sym: P holds a symmetric matrix's upper triangle
+void mp_axpby(int limbs, const void *P, int64_t ldp, int64_t m, int64_t n, int acode, const void *alpha, int bcode,
+              const void *beta, void *C, int64_t ldc, int tri, bool packed, bool sym, hipStream_t st);
+// One block of the grouped exact product (device pointers; cols ascending
+// global columns, int64)
+struct BlockScatter {
+    const void *y, *d;
+    int64_t rows, ncols, row0;
+    const int64_t *cols;
+};
+// Y, Z (K x m, ld K, zero beforehand): every block's y and RN(diag(d) y) at
+// its rows and global columns (descs on the device)
+void mp_block_scatter(int limbs, const BlockScatter *descs, int nblocks, int64_t max_elems, void *Z, void *Y,
+                      int64_t K, hipStream_t st);
 // out[p] = G[ia[p] + ib[p] ldg] (ia, ib device)
 void mp_gather(int limbs, const void *G, int64_t ldg, const int64_t *ia, const int64_t *ib, int64_t count, void *out,
                hipStream_t st);

@@ -2166,6 +2166,69 @@ __global__ void abs_values_kernel(uint32_t *X, int64_t count) {
     if (kind < 0) *k = (uint32_t)(-kind);
 }
 
+// c = axpby(alpha, x, beta, c) entrywise, the reference's MPFR axpby
+// (blas/mpfr.rs:61-80): p = 0, x, -x or RN(alpha x) for alpha codes 0 (zero),
+// 1 (one), 2 (minus one), 3 (other); then p, RN(p + c), RN(p - c) or
+// RN(p + RN(beta c)) for beta's codes.  x = P[i + j ldp] (read from
+// P[min + max ldp] when sym: a symmetric product held as its upper
+// triangle); tri 0 every entry, 1 i <= j, 2 i >= j; packed: c holds the
+// column-packed upper triangle.
+template <int W>
+__global__ void axpby_kernel(const uint32_t *P, int64_t ldp, int64_t m, int64_t n, int acode, const uint32_t *alpha,
+                             int bcode, const uint32_t *beta, uint32_t *C, int64_t ldc, int tri, int packed,
+                             int sym) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m * n) return;
+    const int64_t i = e % m, j = e / m;
+    if ((tri == 1 && i > j) || (tri == 2 && i < j)) return;
+    uint32_t *c = rec_at<W>(C, packed ? j * (j + 1) / 2 + i : i + j * ldc);
+    Val<W> p, y, t;
+    set_zero(p);
+    if (acode != 0) {
+        const int64_t pi = sym && i > j ? j : i, pj = sym && i > j ? i : j;
+        load(rec_at<W>(P, pi + pj * ldp), p);
+        if (acode == 2) {
+            if (is_regular(p) || p.kind == mpd::KIND_INF || p.kind == -mpd::KIND_INF) p.kind = -p.kind;
+        } else if (acode == 3) {
+            Val<W> a;
+            load(alpha, a);
+            mul(a, p, t);
+            p = t;
+        }
+    }
+    if (bcode != 0) {
+        load(c, y);
+        if (bcode == 3) {
+            Val<W> b;
+            load(beta, b);
+            mul(b, y, t);
+            y = t;
+        }
+        add_signed(p, y, bcode == 2, t);
+        p = t;
+    }
+    store(c, p);
+}
+
+// Grouped exact blocks on the device: block b's y (rows x ncols, ld rows)
+// into Y and Z = RN(d_r y) into Z, rows row0.. of the dense K x m operands
+// (ld K) at its global columns; one workgroup row per block
+template <int W>
+__global__ void block_scatter_kernel(const BlockScatter *descs, uint32_t *Z, uint32_t *Y, int64_t K) {
+    const BlockScatter b = descs[blockIdx.y];
+    const int64_t total = b.rows * b.ncols;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = e % b.rows, c = e / b.rows;
+        const int64_t dst = (b.row0 + r) + b.cols[c] * K;
+        Val<W> y, d, z;
+        load(rec_at<W>((const uint32_t *)b.y, e), y);
+        load(rec_at<W>((const uint32_t *)b.d, r), d);
+        mul(d, y, z);
+        store(rec_at<W>(Y, dst), y);
+        store(rec_at<W>(Z, dst), z);
+    }
+}
+
 // out[p] = G[ia[p] + ib[p] ldg]
 template <int W>
 __global__ void gather_kernel(const uint32_t *G, int64_t ldg, const int64_t *ia, const int64_t *ib, int64_t count,
@@ -2584,6 +2647,28 @@ void mp_abs_values(int limbs, void *X, int64_t count, hipStream_t st) {
     SDPX_DISPATCH_W(limbs, hipLaunchKernelGGL(abs_values_kernel<W>, dim3(blocks_for(count, 256)), dim3(256), 0, st,
                                               (uint32_t *)X, count));
     HIP_CHECK(hipGetLastError());
+}
+
+void mp_axpby(int limbs, const void *P, int64_t ldp, int64_t m, int64_t n, int acode, const void *alpha, int bcode,
+              const void *beta, void *C, int64_t ldc, int tri, bool packed, bool sym, hipStream_t st) {
+    if (m * n == 0) return;
+    SDPX_DISPATCH_W(limbs, hipLaunchKernelGGL(axpby_kernel<W>, dim3(blocks_for(m * n, 256)), dim3(256), 0, st,
+                                              (const uint32_t *)P, ldp, m, n, acode, (const uint32_t *)alpha, bcode,
+                                              (const uint32_t *)beta, (uint32_t *)C, ldc, tri, packed ? 1 : 0,
+                                              sym ? 1 : 0));
+    HIP_CHECK(hipGetLastError());
+}
+
+void mp_block_scatter(int limbs, const BlockScatter *descs, int nblocks, int64_t max_elems, void *Z, void *Y,
+                      int64_t K, hipStream_t st) {
+    if (nblocks == 0 || max_elems == 0) return;
+    const unsigned gx = (unsigned)std::min<int64_t>(blocks_for(max_elems, 256), 1024);
+    for (int b0 = 0; b0 < nblocks; b0 += 65535) {
+        const int nb = std::min(65535, nblocks - b0);
+        SDPX_DISPATCH_W(limbs, hipLaunchKernelGGL(block_scatter_kernel<W>, dim3(gx, nb), dim3(256), 0, st, descs + b0,
+                                                  (uint32_t *)Z, (uint32_t *)Y, K));
+        HIP_CHECK(hipGetLastError());
+    }
 }
 
 void mp_gather(int limbs, const void *G, int64_t ldg, const int64_t *ia, const int64_t *ib, int64_t count, void *out,

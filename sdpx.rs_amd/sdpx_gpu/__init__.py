@@ -136,6 +136,12 @@ def _declare(L):
         f = getattr(L, name)
         f.argtypes = gemm_args
         f.restype = i32
+    for name in ("sdpx_gpu_xgemm", "sdpx_gpu_xgemm_dev"):
+        getattr(L, name).argtypes = [u32, u8, u8, u64, u64, u64, vp, vp, u64, vp, u64, vp, vp, u64]
+        getattr(L, name).restype = i32
+    for name in ("sdpx_gpu_xsyrk", "sdpx_gpu_xsyrk_dev"):
+        getattr(L, name).argtypes = [u32, u8, u8, u64, u64, vp, vp, u64, vp, vp, u64, u64]
+        getattr(L, name).restype = i32
     L.sdpx_gpu_diag_congruence_upper_exact.argtypes = [u32, u64, u64, vp, vp, vp]
     L.sdpx_gpu_diag_congruence_upper_exact_dev.argtypes = [u32, u64, u64, vp, vp, vp, u32]
     for name in ("sdpx_gpu_congruence_exact", "sdpx_gpu_congruence_exact_dev"):
@@ -350,14 +356,21 @@ def mp_ldl_stats():
             "gpu_mp_macs": d.value, "declined": e.value, "retried": f.value}
 
 
-class SampledLeaf(ctypes.Structure):
-    _fields_ = [("side", ctypes.c_uint64), ("samples", ctypes.c_uint64),
+class _Sized(ctypes.Structure):
+    """An ABI struct whose first member, struct_size, is filled in."""
+
+    def __init__(self, *args, **kw):
+        super().__init__(ctypes.sizeof(self), *args, **kw)
+
+
+class SampledLeaf(_Sized):
+    _fields_ = [("struct_size", ctypes.c_uint64), ("side", ctypes.c_uint64), ("samples", ctypes.c_uint64),
                 ("rinv", ctypes.c_void_p), ("basis", ctypes.c_void_p),
                 ("weights", ctypes.c_void_p), ("coupling", ctypes.c_void_p)]
 
 
-class SampledFactor(ctypes.Structure):
-    _fields_ = [("l", ctypes.c_void_p), ("dinv", ctypes.c_void_p), ("y", ctypes.c_void_p)]
+class SampledFactor(_Sized):
+    _fields_ = [("struct_size", ctypes.c_uint64), ("l", ctypes.c_void_p), ("dinv", ctypes.c_void_p), ("y", ctypes.c_void_p)]
 
 
 def sampled_arrow(problem, nl, border_signs=(-1,), flags=0, want_factors=False):
@@ -498,8 +511,8 @@ def sampled_stats():
             "retried": rt.value, "failed_leaf": fl.value}
 
 
-class SampledLeafF64(ctypes.Structure):
-    _fields_ = [("side", ctypes.c_uint64), ("samples", ctypes.c_uint64),
+class SampledLeafF64(_Sized):
+    _fields_ = [("struct_size", ctypes.c_uint64), ("side", ctypes.c_uint64), ("samples", ctypes.c_uint64),
                 ("rinv", ctypes.c_void_p), ("basis", ctypes.c_void_p),
                 ("weights", ctypes.c_void_p), ("coupling", ctypes.c_void_p)]
 
@@ -580,7 +593,7 @@ def adjoint_psd_full_f64(factor, rinv, a_svec, y, x_svec, c_svec=None, ybar=None
     return bgrad, cgrad, agrad
 
 
-class PsdBlockF64(ctypes.Structure):
+class PsdBlockF64(_Sized):
     _fields_ = [("struct_size", ctypes.c_uint64), ("n", ctypes.c_uint64), ("ncols", ctypes.c_uint64),
                 ("rinv", ctypes.c_void_p), ("a_svec", ctypes.c_void_p), ("columns", ctypes.c_void_p)]
 
@@ -601,7 +614,7 @@ def condensed_assemble_f64(m, p_csc, blocks, b_rows=None, w=None):
         a = dev(np.asfortranarray(a_svec).T.reshape(-1))
         c = dev(np.asarray(cols, dtype=np.int64), torch.int64)
         keep += [r, a, c]
-        bl[i] = PsdBlockF64(ctypes.sizeof(PsdBlockF64), n, len(cols), r.data_ptr(), a.data_ptr(), c.data_ptr())
+        bl[i] = PsdBlockF64(n, len(cols), r.data_ptr(), a.data_ptr(), c.data_ptr())
     S = torch.zeros(m * m, dtype=torch.float64, device="cuda")
     pc = pr = pv = None
     if p_csc is not None:
@@ -618,8 +631,8 @@ def condensed_assemble_f64(m, p_csc, blocks, b_rows=None, w=None):
     return S.cpu().numpy().reshape(m, m).T.copy()
 
 
-class ExactBlock(ctypes.Structure):
-    _fields_ = [("rows", ctypes.c_uint64), ("ncols", ctypes.c_uint64), ("y", ctypes.c_void_p),
+class ExactBlock(_Sized):
+    _fields_ = [("struct_size", ctypes.c_uint64), ("rows", ctypes.c_uint64), ("ncols", ctypes.c_uint64), ("y", ctypes.c_void_p),
                 ("d", ctypes.c_void_p), ("columns", ctypes.c_void_p)]
 
 
@@ -635,6 +648,27 @@ def xgemm_blocks_upper_exact(m, blocks, out):
         arr[i] = ExactBlock(d.size, cols.size, _ptr(y), _ptr(d), _ptr(cols))
     nl = mp.limbs_of(out)
     code = lib().sdpx_gpu_rns_gemm_blocks_upper(nl, m, len(blocks), arr, _ptr(out))
+    check(code, allow_decline=True)
+    return code == OK
+
+
+def xgemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+    """XgemmScalar::xgemm for MPFR (blas/mpfr.rs:245-348): c = axpby(alpha,
+    RN(op(a) op(b)), beta, c) with c of leading dimension ldc; alpha and beta
+    are one-value record arrays.  False on a decline (c untouched)."""
+    code = lib().sdpx_gpu_xgemm(mp.limbs_of(c), ord(ta), ord(tb), m, n, k, _ptr(alpha), _ptr(a), lda, _ptr(b), ldb,
+                                _ptr(beta), _ptr(c), ldc)
+    check(code, allow_decline=True)
+    return code == OK
+
+
+def xsyrk(uplo, trans, n, k, alpha, a, lda, beta, c, ldc):
+    """XsyrkScalar::xsyrk for MPFR (blas/mpfr.rs:540-626): the uplo triangle
+    of c = axpby(alpha, RN(op(a) op(a)^T), beta, c); c column-packed upper
+    when uplo is 'U', ldc == n and c has n(n+1)/2 values.  False on a
+    decline (c untouched)."""
+    code = lib().sdpx_gpu_xsyrk(mp.limbs_of(c), ord(uplo), ord(trans), n, k, _ptr(alpha), _ptr(a), lda,
+                                _ptr(beta), _ptr(c), ldc, c.size)
     check(code, allow_decline=True)
     return code == OK
 

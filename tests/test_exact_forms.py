@@ -175,3 +175,30 @@ def test_declines_follow_the_reference():
     assert mpexact.diag_congruence_exact(1, 2, spread, vals_of([1, 1], nl)) is None
     zeros = [mpexact.zero(nl)] * 4
     assert values(mpexact.congruence_exact("N", 2, 2, zeros, 2, vals_of([1, 2, 3, 4], nl), 2)) == [0] * 4
+
+
+def axpby_case(p):
+    """alpha = 1 + u, x = 1 + 3u, beta = -1, y = 1 (u = 2^(1-p)): p =
+    RN(alpha x) = 1 + 4u and p - y = 4u, where the exact alpha x - y =
+    4u + 3u^2 rounds once to 4u + 4u^2.  Returns (alpha, x, beta, y, want,
+    fused)."""
+    u = Fraction(1, 2 ** (p - 1))
+    alpha, x, beta, y = 1 + u, 1 + 3 * u, Fraction(-1), Fraction(1)
+    want, fused = 4 * u, pins.rn(alpha * x - y, p)
+    assert fused == 4 * u + 4 * u * u != want
+    return alpha, x, beta, y, want, fused
+
+
+@pytest.mark.parametrize("nl", [2, 4])
+def test_axpby_rounds_the_scaled_product_before_the_sum(nl):
+    alpha, x, beta, y, want, _ = axpby_case(64 * nl)
+    a, xv, b, yv = vals_of([alpha, x, beta, y], nl)
+    assert mpexact.axpby(a, xv, b, yv).to_fraction() == want
+    # the special scalars: 0, 1, -1 for alpha; 0, 1, -1 for beta; other beta
+    one, mone, z = vals_of([1, -1, 0], nl)
+    assert mpexact.axpby(z, xv, z, yv).is_zero()
+    assert mpexact.axpby(one, xv, z, yv).to_fraction() == x
+    assert mpexact.axpby(mone, xv, one, yv).to_fraction() == pins.rn(-x + y, 64 * nl)
+    assert mpexact.axpby(one, xv, mone, yv).to_fraction() == pins.rn(x - y, 64 * nl)
+    b3 = vals_of([Fraction(3, 4)], nl)[0]
+    assert mpexact.axpby(one, xv, b3, xv).to_fraction() == pins.rn(x + pins.rn(Fraction(3, 4) * x, 64 * nl), 64 * nl)

@@ -7,6 +7,7 @@ rns_blas_tests.rs:226-229), zero operands, decline rules
 (rns_blas_tests.rs:476-495 forces Inf), and a spot check at the full
 4096^3 workload size.
 """
+import ctypes
 import hashlib
 import json
 import os
@@ -247,6 +248,32 @@ def test_grouped_blocks_upper(gpu):
                         pairs.append((M.mul(dr, yi), yj))
             ref = M.exact_dot(pairs, nl)
             assert mp.same_bits(out[i + j * m: i + j * m + 1], from_mp([ref], nl))[0], (i, j)
+    # the strict lower triangle is left as given
+    sentinel = mp.from_float64(np.full(m * m, -0.375), nl)
+    kept = sentinel.copy()
+    assert gpu.xgemm_blocks_upper_exact(m, blocks, kept)
+    for j in range(m):
+        for i in range(m):
+            e = i + j * m
+            same = kept[e].tobytes() == (out[e] if i <= j else sentinel[e]).tobytes()
+            assert same, (i, j)
+    # the device-pointer form gives the same bits
+    import torch
+    dev = lambda rec: torch.from_numpy(np.ascontiguousarray(rec).view(np.uint8).copy()).cuda()
+    keep = []
+    arr = (gpu.ExactBlock * len(blocks))()
+    for b, (y, d, cols) in enumerate(blocks):
+        ty, td, tc = dev(y), dev(d), np.ascontiguousarray(cols, np.uint64)
+        keep += [ty, td, tc]
+        arr[b] = gpu.ExactBlock(d.size, cols.size, ty.data_ptr(), td.data_ptr(), tc.ctypes.data)
+    tout = dev(mp.zeros(m * m, nl))
+    torch.cuda.synchronize()
+    L = gpu.lib()
+    L.sdpx_gpu_rns_gemm_blocks_upper_dev.restype = ctypes.c_int32
+    gpu.check(L.sdpx_gpu_rns_gemm_blocks_upper_dev(nl, m, len(blocks), arr, ctypes.c_void_p(tout.data_ptr())))
+    got = tout.cpu().numpy().view(mp.dtype(nl))
+    for j in range(m):
+        assert got[j * m: j * m + j + 1].tobytes() == out[j * m: j * m + j + 1].tobytes(), j
 
 
 @pytest.mark.parametrize("bits", [1, 2, 3, 8, 13, 24, 31, 40, 53])
