@@ -107,6 +107,30 @@ void product(DCtx &c, char ta, char tb, size_t m, size_t n, size_t k, const uint
     c.stats->gpu_mp_macs += (double)k * (packed ? n * (n + 1) / 2.0 : (double)m * n);
 }
 
+// Several products with one host wait (rns_gemm_batch); after(i) queues the
+// work that consumes item i's output before item i + 1 runs.
+void products_batch(DCtx &c, const std::vector<RnsGemmArgs> &items, const std::function<void(size_t)> &after) {
+    if (items.empty() || c.declined) return;
+    PhaseMark pm(3, c.st);
+    size_t failed = 0;
+    const int r = rns_gemm_batch(
+        items,
+        [&](size_t i) {
+            const RnsGemmArgs &g = items[i];
+            if (g.m > 0 && g.n > 0) {
+                c.stats->gpu_products++;
+                c.stats->gpu_mp_macs += (double)g.k * (g.upper ? g.n * (g.n + 1) / 2.0 : (double)g.m * g.n);
+            }
+            if (after) after(i);
+        },
+        &failed);
+    if (r == SDPX_GPU_DECLINED) {
+        c.declined = true;
+        return;
+    }
+    if (r != SDPX_GPU_OK) throw Error(r, "device exact product failed");
+}
+
 // ldl_dots of one block on the GPU, queued without a wait: a failed pivot
 // (status 1) or a dot beyond its window (status 2) stays in d_status[0]
 // (the kernels keep the largest code and every later launch returns at
@@ -494,18 +518,19 @@ int sampled_arrow_device(int limbs, const std::vector<SampledLeafDev> &leaves, s
                          const int8_t *border_signs, size_t nborder_signs, uint8_t *d_border_dinv,
                          SampledStats &st) {
     const size_t vb = value_bytes(limbs);
-    size_t kmax = 0, hmax = 0, ksum = 0;
+    size_t kmax = 0, hmax = 0, ksum = 0, build_vals = 0;
     std::vector<size_t> loff, doff, yoff;
     for (auto &l : leaves) {
         kmax = std::max(kmax, l.k);
         hmax = std::max(hmax, l.h);
+        build_vals += l.h * l.k + l.k * (l.k + 1) / 2 + 2 * 256 / vb + 2;  // V, Gram (and alignment)
         loff.push_back(loff.empty() ? 0 : loff.back() + leaves[loff.size() - 1].k * leaves[loff.size() - 1].k);
         doff.push_back(ksum);
         yoff.push_back(ksum * t);
         ksum += l.k;
     }
     // leaf scratch, Z of every leaf, one packed contribution, the border factor
-    size_t arena = ldl_arena_bytes(t, vb) + (hmax * kmax + kmax * (kmax + 1) / 2 + ksum * t) * vb +
+    size_t arena = ldl_arena_bytes(t, vb) + (build_vals + ksum * t) * vb +
                    t * (t + 1) / 2 * vb + 16 * ksum + (64u << 20);
     size_t per_leaf = ldl_scratch_records(kmax);
     size_t scratch = std::max<size_t>(leaves.size() * per_leaf * vb, ldl_scratch_records(128) * vb) + 4096;
@@ -516,16 +541,26 @@ int sampled_arrow_device(int limbs, const std::vector<SampledLeafDev> &leaves, s
     hipEvent_t ev[4];
     for (auto &e : ev) HIP_CHECK(hipEventCreate(&e));
     HIP_CHECK(hipEventRecord(ev[0], s));
-    // 1. Schur build of every leaf: V = Rinv Q, Gram = V^T V, entries
-    for (size_t i = 0; i < leaves.size() && !c.declined; i++) {
-        const auto &lf = leaves[i];
-        size_t mark = c.arena->top;
-        uint8_t *V = (uint8_t *)c.arena->alloc(lf.h * lf.k * vb);
-        uint8_t *G = (uint8_t *)c.arena->alloc(lf.k * (lf.k + 1) / 2 * vb);
-        product(c, 'N', 'N', lf.h, lf.k, lf.h, lf.rinv, lf.h, lf.q, lf.h, V, false);
-        product(c, 'T', 'N', lf.k, lf.k, lf.h, V, lf.h, V, lf.h, G, true);
-        if (c.declined) break;
-        mp_sampled_entries(limbs, G, lf.w, (int64_t)lf.k, c.at(d_l_all, loff[i]), s);
+    // 1. Schur build of every leaf: V = Rinv Q (every leaf, one host wait),
+    // Gram = V^T V (one wait), then the sampled entries
+    {
+        const size_t mark = c.arena->top;
+        std::vector<uint8_t *> Vs(leaves.size()), Gs(leaves.size());
+        std::vector<RnsGemmArgs> vprod, gram;
+        for (size_t i = 0; i < leaves.size(); i++) {
+            const auto &lf = leaves[i];
+            Vs[i] = (uint8_t *)c.arena->alloc(lf.h * lf.k * vb);
+            Gs[i] = (uint8_t *)c.arena->alloc(lf.k * (lf.k + 1) / 2 * vb);
+            vprod.push_back(RnsGemmArgs{limbs, 'N', 'N', (int64_t)lf.h, (int64_t)lf.k, (int64_t)lf.h, lf.rinv,
+                                        (int64_t)lf.h, lf.q, (int64_t)lf.h, false, (uint64_t)(lf.h * lf.k), Vs[i]});
+            gram.push_back(RnsGemmArgs{limbs, 'T', 'N', (int64_t)lf.k, (int64_t)lf.k, (int64_t)lf.h, Vs[i],
+                                       (int64_t)lf.h, Vs[i], (int64_t)lf.h, true,
+                                       (uint64_t)(lf.k * (lf.k + 1) / 2), Gs[i]});
+        }
+        products_batch(c, vprod, nullptr);
+        products_batch(c, gram, [&](size_t i) {
+            mp_sampled_entries(limbs, Gs[i], leaves[i].w, (int64_t)leaves[i].k, c.at(d_l_all, loff[i]), s);
+        });
         c.arena->top = mark;
     }
     HIP_CHECK(hipEventRecord(ev[1], s));
@@ -595,13 +630,24 @@ int sampled_arrow_device(int limbs, const std::vector<SampledLeafDev> &leaves, s
             for (size_t i = 0; i < leaves.size(); i++)
                 mp_ew_mul_diag(limbs, c.at(d_y_all, yoff[i]), leaves[i].k, c.at(d_dinv_all, doff[i]),
                                c.at(Z_all, yoff[i]), leaves[i].k, leaves[i].k, t, true, s);
+            // every leaf's product with one host wait; the matrix-core CRT
+            // subtracts each rounded entry from the border itself, else the
+            // packed product P is subtracted before the next leaf's product
             uint8_t *P = (uint8_t *)c.arena->alloc(t * (t + 1) / 2 * vb);
-            for (size_t i = 0; i < leaves.size() && !c.declined; i++) {
-                const size_t K = leaves[i].k;
-                product(c, 'T', 'N', t, t, K, c.at(d_y_all, yoff[i]), K, c.at(Z_all, yoff[i]), K, P, true);
-                if (c.declined) break;
-                mp_ew_sub_packed(limbs, d_border, t, P, t, 0, s);
+            std::vector<RnsGemmArgs> contrib;
+            std::vector<int> fused(leaves.size(), 0);
+            for (size_t i = 0; i < leaves.size(); i++) {
+                const int64_t K = (int64_t)leaves[i].k;
+                RnsGemmArgs g{limbs, 'T', 'N', (int64_t)t, (int64_t)t, K, c.at(d_y_all, yoff[i]), K,
+                              c.at(Z_all, yoff[i]), K, true, (uint64_t)(t * (t + 1) / 2), P};
+                g.sub = d_border;
+                g.sub_ld = (int64_t)t;
+                g.sub_done = &fused[i];
+                contrib.push_back(g);
             }
+            products_batch(c, contrib, [&](size_t i) {
+                if (!fused[i]) mp_ew_sub_packed(limbs, d_border, t, P, t, 0, s);
+            });
             c.arena->top = mark3;
         }
     }

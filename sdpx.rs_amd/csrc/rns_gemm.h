@@ -1,5 +1,7 @@
 // Exact RNS product on device-resident values (see rns_gemm.hip).
 #pragma once
+#include <functional>
+#include <vector>
 #include <cstdint>
 
 namespace sdpx {
@@ -27,6 +29,18 @@ struct RnsGemmArgs {
     // needed or the plan takes the VALU CRT
     int split_capacity = 0;
     int *split_used = nullptr;
+    // fused subtraction: with sub set (an n x n upper product, matrix-core
+    // CRT, 2, 4 or 8 limbs), entry (i, j) is not stored; instead
+    // sub[i + j sub_ld] = RN(sub[i + j sub_ld] - RN(entry)) and *sub_done = 1.
+    // Otherwise out is written as usual and *sub_done = 0.
+    void *sub = nullptr;
+    int64_t sub_ld = 0;
+    int *sub_done = nullptr;
+    // batched form (rns_gemm_batch): the scans already ran; their results
+    // (int4 a row / column) and the 16 summary words (host-readable) are
+    // read here instead of scanning and waiting
+    const void *scan_a_in = nullptr, *scan_b_in = nullptr;
+    const int64_t *summary = nullptr;
 };
 
 // Size of the per-device array of the allocation-free exact product.
@@ -63,5 +77,14 @@ RnsTotals &rns_totals();
 // Returns an SDPX_GPU_* code; throws sdpx::Error on device failures.
 int rns_gemm_device(const RnsGemmArgs &args);
 const RnsStats &rns_last_stats();
+// Several products on the library stream with one host wait: every item's
+// scans first, then one wait for all windows, then each item's encode,
+// products and CRT (after(i) runs on the host once item i is queued, e.g.
+// to queue work that consumes its output before the next item reuses the
+// buffers).  Stops at the first item that does not return SDPX_GPU_OK and
+// returns that code, with *failed its index.  Items must not set scratch
+// or the batched-form fields.
+int rns_gemm_batch(const std::vector<RnsGemmArgs> &items, const std::function<void(size_t)> &after,
+                   size_t *failed);
 
 }  // namespace sdpx

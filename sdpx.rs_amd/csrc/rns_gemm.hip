@@ -30,6 +30,7 @@
 #include <cstring>
 
 #include "common.h"
+#include "mpdev.h"
 #include "rns_gemm.h"
 #include "rns_plan.h"
 
@@ -880,13 +881,28 @@ __global__ __launch_bounds__(BS) void crt_kernel(
 constexpr int CRT_MAXW = 16;        // mantissa words handled (up to 512 bits)
 constexpr size_t CRT_NU_MAX = 40;   // words of M the matrix-core CRT reads (zero-padded)
 
+// The fused subtraction of the arrow contributions: srec = RN(srec - c) for
+// the rounded entry c (words mant, kind, exponent) in registers.
+template <int NL>
+__device__ __forceinline__ void crt_sub_store(uint32_t *srec, const uint32_t *mant, int32_t kind, int32_t expo) {
+    constexpr int W = 2 * NL;
+    mpd::Val<W> c, sv, r;
+#pragma unroll
+    for (int w = 0; w < W; w++) c.m[w] = mant[w];
+    c.kind = kind;
+    c.e = expo;
+    mpd::load(srec, sv);
+    mpd::add_signed(sv, c, true, r);
+    mpd::store(srec, r);
+}
+
 template <int NUMAX, int NBMAX, int NL>
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void crt_mfma_kernel(
     const uint16_t *__restrict__ X, int L, int lpad, const int2 *__restrict__ tiles, int ntiles,
     int tiles_m, int tiles_n, const int8_t *__restrict__ table, int nt, int npad,
     const uint32_t *__restrict__ mlimbs, int nu, double m_top, const int4 *__restrict__ scan_a,
     const int4 *__restrict__ scan_b, int64_t m, int64_t nn, int upper, int packed, int n_arg,
-    uint32_t *__restrict__ out, int chunks, int64_t chunk_stride) {
+    uint32_t *__restrict__ out, int chunks, int64_t chunk_stride, uint32_t *__restrict__ sub, int64_t sub_ld) {
     const int n = NL ? NL : n_arg;  // output limbs (a constant in the common widths)
     extern __shared__ __attribute__((aligned(16))) int8_t csm[];
     const int trow = 2 * lpad + 16;  // bytes per table row
@@ -1028,6 +1044,11 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void c
         for (int u = 0; u < NUMAX; u++) sl[u * 128 + tid] = lim[u];
         if (!active) continue;
         uint32_t *rec = out + (size_t)(packed ? j * (j + 1) / 2 + i : i + j * m) * (2 * n + 2);
+        uint32_t *srec = nullptr;  // fused subtraction: the target entry
+        if constexpr (NL > 0) {
+            if (sub) srec = sub + (size_t)(i + j * sub_ld) * (2 * NL + 2);
+        }
+        if (top < 0 && srec) continue;  // RN(s - 0) = s
         if (top < 0) {
             for (int cidx = 0; cidx < (chunks > 0 ? chunks : 1); cidx++) {
                 uint32_t *crec = chunks > 0 ? out + ((size_t)cidx * chunk_stride + (size_t)(i + j * m)) * (2 * n + 2)
@@ -1120,16 +1141,30 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void c
         }
         const int64_t expo = (int64_t)scan_a[i].x + scan_b[j].x + shift + P;
         if (expo > (int64_t)((1 << 30) - 1)) {
+            if constexpr (NL > 0) {
+                if (srec) {
+                    uint32_t zw[2 * (NL > 0 ? NL : 1)] = {};
+                    crt_sub_store<NL>(srec, zw, negative ? -KIND_INF : KIND_INF, 0);
+                    continue;
+                }
+            }
             for (int w = 0; w < 2 * n; w++) rec[w] = 0;
             rec[2 * n] = (uint32_t)(negative ? -KIND_INF : KIND_INF);
             rec[2 * n + 1] = 0;
             continue;
         }
         if (expo < (int64_t)(1 - (1 << 30))) {
+            if (srec) continue;  // the entry rounds to zero: RN(s - 0) = s
             for (int w = 0; w < 2 * n; w++) rec[w] = 0;
             rec[2 * n] = KIND_ZERO;
             rec[2 * n + 1] = 0;
             continue;
+        }
+        if constexpr (NL > 0) {
+            if (srec) {
+                crt_sub_store<NL>(srec, mant, negative ? -KIND_REGULAR : KIND_REGULAR, (int32_t)expo);
+                continue;
+            }
         }
 #pragma unroll
         for (int w = 0; w < CRT_MAXW; w++)
@@ -1159,7 +1194,7 @@ namespace {
 
 enum Slot {
     S_SCAN_A = 0, S_SCAN_B, S_PLANES_A, S_PLANES_B, S_X, S_POW_A, S_POW_B, S_MODS, S_CRT,
-    S_TILES, S_ENC_A, S_ENC_B, S_CRT2, S_COUNT
+    S_TILES, S_ENC_A, S_ENC_B, S_CRT2, S_BATCH_SCAN, S_COUNT
 };
 
 struct Operand {
@@ -1632,6 +1667,7 @@ int rns_gemm_device(const RnsGemmArgs &g) {
     const int64_t m = g.m, nn = g.n, k = g.k;
     const bool packed = g.upper && m == nn && g.out_len == (uint64_t)(nn * (nn + 1) / 2);
     t_stats = RnsStats{};
+    if (g.sub_done) *g.sub_done = 0;
     if (m == 0 || nn == 0) return SDPX_GPU_OK;
 
     Operand oa{(const uint32_t *)g.a, m, k, tan ? 1 : g.lda, tan ? g.lda : 1};
@@ -1654,14 +1690,21 @@ int rns_gemm_device(const RnsGemmArgs &g) {
     // 1. scan
     StageEvents sev(g_profile && !fixed);
     sev.mark(0, s);
-    int4 *scan_a = (int4 *)ws.get(S_SCAN_A, sizeof(int4) * m);
-    int4 *scan_b = same ? scan_a : (int4 *)ws.get(S_SCAN_B, sizeof(int4) * nn);
+    int4 *scan_a, *scan_b;
+    int64_t rb_host[16] = {};
+    int64_t *rb;
+    if (g.summary) {  // batched form: scanned and waited for by rns_gemm_batch
+        scan_a = (int4 *)g.scan_a_in;
+        scan_b = same ? scan_a : (int4 *)g.scan_b_in;
+        rb = (int64_t *)g.summary;
+    } else {
+    scan_a = (int4 *)ws.get(S_SCAN_A, sizeof(int4) * m);
+    scan_b = same ? scan_a : (int4 *)ws.get(S_SCAN_B, sizeof(int4) * nn);
     launch_scan(oa, n, scan_a, s);
     if (!same) launch_scan(ob, n, scan_b, s);
     // the five-word summaries: pinned read-back words, or (allocation-free
     // form) 16 words of the scratch copied back
-    int64_t rb_host[16] = {};
-    int64_t *rb = fixed ? (int64_t *)ws.get(-1, sizeof rb_host) : readback_words();
+    rb = fixed ? (int64_t *)ws.get(-1, sizeof rb_host) : readback_words();
     hipLaunchKernelGGL(scan_summary, dim3(1), dim3(256), 0, s, (const int4 *)scan_a, m, rb);
     if (!same) hipLaunchKernelGGL(scan_summary, dim3(1), dim3(256), 0, s, (const int4 *)scan_b, nn, rb + 8);
     HIP_CHECK(hipGetLastError());
@@ -1670,6 +1713,7 @@ int rns_gemm_device(const RnsGemmArgs &g) {
         rb = rb_host;
     }
     HIP_CHECK(hipStreamSynchronize(s));
+    }
     Window wa = window_of(rb), wb = same ? wa : window_of(rb + 8);
     if (wa.nonfinite || wb.nonfinite) return SDPX_GPU_DECLINED;
     if ((wa.any && wa.emax - wa.emin > MAX_SPREAD) || (wb.any && wb.emax - wb.emin > MAX_SPREAD))
@@ -1837,6 +1881,9 @@ int rns_gemm_device(const RnsGemmArgs &g) {
     uint32_t *outw = (uint32_t *)g.out;
     const int nbl = plan->crt_npad / 32;
     const bool crt_mma = crt_matrix_core(*plan, n);
+    // the fused subtraction needs the matrix-core CRT at a compiled width
+    const bool fuse_sub = g.sub && crt_mma && split == 0 && g.upper && m == nn && (n == 2 || n == 4 || n == 8);
+    if (g.sub_done) *g.sub_done = fuse_sub ? 1 : 0;
     if (crt_mma) {
         const int lpad = plan->crt_lpad, npad = plan->crt_npad, nu = plan->crt_nu;
         const size_t tb = plan->crt_table.size();
@@ -1854,7 +1901,7 @@ int rns_gemm_device(const RnsGemmArgs &g) {
                                (const int2 *)d_tiles, ntiles, tiles_m, tiles_n, (const int8_t *)d_ct,
                                plan->crt_nt, npad, (const uint32_t *)(d_ct + tb), nu, plan->m_top,
                                (const int4 *)scan_a, (const int4 *)scan_b, m, nn, up, pk, n, outw, split,
-                               (int64_t)(m * nn));
+                               (int64_t)(m * nn), fuse_sub ? (uint32_t *)g.sub : nullptr, g.sub_ld);
         };
         auto pick = [&](auto k2, auto k4, auto k8, auto kn) {
             if (n == 2) launch(k2);
@@ -1895,6 +1942,82 @@ int rns_gemm_device(const RnsGemmArgs &g) {
         t_totals.int8_ops += 2.0 * outs * (double)k * gemms;
         for (int i = 0; i < 4; i++) t_totals.stage_ms[i] += t_stats.stage_ms[i];
         t_totals.calls++;
+    }
+    return SDPX_GPU_OK;
+}
+
+namespace {
+// grow-only pinned words for the batched scan summaries (16 an item)
+int64_t *batch_words(size_t count) {
+    static int64_t *words = nullptr;
+    static size_t have = 0;
+    if (count > have) {
+        if (words) HIP_CHECK(hipHostFree(words));
+        words = nullptr;
+        have = 0;
+        const size_t want = std::max<size_t>(count, 1024);
+        HIP_CHECK(hipHostMalloc((void **)&words, want * sizeof(int64_t), hipHostMallocDefault));
+        have = want;
+    }
+    return words;
+}
+}  // namespace
+
+int rns_gemm_batch(const std::vector<RnsGemmArgs> &items, const std::function<void(size_t)> &after,
+                   size_t *failed) {
+    Device &dev = device();
+    hipStream_t s = dev.stream;
+    const size_t N = items.size();
+    if (failed) *failed = 0;
+    // every item's scans, each into its own rows and summary words
+    std::vector<size_t> off_a(N), off_b(N);
+    std::vector<char> scanned(N, 0), same(N, 0);
+    size_t rows = 0;
+    for (size_t i = 0; i < N; i++) {
+        const RnsGemmArgs &g = items[i];
+        const bool tan = (g.ta == 'N' || g.ta == 'n'), tbn = (g.tb == 'N' || g.tb == 'n');
+        const Operand oa{(const uint32_t *)g.a, g.m, g.k, tan ? 1 : g.lda, tan ? g.lda : 1};
+        const Operand ob{(const uint32_t *)g.b, g.n, g.k, tbn ? g.ldb : 1, tbn ? 1 : g.ldb};
+        same[i] = g.a == g.b && oa.sv == ob.sv && oa.sp == ob.sp && g.m == g.n && g.lda == g.ldb;
+        scanned[i] = g.m > 0 && g.n > 0 && g.k > 0;
+        if (!scanned[i]) continue;
+        off_a[i] = rows;
+        rows += (size_t)g.m;
+        off_b[i] = same[i] ? off_a[i] : rows;
+        if (!same[i]) rows += (size_t)g.n;
+    }
+    int4 *scan = (int4 *)dev.ws.get(S_BATCH_SCAN, sizeof(int4) * std::max<size_t>(rows, 1));
+    int64_t *words = batch_words(16 * std::max<size_t>(N, 1));
+    for (size_t i = 0; i < N; i++) {
+        if (!scanned[i]) continue;
+        const RnsGemmArgs &g = items[i];
+        const bool tan = (g.ta == 'N' || g.ta == 'n'), tbn = (g.tb == 'N' || g.tb == 'n');
+        const Operand oa{(const uint32_t *)g.a, g.m, g.k, tan ? 1 : g.lda, tan ? g.lda : 1};
+        const Operand ob{(const uint32_t *)g.b, g.n, g.k, tbn ? g.ldb : 1, tbn ? 1 : g.ldb};
+        launch_scan(oa, g.limbs, scan + off_a[i], s);
+        hipLaunchKernelGGL(scan_summary, dim3(1), dim3(256), 0, s, (const int4 *)(scan + off_a[i]), g.m,
+                           words + 16 * i);
+        if (!same[i]) {
+            launch_scan(ob, g.limbs, scan + off_b[i], s);
+            hipLaunchKernelGGL(scan_summary, dim3(1), dim3(256), 0, s, (const int4 *)(scan + off_b[i]), g.n,
+                               words + 16 * i + 8);
+        }
+        HIP_CHECK(hipGetLastError());
+    }
+    HIP_CHECK(hipStreamSynchronize(s));  // the one wait for every window
+    for (size_t i = 0; i < N; i++) {
+        RnsGemmArgs g = items[i];
+        if (scanned[i]) {
+            g.scan_a_in = scan + off_a[i];
+            g.scan_b_in = scan + off_b[i];
+            g.summary = words + 16 * i;
+        }
+        const int r = rns_gemm_device(g);
+        if (r != SDPX_GPU_OK) {
+            if (failed) *failed = i;
+            return r;
+        }
+        if (after) after(i);
     }
     return SDPX_GPU_OK;
 }
