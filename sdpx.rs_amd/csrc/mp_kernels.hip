@@ -1644,6 +1644,17 @@ __global__ __launch_bounds__(3 * ROW_THREADS) void ldl_rows_kernel(
     const int64_t fi = c.first[i];
     if (threadIdx.x < 2) nxt_base[threadIdx.x] = INT64_MIN;
     __syncthreads();
+    // wave 0: pivot i's operands, loaded now so that no global load sits on
+    // the pivot's critical path
+    wmp::WV<W> aii_w = wmp::zero<W>(), eps_w = wmp::zero<W>(), delta_w = wmp::zero<W>();
+    if (wave == 0) {
+        aii_w = wmp::load<W>(rec_at<W>((const uint32_t *)c.a, i + i * c.ld));
+        if (reg_eps) {
+            eps_w = wmp::load<W>(reg_eps);
+            delta_w = wmp::load<W>(reg_delta);
+        }
+    }
+    const int sg_i = sign_mode == 0 ? 1 : (sign_mode == 1 ? signs[0] : signs[B.sign_off + i]);
     int4 ri = make_int4(INT32_MAX, INT32_MIN, INT32_MAX, INT32_MIN);  // wave 0: row i's ranges so far
     wmp::WV<W> vprev = wmp::zero<W>();                                  // wave 0: r_{i, j - 1}
     for (int64_t j = 0; j < i; j++) {
@@ -1699,12 +1710,13 @@ __global__ __launch_bounds__(3 * ROW_THREADS) void ldl_rows_kernel(
                 if (lane == 0) nxt_base[par ^ 1] = lo_r;
             }
         } else {  // wave 0: entry (i, j) up to its product with RN(1 / d_j)
+            // a_ij is input: its load runs beside the wait
+            const wmp::WV<W> aij = zero_row ? wmp::zero<W>() : wmp::load<W>(rec_at<W>((const uint32_t *)c.a, i + j * c.ld));
             okw = ldl_wait(pivot_ready + j, 1, status);
             if (okw) {
                 dj = wmp::load<W>(rec_at<W>((const uint32_t *)c.dstore, j));
                 if (!zero_row) {
                     const int4 rj = c.rng[j];
-                    const wmp::WV<W> aij = wmp::load<W>(rec_at<W>((const uint32_t *)c.a, i + j * c.ld));
                     int64_t hl, hh;
                     range_hint<W>(ri.x, ri.y, rj.z, rj.w, hl, hh);
                     wmp::WV<W> t;
@@ -1776,13 +1788,8 @@ __global__ __launch_bounds__(3 * ROW_THREADS) void ldl_rows_kernel(
                     int code = 3;
                     const int64_t plo = pre_lo;
                     if (plo != INT64_MIN) {
-                        const wmp::WV<W> aii = wmp::load<W>(rec_at<W>((const uint32_t *)c.a, i + i * c.ld));
-                        wmp::WV<W> eps = wmp::zero<W>(), delta = wmp::zero<W>();
-                        if (reg_eps) {
-                            eps = wmp::load<W>(reg_eps);
-                            delta = wmp::load<W>(reg_delta);
-                        }
-                        const int sg = sign_mode == 0 ? 1 : (sign_mode == 1 ? signs[0] : signs[B.sign_off + i]);
+                        const wmp::WV<W> &aii = aii_w, &eps = eps_w, &delta = delta_w;
+                        const int sg = sg_i;
                         int64_t T = pre_tot[lane];
                         bool fits = true;
                         if (!wmp::is_zero(v) && !wmp::is_zero(wv)) {
