@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define SDPX_GPU_ABI_VERSION 3u
+#define SDPX_GPU_ABI_VERSION 4u
 #define SDPX_GPU_OK 0
 #define SDPX_GPU_INVALID 1
 #define SDPX_GPU_DECLINED 2
@@ -370,6 +370,48 @@ int32_t sdpx_gpu_sampled_arrow_solve_dev(uint32_t limbs, uint64_t nleaves, const
                                          const void *l_all, const void *dinv_all, const void *y_all,
                                          const void *border, const void *border_dinv, uint64_t nrhs, void *x,
                                          uint64_t ldx);
+
+/* The sampled Schur build alone, device pointers: SampledSchurWorkspace::
+ * update_with_pool and entry (crates/solver/src/solver/sampled/mod.rs:
+ * 1836-2143, 1973-1988) for every leaf, leaf i's H (K_i x K_i, lower
+ * triangle, zeros above) at record sum_{j<i} K_j^2 of h_all: the values
+ * sdpx_gpu_sampled_arrow_dev factors.  Only side, samples, rinv, basis and
+ * weights of each leaf are read.  2: a product declined. */
+int32_t sdpx_gpu_sampled_schur_dev(uint32_t limbs, uint64_t nleaves, const sdpx_gpu_sampled_leaf *leaves,
+                                   void *h_all);
+
+/* DirectLDLKKTSolver::solve_many with stationary iterative refinement
+ * (crates/solver/src/solver/kkt/direct/solver.rs:470-520; refine 1046-1129;
+ * settings of default/settings.rs:252-270) on the sampled arrow KKT
+ * K = [blockdiag(H_i) B; B^T C], device pointers except settings, ok, norms
+ * and refinements (host): x = K^-1 b by sdpx_gpu_sampled_arrow_solve_dev's
+ * sweeps, then per column, while its residual norm exceeds
+ * abstol + reltol |b|_inf and at most max_iter times: dx = K^-1 e,
+ * x' = RN(x + dx), e' = b - K x'; x' is kept unless |e|/|e'| <= 1, and a
+ * ratio below stop_ratio ends the column.  Every residual row is
+ * RN(b_i - sum_j K_ij x_j), exact and rounded once (ExactRows::residual,
+ * 1486-1670), computed as one exact product [K | b] [-x; e_c] on the INT8
+ * residue engine.  h_all from sdpx_gpu_sampled_schur_dev; leaves[i].coupling
+ * is leaf i's B (K_i x t); c holds the border block C in its upper triangle
+ * (t x t, ld t; the input of sdpx_gpu_sampled_arrow_dev, not its output);
+ * l_all .. border_dinv are the factors sdpx_gpu_sampled_arrow_dev left.
+ * b (n x nrhs, ldb >= n, n = sum K_i + t) is read, x (ldx >= n) written;
+ * ok[c] is 1 when column c stayed finite, norms (nrhs records) receive the
+ * infinity norm of the returned column's residual, refinements the
+ * correction solves over all columns.  Workspace: (n + nrhs) n records for
+ * [K | b].  2: a product declined (x unspecified). */
+typedef struct {
+    uint64_t struct_size;        /* sizeof(sdpx_gpu_refine_settings) */
+    const void *reltol, *abstol; /* records of the solve's precision */
+    const void *stop_ratio;      /* record */
+    uint32_t max_iter;
+} sdpx_gpu_refine_settings;
+int32_t sdpx_gpu_sampled_kkt_solve_dev(uint32_t limbs, uint64_t nleaves, const sdpx_gpu_sampled_leaf *leaves,
+                                       uint64_t t, const void *h_all, const void *c, const void *l_all,
+                                       const void *dinv_all, const void *y_all, const void *border,
+                                       const void *border_dinv, const sdpx_gpu_refine_settings *settings,
+                                       uint64_t nrhs, const void *b, uint64_t ldb, void *x, uint64_t ldx,
+                                       int32_t *ok, void *norms, int64_t *refinements);
 
 /* Phase times (s: build, leaves, border), product counts, declined (1 when
  * the device pipeline declined and host panels produced the result),

@@ -806,6 +806,49 @@ typedef struct {
     const void *rinv, *basis, *weights, *coupling;
 } ora_leaf;
 
+/* One leaf's sampled Schur block: V = Rinv Q, Gram = V^T V (each entry
+ * exact and rounded once), H_pq = RN(RN(RN(w_p w_q) 1/2) RN(K_pq^2 + K_pq^2))
+ * (entry, sampled/mod.rs:1973-1988) in the lower triangle of H (K x K),
+ * zeros above. */
+static void ora_sampled_h(const ora_ctx *c, const ora_leaf *lf, uint8_t *H, int threads) {
+    const int nl = c->n;
+    size_t st = c->stride;
+    size_t h = lf->side, K = lf->samples;
+    uint8_t half[8 * 32 + 8];
+    memset(half, 0, st);
+    ((uint64_t *)half)[nl - 1] = 1ull << 63;
+    *(int32_t *)(half + 8 * nl) = KIND_REG;
+    *(int32_t *)(half + 8 * nl + 4) = 0;
+    uint8_t *V = malloc(st * h * K), *G = malloc(st * (K * (K + 1) / 2));
+    oracle_gemm_exact(nl, 'N', 'N', h, K, h, lf->rinv, h, lf->basis, h, 0, 0, V, threads);
+    oracle_gemm_exact(nl, 'T', 'N', K, K, h, V, h, V, h, 1, 1, G, threads);
+    for (size_t e = 0; e < K * K; e++) write_zero(H, st, e, nl);
+    for (size_t q = 0; q < K; q++)
+        for (size_t p = q; p < K; p++) {
+            uint8_t t1[8 * 32 + 8], t2[8 * 32 + 8], u1[8 * 32 + 8], u[8 * 32 + 8];
+            const uint8_t *kpq = AT(G, p * (p + 1) / 2 + q);
+            ora_mul(c, AT((const uint8_t *)lf->weights, p), AT((const uint8_t *)lf->weights, q), t1);
+            ora_mul(c, t1, half, t2);
+            ora_mul(c, kpq, kpq, u1);
+            {
+                const uint8_t *xs[2] = {u1, u1}, *ys[2] = {c->one, c->one};
+                ora_dot_ptrs(c, 2, xs, ys, u);
+            }
+            ora_mul(c, t2, u, AT(H, p + q * K));
+        }
+    free(V);
+    free(G);
+}
+
+/* The sampled Schur blocks alone: leaf b's H (lower triangle, zeros above)
+ * into hout[b]. */
+int oracle_sampled_schur(int nl, size_t nleaves, const ora_leaf *leaves, uint8_t **hout, int threads) {
+    ora_ctx cc;
+    ora_init(&cc, nl);
+    for (size_t b = 0; b < nleaves; b++) ora_sampled_h(&cc, &leaves[b], hout[b], threads);
+    return 1;
+}
+
 int oracle_sampled_arrow(int nl, size_t nleaves, const ora_leaf *leaves, size_t t, uint8_t *border,
                          const int8_t *bsigns, size_t nbsigns, uint8_t *border_dinv, uint8_t **lout,
                          uint8_t **dout, uint8_t **yout, int threads) {
@@ -813,11 +856,6 @@ int oracle_sampled_arrow(int nl, size_t nleaves, const ora_leaf *leaves, size_t 
     ora_init(&cc, nl);
     const ora_ctx *c = &cc;
     size_t st = c->stride;
-    uint8_t half[8 * 32 + 8];
-    memset(half, 0, st);
-    ((uint64_t *)half)[nl - 1] = 1ull << 63;
-    *(int32_t *)(half + 8 * nl) = KIND_REG;
-    *(int32_t *)(half + 8 * nl + 4) = 0;
     /* Leaves of order 256 and more factor recursively and solve Y by
      * recursive panels in the reference (factor_loaded, arrow.rs:819;
      * forward_panel_rows, arrow.rs:1596-1632); not restated here. */
@@ -825,27 +863,9 @@ int oracle_sampled_arrow(int nl, size_t nleaves, const ora_leaf *leaves, size_t 
         if (leaves[b].samples >= 256) return -1;
     for (size_t b = 0; b < nleaves; b++) {
         const ora_leaf *lf = &leaves[b];
-        size_t h = lf->side, K = lf->samples;
-        uint8_t *V = malloc(st * h * K), *G = malloc(st * (K * (K + 1) / 2));
-        oracle_gemm_exact(nl, 'N', 'N', h, K, h, lf->rinv, h, lf->basis, h, 0, 0, V, threads);
-        oracle_gemm_exact(nl, 'T', 'N', K, K, h, V, h, V, h, 1, 1, G, threads);
+        size_t K = lf->samples;
         uint8_t *H = calloc(K * K, st);
-        for (size_t e = 0; e < K * K; e++) write_zero(H, st, e, nl);
-        for (size_t q = 0; q < K; q++)
-            for (size_t p = q; p < K; p++) {
-                uint8_t t1[8 * 32 + 8], t2[8 * 32 + 8], u1[8 * 32 + 8], u[8 * 32 + 8];
-                const uint8_t *kpq = AT(G, p * (p + 1) / 2 + q);
-                ora_mul(c, AT((const uint8_t *)lf->weights, p), AT((const uint8_t *)lf->weights, q), t1);
-                ora_mul(c, t1, half, t2);
-                ora_mul(c, kpq, kpq, u1);
-                {
-                    const uint8_t *xs[2] = {u1, u1}, *ys[2] = {c->one, c->one};
-                    ora_dot_ptrs(c, 2, xs, ys, u);
-                }
-                ora_mul(c, t2, u, AT(H, p + q * K));
-            }
-        free(V);
-        free(G);
+        ora_sampled_h(c, lf, H, threads);
         uint8_t *dinv = malloc(st * K);
         long long reg = 0;
         int8_t plus = 1;
@@ -1033,6 +1053,222 @@ int oracle_sampled_arrow_solve(int nl, size_t nleaves, const uint64_t *ks, size_
         free(xs);
         free(ys);
     }
+    return 1;
+}
+
+/* ------------------------------------------------------------------------
+ * Iterative refinement of the sampled arrow KKT solve: DirectLDLKKTSolver::
+ * solve_many (direct/solver.rs:470-520) followed by its stationary refine
+ * (1046-1129) with the exact-rows residual (ExactRows::residual_negated,
+ * 1486-1670): K = [blockdiag(H_l) B; B^T C] with H_l the leaves' sampled
+ * Schur blocks (lower triangles, h_all), B_l = coupling (K_l x t each, in
+ * b_all leaf after leaf) and C the border block (upper triangle of c, t x t).
+ * Per column c:
+ *   x = K^-1 b (the arrow solve above); e = RN(b - K x) per row, one exact
+ *   dot over (b_i, 1) and (K_ij, -x_j); normb = |b|_inf, norme = |e|_inf
+ *   (infinite when e holds a NaN or infinity); ok = finite(norme)
+ *   repeat max_iter times, columns in lockstep:
+ *     a column stops once norme <= RN(abstol + RN(reltol normb));
+ *     dx = K^-1 e; dx = RN(x + dx); e = RN(b - K dx) (exact rows)
+ *     non-finite norme: ok = 0, stop; ratio = RN(last / norme);
+ *     ratio < stop_ratio stops the column, and keeps x when ratio <= 1;
+ *     otherwise x = dx.
+ * norms receive the residual norm of the returned x per column;
+ * *refinements the correction solves over all columns.
+ * ---------------------------------------------------------------------- */
+static int ora_cmp_abs(const ora_ctx *c, const uint8_t *a, const uint8_t *b) {
+    int ka = abs(*(const int32_t *)(a + 8 * c->n)), kb = abs(*(const int32_t *)(b + 8 * c->n));
+    int ca = ka == KIND_ZERO ? 0 : (ka == KIND_REG ? 1 : 2), cb = kb == KIND_ZERO ? 0 : (kb == KIND_REG ? 1 : 2);
+    if (ca != cb) return ca < cb ? -1 : 1;
+    if (ca != 1) return 0;
+    int32_t ea = *(const int32_t *)(a + 8 * c->n + 4), eb = *(const int32_t *)(b + 8 * c->n + 4);
+    if (ea != eb) return ea < eb ? -1 : 1;
+    for (int i = c->n - 1; i >= 0; i--) {
+        uint64_t x = ((const uint64_t *)a)[i], y = ((const uint64_t *)b)[i];
+        if (x != y) return x < y ? -1 : 1;
+    }
+    return 0;
+}
+/* sign of a - b, neither NaN (+0 == -0) */
+static int ora_cmp(const ora_ctx *c, const uint8_t *a, const uint8_t *b) {
+    int sa = ora_is_zero(c, a) ? 0 : (ora_is_neg(c, a) ? -1 : 1);
+    int sb = ora_is_zero(c, b) ? 0 : (ora_is_neg(c, b) ? -1 : 1);
+    if (sa != sb) return sa < sb ? -1 : 1;
+    return sa >= 0 ? ora_cmp_abs(c, a, b) : -ora_cmp_abs(c, a, b);
+}
+/* |v|_inf of n values, +infinity when one is NaN or infinite */
+static void ora_norm_inf(const ora_ctx *c, const uint8_t *v, size_t n, uint8_t *out) {
+    size_t st = c->stride;
+    write_zero(out, st, 0, c->n);
+    for (size_t i = 0; i < n; i++) {
+        const uint8_t *x = AT(v, i);
+        if (!ora_is_finite(c, x)) {
+            memset(out, 0, st);
+            *(int32_t *)(out + 8 * c->n) = KIND_INF;
+            return;
+        }
+        if (ora_cmp_abs(c, x, out) > 0) {
+            memcpy(out, x, st);
+            *(int32_t *)(out + 8 * c->n) = abs(*(int32_t *)(out + 8 * c->n));
+        }
+    }
+}
+
+typedef struct {
+    size_t nleaves, t, ksum, n;
+    const uint64_t *ks;
+    const uint8_t *h_all, *b_all, *cmat;
+} ora_kkt;
+
+/* e = RN(b - K x) row by row (x, b, e: n values each) */
+static void ora_kkt_residual(const ora_ctx *c, const ora_kkt *k, const uint8_t *b, const uint8_t *x, uint8_t *e,
+                             int threads) {
+    size_t st = c->stride, n = k->n, t = k->t, ksum = k->ksum;
+    uint8_t *neg = malloc(st * n);
+    memcpy(neg, x, st * n);
+    for (size_t i = 0; i < n; i++) ora_neg(c, AT(neg, i));
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads > 0 ? threads : 1)
+    for (long long ii = 0; ii < (long long)n; ii++) {
+        size_t i = (size_t)ii;
+        const uint8_t **xs = malloc(sizeof(uint8_t *) * (n + 1)), **ys = malloc(sizeof(uint8_t *) * (n + 1));
+        size_t m = 0;
+        xs[m] = AT(b, i);
+        ys[m++] = c->one;
+        if (i < ksum) {
+            size_t l = 0, ko = 0, ho = 0;
+            while (i >= ko + k->ks[l]) {
+                ko += k->ks[l];
+                ho += k->ks[l] * k->ks[l];
+                l++;
+            }
+            size_t K = k->ks[l], r = i - ko;
+            const uint8_t *H = k->h_all + ho * st;
+            for (size_t q = 0; q < K; q++) {
+                xs[m] = r >= q ? AT(H, r + q * K) : AT(H, q + r * K);
+                ys[m++] = AT(neg, ko + q);
+            }
+            const uint8_t *B = k->b_all + ko * t * st;
+            for (size_t j = 0; j < t; j++) {
+                xs[m] = AT(B, r + j * K);
+                ys[m++] = AT(neg, ksum + j);
+            }
+        } else {
+            size_t j = i - ksum, ko = 0;
+            for (size_t l = 0; l < k->nleaves; l++) {
+                size_t K = k->ks[l];
+                const uint8_t *B = k->b_all + ko * t * st;
+                for (size_t r = 0; r < K; r++) {
+                    xs[m] = AT(B, r + j * K);
+                    ys[m++] = AT(neg, ko + r);
+                }
+                ko += K;
+            }
+            for (size_t q = 0; q < t; q++) {
+                xs[m] = j <= q ? AT(k->cmat, j + q * t) : AT(k->cmat, q + j * t);
+                ys[m++] = AT(neg, ksum + q);
+            }
+        }
+        ora_dot_ptrs(c, (int)m, xs, ys, AT(e, i));
+        free(xs);
+        free(ys);
+    }
+    free(neg);
+}
+
+/* e = RN(b - K x) of one point (the exact-rows residual alone). */
+void oracle_sampled_residual(int nl, size_t nleaves, const uint64_t *ks, size_t t, const uint8_t *h_all,
+                             const uint8_t *b_all, const uint8_t *cmat, const uint8_t *b, const uint8_t *x, uint8_t *e,
+                             int threads) {
+    ora_ctx cc;
+    ora_init(&cc, nl);
+    size_t ksum = 0;
+    for (size_t l = 0; l < nleaves; l++) ksum += ks[l];
+    ora_kkt k = {nleaves, t, ksum, ksum + t, ks, h_all, b_all, cmat};
+    ora_kkt_residual(&cc, &k, b, x, e, threads);
+}
+
+int oracle_sampled_refine(int nl, size_t nleaves, const uint64_t *ks, size_t t, const uint8_t *h_all,
+                          const uint8_t *b_all, const uint8_t *cmat, const uint8_t *l_all, const uint8_t *dinv_all,
+                          const uint8_t *y_all, const uint8_t *border, const uint8_t *border_dinv,
+                          const uint8_t *reltol, const uint8_t *abstol, const uint8_t *stop_ratio,
+                          unsigned max_iter, size_t nrhs, const uint8_t *b, uint8_t *x, int *ok, uint8_t *norms,
+                          long long *refinements, int threads) {
+    ora_ctx cc;
+    ora_init(&cc, nl);
+    const ora_ctx *c = &cc;
+    size_t st = c->stride, ksum = 0;
+    for (size_t l = 0; l < nleaves; l++) ksum += ks[l];
+    ora_kkt k = {nleaves, t, ksum, ksum + t, ks, h_all, b_all, cmat};
+    size_t n = k.n;
+    *refinements = 0;
+    if (n == 0 || nrhs == 0) return 1;
+    memcpy(x, b, st * n * nrhs);
+    oracle_sampled_arrow_solve(nl, nleaves, ks, t, l_all, dinv_all, y_all, border, border_dinv, nrhs, x, n, threads);
+    uint8_t *e = malloc(st * n * nrhs), *dx = malloc(st * n * nrhs);
+    uint8_t *normb = malloc(st * nrhs), *norme = malloc(st * nrhs), *last = malloc(st * nrhs);
+    uint8_t *tol = malloc(st * nrhs);
+    int *active = malloc(sizeof(int) * nrhs);
+    for (size_t col = 0; col < nrhs; col++) {
+        uint8_t t1[8 * 32 + 8];
+        ora_norm_inf(c, b + col * n * st, n, AT(normb, col));
+        ora_mul(c, reltol, AT(normb, col), t1);
+        const uint8_t *xs[2] = {abstol, t1}, *ys[2] = {c->one, c->one};
+        ora_dot_ptrs(c, 2, xs, ys, AT(tol, col));
+        ora_kkt_residual(c, &k, b + col * n * st, x + col * n * st, e + col * n * st, threads);
+        ora_norm_inf(c, e + col * n * st, n, AT(norme, col));
+        memcpy(AT(norms, col), AT(norme, col), st);
+        ok[col] = active[col] = ora_is_finite(c, AT(norme, col));
+    }
+    for (unsigned it = 0; it < max_iter; it++) {
+        int any = 0;
+        for (size_t col = 0; col < nrhs; col++) {
+            if (active[col] && ora_cmp(c, AT(norme, col), AT(tol, col)) <= 0) active[col] = 0;
+            any |= active[col];
+        }
+        if (!any) break;
+        memcpy(last, norme, st * nrhs);
+        for (size_t col = 0; col < nrhs; col++) {
+            if (!active[col]) continue;
+            (*refinements)++;
+            uint8_t *d = dx + col * n * st;
+            memcpy(d, e + col * n * st, st * n);
+            oracle_sampled_arrow_solve(nl, nleaves, ks, t, l_all, dinv_all, y_all, border, border_dinv, 1, d, n,
+                                       threads);
+            for (size_t i = 0; i < n; i++) {
+                const uint8_t *xs[2] = {AT(x + col * n * st, i), AT(d, i)}, *ys[2] = {c->one, c->one};
+                uint8_t r[8 * 32 + 8];
+                ora_dot_ptrs(c, 2, xs, ys, r);
+                memcpy(AT(d, i), r, st);
+            }
+            ora_kkt_residual(c, &k, b + col * n * st, d, e + col * n * st, threads);
+            ora_norm_inf(c, e + col * n * st, n, AT(norme, col));
+            if (!ora_is_finite(c, AT(norme, col))) {
+                ok[col] = active[col] = 0;
+                continue;
+            }
+            /* ratio = RN(last / norme); a zero norm gives +infinity, above every bound */
+            int below_stop = 0, not_better = 0;
+            if (!ora_is_zero(c, AT(norme, col))) {
+                uint8_t ratio[8 * 32 + 8];
+                ora_div(c, AT(last, col), AT(norme, col), ratio);
+                below_stop = ora_cmp(c, ratio, stop_ratio) < 0;
+                not_better = ora_cmp(c, ratio, c->one) <= 0;
+            }
+            if (below_stop) {
+                active[col] = 0;
+                if (not_better) continue;
+            }
+            memcpy(x + col * n * st, d, st * n);
+            memcpy(AT(norms, col), AT(norme, col), st);
+        }
+    }
+    free(e);
+    free(dx);
+    free(normb);
+    free(norme);
+    free(last);
+    free(tol);
+    free(active);
     return 1;
 }
 

@@ -359,6 +359,74 @@ int32_t sdpx_gpu_sampled_arrow_solve_dev(uint32_t limbs, uint64_t nleaves, const
     });
 }
 
+int32_t sdpx_gpu_sampled_schur_dev(uint32_t limbs, uint64_t nleaves, const sdpx_gpu_sampled_leaf *leaves,
+                                   void *h_all) {
+    return sdpx::abi_boundary([&]() -> int32_t {
+        if (!sdpx::mp_kernels_support((int)limbs))
+            return sdpx::abi_invalid("the sampled Schur build supports 128, 256 and 512 bits");
+        if (nleaves && (!leaves || !h_all)) return sdpx::abi_invalid("null pointer");
+        std::vector<sdpx::SampledLeafDev> in(nleaves);
+        for (uint64_t i = 0; i < nleaves; i++) {
+            const auto &l = leaves[i];
+            if (l.struct_size != sizeof(sdpx_gpu_sampled_leaf))
+                return sdpx::abi_invalid("leaf struct_size is not sizeof(sdpx_gpu_sampled_leaf)");
+            if (!l.rinv || !l.basis || !l.weights) return sdpx::abi_invalid("null leaf array");
+            in[i] = sdpx::SampledLeafDev{l.side, l.samples, (const uint8_t *)l.rinv, (const uint8_t *)l.basis,
+                                         (const uint8_t *)l.weights, (const uint8_t *)l.coupling};
+        }
+        if (nleaves == 0) return SDPX_GPU_OK;
+        sdpx::Device &d = sdpx::device();
+        std::lock_guard<std::recursive_mutex> g(d.mu);
+        return sdpx::sampled_schur_device((int)limbs, in, (uint8_t *)h_all) < 0 ? SDPX_GPU_DECLINED : SDPX_GPU_OK;
+    });
+}
+
+int32_t sdpx_gpu_sampled_kkt_solve_dev(uint32_t limbs, uint64_t nleaves, const sdpx_gpu_sampled_leaf *leaves,
+                                       uint64_t t, const void *h_all, const void *c, const void *l_all,
+                                       const void *dinv_all, const void *y_all, const void *border,
+                                       const void *border_dinv, const sdpx_gpu_refine_settings *settings,
+                                       uint64_t nrhs, const void *b, uint64_t ldb, void *x, uint64_t ldx,
+                                       int32_t *ok, void *norms, int64_t *refinements) {
+    return sdpx::abi_boundary([&]() -> int32_t {
+        if (!sdpx::mp_kernels_support((int)limbs))
+            return sdpx::abi_invalid("the refined arrow solve supports 128, 256 and 512 bits");
+        if (!settings || settings->struct_size != sizeof(sdpx_gpu_refine_settings))
+            return sdpx::abi_invalid("settings struct_size is not sizeof(sdpx_gpu_refine_settings)");
+        if (!settings->reltol || !settings->abstol || !settings->stop_ratio)
+            return sdpx::abi_invalid("null refinement tolerance");
+        if (nleaves && !leaves) return sdpx::abi_invalid("null leaves");
+        uint64_t n = t;
+        std::vector<size_t> ks(nleaves);
+        std::vector<const uint8_t *> coupling(nleaves);
+        for (uint64_t i = 0; i < nleaves; i++) {
+            if (leaves[i].struct_size != sizeof(sdpx_gpu_sampled_leaf))
+                return sdpx::abi_invalid("leaf struct_size is not sizeof(sdpx_gpu_sampled_leaf)");
+            if (t && !leaves[i].coupling) return sdpx::abi_invalid("null leaf coupling");
+            ks[i] = leaves[i].samples;
+            coupling[i] = (const uint8_t *)leaves[i].coupling;
+            n += leaves[i].samples;
+        }
+        if (ldb < n || ldx < n) return sdpx::abi_invalid("ldb or ldx below the system order");
+        if (nrhs && n && (!b || !x)) return sdpx::abi_invalid("null right-hand sides or solutions");
+        if (nleaves && (!h_all || !l_all || !dinv_all || (t && !y_all))) return sdpx::abi_invalid("null leaf array");
+        if (t && (!c || !border || !border_dinv)) return sdpx::abi_invalid("null border array");
+        if (refinements) *refinements = 0;
+        if (nrhs == 0 || n == 0) return SDPX_GPU_OK;
+        sdpx::Device &d = sdpx::device();
+        std::lock_guard<std::recursive_mutex> g(d.mu);
+        sdpx::RefineSettings rs;
+        rs.reltol = (const uint8_t *)settings->reltol;
+        rs.abstol = (const uint8_t *)settings->abstol;
+        rs.stop_ratio = (const uint8_t *)settings->stop_ratio;
+        rs.max_iter = settings->max_iter;
+        const int r = sdpx::sampled_kkt_solve_device(
+            (int)limbs, ks, t, (const uint8_t *)h_all, coupling, (const uint8_t *)c, (const uint8_t *)l_all,
+            (const uint8_t *)dinv_all, (const uint8_t *)y_all, (const uint8_t *)border, (const uint8_t *)border_dinv,
+            rs, nrhs, (const uint8_t *)b, ldb, (uint8_t *)x, ldx, ok, (uint8_t *)norms, refinements);
+        return r < 0 ? SDPX_GPU_DECLINED : SDPX_GPU_OK;
+    });
+}
+
 int32_t sdpx_gpu_panel_phase_times(double *ms4, int64_t *launches4, int32_t reset) {
     return sdpx::abi_boundary([&]() -> int32_t {
         sdpx::panel_phase_times(ms4, launches4, reset != 0);

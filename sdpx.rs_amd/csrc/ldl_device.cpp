@@ -14,6 +14,7 @@
 #include "ldl.h"
 #include "ldl_device.h"
 #include "mp_kernels.h"
+#include "mpx.h"
 #include "rns_gemm.h"
 #include "sampled.h"
 
@@ -397,6 +398,39 @@ int factor_loaded_device(DeviceSetup &ds, uint8_t *l, size_t ld, size_t n, uint8
     return 1;
 }
 
+// The sampled Schur build of every leaf (SampledSchurWorkspace::
+// update_with_pool and entry, sampled/mod.rs:1836-2143, 1973-1988):
+// V = Rinv Q (every leaf, one host wait), Gram = V^T V (one wait), then the
+// entries H_pq = RN(RN(w_p w_q) RN(K_pq^2)) into leaf i's K x K block at
+// record loff[i] of d_h, lower triangle, zeros above.
+void build_leaves(DCtx &c, const std::vector<SampledLeafDev> &leaves, const std::vector<size_t> &loff, uint8_t *d_h) {
+    const size_t vb = c.vb;
+    const size_t mark = c.arena->top;
+    std::vector<uint8_t *> Vs(leaves.size()), Gs(leaves.size());
+    std::vector<RnsGemmArgs> vprod, gram;
+    for (size_t i = 0; i < leaves.size(); i++) {
+        const auto &lf = leaves[i];
+        Vs[i] = (uint8_t *)c.arena->alloc(lf.h * lf.k * vb);
+        Gs[i] = (uint8_t *)c.arena->alloc(lf.k * (lf.k + 1) / 2 * vb);
+        vprod.push_back(RnsGemmArgs{c.n, 'N', 'N', (int64_t)lf.h, (int64_t)lf.k, (int64_t)lf.h, lf.rinv,
+                                    (int64_t)lf.h, lf.q, (int64_t)lf.h, false, (uint64_t)(lf.h * lf.k), Vs[i]});
+        gram.push_back(RnsGemmArgs{c.n, 'T', 'N', (int64_t)lf.k, (int64_t)lf.k, (int64_t)lf.h, Vs[i], (int64_t)lf.h,
+                                   Vs[i], (int64_t)lf.h, true, (uint64_t)(lf.k * (lf.k + 1) / 2), Gs[i]});
+    }
+    products_batch(c, vprod, nullptr);
+    products_batch(c, gram, [&](size_t i) {
+        mp_sampled_entries(c.n, Gs[i], leaves[i].w, (int64_t)leaves[i].k, c.at(d_h, loff[i]), c.st);
+    });
+    c.arena->top = mark;
+}
+
+// arena bytes of build_leaves: V and the packed Gram of every leaf
+size_t build_arena_bytes(const std::vector<SampledLeafDev> &leaves, size_t vb) {
+    size_t vals = 0;
+    for (auto &l : leaves) vals += l.h * l.k + l.k * (l.k + 1) / 2 + 2 * 256 / vb + 2;
+    return vals * vb;
+}
+
 }  // namespace
 
 void panel_phase_times(double *ms, int64_t *launches, bool reset) {
@@ -541,28 +575,8 @@ int sampled_arrow_device(int limbs, const std::vector<SampledLeafDev> &leaves, s
     hipEvent_t ev[4];
     for (auto &e : ev) HIP_CHECK(hipEventCreate(&e));
     HIP_CHECK(hipEventRecord(ev[0], s));
-    // 1. Schur build of every leaf: V = Rinv Q (every leaf, one host wait),
-    // Gram = V^T V (one wait), then the sampled entries
-    {
-        const size_t mark = c.arena->top;
-        std::vector<uint8_t *> Vs(leaves.size()), Gs(leaves.size());
-        std::vector<RnsGemmArgs> vprod, gram;
-        for (size_t i = 0; i < leaves.size(); i++) {
-            const auto &lf = leaves[i];
-            Vs[i] = (uint8_t *)c.arena->alloc(lf.h * lf.k * vb);
-            Gs[i] = (uint8_t *)c.arena->alloc(lf.k * (lf.k + 1) / 2 * vb);
-            vprod.push_back(RnsGemmArgs{limbs, 'N', 'N', (int64_t)lf.h, (int64_t)lf.k, (int64_t)lf.h, lf.rinv,
-                                        (int64_t)lf.h, lf.q, (int64_t)lf.h, false, (uint64_t)(lf.h * lf.k), Vs[i]});
-            gram.push_back(RnsGemmArgs{limbs, 'T', 'N', (int64_t)lf.k, (int64_t)lf.k, (int64_t)lf.h, Vs[i],
-                                       (int64_t)lf.h, Vs[i], (int64_t)lf.h, true,
-                                       (uint64_t)(lf.k * (lf.k + 1) / 2), Gs[i]});
-        }
-        products_batch(c, vprod, nullptr);
-        products_batch(c, gram, [&](size_t i) {
-            mp_sampled_entries(limbs, Gs[i], leaves[i].w, (int64_t)leaves[i].k, c.at(d_l_all, loff[i]), s);
-        });
-        c.arena->top = mark;
-    }
+    // 1. Schur build of every leaf
+    build_leaves(c, leaves, loff, d_l_all);
     HIP_CHECK(hipEventRecord(ev[1], s));
     if (!c.declined) {
         // 2. every leaf factor in one launch, one workgroup per leaf
@@ -672,6 +686,174 @@ int sampled_arrow_device(int limbs, const std::vector<SampledLeafDev> &leaves, s
     st.gpu_products += ls.gpu_products;
     st.retried += ls.retried;
     return border_result;
+}
+
+int sampled_schur_device(int limbs, const std::vector<SampledLeafDev> &leaves, uint8_t *d_h_all) {
+    const size_t vb = value_bytes(limbs);
+    std::vector<size_t> loff;
+    size_t hs = 0;
+    for (auto &l : leaves) {
+        loff.push_back(hs);
+        hs += l.k * l.k;
+    }
+    LdlStats ls;
+    DeviceSetup ds(limbs, build_arena_bytes(leaves, vb) + (64u << 20), 4096, nullptr, 0, nullptr, nullptr, &ls);
+    build_leaves(ds.ctx, leaves, loff, d_h_all);
+    HIP_CHECK(hipStreamSynchronize(ds.ctx.st));
+    return ds.ctx.declined ? -1 : 1;
+}
+
+namespace {
+enum { WS_RF_A = 160, WS_RF_X, WS_RF_E, WS_RF_P, WS_RF_R, WS_RF_N };
+}
+
+int sampled_kkt_solve_device(int limbs, const std::vector<size_t> &ks, size_t t, const uint8_t *d_h_all,
+                             const std::vector<const uint8_t *> &d_coupling, const uint8_t *d_c,
+                             const uint8_t *d_l_all, const uint8_t *d_dinv_all, const uint8_t *d_y_all,
+                             const uint8_t *d_border, const uint8_t *d_border_dinv, const RefineSettings &rs,
+                             size_t nrhs, const uint8_t *d_b, size_t ldb, uint8_t *d_x, size_t ldx, int32_t *ok,
+                             uint8_t *norms, int64_t *refinements) {
+    const size_t vb = value_bytes(limbs);
+    size_t ksum = 0, hs = 0;
+    std::vector<size_t> koff, hoff;
+    for (size_t K : ks) {
+        koff.push_back(ksum);
+        hoff.push_back(hs);
+        ksum += K;
+        hs += K * K;
+    }
+    const size_t n = ksum + t, kk = n + nrhs;
+    if (refinements) *refinements = 0;
+    if (nrhs == 0 || n == 0) return 1;
+    Device &dev = device();
+    hipStream_t s = dev.stream;
+    auto at = [vb](const uint8_t *p, size_t i) { return (uint8_t *)p + i * vb; };
+    // [K | B]: the unshifted arrow KKT (n x n, both triangles) with the
+    // right-hand sides as nrhs more columns, so that with X = [-p; e_c] one
+    // exact product gives every row's b_ic - sum_j K_ij p_j rounded once
+    uint8_t *A = (uint8_t *)dev.ws.get(WS_RF_A, n * kk * vb);
+    uint8_t *X = (uint8_t *)dev.ws.get(WS_RF_X, kk * nrhs * vb);
+    uint8_t *E = (uint8_t *)dev.ws.get(WS_RF_E, n * nrhs * vb);  // residual of column c at c n
+    uint8_t *P = (uint8_t *)dev.ws.get(WS_RF_P, n * nrhs * vb);  // corrections, then trial points
+    uint8_t *R = (uint8_t *)dev.ws.get(WS_RF_R, n * nrhs * vb);  // product output
+    uint8_t *dn = (uint8_t *)dev.ws.get(WS_RF_N, (nrhs + 1) * vb);
+    uint8_t *one_dev = at(dn, nrhs);
+    std::vector<uint8_t> one(vb, 0);
+    ((uint64_t *)one.data())[limbs - 1] = 1ULL << 63;
+    *(int32_t *)(one.data() + 8 * limbs) = mpx::KIND_REG;
+    *(int32_t *)(one.data() + 8 * limbs + 4) = 1;
+    upload_async(one_dev, one.data(), vb);
+    mp_zero_values(limbs, A, (int64_t)(n * kk), s);
+    for (size_t l = 0; l < ks.size(); l++) {
+        const size_t K = ks[l], o = koff[l];
+        if (K == 0) continue;
+        HIP_CHECK(hipMemcpy2DAsync(at(A, o + o * n), n * vb, at(d_h_all, hoff[l]), K * vb, K * vb, K,
+                                   hipMemcpyDeviceToDevice, s));
+        mp_mirror_lower(limbs, at(A, o + o * n), (int64_t)K, (int64_t)n, s);
+        if (t) {
+            HIP_CHECK(hipMemcpy2DAsync(at(A, o + ksum * n), n * vb, d_coupling[l], K * vb, K * vb, t,
+                                       hipMemcpyDeviceToDevice, s));
+            mp_transpose(limbs, d_coupling[l], (int64_t)K, (int64_t)K, (int64_t)t, at(A, ksum + o * n), (int64_t)n, s);
+        }
+    }
+    if (t) {
+        HIP_CHECK(hipMemcpy2DAsync(at(A, ksum + ksum * n), n * vb, d_c, t * vb, t * vb, t, hipMemcpyDeviceToDevice, s));
+        mp_mirror_upper(limbs, at(A, ksum + ksum * n), (int64_t)t, (int64_t)n, s);
+    }
+    HIP_CHECK(hipMemcpy2DAsync(at(A, n * n), n * vb, d_b, ldb * vb, n * vb, nrhs, hipMemcpyDeviceToDevice, s));
+
+    std::vector<uint8_t> hn(nrhs * vb), tmp(nrhs * vb);
+    auto fetch_norms = [&](size_t m) {
+        HIP_CHECK(hipMemcpyAsync(tmp.data(), dn, m * vb, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+    };
+    // ExactRows::residual of points Q[:, j] (ld ldq) for columns cols[j]:
+    // E[:, cols[j]] = RN(b_c - K Q[:, j]), their infinity norms into hn;
+    // false when the product declines
+    auto residual = [&](const uint8_t *Q, size_t ldq, const std::vector<size_t> &cols) -> bool {
+        const size_t m = cols.size();
+        mp_zero_values(limbs, X, (int64_t)(kk * m), s);
+        for (size_t j = 0; j < m; j++) {
+            mp_axpby(limbs, at(Q, j * ldq), (int64_t)ldq, (int64_t)n, 1, 2, nullptr, 0, nullptr, at(X, j * kk),
+                     (int64_t)kk, 0, false, false, s);
+            HIP_CHECK(hipMemcpyAsync(at(X, n + cols[j] + j * kk), one_dev, vb, hipMemcpyDeviceToDevice, s));
+        }
+        RnsGemmArgs g{limbs, 'N', 'N', (int64_t)n, (int64_t)m, (int64_t)kk, A, (int64_t)n, X, (int64_t)kk,
+                      false, (uint64_t)(n * m), R};
+        const int r = rns_gemm_device(g);
+        if (r == SDPX_GPU_DECLINED) return false;
+        if (r != SDPX_GPU_OK) throw Error(r, "residual product failed");
+        for (size_t j = 0; j < m; j++)
+            HIP_CHECK(hipMemcpyAsync(at(E, cols[j] * n), at(R, j * n), n * vb, hipMemcpyDeviceToDevice, s));
+        mp_col_absmax(limbs, R, (int64_t)n, (int64_t)n, (int64_t)m, dn, s);
+        fetch_norms(m);
+        for (size_t j = 0; j < m; j++) std::memcpy(&hn[cols[j] * vb], &tmp[j * vb], vb);
+        return true;
+    };
+    auto view = [&](const std::vector<uint8_t> &v, size_t c) { return mpx::View{limbs, v.data() + c * vb}; };
+
+    // x = K^-1 b (DirectLDLKKTSolver::solve_many), then the refinement
+    HIP_CHECK(hipMemcpy2DAsync(d_x, ldx * vb, d_b, ldb * vb, n * vb, nrhs, hipMemcpyDeviceToDevice, s));
+    if (sampled_arrow_solve_device(limbs, ks, t, d_l_all, d_dinv_all, d_y_all, d_border, d_border_dinv, nrhs, d_x,
+                                   ldx) < 0)
+        return -1;
+    mp_col_absmax(limbs, d_b, (int64_t)ldb, (int64_t)n, (int64_t)nrhs, dn, s);
+    fetch_norms(nrhs);
+    std::vector<uint8_t> tol(nrhs * vb), last(nrhs * vb), xnorm(nrhs * vb), t1(vb), ratio(vb);
+    for (size_t c = 0; c < nrhs; c++) {  // abstol + reltol * normb
+        mpx::mul(mpx::View{limbs, rs.reltol}, mpx::View{limbs, &tmp[c * vb]}, t1.data());
+        mpx::add(mpx::View{limbs, rs.abstol}, mpx::View{limbs, t1.data()}, &tol[c * vb]);
+    }
+    std::vector<size_t> all(nrhs);
+    for (size_t c = 0; c < nrhs; c++) all[c] = c;
+    if (!residual(d_x, ldx, all)) return -1;
+    xnorm = hn;
+    std::vector<char> okv(nrhs), active(nrhs);
+    for (size_t c = 0; c < nrhs; c++) okv[c] = active[c] = view(hn, c).finite();
+    const mpx::View stop{limbs, rs.stop_ratio}, unit{limbs, one.data()};
+    int64_t nref = 0;
+    for (uint32_t it = 0; it < rs.max_iter; it++) {
+        std::vector<size_t> act;
+        for (size_t c = 0; c < nrhs; c++) {
+            if (active[c] && mpx::cmp(view(hn, c), view(tol, c)) <= 0) active[c] = 0;
+            if (active[c]) act.push_back(c);
+        }
+        if (act.empty()) break;
+        last = hn;
+        nref += (int64_t)act.size();
+        const size_t m = act.size();
+        for (size_t j = 0; j < m; j++)
+            HIP_CHECK(hipMemcpyAsync(at(P, j * n), at(E, act[j] * n), n * vb, hipMemcpyDeviceToDevice, s));
+        if (sampled_arrow_solve_device(limbs, ks, t, d_l_all, d_dinv_all, d_y_all, d_border, d_border_dinv, m, P,
+                                       n) < 0)
+            return -1;
+        for (size_t j = 0; j < m; j++)  // the prospective solution x + dx
+            mp_axpby(limbs, at(d_x, act[j] * ldx), (int64_t)ldx, (int64_t)n, 1, 1, nullptr, 1, nullptr, at(P, j * n),
+                     (int64_t)n, 0, false, false, s);
+        if (!residual(P, n, act)) return -1;
+        for (size_t j = 0; j < m; j++) {
+            const size_t c = act[j];
+            if (!view(hn, c).finite()) {
+                okv[c] = active[c] = 0;
+                continue;
+            }
+            mpx::div(view(last, c), view(hn, c), ratio.data());
+            const mpx::View r{limbs, ratio.data()};
+            if (mpx::cmp(r, stop) < 0) {
+                active[c] = 0;
+                if (mpx::cmp(r, unit) <= 0) continue;
+            }
+            HIP_CHECK(hipMemcpyAsync(at(d_x, c * ldx), at(P, j * n), n * vb, hipMemcpyDeviceToDevice, s));
+            std::memcpy(&xnorm[c * vb], &hn[c * vb], vb);
+        }
+    }
+    HIP_CHECK(hipStreamSynchronize(s));
+    for (size_t c = 0; c < nrhs; c++) {
+        if (ok) ok[c] = okv[c];
+        if (norms) std::memcpy(norms + c * vb, &xnorm[c * vb], vb);
+    }
+    if (refinements) *refinements = nref;
+    return 1;
 }
 
 }  // namespace sdpx

@@ -45,4 +45,35 @@ int sampled_arrow_solve_device(int limbs, const std::vector<size_t> &ks, size_t 
                                const uint8_t *d_dinv_all, const uint8_t *d_y_all, const uint8_t *d_border,
                                const uint8_t *d_border_dinv, size_t nrhs, uint8_t *d_x, size_t ldx);
 
+// The sampled Schur build alone (SampledSchurWorkspace::update_with_pool and
+// entry, sampled/mod.rs:1836-2143, 1973-1988): leaf i's H (K x K, lower
+// triangle, zeros above) at record sum_{j<i} K_j^2 of d_h_all, the same
+// values sampled_arrow_device factors.  1 ok, -1 a product declined.
+int sampled_schur_device(int limbs, const std::vector<SampledLeafDev> &leaves, uint8_t *d_h_all);
+
+// Iterative-refinement settings (host records of the solve's precision):
+// the reference's iterative_refinement_{reltol, abstol, stop_ratio, max_iter}
+struct RefineSettings {
+    const uint8_t *reltol = nullptr, *abstol = nullptr, *stop_ratio = nullptr;
+    uint32_t max_iter = 0;
+};
+
+// DirectLDLKKTSolver::solve_many with stationary refinement (direct/
+// solver.rs:470-520, refine 1046-1129) on the sampled arrow KKT
+// K = [blockdiag(H_i) B; B^T C]: x = K^-1 b by sampled_arrow_solve_device,
+// then per column the reference's loop, every residual RN(b - K x) exact and
+// rounded once per row (ExactRows::residual, 1486-1670).  d_h_all as from
+// sampled_schur_device, d_coupling[i] leaf i's B (K_i x t), d_c the border
+// block C in its upper triangle (t x t), the factors as sampled_arrow_device
+// left them.  b (n x nrhs, ldb) is read, x (ldx) written; ok[c] (host) the
+// column's finiteness, norms (host, nrhs records) the infinity norm of the
+// returned x's residual, refinements the correction solves summed over the
+// columns.  1 ok, -1 a product declined (x unspecified).
+int sampled_kkt_solve_device(int limbs, const std::vector<size_t> &ks, size_t t, const uint8_t *d_h_all,
+                             const std::vector<const uint8_t *> &d_coupling, const uint8_t *d_c,
+                             const uint8_t *d_l_all, const uint8_t *d_dinv_all, const uint8_t *d_y_all,
+                             const uint8_t *d_border, const uint8_t *d_border_dinv, const RefineSettings &rs,
+                             size_t nrhs, const uint8_t *d_b, size_t ldb, uint8_t *d_x, size_t ldx, int32_t *ok,
+                             uint8_t *norms, int64_t *refinements);
+
 }  // namespace sdpx

@@ -65,6 +65,11 @@ void mp_transpose(int limbs, const void *in, int64_t ldi, int64_t m, int64_t n, 
                   int64_t ldo, hipStream_t st);
 void mp_envelope(int limbs, const void *A, int64_t ld, int64_t n, int32_t *first, hipStream_t st);
 void mp_mirror_upper(int limbs, void *S, int64_t n, int64_t ld, hipStream_t st);
+// the strictly lower triangle of an n x n block copied onto its upper one
+void mp_mirror_lower(int limbs, void *S, int64_t n, int64_t ld, hipStream_t st);
+// out[c] (ncols records) = max_i |X[i + c ldx]| over m rows, +infinity when
+// the column holds a NaN or an infinity (norm_inf of each column)
+void mp_col_absmax(int limbs, const void *X, int64_t ldx, int64_t m, int64_t ncols, void *out, hipStream_t st);
 // FMA sweeps of the arrow solves (DenseLeaf forward_many / backward order):
 // descs (device) lists ndesc factors, each an order-n unit lower L (ld n) at
 // record l_off of L_all and right-hand sides x (column c at record x_off +

@@ -2314,6 +2314,74 @@ __global__ void mirror_upper_kernel(uint32_t *S, int64_t n, int64_t ld) {
     for (int u = 0; u < W + 2; u++) dst[u] = src[u];
 }
 
+template <int W>
+__global__ void mirror_lower_kernel(uint32_t *S, int64_t n, int64_t ld) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * n) return;
+    const int64_t i = e % n, j = e / n;
+    if (i <= j) return;
+    const uint32_t *src = rec_at<W>((const uint32_t *)S, i + j * ld);
+    uint32_t *dst = rec_at<W>(S, j + i * ld);
+    for (int u = 0; u < W + 2; u++) dst[u] = src[u];
+}
+
+// |a| > |b| for finite records (zeros of either sign smallest)
+template <int W>
+__device__ __forceinline__ bool abs_greater(const uint32_t *a, const uint32_t *b) {
+    const int32_t ka = (int32_t)a[W], kb = (int32_t)b[W];
+    if (ka == mpd::KIND_ZERO || ka == -mpd::KIND_ZERO) return false;
+    if (kb == mpd::KIND_ZERO || kb == -mpd::KIND_ZERO) return true;
+    const int32_t ea = (int32_t)a[W + 1], eb = (int32_t)b[W + 1];
+    if (ea != eb) return ea > eb;
+    for (int i = W - 1; i >= 0; i--)
+        if (a[i] != b[i]) return a[i] > b[i];
+    return false;
+}
+
+// out[c] = max_i |X[i + c ldx]| over m rows (the infinity norm of column c),
+// +infinity when the column holds a NaN or an infinity; one workgroup of
+// 256 threads per column, the largest entry's index reduced in LDS.
+template <int W>
+__global__ void __launch_bounds__(256) col_absmax_kernel(const uint32_t *X, int64_t ldx, int64_t m, uint32_t *out) {
+    __shared__ int64_t best[256];
+    __shared__ int nonfinite;
+    const uint32_t *col = rec_at<W>(X, (int64_t)blockIdx.x * ldx);
+    if (threadIdx.x == 0) nonfinite = 0;
+    __syncthreads();
+    int64_t b = -1;
+    for (int64_t i = threadIdx.x; i < m; i += blockDim.x) {
+        const uint32_t *r = rec_at<W>(col, i);
+        const int32_t k = (int32_t)r[W];
+        if (k != mpd::KIND_ZERO && k != -mpd::KIND_ZERO && k != mpd::KIND_REG && k != -mpd::KIND_REG) {
+            nonfinite = 1;
+            continue;
+        }
+        if (b < 0 || abs_greater<W>(r, rec_at<W>(col, b))) b = i;
+    }
+    best[threadIdx.x] = b;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            const int64_t o = best[threadIdx.x + s], cur = best[threadIdx.x];
+            if (o >= 0 && (cur < 0 || abs_greater<W>(rec_at<W>(col, o), rec_at<W>(col, cur)))) best[threadIdx.x] = o;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    uint32_t *dst = rec_at<W>(out, (int64_t)blockIdx.x);
+    const int64_t i = best[0];
+    if (nonfinite || i < 0) {
+        for (int u = 0; u < W; u++) dst[u] = 0;
+        dst[W] = (uint32_t)(nonfinite ? mpd::KIND_INF : mpd::KIND_ZERO);
+        dst[W + 1] = 0;
+        return;
+    }
+    const uint32_t *src = rec_at<W>(col, i);
+    for (int u = 0; u < W + 2; u++) dst[u] = src[u];
+    const int32_t k = (int32_t)src[W];
+    dst[W] = (uint32_t)(k < 0 ? -k : k);
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -2773,6 +2841,20 @@ void mp_mirror_upper(int limbs, void *S, int64_t n, int64_t ld, hipStream_t st) 
     if (n == 0) return;
     SDPX_DISPATCH_W(limbs, hipLaunchKernelGGL(mirror_upper_kernel<W>, dim3(blocks_for(n * n, 256)),
                                               dim3(256), 0, st, (uint32_t *)S, n, ld));
+    HIP_CHECK(hipGetLastError());
+}
+
+void mp_mirror_lower(int limbs, void *S, int64_t n, int64_t ld, hipStream_t st) {
+    if (n == 0) return;
+    SDPX_DISPATCH_W(limbs, hipLaunchKernelGGL(mirror_lower_kernel<W>, dim3(blocks_for(n * n, 256)),
+                                              dim3(256), 0, st, (uint32_t *)S, n, ld));
+    HIP_CHECK(hipGetLastError());
+}
+
+void mp_col_absmax(int limbs, const void *X, int64_t ldx, int64_t m, int64_t ncols, void *out, hipStream_t st) {
+    if (ncols == 0) return;
+    SDPX_DISPATCH_W(limbs, hipLaunchKernelGGL(col_absmax_kernel<W>, dim3((unsigned)ncols), dim3(256), 0, st,
+                                              (const uint32_t *)X, ldx, m, (uint32_t *)out));
     HIP_CHECK(hipGetLastError());
 }
 

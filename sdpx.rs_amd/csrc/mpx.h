@@ -258,6 +258,79 @@ inline void recip(const View &d, uint8_t *out) {
     round_scaled(d.negative(), mag.data(), (int)mag.size(), -p - 1 - d.exponent() - 1, n, out);
 }
 
+// RN(a / b) by restoring long division: Q = floor(Ma 2^(P+1) / Mb) has P + 1
+// or P + 2 bits, sticky = rem != 0.  a, b not NaN; b zero gives a signed
+// infinity, a zero a signed zero (0/0 and inf/inf are not admitted).
+inline void div(const View &a, const View &b, uint8_t *out) {
+    const int n = a.n;
+    const bool neg = a.negative() != b.negative();
+    if (b.zero() || !a.finite()) {
+        set_special(out, n, neg ? -KIND_INF : KIND_INF);
+        return;
+    }
+    if (a.zero() || !b.finite()) {
+        set_zero(out, n);
+        if (neg) *(int32_t *)(out + 8 * n) = -KIND_ZERO;
+        return;
+    }
+    const uint64_t *ma = a.limbs(), *mb = b.limbs();
+    const int64_t p = 64 * (int64_t)n;
+    std::vector<uint64_t> rem(n + 1, 0), q((p + 2 + 63) / 64 + 1, 0);
+    // numerator Ma 2^(P+1): bit positions 2P .. 0, Ma's bits on top
+    for (int64_t bit = 2 * p; bit >= 0; bit--) {
+        const int64_t src = bit - (p + 1);
+        uint64_t carry = src >= 0 ? (ma[src / 64] >> (src % 64)) & 1 : 0;
+        for (int i = 0; i <= n; i++) {
+            uint64_t v = rem[i];
+            rem[i] = (v << 1) | carry;
+            carry = v >> 63;
+        }
+        int c = rem[n] ? 1 : 0;
+        for (int i = n - 1; i >= 0 && !c; i--)
+            if (rem[i] != mb[i]) c = rem[i] > mb[i] ? 1 : -1;
+        if (c >= 0) {
+            uint64_t borrow = 0;
+            for (int i = 0; i <= n; i++) {
+                const uint64_t mi = i < n ? mb[i] : 0;
+                const u128 dd = (u128)rem[i] - mi - borrow;
+                rem[i] = (uint64_t)dd;
+                borrow = (uint64_t)(dd >> 64) ? 1 : 0;
+            }
+            q[bit / 64] |= 1ULL << (bit % 64);
+        }
+    }
+    bool sticky = false;
+    for (auto v : rem) sticky |= v != 0;
+    // a / b = Q 2^(-(P+1) + ea - eb) + (below Q's last bit): value = (2Q + sticky) 2^(ea - eb - P - 2)
+    std::vector<uint64_t> mag(q.size() + 1, 0);
+    uint64_t c = sticky ? 1 : 0;
+    for (size_t i = 0; i < q.size(); i++) {
+        mag[i] = (q[i] << 1) | c;
+        c = q[i] >> 63;
+    }
+    mag[q.size()] = c;
+    round_scaled(neg, mag.data(), (int)mag.size(), (int64_t)a.exponent() - b.exponent() - p - 2, n, out);
+}
+
+// Sign of |a| - |b| for values that are not NaN (zeros of either sign are
+// equal and smallest, infinities largest).
+inline int cmp_abs(const View &a, const View &b) {
+    const int ca = a.zero() ? 0 : (a.regular() ? 1 : 2), cb = b.zero() ? 0 : (b.regular() ? 1 : 2);
+    if (ca != cb) return ca < cb ? -1 : 1;
+    if (ca != 1) return 0;
+    if (a.exponent() != b.exponent()) return a.exponent() < b.exponent() ? -1 : 1;
+    for (int i = a.n - 1; i >= 0; i--)
+        if (a.limbs()[i] != b.limbs()[i]) return a.limbs()[i] < b.limbs()[i] ? -1 : 1;
+    return 0;
+}
+
+// Sign of a - b for values that are not NaN (+0 == -0).
+inline int cmp(const View &a, const View &b) {
+    const int sa = a.zero() ? 0 : (a.negative() ? -1 : 1), sb = b.zero() ? 0 : (b.negative() ? -1 : 1);
+    if (sa != sb) return sa < sb ? -1 : 1;
+    return sa >= 0 ? cmp_abs(a, b) : -cmp_abs(a, b);
+}
+
 // Exact dot of k pairs (x_i, y_i) rounded once.  Non-finite terms are not
 // admitted on this path (the caller's values are finite).
 struct DotAcc {

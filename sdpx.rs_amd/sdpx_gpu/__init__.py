@@ -100,6 +100,11 @@ def _declare(L):
     L.sdpx_gpu_panel_phase_times.restype = i32
     L.sdpx_gpu_sampled_arrow_solve_dev.argtypes = [u32, u64, vp, u64, vp, vp, vp, vp, vp, u64, vp, u64]
     L.sdpx_gpu_sampled_arrow_solve_dev.restype = i32
+    L.sdpx_gpu_sampled_schur_dev.argtypes = [u32, u64, vp, vp]
+    L.sdpx_gpu_sampled_schur_dev.restype = i32
+    L.sdpx_gpu_sampled_kkt_solve_dev.argtypes = [u32, u64, vp, u64, vp, vp, vp, vp, vp, vp, vp, vp, u64, vp, u64,
+                                                 vp, u64, vp, vp, ctypes.POINTER(ctypes.c_int64)]
+    L.sdpx_gpu_sampled_kkt_solve_dev.restype = i32
     L.sdpx_gpu_sampled_stats.restype = i32
     L.sdpx_gpu_sampled_arrow_f64_dev.argtypes = [u64, vp, u64, vp, vp, vp, ctypes.c_double,
                                                  ctypes.c_double, ctypes.POINTER(i32), dp]
@@ -369,6 +374,11 @@ class SampledLeaf(_Sized):
                 ("weights", ctypes.c_void_p), ("coupling", ctypes.c_void_p)]
 
 
+class RefineSettings(_Sized):
+    _fields_ = [("struct_size", ctypes.c_uint64), ("reltol", ctypes.c_void_p), ("abstol", ctypes.c_void_p),
+                ("stop_ratio", ctypes.c_void_p), ("max_iter", ctypes.c_uint32)]
+
+
 class SampledFactor(_Sized):
     _fields_ = [("struct_size", ctypes.c_uint64), ("l", ctypes.c_void_p), ("dinv", ctypes.c_void_p), ("y", ctypes.c_void_p)]
 
@@ -482,6 +492,51 @@ class SampledDev:
                                                      self.border.data_ptr(), self.bdinv.data_ptr(), nrhs,
                                                      dx.data_ptr(), n))
         return dx.cpu().numpy().view(dt).copy()
+
+    def schur(self):
+        """The leaves' sampled Schur blocks (sdpx_gpu_sampled_schur_dev):
+        H of every leaf (K x K, lower triangle, zeros above) concatenated in
+        leaf order, as a device tensor (kept for solve_refined) and host
+        records."""
+        import torch
+
+        vb = mp.dtype(self.nl).itemsize
+        self.h_all = torch.zeros(max(1, sum(k * k for k in self.ks)) * vb, dtype=torch.uint8, device="cuda")
+        check(lib().sdpx_gpu_sampled_schur_dev(self.nl, len(self.leaves), self.leaves, self.h_all.data_ptr()))
+        return self.h_all.cpu().numpy().view(mp.dtype(self.nl)).copy()
+
+    def solve_refined(self, rhs, nrhs, reltol, abstol, stop_ratio=5.0, max_iter=10):
+        """DirectLDLKKTSolver::solve_many with iterative refinement against
+        the unshifted KKT (sdpx_gpu_sampled_kkt_solve_dev) on the factors of
+        the last run() and the blocks of schur(): rhs (n x nrhs records) ->
+        (x, ok per column, residual norms, correction solves).  reltol,
+        abstol, stop_ratio: records (or floats, converted exactly)."""
+        import torch
+
+        nl = self.nl
+        n = sum(self.ks) + self.t
+        dt = mp.dtype(nl)
+
+        def rec(v):
+            return v if isinstance(v, np.ndarray) else mp.from_float64(np.array([float(v)]), nl)
+
+        rt, at, sr = rec(reltol), rec(abstol), rec(stop_ratio)
+        settings = RefineSettings(reltol=_ptr(rt), abstol=_ptr(at), stop_ratio=_ptr(sr), max_iter=max_iter)
+        rhs = np.ascontiguousarray(rhs, dtype=dt).reshape(-1)
+        assert rhs.size == n * nrhs
+        db = torch.from_numpy(rhs.view(np.uint8).copy()).to("cuda")
+        dx = torch.zeros_like(db)
+        torch.cuda.synchronize()
+        ok = (ctypes.c_int32 * max(1, nrhs))()
+        norms = mp.zeros(max(1, nrhs), nl)
+        count = ctypes.c_int64()
+        check(lib().sdpx_gpu_sampled_kkt_solve_dev(
+            nl, len(self.leaves), self.leaves, self.t, self.h_all.data_ptr(), self.border0.data_ptr(),
+            self.l_all.data_ptr(), self.dinv_all.data_ptr(), self.y_all.data_ptr(), self.border.data_ptr(),
+            self.bdinv.data_ptr(), ctypes.byref(settings), nrhs, db.data_ptr(), n, dx.data_ptr(), n, ok,
+            _ptr(norms), ctypes.byref(count)))
+        return (dx.cpu().numpy().view(dt).copy(), [bool(ok[c]) for c in range(nrhs)], norms[:nrhs],
+                count.value)
 
     def result(self):
         """(border factor, border dinv) as host record arrays."""

@@ -205,3 +205,75 @@ def oracle_sampled_arrow_solve(outs, t, border, bdinv, rhs, nrhs, threads=8):
     L.oracle_sampled_arrow_solve(nl, len(outs), _p(ks), t, _p(l_all), _p(d_all), _p(y_all), _p(border),
                                  _p(bdinv), nrhs, _p(x), n, threads)
     return x
+
+
+def _leaves_of(problem):
+    blocks = problem["blocks"]
+    leaves = (OraLeaf * max(1, len(blocks)))()
+    for i, b in enumerate(blocks):
+        leaves[i] = OraLeaf(b["h"], b["k"], _p(b["rinv"]), _p(b["q"]), _p(b["w"]), _p(b["b"]))
+    return leaves
+
+
+def oracle_sampled_schur(problem, nl, threads=8):
+    """C oracle of the sampled Schur blocks: every leaf's H (K x K, lower
+    triangle, zeros above) concatenated in leaf order."""
+    L = oracle()
+    L.oracle_sampled_schur.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_int]
+    L.oracle_sampled_schur.restype = ctypes.c_int
+    blocks = problem["blocks"]
+    hs = [mp.zeros(b["k"] ** 2, nl) for b in blocks]
+    hp = (ctypes.c_void_p * max(1, len(blocks)))(*[_p(h) for h in hs])
+    L.oracle_sampled_schur(nl, len(blocks), _leaves_of(problem), hp, threads)
+    return np.concatenate(hs) if hs else mp.zeros(0, nl)
+
+
+def _kkt_arrays(problem, h_all, nl):
+    ks = np.array([b["k"] for b in problem["blocks"]], dtype=np.uint64)
+    b_all = np.concatenate([b["b"] for b in problem["blocks"]]) if problem["blocks"] else mp.zeros(1, nl)
+    return ks, np.ascontiguousarray(b_all), np.ascontiguousarray(problem["border"])
+
+
+def oracle_sampled_residual(problem, h_all, rhs, x, threads=8):
+    """C oracle of the exact-rows residual RN(b - K x) of one point."""
+    L = oracle()
+    vp, sz = ctypes.c_void_p, ctypes.c_size_t
+    L.oracle_sampled_residual.argtypes = [ctypes.c_int, sz, vp, sz, vp, vp, vp, vp, vp, vp, ctypes.c_int]
+    L.oracle_sampled_residual.restype = None
+    nl = mp.limbs_of(rhs)
+    ks, b_all, cmat = _kkt_arrays(problem, h_all, nl)
+    e = mp.zeros(rhs.size, nl)
+    L.oracle_sampled_residual(nl, len(problem["blocks"]), _p(ks), problem["t"], _p(h_all), _p(b_all), _p(cmat),
+                              _p(np.ascontiguousarray(rhs)), _p(np.ascontiguousarray(x)), _p(e), threads)
+    return e
+
+
+def oracle_sampled_refine(problem, h_all, outs, border, bdinv, rhs, nrhs, reltol, abstol, stop_ratio, max_iter,
+                          threads=8):
+    """C oracle of DirectLDLKKTSolver::solve_many with stationary refinement
+    on the sampled arrow KKT: returns (x, ok per column, norms, refinements)."""
+    L = oracle()
+    vp, sz = ctypes.c_void_p, ctypes.c_size_t
+    L.oracle_sampled_refine.argtypes = [ctypes.c_int, sz, vp, sz, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                                        ctypes.c_uint, sz, vp, vp, vp, vp, ctypes.POINTER(ctypes.c_longlong),
+                                        ctypes.c_int]
+    L.oracle_sampled_refine.restype = ctypes.c_int
+    nl = mp.limbs_of(border)
+    ks, b_all, cmat = _kkt_arrays(problem, h_all, nl)
+    l_all = np.concatenate([l for l, _, _ in outs]) if outs else mp.zeros(1, nl)
+    d_all = np.concatenate([d for _, d, _ in outs]) if outs else mp.zeros(1, nl)
+    y_all = np.concatenate([y for _, _, y in outs]) if outs else mp.zeros(1, nl)
+
+    def rec(v):
+        return v if isinstance(v, np.ndarray) else mp.from_float64(np.array([float(v)]), nl)
+
+    rt, at, sr = rec(reltol), rec(abstol), rec(stop_ratio)
+    x = mp.zeros(rhs.size, nl)
+    ok = np.zeros(max(1, nrhs), np.int32)
+    norms = mp.zeros(max(1, nrhs), nl)
+    cnt = ctypes.c_longlong()
+    L.oracle_sampled_refine(nl, len(outs), _p(ks), problem["t"], _p(h_all), _p(b_all), _p(cmat), _p(l_all),
+                            _p(d_all), _p(y_all), _p(border), _p(bdinv), _p(rt), _p(at), _p(sr), max_iter, nrhs,
+                            _p(np.ascontiguousarray(rhs)), _p(x), _p(ok), _p(norms), ctypes.byref(cnt), threads)
+    return x, [bool(v) for v in ok[:nrhs]], norms[:nrhs], cnt.value
