@@ -544,13 +544,35 @@ def fp64_workload(args):
                                                       vp(d_cg.data_ptr())))
     adt = timed(adjoint, max(5, args.steps))
     progress(f"adjoint, 64 cotangents: {adt * 1e3:.3f} ms")
+
+    # NT scaling of the PSD blocks (the step before the Schur build): one
+    # block of order 200, one of order 50, and 40 blocks of orders 10..60
+    # in one launch; iterates S, Z = Q diag(logspace(0, -2)) Q^T
+    def nt_case(dims, seed):
+        rng = np.random.default_rng(seed)
+
+        def spd(k):
+            q, _ = np.linalg.qr(rng.standard_normal((k, k)))
+            return (q * np.logspace(0, -2, k)) @ q.T
+
+        nt = sdpx_gpu.NtScaling(dims)
+        nt.s.copy_(torch.from_numpy(np.concatenate([sdpx_gpu.svec_f64(spd(k)) for k in dims])))
+        nt.z.copy_(torch.from_numpy(np.concatenate([sdpx_gpu.svec_f64(spd(k)) for k in dims])))
+        assert not any(nt.run())
+        return timed(nt.run, max(5, args.steps))
+
+    sampled_dims = [int(v) for v in np.random.default_rng(7).integers(10, 61, 40)]
+    nt_ms = {"n200": nt_case([200], 1) * 1e3, "n50": nt_case([50], 2) * 1e3,
+             "sampled_40_blocks_10_60": nt_case(sampled_dims, 3) * 1e3}
+    progress(f"NT scaling: {nt_ms}")
     flops = m * 4.0 * n ** 3 + m * (m + 1) * s_len + m ** 3 / 3.0
     result = {
         "value": dt, "unit": "s/IPM-iter", "ms_per_step": dt * 1e3, "dtype": "f64",
         "config": {"workload": "Float64 dense SDP, one PSD block n=200, m=1000: Schur build (FP64 MFMA "
                                "congruences + SYRK) + Cholesky per IPM iteration",
                    "n": n, "m": m, "cfg_n50_m100_s_per_iter": out[(50, 100)][0],
-                   "adjoint_64_cotangents_s": adt, "flops_per_iter": flops},
+                   "adjoint_64_cotangents_s": adt, "flops_per_iter": flops,
+                   "nt_scaling_ms": nt_ms},
         "roofline": {"bound": "mfma", "achieved": flops / dt / 1e12, "peak": FP64_PEAK / 1e12,
                      "unit": "TFLOP/s (FP64)", "frac": flops / dt / FP64_PEAK, "traffic": None,
                      "kernel": "schur_psd_f64 + dense Cholesky (whole iteration)"},

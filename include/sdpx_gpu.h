@@ -483,6 +483,27 @@ int32_t sdpx_gpu_schur_psd_f64(uint64_t n, uint64_t m, const double *rinv, const
 int32_t sdpx_gpu_schur_psd_f64_dev(uint64_t n, uint64_t m, const double *rinv,
                                    const double *a_svec, double *s, uint64_t lds);
 
+/* Nesterov-Todd scaling of a batch of PSD blocks in binary64, device
+ * pointers except dims and status (host): replaces
+ * PSDTriangleCone::update_scaling (crates/solver/src/solver/cones/
+ * psdtrianglecone.rs:311-430) for every PSD block of the composite cone at
+ * once (CompositeCone::update_scaling, cones/compositecone.rs:1603).  Block i
+ * of order dims[i] <= 256 reads its slack s and dual z as svec'd upper
+ * triangles (off-diagonals times sqrt(2); svec_to_mat, algebra/dense/
+ * matrix_math.rs:156-179), concatenated block after block, and writes
+ *   R    = L1 V diag(lambda)^-1/2       (n x n, column-major)
+ *   Rinv = diag(lambda)^1/2 (L1^-T V)^T (Rinv R = I)
+ *   lambda, the singular values of L2^T L1, descending
+ *   G    = R R^T                         (both triangles)
+ * with L1 = chol(S), L2 = chol(Z) and V the right singular vectors; the
+ * matrices are concatenated (sum n^2), lambda too (sum n).  status[i]: 0 ok,
+ * 1 S not positive definite, 2 Z not positive definite (both: the
+ * reference's update_scaling returns false), 3 the singular value sweeps did
+ * not converge.  Agrees with the reference's binary64 path to rounding
+ * (LAPACK's summation and SVD algorithm are not specified bit for bit). */
+int32_t sdpx_gpu_psd_nt_scaling_f64_dev(uint64_t nblocks, const uint64_t *dims, const double *s, const double *z,
+                                        double *r, double *rinv, double *lambda, double *g, int32_t *status);
+
 /* Condensed Schur assembly of a reduced KKT with several cones (replaces
  * CondensedKKT::assemble, kkt/condensed/mod.rs:1962-2147, for Float64
  * PSD and orthant cones; device pointers): the upper triangle of S (m x m,

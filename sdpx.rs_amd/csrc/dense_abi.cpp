@@ -1,4 +1,5 @@
 // C ABI of the Float64 dense factor/solve and PSD Schur assembly.
+#include <algorithm>
 #include <cstring>
 #include <functional>
 
@@ -379,6 +380,63 @@ int32_t sdpx_gpu_condensed_assemble_f64_dev(uint64_t m, const int64_t *p_colptr,
         sdpx::condensed_assemble_f64((int64_t)m, p_colptr, p_rowval, p_nzval, bl, (int64_t)rows, b, (int64_t)ldb, w,
                                      s, (int64_t)lds, d.stream);
         HIP_CHECK(hipStreamSynchronize(d.stream));
+        return SDPX_GPU_OK;
+    });
+}
+
+}  // extern "C"
+
+#include "nt_scaling.h"
+
+extern "C" {
+
+int32_t sdpx_gpu_psd_nt_scaling_f64_dev(uint64_t nblocks, const uint64_t *dims, const double *s, const double *z,
+                                        double *r, double *rinv, double *lambda, double *g, int32_t *status) {
+    return abi_boundary([&]() -> int32_t {
+        if (nblocks && (!dims || !status)) return abi_invalid("null dims or status");
+        std::vector<int64_t> dv(nblocks);
+        std::vector<sdpx::NtBlock> small, large;
+        std::vector<size_t> small_idx, large_idx;
+        int64_t so = 0, mo = 0, vo = 0, smax = 0, lmax = 0;
+        for (uint64_t i = 0; i < nblocks; i++) {
+            const int64_t n = (int64_t)dims[i];
+            if (n > sdpx::NT_MAX_ORDER) return abi_invalid("PSD block order above 256");
+            dv[i] = n;
+            const sdpx::NtBlock b{n, so, mo, vo};
+            if (n == 0) {
+                status[i] = 0;
+            } else if (n <= 64) {
+                small.push_back(b);
+                small_idx.push_back(i);
+                smax = std::max(smax, n);
+            } else {
+                large.push_back(b);
+                large_idx.push_back(i);
+                lmax = std::max(lmax, n);
+            }
+            so += n * (n + 1) / 2;
+            mo += n * n;
+            vo += n;
+        }
+        if (so && (!s || !z || !r || !rinv || !lambda || !g)) return abi_invalid("null block arrays");
+        if (small.empty() && large.empty()) return SDPX_GPU_OK;
+        sdpx::Device &d = sdpx::device();
+        std::lock_guard<std::recursive_mutex> guard(d.mu);
+        const size_t nb = small.size() + large.size();
+        sdpx::NtBlock *db = (sdpx::NtBlock *)d.ws.get(170, sizeof(sdpx::NtBlock) * nb);
+        int *dst = (int *)d.ws.get(171, sizeof(int) * nb);
+        double *work = large.empty() ? nullptr
+                                     : (double *)d.ws.get(172, sizeof(double) * 3 * lmax * lmax * large.size());
+        std::vector<sdpx::NtBlock> all(small);
+        all.insert(all.end(), large.begin(), large.end());
+        HIP_CHECK(hipMemcpyAsync(db, all.data(), sizeof(sdpx::NtBlock) * nb, hipMemcpyHostToDevice, d.stream));
+        sdpx::nt_scaling_f64(dv, db, (int)small.size(), smax, db + small.size(), (int)large.size(), lmax, s, z, r,
+                             rinv, lambda, g, work, dst, dst + small.size(), 60, d.stream);
+        std::vector<int> hs(nb);
+        HIP_CHECK(hipMemcpyAsync(hs.data(), dst, sizeof(int) * nb, hipMemcpyDeviceToHost, d.stream));
+        HIP_CHECK(hipStreamSynchronize(d.stream));
+        for (size_t i = 0; i < small.size(); i++) status[small_idx[i]] = hs[i];
+        for (size_t i = 0; i < large.size(); i++) status[large_idx[i]] = hs[small.size() + i];
         return SDPX_GPU_OK;
     });
 }

@@ -100,6 +100,8 @@ def _declare(L):
     L.sdpx_gpu_panel_phase_times.restype = i32
     L.sdpx_gpu_sampled_arrow_solve_dev.argtypes = [u32, u64, vp, u64, vp, vp, vp, vp, vp, u64, vp, u64]
     L.sdpx_gpu_sampled_arrow_solve_dev.restype = i32
+    L.sdpx_gpu_psd_nt_scaling_f64_dev.argtypes = [u64, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.sdpx_gpu_psd_nt_scaling_f64_dev.restype = i32
     L.sdpx_gpu_sampled_schur_dev.argtypes = [u32, u64, vp, vp]
     L.sdpx_gpu_sampled_schur_dev.restype = i32
     L.sdpx_gpu_sampled_kkt_solve_dev.argtypes = [u32, u64, vp, u64, vp, vp, vp, vp, vp, vp, vp, vp, u64, vp, u64,
@@ -327,6 +329,71 @@ def schur_psd_f64(rinv, a_svec):
                                        a_svec.ctypes.data_as(ctypes.c_void_p),
                                        S.ctypes.data_as(ctypes.c_void_p), m))
     return S
+
+
+def svec_f64(mat):
+    """The svec of a symmetric matrix: its upper triangle column by column,
+    off-diagonals times sqrt(2) (the reference's svec order)."""
+    n = mat.shape[0]
+    iu = np.triu_indices(n)
+    order = np.lexsort((iu[0], iu[1]))
+    i, j = iu[0][order], iu[1][order]
+    return np.where(i == j, 1.0, np.sqrt(2.0)) * mat[i, j]
+
+
+class NtScaling:
+    """Device buffers for sdpx_gpu_psd_nt_scaling_f64_dev over blocks of the
+    given orders; run(s, z) takes the svec'd slacks and duals (concatenated
+    device tensors) and leaves R, Rinv, lambda and G in the tensors of the
+    same names."""
+
+    def __init__(self, dims):
+        import torch
+
+        self.dims = [int(n) for n in dims]
+        sv = sum(n * (n + 1) // 2 for n in self.dims)
+        mats = sum(n * n for n in self.dims)
+        f64 = dict(dtype=torch.float64, device="cuda")
+        self.s = torch.zeros(max(1, sv), **f64)
+        self.z = torch.zeros(max(1, sv), **f64)
+        self.r = torch.zeros(max(1, mats), **f64)
+        self.rinv = torch.zeros(max(1, mats), **f64)
+        self.g = torch.zeros(max(1, mats), **f64)
+        self.lam = torch.zeros(max(1, sum(self.dims)), **f64)
+        self._dims = (ctypes.c_uint64 * max(1, len(self.dims)))(*self.dims)
+        self.status = (ctypes.c_int32 * max(1, len(self.dims)))()
+
+    def run(self):
+        import torch
+
+        torch.cuda.current_stream().synchronize()
+        vp = ctypes.c_void_p
+        check(lib().sdpx_gpu_psd_nt_scaling_f64_dev(len(self.dims), self._dims, vp(self.s.data_ptr()),
+                                                    vp(self.z.data_ptr()), vp(self.r.data_ptr()),
+                                                    vp(self.rinv.data_ptr()), vp(self.lam.data_ptr()),
+                                                    vp(self.g.data_ptr()), self.status))
+        return [int(self.status[i]) for i in range(len(self.dims))]
+
+
+def psd_nt_scaling_f64(pairs):
+    """PSDTriangleCone::update_scaling of several blocks on the GPU: pairs of
+    (S, Z) symmetric matrices -> [(R, Rinv, lambda, G, status)] per block."""
+    import torch
+
+    dims = [np.asarray(S).shape[0] for S, _ in pairs]
+    nt = NtScaling(dims)
+    if pairs:
+        nt.s.copy_(torch.from_numpy(np.concatenate([svec_f64(np.asarray(S, np.float64)) for S, _ in pairs])))
+        nt.z.copy_(torch.from_numpy(np.concatenate([svec_f64(np.asarray(Z, np.float64)) for _, Z in pairs])))
+    status = nt.run()
+    r, rinv, g, lam = (t.cpu().numpy() for t in (nt.r, nt.rinv, nt.g, nt.lam))
+    out, mo, vo = [], 0, 0
+    for n, st in zip(dims, status):
+        mat = lambda a: a[mo:mo + n * n].reshape(n, n).T.copy()
+        out.append((mat(r), mat(rinv), lam[vo:vo + n].copy(), mat(g), st))
+        mo += n * n
+        vo += n
+    return out
 
 
 LDL_DOTS, LDL_HOST_PRODUCTS = 1, 2

@@ -14,14 +14,15 @@ condensed Schur matrix is M_ij = <A_i, W A_j W> with the NT scaling W; with
 W = Rinv^T Rinv this is exactly the reference's S_ab = <A_a, Ginv A_b Ginv>
 (kkt/condensed/psd.rs:1299-1456, kkt.rs:1119-1127).
 
-backend "gpu": sdpx_gpu_schur_psd_f64 + the dense GPU factor / solve;
+backend "gpu": the NT scaling (sdpx_gpu_psd_nt_scaling_f64_dev),
+sdpx_gpu_schur_psd_f64 and the dense GPU factor / solve;
 backend "cpu": the same formulas in numpy / LAPACK.
 """
 import time
 
 import numpy as np
 
-from . import DenseFactor, schur_psd_f64
+from . import DenseFactor, psd_nt_scaling_f64, schur_psd_f64
 
 
 def svec(mat):
@@ -116,7 +117,14 @@ def solve(prob, backend="gpu", tol=1e-8, max_iter=80, verbose=False, target_mu=N
         elif gap < tol and pinf < tol and dinf < tol:
             info["status"] = "solved"
             break
-        W, rinv = _nt_scaling(X, S)
+        if backend == "gpu":
+            r, _, _, W, status = psd_nt_scaling_f64([(X, S)])[0]
+            if status != 0:
+                info["status"] = "scaling_failed"
+                break
+            rinv = r.T
+        else:
+            W, rinv = _nt_scaling(X, S)
         t0 = time.perf_counter()
         if backend == "gpu":
             M = schur_psd_f64(rinv, a_svec)
