@@ -126,22 +126,27 @@ def planes_ops(moduli, planes):
 
 
 def cpu_baseline_gemm(a, b, m, n, k, nl):
-    """The C oracle (exact dots, OpenMP over all host cores) on a bounded
-    sample of outputs, scaled to the full product."""
+    """The reference's CPU exact product restated in C (residues over 18-26
+    bit primes as binary64 products, CRT, one rounding; OpenMP over all
+    host cores) on a bounded sample: the leading sub x sub output block with
+    the full inner dimension, scaled to the full product."""
     sys.path[:0] = [os.path.join(ROOT, "tests"), os.path.join(ROOT, "oracle")]
-    from helpers import oracle_entries
+    from helpers import oracle_gemm_rns
+    from sdpx_gpu import mp
 
-    rng = np.random.default_rng(0)
     cores = os.cpu_count() or 1
-    count = 1 << 20  # about 10-15 s of exact dots on 16 cores
-    rows = rng.integers(0, m, count)
-    cols = rng.integers(0, n, count)
+    sub = 512
+    dt = mp.dtype(nl)
+    a_sub = np.ascontiguousarray(a.view(dt).reshape(k, m)[:, :sub]).reshape(-1)  # column-major m x k
+    b_sub = np.ascontiguousarray(b.view(dt).reshape(n, k)[:sub]).reshape(-1)      # column-major k x n
     t0 = time.perf_counter()
-    oracle_entries("N", "N", k, a, m, b, k, rows, cols, threads=cores)
-    dt = time.perf_counter() - t0
-    return {"value": dt * (m * n) / count, "unit": "s/GEMM", "cores": cores, "kind": "port", "cpu": cpu_model(),
-            "sample": f"{count} random outputs (exact {k}-term dots) of the same product, "
-                      f"{dt:.2f} s, scaled by {m * n}/{count}"}
+    assert oracle_gemm_rns("N", "N", sub, sub, k, a_sub, sub, b_sub, k, threads=cores) is not None
+    el = time.perf_counter() - t0
+    return {"value": el * (m * n) / (sub * sub), "unit": "s/GEMM", "cores": cores, "kind": "port",
+            "cpu": cpu_model(),
+            "sample": f"the residue product (the reference's CPU algorithm, rns_blas.rs) of the leading "
+                      f"{sub}x{sub} outputs with the full inner dimension {k}: {el:.2f} s, scaled by "
+                      f"{m * n}/{sub * sub}"}
 
 
 def sampled_workload(args):
@@ -250,16 +255,22 @@ def sampled_workload(args):
         result["config"]["target_8192"] = sampled_target_point(args)
     if args.cpu_baseline and t <= 2000:  # larger borders: minutes of CPU time
         sys.path[:0] = [os.path.join(ROOT, "tests"), os.path.join(ROOT, "oracle")]
-        from helpers import oracle_sampled_arrow
+        from helpers import oracle_sampled_arrow, oracle_set_rns_products
 
         cores = os.cpu_count() or 1
-        c0 = time.perf_counter()
-        ok, *_ = oracle_sampled_arrow(prob, nl, threads=cores)
-        cdt = time.perf_counter() - c0
+        oracle_set_rns_products(True)
+        try:
+            c0 = time.perf_counter()
+            ok, *_ = oracle_sampled_arrow(prob, nl, threads=cores)
+            cdt = time.perf_counter() - c0
+        finally:
+            oracle_set_rns_products(False)
         result["cpu_baseline"] = {"value": cdt, "unit": "s/IPM-iter", "cores": cores, "kind": "port",
                                   "cpu": cpu_model(),
-                                  "sample": "the whole workload (one iteration) through the C "
-                                            "restatement with exact dots, OpenMP on all cores"}
+                                  "sample": "the whole workload (one iteration) through the C restatement, "
+                                            "the reference's CPU algorithm: residue products (binary64 "
+                                            "residues over 18-26 bit primes, CRT) where it takes them "
+                                            "(mpfr.rs:349-361), exact dots elsewhere; OpenMP on all cores"}
     return result
 
 
@@ -421,18 +432,23 @@ def chol_workload(args):
     }
     if args.cpu_baseline:
         sys.path[:0] = [os.path.join(ROOT, "tests"), os.path.join(ROOT, "oracle")]
-        from helpers import oracle_ldl
+        from helpers import oracle_ldl, oracle_set_rns_products
 
         sub = 2048
         a_sub = mp.from_float64(((q[:sub] * s) @ q[:sub].T).T.reshape(-1), nl)
         cores = os.cpu_count() or 1
-        c0 = time.perf_counter()
-        oracle_ldl(a_sub, sub, threads=cores)
-        cdt = time.perf_counter() - c0
+        oracle_set_rns_products(True)
+        try:
+            c0 = time.perf_counter()
+            oracle_ldl(a_sub, sub, threads=cores)
+            cdt = time.perf_counter() - c0
+        finally:
+            oracle_set_rns_products(False)
         result["cpu_baseline"] = {"value": cdt * (n / sub) ** 3, "unit": "s/factor", "cores": cores,
                                   "kind": "port", "cpu": cpu_model(),
-                                  "sample": f"C restatement (exact dots) on the leading {sub}x{sub} "
-                                            f"block ({cdt:.1f} s), scaled by (n/{sub})^3"}
+                                  "sample": f"C restatement of ldl_rec on the leading {sub}x{sub} block, "
+                                            f"trailing products by the reference's CPU residue product, "
+                                            f"base cases by exact dots ({cdt:.1f} s), scaled by (n/{sub})^3"}
     return result
 
 

@@ -17,6 +17,7 @@
  * own rounding code.  Checked against oracle/mpexact.py (plain Python
  * integers) and the reference's known answers in tests/test_oracle.py.
  */
+#include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -253,6 +254,215 @@ int oracle_gemm_exact(int n, char ta, char tb, size_t m, size_t nn, size_t k, co
         free(yi);
     }
     return 1;
+}
+
+/* ------------------------------------------------------------------------
+ * The reference's CPU exact product restated: residues over primes of 18-26
+ * bits computed in binary64 (rns_blas.rs: primes 95-112, prime_bits
+ * 184-188, Plan 705-737, the gemm contract 2134-2417), for the CPU
+ * baseline, taken where the reference takes it
+ * (residue_product_profitable, blas/mpfr.rs:349-361).  Per operand one
+ * window (the lowest set weight), each value an integer X = M 2^s there;
+ * the primes (descending below 2^b, b = floor((55 - ceil log2 k) / 2) in
+ * [18, 26]) multiply to at least 2^(wa + wb + log2 k + 3) for operand
+ * widths wa, wb; residues |r| < p/2; per prime a binary64 product exact
+ * over inner chunks of 2^(55 - 2b) terms, reduced mod p; then the CRT
+ * S = sum x_q (M/q) - round(sum x_q / q) M with x_q = c_q (M/q)^-1 mod q
+ * (the reference's crt: signed lift with an estimate kept inside
+ * (-1/4, 1/4) by M > 4|S|), and one rounding.  Same bits as the exact dots
+ * by construction; tests check it.
+ * ---------------------------------------------------------------------- */
+static int rns_products = 0;
+void oracle_set_rns_products(int on) { rns_products = on; }
+
+static int is_prime_small(uint64_t v) {
+    if (v < 2) return 0;
+    for (uint64_t d = 2; d * d <= v; d++)
+        if (v % d == 0) return 0;
+    return 1;
+}
+
+static uint64_t powmod_u64(uint64_t b, uint64_t e, uint64_t q) {
+    uint64_t r = 1 % q;
+    b %= q;
+    while (e) {
+        if (e & 1) r = (uint64_t)((u128)r * b % q);
+        b = (uint64_t)((u128)b * b % q);
+        e >>= 1;
+    }
+    return r;
+}
+
+/* residue of the value at index idx, as an integer at window lo, mod q, in (-q/2, q/2) */
+static double rns_residue(int n, const void *base, size_t stride, size_t idx, int64_t lo, uint64_t q) {
+    int32_t k = kind_of(base, stride, idx, n);
+    if (abs(k) != KIND_REG) return 0.0;
+    const uint64_t *l = limbs_of(base, stride, idx);
+    uint64_t r = 0;
+    const uint64_t b64 = (uint64_t)(((u128)1 << 64) % q);
+    for (int i = n - 1; i >= 0; i--) r = (uint64_t)(((u128)r * b64 + l[i] % q) % q);
+    int64_t sh = (int64_t)exp_of(base, stride, idx, n) - 64 * (int64_t)n - lo;
+    r = (uint64_t)((u128)r * powmod_u64(2, (uint64_t)sh, q) % q);
+    if (k < 0 && r) r = q - r;
+    return r > (q - 1) / 2 ? (double)r - (double)q : (double)r;
+}
+
+static uint64_t inv_mod(uint64_t a, uint64_t q) { return powmod_u64(a, q - 2, q); }
+
+int oracle_gemm_rns(int n, char ta, char tb, size_t m, size_t nn, size_t k, const void *a, size_t lda,
+                    const void *b, size_t ldb, int upper_only, int packed, void *out, int threads) {
+    size_t stride = 8 * (size_t)n + 8;
+    int tan = (ta == 'N' || ta == 'n'), tbn = (tb == 'N' || tb == 'n');
+    int64_t la, ha, lb, hb;
+    if (exponent_range(n, a, tan ? m : k, tan ? k : m, lda, &la, &ha) < 0) return 0;
+    if (exponent_range(n, b, tbn ? k : nn, tbn ? nn : k, ldb, &lb, &hb) < 0) return 0;
+    if ((la <= ha && ha - la > MAX_SPREAD) || (lb <= hb && hb - lb > MAX_SPREAD)) return 0;
+    packed = packed && upper_only && m == nn;
+    const int64_t P = 64 * (int64_t)n;
+    if (la > ha || lb > hb || k == 0) { /* a zero operand: zeros */
+        for (size_t j = 0; j < nn; j++)
+            for (size_t i = 0; i < (upper_only ? (j + 1 < m ? j + 1 : m) : m); i++)
+                write_zero(out, stride, packed ? j * (j + 1) / 2 + i : i + j * m, n);
+        return 1;
+    }
+    int lk = 0;
+    while (((size_t)1 << lk) < k) lk++;
+    int bits = (int)((53.0 + 2.0 - lk) / 2.0);
+    if (bits < 18) bits = 18;
+    if (bits > 26) bits = 26;
+    const double needed = (double)(P + ha - la) + (double)(P + hb - lb) + log2((double)k) + 3.0;
+    uint64_t qs[1024];
+    int L = 0;
+    double have = 0.0;
+    for (uint64_t v = (1ull << bits) - 1; have < needed; v -= 2) {
+        if (!is_prime_small(v)) continue;
+        if (L == 1024) return 0;
+        qs[L++] = v;
+        have += log2((double)v);
+    }
+    const size_t kc = (size_t)1 << (53 + 2 - 2 * bits < 40 ? 53 + 2 - 2 * bits : 40);
+    /* residues: A rows (i, p) at i k + p, B columns (p, j) at j k + p */
+    double *ra = malloc(sizeof(double) * (size_t)L * m * k), *rb = malloc(sizeof(double) * (size_t)L * nn * k);
+#pragma omp parallel for schedule(static) num_threads(threads > 0 ? threads : 1)
+    for (long long qi = 0; qi < (long long)L; qi++) {
+        for (size_t i = 0; i < m; i++)
+            for (size_t p = 0; p < k; p++)
+                ra[((size_t)qi * m + i) * k + p] = rns_residue(n, a, stride, tan ? i + p * lda : p + i * lda, la, qs[qi]);
+        for (size_t j = 0; j < nn; j++)
+            for (size_t p = 0; p < k; p++)
+                rb[((size_t)qi * nn + j) * k + p] = rns_residue(n, b, stride, tbn ? p + j * ldb : j + p * ldb, lb, qs[qi]);
+    }
+    /* M, M/q (limbs) and the CRT weights (M/q)^-1 mod q */
+    int lm = (int)(have / 64.0) + 2;
+    uint64_t *M = calloc((size_t)lm, 8), *Mq = calloc((size_t)L * lm, 8), *w = malloc(8 * (size_t)L);
+    M[0] = 1;
+    for (int q = 0; q < L; q++) {
+        u128 c = 0;
+        for (int i = 0; i < lm; i++) {
+            c += (u128)M[i] * qs[q];
+            M[i] = (uint64_t)c;
+            c >>= 64;
+        }
+    }
+    for (int q = 0; q < L; q++) {
+        uint64_t *d = Mq + (size_t)q * lm;
+        d[0] = 1;
+        uint64_t r = 1; /* (M/q) mod q */
+        for (int o = 0; o < L; o++) {
+            if (o == q) continue;
+            u128 c = 0;
+            for (int i = 0; i < lm; i++) {
+                c += (u128)d[i] * qs[o];
+                d[i] = (uint64_t)c;
+                c >>= 64;
+            }
+            r = (uint64_t)((u128)r * (qs[o] % qs[q]) % qs[q]);
+        }
+        w[q] = inv_mod(r, qs[q]);
+    }
+    long long total = (long long)nn;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 0 ? threads : 1)
+    for (long long jj = 0; jj < total; jj++) {
+        size_t j = (size_t)jj;
+        uint64_t *acc = malloc(8 * (size_t)(lm + 1));
+        uint32_t *cres = malloc(4 * (size_t)L);
+        size_t iend = upper_only ? (j + 1 < m ? j + 1 : m) : m;
+        for (size_t i = 0; i < iend; i++) {
+            for (int q = 0; q < L; q++) {
+                const double *x = ra + ((size_t)q * m + i) * k, *y = rb + ((size_t)q * nn + j) * k;
+                const double qd = (double)qs[q];
+                double r = 0.0;
+                for (size_t p0 = 0; p0 < k; p0 += kc) {
+                    size_t p1 = p0 + kc < k ? p0 + kc : k;
+                    double s = 0.0;
+                    for (size_t p = p0; p < p1; p++) s += x[p] * y[p];
+                    r = fmod(r + fmod(s, qd), qd);
+                }
+                if (r < 0) r += qd;
+                cres[q] = (uint32_t)r;
+            }
+            /* S = sum x_q (M/q) - K M, K = round(sum x_q / q) */
+            memset(acc, 0, 8 * (size_t)(lm + 1));
+            double frac = 0.0;
+            for (int q = 0; q < L; q++) {
+                uint64_t xq = (uint64_t)((u128)cres[q] * w[q] % qs[q]);
+                frac += (double)xq / (double)qs[q];
+                const uint64_t *d = Mq + (size_t)q * lm;
+                u128 c = 0;
+                for (int t = 0; t < lm; t++) {
+                    c += (u128)d[t] * xq + acc[t];
+                    acc[t] = (uint64_t)c;
+                    c >>= 64;
+                }
+                acc[lm] += (uint64_t)c;
+            }
+            uint64_t K = (uint64_t)llround(frac);
+            u128 bor = 0;
+            {
+                u128 c = 0;
+                for (int t = 0; t <= lm; t++) {
+                    c += (u128)(t < lm ? M[t] : 0) * K;
+                    uint64_t sub = (uint64_t)c;
+                    c >>= 64;
+                    u128 dd = (u128)acc[t] - sub - bor;
+                    acc[t] = (uint64_t)dd;
+                    bor = (dd >> 64) ? 1 : 0;
+                }
+            }
+            int neg = (acc[lm] >> 63) != 0;
+            if (neg) {
+                int carry = 1;
+                for (int t = 0; t <= lm; t++) {
+                    acc[t] = ~acc[t];
+                    if (carry) {
+                        acc[t] += 1;
+                        carry = acc[t] == 0;
+                    }
+                }
+            }
+            from_scaled_integer(neg, acc, lm + 1, la + lb, n, out, stride,
+                                packed ? j * (j + 1) / 2 + i : i + j * m);
+        }
+        free(acc);
+        free(cres);
+    }
+    free(ra);
+    free(rb);
+    free(M);
+    free(Mq);
+    free(w);
+    return 1;
+}
+
+/* The product the restatements use: the CPU residue product where the
+ * reference takes it (when enabled for the CPU baseline), else exact dots. */
+static int ora_gemm(int n, char ta, char tb, size_t m, size_t nn, size_t k, const void *a, size_t lda,
+                    const void *b, size_t ldb, int upper_only, int packed, void *out, int threads) {
+    int profitable = (n >= 4 && k >= 24 && m * nn >= 576) || (n >= 2 && k >= 64 && m * nn >= 4096);
+    if (rns_products && profitable &&
+        oracle_gemm_rns(n, ta, tb, m, nn, k, a, lda, b, ldb, upper_only, packed, out, threads))
+        return 1;
+    return oracle_gemm_exact(n, ta, tb, m, nn, k, a, lda, b, ldb, upper_only, packed, out, threads);
 }
 
 /* Exact dots of an explicit list of outputs (spot checks of large products):
@@ -625,7 +835,7 @@ static int ora_ldl_dots(const ora_ctx *c, uint8_t *l, size_t ld, size_t n, const
 static void ora_product_nt(const ora_ctx *c, size_t m, size_t n, size_t k, const uint8_t *a,
                            size_t lda, const uint8_t *b, size_t ldb, uint8_t *out, int upper_packed,
                            int threads) {
-    oracle_gemm_exact(c->n, 'N', 'T', m, n, k, a, lda, b, ldb, upper_packed, upper_packed, out, threads);
+    ora_gemm(c->n, 'N', 'T', m, n, k, a, lda, b, ldb, upper_packed, upper_packed, out, threads);
 }
 
 static void ora_solve_unit_lower_t(const ora_ctx *c, size_t m, size_t n, const uint8_t *L, size_t ldl,
@@ -820,8 +1030,8 @@ static void ora_sampled_h(const ora_ctx *c, const ora_leaf *lf, uint8_t *H, int 
     *(int32_t *)(half + 8 * nl) = KIND_REG;
     *(int32_t *)(half + 8 * nl + 4) = 0;
     uint8_t *V = malloc(st * h * K), *G = malloc(st * (K * (K + 1) / 2));
-    oracle_gemm_exact(nl, 'N', 'N', h, K, h, lf->rinv, h, lf->basis, h, 0, 0, V, threads);
-    oracle_gemm_exact(nl, 'T', 'N', K, K, h, V, h, V, h, 1, 1, G, threads);
+    ora_gemm(nl, 'N', 'N', h, K, h, lf->rinv, h, lf->basis, h, 0, 0, V, threads);
+    ora_gemm(nl, 'T', 'N', K, K, h, V, h, V, h, 1, 1, G, threads);
     for (size_t e = 0; e < K * K; e++) write_zero(H, st, e, nl);
     for (size_t q = 0; q < K; q++)
         for (size_t p = q; p < K; p++) {
@@ -910,7 +1120,7 @@ int oracle_sampled_arrow(int nl, size_t nleaves, const ora_leaf *leaves, size_t 
         free(first);
         uint8_t *Z = malloc(st * K * t), *P = malloc(st * (t * (t + 1) / 2));
         for (size_t e = 0; e < K * t; e++) ora_mul(c, AT(Y, e), AT(dinv, e % K), AT(Z, e));
-        oracle_gemm_exact(nl, 'T', 'N', t, t, K, Y, K, Z, K, 1, 1, P, threads);
+        ora_gemm(nl, 'T', 'N', t, t, K, Y, K, Z, K, 1, 1, P, threads);
 #pragma omp parallel for schedule(dynamic, 16) num_threads(threads > 0 ? threads : 1)
         for (long long jj = 0; jj < (long long)t; jj++) {
             size_t j = (size_t)jj;

@@ -277,3 +277,25 @@ def oracle_sampled_refine(problem, h_all, outs, border, bdinv, rhs, nrhs, reltol
                             _p(d_all), _p(y_all), _p(border), _p(bdinv), _p(rt), _p(at), _p(sr), max_iter, nrhs,
                             _p(np.ascontiguousarray(rhs)), _p(x), _p(ok), _p(norms), ctypes.byref(cnt), threads)
     return x, [bool(v) for v in ok[:nrhs]], norms[:nrhs], cnt.value
+
+
+def oracle_gemm_rns(ta, tb, m, n, k, a, lda, b, ldb, upper=False, packed=None, threads=8):
+    """The CPU residue product (the reference's algorithm, the CPU
+    baseline's exact product): None on decline, else the output records."""
+    L = oracle()
+    L.oracle_gemm_rns.argtypes = L.oracle_gemm_exact.argtypes
+    L.oracle_gemm_rns.restype = ctypes.c_int
+    nl = mp.limbs_of(a if a.size else b)
+    if packed is None:
+        packed = upper and m == n
+    out = mp.zeros(n * (n + 1) // 2 if packed else m * n, nl)
+    ok = L.oracle_gemm_rns(nl, ta.encode(), tb.encode(), m, n, k, _p(a), lda, _p(b), ldb,
+                           1 if upper else 0, 1 if packed else 0, _p(out), threads)
+    return out if ok else None
+
+
+def oracle_set_rns_products(on):
+    """Make the restatements' exact products the CPU residue product where
+    the reference takes it (CPU baseline timing); off: exact dots."""
+    oracle().oracle_set_rns_products.argtypes = [ctypes.c_int]
+    oracle().oracle_set_rns_products(1 if on else 0)

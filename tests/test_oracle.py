@@ -262,3 +262,30 @@ def test_potrf_reports_failing_pivot():
     assert mp.to_float64(ch)[4] == 0.0
     py = list(to_mp(a))
     assert M.potrf("L", 3, py, 3) == 2
+
+
+@pytest.mark.parametrize("nl,m,n,k,spread", [(4, 7, 5, 45, 768), (4, 20, 20, 20, 768), (2, 3, 4, 1, 512),
+                                             (4, 9, 6, 300, 256), (8, 35, 37, 45, 1024), (4, 64, 64, 70, 4)])
+def test_cpu_residue_product_matches_exact_dots(nl, m, n, k, spread):
+    """The CPU residue product (primes of 18-26 bits in binary64, CRT, one
+    rounding: the reference's CPU algorithm, used as the CPU baseline's
+    product) gives the exact dots' bits on the reference's test shapes and
+    spreads (rns_blas_tests.rs:137-145), every transpose, full and packed
+    upper; a planted operand change is seen."""
+    from helpers import oracle_gemm_rns
+
+    for ta in "NT":
+        for tb in "NT":
+            a, lda, b, ldb = lcg_operands(m, n, k, spread, 7 + m + k, nl, ta, tb)
+            want = oracle_gemm(ta, tb, m, n, k, a, lda, b, ldb)
+            got = oracle_gemm_rns(ta, tb, m, n, k, a, lda, b, ldb)
+            assert got is not None and np.all(mp.same_bits(got, want)), (ta, tb)
+            if m == n:
+                want = oracle_gemm(ta, tb, m, n, k, a, lda, b, ldb, upper=True)
+                got = oracle_gemm_rns(ta, tb, m, n, k, a, lda, b, ldb, upper=True)
+                assert np.all(mp.same_bits(got, want)), (ta, tb, "upper")
+    a2 = a.copy()
+    top = np.argmax(a["exponent"] * (np.abs(a["kind"]) == 3))  # the largest entry's high bits
+    a2[top]["limbs"][nl - 1] ^= np.uint64(1 << 40)
+    assert not np.all(mp.same_bits(oracle_gemm_rns(ta, tb, m, n, k, a2, lda, b, ldb),
+                                   oracle_gemm(ta, tb, m, n, k, a, lda, b, ldb)))

@@ -141,3 +141,24 @@ def test_oracle_arrow_solve_is_the_factored_solve():
     assert worst < Fraction(1, 2 ** 150), float(worst)
     bad = _exact_factored_solve(outs, prob["t"], border, bdinv, rhs, nrhs, plant="coupling without D^-1")
     assert max(abs(a - b) for a, b in zip(got, bad)) / scale > Fraction(1, 2 ** 20)
+
+
+def test_cpu_residue_products_keep_the_arrow_bits():
+    """The sampled arrow restatement with its products on the CPU residue
+    product (as the CPU baseline runs it) gives the exact-dot restatement's
+    bits: leaf factors, Y, the border factor."""
+    from helpers import oracle_set_rns_products
+
+    nl = 4
+    prob = workloads.sampled_problem(nl, nblocks=4, hmin=12, hmax=30, t=300, seed=21)
+    ok, border, bdinv, outs = oracle_sampled_arrow(prob, nl)
+    oracle_set_rns_products(True)
+    try:
+        ok2, border2, bdinv2, outs2 = oracle_sampled_arrow(prob, nl)
+    finally:
+        oracle_set_rns_products(False)
+    assert ok and ok2
+    assert np.all(mp.same_bits(border2, border)) and np.all(mp.same_bits(bdinv2, bdinv))
+    for a, b in zip(outs, outs2):
+        for x, y in zip(a, b):
+            assert np.all(mp.same_bits(x, y))
