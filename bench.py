@@ -158,7 +158,7 @@ def sampled_workload(args):
     import torch
 
     import sdpx_gpu
-    from sdpx_gpu import workloads
+    from sdpx_gpu import mp, workloads
 
     nl = 4
     sdpx_gpu.check(sdpx_gpu.lib().sdpx_gpu_init(0))
@@ -196,6 +196,18 @@ def sampled_workload(args):
     tot = sdpx_gpu.rns_totals(reset=True)
     pms, plaunch = sdpx_gpu.panel_phase_times(reset=True)
     sdpx_gpu.set_profiling(False)
+    # the per-iteration solves: one right-hand side through the factor and
+    # stationary refinement against the exact-rows residual (tolerances
+    # eps^(3/4) = 2^-191.25, the reference's MPFR default, stop ratio 5)
+    run.schur()
+    n_kkt = sum(b["k"] for b in prob["blocks"]) + t
+    rhs = mp.from_float64(np.random.default_rng(5).standard_normal(n_kkt), nl)
+    run.solve_refined(rhs, 1, 2.0 ** -191.25, 2.0 ** -191.25)
+    torch.cuda.synchronize()
+    r0 = time.perf_counter()
+    _, rok, rnorm, rcount = run.solve_refined(rhs, 1, 2.0 ** -191.25, 2.0 ** -191.25)
+    refine_dt = time.perf_counter() - r0
+    progress(f"refined solve: {refine_dt * 1e3:.1f} ms, {rcount} corrections")
     # float64 on the same GPU, same problem (dynamic pivot rule: the 256-bit
     # leaves are singular in binary64)
     dev = sdpx_gpu.SampledF64(prob)
@@ -241,7 +253,9 @@ def sampled_workload(args):
                    "fp64_gpu_s_per_iter": fdt, "fp64_phase_ms": ftimes,
                    "mp256_over_fp64": dt / fdt,
                    "phase_table_ms": phase_table,
-                   "host_entry_s_per_iter": host_dt},
+                   "host_entry_s_per_iter": host_dt,
+                   "solve_refined_1rhs_s": refine_dt, "solve_refined_corrections": rcount,
+                   "solve_refined_residual_log2": float(np.log2(max(mp.to_float64(rnorm)[0], 1e-300)))},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": INT8_PEAK_OPS / 1e12,
                      "unit": "TOP/s (INT8)", "frac": achieved / INT8_PEAK_OPS,
                      "traffic": product_traffic("sampled") if t == 2000 else None,

@@ -293,18 +293,27 @@ static uint64_t powmod_u64(uint64_t b, uint64_t e, uint64_t q) {
     return r;
 }
 
-/* residue of the value at index idx, as an integer at window lo, mod q, in (-q/2, q/2) */
-static double rns_residue(int n, const void *base, size_t stride, size_t idx, int64_t lo, uint64_t q) {
+/* residue of the value at index idx, as an integer at window lo, mod q, in
+ * (-q/2, q/2); pw[s] = 2^s mod q for s up to the operand's width */
+static double rns_residue(int n, const void *base, size_t stride, size_t idx, int64_t lo, uint64_t q,
+                          const uint64_t *pw) {
     int32_t k = kind_of(base, stride, idx, n);
     if (abs(k) != KIND_REG) return 0.0;
     const uint64_t *l = limbs_of(base, stride, idx);
-    uint64_t r = 0;
-    const uint64_t b64 = (uint64_t)(((u128)1 << 64) % q);
-    for (int i = n - 1; i >= 0; i--) r = (uint64_t)(((u128)r * b64 + l[i] % q) % q);
     int64_t sh = (int64_t)exp_of(base, stride, idx, n) - 64 * (int64_t)n - lo;
-    r = (uint64_t)((u128)r * powmod_u64(2, (uint64_t)sh, q) % q);
+    uint64_t r = 0;
+    for (int i = 0; i < n; i++) r += (uint64_t)((u128)(l[i] % q) * pw[sh + 64 * i] % q);
+    r %= q;
     if (k < 0 && r) r = q - r;
     return r > (q - 1) / 2 ? (double)r - (double)q : (double)r;
+}
+
+/* x mod q in [0, q) for an integer-valued x with |x| < 2^53 */
+static inline double mod_exact(double x, double q, double qinv) {
+    double r = x - q * nearbyint(x * qinv);
+    if (r < 0) r += q;
+    if (r >= q) r -= q;
+    return r;
 }
 
 static uint64_t inv_mod(uint64_t a, uint64_t q) { return powmod_u64(a, q - 2, q); }
@@ -343,15 +352,31 @@ int oracle_gemm_rns(int n, char ta, char tb, size_t m, size_t nn, size_t k, cons
     const size_t kc = (size_t)1 << (53 + 2 - 2 * bits < 40 ? 53 + 2 - 2 * bits : 40);
     /* residues: A rows (i, p) at i k + p, B columns (p, j) at j k + p */
     double *ra = malloc(sizeof(double) * (size_t)L * m * k), *rb = malloc(sizeof(double) * (size_t)L * nn * k);
-#pragma omp parallel for schedule(static) num_threads(threads > 0 ? threads : 1)
-    for (long long qi = 0; qi < (long long)L; qi++) {
-        for (size_t i = 0; i < m; i++)
-            for (size_t p = 0; p < k; p++)
-                ra[((size_t)qi * m + i) * k + p] = rns_residue(n, a, stride, tan ? i + p * lda : p + i * lda, la, qs[qi]);
-        for (size_t j = 0; j < nn; j++)
-            for (size_t p = 0; p < k; p++)
-                rb[((size_t)qi * nn + j) * k + p] = rns_residue(n, b, stride, tbn ? p + j * ldb : j + p * ldb, lb, qs[qi]);
+    const size_t npw = (size_t)(P + (ha - la > hb - lb ? ha - la : hb - lb)) + 64;
+    uint64_t *pw = malloc(8 * (size_t)L * npw);
+    for (int q = 0; q < L; q++) {
+        uint64_t *t = pw + (size_t)q * npw;
+        t[0] = 1 % qs[q];
+        for (size_t e = 1; e < npw; e++) t[e] = (t[e - 1] * 2) % qs[q];
     }
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads > 0 ? threads : 1)
+    for (long long row = 0; row < (long long)(m + nn); row++) {
+        for (int q = 0; q < L; q++) {
+            const uint64_t *t = pw + (size_t)q * npw;
+            if ((size_t)row < m) {
+                size_t i = (size_t)row;
+                for (size_t p = 0; p < k; p++)
+                    ra[((size_t)q * m + i) * k + p] =
+                        rns_residue(n, a, stride, tan ? i + p * lda : p + i * lda, la, qs[q], t);
+            } else {
+                size_t j = (size_t)row - m;
+                for (size_t p = 0; p < k; p++)
+                    rb[((size_t)q * nn + j) * k + p] =
+                        rns_residue(n, b, stride, tbn ? p + j * ldb : j + p * ldb, lb, qs[q], t);
+            }
+        }
+    }
+    free(pw);
     /* M, M/q (limbs) and the CRT weights (M/q)^-1 mod q */
     int lm = (int)(have / 64.0) + 2;
     uint64_t *M = calloc((size_t)lm, 8), *Mq = calloc((size_t)L * lm, 8), *w = malloc(8 * (size_t)L);
@@ -390,15 +415,16 @@ int oracle_gemm_rns(int n, char ta, char tb, size_t m, size_t nn, size_t k, cons
         for (size_t i = 0; i < iend; i++) {
             for (int q = 0; q < L; q++) {
                 const double *x = ra + ((size_t)q * m + i) * k, *y = rb + ((size_t)q * nn + j) * k;
-                const double qd = (double)qs[q];
+                const double qd = (double)qs[q], qinv = 1.0 / qd;
                 double r = 0.0;
                 for (size_t p0 = 0; p0 < k; p0 += kc) {
                     size_t p1 = p0 + kc < k ? p0 + kc : k;
                     double s = 0.0;
+                    /* every partial sum is an integer below 2^53: any order gives the same value */
+#pragma omp simd reduction(+ : s)
                     for (size_t p = p0; p < p1; p++) s += x[p] * y[p];
-                    r = fmod(r + fmod(s, qd), qd);
+                    r = mod_exact(r + mod_exact(s, qd, qinv), qd, qinv);
                 }
-                if (r < 0) r += qd;
                 cres[q] = (uint32_t)r;
             }
             /* S = sum x_q (M/q) - K M, K = round(sum x_q / q) */

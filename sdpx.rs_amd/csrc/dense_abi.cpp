@@ -394,14 +394,12 @@ int32_t sdpx_gpu_psd_nt_scaling_f64_dev(uint64_t nblocks, const uint64_t *dims, 
                                         double *r, double *rinv, double *lambda, double *g, int32_t *status) {
     return abi_boundary([&]() -> int32_t {
         if (nblocks && (!dims || !status)) return abi_invalid("null dims or status");
-        std::vector<int64_t> dv(nblocks);
         std::vector<sdpx::NtBlock> small, large;
         std::vector<size_t> small_idx, large_idx;
         int64_t so = 0, mo = 0, vo = 0, smax = 0, lmax = 0;
         for (uint64_t i = 0; i < nblocks; i++) {
             const int64_t n = (int64_t)dims[i];
             if (n > sdpx::NT_MAX_ORDER) return abi_invalid("PSD block order above 256");
-            dv[i] = n;
             const sdpx::NtBlock b{n, so, mo, vo};
             if (n == 0) {
                 status[i] = 0;
@@ -422,21 +420,27 @@ int32_t sdpx_gpu_psd_nt_scaling_f64_dev(uint64_t nblocks, const uint64_t *dims, 
         if (small.empty() && large.empty()) return SDPX_GPU_OK;
         sdpx::Device &d = sdpx::device();
         std::lock_guard<std::recursive_mutex> guard(d.mu);
-        const size_t nb = small.size() + large.size();
-        sdpx::NtBlock *db = (sdpx::NtBlock *)d.ws.get(170, sizeof(sdpx::NtBlock) * nb);
-        int *dst = (int *)d.ws.get(171, sizeof(int) * nb);
-        double *work = large.empty() ? nullptr
-                                     : (double *)d.ws.get(172, sizeof(double) * 3 * lmax * lmax * large.size());
-        std::vector<sdpx::NtBlock> all(small);
-        all.insert(all.end(), large.begin(), large.end());
-        HIP_CHECK(hipMemcpyAsync(db, all.data(), sizeof(sdpx::NtBlock) * nb, hipMemcpyHostToDevice, d.stream));
-        sdpx::nt_scaling_f64(dv, db, (int)small.size(), smax, db + small.size(), (int)large.size(), lmax, s, z, r,
-                             rinv, lambda, g, work, dst, dst + small.size(), 60, d.stream);
-        std::vector<int> hs(nb);
-        HIP_CHECK(hipMemcpyAsync(hs.data(), dst, sizeof(int) * nb, hipMemcpyDeviceToHost, d.stream));
-        HIP_CHECK(hipStreamSynchronize(d.stream));
-        for (size_t i = 0; i < small.size(); i++) status[small_idx[i]] = hs[i];
-        for (size_t i = 0; i < large.size(); i++) status[large_idx[i]] = hs[small.size() + i];
+        if (!small.empty()) {
+            sdpx::NtBlock *db = (sdpx::NtBlock *)d.ws.get(170, sizeof(sdpx::NtBlock) * small.size());
+            int *dst = (int *)d.ws.get(171, sizeof(int) * small.size());
+            HIP_CHECK(hipMemcpyAsync(db, small.data(), sizeof(sdpx::NtBlock) * small.size(), hipMemcpyHostToDevice,
+                                     d.stream));
+            sdpx::nt_scaling_f64(db, (int)small.size(), smax, s, z, r, rinv, lambda, g, dst, 60, d.stream);
+            std::vector<int> hs(small.size());
+            HIP_CHECK(hipMemcpyAsync(hs.data(), dst, sizeof(int) * hs.size(), hipMemcpyDeviceToHost, d.stream));
+            HIP_CHECK(hipStreamSynchronize(d.stream));
+            for (size_t i = 0; i < small.size(); i++) status[small_idx[i]] = hs[i];
+        }
+        if (!large.empty()) {
+            double *work = (double *)d.ws.get(172, sdpx::nt_scaling_large_workspace_bytes(lmax));
+            for (size_t i = 0; i < large.size(); i++) {
+                const sdpx::NtBlock &b = large[i];
+                status[large_idx[i]] = sdpx::nt_scaling_large_f64(b.n, s + b.svec_off, z + b.svec_off, r + b.mat_off,
+                                                                  rinv + b.mat_off, lambda + b.vec_off,
+                                                                  g + b.mat_off, work, 60, d.stream);
+            }
+            HIP_CHECK(hipStreamSynchronize(d.stream));
+        }
         return SDPX_GPU_OK;
     });
 }

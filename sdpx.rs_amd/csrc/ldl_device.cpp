@@ -524,8 +524,13 @@ int sampled_arrow_solve_device(int limbs, const std::vector<size_t> &ks, size_t 
         if (c.declined) return -1;
         mp_ew_sub_rect(limbs, xb, (int64_t)ldx, P, (int64_t)t, (int64_t)t, (int64_t)nrhs, s);
     }
-    // trunk: the border factor's sweeps (backward blocks of 32, FORWARD_BLOCK)
-    if (t) {
+    // trunk: the border factor's sweeps (backward blocks of 32, FORWARD_BLOCK);
+    // a border of more than 64 rows spreads each block's updates over the chip
+    if (t > 64) {
+        mp_sweep_blocked(limbs, d_border, (int64_t)t, (int64_t)nrhs, xb, (int64_t)ldx, false, 32, s);
+        mp_ew_mul_diag(limbs, xb, (int64_t)ldx, d_border_dinv, xb, (int64_t)ldx, (int64_t)t, (int64_t)nrhs, true, s);
+        mp_sweep_blocked(limbs, d_border, (int64_t)t, (int64_t)nrhs, xb, (int64_t)ldx, true, 32, s);
+    } else if (t) {
         mp_arrow_forward(limbs, d_border, d_border_dinv, d_bd, 1, (int64_t)t, (int64_t)nrhs, d_x, nullptr, nullptr, 0,
                          s);
         mp_arrow_backward(limbs, d_border, d_bd, 1, (int64_t)t, (int64_t)nrhs, d_x, 32, s);

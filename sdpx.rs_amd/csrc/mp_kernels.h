@@ -85,6 +85,12 @@ void mp_arrow_forward(int limbs, const void *L_all, const void *dinv_all, const 
                       int64_t nmax, int64_t cols, void *X, void *Wout, void *Vout, int64_t ldw, hipStream_t st);
 void mp_arrow_backward(int limbs, const void *L_all, const void *descs, int ndesc, int64_t nmax, int64_t cols,
                        void *X, int back, hipStream_t st);
+// The same sweeps for one factor L (n x n, ld n) of any order, its
+// right-hand sides X (n x cols, ldx) in place: blocks of 32 rows forward,
+// of `back` (<= 32) rows from the bottom backward, each block's own rows
+// on one wave and every other row's updates spread over the chip.
+void mp_sweep_blocked(int limbs, const void *L, int64_t n, int64_t cols, void *X, int64_t ldx, bool backward,
+                      int back, hipStream_t st);
 void mp_zero_values(int limbs, void *P, int64_t count, hipStream_t st);
 // hi = RN(d x), lo = d x - hi (exact) for an m x n block of X; d per row
 // (mode 0), per column (1) or a single value (2)

@@ -2101,6 +2101,154 @@ __global__ __launch_bounds__(1024) void arrow_backward_kernel(const uint32_t *L_
 }
 
 // ---------------------------------------------------------------------------
+// Blocked FMA sweeps of one large factor (the border's): the same sequence
+// of single-rounding updates as the two kernels above, with the rows after
+// (before) a block spread over many workgroups.  Per block of b <= 32 rows:
+// a diagonal kernel finishes the block's own rows on one wave per
+// right-hand side (the block's strictly lower triangle of L staged in LDS
+// first), then an update kernel applies the block's b columns, ascending,
+// to every later (earlier) row, one thread a row, L's entries fetched four
+// steps ahead.  Forward: row i takes k ascending, whatever the blocks.
+// Backward (blocks of `back` rows from the bottom): the block's rows finish
+// from its last, each over the block's later rows ascending; then every
+// earlier row applies the whole block ascending.
+// ---------------------------------------------------------------------------
+constexpr int SWEEP_BLOCK = 32;
+
+// the block's strictly lower triangle, entry (i, k) (k < i, both relative to
+// r0) at tri(i) + k, into LDS
+template <int W>
+__device__ __forceinline__ void stage_block_tri(const uint32_t *L, int64_t n, int64_t r0, int b, uint32_t *sh) {
+    const int total = b * (b - 1) / 2;
+    for (int e = threadIdx.x; e < total * (W + 2); e += blockDim.x) {
+        const int rec = e / (W + 2), word = e % (W + 2);
+        int i = (int)((sqrtf(8.0f * rec + 1.0f) + 1.0f) * 0.5f);
+        while (i * (i - 1) / 2 > rec) i--;
+        while ((i + 1) * i / 2 <= rec) i++;
+        const int k = rec - i * (i - 1) / 2;
+        sh[e] = rec_at<W>(L, r0 + i + (r0 + k) * n)[word];
+    }
+}
+
+template <int W>
+__global__ __launch_bounds__(64) void sweep_fwd_diag_kernel(const uint32_t *L, int64_t n, uint32_t *X, int64_t ldx,
+                                                            int64_t k0, int64_t k1) {
+    extern __shared__ uint32_t tri_sh[];
+    __shared__ uint32_t xk_sh[2][W + 2];
+    uint32_t *x = X + (int64_t)blockIdx.x * ldx * (W + 2);
+    const int b = (int)(k1 - k0), lane = threadIdx.x;
+    stage_block_tri<W>(L, n, k0, b, tri_sh);
+    const bool mine = lane < b;
+    Val<W> v;
+    if (mine) load(rec_at<W>((const uint32_t *)x, k0 + lane), v);
+    __syncthreads();
+    for (int k = 0; k < b; k++) {
+        const int par = k & 1;
+        if (lane == k) store(xk_sh[par], v);  // final: every k' < k applied
+        __syncthreads();
+        if (mine && lane > k) {
+            Val<W> xk, l, t;
+            load((const uint32_t *)xk_sh[par], xk);
+            load((const uint32_t *)(tri_sh + (lane * (lane - 1) / 2 + k) * (W + 2)), l);
+            fms(v, l, xk, t);
+            v = t;
+        }
+    }
+    if (mine) store(rec_at<W>(x, k0 + lane), v);
+}
+
+// v = RN(... RN(v - l_0 xb_0) ... - l_(b-1) xb_(b-1)), l_u at L + (base + u step)
+// records, four fetches in flight
+template <int W>
+__device__ __forceinline__ void apply_block(Val<W> &v, const uint32_t *L, int64_t base, int64_t step,
+                                            const uint32_t (*xb)[W + 2], int b) {
+    Val<W> r0, r1, r2, r3, xk, t;
+    auto fetch = [&](int u, Val<W> &dst) {
+        if (u < b) load(rec_at<W>(L, base + u * step), dst);
+    };
+    auto apply = [&](int u, const Val<W> &l) {
+        if (u < b) {
+            load((const uint32_t *)xb[u], xk);
+            fms(v, l, xk, t);
+            v = t;
+        }
+    };
+    fetch(0, r0);
+    fetch(1, r1);
+    fetch(2, r2);
+    fetch(3, r3);
+    for (int u = 0; u < b; u += 4) {
+        apply(u, r0);
+        fetch(u + 4, r0);
+        apply(u + 1, r1);
+        fetch(u + 5, r1);
+        apply(u + 2, r2);
+        fetch(u + 6, r2);
+        apply(u + 3, r3);
+        fetch(u + 7, r3);
+    }
+}
+
+template <int W>
+__global__ __launch_bounds__(64) void sweep_fwd_update_kernel(const uint32_t *L, int64_t n, uint32_t *X,
+                                                              int64_t ldx, int64_t k0, int64_t k1) {
+    __shared__ uint32_t xb[SWEEP_BLOCK][W + 2];
+    uint32_t *x = X + (int64_t)blockIdx.y * ldx * (W + 2);
+    const int b = (int)(k1 - k0);
+    for (int u = threadIdx.x; u < b * (W + 2); u += 64) xb[u / (W + 2)][u % (W + 2)] = rec_at<W>(x, k0)[u];
+    __syncthreads();
+    const int64_t i = k1 + (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (i >= n) return;
+    Val<W> v;
+    load(rec_at<W>((const uint32_t *)x, i), v);
+    apply_block<W>(v, L, i + k0 * n, n, xb, b);  // L(i, k0 + u)
+    store(rec_at<W>(x, i), v);
+}
+
+template <int W>
+__global__ __launch_bounds__(64) void sweep_bwd_diag_kernel(const uint32_t *L, int64_t n, uint32_t *X, int64_t ldx,
+                                                            int64_t r0, int64_t r1) {
+    extern __shared__ uint32_t tri_sh[];
+    __shared__ uint32_t xb[SWEEP_BLOCK][W + 2];
+    uint32_t *x = X + (int64_t)blockIdx.x * ldx * (W + 2);
+    const int b = (int)(r1 - r0), lane = threadIdx.x;
+    stage_block_tri<W>(L, n, r0, b, tri_sh);
+    Val<W> v;
+    if (lane < b) load(rec_at<W>((const uint32_t *)x, r0 + lane), v);
+    __syncthreads();
+    for (int row = b - 1; row >= 0; row--) {
+        if (lane == row) {
+            Val<W> l, xk, t;
+            for (int k = row + 1; k < b; k++) {  // L(k, row): entry (k, row) of the block's triangle
+                load((const uint32_t *)(tri_sh + (k * (k - 1) / 2 + row) * (W + 2)), l);
+                load((const uint32_t *)xb[k], xk);
+                fms(v, l, xk, t);
+                v = t;
+            }
+            store(xb[row], v);
+        }
+        __syncthreads();
+    }
+    if (lane < b) store(rec_at<W>(x, r0 + lane), v);
+}
+
+template <int W>
+__global__ __launch_bounds__(64) void sweep_bwd_update_kernel(const uint32_t *L, int64_t n, uint32_t *X,
+                                                              int64_t ldx, int64_t r0, int64_t r1) {
+    __shared__ uint32_t xb[SWEEP_BLOCK][W + 2];
+    uint32_t *x = X + (int64_t)blockIdx.y * ldx * (W + 2);
+    const int b = (int)(r1 - r0);
+    for (int u = threadIdx.x; u < b * (W + 2); u += 64) xb[u / (W + 2)][u % (W + 2)] = rec_at<W>(x, r0)[u];
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (i >= r0) return;
+    Val<W> v;
+    load(rec_at<W>((const uint32_t *)x, i), v);
+    apply_block<W>(v, L, r0 + i * n, 1, xb, b);  // L(r0 + u, i)
+    store(rec_at<W>(x, i), v);
+}
+
+// ---------------------------------------------------------------------------
 // Helpers of the exact multilinear forms (exact_forms.cpp)
 // ---------------------------------------------------------------------------
 // The exact product d x as hi + lo: hi = RN(d x), lo = x d - hi = -RN(hi -
@@ -2690,6 +2838,36 @@ void mp_arrow_backward(int limbs, const void *L_all, const void *descs, int ndes
     SDPX_DISPATCH_W(limbs, SDPX_DISPATCH_R(R, hipLaunchKernelGGL((arrow_backward_kernel<W, R>),
         dim3((unsigned)cols, ndesc), dim3(threads), lds, st, (const uint32_t *)L_all, (const SweepDesc *)descs,
         (uint32_t *)X, back)));
+    HIP_CHECK(hipGetLastError());
+}
+
+void mp_sweep_blocked(int limbs, const void *L, int64_t n, int64_t cols, void *X, int64_t ldx, bool backward,
+                      int back, hipStream_t st) {
+    if (n == 0 || cols == 0) return;
+    if (back < 1 || back > SWEEP_BLOCK) throw Error(SDPX_GPU_INVALID, "sweep block above 32 rows");
+    SDPX_DISPATCH_W(limbs, {
+        const size_t tri = (size_t)SWEEP_BLOCK * (SWEEP_BLOCK - 1) / 2 * (W + 2) * 4;
+        if (!backward) {
+            for (int64_t k0 = 0; k0 < n; k0 += SWEEP_BLOCK) {
+                const int64_t k1 = std::min<int64_t>(n, k0 + SWEEP_BLOCK);
+                hipLaunchKernelGGL(sweep_fwd_diag_kernel<W>, dim3((unsigned)cols), dim3(64), tri, st,
+                                   (const uint32_t *)L, n, (uint32_t *)X, ldx, k0, k1);
+                if (k1 < n)
+                    hipLaunchKernelGGL(sweep_fwd_update_kernel<W>, dim3((unsigned)((n - k1 + 63) / 64), (unsigned)cols),
+                                       dim3(64), 0, st, (const uint32_t *)L, n, (uint32_t *)X, ldx, k0, k1);
+            }
+        } else {
+            for (int64_t r1 = n; r1 > 0;) {
+                const int64_t r0 = r1 > back ? r1 - back : 0;
+                hipLaunchKernelGGL(sweep_bwd_diag_kernel<W>, dim3((unsigned)cols), dim3(64), tri, st,
+                                   (const uint32_t *)L, n, (uint32_t *)X, ldx, r0, r1);
+                if (r0 > 0)
+                    hipLaunchKernelGGL(sweep_bwd_update_kernel<W>, dim3((unsigned)((r0 + 63) / 64), (unsigned)cols),
+                                       dim3(64), 0, st, (const uint32_t *)L, n, (uint32_t *)X, ldx, r0, r1);
+                r1 = r0;
+            }
+        }
+    });
     HIP_CHECK(hipGetLastError());
 }
 

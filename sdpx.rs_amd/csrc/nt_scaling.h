@@ -16,16 +16,17 @@ struct NtBlock {
 
 constexpr int64_t NT_MAX_ORDER = 256;
 
-// Global workspace for the blocks of order above 64 (three n x n matrices
-// each; the smaller blocks work in LDS).
-size_t nt_scaling_workspace_bytes(const std::vector<int64_t> &dims);
+// Blocks of order <= 64 (d_blocks, device), one workgroup each: status per
+// block 0 ok, 1 S not positive definite, 2 Z not positive definite, 3 the
+// Jacobi sweeps did not converge.
+void nt_scaling_f64(const NtBlock *d_blocks, int nblocks, int64_t nmax, const double *s_all, const double *z_all,
+                    double *r_all, double *rinv_all, double *lam_all, double *g_all, int *d_status, int max_sweeps,
+                    hipStream_t st);
 
-// Blocks of order <= 64 (d_small, device, nsmall of them, largest small_max)
-// and the larger ones (d_large): status per block 0 ok, 1 S not positive
-// definite, 2 Z not positive definite, 3 the Jacobi sweeps did not converge.
-void nt_scaling_f64(const std::vector<int64_t> &dims, const NtBlock *d_small, int nsmall, int64_t small_max,
-                    const NtBlock *d_large, int nlarge, int64_t large_max, const double *s_all, const double *z_all,
-                    double *r_all, double *rinv_all, double *lam_all, double *g_all, double *work,
-                    int *d_status_small, int *d_status_large, int max_sweeps, hipStream_t st);
+// One block of order 65..256 spread over the chip (see nt_scaling.hip): the
+// status code as above; work holds nt_scaling_large_workspace_bytes(n).
+size_t nt_scaling_large_workspace_bytes(int64_t n);
+int nt_scaling_large_f64(int64_t n, const double *s, const double *z, double *R, double *Rinv, double *lam_out,
+                         double *G, double *work, int max_sweeps, hipStream_t st);
 
 }  // namespace sdpx
