@@ -1574,8 +1574,11 @@ __global__ __launch_bounds__(3 * ROW_THREADS) void ldl_col_wave_kernel(
 constexpr uint64_t LDL_WAIT_LIMIT = 200000000ull;  // 2 s of the 100 MHz clock
 
 __device__ __forceinline__ bool ldl_wait(const int *flag, int target, const int *status) {
+    // The poll is a relaxed load; one agent-scope acquire fence after the
+    // flag is seen orders the later loads (an acquire load per poll would
+    // invalidate the L2 of this XCD on every pass).
     const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
-    while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
         if ((__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) |
              __hip_atomic_load(status + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 0)
             return false;
@@ -1585,6 +1588,7 @@ __device__ __forceinline__ bool ldl_wait(const int *flag, int target, const int 
         }
         __builtin_amdgcn_s_sleep(1);
     }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     return true;
 }
 
