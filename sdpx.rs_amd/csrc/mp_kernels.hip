@@ -2236,6 +2236,39 @@ __global__ __launch_bounds__(64) void sweep_bwd_diag_kernel(const uint32_t *L, i
     if (lane < b) store(rec_at<W>(x, r0 + lane), v);
 }
 
+// The same on the whole wave: each single-rounding update of the block's
+// rows is one wave fms (wmp::fms, the register fms's bits), the block's
+// triangle and finished rows in LDS, operands fetched a step ahead.
+template <int W>
+__global__ __launch_bounds__(64) void sweep_bwd_diag_wave_kernel(const uint32_t *L, int64_t n, uint32_t *X,
+                                                                 int64_t ldx, int64_t r0, int64_t r1) {
+    extern __shared__ uint32_t tri_sh[];
+    __shared__ uint32_t xb[SWEEP_BLOCK][W + 2];
+    uint32_t *x = X + (int64_t)blockIdx.x * ldx * (W + 2);
+    const int b = (int)(r1 - r0);
+    stage_block_tri<W>(L, n, r0, b, tri_sh);
+    __syncthreads();
+    for (int row = b - 1; row >= 0; row--) {
+        wmp::WV<W> v = wmp::load<W>(rec_at<W>((const uint32_t *)x, r0 + row));
+        uint32_t lw = 0u, xw = 0u;
+        if (row + 1 < b) {
+            lw = wmp::load_word<W>(tri_sh + ((row + 1) * row / 2 + row) * (W + 2));
+            xw = wmp::load_word<W>(xb[row + 1]);
+        }
+        for (int k = row + 1; k < b; k++) {  // L(k, row) at entry (k, row) of the triangle, k ascending
+            const wmp::WV<W> l = wmp::from_word<W>(lw), xk = wmp::from_word<W>(xw);
+            if (k + 1 < b) {
+                lw = wmp::load_word<W>(tri_sh + ((k + 1) * k / 2 + row) * (W + 2));
+                xw = wmp::load_word<W>(xb[k + 1]);
+            }
+            v = wmp::fms<W>(v, l, xk);
+        }
+        wmp::store(xb[row], v);
+        __threadfence_block();  // the next rows read this record's words from other lanes
+        wmp::store(rec_at<W>(x, r0 + row), v);
+    }
+}
+
 template <int W>
 __global__ __launch_bounds__(64) void sweep_bwd_update_kernel(const uint32_t *L, int64_t n, uint32_t *X,
                                                               int64_t ldx, int64_t r0, int64_t r1) {
@@ -2669,6 +2702,9 @@ static bool trsm_right_looking() {
 // "ldl_one_launch" (default 1): ldl_dots blocks in one launch of
 // ldl_rows_kernel (rows ordered by flags) instead of one launch per column.
 static std::atomic<int> g_ldl_one_launch{1};
+// "sweep_wave" 1 (default): the border's backward diagonal blocks on the
+// whole wave (sweep_bwd_diag_wave_kernel); 0: one lane a row.
+static std::atomic<int> g_sweep_wave{1};
 static bool ldl_one_launch() { return g_ldl_one_launch.load() != 0; }
 // rows of one launch of ldl_rows_kernel at most (larger blocks take the
 // column launches)
@@ -2677,6 +2713,10 @@ constexpr int64_t LDL_ROWS_MAX = 4096;
 bool mp_set_tuning(const char *name, int64_t value) {
     if (!std::strcmp(name, "ldl_one_launch")) {
         g_ldl_one_launch = value ? 1 : 0;
+        return true;
+    }
+    if (!std::strcmp(name, "sweep_wave")) {
+        g_sweep_wave = value ? 1 : 0;
         return true;
     }
     if (!std::strcmp(name, "trsm_rl")) {
@@ -2863,8 +2903,12 @@ void mp_sweep_blocked(int limbs, const void *L, int64_t n, int64_t cols, void *X
         } else {
             for (int64_t r1 = n; r1 > 0;) {
                 const int64_t r0 = r1 > back ? r1 - back : 0;
-                hipLaunchKernelGGL(sweep_bwd_diag_kernel<W>, dim3((unsigned)cols), dim3(64), tri, st,
-                                   (const uint32_t *)L, n, (uint32_t *)X, ldx, r0, r1);
+                if (g_sweep_wave.load())
+                    hipLaunchKernelGGL(sweep_bwd_diag_wave_kernel<W>, dim3((unsigned)cols), dim3(64), tri, st,
+                                       (const uint32_t *)L, n, (uint32_t *)X, ldx, r0, r1);
+                else
+                    hipLaunchKernelGGL(sweep_bwd_diag_kernel<W>, dim3((unsigned)cols), dim3(64), tri, st,
+                                       (const uint32_t *)L, n, (uint32_t *)X, ldx, r0, r1);
                 if (r0 > 0)
                     hipLaunchKernelGGL(sweep_bwd_update_kernel<W>, dim3((unsigned)((r0 + 63) / 64), (unsigned)cols),
                                        dim3(64), 0, st, (const uint32_t *)L, n, (uint32_t *)X, ldx, r0, r1);

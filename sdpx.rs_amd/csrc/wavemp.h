@@ -357,6 +357,48 @@ __device__ __forceinline__ int64_t add_product(int64_t T, const WV<W> &a, const 
     return k < SLOTS ? (neg ? T - (int64_t)word : T + (int64_t)word) : T;
 }
 
+// T += s * z << off for a magnitude spread one word a lane (z: this lane's
+// word; off >= 0 bits above slot 0, landing below slot SLOTS - 1).
+template <int SLOTS>
+__device__ __forceinline__ int64_t add_words(int64_t T, uint32_t z, int64_t off, bool negate) {
+    const int k = lane();
+    const int wo = (int)(off >> 5), bo = (int)(off & 31);
+    const uint32_t c0 = from_lane(z, k - wo), c1 = from_lane(z, k - wo - 1);
+    const uint32_t word = bo ? __builtin_amdgcn_alignbit(c0, c1, (uint32_t)(32 - bo)) : c0;
+    return k < SLOTS ? (negate ? T - (int64_t)word : T + (int64_t)word) : T;
+}
+
+// RN(c - a * b) with one rounding: fms's result.  c's words and the exact
+// product's 2W words go into one exact window at their bit offsets above
+// the lower of their two scales (the span stays below 3P + 2 bits, SLOTS =
+// 3W + 4), then round_window.  Two far cases: a product below a quarter of
+// c's last unit leaves c (RN(c - t) = c for 0 < |t| < ulp(c) / 4, also at a
+// power of two); a c below the product's last unit by two bits is replaced
+// by a same-signed unit one slot below the product (both lie strictly
+// between the same multiples of the product's last unit, so they round
+// alike).
+template <int W>
+__device__ __forceinline__ WV<W> fms(const WV<W> &c, const WV<W> &a, const WV<W> &b) {
+    if (is_zero(a) || is_zero(b)) return is_zero(c) ? zero<W>() : c;
+    constexpr int P = 32 * W, SLOTS = 3 * W + 4;
+    static_assert(SLOTS <= 64, "one slot a lane");
+    const int64_t pe = (int64_t)a.e + b.e - 2 * P;  // scale of the product's word 0
+    if (is_zero(c)) return round<W>((a.kind < 0) == (b.kind < 0), product_words(a, b), pe);
+    const int64_t ce = (int64_t)c.e - P;  // scale of c's word 0: c's last unit
+    if (pe + 2 * P <= ce - 2) return c;
+    int64_t T, base;
+    if (ce + P <= pe - 2) {
+        base = pe - 32;
+        T = lane() == 0 ? (c.kind < 0 ? -1 : 1) : 0;
+        T = add_product<W, SLOTS>(T, a, b, 32, true);
+    } else {
+        base = ce < pe ? ce : pe;
+        T = add_words<SLOTS>(0, c.w, ce - base, c.kind < 0);
+        T = add_product<W, SLOTS>(T, a, b, pe - base, true);
+    }
+    return round_window<W, SLOTS>(T, base);
+}
+
 // Sign test of a - b for regular or zero operands: -1, 0 or 1 (the sign of
 // add(a, b, true) without rounding it).
 template <int W>

@@ -41,7 +41,8 @@ __device__ void window_serial(int64_t (&t)[S], int64_t base, mpd::Val<W> &out) {
 
 // op: 0 a + b, 1 a - b, 2 a * b, 3 1 / a, 4 sign of a - b (word 0 of the
 // output, +1 / 0 / -1), 5 rounding of the window T, 6 the window T plus
-// (a * b) << off, then its rounding.
+// (a * b) << off, then its rounding, 7 RN(c - a * b) with c the record at
+// the start of the case's row of T.
 template <int W>
 __global__ __launch_bounds__(64) void wmp_case(int op, const uint32_t *A, const uint32_t *B, const int64_t *T,
                                                const int64_t *bo, int n, uint32_t *S, uint32_t *O) {
@@ -74,6 +75,12 @@ __global__ __launch_bounds__(64) void wmp_case(int op, const uint32_t *A, const 
         vr.m[0] = mpd::is_zero(d) ? 0u : (d.kind < 0 ? 0xFFFFFFFFu : 1u);
         const int s = wmp::compare(wa, wb);
         wr.w = k == 0 ? (uint32_t)s : 0u;
+    } else if (op == 7) {
+        const uint32_t *cr = (const uint32_t *)(T + (size_t)c * 64);
+        mpd::Val<W> vc;
+        mpd::load(cr, vc);
+        mpd::fms(vc, va, vb, vr);
+        wr = wmp::fms(wmp::load<W>(cr), wa, wb);
     } else {
         int64_t t[SL];
         for (int s = 0; s < SL; s++) t[s] = T[(size_t)c * 64 + s];

@@ -4,8 +4,9 @@ of mpdev.h (themselves pinned to the host library and the oracle by
 test_mpdev.py) on the GPU: sums and differences across every exponent gap
 class (cancellation, gaps around the word size and around the precision,
 exact ties), products (with overflow to infinity and underflow to zero),
-reciprocals, sign tests, and the rounding of exact-dot windows with and
-without an added exact product.  Each case runs on one wave in both forms;
+reciprocals, sign tests, the rounding of exact-dot windows with and
+without an added exact product, and the fused multiply-subtract
+RN(c - a b) over every gap class between c and the product.  Each case runs on one wave in both forms;
 tests/native/libwavemp_selftest.so is built by build()."""
 import ctypes
 import os
@@ -193,3 +194,38 @@ def test_wave_window_rounding(gpu, nl):
     T[n // 5: n // 3, :] = 0                  # the product alone (the chains above stay)
     S, O = _run(lib, nl, 6, a, b, T, bo)
     _check(S, O, "window + a * b")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nl", [2, 4, 8])
+def test_wave_fms(gpu, nl):
+    lib = _lib()
+    rng = np.random.default_rng(400 + nl)
+    P = 64 * nl
+    # products equal to a (b = +-1): every gap class of a difference, with
+    # cancellation, ties and all-ones carries
+    c1, a1 = _operands(nl, rng)
+    b1 = _pow2(a1.size, nl, np.ones(a1.size, np.int64), rng.integers(0, 2, a1.size).astype(bool))
+    # full 2W-word products with c placed at each gap class around the
+    # product's exponent a.e + b.e (both signs, the far cases' edges)
+    n = 4000
+    a2 = mp.random_full(n, nl, 30, rng, zero_fraction=0)
+    b2 = mp.random_full(n, nl, 30, rng, zero_fraction=0)
+    c2 = mp.random_full(n, nl, 0, rng, zero_fraction=0)
+    gaps = np.array([-3 * P - 64, -2 * P - 4, -2 * P - 3, -2 * P - 2, -2 * P - 1, -2 * P, -P - 1, -P, -P + 1,
+                     -64, -33, -32, -1, 0, 1, 2, 31, 32, 33, P - 1, P, P + 1, P + 2, P + 3, 2 * P, 3 * P + 64])
+    reg = (np.abs(a2["kind"]) == 3) & (np.abs(b2["kind"]) == 3) & (np.abs(c2["kind"]) == 3)
+    c2["exponent"] = np.where(reg, a2["exponent"] + b2["exponent"] + rng.choice(gaps, n), c2["exponent"])
+    # zeros in each operand
+    k = n // 20
+    for arr, sl in ((a2, slice(0, k)), (b2, slice(k, 2 * k)), (c2, slice(2 * k, 3 * k))):
+        arr["limbs"][sl] = 0
+        arr["kind"][sl] = mp.ZERO_KIND
+        arr["exponent"][sl] = 0
+    a = np.concatenate([a1, a2])
+    b = np.concatenate([b1, b2])
+    c = np.concatenate([c1, c2])
+    T = np.zeros((a.size, 64), np.int64)
+    T.view(np.uint32)[:, : 2 * nl + 2] = np.ascontiguousarray(c).view(np.uint32).reshape(a.size, 2 * nl + 2)
+    S, O = _run(lib, nl, 7, a, b, T=T)
+    _check(S, O, "c - a * b", (mp.REGULAR_KIND, mp.ZERO_KIND))
