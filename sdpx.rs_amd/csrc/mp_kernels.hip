@@ -2269,6 +2269,30 @@ __global__ __launch_bounds__(64) void sweep_bwd_diag_wave_kernel(const uint32_t 
     }
 }
 
+// Update of one row per wave (wave fms): row i applies the block's b
+// finished values ascending, L's entry of each step fetched a step ahead.
+// Forward: rows i >= k1 take L(i, k0 + u); backward: rows i < k0 take
+// L(k0 + u, i).
+template <int W>
+__global__ __launch_bounds__(64) void sweep_update_wave_kernel(const uint32_t *L, int64_t n, uint32_t *X, int64_t ldx,
+                                                               int64_t k0, int64_t k1, int backward) {
+    __shared__ uint32_t xb[SWEEP_BLOCK][W + 2];
+    uint32_t *x = X + (int64_t)blockIdx.y * ldx * (W + 2);
+    const int b = (int)(k1 - k0);
+    for (int u = threadIdx.x; u < b * (W + 2); u += 64) xb[u / (W + 2)][u % (W + 2)] = rec_at<W>(x, k0)[u];
+    __syncthreads();
+    const int64_t i = backward ? (int64_t)blockIdx.x : k1 + (int64_t)blockIdx.x;
+    const int64_t base = backward ? k0 + i * n : i + k0 * n, step = backward ? 1 : n;
+    wmp::WV<W> v = wmp::load<W>(rec_at<W>((const uint32_t *)x, i));
+    uint32_t lw = wmp::load_word<W>(rec_at<W>(L, base));
+    for (int u = 0; u < b; u++) {
+        const wmp::WV<W> l = wmp::from_word<W>(lw);
+        if (u + 1 < b) lw = wmp::load_word<W>(rec_at<W>(L, base + (u + 1) * step));
+        v = wmp::fms<W>(v, l, wmp::load<W>(xb[u]));
+    }
+    wmp::store(rec_at<W>(x, i), v);
+}
+
 template <int W>
 __global__ __launch_bounds__(64) void sweep_bwd_update_kernel(const uint32_t *L, int64_t n, uint32_t *X,
                                                               int64_t ldx, int64_t r0, int64_t r1) {
@@ -2702,8 +2726,9 @@ static bool trsm_right_looking() {
 // "ldl_one_launch" (default 1): ldl_dots blocks in one launch of
 // ldl_rows_kernel (rows ordered by flags) instead of one launch per column.
 static std::atomic<int> g_ldl_one_launch{1};
-// "sweep_wave" 1 (default): the border's backward diagonal blocks on the
-// whole wave (sweep_bwd_diag_wave_kernel); 0: one lane a row.
+// "sweep_wave" 1 (default): the border's backward diagonal blocks and the
+// blocks' updates of the other rows on whole waves (wave fms); 0: one lane
+// a row.
 static std::atomic<int> g_sweep_wave{1};
 static bool ldl_one_launch() { return g_ldl_one_launch.load() != 0; }
 // rows of one launch of ldl_rows_kernel at most (larger blocks take the
@@ -2896,7 +2921,10 @@ void mp_sweep_blocked(int limbs, const void *L, int64_t n, int64_t cols, void *X
                 const int64_t k1 = std::min<int64_t>(n, k0 + SWEEP_BLOCK);
                 hipLaunchKernelGGL(sweep_fwd_diag_kernel<W>, dim3((unsigned)cols), dim3(64), tri, st,
                                    (const uint32_t *)L, n, (uint32_t *)X, ldx, k0, k1);
-                if (k1 < n)
+                if (k1 < n && g_sweep_wave.load())
+                    hipLaunchKernelGGL(sweep_update_wave_kernel<W>, dim3((unsigned)(n - k1), (unsigned)cols), dim3(64),
+                                       0, st, (const uint32_t *)L, n, (uint32_t *)X, ldx, k0, k1, 0);
+                else if (k1 < n)
                     hipLaunchKernelGGL(sweep_fwd_update_kernel<W>, dim3((unsigned)((n - k1 + 63) / 64), (unsigned)cols),
                                        dim3(64), 0, st, (const uint32_t *)L, n, (uint32_t *)X, ldx, k0, k1);
             }
@@ -2909,7 +2937,10 @@ void mp_sweep_blocked(int limbs, const void *L, int64_t n, int64_t cols, void *X
                 else
                     hipLaunchKernelGGL(sweep_bwd_diag_kernel<W>, dim3((unsigned)cols), dim3(64), tri, st,
                                        (const uint32_t *)L, n, (uint32_t *)X, ldx, r0, r1);
-                if (r0 > 0)
+                if (r0 > 0 && g_sweep_wave.load())
+                    hipLaunchKernelGGL(sweep_update_wave_kernel<W>, dim3((unsigned)r0, (unsigned)cols), dim3(64), 0, st,
+                                       (const uint32_t *)L, n, (uint32_t *)X, ldx, r0, r1, 1);
+                else if (r0 > 0)
                     hipLaunchKernelGGL(sweep_bwd_update_kernel<W>, dim3((unsigned)((r0 + 63) / 64), (unsigned)cols),
                                        dim3(64), 0, st, (const uint32_t *)L, n, (uint32_t *)X, ldx, r0, r1);
                 r1 = r0;
