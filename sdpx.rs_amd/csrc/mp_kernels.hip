@@ -2161,6 +2161,39 @@ __global__ __launch_bounds__(64) void sweep_fwd_diag_kernel(const uint32_t *L, i
     if (mine) store(rec_at<W>(x, k0 + lane), v);
 }
 
+// The same forward block on 16 waves, each holding two of the block's rows
+// in wave form (wave fms): at step k the owner of row k publishes x_k in
+// LDS, and after one barrier every wave applies it to its rows below k.
+constexpr int SWEEP_FWD_WAVES = SWEEP_BLOCK / 2;
+template <int W>
+__global__ __launch_bounds__(64 * SWEEP_FWD_WAVES) void sweep_fwd_diag_wave_kernel(const uint32_t *L, int64_t n,
+                                                                                   uint32_t *X, int64_t ldx,
+                                                                                   int64_t k0, int64_t k1) {
+    extern __shared__ uint32_t tri_sh[];
+    __shared__ uint32_t xk_sh[2][W + 2];
+    uint32_t *x = X + (int64_t)blockIdx.x * ldx * (W + 2);
+    const int b = (int)(k1 - k0), wave = threadIdx.x >> 6;
+    stage_block_tri<W>(L, n, k0, b, tri_sh);
+    const int ra = 2 * wave, rb = 2 * wave + 1;  // this wave's rows of the block
+    wmp::WV<W> va = wmp::zero<W>(), vb = wmp::zero<W>();
+    if (ra < b) va = wmp::load<W>(rec_at<W>((const uint32_t *)x, k0 + ra));
+    if (rb < b) vb = wmp::load<W>(rec_at<W>((const uint32_t *)x, k0 + rb));
+    __syncthreads();
+    for (int k = 0; k < b; k++) {
+        const int par = k & 1;
+        if (k == ra) wmp::store(xk_sh[par], va);  // final: every k' < k applied
+        if (k == rb) wmp::store(xk_sh[par], vb);
+        __syncthreads();
+        if (rb > k && rb < b) {
+            const wmp::WV<W> xk = wmp::load<W>(xk_sh[par]);
+            if (ra > k) va = wmp::fms<W>(va, wmp::load<W>(tri_sh + (ra * (ra - 1) / 2 + k) * (W + 2)), xk);
+            vb = wmp::fms<W>(vb, wmp::load<W>(tri_sh + (rb * (rb - 1) / 2 + k) * (W + 2)), xk);
+        }
+    }
+    if (ra < b) wmp::store(rec_at<W>(x, k0 + ra), va);
+    if (rb < b) wmp::store(rec_at<W>(x, k0 + rb), vb);
+}
+
 // v = RN(... RN(v - l_0 xb_0) ... - l_(b-1) xb_(b-1)), l_u at L + (base + u step)
 // records, four fetches in flight
 template <int W>
@@ -2726,9 +2759,9 @@ static bool trsm_right_looking() {
 // "ldl_one_launch" (default 1): ldl_dots blocks in one launch of
 // ldl_rows_kernel (rows ordered by flags) instead of one launch per column.
 static std::atomic<int> g_ldl_one_launch{1};
-// "sweep_wave" 1 (default): the border's backward diagonal blocks and the
-// blocks' updates of the other rows on whole waves (wave fms); 0: one lane
-// a row.
+// "sweep_wave" 1 (default): the border's diagonal blocks (forward and
+// backward) and the blocks' updates of the other rows on whole waves (wave
+// fms); 0: one lane a row.
 static std::atomic<int> g_sweep_wave{1};
 static bool ldl_one_launch() { return g_ldl_one_launch.load() != 0; }
 // rows of one launch of ldl_rows_kernel at most (larger blocks take the
@@ -2919,8 +2952,12 @@ void mp_sweep_blocked(int limbs, const void *L, int64_t n, int64_t cols, void *X
         if (!backward) {
             for (int64_t k0 = 0; k0 < n; k0 += SWEEP_BLOCK) {
                 const int64_t k1 = std::min<int64_t>(n, k0 + SWEEP_BLOCK);
-                hipLaunchKernelGGL(sweep_fwd_diag_kernel<W>, dim3((unsigned)cols), dim3(64), tri, st,
-                                   (const uint32_t *)L, n, (uint32_t *)X, ldx, k0, k1);
+                if (g_sweep_wave.load())
+                    hipLaunchKernelGGL(sweep_fwd_diag_wave_kernel<W>, dim3((unsigned)cols), dim3(64 * SWEEP_FWD_WAVES),
+                                       tri, st, (const uint32_t *)L, n, (uint32_t *)X, ldx, k0, k1);
+                else
+                    hipLaunchKernelGGL(sweep_fwd_diag_kernel<W>, dim3((unsigned)cols), dim3(64), tri, st,
+                                       (const uint32_t *)L, n, (uint32_t *)X, ldx, k0, k1);
                 if (k1 < n && g_sweep_wave.load())
                     hipLaunchKernelGGL(sweep_update_wave_kernel<W>, dim3((unsigned)(n - k1), (unsigned)cols), dim3(64),
                                        0, st, (const uint32_t *)L, n, (uint32_t *)X, ldx, k0, k1, 0);
