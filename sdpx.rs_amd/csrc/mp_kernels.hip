@@ -2048,7 +2048,7 @@ __global__ __launch_bounds__(1024) void arrow_forward_kernel(const uint32_t *L_a
 
 template <int W, int R>
 __global__ __launch_bounds__(1024) void arrow_backward_kernel(const uint32_t *L_all, const SweepDesc *descs,
-                                                             uint32_t *X, int back) {
+                                                             uint32_t *X, int back, int wave_chain) {
     extern __shared__ uint32_t blk_sh[];  // [back][W + 2]: the current block's final x
     const SweepDesc d = descs[blockIdx.y];
     const int64_t c = blockIdx.x, n = d.n, T = blockDim.x;
@@ -2064,6 +2064,33 @@ __global__ __launch_bounds__(1024) void arrow_backward_kernel(const uint32_t *L_
     for (int64_t r1 = n; r1 > 0;) {
         const int64_t r0 = r1 > back ? r1 - back : 0;
         // the block's rows, last first: row i applies k = i + 1 .. r1 - 1
+        if (wave_chain) {  // on wave 0 (wave fms), the rows' values through LDS
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int64_t i = threadIdx.x + r * T;
+                if (i >= r0 && i < r1) store(blk_sh + (i - r0) * (W + 2), v[r]);
+            }
+            __syncthreads();
+            if (threadIdx.x < 64) {
+                for (int64_t i = r1 - 1; i >= r0; i--) {
+                    wmp::WV<W> vw = wmp::load<W>(blk_sh + (i - r0) * (W + 2));
+                    uint32_t lw = i + 1 < r1 ? wmp::load_word<W>(rec_at<W>(L, (i + 1) + i * n)) : 0u;
+                    for (int64_t k = i + 1; k < r1; k++) {
+                        const wmp::WV<W> l = wmp::from_word<W>(lw);
+                        if (k + 1 < r1) lw = wmp::load_word<W>(rec_at<W>(L, (k + 1) + i * n));
+                        vw = wmp::fms<W>(vw, l, wmp::load<W>(blk_sh + (k - r0) * (W + 2)));
+                    }
+                    wmp::store(blk_sh + (i - r0) * (W + 2), vw);
+                    __threadfence_block();  // the next rows read this record's words from other lanes
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int64_t i = threadIdx.x + r * T;
+                if (i >= r0 && i < r1) load((const uint32_t *)(blk_sh + (i - r0) * (W + 2)), v[r]);
+            }
+        } else
         for (int64_t i = r1 - 1; i >= r0; i--) {
             if (threadIdx.x == i % T) {
 #pragma unroll
@@ -2937,9 +2964,12 @@ void mp_arrow_backward(int limbs, const void *L_all, const void *descs, int ndes
     const int R = sweep_rows(nmax, threads);
     if (R == 0) throw Error(SDPX_GPU_INVALID, "sweep order above 8192");
     const size_t lds = (size_t)back * (value_words(limbs)) * 4;
+    // the block's own rows on wave 0 in wave form when the workgroup has a
+    // whole wave (threads >= 64)
+    const int wave_chain = g_sweep_wave.load() && threads >= 64 ? 1 : 0;
     SDPX_DISPATCH_W(limbs, SDPX_DISPATCH_R(R, hipLaunchKernelGGL((arrow_backward_kernel<W, R>),
         dim3((unsigned)cols, ndesc), dim3(threads), lds, st, (const uint32_t *)L_all, (const SweepDesc *)descs,
-        (uint32_t *)X, back)));
+        (uint32_t *)X, back, wave_chain)));
     HIP_CHECK(hipGetLastError());
 }
 

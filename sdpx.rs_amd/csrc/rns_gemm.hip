@@ -928,24 +928,38 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void c
         // B fragments of both output tiles (mt = 0, 1) for every k-slice,
         // read once from the planes; every table block reuses them.  Planes
         // L .. lpad - 1 are allocated but never written: their table
-        // coefficients are zero, so whatever they hold adds nothing.
+        // coefficients are zero, so whatever they hold adds nothing.  The
+        // item's address is wave-uniform: one buffer resource per k-slice (16
+        // planes above it) in scalar registers, this lane's byte offset (its
+        // half's 8 planes, its output) in a vector register and each plane's
+        // offset in a scalar one, so no load computes an address on the
+        // vector unit.
         constexpr int NKS = 2 * NBMAX;
         v4i fb[2][NKS];
+        const uint64_t xbu = (uint64_t)(uintptr_t)xb;
+        const uint64_t xbs = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(xbu >> 32)) << 32) |
+                             (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)xbu);
+        const uint32_t pstride = (uint32_t)(xstride * sizeof(uint16_t));  // bytes between planes
+        const uint32_t loff = 8 * (uint32_t)h * pstride + (uint32_t)((64 * wave + col) * sizeof(uint16_t));
 #pragma unroll
-        for (int mt = 0; mt < 2; mt++)
+        for (int ks = 0; ks < NKS; ks++) {
+            fb[0][ks] = fb[1][ks] = v4i{0, 0, 0, 0};
+            if (ks < nks) {
+                const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                    (void *)(uintptr_t)(xbs + (uint64_t)(16 * ks) * pstride), (short)0, (int)0xFFFFFFFF, 0x00020000);
 #pragma unroll
-            for (int ks = 0; ks < NKS; ks++) {
-                fb[mt][ks] = v4i{0, 0, 0, 0};
-                if (ks < nks) {
-                    const int l0 = 16 * ks + 8 * h;
-                    const uint16_t *xs = xb + (size_t)l0 * xstride + 64 * wave + 32 * mt + col;
+                for (int mt = 0; mt < 2; mt++) {
+                    const int vo = (int)(loff + 32 * mt * (uint32_t)sizeof(uint16_t));
                     uint32_t d[4];
 #pragma unroll
                     for (int u = 0; u < 4; u++)
-                        d[u] = (uint32_t)xs[(size_t)(2 * u) * xstride] | ((uint32_t)xs[(size_t)(2 * u + 1) * xstride] << 16);
+                        d[u] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b16(rs, vo, (int)(2 * u * pstride), 0) |
+                               ((uint32_t)__builtin_amdgcn_raw_buffer_load_b16(rs, vo, (int)((2 * u + 1) * pstride), 0)
+                                << 16);
                     fb[mt][ks] = v4i{(int)d[0], (int)d[1], (int)d[2], (int)d[3]};
                 }
             }
+        }
         // partial limb sum_b R_{32 nb + 8 g + 4 h + b} 256^b is limb 8 nb + 2 g + h
         // of output (mt, o); lane h owns output o = 64 wave + 32 h + col, i.e.
         // tile mt = h.  Each table block multiplies both tiles; the two halves
@@ -1640,6 +1654,14 @@ bool crt_matrix_core(const RnsPlan &plan, int n) {
            plan.crt_lpad / 16 <= 2 * (plan.crt_nu <= 20 && nbl <= 3 ? 3 : (plan.crt_nu <= 32 && nbl <= 4 ? 4 : 5));
 }
 
+// Whether the matrix-core CRT's 32-bit buffer offsets (up to 16 planes and
+// a tile column above a k-slice's plane address) fit this output shape
+// (tiles padded to at most 256 rows or columns).
+static bool crt_offsets_fit(int64_t m, int64_t nn) {
+    const uint64_t tiles = (uint64_t)((m + 2 * TILE - 1) / TILE) * (uint64_t)((nn + 2 * TILE - 1) / TILE);
+    return 16 * tiles * TILE * TILE * sizeof(uint16_t) + 4 * TILE * TILE < (1ull << 31);
+}
+
 int rns_gemm_device(const RnsGemmArgs &g) {
     Device &dev = device();
     hipStream_t s = dev.stream;
@@ -1733,7 +1755,7 @@ int rns_gemm_device(const RnsGemmArgs &g) {
     int split = 0;
     if (g.split_capacity > 0) {
         split = (int)std::ceil((plan->bits + 2.0) / (64.0 * n)) + 1;
-        if (split > g.split_capacity || !crt_matrix_core(*plan, n)) return SDPX_GPU_DECLINED;
+        if (split > g.split_capacity || !crt_matrix_core(*plan, n) || !crt_offsets_fit(m, nn)) return SDPX_GPU_DECLINED;
         if (g.split_used) *g.split_used = split;
     }
     t_stats.moduli = L;
@@ -1880,7 +1902,7 @@ int rns_gemm_device(const RnsGemmArgs &g) {
     const int up = g.upper ? 1 : 0, pk = packed ? 1 : 0;
     uint32_t *outw = (uint32_t *)g.out;
     const int nbl = plan->crt_npad / 32;
-    const bool crt_mma = crt_matrix_core(*plan, n);
+    const bool crt_mma = crt_matrix_core(*plan, n) && crt_offsets_fit(m, nn);
     // the fused subtraction needs the matrix-core CRT at a compiled width
     const bool fuse_sub = g.sub && crt_mma && split == 0 && g.upper && m == nn && (n == 2 || n == 4 || n == 8);
     if (g.sub_done) *g.sub_done = fuse_sub ? 1 : 0;
