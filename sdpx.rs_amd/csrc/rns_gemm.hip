@@ -897,7 +897,7 @@ __device__ __forceinline__ void crt_sub_store(uint32_t *srec, const uint32_t *ma
 }
 
 template <int NUMAX, int NBMAX, int NL>
-__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void crt_mfma_kernel(
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(NUMAX <= 24 ? 3 : 2))) void crt_mfma_kernel(
     const uint16_t *__restrict__ X, int L, int lpad, const int2 *__restrict__ tiles, int ntiles,
     int tiles_m, int tiles_n, const int8_t *__restrict__ table, int nt, int npad,
     const uint32_t *__restrict__ mlimbs, int nu, double m_top, const int4 *__restrict__ scan_a,
@@ -1648,18 +1648,26 @@ struct Buffers {
 
 // Whether a plan's CRT runs on the matrix cores (crt_mfma_kernel): the MFMA
 // form holds every k-slice's fragments, at most 2 per 32-row table block.
+// The matrix-core CRT's compiled width for a plan: words of M (NUMAX) and
+// 32-row table blocks (NBMAX); 0 when none fits.
+static int crt_numax(const RnsPlan &plan) {
+    const int nu = plan.crt_nu, nbl = plan.crt_npad / 32;
+    return nu <= 20 && nbl <= 3 ? 20 : (nu <= 24 && nbl <= 3 ? 24 : (nu <= 32 && nbl <= 4 ? 32 : (nu <= 40 && nbl <= 5 ? 40 : 0)));
+}
+static int crt_nbmax(int numax) { return numax <= 24 ? 3 : (numax == 32 ? 4 : 5); }
+
 bool crt_matrix_core(const RnsPlan &plan, int n) {
-    const int nbl = plan.crt_npad / 32;
-    return g_crt_mfma && plan.crt_nu <= 40 && nbl <= 5 && n <= 8 &&
-           plan.crt_lpad / 16 <= 2 * (plan.crt_nu <= 20 && nbl <= 3 ? 3 : (plan.crt_nu <= 32 && nbl <= 4 ? 4 : 5));
+    const int numax = crt_numax(plan);
+    return g_crt_mfma && numax > 0 && n <= 8 && plan.crt_lpad / 16 <= 2 * crt_nbmax(numax);
 }
 
-// Whether the matrix-core CRT's 32-bit buffer offsets (up to 16 planes and
-// a tile column above a k-slice's plane address) fit this output shape
-// (tiles padded to at most 256 rows or columns).
+// Whether the matrix-core CRT's buffer offsets fit this output shape (tiles
+// padded to at most 256 rows or columns): the lane offset (8 planes and a
+// tile column) and the plane offset (up to 7 planes) each below 2^31, so
+// their sum stays below 2^32.
 static bool crt_offsets_fit(int64_t m, int64_t nn) {
     const uint64_t tiles = (uint64_t)((m + 2 * TILE - 1) / TILE) * (uint64_t)((nn + 2 * TILE - 1) / TILE);
-    return 16 * tiles * TILE * TILE * sizeof(uint16_t) + 4 * TILE * TILE < (1ull << 31);
+    return 8 * tiles * TILE * TILE * sizeof(uint16_t) + 4 * TILE * TILE < (1ull << 31);
 }
 
 int rns_gemm_device(const RnsGemmArgs &g) {
@@ -1901,7 +1909,6 @@ int rns_gemm_device(const RnsGemmArgs &g) {
     const double *d_c = d_crt, *d_m = d_crt + (size_t)L * J;
     const int up = g.upper ? 1 : 0, pk = packed ? 1 : 0;
     uint32_t *outw = (uint32_t *)g.out;
-    const int nbl = plan->crt_npad / 32;
     const bool crt_mma = crt_matrix_core(*plan, n) && crt_offsets_fit(m, nn);
     // the fused subtraction needs the matrix-core CRT at a compiled width
     const bool fuse_sub = g.sub && crt_mma && split == 0 && g.upper && m == nn && (n == 2 || n == 4 || n == 8);
@@ -1910,7 +1917,7 @@ int rns_gemm_device(const RnsGemmArgs &g) {
         const int lpad = plan->crt_lpad, npad = plan->crt_npad, nu = plan->crt_nu;
         const size_t tb = plan->crt_table.size();
         const int trow = 2 * lpad + 16;
-        const int numax = nu <= 20 && nbl <= 3 ? 20 : (nu <= 32 && nbl <= 4 ? 32 : 40);
+        const int numax = crt_numax(*plan);
         // the table, then each thread's column of words
         const size_t lds = (size_t)npad * trow + (size_t)numax * 128 * 4;
         const int64_t items = (int64_t)ntiles * TILE;
@@ -1934,6 +1941,9 @@ int rns_gemm_device(const RnsGemmArgs &g) {
         if (numax == 20)
             pick(crt_mfma_kernel<20, 3, 2>, crt_mfma_kernel<20, 3, 4>, crt_mfma_kernel<20, 3, 8>,
                  crt_mfma_kernel<20, 3, 0>);
+        else if (numax == 24)
+            pick(crt_mfma_kernel<24, 3, 2>, crt_mfma_kernel<24, 3, 4>, crt_mfma_kernel<24, 3, 8>,
+                 crt_mfma_kernel<24, 3, 0>);
         else if (numax == 32)
             pick(crt_mfma_kernel<32, 4, 2>, crt_mfma_kernel<32, 4, 4>, crt_mfma_kernel<32, 4, 8>,
                  crt_mfma_kernel<32, 4, 0>);
