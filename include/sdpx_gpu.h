@@ -73,8 +73,12 @@ int32_t sdpx_gpu_set_profiling(int32_t on);
  * no reference counterpart.  "trsm_rl" (0/1): the right-looking pass of the
  * triangular-solve base (default 1); "trsm_rl_room" (bits, default 128): its
  * exact windows' room below their first term, a negative value sends every
- * row to the left-looking kernel.  Results never depend on either.  Returns
- * 1 for an unknown name. */
+ * row to the left-looking kernel; "ldl_one_launch" (0/1): ldl_dots base
+ * blocks as one launch of row workgroups (default 1) or one launch per
+ * column; "sweep_wave" (0/1): the arrow solve's single-rounding updates on
+ * whole waves (default 1) or one lane a row; "fwd_group" (4, 8, 16, 32):
+ * lanes per exact dot of the forward rows (default 8).  Results never
+ * depend on any of them.  Returns 1 for an unknown name or value. */
 int32_t sdpx_gpu_set_tuning(const char *name, int64_t value);
 int32_t sdpx_gpu_last_stage_times(double *ms, uint32_t count);
 /* Totals over this thread's profiled exact products since the last reset:

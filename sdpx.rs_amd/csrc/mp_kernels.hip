@@ -326,14 +326,15 @@ __device__ __forceinline__ void group_accumulate(int64_t k0, int64_t k1, int gl,
 
 // Terms x_k = xb[k * xs], y_k = yb[k * ys] (record strides): every panel
 // kernel's operands are affine in k.  Inlined at each call site, so the
-// operands and the result stay in registers.  Groups are 16, 32 or 64 lanes.
+// operands and the result stay in registers.  Groups are 4, 8, 16, 32 or 64
+// lanes.
 template <int W, int G, int STRIDE>
 __device__ __forceinline__ bool group_dot(int64_t k0, int64_t k1, int gl,
                                                     const uint32_t *xb, int64_t xs,
                                                     const uint32_t *yb, int64_t ys, const Val<W> *c,
                                                     bool negate_terms, int64_t *slots, Val<W> &out,
                                                     int64_t hint_lo, int64_t hint_hi) {
-    static_assert(G == 16 || G == 32 || G == 64, "a group is an aligned part of one wave");
+    static_assert(G == 4 || G == 8 || G == 16 || G == 32 || G == 64, "a group is an aligned part of one wave");
     constexpr int SLOTS = Win<W>::SLOTS;
     auto fx = [&](int64_t k) { return xb + k * xs * (W + 2); };
     auto fy = [&](int64_t k) { return yb + k * ys * (W + 2); };
@@ -2739,14 +2740,22 @@ void mp_exp_ranges(int limbs, const void *L, int64_t ldl, int64_t n, int mode, v
     HIP_CHECK(hipGetLastError());
 }
 
+// tuning switch "fwd_group": lanes per exact dot of the forward rows (4, 8
+// (default), 16 or 32)
+static std::atomic<int> g_fwd_group{8};
+
 void mp_forward_rows(int limbs, const void *L_all, const int32_t *first_all, void *Y_all,
                      const void *ranges_all, const FwdLeaf *leaves, int nleaves, int64_t t, int *fail,
                      hipStream_t st) {
     if (nleaves == 0 || t == 0) return;
+    const int g = g_fwd_group.load();
     SDPX_DISPATCH_W(limbs, {
-        auto kern = forward_rows_kernel<W, DOT_G>;
+        auto kern = g == 4 ? forward_rows_kernel<W, 4>
+                           : (g == 8 ? forward_rows_kernel<W, 8>
+                                     : (g == 32 ? forward_rows_kernel<W, 32> : forward_rows_kernel<W, 16>));
+        const int64_t G = g;
         const size_t lds = panel_lds<W>((const void *)kern, ROW_THREADS);
-        hipLaunchKernelGGL(kern, dim3(blocks_for(t * DOT_G, ROW_THREADS), nleaves), dim3(ROW_THREADS), lds, st,
+        hipLaunchKernelGGL(kern, dim3(blocks_for(t * G, ROW_THREADS), nleaves), dim3(ROW_THREADS), lds, st,
                            (const uint32_t *)L_all, first_all, (uint32_t *)Y_all, (const int2 *)ranges_all,
                            leaves, t, fail);
     });
@@ -2798,6 +2807,11 @@ constexpr int64_t LDL_ROWS_MAX = 4096;
 bool mp_set_tuning(const char *name, int64_t value) {
     if (!std::strcmp(name, "ldl_one_launch")) {
         g_ldl_one_launch = value ? 1 : 0;
+        return true;
+    }
+    if (!std::strcmp(name, "fwd_group")) {
+        if (value != 4 && value != 8 && value != 16 && value != 32) return false;
+        g_fwd_group = (int)value;
         return true;
     }
     if (!std::strcmp(name, "sweep_wave")) {

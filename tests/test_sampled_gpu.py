@@ -34,6 +34,17 @@ def test_reduced_instance(gpu):
     compare(workloads.sampled_problem(4, nblocks=6, hmin=10, hmax=30, t=400, seed=11), 4)
 
 
+@pytest.mark.parametrize("group", [4, 8, 16, 32])
+def test_forward_rows_group_sizes(gpu, group):
+    """Y = L^-1 B of the leaves with 4, 8, 16 or 32 lanes per exact dot
+    (tuning switch fwd_group): the same bits."""
+    sdpx_gpu.set_tuning("fwd_group", group)
+    try:
+        compare(workloads.sampled_problem(4, nblocks=6, hmin=10, hmax=30, t=400, seed=11), 4)
+    finally:
+        sdpx_gpu.set_tuning("fwd_group", 8)
+
+
 def test_full_instance_256(gpu):
     compare(workloads.sampled_problem(4, seed=7), 4)
 
