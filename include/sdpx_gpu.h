@@ -73,9 +73,10 @@ int32_t sdpx_gpu_set_profiling(int32_t on);
  * no reference counterpart.  "trsm_rl" (0/1): the right-looking pass of the
  * triangular-solve base (default 1); "trsm_rl_room" (bits, default 128): its
  * exact windows' room below their first term, a negative value sends every
- * row to the left-looking kernel; "ldl_one_launch" (0/1): ldl_dots base
- * blocks as one launch of row workgroups (default 1) or one launch per
- * column; "sweep_wave" (0/1): the arrow solve's single-rounding updates on
+ * row to the left-looking kernel; "ldl_one_launch" (0/1/2): ldl_dots as
+ * one launch of row workgroups for a single block (1, default), for every
+ * call with all blocks' rows interleaved (2), or one launch per column
+ * (0); "sweep_wave" (0/1): the arrow solve's single-rounding updates on
  * whole waves (default 1) or one lane a row; "fwd_group" (4, 8, 16, 32):
  * lanes per exact dot of the forward rows (default 8).  Results never
  * depend on any of them.  Returns 1 for an unknown name or value. */

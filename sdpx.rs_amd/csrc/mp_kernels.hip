@@ -1610,7 +1610,7 @@ __global__ void ldl_flags_clear_kernel(uint32_t *A, uint32_t *dinv, const int32_
 template <int W>
 __global__ __launch_bounds__(3 * ROW_THREADS) void ldl_rows_kernel(
     uint32_t *A, uint32_t *dinv, const int32_t *first_all, const int8_t *signs, int sign_mode,
-    const LdlBlock *blocks, uint32_t *scratch, const uint32_t *reg_eps, const uint32_t *reg_delta,
+    const LdlBlock *blocks, int nblocks, uint32_t *scratch, const uint32_t *reg_eps, const uint32_t *reg_delta,
     int *status) {
     constexpr int SLOTS = Win<W>::SLOTS;
     extern __shared__ int64_t slots_sh[];  // [SLOTS][64] row windows, then [SLOTS][64] look-ahead windows
@@ -1621,8 +1621,12 @@ __global__ __launch_bounds__(3 * ROW_THREADS) void ldl_rows_kernel(
     __shared__ int64_t pre_lo;
     __shared__ int ok_sh[3];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const LdlBlock B = blocks[blockIdx.y];
-    const int64_t i = blockIdx.x;
+    // row-major over the blocks: workgroup (row i, block bk) at linear index
+    // i * nblocks + bk, so every block's rows start together and each row's
+    // waits (on lower rows of its block) are on earlier workgroups
+    const int bk = (int)(blockIdx.x % nblocks);
+    const int64_t i = blockIdx.x / nblocks;
+    const LdlBlock B = blocks[bk];
     if (i >= B.n) return;  // the whole workgroup
     const LdlCol<W> c(A, dinv, first_all, scratch, B);
     int *const pivot_ready = c.flags, *const progress = c.flags + c.n;
@@ -1631,7 +1635,7 @@ __global__ __launch_bounds__(3 * ROW_THREADS) void ldl_rows_kernel(
     auto fail = [&](int code) {
         if (lane == 0) {
             atomicMax(status, code);
-            if (code == 1) atomicMax(status + 3, (int)blockIdx.y + 1);  // a failing block, 1-based
+            if (code == 1) atomicMax(status + 3, bk + 1);  // a failing block, 1-based
         }
     };
     if (i == 0) {  // pivot 0: an empty dot
@@ -2792,21 +2796,25 @@ static bool trsm_right_looking() {
     }();
     return env;
 }
-// "ldl_one_launch" (default 1): ldl_dots blocks in one launch of
-// ldl_rows_kernel (rows ordered by flags) instead of one launch per column.
+// "ldl_one_launch": 1 (default) a single ldl_dots block (the recursive
+// factor's bases) in one launch of ldl_rows_kernel (rows ordered by flags),
+// several blocks (the leaves) one launch per column; 2 every call in one
+// launch of row workgroups (all blocks' rows interleaved); 0 column launches
+// only.
 static std::atomic<int> g_ldl_one_launch{1};
 // "sweep_wave" 1 (default): the border's diagonal blocks (forward and
 // backward) and the blocks' updates of the other rows on whole waves (wave
 // fms); 0: one lane a row.
 static std::atomic<int> g_sweep_wave{1};
-static bool ldl_one_launch() { return g_ldl_one_launch.load() != 0; }
+static int ldl_one_launch() { return g_ldl_one_launch.load(); }
 // rows of one launch of ldl_rows_kernel at most (larger blocks take the
 // column launches)
 constexpr int64_t LDL_ROWS_MAX = 4096;
 
 bool mp_set_tuning(const char *name, int64_t value) {
     if (!std::strcmp(name, "ldl_one_launch")) {
-        g_ldl_one_launch = value ? 1 : 0;
+        if (value < 0 || value > 2) return false;
+        g_ldl_one_launch = (int)value;
         return true;
     }
     if (!std::strcmp(name, "fwd_group")) {
@@ -2890,17 +2898,17 @@ void mp_ldl_dots(int limbs, void *A, void *dinv, const int32_t *first, const int
         HIP_CHECK(hipGetLastError());
         return;
     }
-    // one block (the recursive factor's base blocks): one launch; many
-    // blocks (the leaves) stay on column launches, which measured faster
-    // there (rows of later blocks wait for residency behind earlier ones)
-    if (ldl_one_launch() && nblocks == 1 && nmax <= LDL_ROWS_MAX) {
+    // one launch of row workgroups (rows of all blocks interleaved) when
+    // ldl_one_launch is on (1: one block, the recursive factor's base blocks;
+    // 2: also several blocks, the leaves); else one launch per column
+    if ((ldl_one_launch() == 2 || (ldl_one_launch() && nblocks == 1)) && nmax <= LDL_ROWS_MAX) {
         SDPX_DISPATCH_W(limbs, {
             hipLaunchKernelGGL(ldl_flags_clear_kernel<W>, dim3(blocks_for(2 * nmax, 256), nblocks), dim3(256), 0, st,
                                (uint32_t *)A, (uint32_t *)dinv, first, (uint32_t *)scratch, (const LdlBlock *)blocks);
             auto kern = ldl_rows_kernel<W>;
             const size_t lds = panel_lds<W>((const void *)kern, 2 * ROW_THREADS);  // row windows, look-ahead windows
-            hipLaunchKernelGGL(kern, dim3((unsigned)nmax, nblocks), dim3(3 * ROW_THREADS), lds, st, (uint32_t *)A,
-                               (uint32_t *)dinv, first, signs, sign_mode, (const LdlBlock *)blocks,
+            hipLaunchKernelGGL(kern, dim3((unsigned)(nmax * nblocks)), dim3(3 * ROW_THREADS), lds, st, (uint32_t *)A,
+                               (uint32_t *)dinv, first, signs, sign_mode, (const LdlBlock *)blocks, nblocks,
                                (uint32_t *)scratch, (const uint32_t *)reg_eps, (const uint32_t *)reg_delta, status);
             hipLaunchKernelGGL(ldl_col_writeout_kernel<W>, dim3(blocks_for(nmax * nmax, 256), nblocks), dim3(256),
                                0, st, (uint32_t *)A, (uint32_t *)dinv, (const LdlBlock *)blocks, (uint32_t *)scratch,

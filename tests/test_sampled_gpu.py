@@ -45,6 +45,17 @@ def test_forward_rows_group_sizes(gpu, group):
         sdpx_gpu.set_tuning("fwd_group", 8)
 
 
+@pytest.mark.parametrize("mode", [0, 2])
+def test_leaf_ldl_launch_forms(gpu, mode):
+    """The leaves' ldl_dots as column launches (0) or as one launch of row
+    workgroups with every leaf's rows interleaved (2): the same bits."""
+    sdpx_gpu.set_tuning("ldl_one_launch", mode)
+    try:
+        compare(workloads.sampled_problem(4, nblocks=6, hmin=10, hmax=30, t=400, seed=11), 4)
+    finally:
+        sdpx_gpu.set_tuning("ldl_one_launch", 1)
+
+
 def test_full_instance_256(gpu):
     compare(workloads.sampled_problem(4, seed=7), 4)
 
