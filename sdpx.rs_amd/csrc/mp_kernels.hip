@@ -532,6 +532,8 @@ __global__ __launch_bounds__(ROW_THREADS) void forward_rows_kernel(const uint32_
 // so LDS holds only the windows and their bases.  Results are those of the
 // left-looking kernel: the same exact dots, each rounded once.
 constexpr int RL_COLS = 128;               // columns (window slots per slot row)
+// wave priority for the latency chains of trsm_rl_kernel and ldl_rows_kernel
+constexpr bool g_chain_prio = true;
 constexpr int RL_THREADS = RL_COLS + 64;   // the chain wave, then one thread a column
 
 template <int W>
@@ -580,6 +582,8 @@ __global__ __launch_bounds__(RL_THREADS) void trsm_rl_kernel(
         return ovf_step[j & 1] != 0;
     };
     if (wave == 0) {  // the chain
+        // the chain's instructions first where workgroups share a SIMD
+        if (g_chain_prio) __builtin_amdgcn_s_setprio(2);
         wmp::WV<W> xprev = wmp::zero<W>();
         // b_j and L_{j,j-1} one step ahead
         uint32_t bn = lo < n ? wmp::load_word<W>(rec_at<W>((const uint32_t *)B, i + lo * ldb)) : 0u, ln = 0u;
@@ -1652,6 +1656,9 @@ __global__ __launch_bounds__(3 * ROW_THREADS) void ldl_rows_kernel(
     }
     const int64_t fi = c.first[i];
     if (threadIdx.x < 2) nxt_base[threadIdx.x] = INT64_MIN;
+    // the pivot chain (wave 0: entries and pivot, wave 1: reciprocals) ahead
+    // of the look-ahead wave where workgroups share a SIMD
+    if (g_chain_prio && wave < 2) __builtin_amdgcn_s_setprio(2);
     __syncthreads();
     // wave 0: pivot i's operands, loaded now so that no global load sits on
     // the pivot's critical path
