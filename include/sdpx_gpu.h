@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define SDPX_GPU_ABI_VERSION 4u
+#define SDPX_GPU_ABI_VERSION 5u
 #define SDPX_GPU_OK 0
 #define SDPX_GPU_INVALID 1
 #define SDPX_GPU_DECLINED 2
@@ -532,6 +532,28 @@ int32_t sdpx_gpu_condensed_assemble_f64_dev(uint64_t m, const int64_t *p_colptr,
                                             const double *p_nzval, uint64_t nblocks,
                                             const sdpx_gpu_psd_block_f64 *blocks, uint64_t rows, const double *b,
                                             uint64_t ldb, const double *w, double *s, uint64_t lds);
+
+/* The condensed KKT's right-hand side and recovery of the eliminated rows,
+ * over the same blocks, orthant rows b and weights w as
+ * sdpx_gpu_condensed_assemble_f64_dev (device pointers).  H^-1 is each
+ * cone's inverse scaling of the eliminated rows: svec(W smat(v) W), W =
+ * Rinv^T Rinv, for a PSD block (PsdScaling::apply with inverse set,
+ * kkt/condensed/psd.rs:1968-2000), w o v for the orthant rows.  bz and z
+ * hold the eliminated rows block after block (block i: its n_i (n_i + 1) / 2
+ * svec rows), then the `rows` orthant rows.
+ *   rhs:     xr = bx + A^T H^-1 bz   (m; replaces CondensedKKT::prepare_rhs,
+ *            kkt/condensed/mod.rs:2416-2507)
+ *   recover: z = H^-1 (A x - bz)     (replaces CondensedKKT::recover_rhs,
+ *            kkt/condensed/mod.rs:2511-2603, without retained rows)
+ * so that x solving S x = xr with S the assembled matrix, and z, solve the
+ * unreduced system P x + A^T z = bx, A x - H z = bz.  Float64; parity by
+ * tolerance. */
+int32_t sdpx_gpu_condensed_rhs_f64_dev(uint64_t m, uint64_t nblocks, const sdpx_gpu_psd_block_f64 *blocks,
+                                       uint64_t rows, const double *b, uint64_t ldb, const double *w,
+                                       const double *bx, const double *bz, double *xr);
+int32_t sdpx_gpu_condensed_recover_f64_dev(uint64_t m, uint64_t nblocks, const sdpx_gpu_psd_block_f64 *blocks,
+                                           uint64_t rows, const double *b, uint64_t ldb, const double *w,
+                                           const double *x, const double *bz, double *z);
 
 /* ---- Adjoint (differentiation) ---------------------------------------
  *

@@ -384,6 +384,61 @@ int32_t sdpx_gpu_condensed_assemble_f64_dev(uint64_t m, const int64_t *p_colptr,
     });
 }
 
+// The blocks of a condensed call, checked (shared by the rhs and recovery).
+static int32_t condensed_blocks(uint64_t nblocks, const sdpx_gpu_psd_block_f64 *blocks,
+                                std::vector<sdpx::PsdBlockF64> &bl, uint64_t &eliminated) {
+    if (nblocks && !blocks) return abi_invalid("null blocks");
+    bl.resize(nblocks);
+    eliminated = 0;
+    for (uint64_t i = 0; i < nblocks; i++) {
+        const auto &x = blocks[i];
+        if (x.struct_size != sizeof(sdpx_gpu_psd_block_f64)) return abi_invalid("psd block struct_size");
+        if (!x.rinv || (x.ncols && (!x.a_svec || !x.columns))) return abi_invalid("null block array");
+        bl[i] = sdpx::PsdBlockF64{(int64_t)x.n, (int64_t)x.ncols, x.rinv, x.a_svec, x.columns};
+        eliminated += x.n * (x.n + 1) / 2;
+    }
+    return SDPX_GPU_OK;
+}
+
+int32_t sdpx_gpu_condensed_rhs_f64_dev(uint64_t m, uint64_t nblocks, const sdpx_gpu_psd_block_f64 *blocks,
+                                       uint64_t rows, const double *b, uint64_t ldb, const double *w,
+                                       const double *bx, const double *bz, double *xr) {
+    return abi_boundary([&]() -> int32_t {
+        if (rows && (!b || !w || ldb < rows)) return abi_invalid("orthant rows need b (ldb >= rows) and w");
+        std::vector<sdpx::PsdBlockF64> bl;
+        uint64_t el = 0;
+        const int32_t r = condensed_blocks(nblocks, blocks, bl, el);
+        if (r != SDPX_GPU_OK) return r;
+        if (m && (!bx || !xr)) return abi_invalid("null bx or xr");
+        if (el + rows && !bz) return abi_invalid("null bz");
+        if (m == 0) return SDPX_GPU_OK;
+        sdpx::Device &d = sdpx::device();
+        std::lock_guard<std::recursive_mutex> g(d.mu);
+        sdpx::condensed_rhs_f64((int64_t)m, bl, (int64_t)rows, b, (int64_t)ldb, w, bx, bz, xr, d.stream);
+        HIP_CHECK(hipStreamSynchronize(d.stream));
+        return SDPX_GPU_OK;
+    });
+}
+
+int32_t sdpx_gpu_condensed_recover_f64_dev(uint64_t m, uint64_t nblocks, const sdpx_gpu_psd_block_f64 *blocks,
+                                           uint64_t rows, const double *b, uint64_t ldb, const double *w,
+                                           const double *x, const double *bz, double *z) {
+    return abi_boundary([&]() -> int32_t {
+        if (rows && (!b || !w || ldb < rows)) return abi_invalid("orthant rows need b (ldb >= rows) and w");
+        std::vector<sdpx::PsdBlockF64> bl;
+        uint64_t el = 0;
+        const int32_t r = condensed_blocks(nblocks, blocks, bl, el);
+        if (r != SDPX_GPU_OK) return r;
+        if (el + rows == 0) return SDPX_GPU_OK;
+        if (!bz || !z || (m && !x)) return abi_invalid("null x, bz or z");
+        sdpx::Device &d = sdpx::device();
+        std::lock_guard<std::recursive_mutex> g(d.mu);
+        sdpx::condensed_recover_f64((int64_t)m, bl, (int64_t)rows, b, (int64_t)ldb, w, x, bz, z, d.stream);
+        HIP_CHECK(hipStreamSynchronize(d.stream));
+        return SDPX_GPU_OK;
+    });
+}
+
 }  // extern "C"
 
 #include "nt_scaling.h"

@@ -69,6 +69,14 @@ struct PsdBlockF64 {
     const double *rinv, *a_svec;
     const uint64_t *columns;
 };
+// The condensed KKT's right-hand side xr = bx + A^T H^-1 bz and recovery
+// z = H^-1 (A x - bz) over the same blocks and orthant rows as
+// condensed_assemble_f64; bz and z hold the eliminated rows block after
+// block (svec rows), then the orthant rows.
+void condensed_rhs_f64(int64_t m, const std::vector<PsdBlockF64> &blocks, int64_t rows, const double *B, int64_t ldb,
+                       const double *w, const double *bx, const double *bz, double *xr, hipStream_t st);
+void condensed_recover_f64(int64_t m, const std::vector<PsdBlockF64> &blocks, int64_t rows, const double *B,
+                           int64_t ldb, const double *w, const double *x, const double *bz, double *z, hipStream_t st);
 void condensed_assemble_f64(int64_t m, const int64_t *p_colptr, const int64_t *p_rowval, const double *p_nzval,
                             const std::vector<PsdBlockF64> &blocks, int64_t rows, const double *B, int64_t ldb,
                             const double *w, double *S, int64_t lds, hipStream_t st);

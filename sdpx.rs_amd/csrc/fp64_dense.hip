@@ -736,6 +736,113 @@ __global__ void zero_upper(double *S, int64_t m, int64_t lds) {
 // triangle = P, then each PSD block's contribution S_ab = <A_a, W A_b W>
 // (W = Rinv^T Rinv) scattered to its global columns in cone order, then the
 // orthant rows' Gram B^T diag(w) B.  Workspace slots 34-36.
+// out[i] = x[cols[i]] (k entries)
+__global__ void gather_cols(const double *x, const uint64_t *cols, int64_t k, double *out) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < k) out[e] = x[cols[e]];
+}
+
+// x[cols[i]] += u[i] (k entries; the columns of one block are distinct)
+__global__ void scatter_add_cols(const double *u, const uint64_t *cols, int64_t k, double *x) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < k) x[cols[e]] += u[e];
+}
+
+// v = -v (n entries)
+__global__ void negate_vec(double *v, int64_t n) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) v[e] = -v[e];
+}
+
+// out = w o v (r entries)
+__global__ void scale_vec(const double *w, const double *v, int64_t r, double *out) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < r) out[e] = w[e] * v[e];
+}
+
+// The eliminated rows' inverse scaling H^-1 of one PSD block on an svec
+// vector: svec(W smat(v) W), W = Rinv^T Rinv, as two congruences with Rinv
+// (the binary64 form of PsdScaling::apply with inverse set,
+// kkt/condensed/psd.rs:1968-2000).
+static void psd_hinv(const PsdBlockF64 &b, const double *v, double *out, hipStream_t st) {
+    Workspace &ws = device().ws;
+    double *stack = (double *)ws.get(36, sizeof(double) * b.n * b.n);
+    double *x = (double *)ws.get(37, sizeof(double) * b.n * b.n);
+    w_congruence(b.rinv, b.n, v, 1, out, stack, x, st);
+}
+
+void condensed_rhs_f64(int64_t m, const std::vector<PsdBlockF64> &blocks, int64_t rows, const double *B, int64_t ldb,
+                       const double *w, const double *bx, const double *bz, double *xr, hipStream_t st) {
+    Workspace &ws = device().ws;
+    HIP_CHECK(hipMemcpyAsync(xr, bx, sizeof(double) * m, hipMemcpyDeviceToDevice, st));
+    int64_t off = 0;
+    for (const auto &b : blocks) {
+        const int64_t s = b.n * (b.n + 1) / 2;
+        if (b.ncols > 0) {
+            double *t = (double *)ws.get(38, sizeof(double) * s);
+            double *u = (double *)ws.get(39, sizeof(double) * b.ncols);
+            psd_hinv(b, bz + off, t, st);
+            Fp64Gemm g;  // u = A_b^T t
+            g.m = b.ncols; g.n = 1; g.k = s;
+            g.A = b.a_svec; g.lda = s; g.ta = true; g.B = t; g.ldb = s;
+            g.C = u; g.ldc = b.ncols;
+            gemm_f64(g, st);
+            hipLaunchKernelGGL(scatter_add_cols, dim3((unsigned)((b.ncols + 255) / 256)), dim3(256), 0, st, u,
+                               b.columns, b.ncols, xr);
+            HIP_CHECK(hipGetLastError());
+        }
+        off += s;
+    }
+    if (rows > 0) {
+        double *t = (double *)ws.get(38, sizeof(double) * rows);
+        hipLaunchKernelGGL(scale_vec, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, st, w, bz + off, rows, t);
+        HIP_CHECK(hipGetLastError());
+        Fp64Gemm g;  // xr += B^T t
+        g.m = m; g.n = 1; g.k = rows;
+        g.A = B; g.lda = ldb; g.ta = true; g.B = t; g.ldb = rows;
+        g.C = xr; g.ldc = m; g.beta = 1.0;
+        gemm_f64(g, st);
+    }
+}
+
+void condensed_recover_f64(int64_t m, const std::vector<PsdBlockF64> &blocks, int64_t rows, const double *B,
+                           int64_t ldb, const double *w, const double *x, const double *bz, double *z, hipStream_t st) {
+    Workspace &ws = device().ws;
+    int64_t off = 0;
+    for (const auto &b : blocks) {
+        const int64_t s = b.n * (b.n + 1) / 2;
+        double *v = (double *)ws.get(38, sizeof(double) * s);
+        HIP_CHECK(hipMemcpyAsync(v, bz + off, sizeof(double) * s, hipMemcpyDeviceToDevice, st));
+        if (b.ncols > 0) {
+            double *g0 = (double *)ws.get(39, sizeof(double) * b.ncols);
+            hipLaunchKernelGGL(gather_cols, dim3((unsigned)((b.ncols + 255) / 256)), dim3(256), 0, st, x, b.columns,
+                               b.ncols, g0);
+            HIP_CHECK(hipGetLastError());
+            Fp64Gemm g;  // v = A_b x[cols] - bz_b
+            g.m = s; g.n = 1; g.k = b.ncols;
+            g.A = b.a_svec; g.lda = s; g.B = g0; g.ldb = b.ncols;
+            g.C = v; g.ldc = s; g.beta = -1.0;
+            gemm_f64(g, st);
+        } else {  // no constraint touches the block: v = -bz_b
+            hipLaunchKernelGGL(negate_vec, dim3((unsigned)((s + 255) / 256)), dim3(256), 0, st, v, s);
+            HIP_CHECK(hipGetLastError());
+        }
+        psd_hinv(b, v, z + off, st);
+        off += s;
+    }
+    if (rows > 0) {
+        double *v = (double *)ws.get(38, sizeof(double) * rows);
+        HIP_CHECK(hipMemcpyAsync(v, bz + off, sizeof(double) * rows, hipMemcpyDeviceToDevice, st));
+        Fp64Gemm g;  // v = B x - bz_orth
+        g.m = rows; g.n = 1; g.k = m;
+        g.A = B; g.lda = ldb; g.B = x; g.ldb = m;
+        g.C = v; g.ldc = rows; g.beta = -1.0;
+        gemm_f64(g, st);
+        hipLaunchKernelGGL(scale_vec, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, st, w, v, rows, z + off);
+        HIP_CHECK(hipGetLastError());
+    }
+}
+
 void condensed_assemble_f64(int64_t m, const int64_t *p_colptr, const int64_t *p_rowval, const double *p_nzval,
                             const std::vector<PsdBlockF64> &blocks, int64_t rows, const double *B, int64_t ldb,
                             const double *w, double *S, int64_t lds, hipStream_t st) {

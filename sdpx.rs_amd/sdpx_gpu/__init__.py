@@ -121,6 +121,9 @@ def _declare(L):
         getattr(L, name).restype = i32
     L.sdpx_gpu_condensed_assemble_f64_dev.argtypes = [u64, vp, vp, vp, u64, vp, u64, vp, u64, vp, vp, u64]
     L.sdpx_gpu_condensed_assemble_f64_dev.restype = i32
+    for name in ("sdpx_gpu_condensed_rhs_f64_dev", "sdpx_gpu_condensed_recover_f64_dev"):
+        getattr(L, name).argtypes = [u64, u64, vp, u64, vp, u64, vp, vp, vp, vp]
+        getattr(L, name).restype = i32
     L.sdpx_gpu_rns_gemm_blocks_upper.argtypes = [u32, u64, u64, vp, vp]
     L.sdpx_gpu_rns_gemm_blocks_upper.restype = i32
     L.sdpx_gpu_mp_ldl_dev.argtypes = [u32, u64, vp, u64, vp, u64, vp, vp, vp, u32,
@@ -720,11 +723,9 @@ class PsdBlockF64(_Sized):
                 ("rinv", ctypes.c_void_p), ("a_svec", ctypes.c_void_p), ("columns", ctypes.c_void_p)]
 
 
-def condensed_assemble_f64(m, p_csc, blocks, b_rows=None, w=None):
-    """CondensedKKT::assemble on the GPU (sdpx_gpu_condensed_assemble_f64_dev):
-    p_csc = (colptr, rowval, nzval) or None; blocks = [(rinv n x n,
-    a_svec s x k, columns (k,))]; b_rows (r x m) and w (r,) the orthant
-    rows.  Returns the m x m upper triangle (lower part zero)."""
+def _condensed_dev(blocks, b_rows, w):
+    """Device copies of a condensed KKT's PSD blocks and orthant rows: the
+    sdpx_gpu_psd_block_f64 array, (b, w, rows) and the tensors to keep."""
     import torch
 
     dev = lambda a, dt=torch.float64: torch.tensor(np.asarray(a), dtype=dt, device="cuda")
@@ -737,20 +738,64 @@ def condensed_assemble_f64(m, p_csc, blocks, b_rows=None, w=None):
         c = dev(np.asarray(cols, dtype=np.int64), torch.int64)
         keep += [r, a, c]
         bl[i] = PsdBlockF64(n, len(cols), r.data_ptr(), a.data_ptr(), c.data_ptr())
-    S = torch.zeros(m * m, dtype=torch.float64, device="cuda")
-    pc = pr = pv = None
-    if p_csc is not None:
-        pc, pr, pv = dev(p_csc[0], torch.int64), dev(p_csc[1], torch.int64), dev(p_csc[2])
     bb = ww = None
     rows = 0
     if b_rows is not None:
         rows = b_rows.shape[0]
         bb, ww = dev(np.asfortranarray(b_rows).T.reshape(-1)), dev(w)
+        keep += [bb, ww]
+    return bl, bb, ww, rows, keep
+
+
+def condensed_assemble_f64(m, p_csc, blocks, b_rows=None, w=None):
+    """CondensedKKT::assemble on the GPU (sdpx_gpu_condensed_assemble_f64_dev):
+    p_csc = (colptr, rowval, nzval) or None; blocks = [(rinv n x n,
+    a_svec s x k, columns (k,))]; b_rows (r x m) and w (r,) the orthant
+    rows.  Returns the m x m upper triangle (lower part zero)."""
+    S = _condensed_assemble_dev(m, p_csc, blocks, b_rows, w)
+    return S.cpu().numpy().reshape(m, m).T.copy()
+
+
+def _condensed_assemble_dev(m, p_csc, blocks, b_rows, w):
+    import torch
+
+    dev = lambda a, dt=torch.float64: torch.tensor(np.asarray(a), dtype=dt, device="cuda")
+    bl, bb, ww, rows, keep = _condensed_dev(blocks, b_rows, w)
+    S = torch.zeros(m * m, dtype=torch.float64, device="cuda")
+    pc = pr = pv = None
+    if p_csc is not None:
+        pc, pr, pv = dev(p_csc[0], torch.int64), dev(p_csc[1], torch.int64), dev(p_csc[2])
     ptr = lambda t: None if t is None else t.data_ptr()
     torch.cuda.synchronize()
     check(lib().sdpx_gpu_condensed_assemble_f64_dev(m, ptr(pc), ptr(pr), ptr(pv), len(blocks), bl, rows, ptr(bb),
                                                     max(rows, 1), ptr(ww), S.data_ptr(), m))
-    return S.cpu().numpy().reshape(m, m).T.copy()
+    return S
+
+
+def condensed_solve_f64(m, p_csc, blocks, b_rows, w, bx, bz):
+    """One condensed KKT solve on the GPU (CondensedKKT::solve_raw =
+    prepare_rhs, the reduced solve, recover_rhs; kkt/condensed/mod.rs:
+    2416-2610): S assembled and factored, xr = bx + A^T H^-1 bz, x = S^-1 xr,
+    z = H^-1 (A x - bz).  bz and z: the blocks' svec rows, then the orthant
+    rows.  Returns (x, z)."""
+    import torch
+
+    S = _condensed_assemble_dev(m, p_csc, blocks, b_rows, w)
+    bl, bb, ww, rows, keep = _condensed_dev(blocks, b_rows, w)
+    dbx = torch.tensor(np.asarray(bx, dtype=np.float64), device="cuda")
+    dbz = torch.tensor(np.asarray(bz, dtype=np.float64), device="cuda")
+    xr = torch.empty(m, dtype=torch.float64, device="cuda")
+    z = torch.empty(dbz.numel(), dtype=torch.float64, device="cuda")
+    ptr = lambda t: None if t is None else t.data_ptr()
+    torch.cuda.synchronize()
+    check(lib().sdpx_gpu_condensed_rhs_f64_dev(m, len(blocks), bl, rows, ptr(bb), max(rows, 1), ptr(ww),
+                                               dbx.data_ptr(), dbz.data_ptr(), xr.data_ptr()))
+    f = DenseFactor(m)
+    assert f.refactor_dev(S.data_ptr(), m), "condensed Schur matrix not positive definite"
+    f.solve_many_dev(xr.data_ptr(), 1, m)
+    check(lib().sdpx_gpu_condensed_recover_f64_dev(m, len(blocks), bl, rows, ptr(bb), max(rows, 1), ptr(ww),
+                                                   xr.data_ptr(), dbz.data_ptr(), z.data_ptr()))
+    return xr.cpu().numpy(), z.cpu().numpy()
 
 
 class ExactBlock(_Sized):

@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_error_channel():
     L = sdpx_gpu.lib()
-    assert L.sdpx_gpu_abi_version() == 4
+    assert L.sdpx_gpu_abi_version() == 5
     # invalid input is rejected before any device work
     from sdpx_gpu import mp
     a = mp.zeros(4, 4)

@@ -193,3 +193,80 @@ def test_condensed_assembly_several_cones(gpu):
     scale = float(np.abs(ref).max())
     err = np.abs(np.triu(S).astype(np.longdouble) - ref).max() / scale
     assert float(err) <= 1e-12, float(err)
+
+
+def test_condensed_solve_recovers_the_unreduced_system(gpu):
+    """CondensedKKT::solve_raw on the GPU (prepare_rhs, the reduced factor
+    solve, recover_rhs; kkt/condensed/mod.rs:2416-2610) for P, three PSD
+    blocks (one with no constraint columns) and orthant rows: (x, z) solve
+    the unreduced system P x + A^T z = bx, A x - H z = bz, with H^-1 the
+    cones' inverse scalings (svec(W smat(.) W), W = Rinv^T Rinv; w o .), in
+    extended precision to 1e-11 relative; z also against the recovery
+    formula and x against numpy's solve of the assembled system."""
+    from sdpx_gpu import ipm
+
+    rng = np.random.default_rng(21)
+    m = 50
+    Pd = np.eye(m) * 0.5
+    for _ in range(40):
+        i, j = sorted(rng.integers(0, m, 2))
+        v = 0.1 * rng.standard_normal()
+        Pd[i, j] += v
+        if i != j:
+            Pd[j, i] += v
+    Pd += m * 0.02 * np.eye(m)
+    colptr, rowval, nzval = [0], [], []
+    for j in range(m):
+        for i in range(j + 1):
+            if Pd[i, j] != 0:
+                rowval.append(i)
+                nzval.append(Pd[i, j])
+        colptr.append(len(rowval))
+    blocks, mats = [], []
+    for n, k in ((7, 20), (12, 35), (4, 0)):
+        cols = np.sort(rng.choice(m, k, replace=False))
+        rinv = np.eye(n) + rng.standard_normal((n, n)) / (4 * np.sqrt(n))
+        A = [(lambda g: g + g.T)(rng.standard_normal((n, n))) for _ in range(k)]
+        a_svec = np.stack([ipm.svec(a) for a in A], axis=1) if k else np.zeros((n * (n + 1) // 2, 0))
+        blocks.append((rinv, a_svec, cols))
+        mats.append((n, rinv, a_svec, cols))
+    r = 15
+    B = rng.standard_normal((r, m))
+    w = np.exp(rng.standard_normal(r))
+    ssum = sum(n * (n + 1) // 2 for n, *_ in mats)
+    bx = rng.standard_normal(m)
+    bz = rng.standard_normal(ssum + r)
+    x, z = gpu.condensed_solve_f64(m, (colptr, rowval, nzval), blocks, B, w, bx, bz)
+
+    ld = np.longdouble
+    hinv = lambda rinv, v: ipm.svec((rinv.T @ rinv) @ ipm.smat(v, rinv.shape[0]) @ (rinv.T @ rinv))
+    xl, zl = x.astype(ld), z.astype(ld)
+    r1 = Pd.astype(ld) @ xl - bx.astype(ld)
+    scale = np.abs(Pd).max() * np.abs(x).max() + np.abs(bx).max()
+    off = 0
+    for n, rinv, a_svec, cols in mats:
+        s = n * (n + 1) // 2
+        zb = z[off:off + s]
+        if len(cols):
+            r1[cols] += a_svec.astype(ld).T @ zl[off:off + s]
+        ax = a_svec @ x[cols] if len(cols) else np.zeros(s)
+        want = hinv(rinv, ax - bz[off:off + s])
+        assert np.abs(zb - want).max() <= 1e-11 * max(1.0, np.abs(want).max())
+        scale += np.abs(a_svec).max(initial=0) * np.abs(zb).max() * max(1, len(cols))
+        off += s
+    zo = z[off:]
+    assert np.abs(zo - w * (B @ x - bz[off:])).max() <= 1e-11 * max(1.0, np.abs(zo).max())
+    r1 += B.astype(ld).T @ zl[off:]
+    assert float(np.abs(r1).max()) <= 1e-11 * scale, float(np.abs(r1).max() / scale)
+    S = gpu.condensed_assemble_f64(m, (colptr, rowval, nzval), blocks, B, w)
+    S = np.triu(S) + np.triu(S, 1).T
+    xr = bx.copy()
+    off = 0
+    for n, rinv, a_svec, cols in mats:
+        s = n * (n + 1) // 2
+        if len(cols):
+            xr[cols] += a_svec.T @ hinv(rinv, bz[off:off + s])
+        off += s
+    xr += B.T @ (w * bz[off:])
+    xn = np.linalg.solve(S, xr)
+    assert np.abs(x - xn).max() <= 1e-9 * np.abs(xn).max()
