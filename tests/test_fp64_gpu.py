@@ -270,3 +270,52 @@ def test_condensed_solve_recovers_the_unreduced_system(gpu):
     xr += B.T @ (w * bz[off:])
     xn = np.linalg.solve(S, xr)
     assert np.abs(x - xn).max() <= 1e-9 * np.abs(xn).max()
+
+
+def test_conic_json_problem_through_the_condensed_solve(gpu):
+    """A conic JSON problem (PSD triangle and nonnegative cones, read by
+    sdpx_gpu.conic_json) mapped onto the condensed assembly, solved on the
+    GPU (assembly, factor, prepare_rhs, recover_rhs): with the problem's own
+    rows A_K in cone order, P x + A_K^T z = bx to 1e-11 relative."""
+    import json
+
+    from sdpx_gpu import conic_json as cj
+
+    rng = np.random.default_rng(8)
+    cones = [("psd", 4), ("nonneg", 6), ("psd", 3)]
+    m = sum(cj.cone_size(c) for c in cones)
+    n = 30
+    A = rng.standard_normal((m, n)) * (rng.random((m, n)) < 0.5)
+    Pd = np.eye(n) + 0.05 * rng.standard_normal((n, n))
+    Pd = Pd @ Pd.T
+
+    def csc(M):
+        cp, rv, nz = [0], [], []
+        for j in range(M.shape[1]):
+            for i in range(M.shape[0]):
+                if M[i, j] != 0:
+                    rv.append(i)
+                    nz.append(float(M[i, j]))
+            cp.append(len(rv))
+        return {"m": M.shape[0], "n": M.shape[1], "colptr": cp, "rowval": rv, "nzval": nz}
+
+    doc = {"P": csc(np.triu(Pd)), "q": [0.0] * n, "A": csc(A), "b": [0.0] * m,
+           "cones": [cj._cone_json(c) for c in cones]}
+    pr = cj.read_conic_json(json.dumps(doc))
+    rinvs = [np.eye(4) + 0.2 * rng.standard_normal((4, 4)), np.eye(3) + 0.2 * rng.standard_normal((3, 3))]
+    w = np.exp(rng.standard_normal(6))
+    p_csc, blocks, B, ww, retained = cj.condensed_inputs(pr, rinvs, w)
+    assert retained == []
+    bx = rng.standard_normal(n)
+    bz_cone = rng.standard_normal(m)  # eliminated rows in cone order
+    psd_rows = [slice(0, 10), slice(16, 22)]
+    bz = np.concatenate([bz_cone[psd_rows[0]], bz_cone[psd_rows[1]], bz_cone[10:16]])
+    x, z = gpu.condensed_solve_f64(n, p_csc, blocks, B, ww, bx, bz)
+    z_cone = np.empty(m)
+    z_cone[psd_rows[0]], z_cone[psd_rows[1]], z_cone[10:16] = z[:10], z[10:16], z[16:]
+    # the library's svec rows are the cone's rows over the touched columns:
+    # A_K x restricted to those columns is A_K x itself
+    ld = np.longdouble
+    r1 = Pd.astype(ld) @ x.astype(ld) + A.astype(ld).T @ z_cone.astype(ld) - bx.astype(ld)
+    scale = np.abs(Pd).max() * np.abs(x).max() + np.abs(A).max() * np.abs(z).max() * m + np.abs(bx).max()
+    assert float(np.abs(r1).max()) <= 1e-11 * scale
