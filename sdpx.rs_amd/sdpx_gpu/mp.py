@@ -41,22 +41,29 @@ def _bitlen64(x):
 
 def _shift_left(limbs, sh):
     """Shift multi-limb integers (rows of ``limbs``, LSB first) left by
-    per-row ``sh`` bits (0 <= sh < 64 * L), dropping overflow."""
+    per-row ``sh`` bits (0 <= sh < 64 * L), dropping overflow.  Rows are
+    grouped by their whole-limb shift, so each group moves by slicing."""
     L = limbs.shape[1]
     whole = sh // 64
     part = (sh % 64).astype(np.uint64)
     out = np.zeros_like(limbs)
-    idx = np.arange(L)[None, :] - whole[:, None]
-    valid = idx >= 0
-    moved = np.where(valid, np.take_along_axis(limbs, np.clip(idx, 0, L - 1), axis=1), 0)
-    prev_idx = idx - 1
-    pvalid = prev_idx >= 0
-    prev = np.where(pvalid, np.take_along_axis(limbs, np.clip(prev_idx, 0, L - 1), axis=1), 0)
-    p = part[:, None]
-    hi = np.where(p > 0, moved << p, moved)
-    lo = np.where(p > 0, prev >> (np.uint64(64) - np.where(p > 0, p, np.uint64(1))), 0)
-    out[:] = hi | lo
-    return out.astype(np.uint64)
+    for w in np.unique(whole):
+        w = int(w)
+        sel = whole == w
+        every = bool(sel.all())
+        src = limbs if every else limbs[sel]
+        p = (part if every else part[sel])[:, None]
+        moved = np.zeros_like(src)
+        moved[:, w:] = src[:, :L - w]
+        prev = np.zeros_like(src)
+        prev[:, w + 1:] = src[:, :L - w - 1]
+        lo = np.where(p > 0, prev >> (np.uint64(64) - np.where(p > 0, p, np.uint64(1))), np.uint64(0))
+        res = (moved << p) | lo
+        if every:
+            out[:] = res
+        else:
+            out[sel] = res
+    return out
 
 
 def from_integers(mag, negative, scale, n):
@@ -67,10 +74,9 @@ def from_integers(mag, negative, scale, n):
     if L < n:
         mag = np.concatenate([mag, np.zeros((V, n - L), np.uint64)], axis=1)
         L = n
-    bl = np.zeros(V, np.int64)
-    for i in range(L):
-        b = _bitlen64(mag[:, i])
-        bl = np.where(b > 0, 64 * i + b, bl)
+    nonzero = mag != 0
+    top = L - 1 - np.argmax(nonzero[:, ::-1], axis=1)  # highest nonzero limb
+    bl = np.where(nonzero.any(axis=1), 64 * top + _bitlen64(mag[np.arange(V), top]), 0)
     if np.any(bl > 64 * n):
         raise ValueError("magnitude wider than the precision")
     out = zeros(V, n)
@@ -117,9 +123,17 @@ def from_float64(x, n):
     if not np.all(np.isfinite(x)):
         raise ValueError("non-finite input")
     mant, ex = np.frexp(np.abs(x))  # |x| = mant * 2**ex, mant in [0.5, 1)
+    # frexp normalises subnormals too, so every nonzero magnitude has a
+    # 53-bit mantissa in [2^52, 2^53): the record's top limb is that
+    # mantissa shifted to bit 63, the lower limbs are zero and the exponent
+    # is ex (value = top * 2^(64(n-1)) * 2^(ex - 64n))
     m53 = (mant * 2.0**53).astype(np.uint64)  # exact
-    mag = m53[:, None]
-    return from_integers(mag, x < 0, ex.astype(np.int64) - 53, n)
+    out = zeros(x.size, n)
+    nz = x != 0
+    out["limbs"][:, n - 1] = np.where(nz, m53 << np.uint64(11), np.uint64(0))
+    out["kind"] = np.where(nz, np.where(x < 0, -REGULAR_KIND, REGULAR_KIND), ZERO_KIND)
+    out["exponent"] = np.where(nz, ex, 0)
+    return out
 
 
 def to_float64(rec):

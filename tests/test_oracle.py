@@ -72,6 +72,23 @@ def test_known_values():
     assert M.from_int(3, 1).limbs == (0xC000000000000000,) and M.from_int(3, 1).exponent == 2
 
 
+@pytest.mark.parametrize("nl", [1, 2, 8])
+def test_from_float64_is_exact(nl):
+    # binary64 -> records (subnormals, signed zeros, extremes, random
+    # magnitudes over the whole range): the record's exact value is the
+    # double's, and the layout is the general integer conversion's
+    rng = np.random.default_rng(nl)
+    x = np.concatenate([[0.0, -0.0, 5e-324, -5e-324, 2.2250738585072014e-308, 1.7976931348623157e308,
+                         1.0, -0.5, 3.0, 1 / 3],
+                        rng.standard_normal(300) * np.exp(rng.uniform(-700, 700, 300))])
+    rec = mp.from_float64(x, nl)
+    for v, r in zip(x, to_mp(rec)):
+        assert r.to_fraction() == Fraction(float(v))
+    mant, ex = np.frexp(np.abs(x))
+    gen = mp.from_integers((mant * 2.0**53).astype(np.uint64)[:, None], x < 0, ex.astype(np.int64) - 53, nl)
+    assert rec.tobytes() == gen.tobytes()
+
+
 def test_mpexact_gemm_brute_force_tiny():
     # exact rational brute force on 2x3x2 with mixed exponents
     rnd = random.Random(11)

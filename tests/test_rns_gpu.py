@@ -187,6 +187,27 @@ def test_full_size_fixed_point_spot_check(gpu):
     assert ok and out2.tobytes() == out.tobytes()
 
 
+def test_full_size_gram_spot_check(gpu):
+    # the SYRK workload B^T B, B 16384 x 4096 of 256-bit fixed-point
+    # entries, packed upper: the corners, the diagonal's ends and 1024
+    # sampled upper entries bit for bit against exact dots over k = 16384
+    rng = np.random.default_rng(16384)
+    nl, k, n = 4, 16384, 4096
+    print("gram", k, n, flush=True)
+    b = np.empty(k * n, mp.dtype(nl))
+    step = k * 256
+    for c in range(0, k * n, step):  # generated in column panels to bound host memory
+        b[c:c + step] = mp.random_fixed(step, nl, 256, rng)
+    ok, out = gpu.rns_gemm("T", "N", n, n, k, b, k, b, k, upper_only=True)
+    assert ok
+    cols = np.concatenate([[0, n - 1, n - 1, 4095], rng.integers(0, n, 1024)])
+    rows = np.concatenate([[0, 0, n - 1, 2048], rng.integers(0, n, 1024)])
+    rows = np.minimum(rows, cols)
+    ref = oracle_entries("T", "N", k, b, k, b, k, rows, cols)
+    bad = ~mp.same_bits(out[cols * (cols + 1) // 2 + rows], ref)
+    assert not bad.any(), f"{bad.sum()} sampled entries differ"
+
+
 @pytest.mark.parametrize("m,n,k,upper", [(2100, 2300, 1100, False), (2050, 2050, 1030, True)])
 def test_large_product_blocks_spot_check(gpu, m, n, k, upper):
     """Products large enough for the 256 x 256 one-digit blocks, with ragged
