@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "common.h"
 #include "fp64_dense.h"
@@ -186,157 +187,256 @@ void gemm_f64(const Fp64Gemm &g, hipStream_t s) {
 // ---------------------------------------------------------------------------
 constexpr int NB = 64;
 
-// Unblocked Cholesky of one nb x nb diagonal block in LDS (nb <= 64).  On a
-// non-positive pivot writes info = global column + 1 and stops.
+// Unblocked right-looking Cholesky of one nb x nb diagonal block (nb <= 64).
+// Thread (i, g) = (tid % 64, tid / 64) keeps row i's entries of the columns
+// c = g + 4 t (t < 16) in registers.  Step j: the threads holding column j
+// publish it to LDS (double-buffered by the step's parity), one barrier,
+// then every thread forms the pivot's square root and its reciprocal
+// itself (the same value everywhere, so the pivot decision needs no second
+// barrier), scales its own row's entry and the column entries it needs by
+// the reciprocal (LAPACK's dpotf2 scales by 1 / sqrt(d) as well) and
+// subtracts the rank-1 term from its entries of the lower triangle.  On a
+// non-positive pivot writes info = global column + 1 and stores nothing.
 // With the dynamic pivot rule (dense_block.rs:329-365): a pivot d <= eps
 // becomes delta unless d < -64 eps_f64 nb |diag_k| (clearly indefinite).
 __global__ __launch_bounds__(256) void potrf_diag(double *A, int64_t lda, int64_t j0, int nb,
                                                   int *info, int rule, double eps, double delta,
                                                   const double *diag) {
-    __shared__ double s[NB][NB + 1];
-    __shared__ int bad;
-    const int tid = threadIdx.x;
-    if (tid == 0) bad = 0;
-    for (int e = tid; e < nb * nb; e += 256) {
-        int i = e % nb, j = e / nb;
-        s[i][j] = A[(j0 + i) + (j0 + j) * lda];
+    __shared__ double col[2][NB];
+    const int tid = threadIdx.x, i = tid & 63, g = tid >> 6;
+    double x[NB / 4];
+#pragma unroll
+    for (int t = 0; t < NB / 4; t++) {
+        const int c = g + 4 * t;
+        x[t] = i < nb && c <= i ? A[(j0 + i) + (j0 + c) * lda] : 0.0;
     }
-    __syncthreads();
-    if (*info != 0) return;
-    for (int j = 0; j < nb; j++) {
-        if (tid == 0) {
-            double d = s[j][j];
+    if (*info != 0) return;  // the whole workgroup
+    int bad = 0;
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+        if (j < nb && !bad) {  // uniform: every thread takes the same pivot decisions
+            double *cj = col[j & 1];
+            if (g == (j & 3)) cj[i] = x[j >> 2];
+            __syncthreads();
+            const double d = cj[j];
+            double djj = 0.0;
             if (!std::isfinite(d)) {
                 bad = (int)(j0 + j + 1);
             } else if (rule && d <= eps) {
                 if (diag && d < -64.0 * 2.220446049250313e-16 * (double)nb * fabs(diag[j0 + j]))
                     bad = (int)(j0 + j + 1);
                 else
-                    s[j][j] = sqrt(delta);
+                    djj = sqrt(delta);
             } else if (!(d > 0.0)) {
                 bad = (int)(j0 + j + 1);
             } else {
-                s[j][j] = sqrt(d);
+                djj = sqrt(d);
+            }
+            if (!bad) {
+                const double r = 1.0 / djj;
+                const double lij = cj[i] * r;
+                // every column entry read unconditionally and the update
+                // selected per entry: a branch around each LDS read would
+                // wait for the reads one at a time (entries past nb are
+                // never selected, so their contents do not matter)
+                // (columns c = g + 4 t <= 4 t + 3 <= j are finished: skipped
+                // at compile time, the loops being unrolled over j and t)
+                double lc[NB / 4];
+#pragma unroll
+                for (int t = 0; t < NB / 4; t++) lc[t] = cj[g + 4 * t];
+#pragma unroll
+                for (int t = 0; t < NB / 4; t++) {
+                    const int c = g + 4 * t;
+                    const double u = fma(-lij, lc[t] * r, x[t]);
+                    x[t] = c > j && c <= i ? u : x[t];  // c <= i implies i > j
+                }
+                if (g == (j & 3) && i >= j) x[j >> 2] = i == j ? djj : lij;
             }
         }
-        __syncthreads();
-        if (bad) break;
-        const double djj = s[j][j];
-        for (int i = j + 1 + tid; i < nb; i += 256) s[i][j] /= djj;
-        __syncthreads();
-        // trailing rank-1 update of the lower triangle: thread (row i = tid %
-        // 64, columns c = tid / 64 + 4 t)
-        {
-            const int i = tid & 63;
-            if (i > j && i < nb) {
-                const double lij = s[i][j];
-                for (int c = j + 1 + (tid >> 6); c <= i; c += 4) s[i][c] -= lij * s[c][j];
-            }
-        }
-        __syncthreads();
     }
     if (bad) {
         if (tid == 0) *info = bad;
         return;
     }
-    for (int e = tid; e < nb * nb; e += 256) {
-        int i = e % nb, j = e / nb;
-        if (i >= j) A[(j0 + i) + (j0 + j) * lda] = s[i][j];
+#pragma unroll
+    for (int t = 0; t < NB / 4; t++) {
+        const int c = g + 4 * t;
+        if (i < nb && c <= i) A[(j0 + i) + (j0 + c) * lda] = x[t];
     }
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int k) {
+    const long long b = __double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)b, k);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(b >> 32), k);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
 // X L^T = B for the rows below a diagonal block: row panel rows r0..r1,
-// columns j0..j0+nb.  One wave a row, lane j holding x_j: step j divides x_j
-// by the diagonal and every later lane subtracts its multiple, so each x_j
-// sees the updates of a row-by-row substitution in the same order; four
-// rows a 256-thread workgroup.
+// columns j0..j0+nb.  One wave a row (the grid's waves stride over the
+// rows), lane j holding x_j: step j takes x_j from lane j (v_readlane)
+// times 1/L_jj (reference BLAS dtrsm scales by the reciprocal as well) and
+// every later lane subtracts its multiple, so each x_j sees the updates of
+// a row-by-row substitution in the same order.
+constexpr int TRSM_PANEL_GRID = 1024;
 __global__ __launch_bounds__(256) void trsm_panel(double *A, int64_t lda, int64_t j0, int nb,
                                                   int64_t r0, int64_t r1, const int *info) {
     __shared__ double L[NB][NB + 1];
+    __shared__ double rinv[NB];
     if (*info != 0) return;
     const int tid = threadIdx.x;
-    for (int e = tid; e < nb * nb; e += 256) {
-        int i = e % nb, j = e / nb;
-        L[i][j] = i >= j ? A[(j0 + i) + (j0 + j) * lda] : 0.0;
+    for (int e = tid; e < NB * NB; e += 256) {
+        int i = e % NB, j = e / NB;
+        L[i][j] = i < nb && j < nb && i >= j ? A[(j0 + i) + (j0 + j) * lda] : 0.0;
     }
     __syncthreads();
+    if (tid < nb) rinv[tid] = 1.0 / L[tid][tid];
+    __syncthreads();
     const int lane = tid & 63;
-    const int64_t row = r0 + (int64_t)blockIdx.x * 4 + (tid >> 6);
-    if (row >= r1) return;  // whole waves
-    double x = lane < nb ? A[row + (j0 + lane) * lda] : 0.0;
-    for (int j = 0; j < nb; j++) {
-        const double xj = __shfl(x, j, 64) / L[j][j];
-        if (lane == j) x = xj;
-        if (lane > j && lane < nb) x -= xj * L[lane][j];
+    for (int64_t row = r0 + (int64_t)blockIdx.x * 4 + (tid >> 6); row < r1; row += (int64_t)gridDim.x * 4) {
+        double x = lane < nb ? A[row + (j0 + lane) * lda] : 0.0;
+        for (int j = 0; j < nb; j++) {
+            const double xj = readlane_f64(x, j) * rinv[j];
+            const double u = fma(-xj, L[lane][j], x);  // read for every lane: no branch around the LDS read
+            x = lane == j ? xj : (lane > j ? u : x);
+        }
+        if (lane < nb) A[row + (j0 + lane) * lda] = x;
     }
-    if (lane < nb) A[row + (j0 + lane) * lda] = x;
 }
 
+// A second stream (default priority) and two events per device for the
+// factor's look-ahead, created on first use and kept until the process ends.
+struct SideStream {
+    hipStream_t s = nullptr;
+    hipEvent_t panel = nullptr, update = nullptr;
+};
+static SideStream &side_stream() {
+    static SideStream side[64];
+    static std::mutex mu;
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> g(mu);
+    SideStream &x = side[dev & 63];
+    if (!x.s) {
+        HIP_CHECK(hipStreamCreateWithFlags(&x.s, hipStreamNonBlocking));
+        HIP_CHECK(hipEventCreateWithFlags(&x.panel, hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&x.update, hipEventDisableTiming));
+    }
+    return x;
+}
+
+static void potrf_panel(double *A, int64_t n, int64_t lda, int64_t j0, int *d_info, hipStream_t s,
+                        const PivotRule *rule) {
+    const int nb = (int)std::min<int64_t>(NB, n - j0);
+    hipLaunchKernelGGL(potrf_diag, dim3(1), dim3(256), 0, s, A, lda, j0, nb, d_info, rule ? 1 : 0,
+                       rule ? rule->eps : 0.0, rule ? rule->delta : 0.0, rule ? rule->diag : nullptr);
+    HIP_CHECK(hipGetLastError());
+    const int64_t r0 = j0 + nb;
+    if (r0 >= n) return;
+    hipLaunchKernelGGL(trsm_panel, dim3((unsigned)std::min<int64_t>((n - r0 + 3) / 4, TRSM_PANEL_GRID)),
+                       dim3(256), 0, s, A, lda, j0, nb, r0, n, d_info);
+    HIP_CHECK(hipGetLastError());
+}
+
+// A[r0:, c0:c0+nc] -= A[r0:, j0:j0+NB] A[c0:c0+nc, j0:j0+NB]^T over the
+// lower triangle (r0 == c0: the columns' diagonal block and everything below)
+static void panel_update(double *A, int64_t n, int64_t lda, int64_t j0, int64_t c0, int64_t nc, double *ws,
+                         size_t ws_bytes, hipStream_t s) {
+    Fp64Gemm g;
+    g.m = n - c0;
+    g.n = nc;
+    g.k = NB;
+    g.A = A + c0 + j0 * lda;
+    g.lda = lda;
+    g.B = A + c0 + j0 * lda;
+    g.ldb = lda;
+    g.tb = true;
+    g.C = A + c0 + c0 * lda;
+    g.ldc = lda;
+    g.alpha = -1.0;
+    g.beta = 1.0;
+    g.tri = 2;
+    g.workspace = ws;
+    g.workspace_bytes = ws_bytes;
+    gemm_f64(g, s);
+}
+
+// Blocked right-looking Cholesky, with look-ahead from order 4096: once
+// panel p is factored (diagonal block, then the rows below it), the next
+// panel's columns take panel p's update on the caller's stream and are
+// factored there at once, while the rest of the trailing matrix takes the
+// same update on a second
+// stream; the next panel's update waits for that.  So the diagonal factor
+// and the panel solve of each step (latency-bound single-block kernels) run
+// beside the step's large update instead of between updates.  Every entry
+// receives the same updates in the same order as without look-ahead.  The
+// two cross-stream waits a step cost more than they hide below order 4096
+// (measured: order 8192 18.4 -> 16.2 ms, order 2000 2.18 -> 2.41 ms).
+constexpr int64_t LOOKAHEAD_MIN_ORDER = 4096;
 void potrf_f64(double *A, int64_t n, int64_t lda, int *d_info, double *ws, size_t ws_bytes,
                hipStream_t s, const PivotRule *rule) {
-    for (int64_t j0 = 0; j0 < n; j0 += NB) {
-        const int nb = (int)std::min<int64_t>(NB, n - j0);
-        hipLaunchKernelGGL(potrf_diag, dim3(1), dim3(256), 0, s, A, lda, j0, nb, d_info,
-                           rule ? 1 : 0, rule ? rule->eps : 0.0, rule ? rule->delta : 0.0,
-                           rule ? rule->diag : nullptr);
-        HIP_CHECK(hipGetLastError());
-        const int64_t r0 = j0 + nb;
-        if (r0 >= n) break;
-        hipLaunchKernelGGL(trsm_panel, dim3((unsigned)((n - r0 + 3) / 4)), dim3(256), 0, s, A,
-                           lda, j0, nb, r0, n, d_info);
-        HIP_CHECK(hipGetLastError());
-        // A22 -= P P^T (lower triangle)
-        Fp64Gemm g;
-        g.m = g.n = n - r0;
-        g.k = nb;
-        g.A = A + r0 + j0 * lda;
-        g.lda = lda;
-        g.B = A + r0 + j0 * lda;
-        g.ldb = lda;
-        g.tb = true;
-        g.C = A + r0 + r0 * lda;
-        g.ldc = lda;
-        g.alpha = -1.0;
-        g.beta = 1.0;
-        g.tri = 2;
-        g.workspace = ws;
-        g.workspace_bytes = ws_bytes;
-        gemm_f64(g, s);
+    if (n < LOOKAHEAD_MIN_ORDER) {
+        for (int64_t j0 = 0; j0 < n; j0 += NB) {
+            potrf_panel(A, n, lda, j0, d_info, s, rule);
+            const int64_t r0 = j0 + NB;
+            if (r0 < n) panel_update(A, n, lda, j0, r0, n - r0, ws, ws_bytes, s);
+        }
+        return;
+    }
+    SideStream &side = side_stream();
+    potrf_panel(A, n, lda, 0, d_info, s, rule);
+    for (int64_t j0 = 0; j0 + NB < n; j0 += NB) {
+        const int64_t r0 = j0 + NB, nb1 = std::min<int64_t>(NB, n - r0), r1 = r0 + nb1;
+        if (r1 < n) {  // the rest of the trailing matrix on the second stream (split-K off: no workspace)
+            HIP_CHECK(hipEventRecord(side.panel, s));
+            HIP_CHECK(hipStreamWaitEvent(side.s, side.panel, 0));
+            panel_update(A, n, lda, j0, r1, n - r1, nullptr, 0, side.s);
+            HIP_CHECK(hipEventRecord(side.update, side.s));
+        }
+        panel_update(A, n, lda, j0, r0, nb1, ws, ws_bytes, s);
+        potrf_panel(A, n, lda, r0, d_info, s, rule);
+        if (r1 < n) HIP_CHECK(hipStreamWaitEvent(s, side.update, 0));
     }
 }
 
 // Forward/back substitution with the factor for nrhs columns:
 // lower: solve L Y = B; upper (trans): solve L^T X = Y, one 64-row diagonal
 // block a launch (then a GEMM update of the remaining rows).  One wave a
-// right-hand side, lane i holding row i: step i divides row i by the
-// diagonal and every later (earlier, for the transpose) row subtracts its
-// multiple, so each row sees the same sequence of updates as a row-by-row
-// substitution; four columns a 256-thread workgroup.
+// right-hand side, lane i holding row i: step i takes row i from lane i
+// (v_readlane) times the diagonal's reciprocal and every later (earlier,
+// for the transpose) row subtracts its multiple, so each row sees the same
+// sequence of updates as a row-by-row substitution; four columns a
+// 256-thread workgroup.
 __global__ __launch_bounds__(256) void trsv_block(const double *L, int64_t ldl, double *B,
                                                   int64_t ldb, int64_t j0, int nb, int64_t nrhs,
                                                   int trans) {
     __shared__ double s[NB][NB + 1];
+    __shared__ double rinv[NB];
     const int tid = threadIdx.x;
     for (int e = tid; e < nb * nb; e += 256) {
         int i = e % nb, j = e / nb;
         s[i][j] = i >= j ? L[(j0 + i) + (j0 + j) * ldl] : 0.0;
     }
     __syncthreads();
+    if (tid < nb) rinv[tid] = 1.0 / s[tid][tid];
+    __syncthreads();
     const int lane = tid & 63;
     const int64_t c = (int64_t)blockIdx.x * 4 + (tid >> 6);
     if (c >= nrhs) return;  // whole waves
     double x = lane < nb ? B[(j0 + lane) + c * ldb] : 0.0;
+    // the LDS entry is read for every lane and the update selected, so no
+    // branch waits around the read (rows past nb are never selected)
     if (!trans) {
         for (int i = 0; i < nb; i++) {
-            const double xi = __shfl(x, i, 64) / s[i][i];
-            if (lane == i) x = xi;
-            if (lane > i && lane < nb) x -= s[lane][i] * xi;
+            const double xi = readlane_f64(x, i) * rinv[i];
+            const double u = fma(-s[lane][i], xi, x);
+            x = lane == i ? xi : (lane > i && lane < nb ? u : x);
         }
     } else {
         for (int ii = nb - 1; ii >= 0; ii--) {
-            const double xi = __shfl(x, ii, 64) / s[ii][ii];
-            if (lane == ii) x = xi;
-            if (lane < ii) x -= s[ii][lane] * xi;
+            const double xi = readlane_f64(x, ii) * rinv[ii];
+            const double u = fma(-s[ii][lane], xi, x);
+            x = lane == ii ? xi : (lane < ii ? u : x);
         }
     }
     if (lane < nb) B[(j0 + lane) + c * ldb] = x;
