@@ -121,24 +121,52 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GemmDesc d, int tiles_m, 
     // C/D layout of v_mfma_f64_16x16x4f64: register r of lane l holds row
     // l/16 + 4 r, column l % 16 (one row per register, unlike the
     // four-consecutive-rows grouping of the f32 forms)
+    if (splits > 1) {
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int64_t gi = i0 + wm * 32 + a * 16 + (lane >> 4) + 4 * r;
+                    const int64_t gj = j0 + wn * 32 + b * 16 + (lane & 15);
+                    if (gi >= d.m || gj >= d.n) continue;
+                    if (d.tri == 1 && gi > gj) continue;
+                    if (d.tri == 2 && gi < gj) continue;
+                    d.partial[(size_t)split * d.m * d.n + gi + gj * d.m] = acc[a][b][r];
+                }
+        return;
+    }
+    // every C entry of the tile loaded at once (addresses clamped into the
+    // matrix; entries outside it or the triangle are loaded but not
+    // stored): a load inside each entry's bounds test would wait for the
+    // loads one at a time
+    double *Cb = d.C + batch * d.sc;
+    double cv[2][2][4];
+    if (d.beta != 0.0) {
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int64_t gi = min(i0 + wm * 32 + a * 16 + (lane >> 4) + 4 * r, d.m - 1);
+                    const int64_t gj = min(j0 + wn * 32 + b * 16 + (lane & 15), d.n - 1);
+                    cv[a][b][r] = Cb[gi + gj * d.ldc];
+                }
+    }
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                int64_t gi = i0 + wm * 32 + a * 16 + (lane >> 4) + 4 * r;
-                int64_t gj = j0 + wn * 32 + b * 16 + (lane & 15);
-                if (gi >= d.m || gj >= d.n) continue;
-                if (d.tri == 1 && gi > gj) continue;
-                if (d.tri == 2 && gi < gj) continue;
-                double v = acc[a][b][r];
-                if (splits > 1) {
-                    d.partial[(size_t)split * d.m * d.n + gi + gj * d.m] = v;
-                } else {
-                    double *c = d.C + batch * d.sc + gi + gj * d.ldc;
-                    *c = d.beta == 0.0 ? d.alpha * v : fma(d.alpha, v, d.beta * *c);
-                }
+                const int64_t gi = i0 + wm * 32 + a * 16 + (lane >> 4) + 4 * r;
+                const int64_t gj = j0 + wn * 32 + b * 16 + (lane & 15);
+                const bool keep = gi < d.m && gj < d.n && !(d.tri == 1 && gi > gj) && !(d.tri == 2 && gi < gj);
+                const double v = acc[a][b][r];
+                const double out = d.beta == 0.0 ? d.alpha * v : fma(d.alpha, v, d.beta * cv[a][b][r]);
+                if (keep) Cb[gi + gj * d.ldc] = out;
             }
 }
 
